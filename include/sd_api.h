@@ -1,0 +1,218 @@
+/*
+ * sd_api.h — C ABI of libstardust_hip.so, the MI355X (gfx950) implementation of
+ * stardust's per-timestep DEM / rigid-body step path.
+ *
+ * Conventions
+ *  - Every call returns sd_status (SD_OK == 0). Nothing calls exit().
+ *  - "stream" arguments are hipStream_t passed as void*; NULL selects the
+ *    null stream. Per-kernel entry points take DEVICE pointers; the context
+ *    API (sd_ctx_*) takes HOST pointers for upload/download and owns every
+ *    device buffer it uses.
+ *  - Results are deterministic and independent of the launch configuration
+ *    (block size, tile size, items per thread).
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the spacesuit825/stardust source tree).
+ */
+#ifndef SD_API_H
+#define SD_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+typedef int sd_status;
+#define SD_OK               0
+#define SD_ERR_ARG          1   /* bad argument (null pointer, negative size, ...) */
+#define SD_ERR_HIP          2   /* HIP runtime error; see sd_last_error() */
+#define SD_ERR_CAPACITY     3   /* a fixed-capacity buffer overflowed; required size reported */
+#define SD_ERR_STATE        4   /* call made in the wrong state (e.g. step before upload) */
+#define SD_ERR_NO_DEVICE    5   /* no HIP device visible */
+#define SD_ERR_NONFINITE    6   /* NaN/Inf detected in particle state */
+
+/* Text of the last error recorded on the calling thread (never NULL). */
+const char* sd_last_error(void);
+/* Library version string. */
+const char* sd_version(void);
+
+/* ------------------------------------------------- device memory helpers
+ * Thin wrappers over the HIP runtime so that callers without a HIP toolchain
+ * (ctypes, cgo, JNI) can stage buffers for the per-kernel entry points. */
+sd_status sd_device_count(int* count);
+sd_status sd_malloc(void** dptr, size_t bytes);
+sd_status sd_free(void* dptr);
+sd_status sd_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+sd_status sd_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+sd_status sd_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+sd_status sd_memset_u32(void* dst, uint32_t value, size_t count, void* stream); /* fill kernel, graph-safe */
+sd_status sd_stream_create(void** stream);
+sd_status sd_stream_destroy(void* stream);
+sd_status sd_stream_synchronize(void* stream);
+sd_status sd_event_create(void** ev);
+sd_status sd_event_destroy(void* ev);
+sd_status sd_event_record(void* ev, void* stream);
+sd_status sd_event_elapsed_ms(void* ev_start, void* ev_stop, float* ms);
+
+/* ------------------------------------------------------------ grid / hash
+ * Uniform hash grid over an axis-aligned domain. Cell coordinates are
+ *   c_a = clamp(floor((p_a - lo_a) / cell), 0, n_a - 1)
+ * and the linear cell id is (c_z * n_y + c_y) * n_x + c_x (x fastest).
+ * Replaces the 3x8-bit truncated cell hash of constructCollisionListCUDA
+ * (src/engine/cuda/sh_collision_detection.cu:27-127), one entry per particle
+ * (its home cell) instead of 9. */
+typedef struct sd_grid {
+    float lo[3];        /* domain lower corner */
+    float cell;         /* cell edge length; must be >= the largest sphere diameter */
+    int   n[3];         /* cells per axis */
+} sd_grid;
+
+/* keys[i] = cell id of particle i, vals[i] = i.
+ * Replaces constructCollisionListCUDA (sh_collision_detection.cu:27). */
+sd_status sd_hash_keys_u32(const float* x, const float* y, const float* z, int n,
+                           const sd_grid* grid, uint32_t* keys, uint32_t* vals, void* stream);
+/* keys[i] = (uint64)cell << 32 | i  — the packed 64-bit (cellId, index) key. */
+sd_status sd_hash_keys_u64(const float* x, const float* y, const float* z, int n,
+                           const sd_grid* grid, uint64_t* keys, void* stream);
+
+/* ------------------------------------------------------------ radix sort
+ * Stable LSD radix sort over key bits [begin_bit, end_bit).
+ * Replaces RadixTabulateCUDA / RadixSumCUDA / RadixReorderCUDA and
+ * SpatialPartition::sortCollisionList (sh_collision_detection.cu:129-283,310-360).
+ * The sorted result is left in keys/vals; keys_alt/vals_alt are ping-pong
+ * buffers of the same size. temp must hold sd_radix_sort_temp_bytes(n) bytes. */
+sd_status sd_radix_sort_temp_bytes(int n, size_t* bytes);
+sd_status sd_radix_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                                  int n, int begin_bit, int end_bit, void* temp, size_t temp_bytes,
+                                  void* stream);
+sd_status sd_radix_sort_keys_u64(uint64_t* keys, uint64_t* keys_alt, int n, int begin_bit, int end_bit,
+                                 void* temp, size_t temp_bytes, void* stream);
+
+/* --------------------------------------------------- cell ranges / pairs
+ * ranges[2*c+0] = first sorted position whose cell is c, ranges[2*c+1] = one
+ * past the last; an empty cell has start == end == 0. Every cell is written.
+ * key_shift is 0 for u32 cell keys and 32 for packed u64 keys.
+ * Replaces the run detection of tranverseCollisionListCUDA
+ * (src/engine/cuda/collision_resolution.cu:100-199). */
+sd_status sd_cell_ranges_u32(const uint32_t* sorted_keys, int n, int n_cells, int* ranges, void* stream);
+sd_status sd_cell_ranges_u64(const uint64_t* sorted_keys, int n, int n_cells, int* ranges, void* stream);
+
+/* Neighbour pairs of a sorted sphere set: every unordered pair (a < b, in sorted
+ * positions) whose cells are 27-neighbours and whose spheres satisfy
+ *   |p_a - p_b|^2 < (r_a + r_b + skin)^2.
+ * pos4 is (x, y, z, r) per sorted position. Output pairs are written as
+ * (a, b) int pairs, grouped by a ascending and, within a, in stencil order.
+ * If the pair count exceeds capacity, *n_pairs receives the required count and
+ * SD_ERR_CAPACITY is returned. Replaces resolveCollisionsCUDA's pair loop
+ * (collision_resolution.cu:56-97) and the SAP pair rows (sap_detection.cu:293-324). */
+sd_status sd_neighbor_pairs(const float* pos4, int n, const sd_grid* grid, const int* ranges, float skin,
+                            int* pairs, int capacity, int* n_pairs, void* scratch, size_t scratch_bytes,
+                            void* stream);
+sd_status sd_neighbor_pairs_scratch_bytes(int n, size_t* bytes);
+
+/* ------------------------------------------------------- contact law
+ * Linear spring-dashpot normal force, tangential spring with history and
+ * Coulomb limit, six optional planar walls. */
+typedef struct sd_contact_params {
+    float kn;            /* normal stiffness [N/m] */
+    float damping_ratio; /* zeta; c = 2 zeta sqrt(k m_eff) for normal and tangential */
+    float kt_ratio;      /* kt = kt_ratio * kn */
+    float mu;            /* Coulomb coefficient */
+    float dt;            /* time step [s] */
+    float gravity[3];    /* [m/s^2] */
+    int   walls;         /* 1: six planar walls at the domain boundary */
+    float wall_lo[3];    /* wall planes (usually the grid domain) */
+    float wall_hi[3];
+    int   max_contacts;  /* per-particle contact-history capacity */
+} sd_contact_params;
+
+/* Sphere state in sorted order (struct of float4 arrays):
+ *   pos4 = (x, y, z, radius), vel4 = (vx, vy, vz, 1/m), omg4 = (wx, wy, wz, 1/I).
+ * Contact history: hist_count[s], hist4[k * n + s] = (dtx, dty, dtz, partner id bits).
+ * Partner ids are stable particle ids (ids[s]); walls are ids -1..-6.
+ * old_slot[s] is the position, in the previous step's order, of the particle now
+ * at s (identity if the order did not change); old history is read there.
+ *
+ * sd_contact_forces: force4/torque4 of every sorted sphere, new history.
+ * Replaces computeParticleToParticleCUDA / reactCollisionsCUDA
+ * (src/engine/cuda/collisions/SAP/sap_reaction.cu:20-160); gather, not scatter. */
+sd_status sd_contact_forces(const float* pos4, const float* vel4, const float* omg4, const int* ids, int n,
+                            const sd_grid* grid, const int* ranges, const sd_contact_params* cp,
+                            const int* old_slot, const int* hist_count_old, const float* hist4_old,
+                            int* hist_count_new, float* hist4_new,
+                            float* force4, float* torque4, int* overflow, void* stream);
+
+/* Semi-implicit Euler for free spheres: v += (F/m + g) dt; x += v dt; w += T/I dt.
+ * Spheres with 1/m == 0 are static. Replaces advectParticlesCUDA
+ * (src/engine/cuda/physics_update.cu:232-310) for single-sphere bodies. */
+sd_status sd_integrate_spheres(float* pos4, float* vel4, float* omg4, const float* force4, const float* torque4,
+                               int n, const sd_contact_params* cp, void* stream);
+
+/* Gather a sphere state into a new order: dst[t] = src[perm[t]] for pos4, vel4,
+ * omg4 and ids. */
+sd_status sd_permute_spheres(const int* perm, int n,
+                             const float* pos4, const float* vel4, const float* omg4, const int* ids,
+                             float* pos4_out, float* vel4_out, float* omg4_out, int* ids_out, void* stream);
+
+/* ------------------------------------------------------- scene contexts
+ * A context owns the device state of one sphere scene and runs whole steps:
+ *   keys -> sort -> cell ranges -> permute -> fused contact + integrate + next keys,
+ * replayed from a hipGraph. Replaces DEMEngine::update
+ * (src/engine/cuda/simulation_kernel.cu:52-153) and cleanBuffers (:30-50). */
+typedef struct sd_sphere_params {
+    sd_grid grid;
+    sd_contact_params contact;
+    int key64;           /* 0: u32 cell keys + u32 index values; 1: packed u64 (cell<<32 | index) */
+    int use_graph;       /* 1: replay steps from a captured hipGraph */
+    int tile_items;      /* sort items per thread (0 = autotuned) */
+} sd_sphere_params;
+
+typedef struct sd_ctx sd_ctx;
+
+sd_status sd_ctx_create(const sd_sphere_params* p, sd_ctx** ctx);
+sd_status sd_ctx_destroy(sd_ctx* ctx);
+/* Host arrays in particle-id order. mass may not be NULL; mass[i] == 0 marks a static sphere. */
+sd_status sd_ctx_upload_spheres(sd_ctx* ctx, int n, const float* x, const float* y, const float* z,
+                                const float* vx, const float* vy, const float* vz,
+                                const float* radius, const float* mass);
+/* Advance nsteps steps. Returns SD_ERR_CAPACITY if any step overflowed the
+ * contact-history capacity (the run still completes; forces of the overflowed
+ * contacts are included, their history is not kept). */
+sd_status sd_ctx_step(sd_ctx* ctx, int nsteps);
+/* Runs nsteps steps without graph replay, recording HIP events on the context
+ * stream around each stage, and adds the time of each stage summed over the
+ * steps to stage_ms: [0] radix sort (all passes), [1] cell ranges (clear + mark),
+ * [2] permute, [3] fused contact + integrate + next keys. */
+sd_status sd_ctx_profile(sd_ctx* ctx, int nsteps, float* stage_ms, int nstages);
+/* Host arrays in particle-id order (any pointer may be NULL). */
+sd_status sd_ctx_download_spheres(sd_ctx* ctx, float* x, float* y, float* z,
+                                  float* vx, float* vy, float* vz, float* wx, float* wy, float* wz);
+/* Number of stored contacts (directed, sphere-sphere + sphere-wall) after the last step. */
+sd_status sd_ctx_contact_count(sd_ctx* ctx, long long* count);
+/* FP64 diagnostics: out[0..2] linear momentum, out[3..5] angular momentum about the
+ * origin (orbital + spin), out[6] kinetic energy, out[7] gravitational potential energy. */
+sd_status sd_ctx_diagnostics(sd_ctx* ctx, double out[8]);
+/* Stream the context launches on (hipStream_t as void*). */
+sd_status sd_ctx_stream(sd_ctx* ctx, void** stream);
+/* Save / restore the complete device state (positions, velocities, contact history,
+ * order) to a host byte buffer. sd_ctx_state_bytes gives the size. */
+sd_status sd_ctx_state_bytes(sd_ctx* ctx, size_t* bytes);
+sd_status sd_ctx_save(sd_ctx* ctx, void* buffer, size_t bytes);
+sd_status sd_ctx_load(sd_ctx* ctx, const void* buffer, size_t bytes);
+
+/* ------------------------------------------------------- scene generators
+ * Seeded SplitMix64 generators (host only). uniform = (u >> 40) * 2^-24,
+ * normal = Box-Muller, lognormal = exp(ln(median) + sigma * normal).
+ * Random sequential addition on a hash grid guarantees no initial overlap;
+ * returns SD_ERR_CAPACITY if the requested count cannot be placed. */
+sd_status sd_gen_spheres(uint64_t seed, int n, const float lo[3], const float hi[3],
+                         float r_median, float r_sigma, float r_min, float r_max,
+                         float* x, float* y, float* z, float* radius);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SD_API_H */

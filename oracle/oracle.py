@@ -1,0 +1,227 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker. The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrGrid(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("cell", C.c_float), ("n", C.c_int * 3)]
+
+
+class OrContact(C.Structure):
+    _fields_ = [("kn", C.c_float), ("damping_ratio", C.c_float), ("kt_ratio", C.c_float), ("mu", C.c_float),
+                ("dt", C.c_float), ("gravity", C.c_float * 3), ("walls", C.c_int), ("wall_lo", C.c_float * 3),
+                ("wall_hi", C.c_float * 3), ("max_contacts", C.c_int)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_SIGS = {
+    "or_hash_keys": ([_I, _P, _P, _P, _I, C.POINTER(OrGrid), _P, _P], None),
+    "or_cell_id": ([C.POINTER(OrGrid), C.c_float, C.c_float, C.c_float], C.c_uint32),
+    "or_sort_pairs_u32": ([_I, _P, _P, _I, _I], None),
+    "or_sort_keys_u64": ([_I, _P, _I, _I], None),
+    "or_cell_ranges_u32": ([_I, _P, _I, _P], None),
+    "or_cell_ranges_u64": ([_I, _P, _I, _P], None),
+    "or_neighbor_pairs": ([_I, _P, C.POINTER(OrGrid), _P, C.c_float, _P, C.c_long], C.c_long),
+    "or_brute_pairs": ([_I, _P, C.c_float, _P, C.c_long], C.c_long),
+    "or_contact_forces": ([_I, _P, _P, _P, _P, C.POINTER(OrGrid), _P, C.POINTER(OrContact), _P, _P, _P, _P, _P, _P,
+                           _P, _I], _I),
+    "or_contact_forces_f64": ([_I, _P, _P, _P, _P, C.POINTER(OrGrid), _P, C.POINTER(OrContact), _P, _P, _P, _P, _P,
+                               _P, _P], _I),
+    "or_integrate": ([_I, _P, _P, _P, _P, _P, C.POINTER(OrContact)], None),
+    "or_sphere_step": ([_I, C.POINTER(OrGrid), C.POINTER(OrContact), _P, _P, _P, _P, _P, _P, _I], _I),
+}
+
+_lib = None
+
+
+def build():
+    r = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = lib
+    return _lib
+
+
+def p(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def grid(g):
+    """OrGrid from any object with lo, cell, n (e.g. stardust_amd.Grid)."""
+    o = OrGrid()
+    for a in range(3):
+        o.lo[a] = g.lo[a]
+        o.n[a] = g.n[a]
+    o.cell = g.cell
+    return o
+
+
+def contact(c):
+    o = OrContact()
+    for f, _ in OrContact._fields_:
+        v = getattr(c, f)
+        if hasattr(v, "__len__"):
+            arr = getattr(o, f)
+            for a in range(3):
+                arr[a] = v[a]
+        else:
+            setattr(o, f, v)
+    return o
+
+
+def hash_keys(x, y, z, g):
+    n = len(x)
+    keys = np.empty(n, np.uint32)
+    vals = np.empty(n, np.uint32)
+    og = grid(g)
+    xs, ys, zs = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))  # kept alive across the call
+    load().or_hash_keys(n, p(xs), p(ys), p(zs), 1, C.byref(og), p(keys), p(vals))
+    return keys, vals
+
+
+def sort_pairs_u32(keys, vals, begin_bit, end_bit):
+    k = np.array(keys, np.uint32)
+    v = np.array(vals, np.uint32)
+    load().or_sort_pairs_u32(len(k), p(k), p(v), begin_bit, end_bit)
+    return k, v
+
+
+def sort_keys_u64(keys, begin_bit, end_bit):
+    k = np.array(keys, np.uint64)
+    load().or_sort_keys_u64(len(k), p(k), begin_bit, end_bit)
+    return k
+
+
+def cell_ranges_u32(keys, n_cells):
+    r = np.empty(2 * n_cells, np.int32)
+    k = np.ascontiguousarray(keys, np.uint32)
+    load().or_cell_ranges_u32(len(k), p(k), n_cells, p(r))
+    return r
+
+
+def cell_ranges_u64(keys, n_cells):
+    r = np.empty(2 * n_cells, np.int32)
+    k = np.ascontiguousarray(keys, np.uint64)
+    load().or_cell_ranges_u64(len(k), p(k), n_cells, p(r))
+    return r
+
+
+def neighbor_pairs(pos4, g, ranges, skin):
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    n = len(pos4)
+    og = grid(g)
+    rng = np.ascontiguousarray(ranges, np.int32)
+    cnt = load().or_neighbor_pairs(n, p(pos4), C.byref(og), p(rng), skin, None, 0)
+    out = np.empty((max(cnt, 1), 2), np.int32)
+    load().or_neighbor_pairs(n, p(pos4), C.byref(og), p(rng), skin, p(out), cnt)
+    return out[:cnt]
+
+
+def brute_pairs(pos4, skin):
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    n = len(pos4)
+    cnt = load().or_brute_pairs(n, p(pos4), skin, None, 0)
+    out = np.empty((max(cnt, 1), 2), np.int32)
+    load().or_brute_pairs(n, p(pos4), skin, p(out), cnt)
+    return out[:cnt]
+
+
+def contact_forces(pos4, vel4, omg4, ids, g, ranges, c, old_slot, hcount_old, hist4_old, nthreads=1):
+    n = len(pos4)
+    maxc = c.max_contacts
+    og, oc = grid(g), contact(c)
+    hcount_new = np.zeros(n, np.int32)
+    hist_new = np.zeros((maxc * n, 4), np.float32)
+    f = np.zeros((n, 4), np.float32)
+    t = np.zeros((n, 4), np.float32)
+    arrs = [np.ascontiguousarray(a) for a in (pos4, vel4, omg4, ids, ranges, old_slot, hcount_old, hist4_old)]
+    mx = load().or_contact_forces(n, p(arrs[0]), p(arrs[1]), p(arrs[2]), p(arrs[3]), C.byref(og), p(arrs[4]),
+                                  C.byref(oc), p(arrs[5]), p(arrs[6]), p(arrs[7]), p(hcount_new), p(hist_new), p(f),
+                                  p(t), nthreads)
+    return {"force": f, "torque": t, "hcount": hcount_new, "hist": hist_new, "max_contacts": mx}
+
+
+def contact_forces_f64(pos4, vel4, omg4, ids, g, ranges, c, old_slot, hcount_old, hist4_old_f64):
+    n = len(pos4)
+    maxc = c.max_contacts
+    og, oc = grid(g), contact(c)
+    hcount_new = np.zeros(n, np.int32)
+    hist_new = np.zeros((maxc * n, 4), np.float64)
+    f = np.zeros((n, 4), np.float64)
+    t = np.zeros((n, 4), np.float64)
+    d = [np.ascontiguousarray(a, np.float64) for a in (pos4, vel4, omg4)]
+    i = [np.ascontiguousarray(a, np.int32) for a in (ids, ranges, old_slot, hcount_old)]
+    h = np.ascontiguousarray(hist4_old_f64, np.float64)
+    load().or_contact_forces_f64(n, p(d[0]), p(d[1]), p(d[2]), p(i[0]), C.byref(og), p(i[1]), C.byref(oc), p(i[2]),
+                                 p(i[3]), p(h), p(hcount_new), p(hist_new), p(f), p(t))
+    return {"force": f, "torque": t, "hcount": hcount_new, "hist": hist_new}
+
+
+def integrate(pos4, vel4, omg4, force4, torque4, c):
+    pos4, vel4, omg4 = (np.array(a, np.float32) for a in (pos4, vel4, omg4))
+    f, t = (np.ascontiguousarray(a, np.float32) for a in (force4, torque4))
+    oc = contact(c)
+    load().or_integrate(len(pos4), p(pos4), p(vel4), p(omg4), p(f), p(t), C.byref(oc))
+    return pos4, vel4, omg4
+
+
+class SphereState:
+    """Sphere state in slot order, as the step keeps it (float4 rows)."""
+
+    def __init__(self, x, y, z, r, m, vx=None, vy=None, vz=None, max_contacts=16):
+        n = len(x)
+        self.n = n
+        self.maxc = max_contacts
+        m = np.asarray(m, np.float32)
+        r = np.asarray(r, np.float32)
+        inv_m = np.where(m > 0, np.float32(1.0) / np.where(m > 0, m, 1), 0).astype(np.float32)
+        I = (np.float32(0.4) * m * r * r).astype(np.float32)
+        inv_I = np.where(m > 0, np.float32(1.0) / np.where(m > 0, I, 1), 0).astype(np.float32)
+        z0 = np.zeros(n, np.float32)
+        self.pos4 = np.stack([x, y, z, r], 1).astype(np.float32)
+        vx = z0 if vx is None else np.where(m > 0, vx, 0)
+        vy = z0 if vy is None else np.where(m > 0, vy, 0)
+        vz = z0 if vz is None else np.where(m > 0, vz, 0)
+        self.vel4 = np.stack([vx, vy, vz, inv_m], 1).astype(np.float32)
+        self.omg4 = np.stack([z0, z0, z0, inv_I], 1).astype(np.float32)
+        self.ids = np.arange(n, dtype=np.int32)
+        self.hcount = np.zeros(n, np.int32)
+        self.hist = np.zeros((max_contacts * n, 4), np.float32)
+
+    def step(self, g, c, nsteps=1, nthreads=1):
+        og, oc = grid(g), contact(c)
+        mx = 0
+        for _ in range(nsteps):
+            mx = max(mx, load().or_sphere_step(self.n, C.byref(og), C.byref(oc), p(self.pos4), p(self.vel4),
+                                               p(self.omg4), p(self.ids), p(self.hcount), p(self.hist), nthreads))
+        return mx
+
+    def by_id(self):
+        """State rows reordered to particle-id order."""
+        inv = np.empty(self.n, np.int64)
+        inv[self.ids] = np.arange(self.n)
+        return {"pos4": self.pos4[inv], "vel4": self.vel4[inv], "omg4": self.omg4[inv]}

@@ -1,0 +1,94 @@
+/*
+ * sd_oracle.h — CPU oracle for the DEM step path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the checker the parity tests, smoke() and
+ * the bench's cpu_baseline leg compare the HIP library against. Nothing in
+ * the product (libstardust_hip.so, the stardust_amd package) links, loads or
+ * calls it.
+ *
+ * It restates, in plain C (FP32, no contraction, same operation order as the
+ * documented contact law), the reference algorithms of spacesuit825/stardust
+ * with the intended well-defined outputs (stable sort order, complete pair
+ * sets); every function cites the reference file:line it follows. Each function
+ * is pinned by the tests in tests/test_oracle.py against closed forms, brute
+ * force or textbook routines.
+ */
+#ifndef SD_ORACLE_H
+#define SD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same memory layout as sd_grid / sd_contact_params of the C ABI. */
+typedef struct or_grid {
+    float lo[3];
+    float cell;
+    int n[3];
+} or_grid;
+
+typedef struct or_contact {
+    float kn;
+    float damping_ratio;
+    float kt_ratio;
+    float mu;
+    float dt;
+    float gravity[3];
+    int walls;
+    float wall_lo[3];
+    float wall_hi[3];
+    int max_contacts;
+} or_contact;
+
+/* Home-cell key of each particle (sh_collision_detection.cu:60-66). */
+void or_hash_keys(int n, const float* x, const float* y, const float* z, int stride, const or_grid* g,
+                  uint32_t* keys, uint32_t* vals);
+uint32_t or_cell_id(const or_grid* g, float x, float y, float z);
+
+/* Stable LSD radix sort, 8-bit digits, over key bits [begin, end)
+ * (sh_collision_detection.cu:129-283,310-360: tabulate, prefix, stable reorder). */
+void or_sort_pairs_u32(int n, uint32_t* keys, uint32_t* vals, int begin_bit, int end_bit);
+void or_sort_keys_u64(int n, uint64_t* keys, int begin_bit, int end_bit);
+
+/* Cell start/end of a sorted key array (collision_resolution.cu:145-178 run detection);
+ * empty cells are (0, 0). key_shift selects the cell bits of a packed key. */
+void or_cell_ranges_u32(int n, const uint32_t* keys, int n_cells, int* ranges);
+void or_cell_ranges_u64(int n, const uint64_t* keys, int n_cells, int* ranges);
+
+/* Neighbour pairs (a < b in sorted order, 27-cell stencil, |d|^2 < (ra+rb+skin)^2),
+ * grouped by a, stencil order within a (collision_resolution.cu:56-97).
+ * Returns the pair count; writes at most capacity pairs. */
+long or_neighbor_pairs(int n, const float* pos4, const or_grid* g, const int* ranges, float skin, int* pairs,
+                       long capacity);
+/* Brute force: all unordered pairs with |d|^2 < (ra+rb+skin)^2 (i < j). */
+long or_brute_pairs(int n, const float* pos4, float skin, int* pairs, long capacity);
+
+/* Contact forces of every sphere of a sorted state (sap_reaction.cu:20-70,72-134,
+ * restated as a deterministic gather with the documented contact law).
+ * Returns the largest contact count seen. */
+int or_contact_forces(int n, const float* pos4, const float* vel4, const float* omg4, const int* ids,
+                      const or_grid* g, const int* ranges, const or_contact* c, const int* old_slot,
+                      const int* hcount_old, const float* hist4_old, int* hcount_new, float* hist4_new, float* force4,
+                      float* torque4, int nthreads);
+/* The same in FP64 (reference for error reports; history in double). */
+int or_contact_forces_f64(int n, const double* pos4, const double* vel4, const double* omg4, const int* ids,
+                          const or_grid* g, const int* ranges, const or_contact* c, const int* old_slot,
+                          const int* hcount_old, const double* hist4_old, int* hcount_new, double* hist4_new,
+                          double* force4, double* torque4);
+
+/* Semi-implicit Euler (physics_update.cu:264-269 for the linear part). */
+void or_integrate(int n, float* pos4, float* vel4, float* omg4, const float* force4, const float* torque4,
+                  const or_contact* c);
+
+/* One full step on a state in slot order: keys, stable sort, ranges, permute,
+ * forces, integrate (simulation_kernel.cu:52-153 step order). All arrays are
+ * updated in place to the new order. Returns the largest contact count. */
+int or_sphere_step(int n, const or_grid* g, const or_contact* c, float* pos4, float* vel4, float* omg4, int* ids,
+                   int* hcount, float* hist4, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

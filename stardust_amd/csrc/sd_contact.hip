@@ -1,0 +1,334 @@
+// Deterministic per-sphere contact gather and semi-implicit integration.
+//
+// Replaces computeParticleToParticleCUDA / reactCollisionsCUDA
+// (src/engine/cuda/collisions/SAP/sap_reaction.cu:20-160) and, for single-sphere
+// bodies, advectParticlesCUDA (src/engine/cuda/physics_update.cu:232-310).
+// The reference scatters pair forces into both spheres without atomics; here
+// each sphere gathers the forces of all its contacts in a fixed traversal order
+// (stencil rows (dz, dy) ascending, sorted positions ascending within a row,
+// then walls 0..5), so results do not depend on the launch configuration.
+//
+// Contact law (per contact, FP32, no fused multiply-add so that the pair force
+// seen by each side is the exact negation of the other):
+//   n      = (x_i - x_j) / |x_i - x_j|,  delta = r_i + r_j - |x_i - x_j| > 0
+//   m_eff  = 1 / (1/m_i + 1/m_j),  c_n = 2 zeta sqrt(k_n m_eff),  c_t = 2 zeta sqrt(k_t m_eff)
+//   v_c    = (v_i - v_j) - (r_i w_i + r_j w_j) x n,  v_n = v_c . n,  v_t = v_c - v_n n
+//   f_n    = k_n delta - c_n v_n
+//   xi_t   = (xi_t - (xi_t . n) n) + v_t dt                (tangential spring history)
+//   F_t    = -k_t xi_t - c_t v_t, limited to mu |f_n|; on slip xi_t = -(F_t + c_t v_t) / k_t
+//   F_i    = f_n n + F_t,  T_i = -r_i (n x F_t)
+// Walls are partners with r = 0, 1/m = 0, v = w = 0 and ids -1..-6.
+#include "sd_internal.hpp"
+#include "sd_grid_dev.hpp"
+
+namespace sd {
+
+ContactDev to_dev(const sd_contact_params& c) {
+    ContactDev d;
+    d.kn = c.kn;
+    d.kt = c.kt_ratio * c.kn;
+    d.c2z = 2.0f * c.damping_ratio;
+    d.mu = c.mu;
+    d.dt = c.dt;
+    d.gx = c.gravity[0];
+    d.gy = c.gravity[1];
+    d.gz = c.gravity[2];
+    d.walls = c.walls;
+    for (int a = 0; a < 3; ++a) {
+        d.wlo[a] = c.wall_lo[a];
+        d.whi[a] = c.wall_hi[a];
+    }
+    d.maxc = c.max_contacts;
+    return d;
+}
+
+sd_status check_contact(const sd_contact_params* c) {
+    SD_REQUIRE(c != nullptr, "contact params are NULL");
+    SD_REQUIRE(c->kn > 0.0f && c->kt_ratio > 0.0f, "contact: kn and kt_ratio must be positive");
+    SD_REQUIRE(c->damping_ratio >= 0.0f && c->mu >= 0.0f, "contact: damping_ratio and mu must be >= 0");
+    SD_REQUIRE(c->dt > 0.0f, "contact: dt must be positive");
+    SD_REQUIRE(c->max_contacts > 0 && c->max_contacts <= 256, "contact: max_contacts must be in [1, 256]");
+    return SD_OK;
+}
+
+struct PairForce {
+    float fx, fy, fz, tx, ty, tz;
+};
+
+__device__ __forceinline__ PairForce contact_law(const ContactDev& c, float nx, float ny, float nz, float delta,
+                                                 float ri, float rj, float inv_mi, float inv_mj, float vrx, float vry,
+                                                 float vrz, float wix, float wiy, float wiz, float wjx, float wjy,
+                                                 float wjz, float& hx, float& hy, float& hz) {
+    const float m_eff = 1.0f / (inv_mi + inv_mj);
+    const float cn = c.c2z * sqrtf(c.kn * m_eff);
+    const float ct = c.c2z * sqrtf(c.kt * m_eff);
+    const float ax = ri * wix + rj * wjx;
+    const float ay = ri * wiy + rj * wjy;
+    const float az = ri * wiz + rj * wjz;
+    const float cx = ay * nz - az * ny;
+    const float cy = az * nx - ax * nz;
+    const float cz = ax * ny - ay * nx;
+    const float vcx = vrx - cx;
+    const float vcy = vry - cy;
+    const float vcz = vrz - cz;
+    const float vn = vcx * nx + vcy * ny + vcz * nz;
+    const float fn = c.kn * delta - cn * vn;
+    const float vtx = vcx - vn * nx;
+    const float vty = vcy - vn * ny;
+    const float vtz = vcz - vn * nz;
+    const float hn = hx * nx + hy * ny + hz * nz;
+    hx = hx - hn * nx;
+    hy = hy - hn * ny;
+    hz = hz - hn * nz;
+    hx = hx + vtx * c.dt;
+    hy = hy + vty * c.dt;
+    hz = hz + vtz * c.dt;
+    float ftx = -(c.kt * hx) - ct * vtx;
+    float fty = -(c.kt * hy) - ct * vty;
+    float ftz = -(c.kt * hz) - ct * vtz;
+    const float ft2 = ftx * ftx + fty * fty + ftz * ftz;
+    const float fmax = c.mu * fabsf(fn);
+    if (ft2 > fmax * fmax) {
+        const float sc = fmax / sqrtf(ft2);
+        ftx = ftx * sc;
+        fty = fty * sc;
+        ftz = ftz * sc;
+        hx = -(ftx + ct * vtx) / c.kt;
+        hy = -(fty + ct * vty) / c.kt;
+        hz = -(ftz + ct * vtz) / c.kt;
+    }
+    PairForce f;
+    f.fx = fn * nx + ftx;
+    f.fy = fn * ny + fty;
+    f.fz = fn * nz + ftz;
+    f.tx = -ri * (ny * ftz - nz * fty);
+    f.ty = -ri * (nz * ftx - nx * ftz);
+    f.tz = -ri * (nx * fty - ny * ftx);
+    return f;
+}
+
+__device__ __forceinline__ void integrate_sphere(const ContactDev& c, float4& p, float4& v, float4& w, float fx,
+                                                 float fy, float fz, float tx, float ty, float tz) {
+    if (v.w > 0.0f) {
+        v.x = v.x + (fx * v.w + c.gx) * c.dt;
+        v.y = v.y + (fy * v.w + c.gy) * c.dt;
+        v.z = v.z + (fz * v.w + c.gz) * c.dt;
+        p.x = p.x + v.x * c.dt;
+        p.y = p.y + v.y * c.dt;
+        p.z = p.z + v.z * c.dt;
+        w.x = w.x + (tx * w.w) * c.dt;
+        w.y = w.y + (ty * w.w) * c.dt;
+        w.z = w.z + (tz * w.w) * c.dt;
+    }
+}
+
+__device__ __forceinline__ int old_slot_of(const ContactArgs& a, int s) {
+    if (a.old_slot_i32) return a.old_slot_i32[s];
+    if (a.old_slot_u32) return (int)a.old_slot_u32[s];
+    return (int)(uint32_t)a.old_slot_u64[s];
+}
+
+// Looks up the tangential history stored for partner pid in the old list.
+__device__ __forceinline__ void find_history(const ContactArgs& a, int os, int oc, int pid, float& hx, float& hy,
+                                             float& hz) {
+    hx = 0.0f;
+    hy = 0.0f;
+    hz = 0.0f;
+    for (int k = 0; k < oc; ++k) {
+        const float4 h = a.hist_old[(size_t)k * a.n + os];
+        if (__float_as_int(h.w) == pid) {
+            hx = h.x;
+            hy = h.y;
+            hz = h.z;
+            break;
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n) return;
+    float4 p = a.pos[s];
+    float4 v = a.vel[s];
+    float4 w = a.omg[s];
+    const int os = old_slot_of(a, s);
+    int oc = a.hcount_old[os];
+    oc = oc < a.c.maxc ? oc : a.c.maxc;
+
+    int cx, cy, cz;
+    cell_coords(a.g, p.x, p.y, p.z, cx, cy, cz);
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    int nc = 0;
+
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= a.g.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= a.g.ny) continue;
+            int lo, hi;
+            row_range(a.g, a.ranges, cx, y, z, lo, hi);
+            for (int t = lo; t < hi; ++t) {
+                if (t == s) continue;
+                const float4 q = a.pos[t];
+                const float dx = p.x - q.x;
+                const float dyy = p.y - q.y;
+                const float dzz = p.z - q.z;
+                const float d2 = dx * dx + dyy * dyy + dzz * dzz;
+                const float rs = p.w + q.w;
+                if (!(d2 < rs * rs) || d2 == 0.0f) continue;
+                const float4 vq = a.vel[t];
+                if (v.w + vq.w == 0.0f) continue;  // two static spheres
+                const float4 wq = a.omg[t];
+                const int pid = a.ids[t];
+                const float d = sqrtf(d2);
+                const float inv_d = 1.0f / d;
+                const float nx = dx * inv_d, ny = dyy * inv_d, nz = dzz * inv_d;
+                float hx, hy, hz;
+                find_history(a, os, oc, pid, hx, hy, hz);
+                const PairForce f = contact_law(a.c, nx, ny, nz, rs - d, p.w, q.w, v.w, vq.w, v.x - vq.x,
+                                                v.y - vq.y, v.z - vq.z, w.x, w.y, w.z, wq.x, wq.y, wq.z, hx, hy, hz);
+                fx = fx + f.fx;
+                fy = fy + f.fy;
+                fz = fz + f.fz;
+                tx = tx + f.tx;
+                ty = ty + f.ty;
+                tz = tz + f.tz;
+                if (nc < a.c.maxc) a.hist_new[(size_t)nc * a.n + s] = make_float4(hx, hy, hz, __int_as_float(pid));
+                ++nc;
+            }
+        }
+    }
+
+    if (a.c.walls && v.w > 0.0f) {
+#pragma unroll
+        for (int wl = 0; wl < 6; ++wl) {
+            const int ax = wl >> 1;
+            const bool upper = wl & 1;
+            const float pc = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
+            const float dist = upper ? a.c.whi[ax] - pc : pc - a.c.wlo[ax];
+            if (!(dist < p.w)) continue;
+            const float sgn = upper ? -1.0f : 1.0f;
+            const float nx = ax == 0 ? sgn : 0.0f;
+            const float ny = ax == 1 ? sgn : 0.0f;
+            const float nz = ax == 2 ? sgn : 0.0f;
+            const int pid = -(wl + 1);
+            float hx, hy, hz;
+            find_history(a, os, oc, pid, hx, hy, hz);
+            const PairForce f = contact_law(a.c, nx, ny, nz, p.w - dist, p.w, 0.0f, v.w, 0.0f, v.x, v.y, v.z, w.x,
+                                            w.y, w.z, 0.0f, 0.0f, 0.0f, hx, hy, hz);
+            fx = fx + f.fx;
+            fy = fy + f.fy;
+            fz = fz + f.fz;
+            tx = tx + f.tx;
+            ty = ty + f.ty;
+            tz = tz + f.tz;
+            if (nc < a.c.maxc) a.hist_new[(size_t)nc * a.n + s] = make_float4(hx, hy, hz, __int_as_float(pid));
+            ++nc;
+        }
+    }
+
+    a.hcount_new[s] = nc < a.c.maxc ? nc : a.c.maxc;
+    if (nc > a.c.maxc) atomicMax(a.overflow, nc);
+
+    if (MODE == CONTACT_FORCES) {
+        a.force[s] = make_float4(fx, fy, fz, 0.0f);
+        a.torque[s] = make_float4(tx, ty, tz, 0.0f);
+    } else {
+        integrate_sphere(a.c, p, v, w, fx, fy, fz, tx, ty, tz);
+        a.pos_o[s] = p;
+        a.vel_o[s] = v;
+        a.omg_o[s] = w;
+        a.ids_o[s] = a.ids[s];
+        if (MODE == CONTACT_FUSED_U32) {
+            a.keys_o32[s] = cell_id(a.g, p.x, p.y, p.z);
+            a.vals_o32[s] = (uint32_t)s;
+        } else {
+            a.keys_o64[s] = ((uint64_t)cell_id(a.g, p.x, p.y, p.z) << 32) | (uint64_t)(uint32_t)s;
+        }
+    }
+}
+
+__global__ void k_integrate(float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ omg,
+                            const float4* __restrict__ f, const float4* __restrict__ t, int n, ContactDev c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos[i], v = vel[i], w = omg[i];
+    const float4 F = f[i], T = t[i];
+    integrate_sphere(c, p, v, w, F.x, F.y, F.z, T.x, T.y, T.z);
+    pos[i] = p;
+    vel[i] = v;
+    omg[i] = w;
+}
+
+sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int block) {
+    if (a.n <= 0) return SD_OK;
+    if (block <= 0) block = 128;
+    const dim3 grid(div_up(a.n, block));
+    if (mode == CONTACT_FORCES)
+        hipLaunchKernelGGL(k_contacts<CONTACT_FORCES>, grid, dim3(block), 0, s, a);
+    else if (mode == CONTACT_FUSED_U32)
+        hipLaunchKernelGGL(k_contacts<CONTACT_FUSED_U32>, grid, dim3(block), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_contacts<CONTACT_FUSED_U64>, grid, dim3(block), 0, s, a);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+sd_status launch_integrate(float4* pos, float4* vel, float4* omg, const float4* f, const float4* t, int n,
+                           const ContactDev& c, hipStream_t s) {
+    if (n <= 0) return SD_OK;
+    hipLaunchKernelGGL(k_integrate, dim3(div_up(n, 256)), dim3(256), 0, s, pos, vel, omg, f, t, n, c);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+}  // namespace sd
+
+using namespace sd;
+
+extern "C" sd_status sd_contact_forces(const float* pos4, const float* vel4, const float* omg4, const int* ids, int n,
+                                       const sd_grid* grid, const int* ranges, const sd_contact_params* cp,
+                                       const int* old_slot, const int* hist_count_old, const float* hist4_old,
+                                       int* hist_count_new, float* hist4_new, float* force4, float* torque4,
+                                       int* overflow, void* stream) {
+    sd_status st = check_grid(grid);
+    if (st != SD_OK) return st;
+    st = check_contact(cp);
+    if (st != SD_OK) return st;
+    SD_REQUIRE(n >= 0, "sd_contact_forces: n < 0");
+    if (n == 0) return SD_OK;
+    SD_REQUIRE(pos4 && vel4 && omg4 && ids && ranges && old_slot && hist_count_old && hist4_old && hist_count_new &&
+                   hist4_new && force4 && torque4 && overflow,
+               "sd_contact_forces: null pointer");
+    ContactArgs a = {};
+    a.pos = (const float4*)pos4;
+    a.vel = (const float4*)vel4;
+    a.omg = (const float4*)omg4;
+    a.ids = ids;
+    a.n = n;
+    a.g = to_dev(*grid);
+    a.ranges = ranges;
+    a.c = to_dev(*cp);
+    a.old_slot_i32 = old_slot;
+    a.hcount_old = hist_count_old;
+    a.hist_old = (const float4*)hist4_old;
+    a.hcount_new = hist_count_new;
+    a.hist_new = (float4*)hist4_new;
+    a.overflow = overflow;
+    a.force = (float4*)force4;
+    a.torque = (float4*)torque4;
+    return launch_contacts(CONTACT_FORCES, a, (hipStream_t)stream, 128);
+}
+
+extern "C" sd_status sd_integrate_spheres(float* pos4, float* vel4, float* omg4, const float* force4,
+                                          const float* torque4, int n, const sd_contact_params* cp, void* stream) {
+    sd_status st = check_contact(cp);
+    if (st != SD_OK) return st;
+    SD_REQUIRE(n >= 0, "sd_integrate_spheres: n < 0");
+    if (n == 0) return SD_OK;
+    SD_REQUIRE(pos4 && vel4 && omg4 && force4 && torque4, "sd_integrate_spheres: null pointer");
+    return launch_integrate((float4*)pos4, (float4*)vel4, (float4*)omg4, (const float4*)force4,
+                            (const float4*)torque4, n, to_dev(*cp), (hipStream_t)stream);
+}

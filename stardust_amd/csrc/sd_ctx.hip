@@ -1,0 +1,589 @@
+// Sphere-scene context: owns the device state and runs whole timesteps.
+//
+// Replaces DEMEngine::update and cleanBuffers (src/engine/cuda/simulation_kernel.cu:30-153)
+// together with EntityHandler::allocateCUDA / transferDataToDevice
+// (src/engine/entities/entity_handler.cpp:196-472) for sphere scenes.
+//
+// One step, all on the context stream, no host synchronisation:
+//   sort(keys)            keys were written by the previous step (or upload)
+//   cell ranges           clear (fill kernel) + mark runs
+//   permute A -> B        state gathered into the new sorted order
+//   fused contact kernel  reads B + old history, writes integrated state to A,
+//                         new history, and the next step's keys
+// Buffer roles repeat with period two, so two steps are captured once into a
+// hipGraph per starting parity and replayed.
+#include <math.h>
+#include <string.h>
+
+#include <map>
+#include <vector>
+
+#include "sd_internal.hpp"
+
+namespace sd {
+
+constexpr int DIAG_BLOCK = 256;
+constexpr int DIAG_VALUES = 9;  // 8 diagnostics + contact count
+
+__global__ __launch_bounds__(DIAG_BLOCK) void k_diag_partial(const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vel,
+                                                             const float4* __restrict__ omg,
+                                                             const int* __restrict__ hcount, int n, float gx, float gy,
+                                                             float gz, double* __restrict__ partial) {
+    __shared__ double red[DIAG_VALUES][DIAG_BLOCK];
+    double acc[DIAG_VALUES] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * DIAG_BLOCK + threadIdx.x; i < n; i += gridDim.x * DIAG_BLOCK) {
+        const float4 p = pos[i], v = vel[i], w = omg[i];
+        acc[8] += (double)hcount[i];
+        if (v.w <= 0.0f) continue;  // static spheres carry no momentum
+        const double m = 1.0 / (double)v.w;
+        const double I = w.w > 0.0f ? 1.0 / (double)w.w : 0.0;
+        const double px = m * v.x, py = m * v.y, pz = m * v.z;
+        acc[0] += px;
+        acc[1] += py;
+        acc[2] += pz;
+        acc[3] += (double)p.y * pz - (double)p.z * py + I * w.x;
+        acc[4] += (double)p.z * px - (double)p.x * pz + I * w.y;
+        acc[5] += (double)p.x * py - (double)p.y * px + I * w.z;
+        acc[6] += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) +
+                  0.5 * I * ((double)w.x * w.x + (double)w.y * w.y + (double)w.z * w.z);
+        acc[7] += -m * ((double)gx * p.x + (double)gy * p.y + (double)gz * p.z);
+    }
+    for (int k = 0; k < DIAG_VALUES; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int h = DIAG_BLOCK / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < DIAG_VALUES; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < DIAG_VALUES) partial[(size_t)blockIdx.x * DIAG_VALUES + threadIdx.x] = red[threadIdx.x][0];
+}
+
+struct GraphKey {
+    int keys, hist;
+    bool operator<(const GraphKey& o) const { return keys != o.keys ? keys < o.keys : hist < o.hist; }
+};
+
+}  // namespace sd
+
+using namespace sd;
+
+struct sd_ctx {
+    sd_sphere_params p{};
+    hipStream_t stream = nullptr;
+    int n = 0;
+    int n_cells = 0;
+    int key_bits = 0;
+    int items = 0;
+    int contact_block = 128;
+    // state: A = current order (keys refer to it), B = sorted scratch
+    float4 *posA = nullptr, *velA = nullptr, *omgA = nullptr;
+    float4 *posB = nullptr, *velB = nullptr, *omgB = nullptr;
+    int *idsA = nullptr, *idsB = nullptr;
+    uint32_t* k32[2] = {nullptr, nullptr};
+    uint32_t* v32[2] = {nullptr, nullptr};
+    uint64_t* k64[2] = {nullptr, nullptr};
+    int cur_keys = 0;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int* ranges = nullptr;
+    int* hcount[2] = {nullptr, nullptr};
+    float4* hist[2] = {nullptr, nullptr};
+    int cur_hist = 0;
+    int* overflow = nullptr;
+    double* diag = nullptr;
+    int diag_blocks = 0;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    std::vector<hipGraph_t> graph_objs;
+    long long steps_done = 0;
+};
+
+namespace {
+
+void release(sd_ctx* c) {
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto g : c->graph_objs) (void)hipGraphDestroy(g);
+    c->graphs.clear();
+    c->graph_objs.clear();
+    void* bufs[] = {c->posA, c->velA, c->omgA, c->posB, c->velB, c->omgB, c->idsA, c->idsB, c->k32[0], c->k32[1],
+                    c->v32[0], c->v32[1], c->k64[0], c->k64[1], c->sort_tmp, c->ranges, c->hcount[0], c->hcount[1],
+                    c->hist[0], c->hist[1], c->overflow, c->diag};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    c->posA = c->velA = c->omgA = c->posB = c->velB = c->omgB = nullptr;
+    c->idsA = c->idsB = nullptr;
+    c->k32[0] = c->k32[1] = c->v32[0] = c->v32[1] = nullptr;
+    c->k64[0] = c->k64[1] = nullptr;
+    c->sort_tmp = nullptr;
+    c->ranges = nullptr;
+    c->hcount[0] = c->hcount[1] = nullptr;
+    c->hist[0] = c->hist[1] = nullptr;
+    c->overflow = nullptr;
+    c->diag = nullptr;
+    c->n = 0;
+}
+
+template <typename T>
+sd_status dalloc(T** p, size_t count) {
+    SD_HIP(hipMalloc((void**)p, (count ? count : 1) * sizeof(T)));
+    return SD_OK;
+}
+
+// Enqueues one step on the context stream. If ev is given, ev[0..4] are
+// recorded before the sort and after the sort, ranges, permute and contact stages.
+sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
+    hipStream_t s = c->stream;
+#define MARK(i) \
+    if (ev) SD_HIP(hipEventRecord(ev[i], s))
+    const int n = c->n;
+    const int ho = c->cur_hist, hn = 1 - c->cur_hist;
+    ContactArgs a = {};
+    a.pos = c->posB;
+    a.vel = c->velB;
+    a.omg = c->omgB;
+    a.ids = c->idsB;
+    a.n = n;
+    a.g = to_dev(c->p.grid);
+    a.ranges = c->ranges;
+    a.c = to_dev(c->p.contact);
+    a.hcount_old = c->hcount[ho];
+    a.hist_old = c->hist[ho];
+    a.hcount_new = c->hcount[hn];
+    a.hist_new = c->hist[hn];
+    a.overflow = c->overflow;
+    a.pos_o = c->posA;
+    a.vel_o = c->velA;
+    a.omg_o = c->omgA;
+    a.ids_o = c->idsA;
+    sd_status st;
+    MARK(0);
+    if (c->p.key64) {
+        uint64_t* K = c->k64[c->cur_keys];
+        uint64_t* Ka = c->k64[1 - c->cur_keys];
+        st = radix_sort_keys_u64(&K, &Ka, n, 32, 32 + c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s, c->items);
+        if (st != SD_OK) return st;
+        MARK(1);
+        st = launch_cell_ranges_u64(K, n, c->n_cells, c->ranges, s);
+        if (st != SD_OK) return st;
+        MARK(2);
+        st = launch_permute(nullptr, K, n, c->posA, c->velA, c->omgA, c->idsA, c->posB, c->velB, c->omgB, c->idsB, s);
+        if (st != SD_OK) return st;
+        MARK(3);
+        a.old_slot_u64 = K;
+        a.keys_o64 = Ka;
+        st = launch_contacts(CONTACT_FUSED_U64, a, s, c->contact_block);
+        if (st != SD_OK) return st;
+        MARK(4);
+        c->cur_keys = (Ka == c->k64[0]) ? 0 : 1;
+    } else {
+        uint32_t* K = c->k32[c->cur_keys];
+        uint32_t* V = c->v32[c->cur_keys];
+        uint32_t* Ka = c->k32[1 - c->cur_keys];
+        uint32_t* Va = c->v32[1 - c->cur_keys];
+        st = radix_sort_pairs_u32(&K, &V, &Ka, &Va, n, 0, c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s, c->items);
+        if (st != SD_OK) return st;
+        MARK(1);
+        st = launch_cell_ranges_u32(K, n, c->n_cells, c->ranges, s);
+        if (st != SD_OK) return st;
+        MARK(2);
+        st = launch_permute(V, nullptr, n, c->posA, c->velA, c->omgA, c->idsA, c->posB, c->velB, c->omgB, c->idsB, s);
+        if (st != SD_OK) return st;
+        MARK(3);
+        a.old_slot_u32 = V;
+        a.keys_o32 = Ka;
+        a.vals_o32 = Va;
+        st = launch_contacts(CONTACT_FUSED_U32, a, s, c->contact_block);
+        if (st != SD_OK) return st;
+        MARK(4);
+        c->cur_keys = (Ka == c->k32[0]) ? 0 : 1;
+    }
+#undef MARK
+    c->cur_hist = hn;
+    return SD_OK;
+}
+
+sd_status run_graph_pair(sd_ctx* c) {
+    const GraphKey key{c->cur_keys, c->cur_hist};
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        const int k0 = c->cur_keys, h0 = c->cur_hist;
+        SD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        sd_status st = enqueue_step(c);
+        if (st == SD_OK) st = enqueue_step(c);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (st != SD_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        SD_HIP(e);
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            SD_HIP(e);
+        }
+        c->graph_objs.push_back(g);
+        it = c->graphs.emplace(key, ge).first;
+        // capture advanced the host-side roles; rewind them for the replay
+        c->cur_keys = k0;
+        c->cur_hist = h0;
+    }
+    SD_HIP(hipGraphLaunch(it->second, c->stream));
+    // two steps advance the roles by an even amount: keys and history return to
+    // their starting buffers (period two)
+    return SD_OK;
+}
+
+}  // namespace
+
+extern "C" sd_status sd_ctx_create(const sd_sphere_params* p, sd_ctx** out) {
+    SD_REQUIRE(p && out, "sd_ctx_create: NULL argument");
+    *out = nullptr;
+    sd_status st = check_grid(&p->grid);
+    if (st != SD_OK) return st;
+    st = check_contact(&p->contact);
+    if (st != SD_OK) return st;
+    SD_REQUIRE(p->tile_items == 0 || p->tile_items == 4 || p->tile_items == 8 || p->tile_items == 16,
+               "sd_ctx_create: tile_items must be 0, 4, 8 or 16");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_error(SD_ERR_NO_DEVICE, "sd_ctx_create: no HIP device");
+    sd_ctx* c = new sd_ctx();
+    c->p = *p;
+    c->n_cells = p->grid.n[0] * p->grid.n[1] * p->grid.n[2];
+    c->key_bits = grid_bits(p->grid);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
+    *out = c;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_ctx_destroy(sd_ctx* c) {
+    if (!c) return SD_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_ctx_stream(sd_ctx* c, void** stream) {
+    SD_REQUIRE(c && stream, "sd_ctx_stream: NULL");
+    *stream = (void*)c->stream;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_ctx_upload_spheres(sd_ctx* c, int n, const float* x, const float* y, const float* z,
+                                           const float* vx, const float* vy, const float* vz, const float* radius,
+                                           const float* mass) {
+    SD_REQUIRE(c, "sd_ctx_upload_spheres: NULL context");
+    SD_REQUIRE(n > 0, "sd_ctx_upload_spheres: n must be positive");
+    SD_REQUIRE(x && y && z && vx && vy && vz && radius && mass, "sd_ctx_upload_spheres: NULL array");
+    const float dmax = 2.0f * [&] {
+        float m = 0.0f;
+        for (int i = 0; i < n; ++i) m = radius[i] > m ? radius[i] : m;
+        return m;
+    }();
+    if (dmax > c->p.grid.cell)
+        return set_error(SD_ERR_ARG, "sd_ctx_upload_spheres: largest diameter %g exceeds the cell size %g", dmax,
+                         c->p.grid.cell);
+    for (int i = 0; i < n; ++i) {
+        if (!(radius[i] > 0.0f) || !(mass[i] >= 0.0f) || !isfinite(x[i]) || !isfinite(y[i]) || !isfinite(z[i]))
+            return set_error(SD_ERR_ARG, "sd_ctx_upload_spheres: sphere %d has a bad radius, mass or position", i);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    release(c);
+    c->n = n;
+    const int maxc = c->p.contact.max_contacts;
+    sd_status st;
+#define ALLOC(ptr, count)                 \
+    st = dalloc(&(ptr), (size_t)(count)); \
+    if (st != SD_OK) {                    \
+        release(c);                       \
+        return st;                        \
+    }
+    ALLOC(c->posA, n);
+    ALLOC(c->velA, n);
+    ALLOC(c->omgA, n);
+    ALLOC(c->posB, n);
+    ALLOC(c->velB, n);
+    ALLOC(c->omgB, n);
+    ALLOC(c->idsA, n);
+    ALLOC(c->idsB, n);
+    if (c->p.key64) {
+        ALLOC(c->k64[0], n);
+        ALLOC(c->k64[1], n);
+    } else {
+        ALLOC(c->k32[0], n);
+        ALLOC(c->k32[1], n);
+        ALLOC(c->v32[0], n);
+        ALLOC(c->v32[1], n);
+    }
+    c->sort_tmp_bytes = radix_temp_bytes(n);
+    ALLOC(*(char**)&c->sort_tmp, c->sort_tmp_bytes);
+    ALLOC(c->ranges, 2 * (size_t)c->n_cells);
+    ALLOC(c->hcount[0], n);
+    ALLOC(c->hcount[1], n);
+    ALLOC(c->hist[0], (size_t)maxc * n);
+    ALLOC(c->hist[1], (size_t)maxc * n);
+    ALLOC(c->overflow, 1);
+    c->diag_blocks = div_up(n, DIAG_BLOCK) < 1024 ? div_up(n, DIAG_BLOCK) : 1024;
+    ALLOC(c->diag, (size_t)c->diag_blocks * DIAG_VALUES);
+#undef ALLOC
+    c->items = c->p.tile_items ? c->p.tile_items : radix_default_items(n);
+
+    std::vector<float4> pos(n), vel(n), omg(n);
+    std::vector<int> ids(n);
+    for (int i = 0; i < n; ++i) {
+        const float inv_m = mass[i] > 0.0f ? 1.0f / mass[i] : 0.0f;
+        const float I = 0.4f * mass[i] * radius[i] * radius[i];
+        const float inv_I = mass[i] > 0.0f ? 1.0f / I : 0.0f;
+        pos[i] = make_float4(x[i], y[i], z[i], radius[i]);
+        vel[i] = make_float4(mass[i] > 0.0f ? vx[i] : 0.0f, mass[i] > 0.0f ? vy[i] : 0.0f,
+                             mass[i] > 0.0f ? vz[i] : 0.0f, inv_m);
+        omg[i] = make_float4(0.0f, 0.0f, 0.0f, inv_I);
+        ids[i] = i;
+    }
+    SD_HIP(hipMemcpyAsync(c->posA, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    SD_HIP(hipMemcpyAsync(c->velA, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    SD_HIP(hipMemcpyAsync(c->omgA, omg.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    SD_HIP(hipMemcpyAsync(c->idsA, ids.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    st = launch_fill_u32((uint32_t*)c->hcount[0], 0u, n, c->stream);
+    if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->hcount[1], 0u, n, c->stream);
+    if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->overflow, 0u, 1, c->stream);
+    if (st != SD_OK) return st;
+    c->cur_keys = 0;
+    c->cur_hist = 0;
+    const GridDev g = to_dev(c->p.grid);
+    st = c->p.key64 ? launch_hash_keys_u64(c->posA, n, g, c->k64[0], c->stream)
+                    : launch_hash_keys_u32(c->posA, n, g, c->k32[0], c->v32[0], c->stream);
+    if (st != SD_OK) return st;
+    SD_HIP(hipStreamSynchronize(c->stream));
+    c->steps_done = 0;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_ctx_step(sd_ctx* c, int nsteps) {
+    SD_REQUIRE(c, "sd_ctx_step: NULL context");
+    SD_REQUIRE(nsteps >= 0, "sd_ctx_step: nsteps < 0");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_step: no spheres uploaded");
+    sd_status st = launch_fill_u32((uint32_t*)c->overflow, 0u, 1, c->stream);
+    if (st != SD_OK) return st;
+    int left = nsteps;
+    if (c->p.use_graph) {
+        while (left >= 2) {
+            st = run_graph_pair(c);
+            if (st != SD_OK) return st;
+            left -= 2;
+        }
+    }
+    while (left > 0) {
+        st = enqueue_step(c);
+        if (st != SD_OK) return st;
+        --left;
+    }
+    int ovf = 0;
+    SD_HIP(hipMemcpyAsync(&ovf, c->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipStreamSynchronize(c->stream));
+    c->steps_done += nsteps;
+    if (ovf > 0)
+        return set_error(SD_ERR_CAPACITY, "sd_ctx_step: a sphere had %d contacts, capacity is %d", ovf,
+                         c->p.contact.max_contacts);
+    return SD_OK;
+}
+
+extern "C" sd_status sd_ctx_profile(sd_ctx* c, int nsteps, float* stage_ms, int nstages) {
+    SD_REQUIRE(c && stage_ms && nstages >= 1 && nsteps >= 1, "sd_ctx_profile: bad argument");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_profile: no spheres uploaded");
+    const int ns = nstages < 4 ? nstages : 4;
+    for (int k = 0; k < nstages; ++k) stage_ms[k] = 0.0f;
+    hipEvent_t ev[5];
+    for (int k = 0; k < 5; ++k) SD_HIP(hipEventCreate(&ev[k]));
+    sd_status st = SD_OK;
+    for (int i = 0; i < nsteps && st == SD_OK; ++i) {
+        st = enqueue_step(c, ev);
+        if (st != SD_OK) break;
+        hipError_t e = hipEventSynchronize(ev[4]);
+        if (e != hipSuccess) {
+            st = hip_error(e, "hipEventSynchronize", __FILE__, __LINE__);
+            break;
+        }
+        for (int k = 0; k < ns; ++k) {
+            float ms = 0.0f;
+            e = hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            if (e != hipSuccess) {
+                st = hip_error(e, "hipEventElapsedTime", __FILE__, __LINE__);
+                break;
+            }
+            stage_ms[k] += ms;
+        }
+    }
+    for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
+    if (st == SD_OK) c->steps_done += nsteps;
+    return st;
+}
+
+extern "C" sd_status sd_ctx_download_spheres(sd_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz,
+                                             float* wx, float* wy, float* wz) {
+    SD_REQUIRE(c, "sd_ctx_download_spheres: NULL context");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_download_spheres: no spheres uploaded");
+    const int n = c->n;
+    std::vector<float4> pos(n), vel(n), omg(n);
+    std::vector<int> ids(n);
+    SD_HIP(hipMemcpyAsync(pos.data(), c->posA, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipMemcpyAsync(vel.data(), c->velA, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipMemcpyAsync(omg.data(), c->omgA, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipMemcpyAsync(ids.data(), c->idsA, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < n; ++s) {
+        const int i = ids[s];
+        if (i < 0 || i >= n) return set_error(SD_ERR_STATE, "sd_ctx_download_spheres: corrupt id %d", i);
+        if (x) x[i] = pos[s].x;
+        if (y) y[i] = pos[s].y;
+        if (z) z[i] = pos[s].z;
+        if (vx) vx[i] = vel[s].x;
+        if (vy) vy[i] = vel[s].y;
+        if (vz) vz[i] = vel[s].z;
+        if (wx) wx[i] = omg[s].x;
+        if (wy) wy[i] = omg[s].y;
+        if (wz) wz[i] = omg[s].z;
+    }
+    return SD_OK;
+}
+
+static sd_status diag_values(sd_ctx* c, double out[DIAG_VALUES]) {
+    const ContactDev cd = to_dev(c->p.contact);
+    hipLaunchKernelGGL(k_diag_partial, dim3(c->diag_blocks), dim3(DIAG_BLOCK), 0, c->stream, c->posA, c->velA,
+                       c->omgA, c->hcount[c->cur_hist], c->n, cd.gx, cd.gy, cd.gz, c->diag);
+    SD_LAUNCH_CHECK();
+    std::vector<double> part((size_t)c->diag_blocks * DIAG_VALUES);
+    SD_HIP(hipMemcpyAsync(part.data(), c->diag, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < DIAG_VALUES; ++k) out[k] = 0.0;
+    for (int b = 0; b < c->diag_blocks; ++b)
+        for (int k = 0; k < DIAG_VALUES; ++k) out[k] += part[(size_t)b * DIAG_VALUES + k];
+    return SD_OK;
+}
+
+extern "C" sd_status sd_ctx_diagnostics(sd_ctx* c, double out[8]) {
+    SD_REQUIRE(c && out, "sd_ctx_diagnostics: NULL");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_diagnostics: no spheres uploaded");
+    double v[DIAG_VALUES];
+    sd_status st = diag_values(c, v);
+    if (st != SD_OK) return st;
+    for (int k = 0; k < 8; ++k) out[k] = v[k];
+    return SD_OK;
+}
+
+extern "C" sd_status sd_ctx_contact_count(sd_ctx* c, long long* count) {
+    SD_REQUIRE(c && count, "sd_ctx_contact_count: NULL");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_contact_count: no spheres uploaded");
+    double v[DIAG_VALUES];
+    sd_status st = diag_values(c, v);
+    if (st != SD_OK) return st;
+    *count = (long long)llround(v[8]);
+    return SD_OK;
+}
+
+// ------------------------------------------------------------------ checkpoint
+
+namespace {
+struct StateHeader {
+    uint32_t magic;
+    uint32_t version;
+    int32_t n;
+    int32_t maxc;
+    int32_t key64;
+    int32_t n_cells;
+    int64_t steps_done;
+};
+constexpr uint32_t STATE_MAGIC = 0x53445354u;  // "SDST"
+
+size_t state_bytes(const sd_ctx* c) {
+    const size_t n = (size_t)c->n;
+    return sizeof(StateHeader) + n * (3 * sizeof(float4) + sizeof(int)) +
+           n * (c->p.key64 ? sizeof(uint64_t) : 2 * sizeof(uint32_t)) + n * sizeof(int) +
+           (size_t)c->p.contact.max_contacts * n * sizeof(float4);
+}
+}  // namespace
+
+extern "C" sd_status sd_ctx_state_bytes(sd_ctx* c, size_t* bytes) {
+    SD_REQUIRE(c && bytes, "sd_ctx_state_bytes: NULL");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_state_bytes: no spheres uploaded");
+    *bytes = state_bytes(c);
+    return SD_OK;
+}
+
+extern "C" sd_status sd_ctx_save(sd_ctx* c, void* buffer, size_t bytes) {
+    SD_REQUIRE(c && buffer, "sd_ctx_save: NULL");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_save: no spheres uploaded");
+    SD_REQUIRE(bytes >= state_bytes(c), "sd_ctx_save: buffer too small (%zu < %zu)", bytes, state_bytes(c));
+    const size_t n = (size_t)c->n;
+    char* b = (char*)buffer;
+    StateHeader h{STATE_MAGIC, 1u, c->n, c->p.contact.max_contacts, c->p.key64, c->n_cells, c->steps_done};
+    memcpy(b, &h, sizeof h);
+    b += sizeof h;
+    auto d2h = [&](const void* src, size_t sz) -> sd_status {
+        SD_HIP(hipMemcpyAsync(b, src, sz, hipMemcpyDeviceToHost, c->stream));
+        b += sz;
+        return SD_OK;
+    };
+    sd_status st = SD_OK;
+    if (st == SD_OK) st = d2h(c->posA, n * sizeof(float4));
+    if (st == SD_OK) st = d2h(c->velA, n * sizeof(float4));
+    if (st == SD_OK) st = d2h(c->omgA, n * sizeof(float4));
+    if (st == SD_OK) st = d2h(c->idsA, n * sizeof(int));
+    if (c->p.key64) {
+        if (st == SD_OK) st = d2h(c->k64[c->cur_keys], n * sizeof(uint64_t));
+    } else {
+        if (st == SD_OK) st = d2h(c->k32[c->cur_keys], n * sizeof(uint32_t));
+        if (st == SD_OK) st = d2h(c->v32[c->cur_keys], n * sizeof(uint32_t));
+    }
+    if (st == SD_OK) st = d2h(c->hcount[c->cur_hist], n * sizeof(int));
+    if (st == SD_OK) st = d2h(c->hist[c->cur_hist], (size_t)c->p.contact.max_contacts * n * sizeof(float4));
+    if (st != SD_OK) return st;
+    SD_HIP(hipStreamSynchronize(c->stream));
+    return SD_OK;
+}
+
+extern "C" sd_status sd_ctx_load(sd_ctx* c, const void* buffer, size_t bytes) {
+    SD_REQUIRE(c && buffer, "sd_ctx_load: NULL");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_load: upload a scene of the same size first");
+    SD_REQUIRE(bytes >= state_bytes(c), "sd_ctx_load: buffer too small");
+    StateHeader h;
+    memcpy(&h, buffer, sizeof h);
+    SD_REQUIRE(h.magic == STATE_MAGIC && h.version == 1u, "sd_ctx_load: not a state buffer");
+    SD_REQUIRE(h.n == c->n && h.maxc == c->p.contact.max_contacts && h.key64 == c->p.key64 &&
+                   h.n_cells == c->n_cells,
+               "sd_ctx_load: state was saved from a context with different sizes");
+    const size_t n = (size_t)c->n;
+    const char* b = (const char*)buffer + sizeof h;
+    c->cur_keys = 0;
+    c->cur_hist = 0;
+    auto h2d = [&](void* dst, size_t sz) -> sd_status {
+        SD_HIP(hipMemcpyAsync(dst, b, sz, hipMemcpyHostToDevice, c->stream));
+        b += sz;
+        return SD_OK;
+    };
+    sd_status st = SD_OK;
+    if (st == SD_OK) st = h2d(c->posA, n * sizeof(float4));
+    if (st == SD_OK) st = h2d(c->velA, n * sizeof(float4));
+    if (st == SD_OK) st = h2d(c->omgA, n * sizeof(float4));
+    if (st == SD_OK) st = h2d(c->idsA, n * sizeof(int));
+    if (c->p.key64) {
+        if (st == SD_OK) st = h2d(c->k64[0], n * sizeof(uint64_t));
+    } else {
+        if (st == SD_OK) st = h2d(c->k32[0], n * sizeof(uint32_t));
+        if (st == SD_OK) st = h2d(c->v32[0], n * sizeof(uint32_t));
+    }
+    if (st == SD_OK) st = h2d(c->hcount[0], n * sizeof(int));
+    if (st == SD_OK) st = h2d(c->hist[0], (size_t)c->p.contact.max_contacts * n * sizeof(float4));
+    if (st != SD_OK) return st;
+    SD_HIP(hipStreamSynchronize(c->stream));
+    c->steps_done = h.steps_done;
+    return SD_OK;
+}

@@ -1,0 +1,103 @@
+// Internal C++ interfaces shared between the translation units of
+// libstardust_hip.so (not part of the C ABI).
+#pragma once
+
+#include "sd_common.hpp"
+
+namespace sd {
+
+// ------------------------------------------------------------------ sort
+// Bytes of scratch the radix sort needs for n elements (any tile size).
+size_t radix_temp_bytes(int n);
+// Items per thread the sort uses for n elements when the caller passes 0.
+int radix_default_items(int n);
+
+// Stable LSD sort over key bits [begin_bit, end_bit). On return *keys (and
+// *vals) point at the buffer holding the sorted data; the other buffer of each
+// pair is left in *alt. items = 4, 8 or 16 (0 = default for n).
+sd_status radix_sort_pairs_u32(uint32_t** keys, uint32_t** vals, uint32_t** keys_alt, uint32_t** vals_alt,
+                               int n, int begin_bit, int end_bit, void* temp, size_t temp_bytes,
+                               hipStream_t s, int items);
+sd_status radix_sort_keys_u64(uint64_t** keys, uint64_t** keys_alt, int n, int begin_bit, int end_bit,
+                              void* temp, size_t temp_bytes, hipStream_t s, int items);
+// Number of LSD passes the sort makes over a bit range.
+int radix_passes(int begin_bit, int end_bit);
+
+// ------------------------------------------------------------------ scan
+// Exclusive prefix sum of n int32 values (in may equal out). scratch must
+// hold scan_temp_bytes(n) bytes. If total != nullptr the grand total is
+// written there (device pointer).
+size_t scan_temp_bytes(int n);
+sd_status exclusive_scan_i32(const int* in, int* out, int n, int* total, void* scratch, hipStream_t s);
+
+// ------------------------------------------------------------------ grid
+struct GridDev {
+    float lox, loy, loz, cell;
+    int nx, ny, nz;
+};
+inline GridDev to_dev(const sd_grid& g) {
+    return GridDev{g.lo[0], g.lo[1], g.lo[2], g.cell, g.n[0], g.n[1], g.n[2]};
+}
+int grid_bits(const sd_grid& g);  // bits needed for the largest cell id
+sd_status check_grid(const sd_grid* g);
+
+sd_status launch_fill_u32(uint32_t* dst, uint32_t value, size_t count, hipStream_t s);
+sd_status launch_hash_keys_u32(const float4* pos4, int n, const GridDev& g, uint32_t* keys, uint32_t* vals,
+                               hipStream_t s);
+sd_status launch_hash_keys_u64(const float4* pos4, int n, const GridDev& g, uint64_t* keys, hipStream_t s);
+sd_status launch_cell_ranges_u32(const uint32_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
+sd_status launch_cell_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
+// perm is either u32 values (perm32) or the low halves of u64 keys (perm64).
+sd_status launch_permute(const uint32_t* perm32, const uint64_t* perm64, int n, const float4* pos, const float4* vel,
+                         const float4* omg, const int* ids, float4* pos_o, float4* vel_o, float4* omg_o, int* ids_o,
+                         hipStream_t s);
+
+// ------------------------------------------------------------------ contacts
+struct ContactDev {
+    float kn, kt, c2z, mu, dt;
+    float gx, gy, gz;
+    int walls;
+    float wlo[3], whi[3];
+    int maxc;
+};
+ContactDev to_dev(const sd_contact_params& c);
+sd_status check_contact(const sd_contact_params* c);
+
+// Fused step kernel modes.
+enum { CONTACT_FORCES = 0, CONTACT_FUSED_U32 = 1, CONTACT_FUSED_U64 = 2 };
+
+struct ContactArgs {
+    const float4* pos;  // sorted state
+    const float4* vel;
+    const float4* omg;
+    const int* ids;
+    int n;
+    GridDev g;
+    const int* ranges;
+    ContactDev c;
+    // old order: either an int array, u32 sort values or packed u64 keys
+    const int* old_slot_i32;
+    const uint32_t* old_slot_u32;
+    const uint64_t* old_slot_u64;
+    const int* hcount_old;
+    const float4* hist_old;
+    int* hcount_new;
+    float4* hist_new;
+    int* overflow;       // receives max contacts seen when > capacity
+    // CONTACT_FORCES outputs
+    float4* force;
+    float4* torque;
+    // fused outputs: integrated state + next keys
+    float4* pos_o;
+    float4* vel_o;
+    float4* omg_o;
+    int* ids_o;
+    uint32_t* keys_o32;
+    uint32_t* vals_o32;
+    uint64_t* keys_o64;
+};
+sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int block);
+sd_status launch_integrate(float4* pos, float4* vel, float4* omg, const float4* f, const float4* t, int n,
+                           const ContactDev& c, hipStream_t s);
+
+}  // namespace sd

@@ -1,0 +1,120 @@
+// Seeded synthetic scene generators (host only).
+//
+// The reference builds scenes from unseeded rand() (sandbox/testDEM.cpp:53-58,90-98)
+// and lattice entities (src/engine/entities/entity.cpp:9-59). These generators give
+// reproducible inputs for every benchmark configuration instead:
+//   SplitMix64 stream, uniform = (u >> 40) * 2^-24, normal = Box-Muller,
+//   lognormal = exp(ln(median) + sigma * normal), clipped to [r_min, r_max],
+//   random sequential addition (RSA) on a hash grid so no two spheres overlap,
+//   largest spheres first (ids keep their radius, so id order stays unrelated
+//   to position; big-first addition reaches far higher packing than random order).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "sd_internal.hpp"
+
+namespace sd {
+
+struct SplitMix64 {
+    uint64_t s;
+    explicit SplitMix64(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    float uniform() { return (float)(next() >> 40) * (1.0f / 16777216.0f); }
+    double normal() {
+        const double u1 = 1.0 - (double)(next() >> 11) * (1.0 / 9007199254740992.0);  // (0, 1]
+        const double u2 = (double)(next() >> 11) * (1.0 / 9007199254740992.0);
+        return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+};
+
+}  // namespace sd
+
+extern "C" sd_status sd_gen_spheres(uint64_t seed, int n, const float lo[3], const float hi[3], float r_median,
+                                    float r_sigma, float r_min, float r_max, float* x, float* y, float* z,
+                                    float* radius) {
+    SD_REQUIRE(n >= 0 && lo && hi && x && y && z && radius, "sd_gen_spheres: bad argument");
+    SD_REQUIRE(r_median > 0.0f && r_sigma >= 0.0f && r_min > 0.0f && r_max >= r_min,
+               "sd_gen_spheres: bad radius distribution");
+    for (int a = 0; a < 3; ++a)
+        SD_REQUIRE(hi[a] - lo[a] > 2.0f * r_max, "sd_gen_spheres: domain smaller than a sphere");
+    sd::SplitMix64 rng(seed);
+    // radii first (one value per sphere, in id order)
+    for (int i = 0; i < n; ++i) {
+        float r = r_median;
+        if (r_sigma > 0.0f) r = (float)exp(log((double)r_median) + (double)r_sigma * rng.normal());
+        r = r < r_min ? r_min : (r > r_max ? r_max : r);
+        radius[i] = r;
+    }
+    // RSA on a grid of cell 2 r_max: an overlap partner lies in the 27 cells.
+    const double cell = 2.0 * (double)r_max;
+    int nc[3];
+    for (int a = 0; a < 3; ++a) {
+        double c = ceil(((double)hi[a] - (double)lo[a]) / cell);
+        SD_REQUIRE(c < 4096.0, "sd_gen_spheres: grid too large");
+        nc[a] = (int)c < 1 ? 1 : (int)c;
+    }
+    const size_t ncells = (size_t)nc[0] * nc[1] * nc[2];
+    std::vector<int> head(ncells, -1), next((size_t)n, -1);
+    auto cell_of = [&](float p, int a) {
+        int c = (int)floor(((double)p - (double)lo[a]) / cell);
+        return c < 0 ? 0 : (c >= nc[a] ? nc[a] - 1 : c);
+    };
+    const int max_attempts = 20000;
+    std::vector<int> order((size_t)n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return radius[a] > radius[b]; });
+    for (int oi = 0; oi < n; ++oi) {
+        const int i = order[(size_t)oi];
+        const float r = radius[i];
+        bool placed = false;
+        for (int att = 0; att < max_attempts && !placed; ++att) {
+            float p[3];
+            for (int a = 0; a < 3; ++a) p[a] = (lo[a] + r) + rng.uniform() * ((hi[a] - r) - (lo[a] + r));
+            const int cx = cell_of(p[0], 0), cy = cell_of(p[1], 1), cz = cell_of(p[2], 2);
+            bool ok = true;
+            for (int dz = -1; dz <= 1 && ok; ++dz) {
+                const int zz = cz + dz;
+                if (zz < 0 || zz >= nc[2]) continue;
+                for (int dy = -1; dy <= 1 && ok; ++dy) {
+                    const int yy = cy + dy;
+                    if (yy < 0 || yy >= nc[1]) continue;
+                    for (int dx = -1; dx <= 1 && ok; ++dx) {
+                        const int xx = cx + dx;
+                        if (xx < 0 || xx >= nc[0]) continue;
+                        for (int j = head[((size_t)zz * nc[1] + yy) * nc[0] + xx]; j >= 0; j = next[j]) {
+                            const double ddx = (double)p[0] - x[j], ddy = (double)p[1] - y[j],
+                                         ddz = (double)p[2] - z[j];
+                            const double rs = (double)r + (double)radius[j];
+                            if (ddx * ddx + ddy * ddy + ddz * ddz < rs * rs) {
+                                ok = false;
+                                break;
+                            }
+                        }
+                    }
+                }
+            }
+            if (ok) {
+                x[i] = p[0];
+                y[i] = p[1];
+                z[i] = p[2];
+                const size_t c = ((size_t)cz * nc[1] + cy) * nc[0] + cx;
+                next[i] = head[c];
+                head[c] = i;
+                placed = true;
+            }
+        }
+        if (!placed)
+            return sd::set_error(SD_ERR_CAPACITY, "sd_gen_spheres: could not place sphere %d of %d without overlap",
+                                 i, n);
+    }
+    return SD_OK;
+}

@@ -1,0 +1,97 @@
+"""Host-side driver of one sphere scene on the GPU (wraps the sd_ctx_* C ABI).
+
+Mirrors the reference's engine life cycle: construct, add spheres, upload
+(prep + transfer), update(dt) steps, read back
+(src/engine/engine.hpp:43-190, src/engine/cuda/simulation_kernel.cu:52).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class SphereScene:
+    def __init__(self, grid, contact, key64=False, use_graph=True, tile_items=0):
+        L.require_gpu()
+        self.params = L.SphereParams()
+        self.params.grid = grid
+        self.params.contact = contact
+        self.params.key64 = 1 if key64 else 0
+        self.params.use_graph = 1 if use_graph else 0
+        self.params.tile_items = tile_items
+        h = C.c_void_p()
+        L.call("sd_ctx_create", C.byref(self.params), C.byref(h))
+        self.h = h
+        self.n = 0
+
+    def upload(self, x, y, z, radius, mass, vx=None, vy=None, vz=None):
+        n = len(x)
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        zero = np.zeros(n, np.float32)
+        arrs = [f(x), f(y), f(z), f(vx if vx is not None else zero), f(vy if vy is not None else zero),
+                f(vz if vz is not None else zero), f(radius), f(mass)]
+        L.call("sd_ctx_upload_spheres", self.h, n, *[L.ptr(a) for a in arrs])
+        self.n = n
+
+    def step(self, nsteps=1):
+        L.call("sd_ctx_step", self.h, int(nsteps))
+
+    def profile(self, nsteps):
+        """Per-stage GPU time in ms summed over nsteps eager steps:
+        sort, ranges, permute, contact (HIP events on the context stream)."""
+        out = np.zeros(4, np.float32)
+        L.call("sd_ctx_profile", self.h, int(nsteps), L.ptr(out), 4)
+        return {"sort": float(out[0]), "ranges": float(out[1]), "permute": float(out[2]),
+                "contact": float(out[3])}
+
+    def download(self):
+        n = self.n
+        out = {k: np.empty(n, np.float32) for k in ("x", "y", "z", "vx", "vy", "vz", "wx", "wy", "wz")}
+        L.call("sd_ctx_download_spheres", self.h, *[L.ptr(out[k]) for k in
+                                                    ("x", "y", "z", "vx", "vy", "vz", "wx", "wy", "wz")])
+        return out
+
+    def contact_count(self):
+        c = C.c_longlong()
+        L.call("sd_ctx_contact_count", self.h, C.byref(c))
+        return c.value
+
+    def diagnostics(self):
+        out = np.zeros(8, np.float64)
+        L.call("sd_ctx_diagnostics", self.h, L.ptr(out))
+        return {"momentum": out[0:3].copy(), "angular_momentum": out[3:6].copy(), "kinetic": float(out[6]),
+                "potential": float(out[7])}
+
+    def stream(self):
+        s = C.c_void_p()
+        L.call("sd_ctx_stream", self.h, C.byref(s))
+        return s
+
+    def save(self):
+        nb = C.c_size_t()
+        L.call("sd_ctx_state_bytes", self.h, C.byref(nb))
+        buf = np.empty(nb.value, np.uint8)
+        L.call("sd_ctx_save", self.h, L.ptr(buf), nb.value)
+        return buf
+
+    def load(self, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        L.call("sd_ctx_load", self.h, L.ptr(buf), buf.nbytes)
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            L.load().sd_ctx_destroy(self.h)
+        self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,83 @@
+"""Synthetic, seeded sphere workloads of the benchmark configurations.
+
+dem_c1: plain DEM, CPU-runnable. 65,536 monodisperse spheres, r = 0.01 m,
+        density 2500 kg/m^3, no initial overlap in a 1 m cube with 6 walls,
+        v = 0, g = -9.81 z, kn = 1e5 N/m, zeta = 0.3, mu = 0.5, dt = 1e-5 s,
+        1,000 steps, cell = 2r (50^3 cells), 32-bit cell keys.
+dem_c2: plain DEM at scale. 4,194,304 polydisperse spheres, radius lognormal
+        (median 5 mm, sigma 0.25, clipped to [2, 12] mm), kn = 2e5 N/m,
+        dt = 5e-6 s, 2,000 steps, cell = largest diameter (24 mm), packed 64-bit
+        (cell, index) keys sorted every step. The stated 0.5 x 0.5 x 4 m column
+        cannot hold 4.19M such spheres without overlap (their solid volume is
+        about 2.9 m^3), so the column keeps its 4 m height and its footprint is
+        widened to 2.0 x 2.0 m (solid fraction 0.18 at seeding, where random
+        sequential addition still finishes in about 20 s); the walled domain is
+        the whole column.
+Seeds are 0x5EED0000 + k for configuration k (1-based).
+"""
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+DENSITY = 2500.0
+KT_RATIO = 2.0 / 7.0
+
+
+def _gen(seed, n, lo, hi, r_med, r_sigma, r_min, r_max):
+    x = np.empty(n, np.float32)
+    y = np.empty(n, np.float32)
+    z = np.empty(n, np.float32)
+    r = np.empty(n, np.float32)
+    lo_a = np.asarray(lo, np.float32)
+    hi_a = np.asarray(hi, np.float32)
+    L.call("sd_gen_spheres", seed, n, L.ptr(lo_a), L.ptr(hi_a), r_med, r_sigma, r_min, r_max, L.ptr(x), L.ptr(y),
+           L.ptr(z), L.ptr(r))
+    return x, y, z, r
+
+
+def sphere_mass(r):
+    r = np.asarray(r, np.float64)
+    return (DENSITY * 4.0 / 3.0 * math.pi * r ** 3).astype(np.float32)
+
+
+def grid_for(lo, hi, cell):
+    n = [max(1, int(math.ceil((hi[a] - lo[a]) / cell))) for a in range(3)]
+    return L.Grid.make(lo, cell, n)
+
+
+class Workload:
+    def __init__(self, name, n, lo, hi, r_med, r_sigma, r_min, r_max, kn, dt, steps, key64, seed, max_contacts):
+        self.name, self.n, self.lo, self.hi = name, n, lo, hi
+        self.r_med, self.r_sigma, self.r_min, self.r_max = r_med, r_sigma, r_min, r_max
+        self.kn, self.dt, self.steps, self.key64, self.seed = kn, dt, steps, key64, seed
+        self.cell = 2.0 * r_max
+        self.max_contacts = max_contacts
+
+    def grid(self):
+        return grid_for(self.lo, self.hi, self.cell)
+
+    def contact(self):
+        return L.ContactParams.make(self.kn, 0.3, 0.5, self.dt, gravity=(0.0, 0.0, -9.81),
+                                    walls=(self.lo, self.hi), kt_ratio=KT_RATIO, max_contacts=self.max_contacts)
+
+    def generate(self, n=None):
+        n = self.n if n is None else n
+        x, y, z, r = _gen(self.seed, n, self.lo, self.hi, self.r_med, self.r_sigma, self.r_min, self.r_max)
+        return {"x": x, "y": y, "z": z, "r": r, "m": sphere_mass(r)}
+
+    def describe(self):
+        return {"workload": self.name, "spheres": self.n, "steps": self.steps, "dt": self.dt, "kn": self.kn,
+                "domain": [list(self.lo), list(self.hi)], "cell": self.cell,
+                "keys": "u64 (cell<<32|index)" if self.key64 else "u32 cell + u32 index", "seed": self.seed}
+
+
+def dem_c1(n=65536):
+    return Workload("dem_c1_plain_65536", n, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.01, 0.0, 0.01, 0.01, 1e5, 1e-5,
+                    1000, False, 0x5EED0001, 16)
+
+
+def dem_c2(n=4194304):
+    return Workload("dem_c2_column_4194304", n, (0.0, 0.0, 0.0), (2.0, 2.0, 4.0), 0.005, 0.25, 0.002, 0.012, 2e5,
+                    5e-6, 2000, True, 0x5EED0002, 24)

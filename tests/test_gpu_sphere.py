@@ -1,0 +1,375 @@
+"""GPU parity of the sphere DEM step path against the CPU oracle.
+
+Integer outputs (keys, sort order, cell ranges, pair lists, contact histories)
+must be bit-identical. Floating-point outputs (forces, torques, post-step
+state) follow the same FP32 operation order as the oracle, so they are checked
+bit-identical too; the stated acceptance bar is 1e-5 relative, and the FP64
+error of each quantity is reported.
+"""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from stardust_amd import _lib as L
+from stardust_amd import workloads as W
+from stardust_amd.scene import SphereScene
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    L.require_gpu()
+
+
+def dbuf(a):
+    return L.DeviceBuffer.from_array(np.ascontiguousarray(a))
+
+
+def empty(nbytes):
+    return L.DeviceBuffer(nbytes)
+
+
+# ------------------------------------------------------------------ keys
+
+@pytest.mark.parametrize("n", [1, 1000, 65536])
+def test_hash_keys_u32_u64(oracle, n):
+    say("hash_keys", n)
+    rng = np.random.default_rng(n)
+    pos = rng.uniform(-0.1, 1.1, (n, 3)).astype(np.float32)
+    if n > 10:
+        pos[:5] = [[0.5, 0.5, 0.5], [0.25, 0.0, 1.0], [np.nan, 0.2, 0.3], [1e30, -1e30, 0.0], [0.02, 0.04, 0.06]]
+    g = L.Grid.make((0.0, 0.0, 0.0), 0.02, (50, 50, 50))
+    ek, ev = oracle.hash_keys(pos[:, 0], pos[:, 1], pos[:, 2], g)
+    x, y, z = dbuf(pos[:, 0]), dbuf(pos[:, 1]), dbuf(pos[:, 2])
+    k, v, k64 = empty(4 * n), empty(4 * n), empty(8 * n)
+    L.call("sd_hash_keys_u32", x.ptr, y.ptr, z.ptr, n, C.byref(g), k.ptr, v.ptr, None)
+    L.call("sd_hash_keys_u64", x.ptr, y.ptr, z.ptr, n, C.byref(g), k64.ptr, None)
+    L.synchronize()
+    assert np.array_equal(k.download(np.uint32, n), ek)
+    assert np.array_equal(v.download(np.uint32, n), ev)
+    packed = (ek.astype(np.uint64) << np.uint64(32)) | np.arange(n, dtype=np.uint64)
+    assert np.array_equal(k64.download(np.uint64, n), packed)
+
+
+# ------------------------------------------------------------------ sort
+
+def gpu_sort_u32(keys, vals, b0, b1):
+    n = len(keys)
+    k, v = dbuf(keys), dbuf(vals)
+    ka, va = empty(4 * n), empty(4 * n)
+    nb = C.c_size_t()
+    L.call("sd_radix_sort_temp_bytes", n, C.byref(nb))
+    tmp = empty(nb.value)
+    L.call("sd_radix_sort_pairs_u32", k.ptr, v.ptr, ka.ptr, va.ptr, n, b0, b1, tmp.ptr, nb.value, None)
+    L.synchronize()
+    return k.download(np.uint32, n), v.download(np.uint32, n)
+
+
+def gpu_sort_u64(keys, b0, b1):
+    n = len(keys)
+    k, ka = dbuf(keys), empty(8 * n)
+    nb = C.c_size_t()
+    L.call("sd_radix_sort_temp_bytes", n, C.byref(nb))
+    tmp = empty(nb.value)
+    L.call("sd_radix_sort_keys_u64", k.ptr, ka.ptr, n, b0, b1, tmp.ptr, nb.value, None)
+    L.synchronize()
+    return k.download(np.uint64, n)
+
+
+@pytest.mark.parametrize("n", [2, 1000, 4097, 65536, (1 << 20) + 17])
+def test_radix_sort_pairs_u32(oracle, n):
+    rng = np.random.default_rng(100 + n)
+    cases = {
+        "random17": rng.integers(0, 1 << 17, n, dtype=np.uint32),
+        "all_equal": np.full(n, 77, np.uint32),
+        "descending": np.arange(n, 0, -1, dtype=np.uint32),
+        "digits_0f_ff": (rng.choice(np.array([0x0F, 0xFF, 0xF0], np.uint32), n) |
+                         (rng.choice(np.array([0x0F, 0xFF], np.uint32), n) << 8)).astype(np.uint32),
+        "full32": rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32),
+    }
+    for name, keys in cases.items():
+        for b0, b1 in ((0, 17), (0, 32), (4, 13)):
+            say("radix_sort_pairs_u32", n, name, b0, b1)
+            vals = rng.permutation(n).astype(np.uint32)
+            ek, ev = oracle.sort_pairs_u32(keys, vals, b0, b1)
+            gk, gv = gpu_sort_u32(keys, vals, b0, b1)
+            assert np.array_equal(gv, ev), (name, b0, b1)
+            assert np.array_equal(gk, ek), (name, b0, b1)
+
+
+@pytest.mark.parametrize("n", [3, 100000, 4194304])
+def test_radix_sort_keys_u64(oracle, n):
+    rng = np.random.default_rng(200 + n)
+    cells = rng.integers(0, 1 << 20, n, dtype=np.uint64)
+    packed = (cells << np.uint64(32)) | np.arange(n, dtype=np.uint64)
+    say("radix_sort_keys_u64 packed", n)
+    assert np.array_equal(gpu_sort_u64(packed, 32, 52), oracle.sort_keys_u64(packed, 32, 52))
+    full = rng.integers(0, 1 << 63, n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, n, dtype=np.uint64)
+    say("radix_sort_keys_u64 full", n)
+    assert np.array_equal(gpu_sort_u64(full, 0, 64), oracle.sort_keys_u64(full, 0, 64))
+
+
+# ------------------------------------------------------------------ ranges + pairs
+
+def test_cell_ranges(oracle):
+    say("cell_ranges")
+    rng = np.random.default_rng(9)
+    n_cells = 125000
+    keys = np.sort(rng.integers(0, n_cells, 300000, dtype=np.uint32))
+    r = empty(8 * n_cells)
+    kb = dbuf(keys)
+    L.call("sd_cell_ranges_u32", kb.ptr, len(keys), n_cells, r.ptr, None)
+    L.synchronize()
+    assert np.array_equal(r.download(np.int32, 2 * n_cells), oracle.cell_ranges_u32(keys, n_cells))
+    packed = (keys.astype(np.uint64) << np.uint64(32)) | np.arange(len(keys), dtype=np.uint64)
+    kb64 = dbuf(packed)
+    L.call("sd_cell_ranges_u64", kb64.ptr, len(keys), n_cells, r.ptr, None)
+    L.synchronize()
+    assert np.array_equal(r.download(np.int32, 2 * n_cells), oracle.cell_ranges_u64(packed, n_cells))
+
+
+def sorted_state(oracle, st, g):
+    """Sorts an oracle SphereState's rows like the step does (CPU)."""
+    keys, vals = oracle.hash_keys(st.pos4[:, 0], st.pos4[:, 1], st.pos4[:, 2], g)
+    bits = max(1, int(math.ceil(math.log2(g.n_cells))))
+    k, v = oracle.sort_pairs_u32(keys, vals, 0, bits)
+    ranges = oracle.cell_ranges_u32(k, g.n_cells)
+    return {"pos4": st.pos4[v], "vel4": st.vel4[v], "omg4": st.omg4[v], "ids": st.ids[v],
+            "ranges": ranges, "old_slot": v.astype(np.int32), "keys": k}
+
+
+def dense_state(oracle, n=6000, seed=3, box=0.2, r=0.005, maxc=32, warm_steps=5, walls=True, static=10):
+    """Random overlapping spheres (a few static); returns the oracle state after
+    warm_steps steps, the grid, the contact parameters and the initial arrays."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(r, box - r, (n, 3)).astype(np.float32)
+    rad = rng.uniform(0.6 * r, r, n).astype(np.float32)
+    m = W.sphere_mass(rad)
+    m[:static] = 0.0
+    vel = rng.normal(0, 0.2, (n, 3)).astype(np.float32)
+    init = {"x": pos[:, 0].copy(), "y": pos[:, 1].copy(), "z": pos[:, 2].copy(), "r": rad, "m": m,
+            "vx": vel[:, 0].copy(), "vy": vel[:, 1].copy(), "vz": vel[:, 2].copy()}
+    st = oracle.SphereState(pos[:, 0], pos[:, 1], pos[:, 2], rad, m, vx=vel[:, 0], vy=vel[:, 1], vz=vel[:, 2],
+                            max_contacts=maxc)
+    nc = int(math.ceil(box / (2 * r)))
+    g = L.Grid.make((0.0, 0.0, 0.0), 2 * r, (nc, nc, nc))
+    c = L.ContactParams.make(1e5, 0.3, 0.5, 2e-6, gravity=(0.0, 0.0, -9.81),
+                             walls=((0.0, 0.0, 0.0), (box, box, box)) if walls else None, max_contacts=maxc)
+    st.step(g, c, nsteps=warm_steps, nthreads=8)
+    return st, g, c, init
+
+
+def test_neighbor_pairs(oracle):
+    say("neighbor_pairs")
+    st, g, c, _ = dense_state(oracle, warm_steps=0)
+    for skin in (0.0, 0.001):
+        g2 = L.Grid.make((0.0, 0.0, 0.0), 0.01 + skin, [int(math.ceil(0.2 / (0.01 + skin)))] * 3)
+        s2 = sorted_state(oracle, st, g2)
+        expect = oracle.neighbor_pairs(s2["pos4"], g2, s2["ranges"], skin)
+        n = len(s2["pos4"])
+        nb = C.c_size_t()
+        L.call("sd_neighbor_pairs_scratch_bytes", n, C.byref(nb))
+        scratch = empty(nb.value)
+        p4, rg = dbuf(s2["pos4"]), dbuf(s2["ranges"])
+        cap = len(expect) + 10
+        out = empty(8 * cap)
+        npairs = C.c_int()
+        L.call("sd_neighbor_pairs", p4.ptr, n, C.byref(g2), rg.ptr, skin, out.ptr, cap, C.byref(npairs), scratch.ptr,
+               nb.value, None)
+        assert npairs.value == len(expect)
+        got = out.download(np.int32, 2 * npairs.value).reshape(-1, 2)
+        assert np.array_equal(got, expect)
+        # capacity overflow reports the required count
+        lib = L.load()
+        st_code = lib.sd_neighbor_pairs(p4.ptr, n, C.byref(g2), rg.ptr, skin, out.ptr, 5, C.byref(npairs),
+                                        scratch.ptr, nb.value, None)
+        assert st_code == L.SD_ERR_CAPACITY and npairs.value == len(expect)
+
+
+# ------------------------------------------------------------------ contact forces + integrate
+
+def gpu_contact_forces(s, st, g, c):
+    n = len(s["pos4"])
+    maxc = c.max_contacts
+    bufs = {k: dbuf(s[k]) for k in ("pos4", "vel4", "omg4", "ids", "ranges", "old_slot")}
+    hc_old, h_old = dbuf(st.hcount), dbuf(st.hist)
+    hc_new, h_new = empty(4 * n), empty(16 * maxc * n)
+    f, t, ovf = empty(16 * n), empty(16 * n), dbuf(np.zeros(1, np.int32))
+    L.call("sd_contact_forces", bufs["pos4"].ptr, bufs["vel4"].ptr, bufs["omg4"].ptr, bufs["ids"].ptr, n,
+           C.byref(g), bufs["ranges"].ptr, C.byref(c), bufs["old_slot"].ptr, hc_old.ptr, h_old.ptr, hc_new.ptr,
+           h_new.ptr, f.ptr, t.ptr, ovf.ptr, None)
+    L.synchronize()
+    hc = hc_new.download(np.int32, n)
+    h = h_new.download(np.float32, 4 * maxc * n).reshape(maxc, n, 4)
+    return {"force": f.download(np.float32, 4 * n).reshape(n, 4), "torque": t.download(np.float32, 4 * n).reshape(n, 4),
+            "hcount": hc, "hist": h, "overflow": int(ovf.download(np.int32, 1)[0])}
+
+
+def valid_hist(h, hc):
+    maxc, n, _ = h.shape
+    mask = np.arange(maxc)[:, None] < hc[None, :]
+    return h[mask]
+
+
+def rel_err(a, b):
+    scale = max(np.abs(b).max(), 1e-30)
+    return float(np.abs(a - b).max() / scale)
+
+
+def test_contact_forces_parity_and_fp64_error(oracle):
+    say("contact_forces dense")
+    st, g, c, _ = dense_state(oracle, warm_steps=8)
+    s = sorted_state(oracle, st, g)
+    exp = oracle.contact_forces(s["pos4"], s["vel4"], s["omg4"], s["ids"], g, s["ranges"], c, s["old_slot"],
+                                st.hcount, st.hist, nthreads=8)
+    got = gpu_contact_forces(s, st, g, c)
+    assert exp["max_contacts"] > 3
+    assert np.array_equal(got["hcount"], exp["hcount"])
+    eh = exp["hist"].reshape(c.max_contacts, -1, 4)
+    assert np.array_equal(valid_hist(got["hist"], got["hcount"]).view(np.uint32),
+                          valid_hist(eh, exp["hcount"]).view(np.uint32))
+    assert np.array_equal(got["force"], exp["force"])
+    assert np.array_equal(got["torque"], exp["torque"])
+    assert rel_err(got["force"], exp["force"]) <= 1e-5
+    # error of the FP32 GPU forces against the FP64 oracle, per quantity
+    h64 = st.hist.astype(np.float64)
+    h64[:, 3] = st.hist[:, 3].view(np.int32)
+    e64 = oracle.contact_forces_f64(s["pos4"], s["vel4"], s["omg4"], s["ids"], g, s["ranges"], c, s["old_slot"],
+                                    st.hcount, h64)
+    ef, et = rel_err(got["force"][:, :3], e64["force"][:, :3]), rel_err(got["torque"][:, :3], e64["torque"][:, :3])
+    say("fp64 error: force %.3e torque %.3e (relative to max magnitude)" % (ef, et))
+    assert ef < 1e-4 and et < 1e-3
+
+
+def test_integrate_parity(oracle):
+    say("integrate")
+    st, g, c, _ = dense_state(oracle, n=3000, warm_steps=2)
+    s = sorted_state(oracle, st, g)
+    exp = oracle.contact_forces(s["pos4"], s["vel4"], s["omg4"], s["ids"], g, s["ranges"], c, s["old_slot"],
+                                st.hcount, st.hist)
+    ep, ev, ew = oracle.integrate(s["pos4"], s["vel4"], s["omg4"], exp["force"], exp["torque"], c)
+    p4, v4, w4 = dbuf(s["pos4"]), dbuf(s["vel4"]), dbuf(s["omg4"])
+    f4, t4 = dbuf(exp["force"]), dbuf(exp["torque"])
+    n = len(ep)
+    L.call("sd_integrate_spheres", p4.ptr, v4.ptr, w4.ptr, f4.ptr, t4.ptr, n, C.byref(c), None)
+    L.synchronize()
+    assert np.array_equal(p4.download(np.float32, 4 * n).reshape(n, 4), ep)
+    assert np.array_equal(v4.download(np.float32, 4 * n).reshape(n, 4), ev)
+    assert np.array_equal(w4.download(np.float32, 4 * n).reshape(n, 4), ew)
+
+
+def test_permute(oracle):
+    say("permute")
+    st, g, c, _ = dense_state(oracle, n=2000, warm_steps=0)
+    perm = np.random.default_rng(1).permutation(2000).astype(np.int32)
+    bufs = [dbuf(a) for a in (perm, st.pos4, st.vel4, st.omg4, st.ids)]
+    outs = [empty(16 * 2000), empty(16 * 2000), empty(16 * 2000), empty(4 * 2000)]
+    L.call("sd_permute_spheres", bufs[0].ptr, 2000, *[b.ptr for b in bufs[1:]], *[o.ptr for o in outs], None)
+    L.synchronize()
+    assert np.array_equal(outs[0].download(np.float32, 8000).reshape(-1, 4), st.pos4[perm])
+    assert np.array_equal(outs[3].download(np.int32, 2000), st.ids[perm])
+
+
+# ------------------------------------------------------------------ whole steps
+
+def make_scene(init, g, c, key64, graph, items=0):
+    sc = SphereScene(g, c, key64=key64, use_graph=graph, tile_items=items)
+    zero = np.zeros_like(init["x"])
+    sc.upload(init["x"], init["y"], init["z"], init["r"], init["m"], vx=init.get("vx", zero),
+              vy=init.get("vy", zero), vz=init.get("vz", zero))
+    return sc
+
+
+def compare_scene(sc, st):
+    got = sc.download()
+    exp = st.by_id()
+    for k, (arr, col) in {"x": ("pos4", 0), "y": ("pos4", 1), "z": ("pos4", 2), "vx": ("vel4", 0),
+                          "vy": ("vel4", 1), "vz": ("vel4", 2), "wx": ("omg4", 0), "wy": ("omg4", 1),
+                          "wz": ("omg4", 2)}.items():
+        e = exp[arr][:, col]
+        g = got[k]
+        assert np.array_equal(g, e), (k, rel_err(g, e))
+
+
+@pytest.mark.parametrize("key64,graph,items", [(False, False, 0), (False, True, 4), (True, True, 8),
+                                               (True, False, 16)])
+def test_step_parity(oracle, key64, graph, items):
+    say("step parity", key64, graph, items)
+    st, g, c, init = dense_state(oracle, n=5000, seed=7, warm_steps=0)
+    # the oracle starts from id order, exactly as the upload does
+    sc = make_scene(init, g, c, key64, graph, items)
+    st.step(g, c, nsteps=25, nthreads=8)
+    sc.step(25)
+    compare_scene(sc, st)
+    assert sc.contact_count() == int(st.hcount.sum())
+    sc.close()
+
+
+def test_step_c1_workload(oracle):
+    say("step c1 workload")
+    w = W.dem_c1()
+    s = w.generate()
+    g, c = w.grid(), w.contact()
+    st = oracle.SphereState(s["x"], s["y"], s["z"], s["r"], s["m"], max_contacts=c.max_contacts)
+    sc = make_scene(s, g, c, key64=False, graph=True)
+    st.step(g, c, nsteps=20, nthreads=8)
+    sc.step(20)
+    compare_scene(sc, st)
+    sc.close()
+
+
+def test_step_deterministic_and_resume(oracle):
+    say("determinism + checkpoint")
+    st, g, c, init = dense_state(oracle, n=4000, seed=9, warm_steps=0)
+    a = make_scene(init, g, c, key64=True, graph=True)
+    b = make_scene(init, g, c, key64=True, graph=True)
+    a.step(10)
+    b.step(10)
+    saved = b.save()
+    a.step(9)
+    b.step(9)
+    da, db = a.download(), b.download()
+    for k in da:
+        assert np.array_equal(da[k], db[k])
+    b.load(saved)
+    b.step(9)
+    db = b.download()
+    for k in da:
+        assert np.array_equal(da[k], db[k])
+    a.close()
+    b.close()
+
+
+def test_contact_capacity_overflow_reported(oracle):
+    say("capacity overflow")
+    st, g, c, init = dense_state(oracle, n=4000, seed=4, warm_steps=0, maxc=2)
+    sc = make_scene(init, g, c, key64=False, graph=False)
+    with pytest.raises(L.SdError) as e:
+        sc.step(3)
+    assert e.value.code == L.SD_ERR_CAPACITY
+    sc.close()
+
+
+def test_diagnostics_momentum(oracle):
+    say("diagnostics")
+    st, g, c0, init = dense_state(oracle, n=3000, seed=5, warm_steps=0, walls=False, static=0)
+    c = L.ContactParams.make(1e5, 0.3, 0.5, 2e-6, gravity=(0.0, 0.0, 0.0), walls=None, max_contacts=32)
+    sc = make_scene(init, g, c, key64=False, graph=True)
+    d0 = sc.diagnostics()
+    sc.step(40)
+    d1 = sc.diagnostics()
+    byid = st.by_id()
+    m = np.where(byid["vel4"][:, 3] > 0, 1.0 / np.where(byid["vel4"][:, 3] > 0, byid["vel4"][:, 3], 1), 0)
+    scale = float((m[:, None] * np.abs(byid["vel4"][:, :3])).sum())
+    assert np.all(np.abs(d1["momentum"] - d0["momentum"]) < 1e-5 * scale)
+    assert d1["kinetic"] < d0["kinetic"] * 1.01  # damped contacts do not create energy
+    sc.close()
