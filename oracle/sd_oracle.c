@@ -213,8 +213,11 @@ static void law_pair_f(const law_f* L, v3f n, float delta, float ri, float rj, f
     const float m_eff = 1.0f / (inv_mi + inv_mj);
     const float cn = L->c2z * sqrtf(L->kn * m_eff);
     const float ct = L->c2z * sqrtf(L->kt * m_eff);
-    /* relative velocity of the contact points: v_r - (ri wi + rj wj) x n */
-    v3f a = {ri * wi.x + rj * wj.x, ri * wi.y + rj * wj.y, ri * wi.z + rj * wj.z};
+    /* lever arms to the mid-overlap point, shared by both spheres */
+    const float li = ri - 0.5f * delta;
+    const float lj = rj - 0.5f * delta;
+    /* relative velocity of the contact points: v_r - (li wi + lj wj) x n */
+    v3f a = {li * wi.x + lj * wj.x, li * wi.y + lj * wj.y, li * wi.z + lj * wj.z};
     v3f axn = {a.y * n.z - a.z * n.y, a.z * n.x - a.x * n.z, a.x * n.y - a.y * n.x};
     v3f vc = {vr.x - axn.x, vr.y - axn.y, vr.z - axn.z};
     const float vn = vc.x * n.x + vc.y * n.y + vc.z * n.z;
@@ -243,9 +246,9 @@ static void law_pair_f(const law_f* L, v3f n, float delta, float ri, float rj, f
     F->x = fn * n.x + ft.x;
     F->y = fn * n.y + ft.y;
     F->z = fn * n.z + ft.z;
-    T->x = -ri * (n.y * ft.z - n.z * ft.y);
-    T->y = -ri * (n.z * ft.x - n.x * ft.z);
-    T->z = -ri * (n.x * ft.y - n.y * ft.x);
+    T->x = -li * (n.y * ft.z - n.z * ft.y);
+    T->y = -li * (n.z * ft.x - n.x * ft.z);
+    T->z = -li * (n.x * ft.y - n.y * ft.x);
 }
 
 static void history_lookup_f(const float* hist4_old, int n, int os, int oc, int pid, v3f* h) {
@@ -382,7 +385,8 @@ static void law_pair_d(const or_contact* c, v3d n, double delta, double ri, doub
     const double kn = c->kn, kt = (double)c->kt_ratio * c->kn, c2z = 2.0 * c->damping_ratio;
     const double m_eff = 1.0 / (inv_mi + inv_mj);
     const double cn = c2z * sqrt(kn * m_eff), ct = c2z * sqrt(kt * m_eff);
-    v3d a = {ri * wi.x + rj * wj.x, ri * wi.y + rj * wj.y, ri * wi.z + rj * wj.z};
+    const double li = ri - 0.5 * delta, lj = rj - 0.5 * delta;
+    v3d a = {li * wi.x + lj * wj.x, li * wi.y + lj * wj.y, li * wi.z + lj * wj.z};
     v3d vc = {vr.x - (a.y * n.z - a.z * n.y), vr.y - (a.z * n.x - a.x * n.z), vr.z - (a.x * n.y - a.y * n.x)};
     const double vn = vc.x * n.x + vc.y * n.y + vc.z * n.z;
     const double fn = kn * delta - cn * vn;
@@ -406,9 +410,9 @@ static void law_pair_d(const or_contact* c, v3d n, double delta, double ri, doub
     F->x = fn * n.x + ft.x;
     F->y = fn * n.y + ft.y;
     F->z = fn * n.z + ft.z;
-    T->x = -ri * (n.y * ft.z - n.z * ft.y);
-    T->y = -ri * (n.z * ft.x - n.x * ft.z);
-    T->z = -ri * (n.x * ft.y - n.y * ft.x);
+    T->x = -li * (n.y * ft.z - n.z * ft.y);
+    T->y = -li * (n.z * ft.x - n.x * ft.z);
+    T->z = -li * (n.x * ft.y - n.y * ft.x);
 }
 
 int or_contact_forces_f64(int n, const double* pos4, const double* vel4, const double* omg4, const int* ids,

@@ -11,12 +11,14 @@
 // Contact law (per contact, FP32, no fused multiply-add so that the pair force
 // seen by each side is the exact negation of the other):
 //   n      = (x_i - x_j) / |x_i - x_j|,  delta = r_i + r_j - |x_i - x_j| > 0
+//   l_i    = r_i - delta/2 (lever arm to the mid-overlap contact point; the same
+//            point for both spheres, so pair torques conserve angular momentum)
 //   m_eff  = 1 / (1/m_i + 1/m_j),  c_n = 2 zeta sqrt(k_n m_eff),  c_t = 2 zeta sqrt(k_t m_eff)
-//   v_c    = (v_i - v_j) - (r_i w_i + r_j w_j) x n,  v_n = v_c . n,  v_t = v_c - v_n n
+//   v_c    = (v_i - v_j) - (l_i w_i + l_j w_j) x n,  v_n = v_c . n,  v_t = v_c - v_n n
 //   f_n    = k_n delta - c_n v_n
 //   xi_t   = (xi_t - (xi_t . n) n) + v_t dt                (tangential spring history)
 //   F_t    = -k_t xi_t - c_t v_t, limited to mu |f_n|; on slip xi_t = -(F_t + c_t v_t) / k_t
-//   F_i    = f_n n + F_t,  T_i = -r_i (n x F_t)
+//   F_i    = f_n n + F_t,  T_i = -l_i (n x F_t)
 // Walls are partners with r = 0, 1/m = 0, v = w = 0 and ids -1..-6.
 #include "sd_internal.hpp"
 #include "sd_grid_dev.hpp"
@@ -62,9 +64,11 @@ __device__ __forceinline__ PairForce contact_law(const ContactDev& c, float nx, 
     const float m_eff = 1.0f / (inv_mi + inv_mj);
     const float cn = c.c2z * sqrtf(c.kn * m_eff);
     const float ct = c.c2z * sqrtf(c.kt * m_eff);
-    const float ax = ri * wix + rj * wjx;
-    const float ay = ri * wiy + rj * wjy;
-    const float az = ri * wiz + rj * wjz;
+    const float li = ri - 0.5f * delta;
+    const float lj = rj - 0.5f * delta;
+    const float ax = li * wix + lj * wjx;
+    const float ay = li * wiy + lj * wjy;
+    const float az = li * wiz + lj * wjz;
     const float cx = ay * nz - az * ny;
     const float cy = az * nx - ax * nz;
     const float cz = ax * ny - ay * nx;
@@ -101,9 +105,9 @@ __device__ __forceinline__ PairForce contact_law(const ContactDev& c, float nx, 
     f.fx = fn * nx + ftx;
     f.fy = fn * ny + fty;
     f.fz = fn * nz + ftz;
-    f.tx = -ri * (ny * ftz - nz * fty);
-    f.ty = -ri * (nz * ftx - nx * ftz);
-    f.tz = -ri * (nx * fty - ny * ftx);
+    f.tx = -li * (ny * ftz - nz * fty);
+    f.ty = -li * (nz * ftx - nx * ftz);
+    f.tz = -li * (nx * fty - ny * ftx);
     return f;
 }
 

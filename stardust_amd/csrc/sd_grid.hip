@@ -93,10 +93,11 @@ sd_status check_grid(const sd_grid* g) {
 // ------------------------------------------------------------------ ranges
 
 template <typename K, int SHIFT>
-__global__ void k_cell_ranges(const K* __restrict__ keys, int n, int* __restrict__ ranges) {
+__global__ void k_cell_ranges(const K* __restrict__ keys, int n, int n_cells, int* __restrict__ ranges) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t c = (uint32_t)(keys[i] >> SHIFT);
+    if (c >= (uint32_t)n_cells) return;  // not a cell of this grid
     if (i == 0 || (uint32_t)(keys[i - 1] >> SHIFT) != c) ranges[2 * (size_t)c] = i;
     if (i == n - 1 || (uint32_t)(keys[i + 1] >> SHIFT) != c) ranges[2 * (size_t)c + 1] = i + 1;
 }
@@ -104,7 +105,7 @@ __global__ void k_cell_ranges(const K* __restrict__ keys, int n, int* __restrict
 sd_status launch_cell_ranges_u32(const uint32_t* keys, int n, int n_cells, int* ranges, hipStream_t s) {
     sd_status st = launch_fill_u32((uint32_t*)ranges, 0u, 2 * (size_t)n_cells, s);
     if (st != SD_OK || n <= 0) return st;
-    hipLaunchKernelGGL((k_cell_ranges<uint32_t, 0>), dim3(div_up(n, 256)), dim3(256), 0, s, keys, n, ranges);
+    hipLaunchKernelGGL((k_cell_ranges<uint32_t, 0>), dim3(div_up(n, 256)), dim3(256), 0, s, keys, n, n_cells, ranges);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
@@ -112,7 +113,8 @@ sd_status launch_cell_ranges_u32(const uint32_t* keys, int n, int n_cells, int* 
 sd_status launch_cell_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, hipStream_t s) {
     sd_status st = launch_fill_u32((uint32_t*)ranges, 0u, 2 * (size_t)n_cells, s);
     if (st != SD_OK || n <= 0) return st;
-    hipLaunchKernelGGL((k_cell_ranges<uint64_t, 32>), dim3(div_up(n, 256)), dim3(256), 0, s, keys, n, ranges);
+    hipLaunchKernelGGL((k_cell_ranges<uint64_t, 32>), dim3(div_up(n, 256)), dim3(256), 0, s, keys, n, n_cells,
+                       ranges);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
@@ -126,6 +128,7 @@ __global__ void k_permute(const uint32_t* __restrict__ perm32, const uint64_t* _
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t src = perm32 ? perm32[t] : (uint32_t)perm64[t];
+    if (src >= (uint32_t)n) return;  // not a permutation of 0..n-1: leave the row untouched
     pos_o[t] = pos[src];
     vel_o[t] = vel[src];
     omg_o[t] = omg[src];
