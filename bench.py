@@ -46,8 +46,8 @@ def stage_bytes(w, n, contacts_per_sphere, n_cells):
         "sort": n * (key_b + passes * 2 * (key_b + val_b)),
         # read every sorted key once, clear + write start/end of every cell
         "ranges": n * key_b + 8 * n_cells,
-        # read old order (key/value) + gather pos4, vel4, omg4, id; write the same
-        "permute": n * ((key_b + val_b) + 2 * (3 * 16 + 4)),
+        # read the permutation (key/value), gather pos4, write sorted pos4
+        "permute": n * ((key_b + val_b) + 2 * 16),
         # own state 52 B + old slot + history count in/out 8 B + state out 52 B + next key;
         # per contact: old history row 16 B read + new row 16 B written
         "contact": n * (52 + (key_b + val_b) + 8 + 52 + (key_b + val_b)) + n * c * 32,
@@ -64,6 +64,27 @@ def cpu_baseline(w, s, steps, threads):
     return w.n * steps / dt, dt
 
 
+def calibrate_copy(nbytes=256 << 20, reps=5):
+    """Runs k_copy_u32 over nbytes: each dispatch reads nbytes and writes nbytes."""
+    import ctypes
+    from stardust_amd import _lib as L
+    L.require_gpu()
+    a, b = L.DeviceBuffer(nbytes), L.DeviceBuffer(nbytes)
+    L.call("sd_memset_u32", a.ptr, 1, nbytes // 4, None)
+    for _ in range(reps):
+        L.call("sd_memcpy_dtod", b.ptr, a.ptr, nbytes, None)
+    L.synchronize()
+    ev0, ev1, ms = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_float()
+    L.call("sd_event_create", ctypes.byref(ev0))
+    L.call("sd_event_create", ctypes.byref(ev1))
+    L.call("sd_event_record", ev0, None)
+    L.call("sd_memcpy_dtod", b.ptr, a.ptr, nbytes, None)
+    L.call("sd_event_record", ev1, None)
+    L.call("sd_event_elapsed_ms", ev0, ev1, ctypes.byref(ms))
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "dispatches": reps + 1,
+                      "copy_GBs": 2 * nbytes / (ms.value / 1000.0) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,7 +94,12 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the plain copy kernel on 256 MiB (reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and the path has no multi-GPU mode "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)

@@ -558,7 +558,8 @@ int or_sphere_step(int n, const or_grid* g, const or_contact* c, float* pos4, fl
     int* id2 = (int*)malloc((size_t)n * sizeof(int));
     int* slot = (int*)malloc((size_t)n * sizeof(int));
     int* hc2 = (int*)malloc((size_t)n * sizeof(int));
-    float* h2 = (float*)malloc(4 * (size_t)c->max_contacts * n * sizeof(float));
+    /* zeroed: history rows past a sphere's count stay 0 rather than stale */
+    float* h2 = (float*)calloc(4 * (size_t)c->max_contacts * n, sizeof(float));
     float* f = (float*)malloc(4 * (size_t)n * sizeof(float));
     float* t = (float*)malloc(4 * (size_t)n * sizeof(float));
 

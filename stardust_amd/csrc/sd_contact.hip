@@ -149,14 +149,33 @@ __device__ __forceinline__ void find_history(const ContactArgs& a, int os, int o
     }
 }
 
-template <int MODE>
+constexpr int XCDS = 8;       // gfx950: 8 accelerator complex dies, one L2 each
+constexpr int CAND_UNROLL = 4; // candidate positions loaded per batch (loads in flight)
+
+// Workgroup i is dispatched to XCD i % 8. Remap so that every XCD works on a
+// contiguous range of logical blocks: blocks resident on one XCD then touch
+// neighbouring cells and share that XCD's L2. Returns -1 for padding blocks.
+__device__ __forceinline__ int xcd_logical_block(int nblocks) {
+    const int per = (nblocks + XCDS - 1) / XCDS;
+    const int lb = (int)(blockIdx.x % XCDS) * per + (int)(blockIdx.x / XCDS);
+    return lb < nblocks ? lb : -1;
+}
+
+// MODE: CONTACT_FORCES writes force/torque; CONTACT_FUSED_* integrates and writes
+// the next state and keys. INDIRECT: vel/omg/ids are in the previous step's order
+// and are read through old_slot (positions are always in sorted order).
+template <int MODE, bool INDIRECT>
 __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_logical_block((a.n + (int)blockDim.x - 1) / (int)blockDim.x);
+    if (lb < 0) return;
+    const int s = lb * (int)blockDim.x + (int)threadIdx.x;
     if (s >= a.n) return;
     float4 p = a.pos[s];
-    float4 v = a.vel[s];
-    float4 w = a.omg[s];
     const int os = old_slot_of(a, s);
+    const int self = INDIRECT ? os : s;
+    float4 v = a.vel[self];
+    float4 w = a.omg[self];
+    const int myid = a.ids[self];
     int oc = a.hcount_old[os];
     oc = oc < a.c.maxc ? oc : a.c.maxc;
 
@@ -164,6 +183,36 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     cell_coords(a.g, p.x, p.y, p.z, cx, cy, cz);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
     int nc = 0;
+
+    // one sphere-sphere contact with the sphere at sorted position t
+    auto visit = [&](int t) {
+        const float4 q = a.pos[t];
+        const float dx = p.x - q.x;
+        const float dyy = p.y - q.y;
+        const float dzz = p.z - q.z;
+        const float d2 = dx * dx + dyy * dyy + dzz * dzz;
+        const float rs = p.w + q.w;
+        const int ot = INDIRECT ? old_slot_of(a, t) : t;
+        const float4 vq = a.vel[ot];
+        if (v.w + vq.w == 0.0f) return;  // two static spheres
+        const float4 wq = a.omg[ot];
+        const int pid = a.ids[ot];
+        const float d = sqrtf(d2);
+        const float inv_d = 1.0f / d;
+        const float nx = dx * inv_d, ny = dyy * inv_d, nz = dzz * inv_d;
+        float hx, hy, hz;
+        find_history(a, os, oc, pid, hx, hy, hz);
+        const PairForce f = contact_law(a.c, nx, ny, nz, rs - d, p.w, q.w, v.w, vq.w, v.x - vq.x, v.y - vq.y,
+                                        v.z - vq.z, w.x, w.y, w.z, wq.x, wq.y, wq.z, hx, hy, hz);
+        fx = fx + f.fx;
+        fy = fy + f.fy;
+        fz = fz + f.fz;
+        tx = tx + f.tx;
+        ty = ty + f.ty;
+        tz = tz + f.tz;
+        if (nc < a.c.maxc) a.hist_new[(size_t)nc * a.n + s] = make_float4(hx, hy, hz, __int_as_float(pid));
+        ++nc;
+    };
 
     for (int dz = -1; dz <= 1; ++dz) {
         const int z = cz + dz;
@@ -173,34 +222,28 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
             if (y < 0 || y >= a.g.ny) continue;
             int lo, hi;
             row_range(a.g, a.ranges, cx, y, z, lo, hi);
-            for (int t = lo; t < hi; ++t) {
-                if (t == s) continue;
-                const float4 q = a.pos[t];
-                const float dx = p.x - q.x;
-                const float dyy = p.y - q.y;
-                const float dzz = p.z - q.z;
-                const float d2 = dx * dx + dyy * dyy + dzz * dzz;
-                const float rs = p.w + q.w;
-                if (!(d2 < rs * rs) || d2 == 0.0f) continue;
-                const float4 vq = a.vel[t];
-                if (v.w + vq.w == 0.0f) continue;  // two static spheres
-                const float4 wq = a.omg[t];
-                const int pid = a.ids[t];
-                const float d = sqrtf(d2);
-                const float inv_d = 1.0f / d;
-                const float nx = dx * inv_d, ny = dyy * inv_d, nz = dzz * inv_d;
-                float hx, hy, hz;
-                find_history(a, os, oc, pid, hx, hy, hz);
-                const PairForce f = contact_law(a.c, nx, ny, nz, rs - d, p.w, q.w, v.w, vq.w, v.x - vq.x,
-                                                v.y - vq.y, v.z - vq.z, w.x, w.y, w.z, wq.x, wq.y, wq.z, hx, hy, hz);
-                fx = fx + f.fx;
-                fy = fy + f.fy;
-                fz = fz + f.fz;
-                tx = tx + f.tx;
-                ty = ty + f.ty;
-                tz = tz + f.tz;
-                if (nc < a.c.maxc) a.hist_new[(size_t)nc * a.n + s] = make_float4(hx, hy, hz, __int_as_float(pid));
-                ++nc;
+            for (int t0 = lo; t0 < hi; t0 += CAND_UNROLL) {
+                // issue all loads of the batch first, then test; contacts are
+                // visited in ascending position order
+                float4 q[CAND_UNROLL];
+#pragma unroll
+                for (int k = 0; k < CAND_UNROLL; ++k) q[k] = a.pos[t0 + k < hi ? t0 + k : t0];
+                uint32_t hit = 0u;
+#pragma unroll
+                for (int k = 0; k < CAND_UNROLL; ++k) {
+                    const float dx = p.x - q[k].x;
+                    const float dyy = p.y - q[k].y;
+                    const float dzz = p.z - q[k].z;
+                    const float d2 = dx * dx + dyy * dyy + dzz * dzz;
+                    const float rs = p.w + q[k].w;
+                    const bool h = (t0 + k < hi) && (t0 + k != s) && (d2 < rs * rs) && (d2 != 0.0f);
+                    hit |= h ? (1u << k) : 0u;
+                }
+                while (hit) {
+                    const int k = __builtin_ctz(hit);
+                    hit &= hit - 1u;
+                    visit(t0 + k);
+                }
             }
         }
     }
@@ -244,7 +287,7 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
         a.pos_o[s] = p;
         a.vel_o[s] = v;
         a.omg_o[s] = w;
-        a.ids_o[s] = a.ids[s];
+        a.ids_o[s] = myid;
         if (MODE == CONTACT_FUSED_U32) {
             a.keys_o32[s] = cell_id(a.g, p.x, p.y, p.z);
             a.vals_o32[s] = (uint32_t)s;
@@ -269,13 +312,15 @@ __global__ void k_integrate(float4* __restrict__ pos, float4* __restrict__ vel, 
 sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int block) {
     if (a.n <= 0) return SD_OK;
     if (block <= 0) block = 128;
-    const dim3 grid(div_up(a.n, block));
+    SD_REQUIRE(block % WAVE == 0 && block <= 256, "contact kernel: block must be a multiple of 64, <= 256");
+    const int nblocks = div_up(a.n, block);
+    const dim3 grid(div_up(nblocks, XCDS) * XCDS);  // padded for the XCD remap
     if (mode == CONTACT_FORCES)
-        hipLaunchKernelGGL(k_contacts<CONTACT_FORCES>, grid, dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_contacts<CONTACT_FORCES, false>), grid, dim3(block), 0, s, a);
     else if (mode == CONTACT_FUSED_U32)
-        hipLaunchKernelGGL(k_contacts<CONTACT_FUSED_U32>, grid, dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U32, true>), grid, dim3(block), 0, s, a);
     else
-        hipLaunchKernelGGL(k_contacts<CONTACT_FUSED_U64>, grid, dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U64, true>), grid, dim3(block), 0, s, a);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

@@ -7,9 +7,11 @@
 // One step, all on the context stream, no host synchronisation:
 //   sort(keys)            keys were written by the previous step (or upload)
 //   cell ranges           clear (fill kernel) + mark runs
-//   permute A -> B        state gathered into the new sorted order
-//   fused contact kernel  reads B + old history, writes integrated state to A,
-//                         new history, and the next step's keys
+//   permute positions     sorted positions gathered from the current state
+//   fused contact kernel  reads sorted positions, the current state (through the
+//                         sort permutation) and the old history; writes the
+//                         integrated state into the other state buffer in the
+//                         new order, the new history and the next step's keys
 // Buffer roles repeat with period two, so two steps are captured once into a
 // hipGraph per starting parity and replayed.
 #include <math.h>
@@ -60,8 +62,12 @@ __global__ __launch_bounds__(DIAG_BLOCK) void k_diag_partial(const float4* __res
 }
 
 struct GraphKey {
-    int keys, hist;
-    bool operator<(const GraphKey& o) const { return keys != o.keys ? keys < o.keys : hist < o.hist; }
+    int keys, hist, state;
+    bool operator<(const GraphKey& o) const {
+        if (keys != o.keys) return keys < o.keys;
+        if (hist != o.hist) return hist < o.hist;
+        return state < o.state;
+    }
 };
 
 }  // namespace sd
@@ -76,10 +82,15 @@ struct sd_ctx {
     int key_bits = 0;
     int items = 0;
     int contact_block = 128;
-    // state: A = current order (keys refer to it), B = sorted scratch
-    float4 *posA = nullptr, *velA = nullptr, *omgA = nullptr;
-    float4 *posB = nullptr, *velB = nullptr, *omgB = nullptr;
-    int *idsA = nullptr, *idsB = nullptr;
+    // state in the current order (the keys refer to it), double-buffered:
+    // [cur_state] is read, [1 - cur_state] receives the integrated state;
+    // posB holds the positions gathered into sorted order
+    float4* pos[2] = {nullptr, nullptr};
+    float4* vel[2] = {nullptr, nullptr};
+    float4* omg[2] = {nullptr, nullptr};
+    int* ids[2] = {nullptr, nullptr};
+    int cur_state = 0;
+    float4* posB = nullptr;
     uint32_t* k32[2] = {nullptr, nullptr};
     uint32_t* v32[2] = {nullptr, nullptr};
     uint64_t* k64[2] = {nullptr, nullptr};
@@ -105,13 +116,16 @@ void release(sd_ctx* c) {
     for (auto g : c->graph_objs) (void)hipGraphDestroy(g);
     c->graphs.clear();
     c->graph_objs.clear();
-    void* bufs[] = {c->posA, c->velA, c->omgA, c->posB, c->velB, c->omgB, c->idsA, c->idsB, c->k32[0], c->k32[1],
-                    c->v32[0], c->v32[1], c->k64[0], c->k64[1], c->sort_tmp, c->ranges, c->hcount[0], c->hcount[1],
-                    c->hist[0], c->hist[1], c->overflow, c->diag};
+    void* bufs[] = {c->pos[0], c->pos[1], c->vel[0], c->vel[1], c->omg[0], c->omg[1], c->ids[0], c->ids[1],
+                    c->posB, c->k32[0], c->k32[1], c->v32[0], c->v32[1], c->k64[0], c->k64[1], c->sort_tmp,
+                    c->ranges, c->hcount[0], c->hcount[1], c->hist[0], c->hist[1], c->overflow, c->diag};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    c->posA = c->velA = c->omgA = c->posB = c->velB = c->omgB = nullptr;
-    c->idsA = c->idsB = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        c->pos[k] = c->vel[k] = c->omg[k] = nullptr;
+        c->ids[k] = nullptr;
+    }
+    c->posB = nullptr;
     c->k32[0] = c->k32[1] = c->v32[0] = c->v32[1] = nullptr;
     c->k64[0] = c->k64[1] = nullptr;
     c->sort_tmp = nullptr;
@@ -137,11 +151,12 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
     if (ev) SD_HIP(hipEventRecord(ev[i], s))
     const int n = c->n;
     const int ho = c->cur_hist, hn = 1 - c->cur_hist;
+    const int so = c->cur_state, sn = 1 - c->cur_state;
     ContactArgs a = {};
     a.pos = c->posB;
-    a.vel = c->velB;
-    a.omg = c->omgB;
-    a.ids = c->idsB;
+    a.vel = c->vel[so];
+    a.omg = c->omg[so];
+    a.ids = c->ids[so];
     a.n = n;
     a.g = to_dev(c->p.grid);
     a.ranges = c->ranges;
@@ -151,10 +166,10 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
     a.hcount_new = c->hcount[hn];
     a.hist_new = c->hist[hn];
     a.overflow = c->overflow;
-    a.pos_o = c->posA;
-    a.vel_o = c->velA;
-    a.omg_o = c->omgA;
-    a.ids_o = c->idsA;
+    a.pos_o = c->pos[sn];
+    a.vel_o = c->vel[sn];
+    a.omg_o = c->omg[sn];
+    a.ids_o = c->ids[sn];
     sd_status st;
     MARK(0);
     if (c->p.key64) {
@@ -166,7 +181,7 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
         st = launch_cell_ranges_u64(K, n, c->n_cells, c->ranges, s);
         if (st != SD_OK) return st;
         MARK(2);
-        st = launch_permute(nullptr, K, n, c->posA, c->velA, c->omgA, c->idsA, c->posB, c->velB, c->omgB, c->idsB, s);
+        st = launch_permute_pos(nullptr, K, n, c->pos[so], c->posB, s);
         if (st != SD_OK) return st;
         MARK(3);
         a.old_slot_u64 = K;
@@ -186,7 +201,7 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
         st = launch_cell_ranges_u32(K, n, c->n_cells, c->ranges, s);
         if (st != SD_OK) return st;
         MARK(2);
-        st = launch_permute(V, nullptr, n, c->posA, c->velA, c->omgA, c->idsA, c->posB, c->velB, c->omgB, c->idsB, s);
+        st = launch_permute_pos(V, nullptr, n, c->pos[so], c->posB, s);
         if (st != SD_OK) return st;
         MARK(3);
         a.old_slot_u32 = V;
@@ -199,14 +214,15 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
     }
 #undef MARK
     c->cur_hist = hn;
+    c->cur_state = sn;
     return SD_OK;
 }
 
 sd_status run_graph_pair(sd_ctx* c) {
-    const GraphKey key{c->cur_keys, c->cur_hist};
+    const GraphKey key{c->cur_keys, c->cur_hist, c->cur_state};
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
-        const int k0 = c->cur_keys, h0 = c->cur_hist;
+        const int k0 = c->cur_keys, h0 = c->cur_hist, s0 = c->cur_state;
         SD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         sd_status st = enqueue_step(c);
         if (st == SD_OK) st = enqueue_step(c);
@@ -228,6 +244,7 @@ sd_status run_graph_pair(sd_ctx* c) {
         // capture advanced the host-side roles; rewind them for the replay
         c->cur_keys = k0;
         c->cur_hist = h0;
+        c->cur_state = s0;
     }
     SD_HIP(hipGraphLaunch(it->second, c->stream));
     // two steps advance the roles by an even amount: keys and history return to
@@ -306,14 +323,13 @@ extern "C" sd_status sd_ctx_upload_spheres(sd_ctx* c, int n, const float* x, con
         release(c);                       \
         return st;                        \
     }
-    ALLOC(c->posA, n);
-    ALLOC(c->velA, n);
-    ALLOC(c->omgA, n);
+    for (int k = 0; k < 2; ++k) {
+        ALLOC(c->pos[k], n);
+        ALLOC(c->vel[k], n);
+        ALLOC(c->omg[k], n);
+        ALLOC(c->ids[k], n);
+    }
     ALLOC(c->posB, n);
-    ALLOC(c->velB, n);
-    ALLOC(c->omgB, n);
-    ALLOC(c->idsA, n);
-    ALLOC(c->idsB, n);
     if (c->p.key64) {
         ALLOC(c->k64[0], n);
         ALLOC(c->k64[1], n);
@@ -348,19 +364,20 @@ extern "C" sd_status sd_ctx_upload_spheres(sd_ctx* c, int n, const float* x, con
         omg[i] = make_float4(0.0f, 0.0f, 0.0f, inv_I);
         ids[i] = i;
     }
-    SD_HIP(hipMemcpyAsync(c->posA, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
-    SD_HIP(hipMemcpyAsync(c->velA, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
-    SD_HIP(hipMemcpyAsync(c->omgA, omg.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
-    SD_HIP(hipMemcpyAsync(c->idsA, ids.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    SD_HIP(hipMemcpyAsync(c->pos[0], pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    SD_HIP(hipMemcpyAsync(c->vel[0], vel.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    SD_HIP(hipMemcpyAsync(c->omg[0], omg.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    SD_HIP(hipMemcpyAsync(c->ids[0], ids.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
     st = launch_fill_u32((uint32_t*)c->hcount[0], 0u, n, c->stream);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->hcount[1], 0u, n, c->stream);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->overflow, 0u, 1, c->stream);
     if (st != SD_OK) return st;
     c->cur_keys = 0;
     c->cur_hist = 0;
+    c->cur_state = 0;
     const GridDev g = to_dev(c->p.grid);
-    st = c->p.key64 ? launch_hash_keys_u64(c->posA, n, g, c->k64[0], c->stream)
-                    : launch_hash_keys_u32(c->posA, n, g, c->k32[0], c->v32[0], c->stream);
+    st = c->p.key64 ? launch_hash_keys_u64(c->pos[0], n, g, c->k64[0], c->stream)
+                    : launch_hash_keys_u32(c->pos[0], n, g, c->k32[0], c->v32[0], c->stream);
     if (st != SD_OK) return st;
     SD_HIP(hipStreamSynchronize(c->stream));
     c->steps_done = 0;
@@ -434,10 +451,11 @@ extern "C" sd_status sd_ctx_download_spheres(sd_ctx* c, float* x, float* y, floa
     const int n = c->n;
     std::vector<float4> pos(n), vel(n), omg(n);
     std::vector<int> ids(n);
-    SD_HIP(hipMemcpyAsync(pos.data(), c->posA, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
-    SD_HIP(hipMemcpyAsync(vel.data(), c->velA, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
-    SD_HIP(hipMemcpyAsync(omg.data(), c->omgA, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
-    SD_HIP(hipMemcpyAsync(ids.data(), c->idsA, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    const int cs = c->cur_state;
+    SD_HIP(hipMemcpyAsync(pos.data(), c->pos[cs], n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipMemcpyAsync(vel.data(), c->vel[cs], n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipMemcpyAsync(omg.data(), c->omg[cs], n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipMemcpyAsync(ids.data(), c->ids[cs], n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SD_HIP(hipStreamSynchronize(c->stream));
     for (int s = 0; s < n; ++s) {
         const int i = ids[s];
@@ -457,8 +475,9 @@ extern "C" sd_status sd_ctx_download_spheres(sd_ctx* c, float* x, float* y, floa
 
 static sd_status diag_values(sd_ctx* c, double out[DIAG_VALUES]) {
     const ContactDev cd = to_dev(c->p.contact);
-    hipLaunchKernelGGL(k_diag_partial, dim3(c->diag_blocks), dim3(DIAG_BLOCK), 0, c->stream, c->posA, c->velA,
-                       c->omgA, c->hcount[c->cur_hist], c->n, cd.gx, cd.gy, cd.gz, c->diag);
+    const int cs = c->cur_state;
+    hipLaunchKernelGGL(k_diag_partial, dim3(c->diag_blocks), dim3(DIAG_BLOCK), 0, c->stream, c->pos[cs], c->vel[cs],
+                       c->omg[cs], c->hcount[c->cur_hist], c->n, cd.gx, cd.gy, cd.gz, c->diag);
     SD_LAUNCH_CHECK();
     std::vector<double> part((size_t)c->diag_blocks * DIAG_VALUES);
     SD_HIP(hipMemcpyAsync(part.data(), c->diag, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -533,10 +552,11 @@ extern "C" sd_status sd_ctx_save(sd_ctx* c, void* buffer, size_t bytes) {
         return SD_OK;
     };
     sd_status st = SD_OK;
-    if (st == SD_OK) st = d2h(c->posA, n * sizeof(float4));
-    if (st == SD_OK) st = d2h(c->velA, n * sizeof(float4));
-    if (st == SD_OK) st = d2h(c->omgA, n * sizeof(float4));
-    if (st == SD_OK) st = d2h(c->idsA, n * sizeof(int));
+    const int cs = c->cur_state;
+    if (st == SD_OK) st = d2h(c->pos[cs], n * sizeof(float4));
+    if (st == SD_OK) st = d2h(c->vel[cs], n * sizeof(float4));
+    if (st == SD_OK) st = d2h(c->omg[cs], n * sizeof(float4));
+    if (st == SD_OK) st = d2h(c->ids[cs], n * sizeof(int));
     if (c->p.key64) {
         if (st == SD_OK) st = d2h(c->k64[c->cur_keys], n * sizeof(uint64_t));
     } else {
@@ -564,16 +584,17 @@ extern "C" sd_status sd_ctx_load(sd_ctx* c, const void* buffer, size_t bytes) {
     const char* b = (const char*)buffer + sizeof h;
     c->cur_keys = 0;
     c->cur_hist = 0;
+    c->cur_state = 0;
     auto h2d = [&](void* dst, size_t sz) -> sd_status {
         SD_HIP(hipMemcpyAsync(dst, b, sz, hipMemcpyHostToDevice, c->stream));
         b += sz;
         return SD_OK;
     };
     sd_status st = SD_OK;
-    if (st == SD_OK) st = h2d(c->posA, n * sizeof(float4));
-    if (st == SD_OK) st = h2d(c->velA, n * sizeof(float4));
-    if (st == SD_OK) st = h2d(c->omgA, n * sizeof(float4));
-    if (st == SD_OK) st = h2d(c->idsA, n * sizeof(int));
+    if (st == SD_OK) st = h2d(c->pos[0], n * sizeof(float4));
+    if (st == SD_OK) st = h2d(c->vel[0], n * sizeof(float4));
+    if (st == SD_OK) st = h2d(c->omg[0], n * sizeof(float4));
+    if (st == SD_OK) st = h2d(c->ids[0], n * sizeof(int));
     if (c->p.key64) {
         if (st == SD_OK) st = h2d(c->k64[0], n * sizeof(uint64_t));
     } else {

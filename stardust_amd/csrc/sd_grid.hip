@@ -17,6 +17,21 @@ __global__ void k_fill_u32(uint32_t* __restrict__ dst, uint32_t value, size_t co
         dst[i] = value;
 }
 
+__global__ void k_copy_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+sd_status launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t count, hipStream_t s) {
+    if (count == 0) return SD_OK;
+    const int block = 256;
+    long blocks = (long)((count + block - 1) / block);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_copy_u32, dim3((int)blocks), dim3(block), 0, s, dst, src, count);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
 sd_status launch_fill_u32(uint32_t* dst, uint32_t value, size_t count, hipStream_t s) {
     if (count == 0) return SD_OK;
     const int block = 256;
@@ -141,6 +156,23 @@ sd_status launch_permute(const uint32_t* perm32, const uint64_t* perm64, int n, 
     if (n <= 0) return SD_OK;
     hipLaunchKernelGGL(k_permute, dim3(div_up(n, 256)), dim3(256), 0, s, perm32, perm64, n, pos, vel, omg, ids, pos_o,
                        vel_o, omg_o, ids_o);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+__global__ void k_permute_pos(const uint32_t* __restrict__ perm32, const uint64_t* __restrict__ perm64, int n,
+                              const float4* __restrict__ pos, float4* __restrict__ pos_o) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t src = perm32 ? perm32[t] : (uint32_t)perm64[t];
+    if (src >= (uint32_t)n) return;
+    pos_o[t] = pos[src];
+}
+
+sd_status launch_permute_pos(const uint32_t* perm32, const uint64_t* perm64, int n, const float4* pos, float4* pos_o,
+                             hipStream_t s) {
+    if (n <= 0) return SD_OK;
+    hipLaunchKernelGGL(k_permute_pos, dim3(div_up(n, 256)), dim3(256), 0, s, perm32, perm64, n, pos, pos_o);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

@@ -42,6 +42,7 @@ int grid_bits(const sd_grid& g);  // bits needed for the largest cell id
 sd_status check_grid(const sd_grid* g);
 
 sd_status launch_fill_u32(uint32_t* dst, uint32_t value, size_t count, hipStream_t s);
+sd_status launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t count, hipStream_t s);
 sd_status launch_hash_keys_u32(const float4* pos4, int n, const GridDev& g, uint32_t* keys, uint32_t* vals,
                                hipStream_t s);
 sd_status launch_hash_keys_u64(const float4* pos4, int n, const GridDev& g, uint64_t* keys, hipStream_t s);
@@ -51,6 +52,9 @@ sd_status launch_cell_ranges_u64(const uint64_t* keys, int n, int n_cells, int* 
 sd_status launch_permute(const uint32_t* perm32, const uint64_t* perm64, int n, const float4* pos, const float4* vel,
                          const float4* omg, const int* ids, float4* pos_o, float4* vel_o, float4* omg_o, int* ids_o,
                          hipStream_t s);
+
+sd_status launch_permute_pos(const uint32_t* perm32, const uint64_t* perm64, int n, const float4* pos, float4* pos_o,
+                             hipStream_t s);
 
 // ------------------------------------------------------------------ contacts
 struct ContactDev {

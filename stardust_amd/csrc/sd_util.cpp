@@ -73,6 +73,9 @@ extern "C" sd_status sd_memcpy_dtoh(void* dst, const void* src, size_t bytes, vo
 extern "C" sd_status sd_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
     if (bytes == 0) return SD_OK;
     SD_REQUIRE(dst && src, "sd_memcpy_dtod: NULL pointer");
+    // a plain copy kernel (graph-safe; also the known-size reference for HBM counters)
+    if (bytes % 4 == 0 && ((uintptr_t)dst % 4) == 0 && ((uintptr_t)src % 4) == 0)
+        return sd::launch_copy_u32((uint32_t*)dst, (const uint32_t*)src, bytes / 4, (hipStream_t)stream);
     SD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
     return SD_OK;
 }
