@@ -212,6 +212,53 @@ sd_status sd_gen_spheres(uint64_t seed, int n, const float lo[3], const float hi
                          float r_median, float r_sigma, float r_min, float r_max,
                          float* x, float* y, float* z, float* radius);
 
+/* Convex hulls: points on ellipsoids (all of them hull vertices). Arrays:
+ * pos4/vel4 (x, y, z, 0), quat4 (w, x, y, z) per hull, vstart/nv per hull,
+ * verts4 (x, y, z, 0) local-frame vertices, capacity verts_capacity. */
+sd_status sd_gen_hulls(uint64_t seed, int n, const float lo[3], const float hi[3], int nv_min, int nv_max,
+                       float ax_min, float ax_max, float v_sigma, float* pos4, float* quat4, float* vel4,
+                       int* vstart, int* nv, float* verts4, int verts_capacity, int* n_verts);
+
+/* ------------------------------------------------------- convex hulls
+ * Hull set (device arrays): pos4 = (x, y, z, r) centre (r: radius of a sphere
+ * shape, nv == 0), quat4 = unit quaternion (w, x, y, z), vstart/nv = vertex
+ * range of each hull in verts4 (local-frame (x, y, z, 0)).
+ *
+ * World AABB of every hull: lo4/hi4 = (x, y, z, 0).
+ * Replaces computeAABBCUDA (sap_detection.cu:178-217) for rotated vertex sets. */
+sd_status sd_hull_aabbs(const float* pos4, const float* quat4, const int* vstart, const int* nv, const float* verts4,
+                        int n, float* lo4, float* hi4, void* stream);
+
+/* Sweep-and-prune over AABBs: every unordered overlapping pair (closed
+ * intervals on all three axes), written as (min id, max id), grouped by the
+ * sweep position of the first box. The sweep axis is the one with the largest
+ * variance of box centres (written to *axis_out if not NULL). If the pair count
+ * exceeds capacity, *n_pairs receives the required count and SD_ERR_CAPACITY is
+ * returned. Replaces SAPCollision::processCollisions (sap_detection.cu:668-706),
+ * initIdx / projectAABB / sortLowestExtents / sweepAndPruneCUDA (:135-488). */
+sd_status sd_sap_scratch_bytes(int n, size_t* bytes);
+sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int* pairs, int capacity, int* n_pairs,
+                       int* axis_out, void* scratch, size_t scratch_bytes, void* stream);
+
+/* GJK distance / EPA penetration for each pair (a, b) of hull ids. Outputs per
+ * pair: sep (> 0 separation distance, < 0 penetration depth), normal3 (unit,
+ * from a to b), flags (SD_GJK_* bits; -1 for ids or vertex ranges outside the
+ * set), iters (support iterations spent, GJK + EPA, warm + cold).
+ * warm_dir (nullable, 3 floats per pair): initial direction of a warm-started
+ * query limited to warm_budget iterations; a warm query that does not converge
+ * within its budget, or ends within near_margin of contact, is redone from the
+ * cold start (centre difference) with max_iter iterations.
+ * Replaces gjkCUDA (sandbox/testGJK.cu:237-315). */
+#define SD_GJK_HIT        1
+#define SD_GJK_CONVERGED  2
+#define SD_GJK_BUDGET     4
+#define SD_GJK_EPA_LIMIT  8
+#define SD_GJK_DEGENERATE 16
+#define SD_GJK_WARM_USED  32
+sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, const float* quat4, const int* vstart,
+                 const int* nv, const float* verts4, int n_hulls, int n_verts, const float* warm_dir, int warm_budget,
+                 int max_iter, float near_margin, float* sep, float* normal3, int* flags, int* iters, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,7 +40,58 @@ _SIGS = {
                                _P, _P], _I),
     "or_integrate": ([_I, _P, _P, _P, _P, _P, C.POINTER(OrContact)], None),
     "or_sphere_step": ([_I, C.POINTER(OrGrid), C.POINTER(OrContact), _P, _P, _P, _P, _P, _P, _I], _I),
+    "or_hull_aabbs": ([_I, _P, _P, _P, _P, _P, _P, _P], None),
+    "or_sap_axis": ([_I, _P, _P], _I),
+    "or_sap_pairs": ([_I, _P, _P, _I, _P, C.c_long], C.c_long),
+    "or_brute_aabb_pairs": ([_I, _P, _P, _P, C.c_long], C.c_long),
+    "or_gjk": ([_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, C.c_float, _P, _P, _P, _P, _I], None),
 }
+
+
+def hull_aabbs(h):
+    n = len(h["pos4"])
+    lo = np.zeros((n, 4), np.float32)
+    hi = np.zeros((n, 4), np.float32)
+    a = [np.ascontiguousarray(h[k]) for k in ("pos4", "quat4", "vstart", "nv", "verts4")]
+    load().or_hull_aabbs(n, *[p(x) for x in a], p(lo), p(hi))
+    return lo, hi
+
+
+def sap_axis(lo, hi):
+    lo, hi = np.ascontiguousarray(lo, np.float32), np.ascontiguousarray(hi, np.float32)
+    return load().or_sap_axis(len(lo), p(lo), p(hi))
+
+
+def sap_pairs(lo, hi, axis=None):
+    lo, hi = np.ascontiguousarray(lo, np.float32), np.ascontiguousarray(hi, np.float32)
+    if axis is None:
+        axis = sap_axis(lo, hi)
+    cnt = load().or_sap_pairs(len(lo), p(lo), p(hi), axis, None, 0)
+    out = np.empty((max(cnt, 1), 2), np.int32)
+    load().or_sap_pairs(len(lo), p(lo), p(hi), axis, p(out), cnt)
+    return out[:cnt], axis
+
+
+def brute_aabb_pairs(lo, hi):
+    lo, hi = np.ascontiguousarray(lo, np.float32), np.ascontiguousarray(hi, np.float32)
+    cnt = load().or_brute_aabb_pairs(len(lo), p(lo), p(hi), None, 0)
+    out = np.empty((max(cnt, 1), 2), np.int32)
+    load().or_brute_aabb_pairs(len(lo), p(lo), p(hi), p(out), cnt)
+    return out[:cnt]
+
+
+def gjk(h, pairs, warm=None, warm_budget=16, max_iter=64, near_margin=1e-6, nthreads=1):
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    n = len(pairs)
+    sep = np.zeros(n, np.float32)
+    nrm = np.zeros((n, 3), np.float32)
+    flags = np.zeros(n, np.int32)
+    iters = np.zeros(n, np.int32)
+    a = [np.ascontiguousarray(h[k]) for k in ("pos4", "quat4", "vstart", "nv", "verts4")]
+    w = None if warm is None else np.ascontiguousarray(warm, np.float32)
+    load().or_gjk(n, p(pairs), *[p(x) for x in a], None if w is None else p(w), warm_budget, max_iter, near_margin,
+                  p(sep), p(nrm), p(flags), p(iters), nthreads)
+    return {"sep": sep, "normal": nrm, "flags": flags, "iters": iters}
 
 _lib = None
 

@@ -88,6 +88,21 @@ void or_integrate(int n, float* pos4, float* vel4, float* omg4, const float* for
 int or_sphere_step(int n, const or_grid* g, const or_contact* c, float* pos4, float* vel4, float* omg4, int* ids,
                    int* hcount, float* hist4, int nthreads);
 
+/* ------------------------------------------------------------------ hulls
+ * (sd_oracle_hull.c) hull set: pos4 (x, y, z, sphere radius), quat4 (w, x, y, z),
+ * vstart/nv vertex ranges into verts4 (local (x, y, z, 0)). */
+void or_hull_aabbs(int n, const float* pos4, const float* quat4, const int* vstart, const int* nv,
+                   const float* verts4, float* lo4, float* hi4);
+/* Axis of largest box-centre variance (sap_detection.cu:589 TODO). */
+int or_sap_axis(int n, const float* lo4, const float* hi4);
+/* Serial sweep-and-prune (sap_detection.cu:668-706 pipeline, exact pair set). */
+long or_sap_pairs(int n, const float* lo4, const float* hi4, int axis, int* pairs, long capacity);
+long or_brute_aabb_pairs(int n, const float* lo4, const float* hi4, int* pairs, long capacity);
+/* GJK distance + EPA depth per pair (testGJK.cu:52-315 restated, terminating). */
+void or_gjk(int npairs, const int* pairs, const float* pos4, const float* quat4, const int* vstart, const int* nv,
+            const float* verts4, const float* warm, int warm_budget, int max_iter, float near_margin, float* sep,
+            float* normal3, int* flags, int* iters, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -120,7 +120,13 @@ _SIGS = {
     "sd_ctx_save": [_P, _P, _SZ],
     "sd_ctx_load": [_P, _P, _SZ],
     "sd_gen_spheres": [C.c_uint64, _I, _P, _P, _F, _F, _F, _F, _P, _P, _P, _P],
+    "sd_gen_hulls": [C.c_uint64, _I, _P, _P, _I, _I, _F, _F, _F, _P, _P, _P, _P, _P, _P, _I, C.POINTER(C.c_int)],
+    "sd_hull_aabbs": [_P, _P, _P, _P, _P, _I, _P, _P, _P],
+    "sd_sap_scratch_bytes": [_I, C.POINTER(_SZ)],
+    "sd_sap_pairs": [_P, _P, _I, _P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), _P, _SZ, _P],
+    "sd_gjk": [_P, _I, _P, _P, _P, _P, _P, _I, _I, _P, _I, _I, _F, _P, _P, _P, _P, _P],
 }
+GJK_HIT, GJK_CONVERGED, GJK_BUDGET, GJK_EPA_LIMIT, GJK_DEGENERATE, GJK_WARM_USED = 1, 2, 4, 8, 16, 32
 _RESTYPE = {"sd_last_error": C.c_char_p, "sd_version": C.c_char_p}
 
 _lib = None

@@ -1,0 +1,540 @@
+// Device-side convex-hull geometry: quaternion rotation, support mapping,
+// GJK distance with Voronoi-region simplex reduction, and EPA penetration depth.
+//
+// Replaces the boolean GJK of the reference sandbox (sandbox/testGJK.cu:52-315:
+// supportPointSoupNaive, computeMinkowskiDifference, expandSimplex3/4, gjkCUDA),
+// which never terminates on a hit and reports no distance (quirk Q16).
+//
+// Conventions: M = A - B is evaluated in A's frame (A's centre at the origin,
+// B's centre at d = c_B - c_A). support_M(u) = s_A(u) - s_B(-u). The result
+// normal n points from A to B: for separated shapes n = -v/|v| (v the closest
+// point of M), for intersecting shapes n is the normal of EPA's closest face.
+// sep > 0 is the separation distance, sep < 0 the penetration depth.
+// Every floating-point expression is written out with an explicit order
+// (no contraction) so a CPU restatement can reproduce the bits.
+#pragma once
+
+#include "sd_internal.hpp"
+
+namespace sd {
+
+constexpr int GJK_MAX_VERTS_PER_HULL = 1024;  // key = ia * 1024 + ib
+constexpr float GJK_REL_EPS = 1e-6f;          // v.v - v.w <= eps * v.v  -> converged
+constexpr float GJK_TOUCH_EPS2 = 1e-14f;      // |v|^2 below this: origin on the boundary
+constexpr float GJK_STALL_EPS = 1e-4f;        // gap accepted when the distance stops decreasing
+constexpr int EPA_MAX_V = 48;
+constexpr int EPA_MAX_F = 96;
+constexpr int EPA_MAX_E = 48;
+
+enum {
+    GJK_HIT = 1,          // shapes intersect
+    GJK_CONVERGED = 2,    // result met the tolerance
+    GJK_BUDGET = 4,       // iteration budget exhausted
+    GJK_EPA_LIMIT = 8,    // EPA ran out of vertex/face capacity (best face returned)
+    GJK_DEGENERATE = 16,  // EPA could not build a tetrahedron (depth 0 reported)
+    GJK_WARM_USED = 32,   // result comes from the warm-started query
+};
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ V3 scl(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Row-major rotation of a unit quaternion stored as (w, x, y, z) in (.x, .y, .z, .w).
+struct Rot {
+    float m[9];
+};
+__device__ __forceinline__ Rot quat_rot(float4 q) {
+    const float w = q.x, x = q.y, y = q.z, z = q.w;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, xz = x * z, yz = y * z;
+    const float wx = w * x, wy = w * y, wz = w * z;
+    Rot r;
+    r.m[0] = 1.0f - 2.0f * (yy + zz);
+    r.m[1] = 2.0f * (xy - wz);
+    r.m[2] = 2.0f * (xz + wy);
+    r.m[3] = 2.0f * (xy + wz);
+    r.m[4] = 1.0f - 2.0f * (xx + zz);
+    r.m[5] = 2.0f * (yz - wx);
+    r.m[6] = 2.0f * (xz - wy);
+    r.m[7] = 2.0f * (yz + wx);
+    r.m[8] = 1.0f - 2.0f * (xx + yy);
+    return r;
+}
+__device__ __forceinline__ V3 rot_mul(const Rot& r, V3 v) {
+    return v3(r.m[0] * v.x + r.m[1] * v.y + r.m[2] * v.z, r.m[3] * v.x + r.m[4] * v.y + r.m[5] * v.z,
+              r.m[6] * v.x + r.m[7] * v.y + r.m[8] * v.z);
+}
+__device__ __forceinline__ V3 rot_tmul(const Rot& r, V3 v) {
+    return v3(r.m[0] * v.x + r.m[3] * v.y + r.m[6] * v.z, r.m[1] * v.x + r.m[4] * v.y + r.m[7] * v.z,
+              r.m[2] * v.x + r.m[5] * v.y + r.m[8] * v.z);
+}
+
+// One convex shape placed in the query frame: a vertex set (nv > 0) or a sphere
+// (nv == 0, radius > 0).
+struct Shape {
+    const float4* verts;  // local-frame vertices
+    int nv;
+    float radius;
+    Rot R;
+    V3 off;  // centre in the query frame
+};
+
+// Index of the vertex with the largest projection on u (first one on ties).
+__device__ __forceinline__ int support_index(const Shape& s, V3 u) {
+    const V3 l = rot_tmul(s.R, u);
+    int best = 0;
+    float bs = -3.402823466e38f;
+    for (int k = 0; k < s.nv; ++k) {
+        const float4 p = s.verts[k];
+        const float d = p.x * l.x + p.y * l.y + p.z * l.z;
+        if (d > bs) {
+            bs = d;
+            best = k;
+        }
+    }
+    return best;
+}
+
+__device__ __forceinline__ V3 support_point(const Shape& s, V3 u, int& idx) {
+    if (s.nv == 0) {
+        idx = -1;
+        const float uu = dot(u, u);
+        if (uu == 0.0f) return s.off;
+        const float k = s.radius / sqrtf(uu);
+        return add(s.off, scl(u, k));
+    }
+    idx = support_index(s, u);
+    const float4 p = s.verts[idx];
+    return add(rot_mul(s.R, v3(p.x, p.y, p.z)), s.off);
+}
+
+struct MPoint {
+    V3 p;
+    int key;  // ia * 1024 + ib, or -1 when a sphere contributed
+};
+
+__device__ __forceinline__ MPoint support_M(const Shape& A, const Shape& B, V3 u) {
+    int ia, ib;
+    const V3 pa = support_point(A, u, ia);
+    const V3 pb = support_point(B, neg(u), ib);
+    MPoint m;
+    m.p = sub(pa, pb);
+    m.key = (ia < 0 || ib < 0) ? -1 : ia * GJK_MAX_VERTS_PER_HULL + ib;
+    return m;
+}
+
+// ---------------------------------------------------------------- simplex
+
+struct Simplex {
+    MPoint w[4];
+    int n;
+};
+
+// Closest point of segment (a, b) to the origin; keeps the feature's vertices.
+__device__ __forceinline__ V3 closest_segment(Simplex& s) {
+    const V3 a = s.w[0].p, b = s.w[1].p;
+    const V3 ab = sub(b, a);
+    const float t = -dot(a, ab);
+    if (t <= 0.0f) {
+        s.n = 1;
+        return a;
+    }
+    const float den = dot(ab, ab);
+    if (t >= den) {
+        s.w[0] = s.w[1];
+        s.n = 1;
+        return b;
+    }
+    return add(a, scl(ab, t / den));
+}
+
+// Closest point of triangle (a, b, c) to the origin (Voronoi-region tests).
+__device__ __forceinline__ V3 closest_triangle(Simplex& s) {
+    const MPoint A = s.w[0], B = s.w[1], Cc = s.w[2];
+    const V3 a = A.p, b = B.p, c = Cc.p;
+    const V3 ab = sub(b, a), ac = sub(c, a);
+    const V3 ap = neg(a);
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        s.n = 1;
+        return a;
+    }
+    const V3 bp = neg(b);
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) {
+        s.w[0] = B;
+        s.n = 1;
+        return b;
+    }
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        const float t = d1 / (d1 - d3);
+        s.n = 2;
+        return add(a, scl(ab, t));
+    }
+    const V3 cp = neg(c);
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) {
+        s.w[0] = Cc;
+        s.n = 1;
+        return c;
+    }
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        const float t = d2 / (d2 - d6);
+        s.w[1] = Cc;
+        s.n = 2;
+        return add(a, scl(ac, t));
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        s.w[0] = B;
+        s.w[1] = Cc;
+        s.n = 2;
+        return add(b, scl(sub(c, b), t));
+    }
+    const float den = 1.0f / (va + vb + vc);
+    const float v = vb * den, w = vc * den;
+    return add(add(a, scl(ab, v)), scl(ac, w));
+}
+
+// True if the origin and d lie strictly on opposite sides of plane (a, b, c).
+__device__ __forceinline__ bool origin_outside(V3 a, V3 b, V3 c, V3 d) {
+    const V3 nrm = cross(sub(b, a), sub(c, a));
+    const float so = -dot(a, nrm);
+    const float sd = dot(sub(d, a), nrm);
+    return so * sd < 0.0f;
+}
+
+// Closest point of tetrahedron w[0..3] to the origin; *inside when it contains it.
+// A near-flat tetrahedron (|6 V| <= 1e-5 |ab| |ac| |ad|) has unreliable side
+// tests; it is never reported as containing the origin, and all four faces are
+// searched instead.
+__device__ __forceinline__ V3 closest_tetra(Simplex& s, bool* inside) {
+    const MPoint P[4] = {s.w[0], s.w[1], s.w[2], s.w[3]};
+    // faces (0,1,2) opp 3, (0,2,3) opp 1, (0,3,1) opp 2, (1,3,2) opp 0
+    const int F[4][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 3, 2, 0}};
+    const V3 ab = sub(P[1].p, P[0].p), ac = sub(P[2].p, P[0].p), ad = sub(P[3].p, P[0].p);
+    const float vol = dot(ad, cross(ab, ac));
+    const float lim = 1e-5f * (sqrtf(dot(ab, ab)) * sqrtf(dot(ac, ac)) * sqrtf(dot(ad, ad)));
+    const bool flat = !(fabsf(vol) > lim);
+    float best = 3.402823466e38f;
+    V3 bestp = v3(0.0f, 0.0f, 0.0f);
+    Simplex bests = s;
+    bool any = false;
+    for (int f = 0; f < 4; ++f) {
+        if (!flat && !origin_outside(P[F[f][0]].p, P[F[f][1]].p, P[F[f][2]].p, P[F[f][3]].p)) continue;
+        any = true;
+        Simplex t;
+        t.w[0] = P[F[f][0]];
+        t.w[1] = P[F[f][1]];
+        t.w[2] = P[F[f][2]];
+        t.n = 3;
+        const V3 q = closest_triangle(t);
+        const float dq = dot(q, q);
+        if (dq < best) {
+            best = dq;
+            bestp = q;
+            bests = t;
+        }
+    }
+    *inside = !any;
+    if (any) s = bests;
+    return bestp;
+}
+
+// Reduces s to the feature closest to the origin and returns that point.
+__device__ __forceinline__ V3 simplex_closest(Simplex& s, bool* inside) {
+    *inside = false;
+    if (s.n == 1) return s.w[0].p;
+    if (s.n == 2) return closest_segment(s);
+    if (s.n == 3) return closest_triangle(s);
+    return closest_tetra(s, inside);
+}
+
+// Sorts the simplex by vertex key so that the same feature always yields the
+// same arithmetic regardless of the order its vertices were found in.
+__device__ __forceinline__ void simplex_canonical(Simplex& s) {
+    for (int i = 1; i < s.n; ++i) {
+        MPoint x = s.w[i];
+        int j = i - 1;
+        while (j >= 0 && s.w[j].key > x.key) {
+            s.w[j + 1] = s.w[j];
+            --j;
+        }
+        s.w[j + 1] = x;
+    }
+}
+
+// ---------------------------------------------------------------- EPA
+
+struct EpaFace {
+    int a, b, c;
+    V3 n;
+    float d;
+};
+
+// Face (a, b, c) oriented away from the origin side interior point `inner`.
+__device__ __forceinline__ bool epa_make_face(const MPoint* P, int a, int b, int c, V3 inner, EpaFace& f) {
+    V3 nrm = cross(sub(P[b].p, P[a].p), sub(P[c].p, P[a].p));
+    const float l2 = dot(nrm, nrm);
+    if (!(l2 > 0.0f)) return false;
+    if (dot(nrm, sub(P[a].p, inner)) < 0.0f) {
+        const int t = b;
+        b = c;
+        c = t;
+        nrm = neg(nrm);
+    }
+    const float inv = 1.0f / sqrtf(l2);
+    f.a = a;
+    f.b = b;
+    f.c = c;
+    f.n = scl(nrm, inv);
+    f.d = dot(f.n, P[a].p);
+    return true;
+}
+
+// Canonical depth / normal of a face: vertices ordered by key.
+__device__ __forceinline__ void epa_face_canonical(const MPoint* P, const EpaFace& f, V3 inner, V3& n, float& d) {
+    MPoint q[3] = {P[f.a], P[f.b], P[f.c]};
+    for (int i = 1; i < 3; ++i) {
+        MPoint x = q[i];
+        int j = i - 1;
+        while (j >= 0 && q[j].key > x.key) {
+            q[j + 1] = q[j];
+            --j;
+        }
+        q[j + 1] = x;
+    }
+    V3 nrm = cross(sub(q[1].p, q[0].p), sub(q[2].p, q[0].p));
+    const float l2 = dot(nrm, nrm);
+    if (dot(nrm, sub(q[0].p, inner)) < 0.0f) nrm = neg(nrm);
+    n = scl(nrm, 1.0f / sqrtf(l2));
+    d = dot(n, q[0].p);
+}
+
+struct GjkOut {
+    float sep;
+    V3 n;
+    int flags;
+    int iters;
+    int epa_iters;
+};
+
+// EPA from a GJK simplex that encloses (or touches) the origin.
+__device__ inline void epa(const Shape& A, const Shape& B, const Simplex& s0, int max_iter, GjkOut& o) {
+    MPoint P[EPA_MAX_V];
+    int np = 0;
+    for (int i = 0; i < s0.n; ++i) P[np++] = s0.w[i];
+    // grow to a tetrahedron
+    const V3 axes[3] = {v3(1.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), v3(0.0f, 0.0f, 1.0f)};
+    if (np == 1) {
+        for (int k = 0; k < 6 && np == 1; ++k) {
+            const V3 u = (k & 1) ? neg(axes[k >> 1]) : axes[k >> 1];
+            const MPoint m = support_M(A, B, u);
+            const V3 dd = sub(m.p, P[0].p);
+            if (dot(dd, dd) > 1e-20f) P[np++] = m;
+        }
+    }
+    if (np == 2) {
+        const V3 ab = sub(P[1].p, P[0].p);
+        const float ax = fabsf(ab.x), ay = fabsf(ab.y), az = fabsf(ab.z);
+        const V3 e = (ax <= ay && ax <= az) ? axes[0] : (ay <= az ? axes[1] : axes[2]);
+        const V3 u = cross(ab, e);
+        for (int k = 0; k < 2 && np == 2; ++k) {
+            const MPoint m = support_M(A, B, k ? neg(u) : u);
+            const V3 c = cross(ab, sub(m.p, P[0].p));
+            if (dot(c, c) > 1e-24f) P[np++] = m;
+        }
+    }
+    if (np == 3) {
+        const V3 nrm = cross(sub(P[1].p, P[0].p), sub(P[2].p, P[0].p));
+        for (int k = 0; k < 2 && np == 3; ++k) {
+            const MPoint m = support_M(A, B, k ? neg(nrm) : nrm);
+            const float vol = dot(nrm, sub(m.p, P[0].p));
+            if (fabsf(vol) > 1e-24f) P[np++] = m;
+        }
+    }
+    if (np < 4) {
+        o.flags |= GJK_HIT | GJK_DEGENERATE;
+        o.sep = 0.0f;
+        return;
+    }
+    const V3 inner = scl(add(add(P[0].p, P[1].p), add(P[2].p, P[3].p)), 0.25f);
+    EpaFace F[EPA_MAX_F];
+    int nf = 0;
+    const int T[4][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
+    for (int k = 0; k < 4; ++k)
+        if (epa_make_face(P, T[k][0], T[k][1], T[k][2], inner, F[nf])) ++nf;
+    if (nf < 4) {
+        o.flags |= GJK_HIT | GJK_DEGENERATE;
+        o.sep = 0.0f;
+        return;
+    }
+    int best = 0;
+    int it = 0;
+    bool converged = false;
+    for (; it < max_iter; ++it) {
+        best = 0;
+        for (int f = 1; f < nf; ++f)
+            if (F[f].d < F[best].d) best = f;
+        const EpaFace fb = F[best];
+        const MPoint m = support_M(A, B, fb.n);
+        const float gap = dot(m.p, fb.n) - fb.d;
+        if (gap <= GJK_REL_EPS * fabsf(fb.d) + 1e-9f) {
+            converged = true;
+            break;
+        }
+        bool dup = false;
+        for (int i = 0; i < np; ++i) dup = dup || (m.key >= 0 && P[i].key == m.key);
+        if (dup || np >= EPA_MAX_V) {
+            converged = dup;
+            if (!dup) o.flags |= GJK_EPA_LIMIT;
+            break;
+        }
+        const int iv = np;
+        P[np++] = m;
+        // remove faces that see the new point, keep the horizon edges
+        int ea[EPA_MAX_E], eb[EPA_MAX_E];
+        int ne = 0;
+        bool overflow = false;
+        int w = 0;
+        for (int f = 0; f < nf; ++f) {
+            if (dot(F[f].n, sub(m.p, P[F[f].a].p)) > 0.0f) {
+                const int e[3][2] = {{F[f].a, F[f].b}, {F[f].b, F[f].c}, {F[f].c, F[f].a}};
+                for (int k = 0; k < 3; ++k) {
+                    int found = -1;
+                    for (int q = 0; q < ne; ++q)
+                        if (ea[q] == e[k][1] && eb[q] == e[k][0]) found = q;
+                    if (found >= 0) {
+                        ea[found] = ea[ne - 1];
+                        eb[found] = eb[ne - 1];
+                        --ne;
+                    } else if (ne < EPA_MAX_E) {
+                        ea[ne] = e[k][0];
+                        eb[ne] = e[k][1];
+                        ++ne;
+                    } else {
+                        overflow = true;
+                    }
+                }
+            } else {
+                F[w++] = F[f];
+            }
+        }
+        nf = w;
+        if (overflow || nf + ne > EPA_MAX_F) {
+            o.flags |= GJK_EPA_LIMIT;
+            break;
+        }
+        for (int q = 0; q < ne; ++q)
+            if (epa_make_face(P, ea[q], eb[q], iv, inner, F[nf])) ++nf;
+        if (nf == 0) {
+            o.flags |= GJK_DEGENERATE;
+            break;
+        }
+    }
+    if (nf == 0) {
+        o.flags |= GJK_HIT | GJK_DEGENERATE;
+        o.sep = 0.0f;
+        o.epa_iters = it;
+        return;
+    }
+    best = 0;
+    for (int f = 1; f < nf; ++f)
+        if (F[f].d < F[best].d) best = f;
+    V3 n;
+    float d;
+    epa_face_canonical(P, F[best], inner, n, d);
+    o.flags |= GJK_HIT | (converged ? GJK_CONVERGED : 0);
+    if (!converged && it >= max_iter) o.flags |= GJK_BUDGET;
+    if (d < 0.0f) o.flags |= GJK_DEGENERATE;  // polytope does not enclose the origin
+    o.sep = -d;
+    o.n = n;
+    o.epa_iters = it;
+}
+
+// GJK distance query. init_dir: initial search direction (warm start or the
+// centre difference); the first point is support_M(init_dir).
+__device__ inline GjkOut gjk_query(const Shape& A, const Shape& B, V3 init_dir, int max_iter) {
+    GjkOut o;
+    o.flags = 0;
+    o.iters = 0;
+    o.epa_iters = 0;
+    o.sep = 0.0f;
+    o.n = v3(1.0f, 0.0f, 0.0f);
+    if (dot(init_dir, init_dir) == 0.0f) init_dir = v3(1.0f, 0.0f, 0.0f);
+    Simplex s;
+    s.w[0] = support_M(A, B, init_dir);
+    s.n = 1;
+    V3 v = s.w[0].p;
+    bool hit = false, converged = false;
+    int it = 0;
+    while (it < max_iter) {
+        const float vv = dot(v, v);
+        if (vv <= GJK_TOUCH_EPS2) {
+            hit = true;
+            break;
+        }
+        const MPoint w = support_M(A, B, neg(v));
+        ++it;
+        const float vw = dot(v, w.p);
+        if (vv - vw <= GJK_REL_EPS * vv) {
+            converged = true;
+            break;
+        }
+        bool dup = false;
+        for (int i = 0; i < s.n; ++i) dup = dup || (w.key >= 0 && s.w[i].key == w.key);
+        if (dup) {
+            converged = true;
+            break;
+        }
+        const Simplex sp = s;
+        s.w[s.n++] = w;
+        bool inside = false;
+        const V3 vn = simplex_closest(s, &inside);
+        if (inside) {
+            hit = true;
+            break;
+        }
+        if (!(dot(vn, vn) < vv)) {
+            // no progress at FP32 resolution: the previous feature is final if the
+            // duality gap certifies it to 1e-4, otherwise the query failed
+            s = sp;
+            converged = vv - vw <= GJK_STALL_EPS * vv;
+            break;
+        }
+        v = vn;
+    }
+    o.iters = it;
+    if (hit) {
+        epa(A, B, s, max_iter, o);
+        return o;
+    }
+    if (!converged) o.flags |= GJK_BUDGET;
+    else o.flags |= GJK_CONVERGED;
+    // canonical recomputation from the converged feature
+    bool canon = true;
+    for (int i = 0; i < s.n; ++i) canon = canon && s.w[i].key >= 0;
+    if (canon) {
+        simplex_canonical(s);
+        bool inside = false;
+        v = simplex_closest(s, &inside);
+    }
+    const float dist = sqrtf(dot(v, v));
+    o.sep = dist;
+    o.n = dist > 0.0f ? scl(v, -1.0f / dist) : v3(1.0f, 0.0f, 0.0f);
+    return o;
+}
+
+}  // namespace sd

@@ -1,0 +1,359 @@
+// Convex-hull broadphase and narrowphase.
+//
+// Sweep-and-prune replaces SAPCollision::processCollisions and its kernels
+// (src/engine/cuda/collisions/SAP/sap_detection.cu:135-706): AABB, projection,
+// sort of the lower extents, windowed sweep. Differences by design: the sweep
+// axis is the one of largest centre variance (the reference's TODO at
+// sap_detection.cu:589), float keys are mapped order-preservingly to u32
+// (quirk Q3), the sweep stops at the first lower extent beyond the upper one
+// and tests all three axes (quirk Q5), and the pair list is exact and complete
+// (count, scan, write) instead of 16 fixed slots per object.
+//
+// GJK/EPA replaces gjkCUDA (sandbox/testGJK.cu:237-315); see sd_gjk_dev.hpp.
+#include "sd_gjk_dev.hpp"
+#include "sd_internal.hpp"
+
+namespace sd {
+
+// ------------------------------------------------------------------ AABBs
+
+__global__ void k_hull_aabb(const float4* __restrict__ pos, const float4* __restrict__ quat,
+                            const int* __restrict__ vstart, const int* __restrict__ nv,
+                            const float4* __restrict__ verts, int n, float4* __restrict__ lo4,
+                            float4* __restrict__ hi4) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = pos[i];
+    const Rot R = quat_rot(quat[i]);
+    const int k0 = vstart[i], kn = nv[i];
+    float lx, ly, lz, hx, hy, hz;
+    if (kn == 0) {  // sphere of radius c.w
+        lx = c.x - c.w;
+        ly = c.y - c.w;
+        lz = c.z - c.w;
+        hx = c.x + c.w;
+        hy = c.y + c.w;
+        hz = c.z + c.w;
+    } else {
+        lx = ly = lz = 3.402823466e38f;
+        hx = hy = hz = -3.402823466e38f;
+        for (int k = 0; k < kn; ++k) {
+            const float4 p = verts[k0 + k];
+            const V3 w = rot_mul(R, v3(p.x, p.y, p.z));
+            lx = fminf(lx, w.x);
+            ly = fminf(ly, w.y);
+            lz = fminf(lz, w.z);
+            hx = fmaxf(hx, w.x);
+            hy = fmaxf(hy, w.y);
+            hz = fmaxf(hz, w.z);
+        }
+        lx = lx + c.x;
+        ly = ly + c.y;
+        lz = lz + c.z;
+        hx = hx + c.x;
+        hy = hy + c.y;
+        hz = hz + c.z;
+    }
+    lo4[i] = make_float4(lx, ly, lz, 0.0f);
+    hi4[i] = make_float4(hx, hy, hz, 0.0f);
+}
+
+// ------------------------------------------------------------------ axis
+
+constexpr int AX_BLOCK = 256;
+constexpr int AX_BLOCKS = 256;
+
+// Per-block sums of centre and centre^2 per axis (FP64), grid-stride.
+__global__ __launch_bounds__(AX_BLOCK) void k_axis_partial(const float4* __restrict__ lo4,
+                                                           const float4* __restrict__ hi4, int n,
+                                                           double* __restrict__ part) {
+    __shared__ double red[6][AX_BLOCK];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * AX_BLOCK + threadIdx.x; i < n; i += gridDim.x * AX_BLOCK) {
+        const float4 l = lo4[i], h = hi4[i];
+        const double cx = 0.5 * ((double)l.x + (double)h.x);
+        const double cy = 0.5 * ((double)l.y + (double)h.y);
+        const double cz = 0.5 * ((double)l.z + (double)h.z);
+        acc[0] += cx;
+        acc[1] += cy;
+        acc[2] += cz;
+        acc[3] += cx * cx;
+        acc[4] += cy * cy;
+        acc[5] += cz * cz;
+    }
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = AX_BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Sums the partials in block order and picks the axis of largest variance.
+__global__ void k_axis_pick(const double* __restrict__ part, int nparts, int n, int* __restrict__ axis) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nparts; ++b)
+        for (int k = 0; k < 6; ++k) s[k] += part[b * 6 + k];
+    int best = 0;
+    double bv = -1.0;
+    for (int a = 0; a < 3; ++a) {
+        const double m = s[a] / (double)n;
+        const double var = s[3 + a] / (double)n - m * m;
+        if (var > bv) {
+            bv = var;
+            best = a;
+        }
+    }
+    *axis = best;
+}
+
+// Order-preserving map of a float to u32 (negative values flip all bits).
+__device__ __forceinline__ uint32_t float_key(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float axis_of(const float4& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+__global__ void k_sap_keys(const float4* __restrict__ lo4, int n, const int* __restrict__ axis,
+                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = float_key(axis_of(lo4[i], *axis));
+    vals[i] = (uint32_t)i;
+}
+
+__global__ void k_sap_gather(const uint32_t* __restrict__ order, int n, const float4* __restrict__ lo4,
+                             const float4* __restrict__ hi4, const int* __restrict__ axis,
+                             float4* __restrict__ slo, float4* __restrict__ shi) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = order[t];
+    const int a = *axis;
+    float4 l = lo4[i], h = hi4[i];
+    // store the sweep axis first: (sweep, other1, other2, id)
+    const float l0 = axis_of(l, a), h0 = axis_of(h, a);
+    const float l1 = axis_of(l, (a + 1) % 3), h1 = axis_of(h, (a + 1) % 3);
+    const float l2 = axis_of(l, (a + 2) % 3), h2 = axis_of(h, (a + 2) % 3);
+    slo[t] = make_float4(l0, l1, l2, __int_as_float((int)i));
+    shi[t] = make_float4(h0, h1, h2, 0.0f);
+}
+
+// WRITE = false: count the overlapping partners after t; WRITE = true: emit
+// (min id, max id) pairs at offsets[t].
+template <bool WRITE>
+__global__ void k_sap_sweep(const float4* __restrict__ slo, const float4* __restrict__ shi, int n,
+                            int* __restrict__ counts, const int* __restrict__ offsets, int* __restrict__ pairs,
+                            int capacity) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float4 l = slo[t], h = shi[t];
+    const int id = __float_as_int(l.w);
+    int o = WRITE ? offsets[t] : 0;
+    int cnt = 0;
+    for (int u = t + 1; u < n; ++u) {
+        const float4 lu = slo[u];
+        if (lu.x > h.x) break;  // sorted by lower extent: nothing further overlaps
+        const float4 hu = shi[u];
+        if (lu.y <= h.y && l.y <= hu.y && lu.z <= h.z && l.z <= hu.z) {
+            if (WRITE && o < capacity) {
+                const int iu = __float_as_int(lu.w);
+                pairs[2 * (size_t)o] = id < iu ? id : iu;
+                pairs[2 * (size_t)o + 1] = id < iu ? iu : id;
+            }
+            ++o;
+            ++cnt;
+        }
+    }
+    if (!WRITE) counts[t] = cnt;
+}
+
+// ------------------------------------------------------------------ GJK
+
+struct HullSet {
+    const float4* pos;
+    const float4* quat;
+    const int* vstart;
+    const int* nv;
+    const float4* verts;
+};
+
+__device__ __forceinline__ void make_shapes(const HullSet& h, int ia, int ib, Shape& A, Shape& B, V3& dc) {
+    const float4 ca = h.pos[ia], cb = h.pos[ib];
+    A.verts = h.verts + h.vstart[ia];
+    A.nv = h.nv[ia];
+    A.radius = ca.w;
+    A.R = quat_rot(h.quat[ia]);
+    A.off = v3(0.0f, 0.0f, 0.0f);
+    B.verts = h.verts + h.vstart[ib];
+    B.nv = h.nv[ib];
+    B.radius = cb.w;
+    B.R = quat_rot(h.quat[ib]);
+    dc = v3(cb.x - ca.x, cb.y - ca.y, cb.z - ca.z);
+    B.off = dc;
+}
+
+// One thread per pair. warm (nullable): per-pair initial direction (A -> B).
+// A warm-started query whose result is not converged within warm_budget
+// iterations, or lies within near_margin of contact, is redone cold with
+// max_iter iterations.
+__global__ void k_gjk(const int* __restrict__ pairs, int npairs, HullSet h, int n_hulls, int n_verts,
+                      const float* __restrict__ warm, int warm_budget, int max_iter, float near_margin,
+                      float* __restrict__ sep, float* __restrict__ normal, int* __restrict__ flags,
+                      int* __restrict__ iters) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
+    const bool bad = ia < 0 || ia >= n_hulls || ib < 0 || ib >= n_hulls ||
+                     h.vstart[ia < 0 || ia >= n_hulls ? 0 : ia] + h.nv[ia < 0 || ia >= n_hulls ? 0 : ia] > n_verts ||
+                     h.vstart[ib < 0 || ib >= n_hulls ? 0 : ib] + h.nv[ib < 0 || ib >= n_hulls ? 0 : ib] > n_verts;
+    if (bad) {  // ids or vertex ranges outside the hull set: reported, never dereferenced
+        sep[p] = 0.0f;
+        flags[p] = -1;
+        iters[p] = 0;
+        return;
+    }
+    Shape A, B;
+    V3 dc;
+    make_shapes(h, ia, ib, A, B, dc);
+    GjkOut o;
+    int spent = 0;
+    bool done = false;
+    if (warm) {
+        const V3 wd = v3(warm[3 * (size_t)p], warm[3 * (size_t)p + 1], warm[3 * (size_t)p + 2]);
+        o = gjk_query(A, B, wd, warm_budget < max_iter ? warm_budget : max_iter);
+        spent = o.iters + o.epa_iters;
+        done = (o.flags & GJK_CONVERGED) && !(o.flags & (GJK_DEGENERATE | GJK_EPA_LIMIT)) &&
+               fabsf(o.sep) > near_margin;
+        if (done) o.flags |= GJK_WARM_USED;
+    }
+    if (!done) {
+        o = gjk_query(A, B, dc, max_iter);
+        spent += o.iters + o.epa_iters;
+    }
+    sep[p] = o.sep;
+    normal[3 * (size_t)p] = o.n.x;
+    normal[3 * (size_t)p + 1] = o.n.y;
+    normal[3 * (size_t)p + 2] = o.n.z;
+    flags[p] = o.flags;
+    iters[p] = spent;
+}
+
+sd_status check_hulls(const float* pos4, const float* quat4, const int* vstart, const int* nv, const float* verts4) {
+    SD_REQUIRE(pos4 && quat4 && vstart && nv && verts4, "hulls: null pointer");
+    return SD_OK;
+}
+
+}  // namespace sd
+
+using namespace sd;
+
+extern "C" sd_status sd_hull_aabbs(const float* pos4, const float* quat4, const int* vstart, const int* nv,
+                                   const float* verts4, int n, float* lo4, float* hi4, void* stream) {
+    SD_REQUIRE(n >= 0, "sd_hull_aabbs: n < 0");
+    if (n == 0) return SD_OK;
+    sd_status st = check_hulls(pos4, quat4, vstart, nv, verts4);
+    if (st != SD_OK) return st;
+    SD_REQUIRE(lo4 && hi4, "sd_hull_aabbs: null output");
+    hipLaunchKernelGGL(k_hull_aabb, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, (const float4*)pos4,
+                       (const float4*)quat4, vstart, nv, (const float4*)verts4, n, (float4*)lo4, (float4*)hi4);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+extern "C" sd_status sd_sap_scratch_bytes(int n, size_t* bytes) {
+    SD_REQUIRE(bytes && n >= 0, "sd_sap_scratch_bytes: bad argument");
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    *bytes = 4 * nn * sizeof(uint32_t)        // keys, vals, alt keys, alt vals
+             + 2 * nn * sizeof(float4)        // sorted lower / upper boxes
+             + 2 * (nn + 2) * sizeof(int)     // counts, offsets (+ total)
+             + (size_t)AX_BLOCKS * 6 * sizeof(double) + 64  // axis partials + axis
+             + radix_temp_bytes(n) + scan_temp_bytes(n) + 8 * 256;
+    return SD_OK;
+}
+
+static char* carve(char*& p, size_t bytes) {
+    char* r = p;
+    p += (bytes + 255) & ~(size_t)255;
+    return r;
+}
+
+extern "C" sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int* pairs, int capacity, int* n_pairs,
+                                  int* axis_out, void* scratch, size_t scratch_bytes, void* stream) {
+    SD_REQUIRE(n >= 0 && capacity >= 0 && n_pairs, "sd_sap_pairs: bad argument");
+    if (n == 0) {
+        *n_pairs = 0;
+        return SD_OK;
+    }
+    SD_REQUIRE(lo4 && hi4 && scratch && (pairs || capacity == 0), "sd_sap_pairs: null pointer");
+    size_t need;
+    sd_sap_scratch_bytes(n, &need);
+    SD_REQUIRE(scratch_bytes >= need, "sd_sap_pairs: scratch too small (%zu < %zu)", scratch_bytes, need);
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)scratch;
+    uint32_t* keys = (uint32_t*)carve(p, (size_t)n * 4);
+    uint32_t* vals = (uint32_t*)carve(p, (size_t)n * 4);
+    uint32_t* keys2 = (uint32_t*)carve(p, (size_t)n * 4);
+    uint32_t* vals2 = (uint32_t*)carve(p, (size_t)n * 4);
+    float4* slo = (float4*)carve(p, (size_t)n * sizeof(float4));
+    float4* shi = (float4*)carve(p, (size_t)n * sizeof(float4));
+    int* counts = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
+    int* offsets = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
+    double* part = (double*)carve(p, (size_t)AX_BLOCKS * 6 * sizeof(double));
+    int* axis = (int*)carve(p, 64);
+    void* sort_tmp = (void*)carve(p, radix_temp_bytes(n));
+    void* scan_tmp = (void*)carve(p, scan_temp_bytes(n));
+    int* total = offsets + n;
+
+    const float4* L = (const float4*)lo4;
+    const float4* H = (const float4*)hi4;
+    const int nb = div_up(n, AX_BLOCK) < AX_BLOCKS ? div_up(n, AX_BLOCK) : AX_BLOCKS;
+    hipLaunchKernelGGL(k_axis_partial, dim3(nb), dim3(AX_BLOCK), 0, s, L, H, n, part);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_axis_pick, dim3(1), dim3(64), 0, s, part, nb, n, axis);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sap_keys, dim3(div_up(n, 256)), dim3(256), 0, s, L, n, axis, keys, vals);
+    SD_LAUNCH_CHECK();
+    sd_status st = radix_sort_pairs_u32(&keys, &vals, &keys2, &vals2, n, 0, 32, sort_tmp, radix_temp_bytes(n), s, 0);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_sap_gather, dim3(div_up(n, 256)), dim3(256), 0, s, vals, n, L, H, axis, slo, shi);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sap_sweep<false>, dim3(div_up(n, 128)), dim3(128), 0, s, slo, shi, n, counts, nullptr,
+                       nullptr, 0);
+    SD_LAUNCH_CHECK();
+    st = exclusive_scan_i32(counts, offsets, n, total, scan_tmp, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_sap_sweep<true>, dim3(div_up(n, 128)), dim3(128), 0, s, slo, shi, n, counts, offsets, pairs,
+                       capacity);
+    SD_LAUNCH_CHECK();
+    int h_total = 0, h_axis = 0;
+    SD_HIP(hipMemcpyAsync(&h_total, total, sizeof(int), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(&h_axis, axis, sizeof(int), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipStreamSynchronize(s));
+    *n_pairs = h_total;
+    if (axis_out) *axis_out = h_axis;
+    if (h_total > capacity)
+        return set_error(SD_ERR_CAPACITY, "sd_sap_pairs: %d pairs exceed capacity %d", h_total, capacity);
+    return SD_OK;
+}
+
+extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, const float* quat4, const int* vstart,
+                            const int* nv, const float* verts4, int n_hulls, int n_verts, const float* warm_dir,
+                            int warm_budget, int max_iter, float near_margin, float* sep, float* normal3, int* flags,
+                            int* iters, void* stream) {
+    SD_REQUIRE(n_pairs >= 0 && n_hulls >= 1 && n_verts >= 0, "sd_gjk: bad sizes");
+    if (n_pairs == 0) return SD_OK;
+    sd_status st = check_hulls(pos4, quat4, vstart, nv, verts4);
+    if (st != SD_OK) return st;
+    SD_REQUIRE(pairs && sep && normal3 && flags && iters, "sd_gjk: null pointer");
+    SD_REQUIRE(max_iter >= 1 && max_iter <= 1024, "sd_gjk: max_iter must be in [1, 1024]");
+    SD_REQUIRE(warm_dir == nullptr || warm_budget >= 1, "sd_gjk: warm_budget must be >= 1");
+    HullSet h{(const float4*)pos4, (const float4*)quat4, vstart, nv, (const float4*)verts4};
+    hipLaunchKernelGGL(k_gjk, dim3(div_up(n_pairs, 64)), dim3(64), 0, (hipStream_t)stream, pairs, n_pairs, h, n_hulls,
+                       n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}

@@ -118,3 +118,61 @@ extern "C" sd_status sd_gen_spheres(uint64_t seed, int n, const float lo[3], con
     }
     return SD_OK;
 }
+
+// Convex hulls of points on ellipsoids. Every point of a strictly convex
+// surface is a vertex of the hull of the sample, so the point set is the hull.
+// Per hull, in this order: vertex count U{nv_min..nv_max}, semi-axes U[ax_min,
+// ax_max] (x, y, z), the vertices (Gaussian 3-vectors normalised and scaled by
+// the semi-axes), an orientation (Shoemake's uniform unit quaternion, stored
+// (w, x, y, z)), a centre uniform in [lo + ax_max, hi - ax_max] and a velocity
+// N(0, v_sigma) per component.
+extern "C" sd_status sd_gen_hulls(uint64_t seed, int n, const float lo[3], const float hi[3], int nv_min, int nv_max,
+                                  float ax_min, float ax_max, float v_sigma, float* pos4, float* quat4, float* vel4,
+                                  int* vstart, int* nv, float* verts4, int verts_capacity, int* n_verts) {
+    SD_REQUIRE(n >= 0 && lo && hi && pos4 && quat4 && vel4 && vstart && nv && verts4 && n_verts,
+               "sd_gen_hulls: bad argument");
+    SD_REQUIRE(nv_min >= 1 && nv_max >= nv_min && nv_max <= 1024, "sd_gen_hulls: vertex counts must be in [1, 1024]");
+    SD_REQUIRE(ax_min > 0.0f && ax_max >= ax_min, "sd_gen_hulls: bad semi-axes");
+    sd::SplitMix64 rng(seed);
+    int total = 0;
+    for (int i = 0; i < n; ++i) {
+        const int k = nv_min + (int)(rng.next() % (uint64_t)(nv_max - nv_min + 1));
+        if (total + k > verts_capacity)
+            return sd::set_error(SD_ERR_CAPACITY, "sd_gen_hulls: vertex capacity %d exceeded", verts_capacity);
+        float ax[3];
+        for (int a = 0; a < 3; ++a) ax[a] = ax_min + rng.uniform() * (ax_max - ax_min);
+        vstart[i] = total;
+        nv[i] = k;
+        for (int j = 0; j < k; ++j) {
+            double g[3], l2 = 0.0;
+            do {
+                l2 = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    g[a] = rng.normal();
+                    l2 += g[a] * g[a];
+                }
+            } while (l2 < 1e-12);
+            const double il = 1.0 / sqrt(l2);
+            float* v = verts4 + 4 * (size_t)(total + j);
+            for (int a = 0; a < 3; ++a) v[a] = (float)(g[a] * il * (double)ax[a]);
+            v[3] = 0.0f;
+        }
+        total += k;
+        const double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+        const double s1 = sqrt(1.0 - u1), s2 = sqrt(u1);
+        const double t1 = 6.283185307179586 * u2, t2 = 6.283185307179586 * u3;
+        float* q = quat4 + 4 * (size_t)i;
+        q[0] = (float)(s2 * cos(t2));  // w
+        q[1] = (float)(s1 * sin(t1));  // x
+        q[2] = (float)(s1 * cos(t1));  // y
+        q[3] = (float)(s2 * sin(t2));  // z
+        float* p = pos4 + 4 * (size_t)i;
+        for (int a = 0; a < 3; ++a) p[a] = (lo[a] + ax_max) + rng.uniform() * ((hi[a] - ax_max) - (lo[a] + ax_max));
+        p[3] = 0.0f;
+        float* v = vel4 + 4 * (size_t)i;
+        for (int a = 0; a < 3; ++a) v[a] = (float)(v_sigma * rng.normal());
+        v[3] = 0.0f;
+    }
+    *n_verts = total;
+    return SD_OK;
+}

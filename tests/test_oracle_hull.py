@@ -1,0 +1,190 @@
+"""Pins the hull oracle (CPU only): AABBs against a float64 rotation, SAP against
+brute force, GJK/EPA against exact FP64 geometry (Qhull facets of the Minkowski
+difference and point-triangle distances) and closed-form known answers."""
+import math
+
+import numpy as np
+import pytest
+
+from stardust_amd import hulls as H
+
+scipy_spatial = pytest.importorskip("scipy.spatial")
+
+
+def qmat(q):
+    w, x, y, z = (np.float64(v) for v in q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def world(h, i, origin):
+    v = h["verts4"][h["vstart"][i]:h["vstart"][i] + h["nv"][i], :3].astype(np.float64)
+    return v @ qmat(h["quat4"][i]).T + (h["pos4"][i, :3].astype(np.float64) - origin)
+
+
+def closest_on_triangle(a, b, c):
+    """Point of triangle abc nearest the origin, FP64 (Voronoi regions)."""
+    p = np.zeros(3)
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = ab @ ap, ac @ ap
+    if d1 <= 0 and d2 <= 0:
+        return a
+    bp = p - b
+    d3, d4 = ab @ bp, ac @ bp
+    if d3 >= 0 and d4 <= d3:
+        return b
+    vc = d1 * d4 - d3 * d2
+    if vc <= 0 and d1 >= 0 and d3 <= 0:
+        return a + d1 / (d1 - d3) * ab
+    cp = p - c
+    d5, d6 = ab @ cp, ac @ cp
+    if d6 >= 0 and d5 <= d6:
+        return c
+    vb = d5 * d2 - d1 * d6
+    if vb <= 0 and d2 >= 0 and d6 <= 0:
+        return a + d2 / (d2 - d6) * ac
+    va = d3 * d6 - d5 * d4
+    if va <= 0 and (d4 - d3) >= 0 and (d5 - d6) >= 0:
+        return b + (d4 - d3) / ((d4 - d3) + (d5 - d6)) * (c - b)
+    den = 1 / (va + vb + vc)
+    return a + ab * (vb * den) + ac * (vc * den)
+
+
+def exact_signed_distance(h, a, b):
+    """FP64: > 0 separation, < 0 minus the penetration depth (Qhull of A - B)."""
+    o = h["pos4"][a, :3].astype(np.float64)
+    A, B = world(h, a, o), world(h, b, o)
+    M = (A[:, None, :] - B[None, :, :]).reshape(-1, 3)
+    hull = scipy_spatial.ConvexHull(M)
+    off = hull.equations[:, 3]  # facet: n . x + off <= 0 inside
+    if np.all(off < 0):
+        return float(off.max())
+    return min(float(np.linalg.norm(closest_on_triangle(*M[s]))) for s in hull.simplices)
+
+
+@pytest.fixture(scope="module")
+def scene(sdlib):
+    return H.generate_hulls(0x5EED0004, 6000)
+
+
+def test_generator_shapes(scene):
+    h = scene
+    assert h["nv"].min() >= 8 and h["nv"].max() <= 32
+    assert np.all(np.abs(np.linalg.norm(h["quat4"], axis=1) - 1) < 1e-6)
+    v = h["verts4"][:, :3].astype(np.float64)
+    r = np.linalg.norm(v, axis=1)
+    assert r.min() >= 0.005 * 0.999 and r.max() <= 0.015 * 1.001 * math.sqrt(1)  # on ellipsoids 5..15 mm
+
+
+def test_aabbs_match_fp64_rotation(oracle, scene):
+    lo, hi = oracle.hull_aabbs(scene)
+    for i in range(0, len(lo), 97):
+        w = world(scene, i, np.zeros(3))
+        assert np.allclose(lo[i, :3], w.min(0), atol=2e-7)
+        assert np.allclose(hi[i, :3], w.max(0), atol=2e-7)
+
+
+def test_sap_equals_brute_force(oracle, scene):
+    lo, hi = oracle.hull_aabbs(scene)
+    got, axis = oracle.sap_pairs(lo, hi)
+    brute = oracle.brute_aabb_pairs(lo, hi)
+    assert len(got) == len(set(map(tuple, got.tolist())))
+    assert set(map(tuple, got.tolist())) == set(map(tuple, brute.tolist()))
+    # the pair set is the same on every sweep axis
+    for ax in range(3):
+        other, _ = oracle.sap_pairs(lo, hi, axis=ax)
+        assert set(map(tuple, other.tolist())) == set(map(tuple, brute.tolist()))
+    c = 0.5 * (lo[:, :3].astype(np.float64) + hi[:, :3])
+    assert axis == int(np.argmax(c.var(0)))
+
+
+def test_gjk_matches_exact_geometry(oracle, scene):
+    lo, hi = oracle.hull_aabbs(scene)
+    pairs, _ = oracle.sap_pairs(lo, hi)
+    r = oracle.gjk(scene, pairs, nthreads=8)
+    assert np.all(r["flags"] & 2 | (r["flags"] & 4))  # every query ends converged or budget-flagged
+    assert (r["flags"] & 4).sum() <= 0.005 * len(pairs)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(len(pairs), min(400, len(pairs)), replace=False)
+    errs_sep, errs_hit = [], []
+    for i in idx:
+        if r["flags"][i] & 4:
+            continue
+        t = exact_signed_distance(scene, *pairs[i])
+        if abs(t) < 1e-6:  # within the contact margin: verdict not asserted
+            continue
+        assert bool(r["flags"][i] & 1) == (t < 0), (i, t, r["sep"][i])
+        e = abs(r["sep"][i] - t)
+        (errs_hit if t < 0 else errs_sep).append(e / max(abs(t), 1e-3))
+    # FP32 floor: coordinates ~1e-2 m carry ~1e-9 m absolute error
+    assert max(errs_sep) < 1e-5 and max(errs_hit) < 1e-5
+    assert np.allclose(np.linalg.norm(r["normal"], axis=1), 1.0, atol=1e-5)
+
+
+def shapes_from(verts_list, centres, quats=None, radii=None):
+    n = len(centres)
+    vstart, nv, vs = [], [], []
+    for v in verts_list:
+        vstart.append(sum(len(x) for x in vs))
+        nv.append(len(v))
+        vs.append(np.asarray(v, np.float32).reshape(-1, 3))
+    verts = np.concatenate(vs) if sum(nv) else np.zeros((0, 3), np.float32)
+    pos4 = np.zeros((n, 4), np.float32)
+    pos4[:, :3] = centres
+    if radii is not None:
+        pos4[:, 3] = radii
+    quat4 = np.tile(np.array([1, 0, 0, 0], np.float32), (n, 1)) if quats is None else np.asarray(quats, np.float32)
+    v4 = np.zeros((max(len(verts), 1), 4), np.float32)
+    v4[:len(verts), :3] = verts
+    return {"pos4": pos4, "quat4": quat4, "vstart": np.array(vstart, np.int32), "nv": np.array(nv, np.int32),
+            "verts4": v4}
+
+
+CUBE = [[sx, sy, sz] for sx in (-0.5, 0.5) for sy in (-0.5, 0.5) for sz in (-0.5, 0.5)]
+
+
+@pytest.mark.parametrize("dx,expect", [(1.5, 0.5), (2.0, 1.0), (0.75, -0.25), (1.0 + 1e-3, 1e-3)])
+def test_gjk_cube_pairs_closed_form(oracle, dx, expect):
+    h = shapes_from([CUBE, CUBE], [[0.1, 0.2, 0.3], [0.1 + dx, 0.2, 0.3]])
+    r = oracle.gjk(h, np.array([[0, 1]], np.int32))
+    assert abs(r["sep"][0] - expect) < 1e-6 * max(1.0, abs(expect)) + 2e-7
+    assert np.allclose(r["normal"][0], [1, 0, 0], atol=1e-6)
+    assert bool(r["flags"][0] & 1) == (expect < 0)
+
+
+def test_gjk_rotated_cube_vertex_face(oracle):
+    # cube B rotated 45 deg about z: its edge x-extent is sqrt(2)/2
+    q = [math.cos(math.pi / 8), 0.0, 0.0, math.sin(math.pi / 8)]
+    h = shapes_from([CUBE, CUBE], [[0, 0, 0], [2.0, 0, 0]], quats=[[1, 0, 0, 0], q])
+    r = oracle.gjk(h, np.array([[0, 1]], np.int32))
+    assert abs(r["sep"][0] - (2.0 - 0.5 - math.sqrt(2) / 2)) < 1e-6
+
+
+def test_gjk_reference_tetrahedron_sphere_known_answer(oracle):
+    # sandbox/testGJK.cu:328-352: unit tetrahedron (0,0,0),(1,0,0),(0,1,0),(0,0,1)
+    # and a sphere of radius 0.5 at (0, 0, 1.5): exactly touching at (0, 0, 1).
+    tet = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
+    for z, expect in ((1.5, 0.0), (1.6, 0.1), (1.4, -0.1)):
+        h = shapes_from([tet, []], [[0, 0, 0], [0, 0, z]], radii=[0.0, 0.5])
+        r = oracle.gjk(h, np.array([[0, 1]], np.int32))
+        assert abs(r["sep"][0] - expect) < 2e-4, (z, r["sep"][0])
+        if expect != 0.0:
+            assert bool(r["flags"][0] & 1) == (expect < 0)
+            # a curved support converges linearly; the gap certificate bounds the
+            # distance to 1e-4 relative, the normal to about its square root
+            assert np.allclose(r["normal"][0], [0, 0, 1], atol=5e-2)
+
+
+def test_warm_start_same_verdicts_and_depths(oracle, scene):
+    lo, hi = oracle.hull_aabbs(scene)
+    pairs, _ = oracle.sap_pairs(lo, hi)
+    cold = oracle.gjk(scene, pairs, nthreads=8)
+    rng = np.random.default_rng(3)
+    noisy = cold["normal"] + rng.normal(0, 0.2, cold["normal"].shape).astype(np.float32)
+    for warm in (cold["normal"], noisy, rng.normal(0, 1, cold["normal"].shape).astype(np.float32)):
+        w = oracle.gjk(scene, pairs, warm=warm, warm_budget=16, nthreads=8)
+        ok = ((cold["flags"] | w["flags"]) & 4) == 0
+        assert np.array_equal(cold["flags"][ok] & 1, w["flags"][ok] & 1)
+        diff = np.abs(w["sep"][ok] - cold["sep"][ok]) / np.maximum(np.abs(cold["sep"][ok]), 1e-3)
+        assert diff.max() <= 1e-5
