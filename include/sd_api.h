@@ -259,6 +259,39 @@ sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, const float* 
                  const int* nv, const float* verts4, int n_hulls, int n_verts, const float* warm_dir, int warm_budget,
                  int max_iter, float near_margin, float* sep, float* normal3, int* flags, int* iters, void* stream);
 
+/* ------------------------------------------------------- learned warm start
+ * No counterpart in the reference (gjkCUDA starts from the centre difference,
+ * sandbox/testGJK.cu:266). MLP 22 -> 64 -> 64 -> 64 -> 3 (SiLU), FP32 master
+ * parameters (sd_mlp_param_count() floats, layout W1 b1 W2 b2 W3 b3 W4 b4 with
+ * W[in][out]); loss 1 - u . t with u = y / |y|.
+ *
+ * Per hull: local half extents (3), mean / max / min vertex radius, count / 32. */
+int sd_mlp_param_count(void);
+int sd_mlp_packed_bytes(void);
+sd_status sd_hull_descriptors(const float* pos4, const int* vstart, const int* nv, const float* verts4, int n,
+                              float* desc7, void* stream);
+/* 22 features per pair (relative pose in a's frame, both descriptors). */
+sd_status sd_mlp_features(const int* pairs, int npairs, const float* pos4, const float* quat4, const float* desc7,
+                          float* X, void* stream);
+/* GJK normals (world, a -> b) expressed in a's frame: the training targets. */
+sd_status sd_mlp_targets(const int* pairs, int npairs, const float* quat4, const float* normal3, float* T,
+                         void* stream);
+sd_status sd_mlp_init(float* params, uint64_t seed, void* stream);
+/* Gradient of the mean loss over rows idx[0..batch) of X/T (deterministic,
+ * fixed reduction order) and the loss itself (loss may be NULL). */
+sd_status sd_mlp_train_scratch_bytes(int batch, size_t* bytes);
+sd_status sd_mlp_forward_backward(const float* X, const float* T, const int* idx, int batch, const float* params,
+                                  float* grad, float* loss, void* scratch, size_t scratch_bytes, void* stream);
+/* Adam with bias correction; step counts from 1. */
+sd_status sd_adam_step(float* params, const float* grad, float* m, float* v, int n, float lr, float beta1,
+                       float beta2, float eps, int step, void* stream);
+/* BF16 copy of the weights in the MFMA inference layout (sd_mlp_packed_bytes() bytes). */
+sd_status sd_mlp_pack_bf16(const float* params, void* packed, void* stream);
+/* Unit directions for n feature rows (BF16 MFMA GEMMs, FP32 accumulation).
+ * With pairs/quat4 given the direction is rotated from a's frame to the world. */
+sd_status sd_mlp_infer_bf16(const float* X, int n, const void* packed, const int* pairs, const float* quat4,
+                            float* out_dir3, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,6 +125,17 @@ _SIGS = {
     "sd_sap_scratch_bytes": [_I, C.POINTER(_SZ)],
     "sd_sap_pairs": [_P, _P, _I, _P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), _P, _SZ, _P],
     "sd_gjk": [_P, _I, _P, _P, _P, _P, _P, _I, _I, _P, _I, _I, _F, _P, _P, _P, _P, _P],
+    "sd_mlp_param_count": [],
+    "sd_mlp_packed_bytes": [],
+    "sd_hull_descriptors": [_P, _P, _P, _P, _I, _P, _P],
+    "sd_mlp_features": [_P, _I, _P, _P, _P, _P, _P],
+    "sd_mlp_targets": [_P, _I, _P, _P, _P, _P],
+    "sd_mlp_init": [_P, C.c_uint64, _P],
+    "sd_mlp_train_scratch_bytes": [_I, C.POINTER(_SZ)],
+    "sd_mlp_forward_backward": [_P, _P, _P, _I, _P, _P, _P, _P, _SZ, _P],
+    "sd_adam_step": [_P, _P, _P, _P, _I, _F, _F, _F, _F, _I, _P],
+    "sd_mlp_pack_bf16": [_P, _P, _P],
+    "sd_mlp_infer_bf16": [_P, _I, _P, _P, _P, _P, _P],
 }
 GJK_HIT, GJK_CONVERGED, GJK_BUDGET, GJK_EPA_LIMIT, GJK_DEGENERATE, GJK_WARM_USED = 1, 2, 4, 8, 16, 32
 _RESTYPE = {"sd_last_error": C.c_char_p, "sd_version": C.c_char_p}
