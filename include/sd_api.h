@@ -259,6 +259,68 @@ sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, const float* 
                  const int* nv, const float* verts4, int n_hulls, int n_verts, const float* warm_dir, int warm_budget,
                  int max_iter, float near_margin, float* sep, float* normal3, int* flags, int* iters, void* stream);
 
+/* ------------------------------------------------------- LBVH broadphase
+ * Replaces the sandbox LBVH pipeline (sandbox/testLBVH.cu:42-954).
+ * Global domain: domain6 = (min x, y, z of lo4, max x, y, z of hi4), computed
+ * on the device (scratch: 256 * 6 floats). Replaces constructGlobalDomain (:61-88). */
+sd_status sd_global_domain(const float* lo4, const float* hi4, int n, float* domain6, void* scratch, void* stream);
+/* keys[i] = morton30(box centroid on a 1024^3 grid over the domain) << 32 | i.
+ * Replaces computeSpaceFillingCurveCUDA (:114-151). */
+sd_status sd_morton30(const float* lo4, const float* hi4, int n, const float* domain6, uint64_t* keys, void* stream);
+/* Karras radix tree over sorted keys (internal nodes 0..n-2, leaf k = node n-1+k),
+ * node boxes by a bottom-up refit, and ropes (next node in depth-first order
+ * after a subtree, -1 at the end). Arrays of 2n-1 entries except left/right
+ * (n-1) and counters (n-1). *error (device int, cleared by the caller) gets
+ * nonzero bits for a malformed tree. Replaces generatePrefixesCUDA ...
+ * findLeafIndexRangeCUDA (:213-751). */
+sd_status sd_lbvh_build(const uint64_t* sorted_keys, int n, const float* lo4, const float* hi4, int* left,
+                        int* right, int* parent, int* first, int* last, int* rope, float* node_lo4, float* node_hi4,
+                        int* counters, int* error, void* stream);
+/* Every unordered pair of overlapping leaf boxes once, as (min id, max id),
+ * grouped by the sorted position of the first leaf. Replaces
+ * findPotentialCollisionsCUDA (:769-864). */
+sd_status sd_lbvh_query_scratch_bytes(int n, size_t* bytes);
+sd_status sd_lbvh_query(const uint64_t* sorted_keys, int n, const int* left, const int* last, const int* rope,
+                        const float* node_lo4, const float* node_hi4, int* pairs, int capacity, int* n_pairs,
+                        int* error, void* scratch, size_t scratch_bytes, void* stream);
+
+/* ------------------------------------------------------- rigid clumps
+ * A clump scene context: bodies made of spheres, LBVH broadphase over body
+ * AABBs, sphere-sphere contacts (the sphere contact law without tangential
+ * history), forces and torques summed per body, quaternion integration.
+ * Walls are not applied. Replaces DEMEngine::update for multi-sphere entities
+ * (simulation_kernel.cu:52-153 with physics_update.cu:26-348). */
+typedef struct sd_clump_params {
+    sd_contact_params contact;
+} sd_clump_params;
+typedef struct sd_clump_ctx sd_clump_ctx;
+sd_status sd_clump_create(const sd_clump_params* p, sd_clump_ctx** ctx);
+sd_status sd_clump_destroy(sd_clump_ctx* ctx);
+/* Bodies: pos4 (x, y, z, mass; mass 0 = static), vel4 (v, 0), quat4 (w, x, y, z),
+ * angvel4 (world angular velocity, 0), inertia6 (body frame: xx, yy, zz, xy, xz, yz
+ * about the centre of mass), first/count sphere ranges. Spheres: local4
+ * (body-frame offset from the centre of mass, radius). */
+sd_status sd_clump_upload(sd_clump_ctx* ctx, int nb, const float* pos4, const float* vel4, const float* quat4,
+                          const float* angvel4, const float* inertia6, const int* first, const int* count, int ns,
+                          const float* local4);
+sd_status sd_clump_step(sd_clump_ctx* ctx, int nsteps);
+/* Stage times (ms, summed over nsteps eager steps): [0] prep + pose + body AABBs,
+ * [1] domain + Morton + sort, [2] LBVH build, [3] query, [4] adjacency,
+ * [5] contacts + integration. */
+sd_status sd_clump_profile(sd_clump_ctx* ctx, int nsteps, float* stage_ms, int nstages);
+sd_status sd_clump_download(sd_clump_ctx* ctx, float* pos4, float* vel4, float* quat4, float* angmom4);
+sd_status sd_clump_pairs(sd_clump_ctx* ctx, int* pairs, int capacity, int* n_pairs);
+sd_status sd_clump_spheres(sd_clump_ctx* ctx, float* pos4);
+/* Clumps of count U{c_min..c_max} spheres with centres N(0, sigma) about the
+ * body centre, radii U[r_min, r_max], density rho; bodies on a jittered cubic
+ * lattice (spacing, jitter) with uniform orientations and velocities
+ * N(0, v_sigma). Outputs as sd_clump_upload takes them (centre of mass and
+ * inertia from the member spheres). local4 capacity: nb * c_max. */
+sd_status sd_gen_clumps(uint64_t seed, int nb, int lattice, float spacing, float jitter, int c_min, int c_max,
+                        float sigma, float r_min, float r_max, float rho, float v_sigma, float* pos4, float* vel4,
+                        float* quat4, float* angvel4, float* inertia6, int* first, int* count, float* local4,
+                        int* ns);
+
 /* ------------------------------------------------------- learned warm start
  * No counterpart in the reference (gjkCUDA starts from the centre difference,
  * sandbox/testGJK.cu:266). MLP 22 -> 64 -> 64 -> 64 -> 3 (SiLU), FP32 master

@@ -45,7 +45,74 @@ _SIGS = {
     "or_sap_pairs": ([_I, _P, _P, _I, _P, C.c_long], C.c_long),
     "or_brute_aabb_pairs": ([_I, _P, _P, _P, C.c_long], C.c_long),
     "or_gjk": ([_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, C.c_float, _P, _P, _P, _P, _I], None),
+    "or_clump_prepare": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
+    "or_clump_pose": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
+    "or_global_domain": ([_I, _P, _P, _P], None),
+    "or_morton30": ([_I, _P, _P, _P, _P], None),
+    "or_clump_contacts_integrate": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_long, _P, C.POINTER(OrContact),
+                                     _P, _P, _I], _I),
+    "or_clump_step": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, C.POINTER(OrContact)], C.c_long),
 }
+
+
+class ClumpState:
+    """Rigid-clump state as the GPU context prepares it (float4 rows)."""
+
+    def __init__(self, s):
+        nb = len(s["pos4"])
+        self.nb, self.ns = nb, len(s["local4"])
+        self.s = s
+        self.P, self.V, self.Q, self.L, self.I0, self.I1 = (np.zeros((nb, 4), np.float32) for _ in range(6))
+        a = [np.ascontiguousarray(s[k], np.float32) for k in ("pos4", "vel4", "quat4", "angvel4", "inertia6")]
+        load().or_clump_prepare(nb, *[p(x) for x in a], p(self.P), p(self.V), p(self.Q), p(self.L), p(self.I0),
+                                p(self.I1))
+        self.first = np.ascontiguousarray(s["first"], np.int32)
+        self.count = np.ascontiguousarray(s["count"], np.int32)
+        self.local4 = np.ascontiguousarray(s["local4"], np.float32)
+
+    def pose(self):
+        sx = np.zeros((self.ns, 4), np.float32)
+        sv = np.zeros((self.ns, 4), np.float32)
+        lo = np.zeros((self.nb, 4), np.float32)
+        hi = np.zeros((self.nb, 4), np.float32)
+        load().or_clump_pose(self.nb, p(self.P), p(self.V), p(self.Q), p(self.L), p(self.I0), p(self.I1),
+                             p(self.first), p(self.count), p(self.local4), p(sx), p(sv), p(lo), p(hi))
+        return sx, sv, lo, hi
+
+    def step(self, c, nsteps=1):
+        oc = contact(c)
+        np_ = 0
+        for _ in range(nsteps):
+            np_ = load().or_clump_step(self.nb, p(self.P), p(self.V), p(self.Q), p(self.L), p(self.I0), p(self.I1),
+                                       p(self.first), p(self.count), p(self.local4), self.ns, C.byref(oc))
+        return np_
+
+    def forces(self, c, pairs):
+        """Force/torque of every body for the current state and a given pair set."""
+        sx, sv, lo, hi = self.pose()
+        f = np.zeros((self.nb, 4), np.float32)
+        t = np.zeros((self.nb, 4), np.float32)
+        oc = contact(c)
+        pr = np.ascontiguousarray(pairs, np.int32)
+        load().or_clump_contacts_integrate(self.nb, p(self.P), p(self.V), p(self.Q), p(self.L), p(self.I0),
+                                           p(self.I1), p(self.first), p(self.count), p(sx), p(sv), len(pr), p(pr),
+                                           C.byref(oc), p(f), p(t), 0)
+        return f, t
+
+
+def global_domain(lo, hi):
+    d = np.zeros(6, np.float32)
+    lo, hi = np.ascontiguousarray(lo, np.float32), np.ascontiguousarray(hi, np.float32)
+    load().or_global_domain(len(lo), p(lo), p(hi), p(d))
+    return d
+
+
+def morton30(lo, hi, dom):
+    lo, hi = np.ascontiguousarray(lo, np.float32), np.ascontiguousarray(hi, np.float32)
+    dom = np.ascontiguousarray(dom, np.float32)
+    k = np.zeros(len(lo), np.uint64)
+    load().or_morton30(len(lo), p(lo), p(hi), p(dom), p(k))
+    return k
 
 
 def hull_aabbs(h):

@@ -103,6 +103,22 @@ void or_gjk(int npairs, const int* pairs, const float* pos4, const float* quat4,
             const float* verts4, const float* warm, int warm_budget, int max_iter, float near_margin, float* sep,
             float* normal3, int* flags, int* iters, int nthreads);
 
+/* ------------------------------------------------------------------ clumps
+ * (sd_oracle_clump.c) body state P (x, y, z, 1/m), V, Q (w, x, y, z), L (world
+ * angular momentum), inverse body inertia I0 (xx, yy, zz, xy) / I1 (xz, yz). */
+void or_clump_prepare(int nb, const float* pos4, const float* vel4, const float* quat4, const float* angvel4,
+                      const float* inertia6, float* P, float* V, float* Q, float* L, float* I0, float* I1);
+void or_clump_pose(int nb, const float* P, const float* V, const float* Q, const float* L, const float* I0,
+                   const float* I1, const int* first, const int* count, const float* local4, float* sx, float* sv,
+                   float* lo4, float* hi4);
+void or_global_domain(int n, const float* lo4, const float* hi4, float* dom6);
+void or_morton30(int n, const float* lo4, const float* hi4, const float* dom6, uint64_t* keys);
+int or_clump_contacts_integrate(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1,
+                                const int* first, const int* count, const float* sx, const float* sv, long npairs,
+                                const int* pairs, const or_contact* c, float* force4, float* torque4, int integrate);
+long or_clump_step(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const int* first,
+                   const int* count, const float* local4, int ns, const or_contact* c);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,379 @@
+/*
+ * sd_oracle_clump.c — CPU oracle for rigid clumps and the LBVH broadphase
+ * (TEST INFRASTRUCTURE ONLY; see sd_oracle.h).
+ *
+ * Restates, as plain C with the documented FP32 operation order:
+ *   - the clump pose, force/torque reduction and rigid integration of
+ *     src/engine/cuda/physics_update.cu:26-348 with the corrected inertia
+ *     (R I^-1 R^T, quirks Q10/Q11) and quaternion renormalisation (Q12);
+ *   - the Morton keys of sandbox/testLBVH.cu:91-171 on a 1024^3 grid over the
+ *     global domain (:61-88).
+ * The broadphase pair set is computed by serial sweep-and-prune over the same
+ * body boxes: the LBVH must return exactly this set (every overlapping pair once).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sd_oracle.h"
+
+typedef struct {
+    float x, y, z;
+} V3;
+static V3 mk(float x, float y, float z) {
+    V3 r = {x, y, z};
+    return r;
+}
+
+static void qmat(const float* q, float m[9]) {
+    const float w = q[0], x = q[1], y = q[2], z = q[3];
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, xz = x * z, yz = y * z;
+    const float wx = w * x, wy = w * y, wz = w * z;
+    m[0] = 1.0f - 2.0f * (yy + zz);
+    m[1] = 2.0f * (xy - wz);
+    m[2] = 2.0f * (xz + wy);
+    m[3] = 2.0f * (xy + wz);
+    m[4] = 1.0f - 2.0f * (xx + zz);
+    m[5] = 2.0f * (yz - wx);
+    m[6] = 2.0f * (xz - wy);
+    m[7] = 2.0f * (yz + wx);
+    m[8] = 1.0f - 2.0f * (xx + yy);
+}
+static V3 mv(const float m[9], V3 v) {
+    return mk(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+              m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+static V3 mtv(const float m[9], V3 v) {
+    return mk(m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+              m[2] * v.x + m[5] * v.y + m[8] * v.z);
+}
+
+/* w = R I^-1 R^T L, I^-1 = [[a, d, e], [d, b, f], [e, f, c]] from i0 = (a, b, c, d), i1 = (e, f) */
+static V3 inv_inertia(const float m[9], const float* i0, const float* i1, V3 L) {
+    const V3 u = mtv(m, L);
+    const V3 w = mk(i0[0] * u.x + i0[3] * u.y + i1[0] * u.z, i0[3] * u.x + i0[1] * u.y + i1[1] * u.z,
+                    i1[0] * u.x + i1[1] * u.y + i0[2] * u.z);
+    return mv(m, w);
+}
+
+static void sincos_small(float a, float* s, float* c) {
+    if (a < 0.25f) {
+        const float a2 = a * a;
+        *s = a * (1.0f - a2 / 6.0f * (1.0f - a2 / 20.0f * (1.0f - a2 / 42.0f * (1.0f - a2 / 72.0f))));
+        *c = 1.0f - a2 / 2.0f * (1.0f - a2 / 12.0f * (1.0f - a2 / 30.0f * (1.0f - a2 / 56.0f)));
+    } else {
+        *s = sinf(a);
+        *c = cosf(a);
+    }
+}
+
+/* Upload-time preparation, double precision on the host:
+ * P = (x, y, z, 1/m), V, Q, L = R I R^T w, I^-1 body frame as i0/i1. */
+void or_clump_prepare(int nb, const float* pos4, const float* vel4, const float* quat4, const float* angvel4,
+                      const float* inertia6, float* P, float* V, float* Q, float* L, float* I0, float* I1) {
+    for (int b = 0; b < nb; ++b) {
+        const float m = pos4[4 * (long)b + 3];
+        P[4 * (long)b] = pos4[4 * (long)b];
+        P[4 * (long)b + 1] = pos4[4 * (long)b + 1];
+        P[4 * (long)b + 2] = pos4[4 * (long)b + 2];
+        P[4 * (long)b + 3] = m > 0.0f ? 1.0f / m : 0.0f;
+        for (int a = 0; a < 3; ++a) V[4 * (long)b + a] = m > 0.0f ? vel4[4 * (long)b + a] : 0.0f;
+        V[4 * (long)b + 3] = 0.0f;
+        for (int a = 0; a < 4; ++a) Q[4 * (long)b + a] = quat4[4 * (long)b + a];
+        const float* I = inertia6 + 6 * (long)b;
+        const double xx = I[0], yy = I[1], zz = I[2], xy = I[3], xz = I[4], yz = I[5];
+        const double c00 = yy * zz - yz * yz, c01 = xz * yz - xy * zz, c02 = xy * yz - xz * yy;
+        const double det = xx * c00 + xy * c01 + xz * c02;
+        double inv[6] = {0, 0, 0, 0, 0, 0};
+        if (m > 0.0f && det > 0.0) {
+            inv[0] = c00 / det;
+            inv[1] = (xx * zz - xz * xz) / det;
+            inv[2] = (xx * yy - xy * xy) / det;
+            inv[3] = c01 / det;
+            inv[4] = c02 / det;
+            inv[5] = (xz * xy - xx * yz) / det;
+        }
+        I0[4 * (long)b] = (float)inv[0];
+        I0[4 * (long)b + 1] = (float)inv[1];
+        I0[4 * (long)b + 2] = (float)inv[2];
+        I0[4 * (long)b + 3] = (float)inv[3];
+        I1[4 * (long)b] = (float)inv[4];
+        I1[4 * (long)b + 1] = (float)inv[5];
+        I1[4 * (long)b + 2] = 0.0f;
+        I1[4 * (long)b + 3] = 0.0f;
+        const float* q = quat4 + 4 * (long)b;
+        const float* w = angvel4 + 4 * (long)b;
+        const double qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+        const double R[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy),
+                             2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx),
+                             2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)};
+        const double Ib[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
+        double u[3], t[3], Lw[3];
+        for (int r = 0; r < 3; ++r) u[r] = R[0 + r] * w[0] + R[3 + r] * w[1] + R[6 + r] * w[2];
+        for (int r = 0; r < 3; ++r) t[r] = Ib[3 * r] * u[0] + Ib[3 * r + 1] * u[1] + Ib[3 * r + 2] * u[2];
+        for (int r = 0; r < 3; ++r) Lw[r] = R[3 * r] * t[0] + R[3 * r + 1] * t[1] + R[3 * r + 2] * t[2];
+        for (int a = 0; a < 3; ++a) L[4 * (long)b + a] = m > 0.0f ? (float)Lw[a] : 0.0f;
+        L[4 * (long)b + 3] = 0.0f;
+    }
+}
+
+/* Sphere world positions / velocities and body AABBs of the current state. */
+void or_clump_pose(int nb, const float* P, const float* V, const float* Q, const float* L, const float* I0,
+                   const float* I1, const int* first, const int* count, const float* local4, float* sx, float* sv,
+                   float* lo4, float* hi4) {
+    for (int b = 0; b < nb; ++b) {
+        float m[9];
+        qmat(Q + 4 * (long)b, m);
+        const V3 w = inv_inertia(m, I0 + 4 * (long)b, I1 + 4 * (long)b,
+                                 mk(L[4 * (long)b], L[4 * (long)b + 1], L[4 * (long)b + 2]));
+        float lo[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+        float hi[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+        for (int s = first[b]; s < first[b] + count[b]; ++s) {
+            const float* l = local4 + 4 * (long)s;
+            const V3 r = mv(m, mk(l[0], l[1], l[2]));
+            float* x = sx + 4 * (long)s;
+            float* v = sv + 4 * (long)s;
+            x[0] = P[4 * (long)b] + r.x;
+            x[1] = P[4 * (long)b + 1] + r.y;
+            x[2] = P[4 * (long)b + 2] + r.z;
+            x[3] = l[3];
+            v[0] = V[4 * (long)b] + (w.y * r.z - w.z * r.y);
+            v[1] = V[4 * (long)b + 1] + (w.z * r.x - w.x * r.z);
+            v[2] = V[4 * (long)b + 2] + (w.x * r.y - w.y * r.x);
+            v[3] = 0.0f;
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fminf(lo[a], x[a] - x[3]);
+                hi[a] = fmaxf(hi[a], x[a] + x[3]);
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            lo4[4 * (long)b + a] = lo[a];
+            hi4[4 * (long)b + a] = hi[a];
+        }
+        lo4[4 * (long)b + 3] = 0.0f;
+        hi4[4 * (long)b + 3] = 0.0f;
+    }
+}
+
+void or_global_domain(int n, const float* lo4, const float* hi4, float* dom6) {
+    for (int a = 0; a < 3; ++a) {
+        dom6[a] = 3.402823466e38f;
+        dom6[3 + a] = -3.402823466e38f;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            dom6[a] = fminf(dom6[a], lo4[4 * (long)i + a]);
+            dom6[3 + a] = fmaxf(dom6[3 + a], hi4[4 * (long)i + a]);
+        }
+}
+
+static uint32_t spread10(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0xFF0000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+static uint32_t maxis(float c, float lo, float hi) {
+    const float ext = hi - lo;
+    float f = ext > 0.0f ? floorf(((c - lo) / ext) * 1024.0f) : 0.0f;
+    f = fmaxf(f, 0.0f);
+    f = fminf(f, 1023.0f);
+    return (uint32_t)f;
+}
+
+void or_morton30(int n, const float* lo4, const float* hi4, const float* dom6, uint64_t* keys) {
+    for (int i = 0; i < n; ++i) {
+        const float* l = lo4 + 4 * (long)i;
+        const float* h = hi4 + 4 * (long)i;
+        const float cx = 0.5f * (l[0] + h[0]), cy = 0.5f * (l[1] + h[1]), cz = 0.5f * (l[2] + h[2]);
+        const uint32_t code = spread10(maxis(cx, dom6[0], dom6[3])) | (spread10(maxis(cy, dom6[1], dom6[4])) << 1) |
+                              (spread10(maxis(cz, dom6[2], dom6[5])) << 2);
+        keys[i] = ((uint64_t)code << 32) | (uint64_t)(uint32_t)i;
+    }
+}
+
+static int cmp_u64(const void* a, const void* b) {
+    const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : (x > y);
+}
+
+/* Contacts + integration of every body given the unique pair set (a, b).
+ * Adjacency: (body, partner) ascending; per contact the sphere law with no
+ * history; torque about the centre of mass. State arrays updated in place. */
+int or_clump_contacts_integrate(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1,
+                                const int* first, const int* count, const float* sx, const float* sv, long npairs,
+                                const int* pairs, const or_contact* c, float* force4, float* torque4, int integrate) {
+    const float kt = c->kt_ratio * c->kn, c2z = 2.0f * c->damping_ratio;
+    uint64_t* adj = (uint64_t*)malloc((size_t)(2 * npairs + 1) * sizeof(uint64_t));
+    for (long p = 0; p < npairs; ++p) {
+        const uint64_t a = (uint64_t)pairs[2 * p], b = (uint64_t)pairs[2 * p + 1];
+        adj[2 * p] = a * (uint64_t)nb + b;
+        adj[2 * p + 1] = b * (uint64_t)nb + a;
+    }
+    qsort(adj, (size_t)(2 * npairs), sizeof(uint64_t), cmp_u64);
+    long e = 0;
+    float* np_ = (float*)malloc((size_t)nb * 16 * sizeof(float));
+    /* omega of every body at the start of the step */
+    float* om = (float*)malloc((size_t)nb * 3 * sizeof(float));
+    for (int b = 0; b < nb; ++b) {
+        float m[9];
+        qmat(Q + 4 * (long)b, m);
+        const V3 w = inv_inertia(m, I0 + 4 * (long)b, I1 + 4 * (long)b,
+                                 mk(L[4 * (long)b], L[4 * (long)b + 1], L[4 * (long)b + 2]));
+        om[3 * b] = w.x;
+        om[3 * b + 1] = w.y;
+        om[3 * b + 2] = w.z;
+    }
+    for (int b = 0; b < nb; ++b) {
+        const float* X = P + 4 * (long)b;
+        float F[3] = {0.0f, 0.0f, 0.0f}, T[3] = {0.0f, 0.0f, 0.0f};
+        while (e < 2 * npairs && (int)(adj[e] / (uint64_t)nb) == b) {
+            const int pb = (int)(adj[e] % (uint64_t)nb);
+            ++e;
+            const float inv_mp = P[4 * (long)pb + 3];
+            if (X[3] + inv_mp == 0.0f) continue;
+            for (int i = first[b]; i < first[b] + count[b]; ++i) {
+                const float* xi = sx + 4 * (long)i;
+                const float* vi = sv + 4 * (long)i;
+                for (int j = first[pb]; j < first[pb] + count[pb]; ++j) {
+                    const float* xj = sx + 4 * (long)j;
+                    const float dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    const float rs = xi[3] + xj[3];
+                    if (!(d2 < rs * rs) || d2 == 0.0f) continue;
+                    const float* vj = sv + 4 * (long)j;
+                    const float d = sqrtf(d2);
+                    const float inv_d = 1.0f / d;
+                    const float nx = dx * inv_d, ny = dy * inv_d, nz = dz * inv_d;
+                    const float delta = rs - d;
+                    /* contact law (sd_oracle.c order), zero tangential history */
+                    const float m_eff = 1.0f / (X[3] + inv_mp);
+                    const float cn = c2z * sqrtf(c->kn * m_eff);
+                    const float ct = c2z * sqrtf(kt * m_eff);
+                    const float li = xi[3] - 0.5f * delta, lj = xj[3] - 0.5f * delta;
+                    const float ax = li * om[3 * b] + lj * om[3 * pb];
+                    const float ay = li * om[3 * b + 1] + lj * om[3 * pb + 1];
+                    const float az = li * om[3 * b + 2] + lj * om[3 * pb + 2];
+                    const float vcx = (vi[0] - vj[0]) - (ay * nz - az * ny);
+                    const float vcy = (vi[1] - vj[1]) - (az * nx - ax * nz);
+                    const float vcz = (vi[2] - vj[2]) - (ax * ny - ay * nx);
+                    const float vn = vcx * nx + vcy * ny + vcz * nz;
+                    const float fn = c->kn * delta - cn * vn;
+                    const float vtx = vcx - vn * nx, vty = vcy - vn * ny, vtz = vcz - vn * nz;
+                    float hx = 0.0f, hy = 0.0f, hz = 0.0f;
+                    const float hn = hx * nx + hy * ny + hz * nz;
+                    hx = hx - hn * nx;
+                    hy = hy - hn * ny;
+                    hz = hz - hn * nz;
+                    hx = hx + vtx * c->dt;
+                    hy = hy + vty * c->dt;
+                    hz = hz + vtz * c->dt;
+                    float ftx = -(kt * hx) - ct * vtx, fty = -(kt * hy) - ct * vty, ftz = -(kt * hz) - ct * vtz;
+                    const float ft2 = ftx * ftx + fty * fty + ftz * ftz;
+                    const float fmax = c->mu * fabsf(fn);
+                    if (ft2 > fmax * fmax) {
+                        const float sc = fmax / sqrtf(ft2);
+                        ftx = ftx * sc;
+                        fty = fty * sc;
+                        ftz = ftz * sc;
+                    }
+                    const float fx = fn * nx + ftx, fy = fn * ny + fty, fz = fn * nz + ftz;
+                    const float tx = -li * (ny * ftz - nz * fty);
+                    const float ty = -li * (nz * ftx - nx * ftz);
+                    const float tz = -li * (nx * fty - ny * ftx);
+                    F[0] = F[0] + fx;
+                    F[1] = F[1] + fy;
+                    F[2] = F[2] + fz;
+                    const float rx = xi[0] - X[0], ry = xi[1] - X[1], rz = xi[2] - X[2];
+                    T[0] = T[0] + ((ry * fz - rz * fy) + tx);
+                    T[1] = T[1] + ((rz * fx - rx * fz) + ty);
+                    T[2] = T[2] + ((rx * fy - ry * fx) + tz);
+                }
+            }
+        }
+        if (force4) {
+            for (int a = 0; a < 3; ++a) {
+                force4[4 * (long)b + a] = F[a];
+                torque4[4 * (long)b + a] = T[a];
+            }
+            force4[4 * (long)b + 3] = 0.0f;
+            torque4[4 * (long)b + 3] = 0.0f;
+        }
+        float* o = np_ + 16 * (long)b;
+        memcpy(o, P + 4 * (long)b, 16);
+        memcpy(o + 4, V + 4 * (long)b, 16);
+        memcpy(o + 8, Q + 4 * (long)b, 16);
+        memcpy(o + 12, L + 4 * (long)b, 16);
+        if (integrate && X[3] > 0.0f) {
+            float* Pn = o;
+            float* Vn = o + 4;
+            float* q = o + 8;
+            float* Ln = o + 12;
+            Vn[0] = Vn[0] + (F[0] * X[3] + c->gravity[0]) * c->dt;
+            Vn[1] = Vn[1] + (F[1] * X[3] + c->gravity[1]) * c->dt;
+            Vn[2] = Vn[2] + (F[2] * X[3] + c->gravity[2]) * c->dt;
+            Pn[0] = Pn[0] + Vn[0] * c->dt;
+            Pn[1] = Pn[1] + Vn[1] * c->dt;
+            Pn[2] = Pn[2] + Vn[2] * c->dt;
+            Ln[0] = Ln[0] + T[0] * c->dt;
+            Ln[1] = Ln[1] + T[1] * c->dt;
+            Ln[2] = Ln[2] + T[2] * c->dt;
+            float m[9];
+            qmat(q, m);
+            const V3 w = inv_inertia(m, I0 + 4 * (long)b, I1 + 4 * (long)b, mk(Ln[0], Ln[1], Ln[2]));
+            const float wn = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+            if (wn > 0.0f) {
+                float sa, ca;
+                sincos_small(wn * (0.5f * c->dt), &sa, &ca);
+                const float k = sa / wn;
+                const float dw = ca, dx = w.x * k, dy = w.y * k, dz = w.z * k;
+                const float nw = dw * q[0] - (dx * q[1] + dy * q[2] + dz * q[3]);
+                const float nx = dw * q[1] + q[0] * dx + (dy * q[3] - dz * q[2]);
+                const float ny = dw * q[2] + q[0] * dy + (dz * q[1] - dx * q[3]);
+                const float nz = dw * q[3] + q[0] * dz + (dx * q[2] - dy * q[1]);
+                const float inv = 1.0f / sqrtf(nw * nw + nx * nx + ny * ny + nz * nz);
+                q[0] = nw * inv;
+                q[1] = nx * inv;
+                q[2] = ny * inv;
+                q[3] = nz * inv;
+            }
+        }
+    }
+    if (integrate)
+        for (int b = 0; b < nb; ++b) {
+            memcpy(P + 4 * (long)b, np_ + 16 * (long)b, 16);
+            memcpy(V + 4 * (long)b, np_ + 16 * (long)b + 4, 16);
+            memcpy(Q + 4 * (long)b, np_ + 16 * (long)b + 8, 16);
+            memcpy(L + 4 * (long)b, np_ + 16 * (long)b + 12, 16);
+        }
+    free(adj);
+    free(np_);
+    free(om);
+    return 0;
+}
+
+/* One full clump step (pose, AABBs, SAP pair set, contacts, integration).
+ * Returns the number of unique body pairs. */
+long or_clump_step(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const int* first,
+                   const int* count, const float* local4, int ns, const or_contact* c) {
+    float* sx = (float*)malloc((size_t)ns * 16);
+    float* sv = (float*)malloc((size_t)ns * 16);
+    float* lo = (float*)malloc((size_t)nb * 16);
+    float* hi = (float*)malloc((size_t)nb * 16);
+    or_clump_pose(nb, P, V, Q, L, I0, I1, first, count, local4, sx, sv, lo, hi);
+    const int axis = or_sap_axis(nb, lo, hi);
+    const long np = or_sap_pairs(nb, lo, hi, axis, NULL, 0);
+    int* pairs = (int*)malloc((size_t)(2 * np + 2) * sizeof(int));
+    or_sap_pairs(nb, lo, hi, axis, pairs, np);
+    or_clump_contacts_integrate(nb, P, V, Q, L, I0, I1, first, count, sx, sv, np, pairs, c, NULL, NULL, 1);
+    free(sx);
+    free(sv);
+    free(lo);
+    free(hi);
+    free(pairs);
+    return np;
+}

@@ -125,6 +125,21 @@ _SIGS = {
     "sd_sap_scratch_bytes": [_I, C.POINTER(_SZ)],
     "sd_sap_pairs": [_P, _P, _I, _P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), _P, _SZ, _P],
     "sd_gjk": [_P, _I, _P, _P, _P, _P, _P, _I, _I, _P, _I, _I, _F, _P, _P, _P, _P, _P],
+    "sd_global_domain": [_P, _P, _I, _P, _P, _P],
+    "sd_morton30": [_P, _P, _I, _P, _P, _P],
+    "sd_lbvh_build": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "sd_lbvh_query_scratch_bytes": [_I, C.POINTER(_SZ)],
+    "sd_lbvh_query": [_P, _I, _P, _P, _P, _P, _P, _P, _I, C.POINTER(C.c_int), _P, _P, _SZ, _P],
+    "sd_clump_create": [_P, C.POINTER(C.c_void_p)],
+    "sd_clump_destroy": [_P],
+    "sd_clump_upload": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P],
+    "sd_clump_step": [_P, _I],
+    "sd_clump_profile": [_P, _I, _P, _I],
+    "sd_clump_download": [_P, _P, _P, _P, _P],
+    "sd_clump_pairs": [_P, _P, _I, C.POINTER(C.c_int)],
+    "sd_clump_spheres": [_P, _P],
+    "sd_gen_clumps": [C.c_uint64, _I, _I, _F, _F, _I, _I, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P,
+                      C.POINTER(C.c_int)],
     "sd_mlp_param_count": [],
     "sd_mlp_packed_bytes": [],
     "sd_hull_descriptors": [_P, _P, _P, _P, _I, _P, _P],

@@ -1,0 +1,633 @@
+// Rigid clumps of spheres: pose, body AABBs, LBVH broadphase, deterministic
+// per-body contact gather and semi-implicit quaternion integration.
+//
+// Replaces updateSphereDataCUDA (src/engine/cuda/physics_update.cu:26-140),
+// computeForcesAndTorquesCUDA (:144-228) and advectParticlesCUDA (:232-348)
+// with their helpers (src/engine/cuda/cuda_helpers.cuh:65-171). Differences by
+// design: the world inverse inertia is R I^-1 R^T (quirk Q11: the reference mixes
+// row- and column-major helpers), a singular inertia is not replaced by its input
+// (Q10), the quaternion is renormalised every step (Q12), static bodies carry an
+// explicit 1/m = 0 instead of mass >= 10 (Q13), and each body gathers the forces
+// of its contacts instead of racing scatters (Q7).
+//
+// Per step: body prep (world angular velocity), sphere pose, body AABBs, global
+// domain, Morton keys, radix sort, LBVH build + query (unique body pairs),
+// canonical both-way adjacency (sorted by (body, partner)), then one kernel per
+// body sums its sphere-sphere contacts (partners ascending, own spheres
+// ascending, partner spheres ascending) and integrates it.
+//
+// Integration of a body with 1/m > 0 (FP32, fixed operation order):
+//   v += (F/m + g) dt;  x += v dt;  L += T dt
+//   w  = R I^-1 R^T L (R from the current q);  a = |w| (dt/2)
+//   dq = (cos a, w sin(a)/|w|), q = normalise(dq q)
+// sin and cos come from fixed Taylor polynomials for a < 0.25 (exact to FP32)
+// so the CPU restatement reproduces the bits; larger angles use sinf/cosf.
+#include <vector>
+
+#include "sd_contact_law.hpp"
+#include "sd_gjk_dev.hpp"
+#include "sd_internal.hpp"
+
+extern "C" sd_status sd_global_domain(const float* lo4, const float* hi4, int n, float* domain6, void* scratch,
+                                      void* stream);
+extern "C" sd_status sd_morton30(const float* lo4, const float* hi4, int n, const float* domain6, uint64_t* keys,
+                                 void* stream);
+extern "C" sd_status sd_lbvh_build(const uint64_t* sorted_keys, int n, const float* lo4, const float* hi4, int* left,
+                                   int* right, int* parent, int* first, int* last, int* rope, float* node_lo4,
+                                   float* node_hi4, int* counters, int* error, void* stream);
+extern "C" sd_status sd_lbvh_query_scratch_bytes(int n, size_t* bytes);
+extern "C" sd_status sd_lbvh_query(const uint64_t* sorted_keys, int n, const int* left, const int* last,
+                                   const int* rope, const float* node_lo4, const float* node_hi4, int* pairs,
+                                   int capacity, int* n_pairs, int* error, void* scratch, size_t scratch_bytes,
+                                   void* stream);
+
+namespace sd {
+
+// sin(a), cos(a) for 0 <= a: Taylor polynomials in Horner form below 0.25.
+__device__ __forceinline__ void sincos_small(float a, float& s, float& c) {
+    if (a < 0.25f) {
+        const float a2 = a * a;
+        s = a * (1.0f - a2 / 6.0f * (1.0f - a2 / 20.0f * (1.0f - a2 / 42.0f * (1.0f - a2 / 72.0f))));
+        c = 1.0f - a2 / 2.0f * (1.0f - a2 / 12.0f * (1.0f - a2 / 30.0f * (1.0f - a2 / 56.0f)));
+    } else {
+        s = sinf(a);
+        c = cosf(a);
+    }
+}
+
+// World inverse inertia R I^-1 R^T applied to L (I^-1 symmetric, body frame:
+// i0 = (xx, yy, zz, xy), i1 = (xz, yz, -, -)).
+__device__ __forceinline__ V3 inv_inertia_apply(const Rot& R, float4 i0, float4 i1, V3 L) {
+    // u = R^T L, then I^-1 u, then R (.)
+    const V3 u = rot_tmul(R, L);
+    const V3 w = v3(i0.x * u.x + i0.w * u.y + i1.x * u.z, i0.w * u.x + i0.y * u.y + i1.y * u.z,
+                    i1.x * u.x + i1.y * u.y + i0.z * u.z);
+    return rot_mul(R, w);
+}
+
+struct ClumpState {
+    const float4* pos;   // (x, y, z, 1/m)
+    const float4* vel;   // (vx, vy, vz, 0)
+    const float4* quat;  // (w, x, y, z)
+    const float4* angm;  // (Lx, Ly, Lz, 0)
+};
+
+__global__ void k_body_prep(ClumpState st, const float4* __restrict__ i0, const float4* __restrict__ i1, int nb,
+                            float4* __restrict__ omega) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const Rot R = quat_rot(st.quat[b]);
+    const float4 L = st.angm[b];
+    const V3 w = inv_inertia_apply(R, i0[b], i1[b], v3(L.x, L.y, L.z));
+    omega[b] = make_float4(w.x, w.y, w.z, 0.0f);
+}
+
+__global__ void k_clump_pose(ClumpState st, const float4* __restrict__ omega, const float4* __restrict__ slocal,
+                             const int* __restrict__ sbody, int ns, float4* __restrict__ sx,
+                             float4* __restrict__ sv) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int b = sbody[s];
+    const Rot R = quat_rot(st.quat[b]);
+    const float4 l = slocal[s];
+    const V3 r = rot_mul(R, v3(l.x, l.y, l.z));
+    const float4 X = st.pos[b], V = st.vel[b], w = omega[b];
+    sx[s] = make_float4(X.x + r.x, X.y + r.y, X.z + r.z, l.w);
+    sv[s] = make_float4(V.x + (w.y * r.z - w.z * r.y), V.y + (w.z * r.x - w.x * r.z), V.z + (w.x * r.y - w.y * r.x),
+                        0.0f);
+}
+
+__global__ void k_body_aabb(const float4* __restrict__ sx, const int* __restrict__ bfirst,
+                            const int* __restrict__ bcount, int nb, float4* __restrict__ lo4,
+                            float4* __restrict__ hi4) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    float lx = 3.402823466e38f, ly = 3.402823466e38f, lz = 3.402823466e38f;
+    float hx = -3.402823466e38f, hy = -3.402823466e38f, hz = -3.402823466e38f;
+    const int k0 = bfirst[b], kn = bcount[b];
+    for (int k = 0; k < kn; ++k) {
+        const float4 p = sx[k0 + k];
+        lx = fminf(lx, p.x - p.w);
+        ly = fminf(ly, p.y - p.w);
+        lz = fminf(lz, p.z - p.w);
+        hx = fmaxf(hx, p.x + p.w);
+        hy = fmaxf(hy, p.y + p.w);
+        hz = fmaxf(hz, p.z + p.w);
+    }
+    lo4[b] = make_float4(lx, ly, lz, 0.0f);
+    hi4[b] = make_float4(hx, hy, hz, 0.0f);
+}
+
+// Both directions of every pair as keys body * nb + partner; padding keys are ~0.
+__global__ void k_adj_keys(const int* __restrict__ pairs, int npairs, int cap2, int nb, uint64_t* __restrict__ keys) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cap2) return;
+    uint64_t k = ~0ull;
+    if (e < 2 * npairs) {
+        const int p = e >> 1;
+        const int a = pairs[2 * (size_t)p], b = pairs[2 * (size_t)p + 1];
+        k = (e & 1) ? (uint64_t)b * (uint64_t)nb + (uint64_t)a : (uint64_t)a * (uint64_t)nb + (uint64_t)b;
+    }
+    keys[e] = k;
+}
+
+// ranges[2b], ranges[2b+1]: the adjacency entries of body b (empty: 0, 0).
+__global__ void k_adj_ranges(const uint64_t* __restrict__ keys, int n, int nb, int* __restrict__ ranges) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t k = keys[e];
+    const int b = (int)(k / (uint64_t)nb);
+    if (e == 0 || (int)(keys[e - 1] / (uint64_t)nb) != b) ranges[2 * (size_t)b] = e;
+    if (e == n - 1 || (int)(keys[e + 1] / (uint64_t)nb) != b) ranges[2 * (size_t)b + 1] = e + 1;
+}
+
+template <bool INTEGRATE>
+__global__ __launch_bounds__(128) void k_clump_contacts(ClumpState st, ClumpState out, const float4* __restrict__ omega,
+                                                        const float4* __restrict__ i0, const float4* __restrict__ i1,
+                                                        const float4* __restrict__ sx, const float4* __restrict__ sv,
+                                                        const int* __restrict__ bfirst, const int* __restrict__ bcount,
+                                                        int nb, const uint64_t* __restrict__ adj,
+                                                        const int* __restrict__ ranges, ContactDev c,
+                                                        float4* __restrict__ force, float4* __restrict__ torque) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const float4 X = st.pos[b];
+    const float4 wb4 = omega[b];
+    const int f0 = bfirst[b], fn = bcount[b];
+    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
+    const int e0 = ranges[2 * (size_t)b], e1 = ranges[2 * (size_t)b + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int p = (int)(adj[e] % (uint64_t)nb);
+        const float inv_mp = st.pos[p].w;
+        if (X.w + inv_mp == 0.0f) continue;  // two static bodies
+        const float4 wp4 = omega[p];
+        const int g0 = bfirst[p], gn = bcount[p];
+        for (int i = f0; i < f0 + fn; ++i) {
+            const float4 xi = sx[i], vi = sv[i];
+            for (int j = g0; j < g0 + gn; ++j) {
+                const float4 xj = sx[j];
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                const float rs = xi.w + xj.w;
+                if (!(d2 < rs * rs) || d2 == 0.0f) continue;
+                const float4 vj = sv[j];
+                const float d = sqrtf(d2);
+                const float inv_d = 1.0f / d;
+                float hx = 0.0f, hy = 0.0f, hz = 0.0f;  // no tangential history for clumps
+                const PairForce f = contact_law(c, dx * inv_d, dy * inv_d, dz * inv_d, rs - d, xi.w, xj.w, X.w, inv_mp,
+                                                vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, wb4.x, wb4.y, wb4.z, wp4.x,
+                                                wp4.y, wp4.z, hx, hy, hz);
+                Fx = Fx + f.fx;
+                Fy = Fy + f.fy;
+                Fz = Fz + f.fz;
+                const float ax = xi.x - X.x, ay = xi.y - X.y, az = xi.z - X.z;
+                Tx = Tx + ((ay * f.fz - az * f.fy) + f.tx);
+                Ty = Ty + ((az * f.fx - ax * f.fz) + f.ty);
+                Tz = Tz + ((ax * f.fy - ay * f.fx) + f.tz);
+            }
+        }
+    }
+    if (!INTEGRATE) {
+        force[b] = make_float4(Fx, Fy, Fz, 0.0f);
+        torque[b] = make_float4(Tx, Ty, Tz, 0.0f);
+        return;
+    }
+    float4 V = st.vel[b], q = st.quat[b], L = st.angm[b];
+    float4 P = X;
+    if (X.w > 0.0f) {
+        V.x = V.x + (Fx * X.w + c.gx) * c.dt;
+        V.y = V.y + (Fy * X.w + c.gy) * c.dt;
+        V.z = V.z + (Fz * X.w + c.gz) * c.dt;
+        P.x = P.x + V.x * c.dt;
+        P.y = P.y + V.y * c.dt;
+        P.z = P.z + V.z * c.dt;
+        L.x = L.x + Tx * c.dt;
+        L.y = L.y + Ty * c.dt;
+        L.z = L.z + Tz * c.dt;
+        const Rot R = quat_rot(q);
+        const V3 w = inv_inertia_apply(R, i0[b], i1[b], v3(L.x, L.y, L.z));
+        const float wn = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+        if (wn > 0.0f) {
+            float sa, ca;
+            sincos_small(wn * (0.5f * c.dt), sa, ca);
+            const float k = sa / wn;
+            const float dw = ca, dx = w.x * k, dy = w.y * k, dz = w.z * k;
+            // dq * q
+            const float nw = dw * q.x - (dx * q.y + dy * q.z + dz * q.w);
+            const float nx = dw * q.y + q.x * dx + (dy * q.w - dz * q.z);
+            const float ny = dw * q.z + q.x * dy + (dz * q.y - dx * q.w);
+            const float nz = dw * q.w + q.x * dz + (dx * q.z - dy * q.y);
+            const float inv = 1.0f / sqrtf(nw * nw + nx * nx + ny * ny + nz * nz);
+            q = make_float4(nw * inv, nx * inv, ny * inv, nz * inv);
+        }
+    }
+    ((float4*)out.pos)[b] = P;
+    ((float4*)out.vel)[b] = V;
+    ((float4*)out.quat)[b] = q;
+    ((float4*)out.angm)[b] = L;
+}
+
+}  // namespace sd
+
+using namespace sd;
+
+// ------------------------------------------------------------------ clump scene context
+
+struct sd_clump_ctx {
+    sd_clump_params p{};
+    hipStream_t stream = nullptr;
+    int nb = 0, ns = 0;
+    float4 *pos[2] = {}, *vel[2] = {}, *quat[2] = {}, *angm[2] = {};
+    int cur = 0;
+    float4 *i0 = nullptr, *i1 = nullptr, *omega = nullptr;
+    int *bfirst = nullptr, *bcount = nullptr, *sbody = nullptr;
+    float4 *slocal = nullptr, *sx = nullptr, *sv = nullptr;
+    float4 *blo = nullptr, *bhi = nullptr;
+    float* domain = nullptr;
+    float* dom_scratch = nullptr;
+    uint64_t *mkeys = nullptr, *mkeys2 = nullptr;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int *left = nullptr, *right = nullptr, *parent = nullptr, *first = nullptr, *last = nullptr, *rope = nullptr;
+    float4 *nlo = nullptr, *nhi = nullptr;
+    int *counters = nullptr, *error = nullptr;
+    void* q_scratch = nullptr;
+    size_t q_scratch_bytes = 0;
+    int* pairs = nullptr;
+    int pair_cap = 0;
+    int npairs = 0;
+    uint64_t *adj = nullptr, *adj2 = nullptr;
+    void* adj_tmp = nullptr;
+    size_t adj_tmp_bytes = 0;
+    int* ranges = nullptr;
+    float4 *force = nullptr, *torque = nullptr;
+    long long steps_done = 0;
+    long long pair_total = 0;
+};
+
+namespace {
+
+template <typename T>
+sd_status dalloc(T** p, size_t count) {
+    SD_HIP(hipMalloc((void**)p, (count ? count : 1) * sizeof(T)));
+    return SD_OK;
+}
+
+void cfree(sd_clump_ctx* c) {
+    void* bufs[] = {c->pos[0],  c->pos[1],    c->vel[0],     c->vel[1],   c->quat[0], c->quat[1],  c->angm[0],
+                    c->angm[1], c->i0,        c->i1,         c->omega,    c->bfirst,  c->bcount,   c->sbody,
+                    c->slocal,  c->sx,        c->sv,         c->blo,      c->bhi,     c->domain,   c->dom_scratch,
+                    c->mkeys,   c->mkeys2,    c->sort_tmp,   c->left,     c->right,   c->parent,   c->first,
+                    c->last,    c->rope,      c->nlo,        c->nhi,      c->counters, c->error,   c->q_scratch,
+                    c->pairs,   c->adj,       c->adj2,       c->adj_tmp,  c->ranges,  c->force,    c->torque};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    sd_clump_params keep = c->p;
+    hipStream_t s = c->stream;
+    *c = sd_clump_ctx();
+    c->p = keep;
+    c->stream = s;
+}
+
+sd_status grow_pairs(sd_clump_ctx* c, int need) {
+    if (need <= c->pair_cap) return SD_OK;
+    int cap = need + need / 2 + 1024;
+    if (c->pairs) (void)hipFree(c->pairs);
+    if (c->adj) (void)hipFree(c->adj);
+    if (c->adj2) (void)hipFree(c->adj2);
+    if (c->adj_tmp) (void)hipFree(c->adj_tmp);
+    c->pairs = nullptr;
+    c->adj = c->adj2 = nullptr;
+    c->adj_tmp = nullptr;
+    sd_status st = dalloc(&c->pairs, 2 * (size_t)cap);
+    if (st == SD_OK) st = dalloc(&c->adj, 2 * (size_t)cap);
+    if (st == SD_OK) st = dalloc(&c->adj2, 2 * (size_t)cap);
+    c->adj_tmp_bytes = radix_temp_bytes(2 * cap);
+    if (st == SD_OK) st = dalloc((char**)&c->adj_tmp, c->adj_tmp_bytes);
+    if (st != SD_OK) return st;
+    c->pair_cap = cap;
+    return SD_OK;
+}
+
+int bits_for(uint64_t v) {
+    int b = 1;
+    while (b < 64 && (1ull << b) < v) ++b;
+    return b;
+}
+
+ClumpState state_of(sd_clump_ctx* c, int k) { return ClumpState{c->pos[k], c->vel[k], c->quat[k], c->angm[k]}; }
+
+// One step. ev (optional, 6 events): prep+pose+aabb | domain+morton+sort | lbvh build |
+// query | adjacency | contacts+integrate.
+sd_status clump_step(sd_clump_ctx* c, hipEvent_t* ev) {
+    hipStream_t s = c->stream;
+    const int nb = c->nb, ns = c->ns;
+    const ClumpState cs = state_of(c, c->cur), nx = state_of(c, 1 - c->cur);
+    const ContactDev cd = to_dev(c->p.contact);
+#define MARK(i) \
+    if (ev) SD_HIP(hipEventRecord(ev[i], s))
+    MARK(0);
+    hipLaunchKernelGGL(k_body_prep, dim3(div_up(nb, 256)), dim3(256), 0, s, cs, c->i0, c->i1, nb, c->omega);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_clump_pose, dim3(div_up(ns, 256)), dim3(256), 0, s, cs, c->omega, c->slocal, c->sbody, ns,
+                       c->sx, c->sv);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_body_aabb, dim3(div_up(nb, 256)), dim3(256), 0, s, c->sx, c->bfirst, c->bcount, nb, c->blo,
+                       c->bhi);
+    SD_LAUNCH_CHECK();
+    MARK(1);
+    sd_status st = sd_global_domain((const float*)c->blo, (const float*)c->bhi, nb, c->domain, c->dom_scratch, s);
+    if (st != SD_OK) return st;
+    st = sd_morton30((const float*)c->blo, (const float*)c->bhi, nb, c->domain, c->mkeys, s);
+    if (st != SD_OK) return st;
+    uint64_t *K = c->mkeys, *Ka = c->mkeys2;
+    st = radix_sort_keys_u64(&K, &Ka, nb, 32, 62, c->sort_tmp, c->sort_tmp_bytes, s, 0);
+    if (st != SD_OK) return st;
+    MARK(2);
+    st = launch_fill_u32((uint32_t*)c->error, 0u, 1, s);
+    if (st != SD_OK) return st;
+    st = sd_lbvh_build(K, nb, (const float*)c->blo, (const float*)c->bhi, c->left, c->right, c->parent, c->first,
+                       c->last, c->rope, (float*)c->nlo, (float*)c->nhi, c->counters, c->error, s);
+    if (st != SD_OK) return st;
+    MARK(3);
+    int npairs = 0;
+    st = sd_lbvh_query(K, nb, c->left, c->last, c->rope, (const float*)c->nlo, (const float*)c->nhi, c->pairs,
+                       c->pair_cap, &npairs, c->error, c->q_scratch, c->q_scratch_bytes, s);
+    if (st == SD_ERR_CAPACITY) {
+        st = grow_pairs(c, npairs);
+        if (st != SD_OK) return st;
+        st = sd_lbvh_query(K, nb, c->left, c->last, c->rope, (const float*)c->nlo, (const float*)c->nhi, c->pairs,
+                           c->pair_cap, &npairs, c->error, c->q_scratch, c->q_scratch_bytes, s);
+    }
+    if (st != SD_OK) return st;
+    c->npairs = npairs;
+    c->pair_total += npairs;
+    MARK(4);
+    // canonical adjacency: (body, partner) ascending
+    const int n2 = 2 * npairs;
+    st = launch_fill_u32((uint32_t*)c->ranges, 0u, 2 * (size_t)nb, s);
+    if (st != SD_OK) return st;
+    if (n2 > 0) {
+        hipLaunchKernelGGL(k_adj_keys, dim3(div_up(n2, 256)), dim3(256), 0, s, c->pairs, npairs, n2, nb, c->adj);
+        SD_LAUNCH_CHECK();
+        uint64_t *A = c->adj, *Aa = c->adj2;
+        st = radix_sort_keys_u64(&A, &Aa, n2, 0, bits_for((uint64_t)nb * (uint64_t)nb), c->adj_tmp, c->adj_tmp_bytes,
+                                 s, 0);
+        if (st != SD_OK) return st;
+        if (A != c->adj) {
+            c->adj2 = c->adj;
+            c->adj = A;
+        }
+        hipLaunchKernelGGL(k_adj_ranges, dim3(div_up(n2, 256)), dim3(256), 0, s, c->adj, n2, nb, c->ranges);
+        SD_LAUNCH_CHECK();
+    }
+    MARK(5);
+    hipLaunchKernelGGL(k_clump_contacts<true>, dim3(div_up(nb, 128)), dim3(128), 0, s, cs, nx, c->omega, c->i0,
+                       c->i1, c->sx, c->sv, c->bfirst, c->bcount, nb, c->adj, c->ranges, cd, nullptr, nullptr);
+    SD_LAUNCH_CHECK();
+    MARK(6);
+#undef MARK
+    c->cur = 1 - c->cur;
+    return SD_OK;
+}
+
+}  // namespace
+
+extern "C" sd_status sd_clump_create(const sd_clump_params* p, sd_clump_ctx** out) {
+    SD_REQUIRE(p && out, "sd_clump_create: NULL argument");
+    *out = nullptr;
+    sd_status st = check_contact(&p->contact);
+    if (st != SD_OK) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_error(SD_ERR_NO_DEVICE, "sd_clump_create: no HIP device");
+    sd_clump_ctx* c = new sd_clump_ctx();
+    c->p = *p;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
+    *out = c;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_clump_destroy(sd_clump_ctx* c) {
+    if (!c) return SD_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    cfree(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SD_OK;
+}
+
+// Bodies: pos4 (x, y, z, mass), vel4 (v, 0), quat4 (w, x, y, z), angvel4 (w, 0) world,
+// inertia6 (body frame xx, yy, zz, xy, xz, yz), first/count sphere ranges.
+// Spheres: local4 (body-frame offset from the centre of mass, radius).
+extern "C" sd_status sd_clump_upload(sd_clump_ctx* c, int nb, const float* pos4, const float* vel4,
+                                     const float* quat4, const float* angvel4, const float* inertia6,
+                                     const int* first, const int* count, int ns, const float* local4) {
+    SD_REQUIRE(c && nb >= 1 && ns >= 1, "sd_clump_upload: bad sizes");
+    SD_REQUIRE(pos4 && vel4 && quat4 && angvel4 && inertia6 && first && count && local4, "sd_clump_upload: NULL array");
+    SD_REQUIRE((uint64_t)nb * (uint64_t)nb < (1ull << 62), "sd_clump_upload: too many bodies");
+    std::vector<int> sbody(ns, -1);
+    for (int b = 0; b < nb; ++b) {
+        if (count[b] < 1 || first[b] < 0 || first[b] + count[b] > ns)
+            return set_error(SD_ERR_ARG, "sd_clump_upload: body %d has a bad sphere range", b);
+        if (!(pos4[4 * (size_t)b + 3] >= 0.0f))
+            return set_error(SD_ERR_ARG, "sd_clump_upload: body %d has a negative mass", b);
+        for (int k = first[b]; k < first[b] + count[b]; ++k) {
+            if (sbody[k] != -1) return set_error(SD_ERR_ARG, "sd_clump_upload: sphere %d belongs to two bodies", k);
+            sbody[k] = b;
+        }
+    }
+    for (int k = 0; k < ns; ++k)
+        if (sbody[k] < 0) return set_error(SD_ERR_ARG, "sd_clump_upload: sphere %d belongs to no body", k);
+    (void)hipStreamSynchronize(c->stream);
+    cfree(c);
+    c->nb = nb;
+    c->ns = ns;
+    sd_status st = SD_OK;
+#define ALLOC(ptr, cnt)                  \
+    if (st == SD_OK) st = dalloc(&(ptr), (size_t)(cnt))
+    for (int k = 0; k < 2; ++k) {
+        ALLOC(c->pos[k], nb);
+        ALLOC(c->vel[k], nb);
+        ALLOC(c->quat[k], nb);
+        ALLOC(c->angm[k], nb);
+    }
+    ALLOC(c->i0, nb);
+    ALLOC(c->i1, nb);
+    ALLOC(c->omega, nb);
+    ALLOC(c->bfirst, nb);
+    ALLOC(c->bcount, nb);
+    ALLOC(c->sbody, ns);
+    ALLOC(c->slocal, ns);
+    ALLOC(c->sx, ns);
+    ALLOC(c->sv, ns);
+    ALLOC(c->blo, nb);
+    ALLOC(c->bhi, nb);
+    ALLOC(c->domain, 8);
+    ALLOC(c->dom_scratch, 256 * 6);
+    ALLOC(c->mkeys, nb);
+    ALLOC(c->mkeys2, nb);
+    c->sort_tmp_bytes = radix_temp_bytes(nb);
+    ALLOC(*(char**)&c->sort_tmp, c->sort_tmp_bytes);
+    ALLOC(c->left, nb);
+    ALLOC(c->right, nb);
+    ALLOC(c->parent, 2 * (size_t)nb);
+    ALLOC(c->first, 2 * (size_t)nb);
+    ALLOC(c->last, 2 * (size_t)nb);
+    ALLOC(c->rope, 2 * (size_t)nb);
+    ALLOC(c->nlo, 2 * (size_t)nb);
+    ALLOC(c->nhi, 2 * (size_t)nb);
+    ALLOC(c->counters, nb);
+    ALLOC(c->error, 1);
+    sd_lbvh_query_scratch_bytes(nb, &c->q_scratch_bytes);
+    ALLOC(*(char**)&c->q_scratch, c->q_scratch_bytes);
+    ALLOC(c->ranges, 2 * (size_t)nb);
+    ALLOC(c->force, nb);
+    ALLOC(c->torque, nb);
+#undef ALLOC
+    if (st == SD_OK) st = grow_pairs(c, 8 * nb);
+    if (st != SD_OK) {
+        cfree(c);
+        return st;
+    }
+    std::vector<float4> P(nb), V(nb), Q(nb), Lm(nb), I0(nb), I1(nb), loc(ns);
+    for (int b = 0; b < nb; ++b) {
+        const float* p = pos4 + 4 * (size_t)b;
+        const float m = p[3];
+        P[b] = make_float4(p[0], p[1], p[2], m > 0.0f ? 1.0f / m : 0.0f);
+        const float* v = vel4 + 4 * (size_t)b;
+        V[b] = m > 0.0f ? make_float4(v[0], v[1], v[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float* q = quat4 + 4 * (size_t)b;
+        Q[b] = make_float4(q[0], q[1], q[2], q[3]);
+        // inverse of the symmetric body inertia (double precision on the host)
+        const float* I = inertia6 + 6 * (size_t)b;
+        const double a = I[0], e = I[1], f = I[2], xy = I[3], xz = I[4], yz = I[5];
+        const double c00 = e * f - yz * yz, c01 = xz * yz - xy * f, c02 = xy * yz - xz * e;
+        const double det = a * c00 + xy * c01 + xz * c02;
+        if (m > 0.0f && !(det > 0.0))
+            return set_error(SD_ERR_ARG, "sd_clump_upload: body %d has a singular inertia tensor", b);
+        double inv[6] = {0, 0, 0, 0, 0, 0};
+        if (m > 0.0f) {
+            inv[0] = c00 / det;
+            inv[1] = (a * f - xz * xz) / det;
+            inv[2] = (a * e - xy * xy) / det;
+            inv[3] = c01 / det;
+            inv[4] = c02 / det;
+            inv[5] = (xz * xy - a * yz) / det;
+        }
+        I0[b] = make_float4((float)inv[0], (float)inv[1], (float)inv[2], (float)inv[3]);
+        I1[b] = make_float4((float)inv[4], (float)inv[5], 0.0f, 0.0f);
+        // L = R I R^T w (world), in double
+        const float* w = angvel4 + 4 * (size_t)b;
+        const double qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+        const double R[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy),
+                             2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx),
+                             2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)};
+        const double Ib[9] = {a, xy, xz, xy, e, yz, xz, yz, f};
+        double u[3], t[3], Lw[3];
+        for (int r = 0; r < 3; ++r) u[r] = R[0 + r] * w[0] + R[3 + r] * w[1] + R[6 + r] * w[2];
+        for (int r = 0; r < 3; ++r) t[r] = Ib[3 * r] * u[0] + Ib[3 * r + 1] * u[1] + Ib[3 * r + 2] * u[2];
+        for (int r = 0; r < 3; ++r) Lw[r] = R[3 * r] * t[0] + R[3 * r + 1] * t[1] + R[3 * r + 2] * t[2];
+        Lm[b] = m > 0.0f ? make_float4((float)Lw[0], (float)Lw[1], (float)Lw[2], 0.0f)
+                         : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    for (int k = 0; k < ns; ++k)
+        loc[k] = make_float4(local4[4 * (size_t)k], local4[4 * (size_t)k + 1], local4[4 * (size_t)k + 2],
+                             local4[4 * (size_t)k + 3]);
+    hipStream_t s = c->stream;
+    SD_HIP(hipMemcpyAsync(c->pos[0], P.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->vel[0], V.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->quat[0], Q.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->angm[0], Lm.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->i0, I0.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->i1, I1.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->bfirst, first, nb * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->bcount, count, nb * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->sbody, sbody.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->slocal, loc.data(), ns * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipStreamSynchronize(s));
+    c->cur = 0;
+    c->steps_done = 0;
+    c->pair_total = 0;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_clump_step(sd_clump_ctx* c, int nsteps) {
+    SD_REQUIRE(c && nsteps >= 0, "sd_clump_step: bad argument");
+    if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_step: no bodies uploaded");
+    for (int i = 0; i < nsteps; ++i) {
+        sd_status st = clump_step(c, nullptr);
+        if (st != SD_OK) return st;
+    }
+    SD_HIP(hipStreamSynchronize(c->stream));
+    c->steps_done += nsteps;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_clump_profile(sd_clump_ctx* c, int nsteps, float* stage_ms, int nstages) {
+    SD_REQUIRE(c && stage_ms && nstages >= 1 && nsteps >= 1, "sd_clump_profile: bad argument");
+    if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_profile: no bodies uploaded");
+    const int ns = nstages < 6 ? nstages : 6;
+    for (int k = 0; k < nstages; ++k) stage_ms[k] = 0.0f;
+    hipEvent_t ev[7];
+    for (int k = 0; k < 7; ++k) SD_HIP(hipEventCreate(&ev[k]));
+    sd_status st = SD_OK;
+    for (int i = 0; i < nsteps && st == SD_OK; ++i) {
+        st = clump_step(c, ev);
+        if (st != SD_OK) break;
+        if (hipEventSynchronize(ev[6]) != hipSuccess) {
+            st = set_error(SD_ERR_HIP, "sd_clump_profile: event wait failed");
+            break;
+        }
+        for (int k = 0; k < ns; ++k) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            stage_ms[k] += ms;
+        }
+    }
+    for (int k = 0; k < 7; ++k) (void)hipEventDestroy(ev[k]);
+    if (st == SD_OK) c->steps_done += nsteps;
+    return st;
+}
+
+// Body state in body order: pos4 (x, y, z, 0), vel4, quat4, angmom4 (world L); any may be NULL.
+extern "C" sd_status sd_clump_download(sd_clump_ctx* c, float* pos4, float* vel4, float* quat4, float* angmom4) {
+    SD_REQUIRE(c, "sd_clump_download: NULL");
+    if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_download: no bodies uploaded");
+    const size_t bytes = (size_t)c->nb * sizeof(float4);
+    hipStream_t s = c->stream;
+    if (pos4) SD_HIP(hipMemcpyAsync(pos4, c->pos[c->cur], bytes, hipMemcpyDeviceToHost, s));
+    if (vel4) SD_HIP(hipMemcpyAsync(vel4, c->vel[c->cur], bytes, hipMemcpyDeviceToHost, s));
+    if (quat4) SD_HIP(hipMemcpyAsync(quat4, c->quat[c->cur], bytes, hipMemcpyDeviceToHost, s));
+    if (angmom4) SD_HIP(hipMemcpyAsync(angmom4, c->angm[c->cur], bytes, hipMemcpyDeviceToHost, s));
+    SD_HIP(hipStreamSynchronize(s));
+    if (pos4)
+        for (int b = 0; b < c->nb; ++b) pos4[4 * (size_t)b + 3] = 0.0f;
+    return SD_OK;
+}
+
+// Unique candidate body pairs (a < b) of the last step's broadphase.
+extern "C" sd_status sd_clump_pairs(sd_clump_ctx* c, int* pairs, int capacity, int* n_pairs) {
+    SD_REQUIRE(c && n_pairs, "sd_clump_pairs: NULL");
+    *n_pairs = c->npairs;
+    if (pairs && capacity > 0) {
+        const int k = c->npairs < capacity ? c->npairs : capacity;
+        SD_HIP(hipMemcpyAsync(pairs, c->pairs, 2 * (size_t)k * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        SD_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (c->npairs > capacity) return set_error(SD_ERR_CAPACITY, "sd_clump_pairs: %d pairs", c->npairs);
+    return SD_OK;
+}
+
+// Sphere world positions (x, y, z, r) of the last pose evaluation, in sphere order.
+extern "C" sd_status sd_clump_spheres(sd_clump_ctx* c, float* pos4) {
+    SD_REQUIRE(c && pos4, "sd_clump_spheres: NULL");
+    if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_spheres: no bodies uploaded");
+    SD_HIP(hipMemcpyAsync(pos4, c->sx, (size_t)c->ns * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipStreamSynchronize(c->stream));
+    return SD_OK;
+}

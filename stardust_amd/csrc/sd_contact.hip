@@ -20,8 +20,9 @@
 //   F_t    = -k_t xi_t - c_t v_t, limited to mu |f_n|; on slip xi_t = -(F_t + c_t v_t) / k_t
 //   F_i    = f_n n + F_t,  T_i = -l_i (n x F_t)
 // Walls are partners with r = 0, 1/m = 0, v = w = 0 and ids -1..-6.
-#include "sd_internal.hpp"
+#include "sd_contact_law.hpp"
 #include "sd_grid_dev.hpp"
+#include "sd_internal.hpp"
 
 namespace sd {
 
@@ -51,64 +52,6 @@ sd_status check_contact(const sd_contact_params* c) {
     SD_REQUIRE(c->dt > 0.0f, "contact: dt must be positive");
     SD_REQUIRE(c->max_contacts > 0 && c->max_contacts <= 256, "contact: max_contacts must be in [1, 256]");
     return SD_OK;
-}
-
-struct PairForce {
-    float fx, fy, fz, tx, ty, tz;
-};
-
-__device__ __forceinline__ PairForce contact_law(const ContactDev& c, float nx, float ny, float nz, float delta,
-                                                 float ri, float rj, float inv_mi, float inv_mj, float vrx, float vry,
-                                                 float vrz, float wix, float wiy, float wiz, float wjx, float wjy,
-                                                 float wjz, float& hx, float& hy, float& hz) {
-    const float m_eff = 1.0f / (inv_mi + inv_mj);
-    const float cn = c.c2z * sqrtf(c.kn * m_eff);
-    const float ct = c.c2z * sqrtf(c.kt * m_eff);
-    const float li = ri - 0.5f * delta;
-    const float lj = rj - 0.5f * delta;
-    const float ax = li * wix + lj * wjx;
-    const float ay = li * wiy + lj * wjy;
-    const float az = li * wiz + lj * wjz;
-    const float cx = ay * nz - az * ny;
-    const float cy = az * nx - ax * nz;
-    const float cz = ax * ny - ay * nx;
-    const float vcx = vrx - cx;
-    const float vcy = vry - cy;
-    const float vcz = vrz - cz;
-    const float vn = vcx * nx + vcy * ny + vcz * nz;
-    const float fn = c.kn * delta - cn * vn;
-    const float vtx = vcx - vn * nx;
-    const float vty = vcy - vn * ny;
-    const float vtz = vcz - vn * nz;
-    const float hn = hx * nx + hy * ny + hz * nz;
-    hx = hx - hn * nx;
-    hy = hy - hn * ny;
-    hz = hz - hn * nz;
-    hx = hx + vtx * c.dt;
-    hy = hy + vty * c.dt;
-    hz = hz + vtz * c.dt;
-    float ftx = -(c.kt * hx) - ct * vtx;
-    float fty = -(c.kt * hy) - ct * vty;
-    float ftz = -(c.kt * hz) - ct * vtz;
-    const float ft2 = ftx * ftx + fty * fty + ftz * ftz;
-    const float fmax = c.mu * fabsf(fn);
-    if (ft2 > fmax * fmax) {
-        const float sc = fmax / sqrtf(ft2);
-        ftx = ftx * sc;
-        fty = fty * sc;
-        ftz = ftz * sc;
-        hx = -(ftx + ct * vtx) / c.kt;
-        hy = -(fty + ct * vty) / c.kt;
-        hz = -(ftz + ct * vtz) / c.kt;
-    }
-    PairForce f;
-    f.fx = fn * nx + ftx;
-    f.fy = fn * ny + fty;
-    f.fz = fn * nz + ftz;
-    f.tx = -li * (ny * ftz - nz * fty);
-    f.ty = -li * (nz * ftx - nx * ftz);
-    f.tz = -li * (nx * fty - ny * ftx);
-    return f;
 }
 
 __device__ __forceinline__ void integrate_sphere(const ContactDev& c, float4& p, float4& v, float4& w, float fx,

@@ -119,6 +119,84 @@ extern "C" sd_status sd_gen_spheres(uint64_t seed, int n, const float lo[3], con
     return SD_OK;
 }
 
+// Rigid clumps. Per body, in this order: sphere count U{c_min..c_max}; per
+// sphere a centre N(0, sigma) per axis and a radius U[r_min, r_max]; an
+// orientation (Shoemake); a lattice jitter U[-jitter, jitter] per axis; a
+// velocity N(0, v_sigma) per axis. Sphere masses rho 4/3 pi r^3; the body sits
+// at its jittered lattice site with the member centres re-expressed about
+// their centre of mass; inertia about the centre of mass is the sum of the
+// spheres' 2/5 m r^2 plus the parallel-axis terms (replaces the per-entity
+// packing of EntityHandler::prepareArrays, entity_handler.cpp:61-94).
+extern "C" sd_status sd_gen_clumps(uint64_t seed, int nb, int lattice, float spacing, float jitter, int c_min,
+                                   int c_max, float sigma, float r_min, float r_max, float rho, float v_sigma,
+                                   float* pos4, float* vel4, float* quat4, float* angvel4, float* inertia6,
+                                   int* first, int* count, float* local4, int* ns) {
+    SD_REQUIRE(nb >= 1 && lattice >= 1 && (long)lattice * lattice * lattice >= nb, "sd_gen_clumps: bad lattice");
+    SD_REQUIRE(c_min >= 1 && c_max >= c_min && r_min > 0.0f && r_max >= r_min && rho > 0.0f,
+               "sd_gen_clumps: bad clump parameters");
+    SD_REQUIRE(pos4 && vel4 && quat4 && angvel4 && inertia6 && first && count && local4 && ns,
+               "sd_gen_clumps: NULL array");
+    sd::SplitMix64 rng(seed);
+    int total = 0;
+    std::vector<double> c, r, m;
+    for (int b = 0; b < nb; ++b) {
+        const int k = c_min + (int)(rng.next() % (uint64_t)(c_max - c_min + 1));
+        c.assign(3 * (size_t)k, 0.0);
+        r.assign(k, 0.0);
+        m.assign(k, 0.0);
+        double M = 0.0, com[3] = {0, 0, 0};
+        for (int s = 0; s < k; ++s) {
+            for (int a = 0; a < 3; ++a) c[3 * s + a] = (double)sigma * rng.normal();
+            r[s] = (double)r_min + (double)rng.uniform() * ((double)r_max - (double)r_min);
+            m[s] = (double)rho * 4.0 / 3.0 * 3.141592653589793 * r[s] * r[s] * r[s];
+            M += m[s];
+            for (int a = 0; a < 3; ++a) com[a] += m[s] * c[3 * s + a];
+        }
+        for (int a = 0; a < 3; ++a) com[a] /= M;
+        double I[6] = {0, 0, 0, 0, 0, 0};  // xx yy zz xy xz yz
+        first[b] = total;
+        count[b] = k;
+        for (int s = 0; s < k; ++s) {
+            const double d[3] = {c[3 * s] - com[0], c[3 * s + 1] - com[1], c[3 * s + 2] - com[2]};
+            const double own = 0.4 * m[s] * r[s] * r[s];
+            const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            I[0] += own + m[s] * (dd - d[0] * d[0]);
+            I[1] += own + m[s] * (dd - d[1] * d[1]);
+            I[2] += own + m[s] * (dd - d[2] * d[2]);
+            I[3] += -m[s] * d[0] * d[1];
+            I[4] += -m[s] * d[0] * d[2];
+            I[5] += -m[s] * d[1] * d[2];
+            float* l = local4 + 4 * (size_t)(total + s);
+            for (int a = 0; a < 3; ++a) l[a] = (float)d[a];
+            l[3] = (float)r[s];
+        }
+        total += k;
+        for (int a = 0; a < 6; ++a) inertia6[6 * (size_t)b + a] = (float)I[a];
+        const double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+        const double s1 = sqrt(1.0 - u1), s2 = sqrt(u1);
+        const double t1 = 6.283185307179586 * u2, t2 = 6.283185307179586 * u3;
+        float* q = quat4 + 4 * (size_t)b;
+        q[0] = (float)(s2 * cos(t2));
+        q[1] = (float)(s1 * sin(t1));
+        q[2] = (float)(s1 * cos(t1));
+        q[3] = (float)(s2 * sin(t2));
+        const int li = b % lattice, lj = (b / lattice) % lattice, lk = b / (lattice * lattice);
+        const int idx[3] = {li, lj, lk};
+        float* p = pos4 + 4 * (size_t)b;
+        for (int a = 0; a < 3; ++a)
+            p[a] = (float)(((double)idx[a] + 0.5) * (double)spacing +
+                           (double)jitter * (2.0 * (double)rng.uniform() - 1.0));
+        p[3] = (float)M;
+        float* v = vel4 + 4 * (size_t)b;
+        for (int a = 0; a < 3; ++a) v[a] = (float)((double)v_sigma * rng.normal());
+        v[3] = 0.0f;
+        float* w = angvel4 + 4 * (size_t)b;
+        w[0] = w[1] = w[2] = w[3] = 0.0f;
+    }
+    *ns = total;
+    return SD_OK;
+}
+
 // Convex hulls of points on ellipsoids. Every point of a strictly convex
 // surface is a vertex of the hull of the sample, so the point set is the hull.
 // Per hull, in this order: vertex count U{nv_min..nv_max}, semi-axes U[ax_min,
