@@ -13,6 +13,14 @@ Prints ONE JSON line with the throughput (particle-steps/s), the roofline of the
 dominant kernel (its time measured with HIP events on the scene stream in an
 eager profiling pass right after the timed region) and a CPU baseline (the
 oracle's full step on the host cores, bounded sample of the same workload).
+
+Two further sections ride on the same line (--sections, default "gjk,clumps"):
+  gjk    — hulls_c4 (131,072 convex hulls): GPU sweep-and-prune, then GJK/EPA
+           pairs/s cold-started ("plain") and with the learned warm start
+           (features + BF16 MFMA inference + warm GJK, "learned"), after training
+           the network on pairs logged from plain GJK on seeded scenes; iteration
+           histograms; CPU serial SAP and CPU GJK (oracle) beside them.
+  clumps — clumps_c3 (262,144 rigid clumps, LBVH broadphase): sphere-steps/s.
 """
 import argparse
 import json
@@ -27,6 +35,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s and GJK pairs/s, batched+learned vs plain, 1×MI355X"
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs on MI355X (FETCH x 2.0 + WRITE x 1.0 KiB, factors from the plain copy kernel);
+# the runs and the per-kernel table are in profiles/bench_hbm_counters.md.
+MEASURED_TRAFFIC = {
+    ("dem_c2_column_4194304", "contact"): (270265.7 * 2.0 + 262144.0) * 1024.0,
+}
 
 
 def log(*a):
@@ -64,6 +79,165 @@ def cpu_baseline(w, s, steps, threads):
     return w.n * steps / dt, dt
 
 
+def event_pair():
+    import ctypes
+    from stardust_amd import _lib as L
+    ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for e in ev:
+        L.call("sd_event_create", ctypes.byref(e))
+    return ev
+
+
+def elapsed_ms(ev):
+    import ctypes
+    from stardust_amd import _lib as L
+    ms = ctypes.c_float()
+    L.call("sd_event_elapsed_ms", ev[0], ev[1], ctypes.byref(ms))
+    return ms.value
+
+
+def timed(fn, reps, stream=None):
+    """Mean ms per call of fn() over reps calls, HIP events on `stream`."""
+    from stardust_amd import _lib as L
+    ev = event_pair()
+    L.synchronize(stream)
+    L.call("sd_event_record", ev[0], stream)
+    for _ in range(reps):
+        fn()
+    L.call("sd_event_record", ev[1], stream)
+    L.synchronize(stream)
+    return elapsed_ms(ev) / reps
+
+
+def gjk_section(args):
+    """hulls_c4: SAP + GJK/EPA pairs/s, plain cold start vs learned warm start."""
+    import ctypes
+
+    import numpy as np
+    from stardust_amd import _lib as L
+    from stardust_amd import hulls as H
+    from stardust_amd import warmstart as WS
+
+    log("gjk: generating hulls_c4")
+    t0 = time.perf_counter()
+    h = H.hulls_c4()
+    log("gjk: generated in %.1f s" % (time.perf_counter() - t0))
+    d = H.DeviceHulls(h)
+    d.aabbs()
+    pairs, npairs, axis = d.sap()
+    cnt, ax = ctypes.c_int(), ctypes.c_int()
+    sap_ms = timed(lambda: L.call("sd_sap_pairs", d.lo4.ptr, d.hi4.ptr, d.n, pairs.ptr, npairs, ctypes.byref(cnt),
+                                  ctypes.byref(ax), d.scratch.ptr, d.scratch_bytes, None), 5)
+    aabb_ms = timed(d.aabbs, 5)
+    log("gjk: %d SAP pairs on axis %d; aabb %.3f ms, sap %.3f ms" % (npairs, axis, aabb_ms, sap_ms))
+
+    # training data: plain GJK's converged directions on seeded scenes of the same generator
+    t0 = time.perf_counter()
+    data = WS.PairData(args.train_samples)
+    seed = 0x5EED1000
+    while data.n < args.train_samples and time.perf_counter() - t0 < args.train_log_seconds:
+        got = data.append_scene(H.generate_hulls(seed, len(h["pos4"])))
+        seed += 1
+        log("gjk: logged scene %d: %d samples (total %d)" % (seed - 0x5EED1000, got, data.n))
+    log_s = time.perf_counter() - t0
+    model = WS.WarmStartModel(seed=11)
+    t0 = time.perf_counter()
+    hist = model.train(data, epochs=args.train_epochs, batch=8192, lr=3e-3, log=lambda *a: log("gjk:", *a))
+    train_s = time.perf_counter() - t0
+    log("gjk: trained on %d samples (%d scenes, logged in %.1f s), %.1f s" % (data.n, seed - 0x5EED1000, log_s,
+                                                                               train_s))
+
+    # hull descriptors depend on the shapes only: computed once with the scene
+    X, desc = WS.pair_features(d, pairs, npairs)
+    warm = model.infer(X, npairs, pairs=pairs, quat=d.buf["quat4"])
+    out_c = d.gjk(pairs, npairs)
+    out_w = d.gjk(pairs, npairs, warm=warm)
+    reps = args.gjk_reps
+    b = d.buf
+    cold_ms = timed(lambda: d.gjk(pairs, npairs, out=out_c), reps)
+    feat_ms = timed(lambda: L.call("sd_mlp_features", pairs.ptr, npairs, b["pos4"].ptr, b["quat4"].ptr, desc.ptr,
+                                   X.ptr, None), reps)
+    infer_ms = timed(lambda: model.infer(X, npairs, pairs=pairs, quat=d.buf["quat4"], out=warm), reps)
+    warm_ms = timed(lambda: d.gjk(pairs, npairs, warm=warm, out=out_w), reps)
+    learned_ms = feat_ms + infer_ms + warm_ms
+    c = d.download_gjk(out_c, npairs)
+    w = d.download_gjk(out_w, npairs)
+    ok = ((c["flags"] | w["flags"]) & L.GJK_BUDGET) == 0
+    same_verdict = bool(np.array_equal(c["flags"] & 1, w["flags"] & 1))
+    rel = np.abs(w["sep"][ok] - c["sep"][ok]) / np.maximum(np.abs(c["sep"][ok]), 1e-3)
+    hmax = int(max(c["iters"].max(), w["iters"].max())) + 1
+    log("gjk: cold %.3f ms (%.2f it), learned %.3f ms = feat %.3f + infer %.3f + warm %.3f (%.2f it)" % (
+        cold_ms, c["iters"].mean(), learned_ms, feat_ms, infer_ms, warm_ms, w["iters"].mean()))
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        lo, hi = d.download_aabbs()
+        t0 = time.perf_counter()
+        cpairs, _ = O.sap_pairs(lo, hi, axis)
+        sap_cpu_s = time.perf_counter() - t0
+        threads = min(os.cpu_count() or 1, 16)
+        pr = d_pairs_host(pairs, npairs)
+        pr = np.ascontiguousarray(pr[:args.cpu_gjk_pairs] if args.cpu_gjk_pairs > 0 else pr)
+        t0 = time.perf_counter()
+        O.gjk(h, pr, nthreads=threads)
+        gjk_cpu_s = time.perf_counter() - t0
+        cpu = {"sap_serial_ms": 1000 * sap_cpu_s, "sap_pairs": int(len(cpairs)), "sap_cores": 1,
+               "gjk_pairs_per_s": len(pr) / gjk_cpu_s, "gjk_cores": threads,
+               "gjk_sample": "%d SAP pairs of hulls_c4, cold GJK/EPA by the C oracle (OpenMP)" % len(pr)}
+        log("gjk: cpu serial SAP %.1f ms, cpu GJK %.3e pairs/s (%d threads)" % (1000 * sap_cpu_s,
+                                                                               cpu["gjk_pairs_per_s"], threads))
+    return {
+        "workload": "hulls_c4", "hulls": d.n, "vertices": d.n_verts, "sap_axis": axis, "pairs": npairs,
+        "aabb_ms": round(aabb_ms, 4), "sap_ms": round(sap_ms, 4),
+        "plain_pairs_per_s": npairs / (cold_ms / 1000.0), "plain_ms": round(cold_ms, 4),
+        "learned_pairs_per_s": npairs / (learned_ms / 1000.0),
+        "learned_ms": {"features": round(feat_ms, 4), "inference_bf16": round(infer_ms, 4),
+                       "gjk_warm": round(warm_ms, 4), "total": round(learned_ms, 4)},
+        "learned_over_plain": cold_ms / learned_ms,
+        "mlp_inference_tflops": 19584.0 * npairs / (infer_ms / 1000.0) / 1e12,
+        "iters_mean": {"plain": float(c["iters"].mean()), "learned": float(w["iters"].mean())},
+        "iters_hist": {"plain": np.bincount(c["iters"], minlength=hmax).tolist(),
+                       "learned": np.bincount(w["iters"], minlength=hmax).tolist()},
+        "warm_used_frac": float(((w["flags"] & L.GJK_WARM_USED) != 0).mean()),
+        "verdicts_identical": same_verdict, "max_rel_sep_diff": float(rel.max()) if len(rel) else 0.0,
+        "hit_frac": float((c["flags"] & 1).mean()),
+        "train": {"samples": data.n, "epochs": args.train_epochs, "loss": [round(x, 5) for x in hist],
+                  "log_s": round(log_s, 2), "train_s": round(train_s, 2)},
+        "cpu": cpu,
+    }
+
+
+def d_pairs_host(pairs, npairs):
+    import numpy as np
+    return pairs.download(np.int32, 2 * npairs).reshape(-1, 2)
+
+
+def clump_section(args):
+    """clumps_c3: rigid clumps with the LBVH broadphase, sphere-steps/s."""
+    from stardust_amd import clumps as CL
+    log("clumps: generating clumps_c3")
+    t0 = time.perf_counter()
+    s = CL.clumps_c3()
+    log("clumps: generated in %.1f s" % (time.perf_counter() - t0))
+    sc = CL.ClumpScene(CL.contact_c3())
+    sc.upload(s)
+    sc.step(args.clump_warmup)
+    t0 = time.perf_counter()
+    sc.step(args.clump_steps)  # returns after the stream is synchronised
+    wall = time.perf_counter() - t0
+    prof = sc.profile(args.profile_steps)
+    per = {k: v / args.profile_steps for k, v in prof.items()}
+    npairs = len(sc.pairs())
+    sc.close()
+    ms = 1000.0 * wall / args.clump_steps
+    log("clumps: %.3f ms/step, stages %s" % (ms, {k: round(v, 4) for k, v in per.items()}))
+    return {"workload": "clumps_c3", "bodies": sc.nb, "spheres": sc.ns, "steps": args.clump_steps,
+            "ms_per_step": round(ms, 4), "sphere_steps_per_s": sc.ns / (ms / 1000.0),
+            "body_steps_per_s": sc.nb / (ms / 1000.0), "stage_ms": {k: round(v, 4) for k, v in per.items()},
+            "broadphase_pairs": npairs, "form": "eager step (pair count read back each step)"}
+
+
 def calibrate_copy(nbytes=256 << 20, reps=5):
     """Runs k_copy_u32 over nbytes: each dispatch reads nbytes and writes nbytes."""
     import ctypes
@@ -94,6 +268,14 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sections", default="gjk,clumps", help="extra sections: gjk, clumps (comma list, or none)")
+    ap.add_argument("--train-samples", type=int, default=10_000_000)
+    ap.add_argument("--train-log-seconds", type=float, default=90.0)
+    ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--gjk-reps", type=int, default=5)
+    ap.add_argument("--cpu-gjk-pairs", type=int, default=0, help="CPU GJK sample size (0: every SAP pair)")
+    ap.add_argument("--clump-warmup", type=int, default=5)
+    ap.add_argument("--clump-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the plain copy kernel on 256 MiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -180,12 +362,18 @@ def main():
                    "keys": "u64 (cell<<32|index)" if w.key64 else "u32 cell + u32 index",
                    "cells": g.n_cells, "form": "plain single-scene, graph-replayed step"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((w.name, dominant)), "algorithmic_bytes": sbytes[dominant]},
         "stages": stages,
         "contacts_per_sphere": contacts / w.n,
         "energy_J": {"kinetic": diag["kinetic"], "potential": diag["potential"]},
         "cpu_baseline": cpu,
     }
+    sections = [x for x in args.sections.split(",") if x and x != "none"]
+    if "gjk" in sections:
+        out["gjk"] = gjk_section(args)
+    if "clumps" in sections:
+        out["clumps"] = clump_section(args)
     print(json.dumps(out), flush=True)
 
 
