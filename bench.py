@@ -14,7 +14,10 @@ dominant kernel (its time measured with HIP events on the scene stream in an
 eager profiling pass right after the timed region) and a CPU baseline (the
 oracle's full step on the host cores, bounded sample of the same workload).
 
-Two further sections ride on the same line (--sections, default "gjk,clumps"):
+Further sections ride on the same line (--sections, default "batch,gjk,clumps"):
+  batch  — batch_c5 (512 independent scenes, 7.7M spheres): particle-steps/s of
+           the batched graph-replayed step (after the autotuner picks the launch
+           shapes) vs the plain form (every scene alone in its own context).
   gjk    — hulls_c4 (131,072 convex hulls): GPU sweep-and-prune, then GJK/EPA
            pairs/s cold-started ("plain") and with the learned warm start
            (features + BF16 MFMA inference + warm GJK, "learned"), after training
@@ -238,6 +241,66 @@ def clump_section(args):
             "broadphase_pairs": npairs, "form": "eager step (pair count read back each step)"}
 
 
+def batch_section(args):
+    """batch_c5: 512 scenes stepped as one batch vs the same scenes stepped one by one."""
+    import numpy as np
+    from stardust_amd import _lib as L
+    from stardust_amd import workloads as W
+    from stardust_amd.scene import BatchScene, SphereScene
+
+    w = W.batch_c5()
+    log("batch: generating %s (%d spheres, %d cells)" % (w.name, w.n, w.cells()))
+    t0 = time.perf_counter()
+    s, scenes = w.generate()
+    log("batch: generated in %.1f s" % (time.perf_counter() - t0))
+    b = BatchScene(use_graph=True)
+    b.upload_scenes(scenes, s["x"], s["y"], s["z"], s["r"], s["m"])
+    tune = b.autotune(trial_steps=4)
+    log("batch: autotuned", tune)
+    b.step(args.batch_warmup)
+    stream = b.stream()
+    ev = event_pair()
+    L.synchronize(stream)
+    t0 = time.perf_counter()
+    L.call("sd_event_record", ev[0], stream)
+    b.step(args.batch_steps)
+    L.call("sd_event_record", ev[1], stream)
+    L.synchronize(stream)
+    wall = time.perf_counter() - t0
+    t = max(wall, elapsed_ms(ev) / 1000.0)
+    batched = w.n * args.batch_steps / t
+    prof = b.profile(args.profile_steps)
+    per = {k: round(v / args.profile_steps, 5) for k, v in prof.items()}
+    contacts = b.contact_count()
+    b.close()
+    log("batch: %.3f ms/step -> %.3e particle-steps/s; stages %s" % (1000 * t / args.batch_steps, batched, per))
+
+    # plain: every scene alone in its own context (graph-replayed steps), one after the other
+    tot_t, tot_work = 0.0, 0
+    firsts = np.concatenate([[0], np.cumsum([d[0] for d in scenes])])
+    for k, (n, g, c) in enumerate(scenes):
+        lo, hi = int(firsts[k]), int(firsts[k + 1])
+        sc = SphereScene(g, c, key64=True, use_graph=True)
+        sc.upload(s["x"][lo:hi], s["y"][lo:hi], s["z"][lo:hi], s["r"][lo:hi], s["m"][lo:hi])
+        sc.step(2)
+        t0 = time.perf_counter()
+        sc.step(args.plain_steps)  # returns after the scene's stream is synchronised
+        tot_t += time.perf_counter() - t0
+        tot_work += n * args.plain_steps
+        sc.close()
+        if k % 128 == 127:
+            log("batch: plain form, %d scenes done" % (k + 1))
+    plain = tot_work / tot_t
+    log("batch: plain %.3e particle-steps/s, batched/plain %.1fx" % (plain, batched / plain))
+    return {"workload": w.name, "scenes": w.nscenes, "spheres": w.n, "cells": w.cells(), "steps": args.batch_steps,
+            "batched_particle_steps_per_s": batched, "batched_ms_per_step": round(1000 * t / args.batch_steps, 4),
+            "plain_particle_steps_per_s": plain, "plain_steps_per_scene": args.plain_steps,
+            "batched_over_plain": batched / plain, "autotune": tune, "stage_ms": per,
+            "contacts_per_sphere": contacts / w.n,
+            "form": "batched: one graph-replayed step for all scenes; plain: each scene alone in its own "
+                    "graph-replayed context, scenes one after the other"}
+
+
 def calibrate_copy(nbytes=256 << 20, reps=5):
     """Runs k_copy_u32 over nbytes: each dispatch reads nbytes and writes nbytes."""
     import ctypes
@@ -268,7 +331,11 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="gjk,clumps", help="extra sections: gjk, clumps (comma list, or none)")
+    ap.add_argument("--sections", default="batch,gjk,clumps",
+                    help="extra sections: batch, gjk, clumps (comma list, or none)")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-warmup", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-log-seconds", type=float, default=90.0)
     ap.add_argument("--train-epochs", type=int, default=3)
@@ -370,6 +437,8 @@ def main():
         "cpu_baseline": cpu,
     }
     sections = [x for x in args.sections.split(",") if x and x != "none"]
+    if "batch" in sections:
+        out["batch"] = batch_section(args)
     if "gjk" in sections:
         out["gjk"] = gjk_section(args)
     if "clumps" in sections:

@@ -202,6 +202,37 @@ sd_status sd_ctx_stream(sd_ctx* ctx, void** stream);
 sd_status sd_ctx_state_bytes(sd_ctx* ctx, size_t* bytes);
 sd_status sd_ctx_save(sd_ctx* ctx, void* buffer, size_t bytes);
 sd_status sd_ctx_load(sd_ctx* ctx, const void* buffer, size_t bytes);
+/* Autotuner: times trial_steps eager steps for every radix-sort tile size
+ * (4, 8, 16 items per thread) and contact block size (64, 128, 256), keeps the
+ * fastest pair and returns it with its time per step. Results of a step do not
+ * depend on these shapes; the state is restored after the trials. Generalises
+ * the block-size sweeps of the reference's profiling runs (profiling/profile_1.py). */
+sd_status sd_ctx_autotune(sd_ctx* ctx, int trial_steps, int* tile_items, int* contact_block, float* ms_per_step);
+
+/* ------------------------------------------------------- batched scenes
+ * Many independent sphere scenes stepped as one: one scene-segmented sort
+ * (keys = (scene cell base + cell) << 32 | slot, so every scene keeps its own
+ * sorted order), one cell-range pass, one contact launch in which every sphere
+ * takes its scene's grid, contact law and walls, one graph per step pair.
+ * A scene stepped in a batch is bit-identical to the same scene stepped alone
+ * in an sd_ctx. dt, gravity and max_contacts must be equal in all scenes.
+ * sd_ctx_step / _profile / _autotune / _download_spheres / _diagnostics /
+ * _save / _load work on a batch context (arrays concatenated in scene order).
+ * Replaces running DEMEngine::update (simulation_kernel.cu:52) once per scene. */
+typedef struct sd_batch_params {
+    int use_graph;       /* 1: replay steps from a captured hipGraph */
+    int tile_items;      /* sort items per thread (0 = default for the size) */
+    int contact_block;   /* contact kernel block size (0 = 128) */
+} sd_batch_params;
+typedef struct sd_scene_desc {
+    int n;                      /* spheres of the scene */
+    sd_grid grid;               /* the scene's own grid (cell >= largest diameter) */
+    sd_contact_params contact;  /* the scene's law and walls */
+} sd_scene_desc;
+sd_status sd_batch_create(const sd_batch_params* p, sd_ctx** ctx);
+sd_status sd_batch_upload(sd_ctx* ctx, int nscenes, const sd_scene_desc* scenes, const float* x, const float* y,
+                          const float* z, const float* vx, const float* vy, const float* vz, const float* radius,
+                          const float* mass);
 
 /* ------------------------------------------------------- scene generators
  * Seeded SplitMix64 generators (host only). uniform = (u >> 40) * 2^-24,

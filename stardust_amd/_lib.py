@@ -71,6 +71,14 @@ class SphereParams(C.Structure):
                 ("tile_items", C.c_int)]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("use_graph", C.c_int), ("tile_items", C.c_int), ("contact_block", C.c_int)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("n", C.c_int), ("grid", Grid), ("contact", ContactParams)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _F = C.c_float
@@ -119,6 +127,9 @@ _SIGS = {
     "sd_ctx_state_bytes": [_P, C.POINTER(_SZ)],
     "sd_ctx_save": [_P, _P, _SZ],
     "sd_ctx_load": [_P, _P, _SZ],
+    "sd_ctx_autotune": [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)],
+    "sd_batch_create": [C.POINTER(BatchParams), C.POINTER(C.c_void_p)],
+    "sd_batch_upload": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "sd_gen_spheres": [C.c_uint64, _I, _P, _P, _F, _F, _F, _F, _P, _P, _P, _P],
     "sd_gen_hulls": [C.c_uint64, _I, _P, _P, _I, _I, _F, _F, _F, _P, _P, _P, _P, _P, _P, _I, C.POINTER(C.c_int)],
     "sd_hull_aabbs": [_P, _P, _P, _P, _P, _I, _P, _P, _P],

@@ -107,12 +107,31 @@ __device__ __forceinline__ int xcd_logical_block(int nblocks) {
 // MODE: CONTACT_FORCES writes force/torque; CONTACT_FUSED_* integrates and writes
 // the next state and keys. INDIRECT: vel/omg/ids are in the previous step's order
 // and are read through old_slot (positions are always in sorted order).
+// CONTACT_FUSED_BATCH: the grid, contact law and cell ranges come from the
+// sphere's scene; the arithmetic is the same code, so a scene stepped in a
+// batch is bit-identical to the same scene stepped alone.
 template <int MODE, bool INDIRECT>
 __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
+    constexpr bool BATCH = MODE == CONTACT_FUSED_BATCH;
     const int lb = xcd_logical_block((a.n + (int)blockDim.x - 1) / (int)blockDim.x);
     if (lb < 0) return;
     const int s = lb * (int)blockDim.x + (int)threadIdx.x;
     if (s >= a.n) return;
+    GridDev g;
+    ContactDev c;
+    const int* ranges;
+    uint32_t cbase = 0u;
+    if constexpr (BATCH) {
+        const SceneDev& S = a.scenes[a.scene_of[s]];
+        g = S.g;
+        c = S.c;
+        cbase = S.cbase;
+        ranges = a.ranges + 2 * (size_t)cbase;
+    } else {
+        g = a.g;
+        c = a.c;
+        ranges = a.ranges;
+    }
     float4 p = a.pos[s];
     const int os = old_slot_of(a, s);
     const int self = INDIRECT ? os : s;
@@ -120,10 +139,10 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     float4 w = a.omg[self];
     const int myid = a.ids[self];
     int oc = a.hcount_old[os];
-    oc = oc < a.c.maxc ? oc : a.c.maxc;
+    oc = oc < c.maxc ? oc : c.maxc;
 
     int cx, cy, cz;
-    cell_coords(a.g, p.x, p.y, p.z, cx, cy, cz);
+    cell_coords(g, p.x, p.y, p.z, cx, cy, cz);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
     int nc = 0;
 
@@ -145,7 +164,7 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
         const float nx = dx * inv_d, ny = dyy * inv_d, nz = dzz * inv_d;
         float hx, hy, hz;
         find_history(a, os, oc, pid, hx, hy, hz);
-        const PairForce f = contact_law(a.c, nx, ny, nz, rs - d, p.w, q.w, v.w, vq.w, v.x - vq.x, v.y - vq.y,
+        const PairForce f = contact_law(c, nx, ny, nz, rs - d, p.w, q.w, v.w, vq.w, v.x - vq.x, v.y - vq.y,
                                         v.z - vq.z, w.x, w.y, w.z, wq.x, wq.y, wq.z, hx, hy, hz);
         fx = fx + f.fx;
         fy = fy + f.fy;
@@ -153,18 +172,18 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
         tx = tx + f.tx;
         ty = ty + f.ty;
         tz = tz + f.tz;
-        if (nc < a.c.maxc) a.hist_new[(size_t)nc * a.n + s] = make_float4(hx, hy, hz, __int_as_float(pid));
+        if (nc < c.maxc) a.hist_new[(size_t)nc * a.n + s] = make_float4(hx, hy, hz, __int_as_float(pid));
         ++nc;
     };
 
     for (int dz = -1; dz <= 1; ++dz) {
         const int z = cz + dz;
-        if (z < 0 || z >= a.g.nz) continue;
+        if (z < 0 || z >= g.nz) continue;
         for (int dy = -1; dy <= 1; ++dy) {
             const int y = cy + dy;
-            if (y < 0 || y >= a.g.ny) continue;
+            if (y < 0 || y >= g.ny) continue;
             int lo, hi;
-            row_range(a.g, a.ranges, cx, y, z, lo, hi);
+            row_range(g, ranges, cx, y, z, lo, hi);
             for (int t0 = lo; t0 < hi; t0 += CAND_UNROLL) {
                 // issue all loads of the batch first, then test; contacts are
                 // visited in ascending position order
@@ -191,13 +210,13 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
         }
     }
 
-    if (a.c.walls && v.w > 0.0f) {
+    if (c.walls && v.w > 0.0f) {
 #pragma unroll
         for (int wl = 0; wl < 6; ++wl) {
             const int ax = wl >> 1;
             const bool upper = wl & 1;
             const float pc = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
-            const float dist = upper ? a.c.whi[ax] - pc : pc - a.c.wlo[ax];
+            const float dist = upper ? c.whi[ax] - pc : pc - c.wlo[ax];
             if (!(dist < p.w)) continue;
             const float sgn = upper ? -1.0f : 1.0f;
             const float nx = ax == 0 ? sgn : 0.0f;
@@ -206,7 +225,7 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
             const int pid = -(wl + 1);
             float hx, hy, hz;
             find_history(a, os, oc, pid, hx, hy, hz);
-            const PairForce f = contact_law(a.c, nx, ny, nz, p.w - dist, p.w, 0.0f, v.w, 0.0f, v.x, v.y, v.z, w.x,
+            const PairForce f = contact_law(c, nx, ny, nz, p.w - dist, p.w, 0.0f, v.w, 0.0f, v.x, v.y, v.z, w.x,
                                             w.y, w.z, 0.0f, 0.0f, 0.0f, hx, hy, hz);
             fx = fx + f.fx;
             fy = fy + f.fy;
@@ -214,28 +233,28 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
             tx = tx + f.tx;
             ty = ty + f.ty;
             tz = tz + f.tz;
-            if (nc < a.c.maxc) a.hist_new[(size_t)nc * a.n + s] = make_float4(hx, hy, hz, __int_as_float(pid));
+            if (nc < c.maxc) a.hist_new[(size_t)nc * a.n + s] = make_float4(hx, hy, hz, __int_as_float(pid));
             ++nc;
         }
     }
 
-    a.hcount_new[s] = nc < a.c.maxc ? nc : a.c.maxc;
-    if (nc > a.c.maxc) atomicMax(a.overflow, nc);
+    a.hcount_new[s] = nc < c.maxc ? nc : c.maxc;
+    if (nc > c.maxc) atomicMax(a.overflow, nc);
 
     if (MODE == CONTACT_FORCES) {
         a.force[s] = make_float4(fx, fy, fz, 0.0f);
         a.torque[s] = make_float4(tx, ty, tz, 0.0f);
     } else {
-        integrate_sphere(a.c, p, v, w, fx, fy, fz, tx, ty, tz);
+        integrate_sphere(c, p, v, w, fx, fy, fz, tx, ty, tz);
         a.pos_o[s] = p;
         a.vel_o[s] = v;
         a.omg_o[s] = w;
         a.ids_o[s] = myid;
         if (MODE == CONTACT_FUSED_U32) {
-            a.keys_o32[s] = cell_id(a.g, p.x, p.y, p.z);
+            a.keys_o32[s] = cell_id(g, p.x, p.y, p.z);
             a.vals_o32[s] = (uint32_t)s;
         } else {
-            a.keys_o64[s] = ((uint64_t)cell_id(a.g, p.x, p.y, p.z) << 32) | (uint64_t)(uint32_t)s;
+            a.keys_o64[s] = ((uint64_t)(cbase + cell_id(g, p.x, p.y, p.z)) << 32) | (uint64_t)(uint32_t)s;
         }
     }
 }
@@ -262,8 +281,10 @@ sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int blo
         hipLaunchKernelGGL((k_contacts<CONTACT_FORCES, false>), grid, dim3(block), 0, s, a);
     else if (mode == CONTACT_FUSED_U32)
         hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U32, true>), grid, dim3(block), 0, s, a);
-    else
+    else if (mode == CONTACT_FUSED_U64)
         hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U64, true>), grid, dim3(block), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_BATCH, true>), grid, dim3(block), 0, s, a);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

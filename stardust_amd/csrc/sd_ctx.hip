@@ -107,18 +107,30 @@ struct sd_ctx {
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::vector<hipGraph_t> graph_objs;
     long long steps_done = 0;
+    // batch of independent scenes (sd_batch_upload): scene table and the scene
+    // of every slot; scenes occupy consecutive slot and cell-id ranges
+    bool batch = false;
+    int nscenes = 0;
+    std::vector<SceneDev> scenes_h;
+    SceneDev* scenes_d = nullptr;
+    int* scene_of = nullptr;
 };
 
 namespace {
 
-void release(sd_ctx* c) {
+void drop_graphs(sd_ctx* c) {
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto g : c->graph_objs) (void)hipGraphDestroy(g);
     c->graphs.clear();
     c->graph_objs.clear();
+}
+
+void release(sd_ctx* c) {
+    drop_graphs(c);
     void* bufs[] = {c->pos[0], c->pos[1], c->vel[0], c->vel[1], c->omg[0], c->omg[1], c->ids[0], c->ids[1],
                     c->posB, c->k32[0], c->k32[1], c->v32[0], c->v32[1], c->k64[0], c->k64[1], c->sort_tmp,
-                    c->ranges, c->hcount[0], c->hcount[1], c->hist[0], c->hist[1], c->overflow, c->diag};
+                    c->ranges, c->hcount[0], c->hcount[1], c->hist[0], c->hist[1], c->overflow, c->diag,
+                    c->scenes_d, c->scene_of};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < 2; ++k) {
@@ -134,6 +146,10 @@ void release(sd_ctx* c) {
     c->hist[0] = c->hist[1] = nullptr;
     c->overflow = nullptr;
     c->diag = nullptr;
+    c->scenes_d = nullptr;
+    c->scene_of = nullptr;
+    c->scenes_h.clear();
+    c->nscenes = 0;
     c->n = 0;
 }
 
@@ -186,7 +202,9 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
         MARK(3);
         a.old_slot_u64 = K;
         a.keys_o64 = Ka;
-        st = launch_contacts(CONTACT_FUSED_U64, a, s, c->contact_block);
+        a.scenes = c->scenes_d;
+        a.scene_of = c->scene_of;
+        st = launch_contacts(c->batch ? CONTACT_FUSED_BATCH : CONTACT_FUSED_U64, a, s, c->contact_block);
         if (st != SD_OK) return st;
         MARK(4);
         c->cur_keys = (Ka == c->k64[0]) ? 0 : 1;
@@ -294,26 +312,25 @@ extern "C" sd_status sd_ctx_stream(sd_ctx* c, void** stream) {
     return SD_OK;
 }
 
-extern "C" sd_status sd_ctx_upload_spheres(sd_ctx* c, int n, const float* x, const float* y, const float* z,
-                                           const float* vx, const float* vy, const float* vz, const float* radius,
-                                           const float* mass) {
-    SD_REQUIRE(c, "sd_ctx_upload_spheres: NULL context");
-    SD_REQUIRE(n > 0, "sd_ctx_upload_spheres: n must be positive");
-    SD_REQUIRE(x && y && z && vx && vy && vz && radius && mass, "sd_ctx_upload_spheres: NULL array");
-    const float dmax = 2.0f * [&] {
-        float m = 0.0f;
-        for (int i = 0; i < n; ++i) m = radius[i] > m ? radius[i] : m;
-        return m;
-    }();
-    if (dmax > c->p.grid.cell)
-        return set_error(SD_ERR_ARG, "sd_ctx_upload_spheres: largest diameter %g exceeds the cell size %g", dmax,
-                         c->p.grid.cell);
+namespace {
+
+sd_status check_spheres(const char* who, int n, const float* x, const float* y, const float* z, const float* radius,
+                        const float* mass, float cell) {
+    float rmax = 0.0f;
     for (int i = 0; i < n; ++i) {
         if (!(radius[i] > 0.0f) || !(mass[i] >= 0.0f) || !isfinite(x[i]) || !isfinite(y[i]) || !isfinite(z[i]))
-            return set_error(SD_ERR_ARG, "sd_ctx_upload_spheres: sphere %d has a bad radius, mass or position", i);
+            return set_error(SD_ERR_ARG, "%s: sphere %d has a bad radius, mass or position", who, i);
+        rmax = radius[i] > rmax ? radius[i] : rmax;
     }
-    (void)hipStreamSynchronize(c->stream);
-    release(c);
+    if (2.0f * rmax > cell)
+        return set_error(SD_ERR_ARG, "%s: largest diameter %g exceeds the cell size %g", who, 2.0f * rmax, cell);
+    return SD_OK;
+}
+
+// Allocates the device state for n spheres and uploads it (slot = particle id);
+// keys are computed by the caller.
+sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const float* z, const float* vx,
+                       const float* vy, const float* vz, const float* radius, const float* mass) {
     c->n = n;
     const int maxc = c->p.contact.max_contacts;
     sd_status st;
@@ -375,12 +392,120 @@ extern "C" sd_status sd_ctx_upload_spheres(sd_ctx* c, int n, const float* x, con
     c->cur_keys = 0;
     c->cur_hist = 0;
     c->cur_state = 0;
+    c->steps_done = 0;
+    return SD_OK;
+}
+
+}  // namespace
+
+extern "C" sd_status sd_ctx_upload_spheres(sd_ctx* c, int n, const float* x, const float* y, const float* z,
+                                           const float* vx, const float* vy, const float* vz, const float* radius,
+                                           const float* mass) {
+    SD_REQUIRE(c, "sd_ctx_upload_spheres: NULL context");
+    SD_REQUIRE(!c->batch || c->n == 0, "sd_ctx_upload_spheres: this is a batch context; use sd_batch_upload");
+    SD_REQUIRE(n > 0, "sd_ctx_upload_spheres: n must be positive");
+    SD_REQUIRE(x && y && z && vx && vy && vz && radius && mass, "sd_ctx_upload_spheres: NULL array");
+    sd_status st = check_spheres("sd_ctx_upload_spheres", n, x, y, z, radius, mass, c->p.grid.cell);
+    if (st != SD_OK) return st;
+    (void)hipStreamSynchronize(c->stream);
+    release(c);
+    st = upload_state(c, n, x, y, z, vx, vy, vz, radius, mass);
+    if (st != SD_OK) {
+        release(c);
+        return st;
+    }
     const GridDev g = to_dev(c->p.grid);
     st = c->p.key64 ? launch_hash_keys_u64(c->pos[0], n, g, c->k64[0], c->stream)
                     : launch_hash_keys_u32(c->pos[0], n, g, c->k32[0], c->v32[0], c->stream);
     if (st != SD_OK) return st;
     SD_HIP(hipStreamSynchronize(c->stream));
-    c->steps_done = 0;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_batch_create(const sd_batch_params* p, sd_ctx** out) {
+    SD_REQUIRE(p && out, "sd_batch_create: NULL argument");
+    *out = nullptr;
+    SD_REQUIRE(p->tile_items == 0 || p->tile_items == 4 || p->tile_items == 8 || p->tile_items == 16,
+               "sd_batch_create: tile_items must be 0, 4, 8 or 16");
+    SD_REQUIRE(p->contact_block == 0 || p->contact_block == 64 || p->contact_block == 128 ||
+                   p->contact_block == 256,
+               "sd_batch_create: contact_block must be 0, 64, 128 or 256");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_error(SD_ERR_NO_DEVICE, "sd_batch_create: no HIP device");
+    sd_ctx* c = new sd_ctx();
+    c->p.key64 = 1;
+    c->p.use_graph = p->use_graph;
+    c->p.tile_items = p->tile_items;
+    c->contact_block = p->contact_block ? p->contact_block : 128;
+    c->batch = true;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
+    *out = c;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_batch_upload(sd_ctx* c, int nscenes, const sd_scene_desc* scenes, const float* x,
+                                     const float* y, const float* z, const float* vx, const float* vy,
+                                     const float* vz, const float* radius, const float* mass) {
+    SD_REQUIRE(c && scenes, "sd_batch_upload: NULL argument");
+    SD_REQUIRE(c->batch, "sd_batch_upload: not a batch context (create it with sd_batch_create)");
+    SD_REQUIRE(nscenes >= 1, "sd_batch_upload: nscenes must be >= 1");
+    SD_REQUIRE(x && y && z && vx && vy && vz && radius && mass, "sd_batch_upload: NULL array");
+    std::vector<SceneDev> table(nscenes);
+    long long n = 0;
+    uint64_t cells = 0;
+    for (int k = 0; k < nscenes; ++k) {
+        const sd_scene_desc& d = scenes[k];
+        SD_REQUIRE(d.n >= 1, "sd_batch_upload: scene %d has no spheres", k);
+        sd_status st = check_grid(&d.grid);
+        if (st == SD_OK) st = check_contact(&d.contact);
+        if (st != SD_OK) return set_error(st, "sd_batch_upload: scene %d: %s", k, sd_last_error());
+        const sd_contact_params& c0 = scenes[0].contact;
+        SD_REQUIRE(d.contact.dt == c0.dt && d.contact.gravity[0] == c0.gravity[0] &&
+                       d.contact.gravity[1] == c0.gravity[1] && d.contact.gravity[2] == c0.gravity[2] &&
+                       d.contact.max_contacts == c0.max_contacts,
+                   "sd_batch_upload: scene %d: dt, gravity and max_contacts must be the same in every scene", k);
+        st = check_spheres("sd_batch_upload", d.n, x + n, y + n, z + n, radius + n, mass + n, d.grid.cell);
+        if (st != SD_OK) return set_error(st, "sd_batch_upload: scene %d: %s", k, sd_last_error());
+        table[k].g = to_dev(d.grid);
+        table[k].c = to_dev(d.contact);
+        table[k].cbase = (uint32_t)cells;
+        table[k].first = (int)n;
+        table[k].n = d.n;
+        table[k].pad = 0;
+        cells += (uint64_t)d.grid.n[0] * (uint64_t)d.grid.n[1] * (uint64_t)d.grid.n[2];
+        n += d.n;
+        SD_REQUIRE(n < (1ll << 31) && cells < (1ull << 31), "sd_batch_upload: more than 2^31 spheres or cells");
+    }
+    (void)hipStreamSynchronize(c->stream);
+    release(c);
+    c->p.grid = scenes[0].grid;
+    c->p.contact = scenes[0].contact;
+    c->n_cells = (int)cells;
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < cells) ++bits;
+    c->key_bits = bits;
+    sd_status st = upload_state(c, (int)n, x, y, z, vx, vy, vz, radius, mass);
+    if (st == SD_OK) st = dalloc(&c->scenes_d, (size_t)nscenes);
+    if (st == SD_OK) st = dalloc(&c->scene_of, (size_t)n);
+    if (st != SD_OK) {
+        release(c);
+        return st;
+    }
+    c->nscenes = nscenes;
+    c->scenes_h = table;
+    std::vector<int> sof((size_t)n);
+    for (int k = 0; k < nscenes; ++k)
+        for (int i = 0; i < table[k].n; ++i) sof[(size_t)table[k].first + i] = k;
+    SD_HIP(hipMemcpyAsync(c->scenes_d, table.data(), nscenes * sizeof(SceneDev), hipMemcpyHostToDevice, c->stream));
+    SD_HIP(hipMemcpyAsync(c->scene_of, sof.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    st = launch_hash_keys_batch(c->pos[0], (int)n, c->scenes_d, c->scene_of, c->k64[0], c->stream);
+    if (st != SD_OK) return st;
+    SD_HIP(hipStreamSynchronize(c->stream));
     return SD_OK;
 }
 
@@ -442,6 +567,138 @@ extern "C" sd_status sd_ctx_profile(sd_ctx* c, int nsteps, float* stage_ms, int 
     for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
     if (st == SD_OK) c->steps_done += nsteps;
     return st;
+}
+
+// ------------------------------------------------------------------ autotuner
+//
+// Sweeps the launch shapes of the step (radix-sort items per thread x contact
+// kernel block size), timing trial_steps eager steps of each with HIP events on
+// the context stream, and keeps the fastest. Every output of the step is
+// independent of these shapes (per-sphere gathers, stable sort), so the choice
+// changes time only; the state is snapshotted on the device before the trials
+// and restored after them, so the simulation does not advance.
+
+namespace {
+
+struct Snapshot {
+    char* buf = nullptr;
+    size_t off[8] = {};
+    size_t sz[8] = {};
+    int roles[3] = {0, 0, 0};
+};
+
+// Role buffers of the current state: pos, vel, omg, ids, keys (k64 or k32), v32, hcount, hist.
+void role_buffers(sd_ctx* c, void* b[8], size_t sz[8]) {
+    const size_t n = (size_t)c->n;
+    const int cs = c->cur_state, ck = c->cur_keys, ch = c->cur_hist;
+    b[0] = c->pos[cs];
+    sz[0] = n * sizeof(float4);
+    b[1] = c->vel[cs];
+    sz[1] = n * sizeof(float4);
+    b[2] = c->omg[cs];
+    sz[2] = n * sizeof(float4);
+    b[3] = c->ids[cs];
+    sz[3] = n * sizeof(int);
+    b[4] = c->p.key64 ? (void*)c->k64[ck] : (void*)c->k32[ck];
+    sz[4] = n * (c->p.key64 ? sizeof(uint64_t) : sizeof(uint32_t));
+    b[5] = c->p.key64 ? nullptr : (void*)c->v32[ck];
+    sz[5] = c->p.key64 ? 0 : n * sizeof(uint32_t);
+    b[6] = c->hcount[ch];
+    sz[6] = n * sizeof(int);
+    b[7] = c->hist[ch];
+    sz[7] = (size_t)c->p.contact.max_contacts * n * sizeof(float4);
+}
+
+sd_status snapshot_take(sd_ctx* c, Snapshot& sn) {
+    void* b[8];
+    size_t total = 0;
+    role_buffers(c, b, sn.sz);
+    for (int k = 0; k < 8; ++k) {
+        sn.off[k] = total;
+        total += (sn.sz[k] + 255) & ~(size_t)255;
+    }
+    SD_HIP(hipMalloc((void**)&sn.buf, total));
+    for (int k = 0; k < 8; ++k)
+        if (sn.sz[k]) SD_HIP(hipMemcpyAsync(sn.buf + sn.off[k], b[k], sn.sz[k], hipMemcpyDeviceToDevice, c->stream));
+    sn.roles[0] = c->cur_keys;
+    sn.roles[1] = c->cur_hist;
+    sn.roles[2] = c->cur_state;
+    return SD_OK;
+}
+
+sd_status snapshot_restore(sd_ctx* c, const Snapshot& sn) {
+    c->cur_keys = sn.roles[0];
+    c->cur_hist = sn.roles[1];
+    c->cur_state = sn.roles[2];
+    void* b[8];
+    size_t sz[8];
+    role_buffers(c, b, sz);
+    for (int k = 0; k < 8; ++k)
+        if (sz[k]) SD_HIP(hipMemcpyAsync(b[k], sn.buf + sn.off[k], sz[k], hipMemcpyDeviceToDevice, c->stream));
+    return launch_fill_u32((uint32_t*)c->overflow, 0u, 1, c->stream);
+}
+
+}  // namespace
+
+extern "C" sd_status sd_ctx_autotune(sd_ctx* c, int trial_steps, int* tile_items, int* contact_block,
+                                     float* best_ms) {
+    SD_REQUIRE(c && trial_steps >= 1, "sd_ctx_autotune: bad argument");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_autotune: no spheres uploaded");
+    const int items_c[3] = {4, 8, 16};
+    const int block_c[3] = {64, 128, 256};
+    Snapshot sn;
+    sd_status st = snapshot_take(c, sn);
+    if (st != SD_OK) {
+        if (sn.buf) (void)hipFree(sn.buf);
+        return st;
+    }
+    const long long steps0 = c->steps_done;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float best = 3.4e38f;
+    int bi = c->items, bb = c->contact_block;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = SD_ERR_HIP;
+    for (int i = 0; i < 3 && st == SD_OK; ++i) {
+        for (int j = 0; j < 3 && st == SD_OK; ++j) {
+            c->items = items_c[i];
+            c->contact_block = block_c[j];
+            st = snapshot_restore(c, sn);
+            if (st == SD_OK) st = enqueue_step(c);  // warm-up
+            if (st != SD_OK) break;
+            if (hipEventRecord(e0, c->stream) != hipSuccess) {
+                st = set_error(SD_ERR_HIP, "sd_ctx_autotune: event record failed");
+                break;
+            }
+            for (int k = 0; k < trial_steps && st == SD_OK; ++k) st = enqueue_step(c);
+            if (st != SD_OK) break;
+            float ms = 0.0f;
+            if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                st = set_error(SD_ERR_HIP, "sd_ctx_autotune: event timing failed");
+                break;
+            }
+            if (ms < best) {
+                best = ms;
+                bi = items_c[i];
+                bb = block_c[j];
+            }
+        }
+    }
+    c->items = bi;
+    c->contact_block = bb;
+    sd_status st2 = snapshot_restore(c, sn);
+    if (hipStreamSynchronize(c->stream) != hipSuccess && st2 == SD_OK)
+        st2 = set_error(SD_ERR_HIP, "sd_ctx_autotune: stream synchronisation failed");
+    (void)hipFree(sn.buf);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    drop_graphs(c);  // captured with other launch shapes
+    c->steps_done = steps0;
+    if (st != SD_OK) return st;
+    if (st2 != SD_OK) return st2;
+    if (tile_items) *tile_items = bi;
+    if (contact_block) *contact_block = bb;
+    if (best_ms) *best_ms = best / (float)trial_steps;
+    return SD_OK;
 }
 
 extern "C" sd_status sd_ctx_download_spheres(sd_ctx* c, float* x, float* y, float* z, float* vx, float* vy, float* vz,

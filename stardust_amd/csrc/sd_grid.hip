@@ -90,6 +90,23 @@ sd_status launch_hash_keys_u64(const float4* pos4, int n, const GridDev& g, uint
     return SD_OK;
 }
 
+__global__ void k_hash_pos4_batch(const float4* __restrict__ p, int n, const SceneDev* __restrict__ scenes,
+                                  const int* __restrict__ scene_of, uint64_t* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = p[i];
+    const SceneDev& S = scenes[scene_of[i]];
+    keys[i] = ((uint64_t)(S.cbase + cell_id(S.g, q.x, q.y, q.z)) << 32) | (uint64_t)(uint32_t)i;
+}
+
+sd_status launch_hash_keys_batch(const float4* pos4, int n, const SceneDev* scenes, const int* scene_of,
+                                 uint64_t* keys, hipStream_t s) {
+    if (n <= 0) return SD_OK;
+    hipLaunchKernelGGL(k_hash_pos4_batch, dim3(div_up(n, 256)), dim3(256), 0, s, pos4, n, scenes, scene_of, keys);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
 int grid_bits(const sd_grid& g) {
     const uint64_t cells = (uint64_t)g.n[0] * (uint64_t)g.n[1] * (uint64_t)g.n[2];
     int b = 0;

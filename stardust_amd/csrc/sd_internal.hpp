@@ -46,6 +46,10 @@ sd_status launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t count, hipS
 sd_status launch_hash_keys_u32(const float4* pos4, int n, const GridDev& g, uint32_t* keys, uint32_t* vals,
                                hipStream_t s);
 sd_status launch_hash_keys_u64(const float4* pos4, int n, const GridDev& g, uint64_t* keys, hipStream_t s);
+struct SceneDev;
+// keys[i] = (scene cbase + cell of pos4[i] in the scene grid) << 32 | i
+sd_status launch_hash_keys_batch(const float4* pos4, int n, const SceneDev* scenes, const int* scene_of,
+                                 uint64_t* keys, hipStream_t s);
 sd_status launch_cell_ranges_u32(const uint32_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
 sd_status launch_cell_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
 // perm is either u32 values (perm32) or the low halves of u64 keys (perm64).
@@ -67,8 +71,21 @@ struct ContactDev {
 ContactDev to_dev(const sd_contact_params& c);
 sd_status check_contact(const sd_contact_params* c);
 
-// Fused step kernel modes.
-enum { CONTACT_FORCES = 0, CONTACT_FUSED_U32 = 1, CONTACT_FUSED_U64 = 2 };
+// Fused step kernel modes. CONTACT_FUSED_BATCH: packed u64 keys, every sphere
+// takes its grid, contact law and cell-id base from its scene's SceneDev.
+enum { CONTACT_FORCES = 0, CONTACT_FUSED_U32 = 1, CONTACT_FUSED_U64 = 2, CONTACT_FUSED_BATCH = 3 };
+
+// One scene of a batch: its grid, its contact law, the first global cell id of
+// its cells and its slot range [first, first + n) (sorted positions of a batch
+// keep scenes contiguous because cell ids are scene-major).
+struct SceneDev {
+    GridDev g;
+    ContactDev c;
+    uint32_t cbase;
+    int first;
+    int n;
+    int pad;
+};
 
 struct ContactArgs {
     const float4* pos;  // sorted state
@@ -99,6 +116,9 @@ struct ContactArgs {
     uint32_t* keys_o32;
     uint32_t* vals_o32;
     uint64_t* keys_o64;
+    // CONTACT_FUSED_BATCH: scene table and scene of every slot
+    const SceneDev* scenes;
+    const int* scene_of;
 };
 sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int block);
 sd_status launch_integrate(float4* pos, float4* vel, float4* omg, const float4* f, const float4* t, int n,

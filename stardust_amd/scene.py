@@ -63,6 +63,12 @@ class SphereScene:
         return {"momentum": out[0:3].copy(), "angular_momentum": out[3:6].copy(), "kinetic": float(out[6]),
                 "potential": float(out[7])}
 
+    def autotune(self, trial_steps=4):
+        """Sweeps sort tile size x contact block size; keeps the fastest (state unchanged)."""
+        ti, cb, ms = C.c_int(), C.c_int(), C.c_float()
+        L.call("sd_ctx_autotune", self.h, int(trial_steps), C.byref(ti), C.byref(cb), C.byref(ms))
+        return {"tile_items": ti.value, "contact_block": cb.value, "ms_per_step": ms.value}
+
     def stream(self):
         s = C.c_void_p()
         L.call("sd_ctx_stream", self.h, C.byref(s))
@@ -95,3 +101,41 @@ class SphereScene:
             self.close()
         except Exception:
             pass
+
+
+class BatchScene(SphereScene):
+    """Many independent sphere scenes stepped as one (wraps sd_batch_create /
+    sd_batch_upload; stepping, profiling, download and checkpoint are the
+    sd_ctx_* calls of SphereScene). Arrays are concatenated in scene order."""
+
+    def __init__(self, use_graph=True, tile_items=0, contact_block=0):
+        L.require_gpu()
+        self.params = L.BatchParams()
+        self.params.use_graph = 1 if use_graph else 0
+        self.params.tile_items = tile_items
+        self.params.contact_block = contact_block
+        h = C.c_void_p()
+        L.call("sd_batch_create", C.byref(self.params), C.byref(h))
+        self.h = h
+        self.n = 0
+        self.counts = []
+
+    def upload_scenes(self, scenes, x, y, z, radius, mass, vx=None, vy=None, vz=None):
+        """scenes: list of (n, Grid, ContactParams)."""
+        desc = (L.SceneDesc * len(scenes))()
+        for k, (n, g, c) in enumerate(scenes):
+            desc[k].n = int(n)
+            desc[k].grid = g
+            desc[k].contact = c
+        n = len(x)
+        assert n == sum(int(sc[0]) for sc in scenes)
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        zero = np.zeros(n, np.float32)
+        arrs = [f(x), f(y), f(z), f(vx if vx is not None else zero), f(vy if vy is not None else zero),
+                f(vz if vz is not None else zero), f(radius), f(mass)]
+        L.call("sd_batch_upload", self.h, len(scenes), C.cast(desc, C.c_void_p), *[L.ptr(a) for a in arrs])
+        self.n = n
+        self.counts = [int(sc[0]) for sc in scenes]
+
+    def upload(self, *a, **k):
+        raise TypeError("BatchScene takes upload_scenes(scenes, x, y, z, radius, mass)")

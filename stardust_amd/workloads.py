@@ -13,6 +13,15 @@ dem_c2: plain DEM at scale. 4,194,304 polydisperse spheres, radius lognormal
         widened to 2.0 x 2.0 m (solid fraction 0.18 at seeding, where random
         sequential addition still finishes in about 20 s); the walled domain is
         the whole column.
+batch_c5: 512 independent scenes in one batch. Per scene (numpy PCG64 seeded
+        with 0x5EED0005): sphere count log-uniform in [1,024, 65,536]; box
+        footprint side U[0.2, 1] m; radius lognormal with median U[2, 8] mm,
+        sigma 0.25, clipped to [0.4, 2.4] x median; kn log-uniform in
+        [5e4, 5e5] N/m; zeta = 0.3, mu = 0.5, dt = 1e-5 s, 1,000 steps; cell =
+        largest diameter. A footprint side of 0.2 m cannot hold 65,536 spheres
+        of 8 mm, so each box is a walled column whose height is the larger of
+        the side and the height that seeds the spheres at solid fraction 0.15.
+        Sphere positions: sd_gen_spheres with seed 0x5EED0005 + 1 + scene.
 Seeds are 0x5EED0000 + k for configuration k (1-based).
 """
 import math
@@ -81,3 +90,51 @@ def dem_c1(n=65536):
 def dem_c2(n=4194304):
     return Workload("dem_c2_column_4194304", n, (0.0, 0.0, 0.0), (2.0, 2.0, 4.0), 0.005, 0.25, 0.002, 0.012, 2e5,
                     5e-6, 2000, True, 0x5EED0002, 24)
+
+
+class BatchWorkload:
+    """A batch of independent sphere scenes (see batch_c5)."""
+
+    def __init__(self, name, nscenes, seed, n_min=1024, n_max=65536, side=(0.2, 1.0), r_med=(0.002, 0.008),
+                 kn=(5e4, 5e5), r_sigma=0.25, phi_seed=0.15, dt=1e-5, steps=1000, max_contacts=16):
+        self.name, self.nscenes, self.seed, self.dt, self.steps = name, nscenes, seed, dt, steps
+        self.max_contacts = max_contacts
+        rng = np.random.Generator(np.random.PCG64(seed))
+        self.scene_n = np.rint(np.exp(rng.uniform(math.log(n_min), math.log(n_max), nscenes))).astype(np.int64)
+        self.side = rng.uniform(side[0], side[1], nscenes)
+        self.r_med = rng.uniform(r_med[0], r_med[1], nscenes)
+        self.kn = np.exp(rng.uniform(math.log(kn[0]), math.log(kn[1]), nscenes))
+        self.r_sigma, self.phi_seed = r_sigma, phi_seed
+        solid = self.scene_n * (4.0 / 3.0) * math.pi * self.r_med ** 3 * math.exp(4.5 * r_sigma ** 2)
+        self.height = np.maximum(self.side, solid / (phi_seed * self.side ** 2))
+        self.n = int(self.scene_n.sum())
+
+    def scene(self, k):
+        lo = (0.0, 0.0, 0.0)
+        hi = (float(self.side[k]), float(self.side[k]), float(self.height[k]))
+        r_max = 2.4 * float(self.r_med[k])
+        g = grid_for(lo, hi, 2.0 * r_max)
+        c = L.ContactParams.make(float(self.kn[k]), 0.3, 0.5, self.dt, gravity=(0.0, 0.0, -9.81), walls=(lo, hi),
+                                 kt_ratio=KT_RATIO, max_contacts=self.max_contacts)
+        return int(self.scene_n[k]), g, c, lo, hi
+
+    def cells(self):
+        return sum(self.scene(k)[1].n_cells for k in range(self.nscenes))
+
+    def generate_scene(self, k):
+        n, g, c, lo, hi = self.scene(k)
+        rm = float(self.r_med[k])
+        x, y, z, r = _gen(self.seed + 1 + k, n, lo, hi, rm, self.r_sigma, 0.4 * rm, 2.4 * rm)
+        return {"x": x, "y": y, "z": z, "r": r, "m": sphere_mass(r)}
+
+    def generate(self, scenes=None):
+        """Concatenated arrays of the given scenes (default: all) and their descriptors."""
+        ks = range(self.nscenes) if scenes is None else scenes
+        parts = [self.generate_scene(k) for k in ks]
+        cat = {key: np.concatenate([p[key] for p in parts]) for key in ("x", "y", "z", "r", "m")}
+        desc = [self.scene(k)[:3] for k in ks]
+        return cat, desc
+
+
+def batch_c5(nscenes=512):
+    return BatchWorkload("batch_c5_512_scenes", nscenes, 0x5EED0005)

@@ -1,0 +1,150 @@
+"""Batched multi-scene stepping on the GPU: every scene of a batch is
+bit-identical to the same scene stepped alone (sd_ctx, packed u64 keys) and to
+the oracle; the autotuner changes launch shapes but no result."""
+import sys
+
+import numpy as np
+import pytest
+
+from stardust_amd import workloads as W
+from stardust_amd.scene import BatchScene, SphereScene
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "wx", "wy", "wz")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_batch(nscenes=12, seed=0x5EEDB001):
+    # dense columns with random velocities so that contacts form within a few hundred steps
+    return W.BatchWorkload("test_batch", nscenes, seed, n_min=200, n_max=3000, side=(0.04, 0.12),
+                           r_med=(0.002, 0.005), kn=(5e4, 5e5), phi_seed=0.3)
+
+
+def velocities(n, seed, sigma=1.5):
+    rng = np.random.default_rng(seed)
+    return [rng.normal(0.0, sigma, n).astype(np.float32) for _ in range(3)]
+
+
+def batch_of(w, scenes, s, v, **kw):
+    b = BatchScene(**kw)
+    b.upload_scenes(scenes, s["x"], s["y"], s["z"], s["r"], s["m"], vx=v[0], vy=v[1], vz=v[2])
+    return b
+
+
+def solo(scene, s, v, lo, hi, graph=True):
+    n, g, c = scene
+    sc = SphereScene(g, c, key64=True, use_graph=graph)
+    sc.upload(s["x"][lo:hi], s["y"][lo:hi], s["z"][lo:hi], s["r"][lo:hi], s["m"][lo:hi], vx=v[0][lo:hi],
+              vy=v[1][lo:hi], vz=v[2][lo:hi])
+    return sc
+
+
+def assert_same(a, b, what):
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), (what, k, np.max(np.abs(a[k] - b[k])))
+
+
+def test_batch_equals_each_scene_alone():
+    say("batch vs solo, 12 scenes")
+    w = small_batch()
+    s, scenes = w.generate()
+    v = velocities(len(s["x"]), 1)
+    b = batch_of(w, scenes, s, v)
+    steps = 150
+    b.step(steps)
+    got = b.download()
+    assert b.contact_count() > 0
+    off = 0
+    for k, sc_desc in enumerate(scenes):
+        n = sc_desc[0]
+        sc = solo(sc_desc, s, v, off, off + n)
+        sc.step(steps)
+        ref = sc.download()
+        assert_same({f: got[f][off:off + n] for f in FIELDS}, ref, "scene %d" % k)
+        sc.close()
+        off += n
+    b.close()
+
+
+def test_batch_scene_equals_oracle(oracle):
+    say("batch vs oracle, 3 scenes")
+    w = small_batch(nscenes=3, seed=0x5EEDB002)
+    s, scenes = w.generate()
+    v = velocities(len(s["x"]), 2)
+    b = batch_of(w, scenes, s, v, use_graph=False)
+    b.step(40)
+    got = b.download()
+    off = 0
+    for k, (n, g, c) in enumerate(scenes):
+        sl = slice(off, off + n)
+        st = oracle.SphereState(s["x"][sl], s["y"][sl], s["z"][sl], s["r"][sl], s["m"][sl], vx=v[0][sl],
+                                vy=v[1][sl], vz=v[2][sl], max_contacts=c.max_contacts)
+        st.step(g, c, nsteps=40, nthreads=8)
+        e = st.by_id()
+        for f, (arr, col) in {"x": ("pos4", 0), "y": ("pos4", 1), "z": ("pos4", 2), "vx": ("vel4", 0),
+                              "vy": ("vel4", 1), "vz": ("vel4", 2), "wx": ("omg4", 0), "wy": ("omg4", 1),
+                              "wz": ("omg4", 2)}.items():
+            assert np.array_equal(got[f][sl], e[arr][:, col]), ("scene", k, f)
+        off += n
+    b.close()
+
+
+def test_autotune_changes_no_result():
+    say("autotune determinism")
+    w = small_batch(nscenes=6, seed=0x5EEDB003)
+    s, scenes = w.generate()
+    v = velocities(len(s["x"]), 3)
+    a = batch_of(w, scenes, s, v)
+    b = batch_of(w, scenes, s, v)
+    a.step(10)
+    b.step(10)
+    choice = b.autotune(trial_steps=3)
+    assert choice["tile_items"] in (4, 8, 16) and choice["contact_block"] in (64, 128, 256)
+    assert choice["ms_per_step"] > 0
+    assert_same(a.download(), b.download(), "after autotune")  # the state was restored
+    a.step(30)
+    b.step(30)
+    assert_same(a.download(), b.download(), "stepping with the tuned shapes")
+    a.close()
+    b.close()
+
+
+def test_batch_rejects_mixed_dt():
+    say("batch argument checks")
+    from stardust_amd import _lib as L
+    w = small_batch(nscenes=2, seed=0x5EEDB004)
+    s, scenes = w.generate()
+    n0, g0, c0 = scenes[0]
+    c_bad = L.ContactParams.make(c0.kn, 0.3, 0.5, c0.dt * 2, walls=None, max_contacts=c0.max_contacts)
+    b = BatchScene()
+    with pytest.raises(L.SdError):
+        b.upload_scenes([scenes[0], (scenes[1][0], scenes[1][1], c_bad)], s["x"], s["y"], s["z"], s["r"], s["m"])
+    b.close()
+
+
+def test_batch_c5_full_size_spot_checks():
+    say("batch c5 full size")
+    w = W.batch_c5()
+    s, scenes = w.generate()
+    zero = [np.zeros(len(s["x"]), np.float32)] * 3
+    b = batch_of(w, scenes, s, zero)
+    steps = 20
+    b.step(steps)
+    got = b.download()
+    for f in FIELDS:
+        assert np.all(np.isfinite(got[f])), f
+    firsts = np.concatenate([[0], np.cumsum([d[0] for d in scenes])])
+    ns = np.array([d[0] for d in scenes])
+    picks = sorted({int(np.argmin(ns)), int(np.argmax(ns)), int(np.argmax(w.height / w.side)), 255})
+    for k in picks:
+        lo, hi = int(firsts[k]), int(firsts[k + 1])
+        sc = solo(scenes[k], s, zero, lo, hi)
+        sc.step(steps)
+        assert_same({f: got[f][lo:hi] for f in FIELDS}, sc.download(), "c5 scene %d" % k)
+        sc.close()
+    b.close()
