@@ -286,9 +286,13 @@ sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int* pairs, in
 #define SD_GJK_EPA_LIMIT  8
 #define SD_GJK_DEGENERATE 16
 #define SD_GJK_WARM_USED  32
+/* scratch: sd_gjk_scratch_bytes(n_pairs) bytes of device memory (the list of
+ * intersecting pairs handed from the GJK kernel to the EPA kernel). */
+sd_status sd_gjk_scratch_bytes(int n_pairs, size_t* bytes);
 sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, const float* quat4, const int* vstart,
                  const int* nv, const float* verts4, int n_hulls, int n_verts, const float* warm_dir, int warm_budget,
-                 int max_iter, float near_margin, float* sep, float* normal3, int* flags, int* iters, void* stream);
+                 int max_iter, float near_margin, float* sep, float* normal3, int* flags, int* iters, void* scratch,
+                 size_t scratch_bytes, void* stream);
 
 /* ------------------------------------------------------- LBVH broadphase
  * Replaces the sandbox LBVH pipeline (sandbox/testLBVH.cu:42-954).

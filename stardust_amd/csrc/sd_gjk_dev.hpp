@@ -219,12 +219,30 @@ __device__ __forceinline__ bool origin_outside(V3 a, V3 b, V3 c, V3 d) {
 // Closest point of tetrahedron w[0..3] to the origin; *inside when it contains it.
 // A near-flat tetrahedron (|6 V| <= 1e-5 |ab| |ac| |ad|) has unreliable side
 // tests; it is never reported as containing the origin, and all four faces are
-// searched instead.
+// searched instead. Faces in search order: (0,1,2) opposite 3, (0,2,3)
+// opposite 1, (0,3,1) opposite 2, (1,3,2) opposite 0. Every index is a
+// compile-time constant so the simplex stays in registers.
+__device__ __forceinline__ void tetra_face(const MPoint& a, const MPoint& b, const MPoint& c, const MPoint& opp,
+                                           bool flat, float& best, V3& bestp, Simplex& bests, bool& any) {
+    if (!flat && !origin_outside(a.p, b.p, c.p, opp.p)) return;
+    any = true;
+    Simplex t;
+    t.w[0] = a;
+    t.w[1] = b;
+    t.w[2] = c;
+    t.n = 3;
+    const V3 q = closest_triangle(t);
+    const float dq = dot(q, q);
+    if (dq < best) {
+        best = dq;
+        bestp = q;
+        bests = t;
+    }
+}
+
 __device__ __forceinline__ V3 closest_tetra(Simplex& s, bool* inside) {
-    const MPoint P[4] = {s.w[0], s.w[1], s.w[2], s.w[3]};
-    // faces (0,1,2) opp 3, (0,2,3) opp 1, (0,3,1) opp 2, (1,3,2) opp 0
-    const int F[4][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 3, 2, 0}};
-    const V3 ab = sub(P[1].p, P[0].p), ac = sub(P[2].p, P[0].p), ad = sub(P[3].p, P[0].p);
+    const MPoint P0 = s.w[0], P1 = s.w[1], P2 = s.w[2], P3 = s.w[3];
+    const V3 ab = sub(P1.p, P0.p), ac = sub(P2.p, P0.p), ad = sub(P3.p, P0.p);
     const float vol = dot(ad, cross(ab, ac));
     const float lim = 1e-5f * (sqrtf(dot(ab, ab)) * sqrtf(dot(ac, ac)) * sqrtf(dot(ad, ad)));
     const bool flat = !(fabsf(vol) > lim);
@@ -232,22 +250,10 @@ __device__ __forceinline__ V3 closest_tetra(Simplex& s, bool* inside) {
     V3 bestp = v3(0.0f, 0.0f, 0.0f);
     Simplex bests = s;
     bool any = false;
-    for (int f = 0; f < 4; ++f) {
-        if (!flat && !origin_outside(P[F[f][0]].p, P[F[f][1]].p, P[F[f][2]].p, P[F[f][3]].p)) continue;
-        any = true;
-        Simplex t;
-        t.w[0] = P[F[f][0]];
-        t.w[1] = P[F[f][1]];
-        t.w[2] = P[F[f][2]];
-        t.n = 3;
-        const V3 q = closest_triangle(t);
-        const float dq = dot(q, q);
-        if (dq < best) {
-            best = dq;
-            bestp = q;
-            bests = t;
-        }
-    }
+    tetra_face(P0, P1, P2, P3, flat, best, bestp, bests, any);
+    tetra_face(P0, P2, P3, P1, flat, best, bestp, bests, any);
+    tetra_face(P0, P3, P1, P2, flat, best, bestp, bests, any);
+    tetra_face(P1, P3, P2, P0, flat, best, bestp, bests, any);
     *inside = !any;
     if (any) s = bests;
     return bestp;
@@ -263,17 +269,42 @@ __device__ __forceinline__ V3 simplex_closest(Simplex& s, bool* inside) {
 }
 
 // Sorts the simplex by vertex key so that the same feature always yields the
-// same arithmetic regardless of the order its vertices were found in.
-__device__ __forceinline__ void simplex_canonical(Simplex& s) {
-    for (int i = 1; i < s.n; ++i) {
-        MPoint x = s.w[i];
-        int j = i - 1;
-        while (j >= 0 && s.w[j].key > x.key) {
-            s.w[j + 1] = s.w[j];
-            --j;
-        }
-        s.w[j + 1] = x;
+// same arithmetic regardless of the order its vertices were found in. Keys of
+// one simplex are distinct; unused entries sort last. A fixed compare-exchange
+// network keeps every index constant.
+__device__ __forceinline__ void cmpx(MPoint& a, MPoint& b, int& ka, int& kb) {
+    if (ka > kb) {
+        const MPoint t = a;
+        a = b;
+        b = t;
+        const int k = ka;
+        ka = kb;
+        kb = k;
     }
+}
+__device__ __forceinline__ void simplex_canonical(Simplex& s) {
+    int k0 = s.w[0].key, k1 = s.n > 1 ? s.w[1].key : 0x7fffffff, k2 = s.n > 2 ? s.w[2].key : 0x7fffffff,
+        k3 = s.n > 3 ? s.w[3].key : 0x7fffffff;
+    cmpx(s.w[0], s.w[1], k0, k1);
+    cmpx(s.w[2], s.w[3], k2, k3);
+    cmpx(s.w[0], s.w[2], k0, k2);
+    cmpx(s.w[1], s.w[3], k1, k3);
+    cmpx(s.w[1], s.w[2], k1, k2);
+}
+
+__device__ __forceinline__ void simplex_push(Simplex& s, const MPoint& w) {
+    if (s.n == 0) s.w[0] = w;
+    else if (s.n == 1) s.w[1] = w;
+    else if (s.n == 2) s.w[2] = w;
+    else s.w[3] = w;
+    ++s.n;
+}
+
+__device__ __forceinline__ bool simplex_has(const Simplex& s, int key) {
+    bool dup = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dup = dup || (i < s.n && key >= 0 && s.w[i].key == key);
+    return dup;
 }
 
 // ---------------------------------------------------------------- EPA
@@ -464,68 +495,74 @@ __device__ inline void epa(const Shape& A, const Shape& B, const Simplex& s0, in
     o.epa_iters = it;
 }
 
-// GJK distance query. init_dir: initial search direction (warm start or the
-// centre difference); the first point is support_M(init_dir).
-__device__ inline GjkOut gjk_query(const Shape& A, const Shape& B, V3 init_dir, int max_iter) {
-    GjkOut o;
-    o.flags = 0;
-    o.iters = 0;
-    o.epa_iters = 0;
-    o.sep = 0.0f;
-    o.n = v3(1.0f, 0.0f, 0.0f);
+// GJK phase of a distance query. init_dir: initial search direction (warm
+// start or the centre difference); the first point is support_M(init_dir).
+// On return s is the final feature, v its closest point, *hit whether the
+// shapes intersect (then s encloses or touches the origin and EPA follows).
+__device__ inline void gjk_phase(const Shape& A, const Shape& B, V3 init_dir, int max_iter, Simplex& s, V3& v,
+                                 bool* hit, bool* converged, int* iters) {
     if (dot(init_dir, init_dir) == 0.0f) init_dir = v3(1.0f, 0.0f, 0.0f);
-    Simplex s;
     s.w[0] = support_M(A, B, init_dir);
     s.n = 1;
-    V3 v = s.w[0].p;
-    bool hit = false, converged = false;
+    v = s.w[0].p;
+    *hit = false;
+    *converged = false;
     int it = 0;
     while (it < max_iter) {
         const float vv = dot(v, v);
         if (vv <= GJK_TOUCH_EPS2) {
-            hit = true;
+            *hit = true;
             break;
         }
         const MPoint w = support_M(A, B, neg(v));
         ++it;
         const float vw = dot(v, w.p);
         if (vv - vw <= GJK_REL_EPS * vv) {
-            converged = true;
+            *converged = true;
             break;
         }
-        bool dup = false;
-        for (int i = 0; i < s.n; ++i) dup = dup || (w.key >= 0 && s.w[i].key == w.key);
-        if (dup) {
-            converged = true;
+        if (simplex_has(s, w.key)) {
+            *converged = true;
             break;
         }
         const Simplex sp = s;
-        s.w[s.n++] = w;
+        simplex_push(s, w);
         bool inside = false;
         const V3 vn = simplex_closest(s, &inside);
         if (inside) {
-            hit = true;
+            *hit = true;
             break;
         }
         if (!(dot(vn, vn) < vv)) {
             // no progress at FP32 resolution: the previous feature is final if the
             // duality gap certifies it to 1e-4, otherwise the query failed
             s = sp;
-            converged = vv - vw <= GJK_STALL_EPS * vv;
+            *converged = vv - vw <= GJK_STALL_EPS * vv;
             break;
         }
         v = vn;
     }
-    o.iters = it;
-    if (hit) {
-        epa(A, B, s, max_iter, o);
-        return o;
-    }
+    *iters = it;
+}
+
+__device__ __forceinline__ GjkOut gjk_out_init() {
+    GjkOut o;
+    o.flags = 0;
+    o.iters = 0;
+    o.epa_iters = 0;
+    o.sep = 0.0f;
+    o.n = v3(1.0f, 0.0f, 0.0f);
+    return o;
+}
+
+// Result of a separated query: distance and normal recomputed canonically from
+// the converged feature.
+__device__ inline void gjk_separated(Simplex& s, V3 v, bool converged, GjkOut& o) {
     if (!converged) o.flags |= GJK_BUDGET;
     else o.flags |= GJK_CONVERGED;
-    // canonical recomputation from the converged feature
     bool canon = true;
-    for (int i = 0; i < s.n; ++i) canon = canon && s.w[i].key >= 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) canon = canon && (i >= s.n || s.w[i].key >= 0);
     if (canon) {
         simplex_canonical(s);
         bool inside = false;
@@ -534,6 +571,17 @@ __device__ inline GjkOut gjk_query(const Shape& A, const Shape& B, V3 init_dir, 
     const float dist = sqrtf(dot(v, v));
     o.sep = dist;
     o.n = dist > 0.0f ? scl(v, -1.0f / dist) : v3(1.0f, 0.0f, 0.0f);
+}
+
+// Complete distance / depth query (GJK, then EPA on a hit).
+__device__ inline GjkOut gjk_query(const Shape& A, const Shape& B, V3 init_dir, int max_iter) {
+    GjkOut o = gjk_out_init();
+    Simplex s;
+    V3 v;
+    bool hit, converged;
+    gjk_phase(A, B, init_dir, max_iter, s, v, &hit, &converged, &o.iters);
+    if (hit) epa(A, B, s, max_iter, o);
+    else gjk_separated(s, v, converged, o);
     return o;
 }
 

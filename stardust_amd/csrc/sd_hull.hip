@@ -196,21 +196,63 @@ __device__ __forceinline__ void make_shapes(const HullSet& h, int ia, int ib, Sh
     B.off = dc;
 }
 
-// One thread per pair. warm (nullable): per-pair initial direction (A -> B).
-// A warm-started query whose result is not converged within warm_budget
-// iterations, or lies within near_margin of contact, is redone cold with
-// max_iter iterations.
-__global__ void k_gjk(const int* __restrict__ pairs, int npairs, HullSet h, int n_hulls, int n_verts,
-                      const float* __restrict__ warm, int warm_budget, int max_iter, float near_margin,
-                      float* __restrict__ sep, float* __restrict__ normal, int* __restrict__ flags,
-                      int* __restrict__ iters) {
+// GJK/EPA runs as two kernels so that the GJK kernel keeps its simplex in
+// registers at full occupancy and the polytope arrays of EPA (kept in private
+// memory) are paid only by intersecting pairs:
+//   k_gjk_phase  one thread per pair: the GJK phase; separated pairs are
+//                finished, intersecting pairs append their final simplex to a
+//                list (order of the list is irrelevant: every entry is
+//                finished independently and written by pair index)
+//   k_epa        one thread per listed pair: EPA from that simplex, and for a
+//                warm-started entry whose result is rejected, the cold query.
+// warm (nullable): per-pair initial direction (A -> B). A warm-started result
+// that is not converged within warm_budget iterations, that EPA could not
+// finish cleanly, or that lies within near_margin of contact, is redone cold
+// with max_iter iterations.
+struct EpaEntry {
+    int4 meta;     // pair, EPA iteration limit, iterations so far, simplex size | mode << 8 (0 cold, 1 warm)
+    float4 w[4];   // simplex points (x, y, z, key bits)
+};
+
+__device__ __forceinline__ bool pair_is_bad(const HullSet& h, int ia, int ib, int n_hulls, int n_verts) {
+    if (ia < 0 || ia >= n_hulls || ib < 0 || ib >= n_hulls) return true;
+    return h.vstart[ia] + h.nv[ia] > n_verts || h.vstart[ib] + h.nv[ib] > n_verts;
+}
+
+__device__ __forceinline__ void write_out(int p, const GjkOut& o, int flags, int spent, float* __restrict__ sep,
+                                          float* __restrict__ normal, int* __restrict__ oflags,
+                                          int* __restrict__ iters) {
+    sep[p] = o.sep;
+    normal[3 * (size_t)p] = o.n.x;
+    normal[3 * (size_t)p + 1] = o.n.y;
+    normal[3 * (size_t)p + 2] = o.n.z;
+    oflags[p] = flags;
+    iters[p] = spent;
+}
+
+// The EPA of a query runs with that query's iteration limit (the warm budget
+// for a warm-started query, max_iter for a cold one).
+__device__ __forceinline__ void push_epa(EpaEntry* __restrict__ list, int* __restrict__ count, int p, int mode,
+                                         int limit, int spent, const Simplex& s) {
+    const int k = atomicAdd(count, 1);
+    EpaEntry e;
+    e.meta = make_int4(p, limit, spent, s.n | (mode << 8));
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        e.w[i] = make_float4(s.w[i].p.x, s.w[i].p.y, s.w[i].p.z, __int_as_float(i < s.n ? s.w[i].key : -1));
+    list[k] = e;
+}
+
+__global__ __launch_bounds__(64) void k_gjk_phase(const int* __restrict__ pairs, int npairs, HullSet h, int n_hulls,
+                                                  int n_verts, const float* __restrict__ warm, int warm_budget,
+                                                  int max_iter, float near_margin, float* __restrict__ sep,
+                                                  float* __restrict__ normal, int* __restrict__ flags,
+                                                  int* __restrict__ iters, EpaEntry* __restrict__ list,
+                                                  int* __restrict__ count) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npairs) return;
     const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
-    const bool bad = ia < 0 || ia >= n_hulls || ib < 0 || ib >= n_hulls ||
-                     h.vstart[ia < 0 || ia >= n_hulls ? 0 : ia] + h.nv[ia < 0 || ia >= n_hulls ? 0 : ia] > n_verts ||
-                     h.vstart[ib < 0 || ib >= n_hulls ? 0 : ib] + h.nv[ib < 0 || ib >= n_hulls ? 0 : ib] > n_verts;
-    if (bad) {  // ids or vertex ranges outside the hull set: reported, never dereferenced
+    if (pair_is_bad(h, ia, ib, n_hulls, n_verts)) {  // reported, never dereferenced
         sep[p] = 0.0f;
         flags[p] = -1;
         iters[p] = 0;
@@ -219,27 +261,73 @@ __global__ void k_gjk(const int* __restrict__ pairs, int npairs, HullSet h, int 
     Shape A, B;
     V3 dc;
     make_shapes(h, ia, ib, A, B, dc);
-    GjkOut o;
-    int spent = 0;
-    bool done = false;
+    Simplex s;
+    V3 v;
+    bool hit, converged;
+    int it = 0, spent = 0;
     if (warm) {
         const V3 wd = v3(warm[3 * (size_t)p], warm[3 * (size_t)p + 1], warm[3 * (size_t)p + 2]);
-        o = gjk_query(A, B, wd, warm_budget < max_iter ? warm_budget : max_iter);
-        spent = o.iters + o.epa_iters;
-        done = (o.flags & GJK_CONVERGED) && !(o.flags & (GJK_DEGENERATE | GJK_EPA_LIMIT)) &&
-               fabsf(o.sep) > near_margin;
-        if (done) o.flags |= GJK_WARM_USED;
+        const int budget = warm_budget < max_iter ? warm_budget : max_iter;
+        gjk_phase(A, B, wd, budget, s, v, &hit, &converged, &it);
+        if (hit) {
+            push_epa(list, count, p, 1, budget, it, s);
+            return;
+        }
+        GjkOut o = gjk_out_init();
+        gjk_separated(s, v, converged, o);
+        if (converged && fabsf(o.sep) > near_margin) {
+            write_out(p, o, o.flags | GJK_WARM_USED, it, sep, normal, flags, iters);
+            return;
+        }
+        spent = it;
     }
-    if (!done) {
+    gjk_phase(A, B, dc, max_iter, s, v, &hit, &converged, &it);
+    if (hit) {
+        push_epa(list, count, p, 0, max_iter, spent + it, s);
+        return;
+    }
+    GjkOut o = gjk_out_init();
+    gjk_separated(s, v, converged, o);
+    write_out(p, o, o.flags, spent + it, sep, normal, flags, iters);
+}
+
+__global__ __launch_bounds__(64) void k_epa(const int* __restrict__ pairs, HullSet h, int max_iter, float near_margin,
+                                            float* __restrict__ sep, float* __restrict__ normal,
+                                            int* __restrict__ flags, int* __restrict__ iters,
+                                            const EpaEntry* __restrict__ list, const int* __restrict__ count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *count) return;
+    const EpaEntry e = list[k];
+    const int p = e.meta.x;
+    const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
+    Shape A, B;
+    V3 dc;
+    make_shapes(h, ia, ib, A, B, dc);
+    Simplex s;
+    s.n = e.meta.w & 0xff;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        s.w[i].p = v3(e.w[i].x, e.w[i].y, e.w[i].z);
+        s.w[i].key = __float_as_int(e.w[i].w);
+    }
+    int spent = e.meta.z;
+    GjkOut o = gjk_out_init();
+    epa(A, B, s, e.meta.y, o);
+    spent += o.epa_iters;
+    if ((e.meta.w >> 8) == 1) {  // warm-started
+        if ((o.flags & GJK_CONVERGED) && !(o.flags & (GJK_DEGENERATE | GJK_EPA_LIMIT)) &&
+            fabsf(o.sep) > near_margin) {
+            write_out(p, o, o.flags | GJK_WARM_USED, spent, sep, normal, flags, iters);
+            return;
+        }
         o = gjk_query(A, B, dc, max_iter);
         spent += o.iters + o.epa_iters;
     }
-    sep[p] = o.sep;
-    normal[3 * (size_t)p] = o.n.x;
-    normal[3 * (size_t)p + 1] = o.n.y;
-    normal[3 * (size_t)p + 2] = o.n.z;
-    flags[p] = o.flags;
-    iters[p] = spent;
+    write_out(p, o, o.flags, spent, sep, normal, flags, iters);
+}
+
+size_t gjk_scratch_bytes(int npairs) {
+    return 256 + (size_t)(npairs > 0 ? npairs : 1) * sizeof(EpaEntry);
 }
 
 sd_status check_hulls(const float* pos4, const float* quat4, const int* vstart, const int* nv, const float* verts4) {
@@ -340,20 +428,37 @@ extern "C" sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int
     return SD_OK;
 }
 
+extern "C" sd_status sd_gjk_scratch_bytes(int n_pairs, size_t* bytes) {
+    SD_REQUIRE(bytes && n_pairs >= 0, "sd_gjk_scratch_bytes: bad argument");
+    *bytes = gjk_scratch_bytes(n_pairs);
+    return SD_OK;
+}
+
 extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, const float* quat4, const int* vstart,
                             const int* nv, const float* verts4, int n_hulls, int n_verts, const float* warm_dir,
                             int warm_budget, int max_iter, float near_margin, float* sep, float* normal3, int* flags,
-                            int* iters, void* stream) {
+                            int* iters, void* scratch, size_t scratch_bytes, void* stream) {
     SD_REQUIRE(n_pairs >= 0 && n_hulls >= 1 && n_verts >= 0, "sd_gjk: bad sizes");
     if (n_pairs == 0) return SD_OK;
     sd_status st = check_hulls(pos4, quat4, vstart, nv, verts4);
     if (st != SD_OK) return st;
-    SD_REQUIRE(pairs && sep && normal3 && flags && iters, "sd_gjk: null pointer");
+    SD_REQUIRE(pairs && sep && normal3 && flags && iters && scratch, "sd_gjk: null pointer");
+    SD_REQUIRE(scratch_bytes >= gjk_scratch_bytes(n_pairs), "sd_gjk: scratch too small (%zu < %zu)", scratch_bytes,
+               gjk_scratch_bytes(n_pairs));
     SD_REQUIRE(max_iter >= 1 && max_iter <= 1024, "sd_gjk: max_iter must be in [1, 1024]");
     SD_REQUIRE(warm_dir == nullptr || warm_budget >= 1, "sd_gjk: warm_budget must be >= 1");
     HullSet h{(const float4*)pos4, (const float4*)quat4, vstart, nv, (const float4*)verts4};
-    hipLaunchKernelGGL(k_gjk, dim3(div_up(n_pairs, 64)), dim3(64), 0, (hipStream_t)stream, pairs, n_pairs, h, n_hulls,
-                       n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters);
+    hipStream_t s = (hipStream_t)stream;
+    int* count = (int*)scratch;
+    EpaEntry* list = (EpaEntry*)((char*)scratch + 256);
+    st = launch_fill_u32((uint32_t*)count, 0u, 1, s);
+    if (st != SD_OK) return st;
+    const dim3 grid(div_up(n_pairs, 64));
+    hipLaunchKernelGGL(k_gjk_phase, grid, dim3(64), 0, s, pairs, n_pairs, h, n_hulls, n_verts, warm_dir, warm_budget,
+                       max_iter, near_margin, sep, normal3, flags, iters, list, count);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_epa, grid, dim3(64), 0, s, pairs, h, max_iter, near_margin, sep, normal3, flags, iters,
+                       (const EpaEntry*)list, (const int*)count);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

@@ -83,9 +83,13 @@ class DeviceHulls:
         if out is None:
             out = {"sep": L.DeviceBuffer(4 * npairs), "normal": L.DeviceBuffer(12 * npairs),
                    "flags": L.DeviceBuffer(4 * npairs), "iters": L.DeviceBuffer(4 * npairs)}
+        nb = C.c_size_t()
+        L.call("sd_gjk_scratch_bytes", npairs, C.byref(nb))
+        if getattr(self, "_gjk_scratch", None) is None or self._gjk_scratch.nbytes < nb.value:
+            self._gjk_scratch = L.DeviceBuffer(nb.value)
         L.call("sd_gjk", pairs.ptr, npairs, *self._args(), self.n, self.n_verts, warm.ptr if warm else None,
                warm_budget, max_iter, near_margin, out["sep"].ptr, out["normal"].ptr, out["flags"].ptr,
-               out["iters"].ptr, stream)
+               out["iters"].ptr, self._gjk_scratch.ptr, self._gjk_scratch.nbytes, stream)
         return out
 
     @staticmethod
