@@ -331,6 +331,9 @@ typedef struct sd_clump_params {
 typedef struct sd_clump_ctx sd_clump_ctx;
 sd_status sd_clump_create(const sd_clump_params* p, sd_clump_ctx** ctx);
 sd_status sd_clump_destroy(sd_clump_ctx* ctx);
+/* New contact law / dt / gravity for the following steps (state kept). Lets
+ * DEMEngine::update(float dt) (simulation_kernel.cu:52) take dt per call. */
+sd_status sd_clump_set_params(sd_clump_ctx* ctx, const sd_clump_params* p);
 /* Bodies: pos4 (x, y, z, mass; mass 0 = static), vel4 (v, 0), quat4 (w, x, y, z),
  * angvel4 (world angular velocity, 0), inertia6 (body frame: xx, yy, zz, xy, xz, yz
  * about the centre of mass), first/count sphere ranges. Spheres: local4

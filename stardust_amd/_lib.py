@@ -144,6 +144,7 @@ _SIGS = {
     "sd_lbvh_query": [_P, _I, _P, _P, _P, _P, _P, _P, _I, C.POINTER(C.c_int), _P, _P, _SZ, _P],
     "sd_clump_create": [_P, C.POINTER(C.c_void_p)],
     "sd_clump_destroy": [_P],
+    "sd_clump_set_params": [_P, _P],
     "sd_clump_upload": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P],
     "sd_clump_step": [_P, _I],
     "sd_clump_profile": [_P, _I, _P, _I],

@@ -63,7 +63,40 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB, "from", len(srcs), "sources")
+    build_engine(verbose)
     return LIB
+
+
+ENGINE_DIR = os.path.join(HERE, "engine")
+ENGINE_LIB = os.path.join(HERE, "libstardust_engine.so")
+CXX = shutil.which("g++") or "g++"
+
+
+def build_engine(verbose=True):
+    """Host-only C++ engine API (STARDUST::DEMEngine) over the C ABI, linked to libstardust_hip.so."""
+    srcs = sorted(os.path.join(ENGINE_DIR, f) for f in os.listdir(ENGINE_DIR) if f.endswith(".cpp"))
+    tmp = ENGINE_LIB + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-ffp-contract=off",
+           "-I" + os.path.join(os.path.dirname(HERE), "include"), "-I" + ENGINE_DIR] + srcs + \
+          ["-L" + HERE, "-lstardust_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("engine build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, ENGINE_LIB)
+    if verbose:
+        print("built", ENGINE_LIB, "from", len(srcs), "sources")
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    # the driver sits in engine/bin/, the libraries two levels up
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-I" + ENGINE_DIR, os.path.join(ENGINE_DIR, "examples", "dem_driver.cpp"),
+           "-L" + HERE, "-lstardust_engine", "-Wl,-rpath,$ORIGIN/../..", "-o", DRIVER + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return ENGINE_LIB
+
+
+DRIVER = os.path.join(ENGINE_DIR, "bin", "sd_dem_driver")
 
 
 if __name__ == "__main__":

@@ -412,6 +412,17 @@ extern "C" sd_status sd_clump_create(const sd_clump_params* p, sd_clump_ctx** ou
     return SD_OK;
 }
 
+// Replaces the contact law / dt / gravity of an existing context (takes effect
+// from the next step; the state is kept).
+extern "C" sd_status sd_clump_set_params(sd_clump_ctx* c, const sd_clump_params* p) {
+    SD_REQUIRE(c && p, "sd_clump_set_params: NULL argument");
+    sd_status st = check_contact(&p->contact);
+    if (st != SD_OK) return st;
+    (void)hipStreamSynchronize(c->stream);
+    c->p = *p;
+    return SD_OK;
+}
+
 extern "C" sd_status sd_clump_destroy(sd_clump_ctx* c) {
     if (!c) return SD_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);

@@ -1,0 +1,107 @@
+// DEMEngine: the STARDUST engine API (src/engine/engine.hpp:21-190,303-305;
+// engine.cpp:601-609; simulation_kernel.cu:52-153) kept as the drop-in
+// boundary. A program written against the reference's engine
+// (sandbox/testDEM.cpp:80-128 is the model) builds and runs unchanged:
+//
+//     STARDUST::engineConfig config{STARDUST::SAP, STARDUST::GPU};
+//     STARDUST::DEMEngine engine(config, 0.005f);
+//     engine.add(STARDUST::DEMParticle(0, 1, 0.5f, 0.5f, 9.0f, pos, vel));
+//     engine.prep();
+//     engine.transfer();
+//     for (int i = 0; i < 300; ++i) engine.update(0.0005f);
+//
+// Every entity is a rigid body made of its spheres. transfer() uploads the
+// bodies to a clump context of libstardust_hip.so (sd_clump_*), and update(dt)
+// advances one step there: sphere pose, LBVH broadphase over body boxes,
+// sphere-sphere contacts with the linear spring-dashpot law, per-body force
+// and torque gather, quaternion integration. Differences from the reference,
+// each one a correction of a documented defect:
+//  - inverse inertia is R I^-1 R^T with I the member spheres' own 2/5 m r^2
+//    plus the parallel-axis term (the reference inverts only det >= 1e-4 and
+//    applies R^T I^-1 R; a single sphere had I = 0);
+//  - the quaternion is renormalised every step;
+//  - the contact law is the linear spring-dashpot law with a Coulomb-limited
+//    tangential spring on the sphere radius (diameter / 2), set by
+//    setContactLaw (the reference multiplied component-wise and used the
+//    diameter as the radius);
+//  - bodies with mass >= the static threshold (10, the reference's rule,
+//    physics_update.cu:168) do not move; setStaticMassThreshold changes it.
+// Errors raise std::runtime_error with the library's message (the reference
+// called exit()).
+#ifndef STARDUST_AMD_ENGINE_ENGINE_HPP
+#define STARDUST_AMD_ENGINE_ENGINE_HPP
+
+#include <string>
+#include <vector>
+
+#include "entity.hpp"
+#include "entity_handler.hpp"
+
+struct sd_clump_ctx;
+
+namespace STARDUST {
+
+enum CollisionEngine { SAP, SHP };
+enum Device { CPU, GPU };
+
+struct engineConfig {
+    CollisionEngine collision_engine;
+    Device device;
+};
+
+class DEMEngine {
+public:
+    DEMEngine(engineConfig config, Scalar domain);
+    ~DEMEngine();
+    DEMEngine(const DEMEngine&) = delete;
+    DEMEngine& operator=(const DEMEngine&) = delete;
+
+    void add(DEMParticle particle);
+    void add(DEMMesh mesh);  // throws: triangle meshes are not supported
+    void prep();
+    void transfer();
+    void update(float dt);
+
+    // Scene file of the reference's format (setup/star_test.json): entities of
+    // type "generic" become DEMParticle(count + 1, 10, size, size, mass, position,
+    // velocity) (the mapping of the commented loader, engine.cpp:64-108, with
+    // the mass field); "domain" and "gravity" (magnitude, along -z) are read.
+    void loadJSONSetup(const std::string& filepath);
+
+    std::vector<DEMParticle> getEntities() const { return entity_handler.entities(); }
+    int getEntityLength() const { return entity_handler.getNumberOfParticles(); }
+    int getNumberOfSpheres() const { return entity_handler.getNumberOfSpheres(); }
+    EntityHandler& getEntityHandler() { return entity_handler; }
+
+    // Copies the current body and sphere state into the EntityHandler host arrays.
+    void sync();
+    long long stepsDone() const { return m_steps; }
+
+    // Contact law and options (defaults: kn = 2e6 N/m as the reference's
+    // constant, damping ratio 0.3, mu = 0.5, kt = 2/7 kn, gravity 9.81 along -z).
+    void setContactLaw(float kn, float damping_ratio, float mu, float kt_ratio);
+    void setGravity(float gx, float gy, float gz);
+    void setStaticMassThreshold(float threshold);
+
+    bool is_first_step = true;
+
+private:
+    void ensure_context(float dt);
+
+    engineConfig engineConfiguration;
+    Scalar m_domain;
+    EntityHandler entity_handler;
+    sd_clump_ctx* m_ctx = nullptr;
+    bool m_uploaded = false;
+    float m_dt = 0.0f;
+    float m_kn = 2e6f, m_zeta = 0.3f, m_mu = 0.5f, m_kt_ratio = 2.0f / 7.0f;
+    float m_g[3] = {0.0f, 0.0f, -9.81f};
+    float m_static_mass = 10.0f;
+    long long m_steps = 0;
+    std::vector<float> up_pos4, up_vel4, up_quat4, up_angvel4, up_inertia6, up_local4;
+    std::vector<int> up_first, up_count;
+};
+
+}  // namespace STARDUST
+
+#endif

@@ -1,0 +1,80 @@
+// EntityHandler of the STARDUST engine API (src/engine/entities/entity_handler.hpp:9-125).
+//
+// Holds the entities added to an engine and, after prepareArrays(), the host
+// arrays the reference packs (entity_handler.cpp:8-98) under the same names:
+// per body position / velocity / mass / quaternion / inertia tensor /
+// momenta, per sphere position / velocity / mass / size / owner / relative
+// position, and entity start / length ranges. The arrays are std::vector
+// storage exposed through the reference's pointer members; the device-side
+// d_* arrays of the reference are owned by libstardust_hip.so instead
+// (DEMEngine::update forwards to it) and are not exposed.
+#ifndef STARDUST_AMD_ENGINE_ENTITY_HANDLER_HPP
+#define STARDUST_AMD_ENGINE_ENTITY_HANDLER_HPP
+
+#include <vector>
+
+#include "entity.hpp"
+
+namespace STARDUST {
+
+class EntityHandler {
+public:
+    EntityHandler() = default;
+    EntityHandler(const EntityHandler& o) { *this = o; }
+    EntityHandler& operator=(const EntityHandler& o);
+
+    int getNumberOfSpheres() const { return n_spheres; }
+    int getNumberOfParticles() const { return (int)particles.size(); }
+    int getNumberOfMeshes() const { return 0; }
+    int getNumberOfPrimitives() const { return n_spheres; }
+
+    void addEntity(const DEMParticle& particle);
+    void addEntity(const DEMMesh& mesh);  // meshes are not supported: throws
+    void removeEntity(int type, int idx);  // not supported after prep (throws), as in the reference's stub
+
+    void prepareArrays();
+    const std::vector<DEMParticle>& entities() const { return particles; }
+
+    // Packed host arrays (valid after prepareArrays; refreshed by DEMEngine::sync).
+    int* h_entity_start_ptr = nullptr;
+    int* h_entity_length_ptr = nullptr;
+    int* h_particle_idx_ptr = nullptr;
+    int* h_particle_sorted_idx_ptr = nullptr;
+    float* h_rigid_body_mass_ptr = nullptr;
+    float4* h_rigid_body_position_ptr = nullptr;
+    float4* h_rigid_body_velocity_ptr = nullptr;
+    float4* h_rigid_body_angular_velocity_ptr = nullptr;
+    float4* h_rigid_body_linear_momentum_ptr = nullptr;
+    float4* h_rigid_body_angular_momentum_ptr = nullptr;
+    float4* h_rigid_body_quaternion_ptr = nullptr;  // (w, x, y, z) in (x, y, z, w)
+    float9* h_rigid_body_inertia_tensor_ptr = nullptr;  // reference formula: point masses (entity_handler.cpp:72-90)
+    float4* h_rigid_body_forces_ptr = nullptr;
+    float4* h_rigid_body_torques_ptr = nullptr;
+    int* h_particle_to_rigid_idx_ptr = nullptr;
+    float4* h_particle_position_ptr = nullptr;
+    float4* h_particle_velocity_ptr = nullptr;
+    float4* h_particle_forces_ptr = nullptr;
+    float* h_particle_mass_ptr = nullptr;
+    float* h_particle_size_ptr = nullptr;
+    float4* h_particle_relative_position_ptr = nullptr;
+    float4* h_particle_init_relative_position_ptr = nullptr;
+
+private:
+    void bind();
+
+    std::vector<DEMParticle> particles;
+    int n_spheres = 0;
+    bool prepared = false;
+
+    std::vector<int> entity_start, entity_length, particle_idx, particle_sorted_idx, particle_to_rigid;
+    std::vector<float> rb_mass, p_mass, p_size;
+    std::vector<float4> rb_pos, rb_vel, rb_angvel, rb_linmom, rb_angmom, rb_quat, rb_force, rb_torque;
+    std::vector<float9> rb_inertia;
+    std::vector<float4> p_pos, p_vel, p_force, p_rel, p_init_rel;
+
+    friend class DEMEngine;
+};
+
+}  // namespace STARDUST
+
+#endif

@@ -1,0 +1,137 @@
+"""The kept engine API (STARDUST::DEMEngine, DEMParticle, EntityHandler) through
+the example driver stardust_amd/engine/bin/sd_dem_driver (built with the library).
+
+CPU: the packed arrays of the reference driver's own entities
+(sandbox/testDEM.cpp:76 and :95) and of setup/star_test.json equal the values
+the reference's formulas give (entity.cpp:9-59, entity_handler.cpp:8-98).
+GPU: DEMEngine::update steps equal the oracle's clump steps on the same bodies,
+packed independently here (sphere radius = diameter / 2; inertia = own
+2/5 m r^2 + parallel-axis term, in float64; mass >= 10 static).
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from stardust_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "stardust_amd", "engine", "bin", "sd_dem_driver")
+STAR = os.path.join(ROOT, "tests", "golden", "star_test.json")
+
+
+def drive(*args, check=True):
+    if not os.path.exists(DRIVER):
+        pytest.fail("engine driver not built: run python -m stardust_amd.build")
+    r = subprocess.run([DRIVER] + [str(a) for a in args], capture_output=True, text=True, timeout=600)
+    if check:
+        assert r.returncode == 0, r.stderr
+        return json.loads(r.stdout)
+    return r
+
+
+def test_reference_driver_entities_kat():
+    d = drive("pack")
+    assert d["bodies"] == 2 and d["spheres"] == 9
+    # DEMParticle(1, 2, 1.0, 0.5, 9, (0.5, 0, 3)): floor(1/0.5)^2 = 4 -> dx = 0.25, 2.25 kg per sphere,
+    # lattice mirrored about its centroid -> (0.5 +- 0.125, +-0.125, 3 +- 0.125)
+    pos = np.array(d["particle_position"])[:8, :3]
+    want = {(0.5 + a, b, 3.0 + c) for a in (-0.125, 0.125) for b in (-0.125, 0.125) for c in (-0.125, 0.125)}
+    assert {tuple(p) for p in pos.tolist()} == want
+    assert d["particle_mass"][:8] == [2.25] * 8 and d["particle_size"] == [0.5] * 9
+    assert np.array_equal(np.array(d["inertia"][0]), np.array([0.5625, 0, 0, 0, 0.5625, 0, 0, 0, 0.5625]))
+    # DEMParticle(0, 1, .5, .5, 9, p): one sphere at p, full mass, zero point-mass inertia
+    assert d["particle_mass"][8] == 9.0 and d["inertia"][1] == [0.0] * 9
+    assert d["particle_relative_position"][8] == [0, 0, 0, 0]
+    assert d["entity_start"] == [0, 8] and d["entity_length"] == [8, 1]
+    assert d["particle_to_rigid"] == [0] * 8 + [1]
+
+
+def test_star_test_json_kat():
+    d = drive("pack-json", STAR)
+    assert d["bodies"] == 2 and d["spheres"] == 2000
+    # size = diameter = 0.15: floor(1)^2 = 1 -> dx = 0.15, each sphere carries the full 3 kg
+    assert set(d["particle_mass"]) == {3.0}
+    assert set(np.array(d["particle_size"], np.float32).tolist()) == {np.float32(0.15).item()}
+    pos = np.array(d["particle_position"])[:, :3]
+    for b, centre in enumerate(([0.5, 0.5, 0.5], [1.0, 1.0, 1.0])):
+        p = pos[1000 * b:1000 * (b + 1)]
+        span = p.max(0) - p.min(0)
+        assert np.allclose(span, 1.35, atol=1e-5)  # a 10^3 lattice of spacing 0.15
+        assert np.allclose(p.mean(0), centre, atol=1e-5)
+        assert np.allclose(d["rigid_body_position"][b][:3], centre)
+    # the two bodies overlap heavily at t = 0 (their lattices interpenetrate)
+    assert pos[:1000].max(0)[0] > pos[1000:].min(0)[0]
+
+
+def test_json_loader_rejects_unknown_entities(tmp_path):
+    f = tmp_path / "bad.json"
+    f.write_text(json.dumps({"scene": {"domain": 1, "entities": [{"type": "mesh"}]}}))
+    r = drive("pack-json", f, check=False)
+    assert r.returncode == 1 and "unsupported entity type" in r.stderr
+    f.write_text("{\"scene\": {\"entities\": [}")
+    r = drive("pack-json", f, check=False)
+    assert r.returncode == 1 and "json" in r.stderr
+
+
+def clump_input_from_pack(d, static_mass=10.0):
+    """Independent restatement of the engine's upload (see module docstring)."""
+    nb = d["bodies"]
+    start, length = d["entity_start"], d["entity_length"]
+    rel = np.array(d["particle_relative_position"], np.float32)
+    pm = np.array(d["particle_mass"], np.float32)
+    rad = (np.float32(0.5) * np.array(d["particle_size"], np.float32)).astype(np.float32)
+    s = {"pos4": np.zeros((nb, 4), np.float32), "vel4": np.zeros((nb, 4), np.float32),
+         "quat4": np.tile(np.array([1, 0, 0, 0], np.float32), (nb, 1)), "angvel4": np.zeros((nb, 4), np.float32),
+         "inertia6": np.zeros((nb, 6), np.float32), "first": np.array(start, np.int32),
+         "count": np.array(length, np.int32),
+         "local4": np.concatenate([rel[:, :3], rad[:, None]], 1).astype(np.float32)}
+    for b in range(nb):
+        m = np.float32(d["rigid_body_mass"][b])
+        s["pos4"][b, :3] = d["rigid_body_position"][b][:3]
+        s["pos4"][b, 3] = 0.0 if m >= static_mass else m
+        s["vel4"][b, :3] = d["rigid_body_velocity"][b][:3]
+        I = np.zeros(6)
+        for k in range(start[b], start[b] + length[b]):
+            dx, dy, dz = (float(v) for v in rel[k, :3])
+            mj, r = float(pm[k]), float(rad[k])
+            own = 0.4 * mj * r * r
+            I += [own + mj * (dy * dy + dz * dz), own + mj * (dx * dx + dz * dz), own + mj * (dx * dx + dy * dy),
+                  -mj * dx * dy, -mj * dx * dz, -mj * dy * dz]
+        s["inertia6"][b] = I.astype(np.float32)
+    return s
+
+
+def run_and_compare(oracle, steps, dt, *scene):
+    d = drive("run", steps, dt, *scene)
+    assert d["steps"] == steps
+    s = clump_input_from_pack(d)
+    st = oracle.ClumpState(s)
+    c = L.ContactParams.make(2e6, 0.3, 0.5, np.float32(dt), gravity=(0.0, 0.0, -9.81), walls=None,
+                             kt_ratio=2.0 / 7.0, max_contacts=16)
+    st.step(c, steps)
+    for key, arr, w in (("final_position", st.P, 3), ("final_velocity", st.V, 3), ("final_quaternion", st.Q, 4),
+                        ("final_angular_momentum", st.L, 3)):
+        got = np.array(d[key], np.float32)[:, :w]
+        assert np.array_equal(got, arr[:, :w]), (key, got, arr[:, :w])
+    return d, st
+
+
+@pytest.mark.gpu
+def test_engine_update_equals_oracle_two_bodies(oracle):
+    print("[gpu-case] engine two bodies", flush=True)
+    d, st = run_and_compare(oracle, 400, 1e-5)
+    # the two dynamic bodies collided (they were closing at 2 m/s with a 2 mm gap)
+    v = np.array(d["final_velocity"])
+    assert v[0][0] < 0.99 and v[1][0] > -0.99
+    # the static body (mass 50 >= 10) did not move
+    assert np.array_equal(np.array(d["final_position"][2][:3], np.float32), np.float32([0.016, -0.02, 0.0]))
+
+
+@pytest.mark.gpu
+def test_engine_update_star_test_scene(oracle):
+    print("[gpu-case] engine star_test.json", flush=True)
+    d, st = run_and_compare(oracle, 3, 1e-5, STAR)
+    assert np.all(np.isfinite(np.array(d["final_position"])))
