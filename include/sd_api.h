@@ -305,7 +305,7 @@ sd_status sd_morton30(const float* lo4, const float* hi4, int n, const float* do
 /* Karras radix tree over sorted keys (internal nodes 0..n-2, leaf k = node n-1+k),
  * node boxes by a bottom-up refit, and ropes (next node in depth-first order
  * after a subtree, -1 at the end). Arrays of 2n-1 entries except left/right
- * (n-1) and counters (n-1). *error (device int, cleared by the caller) gets
+ * (n-1) and counters (3n + 1 ints of scratch). *error (device int, cleared by the caller) gets
  * nonzero bits for a malformed tree. Replaces generatePrefixesCUDA ...
  * findLeafIndexRangeCUDA (:213-751). */
 sd_status sd_lbvh_build(const uint64_t* sorted_keys, int n, const float* lo4, const float* hi4, int* left,

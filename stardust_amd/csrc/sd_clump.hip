@@ -492,7 +492,7 @@ extern "C" sd_status sd_clump_upload(sd_clump_ctx* c, int nb, const float* pos4,
     ALLOC(c->rope, 2 * (size_t)nb);
     ALLOC(c->nlo, 2 * (size_t)nb);
     ALLOC(c->nhi, 2 * (size_t)nb);
-    ALLOC(c->counters, nb);
+    ALLOC(c->counters, 3 * (size_t)nb + 8);
     ALLOC(c->error, 1);
     sd_lbvh_query_scratch_bytes(nb, &c->q_scratch_bytes);
     ALLOC(*(char**)&c->q_scratch, c->q_scratch_bytes);

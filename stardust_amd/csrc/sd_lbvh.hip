@@ -8,7 +8,8 @@
 // one Karras build on 64-bit (code << 32 | body) keys with a 64-bit count of
 // leading zeros (quirk Q15: the reference applied 32-bit __clz to 64-bit keys);
 // the depth probe + level-by-level refit (:546-699, depth+1 launches and a host
-// copy) becomes one bottom-up refit with arrival counters; the 128-entry stack
+// copy) becomes a bottom-up refit in three launches (chunk-local arrival counters
+// in LDS, then the chunk-crossing nodes in one workgroup); the 128-entry stack
 // traversal (:769-864, atomicAdd pair append with a read-after-add race) becomes
 // a stackless rope traversal with count / scan / write output, visiting only
 // leaves to the right of the query so every pair is found once.
@@ -126,13 +127,13 @@ __global__ void k_lbvh_build(const uint64_t* __restrict__ keys, int n, int* __re
     last[i] = hi;
 }
 
-// Leaf boxes (sorted order) + ranges; arrival counters cleared.
+// Leaf boxes (sorted order) + ranges; the refit's list count is cleared.
 __global__ void k_lbvh_leaves(const uint64_t* __restrict__ keys, int n, const float4* __restrict__ lo4,
                               const float4* __restrict__ hi4, float4* __restrict__ nlo, float4* __restrict__ nhi,
                               int* __restrict__ first, int* __restrict__ last, int* __restrict__ parent,
-                              int* __restrict__ count) {
+                              int* __restrict__ list_count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n - 1) count[k] = 0;
+    if (k == 0) *list_count = 0;
     if (k >= n) return;
     const int b = (int)(uint32_t)keys[k];
     const int node = n - 1 + k;
@@ -143,27 +144,110 @@ __global__ void k_lbvh_leaves(const uint64_t* __restrict__ keys, int n, const fl
     if (n == 1) parent[node] = LB_SENTINEL;
 }
 
-// Bottom-up refit: the second child to arrive at a node computes its box.
-// Every walk is bounded by the node count; a malformed tree sets *error.
+// Bottom-up refit without device-wide fences. A node's box is the union of its
+// children's boxes (exact, so the order of evaluation does not matter).
+//  k_refit_chunks    one workgroup per chunk of RF_CHUNK consecutive leaves: every
+//                    internal node whose leaf range lies inside the chunk is
+//                    refit there (an internal node i is an end of its own range,
+//                    so its arrival counter lives in LDS at i - chunk start); the
+//                    second child to arrive computes the box
+//  k_refit_mark      nodes whose range crosses a chunk boundary are listed with
+//                    the number of their children already final
+//  k_refit_boundary  one workgroup finishes the listed nodes bottom-up with
+//                    arrival counters (workgroup-scope ordering only)
+// Every walk is bounded; a malformed tree sets *error.
+constexpr int RF_CHUNK = 256;
+constexpr int RF_FINISH_THREADS = 1024;
+
+__device__ __forceinline__ bool rf_crosses(const int* __restrict__ first, const int* __restrict__ last, int node) {
+    return first[node] / RF_CHUNK != last[node] / RF_CHUNK;
+}
+
+__device__ __forceinline__ void rf_union(int node, const int* __restrict__ left, const int* __restrict__ right,
+                                         float4* nlo, float4* nhi) {
+    const int a = left[node], b = right[node];
+    const float4 la = nlo[a], lb = nlo[b], ha = nhi[a], hb = nhi[b];
+    nlo[node] = make_float4(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z), 0.0f);
+    nhi[node] = make_float4(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z), 0.0f);
+}
+
 // (node boxes are written by other threads during the kernel: no __restrict__)
-__global__ void k_lbvh_refit(int n, const int* __restrict__ left, const int* __restrict__ right,
-                             const int* __restrict__ parent, float4* nlo, float4* nhi, int* count, int* error) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(RF_CHUNK) void k_refit_chunks(int n, const int* __restrict__ left,
+                                                           const int* __restrict__ right,
+                                                           const int* __restrict__ parent,
+                                                           const int* __restrict__ first,
+                                                           const int* __restrict__ last, float4* nlo, float4* nhi,
+                                                           int* error) {
+    __shared__ int cnt[RF_CHUNK];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int c0 = blockIdx.x * RF_CHUNK;
+    const int k = c0 + (int)threadIdx.x;
     if (k >= n || n < 2) return;
     int node = parent[n - 1 + k];
-    for (int guard = 0; node != LB_SENTINEL; ++guard) {
-        if (guard > n || node < 0 || node >= n - 1) {
+    for (int guard = 0; node >= 0; ++guard) {
+        if (guard > RF_CHUNK || node >= n - 1) {
             atomicOr(error, 1);
             return;
         }
-        __threadfence();
-        if (atomicAdd(&count[node], 1) == 0) return;  // first arrival: the sibling finishes the node
-        __threadfence();
-        const int a = left[node], b = right[node];
-        const float4 la = nlo[a], lb = nlo[b], ha = nhi[a], hb = nhi[b];
-        nlo[node] = make_float4(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z), 0.0f);
-        nhi[node] = make_float4(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z), 0.0f);
-        node = node == 0 ? LB_SENTINEL : parent[node];
+        if (rf_crosses(first, last, node)) return;  // finished by k_refit_boundary
+        const int slot = node - c0;
+        if (slot < 0 || slot >= RF_CHUNK) {
+            atomicOr(error, 1);
+            return;
+        }
+        __threadfence_block();
+        if (atomicAdd(&cnt[slot], 1) == 0) return;  // first arrival: the sibling finishes the node
+        __threadfence_block();
+        rf_union(node, left, right, nlo, nhi);
+        node = parent[node];
+    }
+}
+
+// scratch: cnt[n], list[n], ready[n], count[1] (ints)
+__global__ void k_refit_mark(int n, const int* __restrict__ left, const int* __restrict__ right,
+                             const int* __restrict__ first, const int* __restrict__ last, int* __restrict__ cnt,
+                             int* __restrict__ list, int* __restrict__ ready, int* __restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1 || !rf_crosses(first, last, i)) return;
+    const int a = left[i], b = right[i];
+    const int done = (a >= n - 1 || !rf_crosses(first, last, a) ? 1 : 0) +
+                     (b >= n - 1 || !rf_crosses(first, last, b) ? 1 : 0);
+    cnt[i] = done;
+    ready[i] = done;
+    list[atomicAdd(count, 1)] = i;
+}
+
+__global__ __launch_bounds__(RF_FINISH_THREADS) void k_refit_boundary(int n, const int* __restrict__ left,
+                                                                       const int* __restrict__ right,
+                                                                       const int* __restrict__ parent,
+                                                                       const int* __restrict__ first,
+                                                                       const int* __restrict__ last, float4* nlo,
+                                                                       float4* nhi, int* cnt,
+                                                                       const int* __restrict__ list,
+                                                                       const int* __restrict__ ready,
+                                                                       const int* __restrict__ count, int* error) {
+    const int m = *count;
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+        int node = list[e];
+        if (ready[node] != 2) continue;  // waits for a crossing child
+        for (int guard = 0;; ++guard) {
+            if (guard > n || node < 0 || node >= n - 1) {
+                atomicOr(error, 1);
+                break;
+            }
+            rf_union(node, left, right, nlo, nhi);
+            const int p = node == 0 ? -1 : parent[node];
+            if (p < 0) break;
+            if (p >= n - 1 || !rf_crosses(first, last, p)) {
+                atomicOr(error, 1);  // a crossing node's parent crosses too
+                break;
+            }
+            __threadfence_block();
+            if (atomicAdd(&cnt[p], 1) + 1 < 2) break;  // the other child is not final yet
+            __threadfence_block();
+            node = p;
+        }
     }
 }
 
@@ -282,12 +366,23 @@ extern "C" sd_status sd_lbvh_build(const uint64_t* sorted_keys, int n, const flo
                            parent, first, last);
         SD_LAUNCH_CHECK();
     }
+    int* rf_cnt = counters;
+    int* rf_list = counters + n;
+    int* rf_ready = counters + 2 * (size_t)n;
+    int* rf_count = counters + 3 * (size_t)n;
     hipLaunchKernelGGL(k_lbvh_leaves, dim3(div_up(n, 256)), dim3(256), 0, s, sorted_keys, n, (const float4*)lo4,
-                       (const float4*)hi4, (float4*)node_lo4, (float4*)node_hi4, first, last, parent, counters);
+                       (const float4*)hi4, (float4*)node_lo4, (float4*)node_hi4, first, last, parent, rf_count);
     SD_LAUNCH_CHECK();
     if (n > 1) {
-        hipLaunchKernelGGL(k_lbvh_refit, dim3(div_up(n, 256)), dim3(256), 0, s, n, left, right, parent,
-                           (float4*)node_lo4, (float4*)node_hi4, counters, error);
+        hipLaunchKernelGGL(k_refit_chunks, dim3(div_up(n, RF_CHUNK)), dim3(RF_CHUNK), 0, s, n, left, right, parent,
+                           first, last, (float4*)node_lo4, (float4*)node_hi4, error);
+        SD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_refit_mark, dim3(div_up(n - 1, 256)), dim3(256), 0, s, n, left, right, first, last,
+                           rf_cnt, rf_list, rf_ready, rf_count);
+        SD_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_refit_boundary, dim3(1), dim3(RF_FINISH_THREADS), 0, s, n, left, right, parent, first,
+                           last, (float4*)node_lo4, (float4*)node_hi4, rf_cnt, (const int*)rf_list,
+                           (const int*)rf_ready, (const int*)rf_count, error);
         SD_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_lbvh_ropes, dim3(div_up(2 * n - 1, 256)), dim3(256), 0, s, n, left, right, parent, rope,

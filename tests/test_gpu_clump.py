@@ -42,8 +42,8 @@ class Lbvh:
         self.scr = L.DeviceBuffer(256 * 6 * 4)
         self.keys = L.DeviceBuffer(8 * n)
         nn = 2 * n - 1
-        self.buf = {k: L.DeviceBuffer(4 * max(nn, 1)) for k in ("left", "right", "parent", "first", "last", "rope",
-                                                                "counters")}
+        self.buf = {k: L.DeviceBuffer(4 * max(nn, 1)) for k in ("left", "right", "parent", "first", "last", "rope")}
+        self.buf["counters"] = L.DeviceBuffer(4 * (3 * n + 1))
         self.nlo, self.nhi = L.DeviceBuffer(16 * nn), L.DeviceBuffer(16 * nn)
         self.err = L.DeviceBuffer.from_array(np.zeros(4, np.int32))
 
