@@ -23,6 +23,8 @@ Further sections ride on the same line (--sections, default "batch,gjk,clumps"):
            (features + BF16 MFMA inference + warm GJK, "learned"), after training
            the network on pairs logged from plain GJK on seeded scenes; iteration
            histograms; CPU serial SAP and CPU GJK (oracle) beside them.
+  batch_gjk — the 128 hull scenes of batch_c5: GJK pairs/s of one batched pass
+           (batched SAP, learned warm start) vs every scene alone (plain).
   clumps — clumps_c3 (262,144 rigid clumps, LBVH broadphase): sphere-steps/s.
 """
 import argparse
@@ -112,6 +114,122 @@ def timed(fn, reps, stream=None):
     return elapsed_ms(ev) / reps
 
 
+_MODEL = {}
+
+
+def trained_model(args, n_hulls):
+    """The warm-start network trained on plain GJK's converged directions on seeded
+    scenes of the hulls_c4 generator (trained once per run)."""
+    if "model" in _MODEL:
+        return _MODEL["model"], _MODEL["train"]
+    from stardust_amd import hulls as H
+    from stardust_amd import warmstart as WS
+    t0 = time.perf_counter()
+    data = WS.PairData(args.train_samples)
+    seed = 0x5EED1000
+    while data.n < args.train_samples and time.perf_counter() - t0 < args.train_log_seconds:
+        got = data.append_scene(H.generate_hulls(seed, n_hulls))
+        seed += 1
+        log("gjk: logged scene %d: %d samples (total %d)" % (seed - 0x5EED1000, got, data.n))
+    log_s = time.perf_counter() - t0
+    model = WS.WarmStartModel(seed=11)
+    t0 = time.perf_counter()
+    hist = model.train(data, epochs=args.train_epochs, batch=8192, lr=3e-3, log=lambda *a: log("gjk:", *a))
+    train_s = time.perf_counter() - t0
+    log("gjk: trained on %d samples (%d scenes, logged in %.1f s), %.1f s" % (data.n, seed - 0x5EED1000, log_s,
+                                                                               train_s))
+    train = {"samples": data.n, "epochs": args.train_epochs, "loss": [round(x, 5) for x in hist],
+             "log_s": round(log_s, 2), "train_s": round(train_s, 2)}
+    _MODEL["model"], _MODEL["train"] = model, train
+    del data
+    return model, train
+
+
+def batch_gjk_section(args):
+    """batch_c5 hull scenes (128 scenes, 1k-8k hulls each): narrowphase pairs/s of one
+    batched pass (batched SAP, features, BF16 inference, warm GJK) vs the plain form
+    (each scene alone: SAP, cold GJK)."""
+    import ctypes
+
+    import numpy as np
+    from stardust_amd import _lib as L
+    from stardust_amd import hulls as H
+    from stardust_amd import warmstart as WS
+    from stardust_amd import workloads as W
+
+    w = W.batch_c5()
+    t0 = time.perf_counter()
+    sets = [w.generate_hull_scene(j) for j in range(len(w.hull_scenes))]
+    log("batch_gjk: %d hull scenes, %d hulls, generated in %.1f s" % (len(sets), int(w.hull_n.sum()),
+                                                                     time.perf_counter() - t0))
+    model, train = trained_model(args, 131072)
+    reps = args.gjk_reps
+
+    # plain: every scene alone (AABBs + SAP with its count read-back, then cold GJK)
+    plain_gjk_ms, plain_all_s, plain_pairs = 0.0, 0.0, 0
+    for h in sets:
+        d = H.DeviceHulls(h)
+        d.aabbs()
+        pairs, npairs, _ = d.sap()
+        out = d.gjk(pairs, npairs)
+        L.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            d.aabbs()
+            d.sap(capacity=npairs)
+            d.gjk(pairs, npairs, out=out)
+        L.synchronize()
+        plain_all_s += (time.perf_counter() - t0) / reps
+        plain_gjk_ms += timed(lambda: d.gjk(pairs, npairs, out=out), reps)
+        plain_pairs += npairs
+    # batched: all scenes in one set
+    b = H.BatchedHulls(sets)
+    b.aabbs()
+    pairs, npairs = b.sap_batched()
+    assert npairs == plain_pairs, (npairs, plain_pairs)
+    X, desc = WS.pair_features(b, pairs, npairs)
+    warm = model.infer(X, npairs, pairs=pairs, quat=b.buf["quat4"])
+    out_c = b.gjk(pairs, npairs)
+    out_w = b.gjk(pairs, npairs, warm=warm)
+    bb = b.buf
+    cold_ms = timed(lambda: b.gjk(pairs, npairs, out=out_c), reps)
+    feat_ms = timed(lambda: L.call("sd_mlp_features", pairs.ptr, npairs, bb["pos4"].ptr, bb["quat4"].ptr, desc.ptr,
+                                   X.ptr, None), reps)
+    infer_ms = timed(lambda: model.infer(X, npairs, pairs=pairs, quat=b.buf["quat4"], out=warm), reps)
+    warm_ms = timed(lambda: b.gjk(pairs, npairs, warm=warm, out=out_w), reps)
+    L.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.aabbs()
+        b.sap_batched(capacity=npairs)
+        L.call("sd_mlp_features", pairs.ptr, npairs, bb["pos4"].ptr, bb["quat4"].ptr, desc.ptr, X.ptr, None)
+        model.infer(X, npairs, pairs=pairs, quat=b.buf["quat4"], out=warm)
+        b.gjk(pairs, npairs, warm=warm, out=out_w)
+    L.synchronize()
+    batched_all_s = (time.perf_counter() - t0) / reps
+    c = b.download_gjk(out_c, npairs)
+    wv = b.download_gjk(out_w, npairs)
+    learned_ms = feat_ms + infer_ms + warm_ms
+    res = {"workload": "batch_c5 hull scenes", "scenes": len(sets), "hulls": int(b.n), "pairs": int(npairs),
+           "plain_gjk_pairs_per_s": plain_pairs / (plain_gjk_ms / 1000.0),
+           "batched_cold_gjk_pairs_per_s": npairs / (cold_ms / 1000.0),
+           "batched_learned_gjk_pairs_per_s": npairs / (learned_ms / 1000.0),
+           "batched_learned_over_plain": plain_gjk_ms / learned_ms,
+           "ms": {"plain_gjk_sum": round(plain_gjk_ms, 4), "batched_cold_gjk": round(cold_ms, 4),
+                  "batched_features": round(feat_ms, 4), "batched_inference": round(infer_ms, 4),
+                  "batched_warm_gjk": round(warm_ms, 4)},
+           "with_broadphase": {"plain_pairs_per_s": plain_pairs / plain_all_s,
+                               "batched_learned_pairs_per_s": npairs / batched_all_s,
+                               "batched_over_plain": plain_all_s / batched_all_s},
+           "iters_mean": {"cold": float(c["iters"].mean()), "learned": float(wv["iters"].mean())},
+           "verdicts_identical": bool(np.array_equal(c["flags"] & 1, wv["flags"] & 1)),
+           "model": train}
+    log("batch_gjk: plain %.3e pairs/s, batched cold %.3e, batched learned %.3e (%.1fx); with broadphase %.1fx" % (
+        res["plain_gjk_pairs_per_s"], res["batched_cold_gjk_pairs_per_s"], res["batched_learned_gjk_pairs_per_s"],
+        res["batched_learned_over_plain"], res["with_broadphase"]["batched_over_plain"]))
+    return res
+
+
 def gjk_section(args):
     """hulls_c4: SAP + GJK/EPA pairs/s, plain cold start vs learned warm start."""
     import ctypes
@@ -134,21 +252,7 @@ def gjk_section(args):
     aabb_ms = timed(d.aabbs, 5)
     log("gjk: %d SAP pairs on axis %d; aabb %.3f ms, sap %.3f ms" % (npairs, axis, aabb_ms, sap_ms))
 
-    # training data: plain GJK's converged directions on seeded scenes of the same generator
-    t0 = time.perf_counter()
-    data = WS.PairData(args.train_samples)
-    seed = 0x5EED1000
-    while data.n < args.train_samples and time.perf_counter() - t0 < args.train_log_seconds:
-        got = data.append_scene(H.generate_hulls(seed, len(h["pos4"])))
-        seed += 1
-        log("gjk: logged scene %d: %d samples (total %d)" % (seed - 0x5EED1000, got, data.n))
-    log_s = time.perf_counter() - t0
-    model = WS.WarmStartModel(seed=11)
-    t0 = time.perf_counter()
-    hist = model.train(data, epochs=args.train_epochs, batch=8192, lr=3e-3, log=lambda *a: log("gjk:", *a))
-    train_s = time.perf_counter() - t0
-    log("gjk: trained on %d samples (%d scenes, logged in %.1f s), %.1f s" % (data.n, seed - 0x5EED1000, log_s,
-                                                                               train_s))
+    model, train = trained_model(args, len(h["pos4"]))
 
     # hull descriptors depend on the shapes only: computed once with the scene
     X, desc = WS.pair_features(d, pairs, npairs)
@@ -205,8 +309,7 @@ def gjk_section(args):
         "warm_used_frac": float(((w["flags"] & L.GJK_WARM_USED) != 0).mean()),
         "verdicts_identical": same_verdict, "max_rel_sep_diff": float(rel.max()) if len(rel) else 0.0,
         "hit_frac": float((c["flags"] & 1).mean()),
-        "train": {"samples": data.n, "epochs": args.train_epochs, "loss": [round(x, 5) for x in hist],
-                  "log_s": round(log_s, 2), "train_s": round(train_s, 2)},
+        "train": train,
         "cpu": cpu,
     }
 
@@ -331,8 +434,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="batch,gjk,clumps",
-                    help="extra sections: batch, gjk, clumps (comma list, or none)")
+    ap.add_argument("--sections", default="batch,gjk,batch_gjk,clumps",
+                    help="extra sections: batch, gjk, batch_gjk, clumps (comma list, or none)")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-warmup", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=20)
@@ -441,6 +544,8 @@ def main():
         out["batch"] = batch_section(args)
     if "gjk" in sections:
         out["gjk"] = gjk_section(args)
+    if "batch_gjk" in sections:
+        out["batch_gjk"] = batch_gjk_section(args)
     if "clumps" in sections:
         out["clumps"] = clump_section(args)
     print(json.dumps(out), flush=True)

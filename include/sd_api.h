@@ -270,6 +270,17 @@ sd_status sd_hull_aabbs(const float* pos4, const float* quat4, const int* vstart
 sd_status sd_sap_scratch_bytes(int n, size_t* bytes);
 sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int* pairs, int capacity, int* n_pairs,
                        int* axis_out, void* scratch, size_t scratch_bytes, void* stream);
+/* Sweep-and-prune over many hull scenes at once: hulls concatenated scene by
+ * scene (scene_first[0] = 0 <= ... <= scene_first[nscenes] = n, scene_of[i] the
+ * scene of hull i; device arrays), at most 65,536 hulls per scene. Each scene is
+ * swept on its own axis and gives exactly the pairs, in the same order, that
+ * sd_sap_pairs gives for it alone (hull ids are global); no pair crosses a
+ * scene. axes (device, nullable) receives the axis of every scene. Replaces
+ * running processCollisions (sap_detection.cu:668-706) once per scene. */
+sd_status sd_sap_batched_scratch_bytes(int n, int nscenes, size_t* bytes);
+sd_status sd_sap_pairs_batched(const float* lo4, const float* hi4, int n, const int* scene_of,
+                               const int* scene_first, int nscenes, int* pairs, int capacity, int* n_pairs,
+                               int* axes, void* scratch, size_t scratch_bytes, void* stream);
 
 /* GJK distance / EPA penetration for each pair (a, b) of hull ids. Outputs per
  * pair: sep (> 0 separation distance, < 0 penetration depth), normal3 (unit,

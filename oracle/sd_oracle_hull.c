@@ -185,9 +185,10 @@ long or_brute_aabb_pairs(int n, const float* lo4, const float* hi4, int* pairs, 
 /* ------------------------------------------------------------------ GJK / EPA */
 
 #define KEYMUL 1024
-#define REL_EPS 1e-6f
-#define TOUCH_EPS2 1e-14f
-#define STALL_EPS 1e-4f
+#define REL_EPS 1e-6      /* GJK (double) */
+#define TOUCH_EPS2 1e-14
+#define STALL_EPS 1e-4
+#define EPA_REL_EPS 1e-6f /* EPA (float) */
 #define MAXV 48
 #define MAXF 96
 #define MAXE 48
@@ -239,96 +240,125 @@ static MP supM(const Shape* A, const Shape* B, V3 u) {
     return r;
 }
 
+/* The GJK simplex and everything derived from it are evaluated in double (the
+ * support points themselves are FP32 values, stored exactly). */
 typedef struct {
-    MP w[4];
+    double x, y, z;
+} D3;
+static D3 dk(double x, double y, double z) {
+    D3 r = {x, y, z};
+    return r;
+}
+static D3 d_of(V3 a) { return dk((double)a.x, (double)a.y, (double)a.z); }
+static V3 f_of(D3 a) { return mk((float)a.x, (float)a.y, (float)a.z); }
+static D3 dadd(D3 a, D3 b) { return dk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static D3 dsub(D3 a, D3 b) { return dk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static D3 dneg(D3 a) { return dk(-a.x, -a.y, -a.z); }
+static D3 dscl(D3 a, double s) { return dk(a.x * s, a.y * s, a.z * s); }
+static double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static D3 dcross(D3 a, D3 b) { return dk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+typedef struct {
+    D3 p;
+    int key;
+} SP;
+static SP sp_of(MP m) {
+    SP r;
+    r.p = d_of(m.p);
+    r.key = m.key;
+    return r;
+}
+
+typedef struct {
+    SP w[4];
     int n;
 } Simplex;
 
-static V3 cl_segment(Simplex* s) {
-    const V3 a = s->w[0].p, b = s->w[1].p;
-    const V3 ab = vsub(b, a);
-    const float t = -vdot(a, ab);
-    if (t <= 0.0f) {
+static D3 cl_segment(Simplex* s) {
+    const D3 a = s->w[0].p, b = s->w[1].p;
+    const D3 ab = dsub(b, a);
+    const double t = -ddot(a, ab);
+    if (t <= 0.0) {
         s->n = 1;
         return a;
     }
-    const float den = vdot(ab, ab);
+    const double den = ddot(ab, ab);
     if (t >= den) {
         s->w[0] = s->w[1];
         s->n = 1;
         return b;
     }
-    return vadd(a, vscl(ab, t / den));
+    return dadd(a, dscl(ab, t / den));
 }
 
 /* closest point of a triangle to the origin by Voronoi regions (vertex, edge, face) */
-static V3 cl_triangle(Simplex* s) {
-    const MP A = s->w[0], B = s->w[1], C = s->w[2];
-    const V3 a = A.p, b = B.p, c = C.p;
-    const V3 ab = vsub(b, a), ac = vsub(c, a), ap = vneg(a);
-    const float d1 = vdot(ab, ap), d2 = vdot(ac, ap);
-    if (d1 <= 0.0f && d2 <= 0.0f) {
+static D3 cl_triangle(Simplex* s) {
+    const SP A = s->w[0], B = s->w[1], C = s->w[2];
+    const D3 a = A.p, b = B.p, c = C.p;
+    const D3 ab = dsub(b, a), ac = dsub(c, a), ap = dneg(a);
+    const double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
         s->n = 1;
         return a;
     }
-    const V3 bp = vneg(b);
-    const float d3 = vdot(ab, bp), d4 = vdot(ac, bp);
-    if (d3 >= 0.0f && d4 <= d3) {
+    const D3 bp = dneg(b);
+    const double d3 = ddot(ab, bp), d4 = ddot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) {
         s->w[0] = B;
         s->n = 1;
         return b;
     }
-    const float vc = d1 * d4 - d3 * d2;
-    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-        const float t = d1 / (d1 - d3);
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        const double t = d1 / (d1 - d3);
         s->n = 2;
-        return vadd(a, vscl(ab, t));
+        return dadd(a, dscl(ab, t));
     }
-    const V3 cp = vneg(c);
-    const float d5 = vdot(ab, cp), d6 = vdot(ac, cp);
-    if (d6 >= 0.0f && d5 <= d6) {
+    const D3 cp = dneg(c);
+    const double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) {
         s->w[0] = C;
         s->n = 1;
         return c;
     }
-    const float vb = d5 * d2 - d1 * d6;
-    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-        const float t = d2 / (d2 - d6);
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double t = d2 / (d2 - d6);
         s->w[1] = C;
         s->n = 2;
-        return vadd(a, vscl(ac, t));
+        return dadd(a, dscl(ac, t));
     }
-    const float va = d3 * d6 - d5 * d4;
-    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
-        const float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
         s->w[0] = B;
         s->w[1] = C;
         s->n = 2;
-        return vadd(b, vscl(vsub(c, b), t));
+        return dadd(b, dscl(dsub(c, b), t));
     }
-    const float den = 1.0f / (va + vb + vc);
-    const float v = vb * den, w = vc * den;
-    return vadd(vadd(a, vscl(ab, v)), vscl(ac, w));
+    const double den = 1.0 / (va + vb + vc);
+    const double v = vb * den, w = vc * den;
+    return dadd(dadd(a, dscl(ab, v)), dscl(ac, w));
 }
 
-static int outside(V3 a, V3 b, V3 c, V3 d) {
-    const V3 nrm = vcross(vsub(b, a), vsub(c, a));
-    const float so = -vdot(a, nrm);
-    const float sd = vdot(vsub(d, a), nrm);
-    return so * sd < 0.0f;
+static int outside(D3 a, D3 b, D3 c, D3 d) {
+    const D3 nrm = dcross(dsub(b, a), dsub(c, a));
+    const double so = -ddot(a, nrm);
+    const double sd = ddot(dsub(d, a), nrm);
+    return so * sd < 0.0;
 }
 
 /* A near-flat tetrahedron (|6V| <= 1e-5 |ab||ac||ad|) never counts as enclosing
  * the origin; then every face is searched. */
-static V3 cl_tetra(Simplex* s, int* inside) {
-    const MP P[4] = {s->w[0], s->w[1], s->w[2], s->w[3]};
+static D3 cl_tetra(Simplex* s, int* inside) {
+    const SP P[4] = {s->w[0], s->w[1], s->w[2], s->w[3]};
     static const int F[4][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 3, 2, 0}};
-    const V3 ab = vsub(P[1].p, P[0].p), ac = vsub(P[2].p, P[0].p), ad = vsub(P[3].p, P[0].p);
-    const float vol = vdot(ad, vcross(ab, ac));
-    const float lim = 1e-5f * (sqrtf(vdot(ab, ab)) * sqrtf(vdot(ac, ac)) * sqrtf(vdot(ad, ad)));
-    const int flat = !(fabsf(vol) > lim);
-    float best = 3.402823466e38f;
-    V3 bestp = mk(0.0f, 0.0f, 0.0f);
+    const D3 ab = dsub(P[1].p, P[0].p), ac = dsub(P[2].p, P[0].p), ad = dsub(P[3].p, P[0].p);
+    const double vol = ddot(ad, dcross(ab, ac));
+    const double lim = 1e-5 * (sqrt(ddot(ab, ab)) * sqrt(ddot(ac, ac)) * sqrt(ddot(ad, ad)));
+    const int flat = !(fabs(vol) > lim);
+    double best = 1.79769313486231e308;
+    D3 bestp = dk(0.0, 0.0, 0.0);
     Simplex bests = *s;
     int any = 0;
     for (int f = 0; f < 4; ++f) {
@@ -339,8 +369,8 @@ static V3 cl_tetra(Simplex* s, int* inside) {
         t.w[1] = P[F[f][1]];
         t.w[2] = P[F[f][2]];
         t.n = 3;
-        const V3 q = cl_triangle(&t);
-        const float dq = vdot(q, q);
+        const D3 q = cl_triangle(&t);
+        const double dq = ddot(q, q);
         if (dq < best) {
             best = dq;
             bestp = q;
@@ -352,7 +382,7 @@ static V3 cl_tetra(Simplex* s, int* inside) {
     return bestp;
 }
 
-static V3 cl_simplex(Simplex* s, int* inside) {
+static D3 cl_simplex(Simplex* s, int* inside) {
     *inside = 0;
     if (s->n == 1) return s->w[0].p;
     if (s->n == 2) return cl_segment(s);
@@ -360,9 +390,9 @@ static V3 cl_simplex(Simplex* s, int* inside) {
     return cl_tetra(s, inside);
 }
 
-static void sort_by_key(MP* w, int n) {
+static void sort_by_key(SP* w, int n) {
     for (int i = 1; i < n; ++i) {
-        MP x = w[i];
+        SP x = w[i];
         int j = i - 1;
         while (j >= 0 && w[j].key > x.key) {
             w[j + 1] = w[j];
@@ -408,7 +438,11 @@ enum { HIT = 1, CONV = 2, BUDGET = 4, ELIMIT = 8, DEGEN = 16, WARM = 32 };
 static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_iter, Out* o) {
     MP P[MAXV];
     int np = 0;
-    for (int i = 0; i < s0->n; ++i) P[np++] = s0->w[i];
+    for (int i = 0; i < s0->n; ++i) {
+        P[np].p = f_of(s0->w[i].p); /* exact: FP32 support points */
+        P[np].key = s0->w[i].key;
+        ++np;
+    }
     const V3 axes[3] = {mk(1.0f, 0.0f, 0.0f), mk(0.0f, 1.0f, 0.0f), mk(0.0f, 0.0f, 1.0f)};
     if (np == 1) {
         for (int k = 0; k < 6 && np == 1; ++k) {
@@ -461,7 +495,7 @@ static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_i
         const Face fb = F[best];
         const MP m = supM(A, B, fb.n);
         const float gap = vdot(m.p, fb.n) - fb.d;
-        if (gap <= REL_EPS * fabsf(fb.d) + 1e-9f) {
+        if (gap <= EPA_REL_EPS * fabsf(fb.d) + 1e-9f) {
             converged = 1;
             break;
         }
@@ -521,7 +555,15 @@ static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_i
         if (F[f].d < F[best].d) best = f;
     /* canonical depth / normal: the face's vertices ordered by key */
     MP q[3] = {P[F[best].a], P[F[best].b], P[F[best].c]};
-    sort_by_key(q, 3);
+    for (int i = 1; i < 3; ++i) {
+        MP x = q[i];
+        int j = i - 1;
+        while (j >= 0 && q[j].key > x.key) {
+            q[j + 1] = q[j];
+            --j;
+        }
+        q[j + 1] = x;
+    }
     V3 nrm = vcross(vsub(q[1].p, q[0].p), vsub(q[2].p, q[0].p));
     const float l2 = vdot(nrm, nrm);
     if (vdot(nrm, vsub(q[0].p, inner)) < 0.0f) nrm = vneg(nrm);
@@ -544,42 +586,47 @@ static Out query(const Shape* A, const Shape* B, V3 init, int max_iter) {
     o.n = mk(1.0f, 0.0f, 0.0f);
     if (vdot(init, init) == 0.0f) init = mk(1.0f, 0.0f, 0.0f);
     Simplex s;
-    s.w[0] = supM(A, B, init);
+    s.w[0] = sp_of(supM(A, B, init));
     s.n = 1;
-    V3 v = s.w[0].p;
-    int hit = 0, converged = 0, it = 0;
+    D3 v = s.w[0].p;
+    int hit = 0, converged = 0, uncertain = 0, it = 0;
     while (it < max_iter) {
-        const float vv = vdot(v, v);
+        const double vv = ddot(v, v);
         if (vv <= TOUCH_EPS2) {
             hit = 1;
             break;
         }
-        const MP w = supM(A, B, vneg(v));
+        const SP w = sp_of(supM(A, B, f_of(dneg(v))));
         ++it;
-        const float vw = vdot(v, w.p);
+        const double vw = ddot(v, w.p);
         if (vv - vw <= REL_EPS * vv) {
             converged = 1;
             break;
         }
+        /* a repeated support vertex or a stalled distance ends the query: a
+         * separation only when the duality gap certifies it to 1e-4; without a
+         * certificate and with v.w <= 0 the exit is uncertain and goes to EPA */
         int dup = 0;
         for (int i = 0; i < s.n; ++i) dup = dup || (w.key >= 0 && s.w[i].key == w.key);
         if (dup) {
-            converged = 1;
+            if (vv - vw <= STALL_EPS * vv) converged = 1;
+            else if (vw <= 0.0) hit = uncertain = 1;
             break;
         }
         const Simplex sp = s;
         s.w[s.n++] = w;
         int inside = 0;
-        const V3 vn = cl_simplex(&s, &inside);
+        const D3 vn = cl_simplex(&s, &inside);
         if (inside) {
             hit = 1;
             break;
         }
         /* the distance must strictly decrease; otherwise the previous feature is
-         * final when the duality gap certifies it to 1e-4, else the query failed */
-        if (!(vdot(vn, vn) < vv)) {
+         * final when the duality gap certifies it to 1e-4 */
+        if (!(ddot(vn, vn) < vv)) {
             s = sp;
-            converged = vv - vw <= STALL_EPS * vv;
+            if (vv - vw <= STALL_EPS * vv) converged = 1;
+            else if (vw <= 0.0) hit = uncertain = 1;
             break;
         }
         v = vn;
@@ -587,7 +634,16 @@ static Out query(const Shape* A, const Shape* B, V3 init, int max_iter) {
     o.iters = it;
     if (hit) {
         run_epa(A, B, &s, max_iter, &o);
-        return o;
+        if (!(uncertain && (o.flags & DEGEN))) return o;
+        /* no enclosing polytope after an uncertain exit: the uncertified separation */
+        const int ei = o.epa_iters;
+        o.flags = 0;
+        o.epa_iters = ei;
+        o.sep = 0.0f;
+        o.n = mk(1.0f, 0.0f, 0.0f);
+        int inside = 0;
+        v = cl_simplex(&s, &inside);
+        converged = 0;
     }
     o.flags |= converged ? CONV : BUDGET;
     int canon = 1;
@@ -597,9 +653,9 @@ static Out query(const Shape* A, const Shape* B, V3 init, int max_iter) {
         int inside = 0;
         v = cl_simplex(&s, &inside);
     }
-    const float dist = sqrtf(vdot(v, v));
-    o.sep = dist;
-    o.n = dist > 0.0f ? vscl(v, -1.0f / dist) : mk(1.0f, 0.0f, 0.0f);
+    const double dist = sqrt(ddot(v, v));
+    o.sep = (float)dist;
+    o.n = dist > 0.0 ? f_of(dscl(v, -1.0 / dist)) : mk(1.0f, 0.0f, 0.0f);
     return o;
 }
 

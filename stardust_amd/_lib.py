@@ -135,6 +135,8 @@ _SIGS = {
     "sd_hull_aabbs": [_P, _P, _P, _P, _P, _I, _P, _P, _P],
     "sd_sap_scratch_bytes": [_I, C.POINTER(_SZ)],
     "sd_sap_pairs": [_P, _P, _I, _P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), _P, _SZ, _P],
+    "sd_sap_batched_scratch_bytes": [_I, _I, C.POINTER(_SZ)],
+    "sd_sap_pairs_batched": [_P, _P, _I, _P, _P, _I, _P, _I, C.POINTER(C.c_int), _P, _P, _SZ, _P],
     "sd_gjk_scratch_bytes": [_I, C.POINTER(_SZ)],
     "sd_gjk": [_P, _I, _P, _P, _P, _P, _P, _I, _I, _P, _I, _I, _F, _P, _P, _P, _P, _P, _SZ, _P],
     "sd_global_domain": [_P, _P, _I, _P, _P, _P],

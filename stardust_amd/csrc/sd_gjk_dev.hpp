@@ -19,9 +19,10 @@
 namespace sd {
 
 constexpr int GJK_MAX_VERTS_PER_HULL = 1024;  // key = ia * 1024 + ib
-constexpr float GJK_REL_EPS = 1e-6f;          // v.v - v.w <= eps * v.v  -> converged
-constexpr float GJK_TOUCH_EPS2 = 1e-14f;      // |v|^2 below this: origin on the boundary
-constexpr float GJK_STALL_EPS = 1e-4f;        // gap accepted when the distance stops decreasing
+constexpr double GJK_REL_EPS = 1e-6;          // v.v - v.w <= eps * v.v  -> converged
+constexpr double GJK_TOUCH_EPS2 = 1e-14;      // |v|^2 below this: origin on the boundary
+constexpr double GJK_STALL_EPS = 1e-4;        // gap accepted when the distance stops decreasing
+constexpr float EPA_REL_EPS = 1e-6f;          // EPA: support gap <= eps * depth + 1e-9 -> converged
 constexpr int EPA_MAX_V = 48;
 constexpr int EPA_MAX_F = 96;
 constexpr int EPA_MAX_E = 48;
@@ -133,22 +134,49 @@ __device__ __forceinline__ MPoint support_M(const Shape& A, const Shape& B, V3 u
 }
 
 // ---------------------------------------------------------------- simplex
+//
+// The simplex of the GJK phase and every quantity derived from it (closest
+// point, v.v, v.w, duality gap) are evaluated in FP64; support points are the
+// FP32 differences the support mapping returns, stored exactly. The closest
+// point of a feature far from the origin relative to its vertices then keeps
+// its accuracy, and the duality gap can certify the separation.
+
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+__device__ __forceinline__ D3 d3(V3 a) { return D3{(double)a.x, (double)a.y, (double)a.z}; }
+__device__ __forceinline__ V3 to_v3(D3 a) { return v3((float)a.x, (float)a.y, (float)a.z); }
+__device__ __forceinline__ D3 add(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ D3 sub(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ D3 neg(D3 a) { return d3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ D3 scl(D3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross(D3 a, D3 b) {
+    return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+struct SPoint {
+    D3 p;
+    int key;
+};
+__device__ __forceinline__ SPoint spoint(const MPoint& m) { return SPoint{d3(m.p), m.key}; }
 
 struct Simplex {
-    MPoint w[4];
+    SPoint w[4];
     int n;
 };
 
 // Closest point of segment (a, b) to the origin; keeps the feature's vertices.
-__device__ __forceinline__ V3 closest_segment(Simplex& s) {
-    const V3 a = s.w[0].p, b = s.w[1].p;
-    const V3 ab = sub(b, a);
-    const float t = -dot(a, ab);
-    if (t <= 0.0f) {
+__device__ __forceinline__ D3 closest_segment(Simplex& s) {
+    const D3 a = s.w[0].p, b = s.w[1].p;
+    const D3 ab = sub(b, a);
+    const double t = -dot(a, ab);
+    if (t <= 0.0) {
         s.n = 1;
         return a;
     }
-    const float den = dot(ab, ab);
+    const double den = dot(ab, ab);
     if (t >= den) {
         s.w[0] = s.w[1];
         s.n = 1;
@@ -158,62 +186,62 @@ __device__ __forceinline__ V3 closest_segment(Simplex& s) {
 }
 
 // Closest point of triangle (a, b, c) to the origin (Voronoi-region tests).
-__device__ __forceinline__ V3 closest_triangle(Simplex& s) {
-    const MPoint A = s.w[0], B = s.w[1], Cc = s.w[2];
-    const V3 a = A.p, b = B.p, c = Cc.p;
-    const V3 ab = sub(b, a), ac = sub(c, a);
-    const V3 ap = neg(a);
-    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
-    if (d1 <= 0.0f && d2 <= 0.0f) {
+__device__ __forceinline__ D3 closest_triangle(Simplex& s) {
+    const SPoint A = s.w[0], B = s.w[1], Cc = s.w[2];
+    const D3 a = A.p, b = B.p, c = Cc.p;
+    const D3 ab = sub(b, a), ac = sub(c, a);
+    const D3 ap = neg(a);
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) {
         s.n = 1;
         return a;
     }
-    const V3 bp = neg(b);
-    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
-    if (d3 >= 0.0f && d4 <= d3) {
+    const D3 bp = neg(b);
+    const double d3v = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3v >= 0.0 && d4 <= d3v) {
         s.w[0] = B;
         s.n = 1;
         return b;
     }
-    const float vc = d1 * d4 - d3 * d2;
-    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-        const float t = d1 / (d1 - d3);
+    const double vc = d1 * d4 - d3v * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3v <= 0.0) {
+        const double t = d1 / (d1 - d3v);
         s.n = 2;
         return add(a, scl(ab, t));
     }
-    const V3 cp = neg(c);
-    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
-    if (d6 >= 0.0f && d5 <= d6) {
+    const D3 cp = neg(c);
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) {
         s.w[0] = Cc;
         s.n = 1;
         return c;
     }
-    const float vb = d5 * d2 - d1 * d6;
-    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-        const float t = d2 / (d2 - d6);
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double t = d2 / (d2 - d6);
         s.w[1] = Cc;
         s.n = 2;
         return add(a, scl(ac, t));
     }
-    const float va = d3 * d6 - d5 * d4;
-    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
-        const float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const double va = d3v * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3v) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double t = (d4 - d3v) / ((d4 - d3v) + (d5 - d6));
         s.w[0] = B;
         s.w[1] = Cc;
         s.n = 2;
         return add(b, scl(sub(c, b), t));
     }
-    const float den = 1.0f / (va + vb + vc);
-    const float v = vb * den, w = vc * den;
+    const double den = 1.0 / (va + vb + vc);
+    const double v = vb * den, w = vc * den;
     return add(add(a, scl(ab, v)), scl(ac, w));
 }
 
 // True if the origin and d lie strictly on opposite sides of plane (a, b, c).
-__device__ __forceinline__ bool origin_outside(V3 a, V3 b, V3 c, V3 d) {
-    const V3 nrm = cross(sub(b, a), sub(c, a));
-    const float so = -dot(a, nrm);
-    const float sd = dot(sub(d, a), nrm);
-    return so * sd < 0.0f;
+__device__ __forceinline__ bool origin_outside(D3 a, D3 b, D3 c, D3 d) {
+    const D3 nrm = cross(sub(b, a), sub(c, a));
+    const double so = -dot(a, nrm);
+    const double sd = dot(sub(d, a), nrm);
+    return so * sd < 0.0;
 }
 
 // Closest point of tetrahedron w[0..3] to the origin; *inside when it contains it.
@@ -222,8 +250,8 @@ __device__ __forceinline__ bool origin_outside(V3 a, V3 b, V3 c, V3 d) {
 // searched instead. Faces in search order: (0,1,2) opposite 3, (0,2,3)
 // opposite 1, (0,3,1) opposite 2, (1,3,2) opposite 0. Every index is a
 // compile-time constant so the simplex stays in registers.
-__device__ __forceinline__ void tetra_face(const MPoint& a, const MPoint& b, const MPoint& c, const MPoint& opp,
-                                           bool flat, float& best, V3& bestp, Simplex& bests, bool& any) {
+__device__ __forceinline__ void tetra_face(const SPoint& a, const SPoint& b, const SPoint& c, const SPoint& opp,
+                                           bool flat, double& best, D3& bestp, Simplex& bests, bool& any) {
     if (!flat && !origin_outside(a.p, b.p, c.p, opp.p)) return;
     any = true;
     Simplex t;
@@ -231,8 +259,8 @@ __device__ __forceinline__ void tetra_face(const MPoint& a, const MPoint& b, con
     t.w[1] = b;
     t.w[2] = c;
     t.n = 3;
-    const V3 q = closest_triangle(t);
-    const float dq = dot(q, q);
+    const D3 q = closest_triangle(t);
+    const double dq = dot(q, q);
     if (dq < best) {
         best = dq;
         bestp = q;
@@ -240,14 +268,14 @@ __device__ __forceinline__ void tetra_face(const MPoint& a, const MPoint& b, con
     }
 }
 
-__device__ __forceinline__ V3 closest_tetra(Simplex& s, bool* inside) {
-    const MPoint P0 = s.w[0], P1 = s.w[1], P2 = s.w[2], P3 = s.w[3];
-    const V3 ab = sub(P1.p, P0.p), ac = sub(P2.p, P0.p), ad = sub(P3.p, P0.p);
-    const float vol = dot(ad, cross(ab, ac));
-    const float lim = 1e-5f * (sqrtf(dot(ab, ab)) * sqrtf(dot(ac, ac)) * sqrtf(dot(ad, ad)));
-    const bool flat = !(fabsf(vol) > lim);
-    float best = 3.402823466e38f;
-    V3 bestp = v3(0.0f, 0.0f, 0.0f);
+__device__ __forceinline__ D3 closest_tetra(Simplex& s, bool* inside) {
+    const SPoint P0 = s.w[0], P1 = s.w[1], P2 = s.w[2], P3 = s.w[3];
+    const D3 ab = sub(P1.p, P0.p), ac = sub(P2.p, P0.p), ad = sub(P3.p, P0.p);
+    const double vol = dot(ad, cross(ab, ac));
+    const double lim = 1e-5 * (sqrt(dot(ab, ab)) * sqrt(dot(ac, ac)) * sqrt(dot(ad, ad)));
+    const bool flat = !(fabs(vol) > lim);
+    double best = 1.79769313486231e308;
+    D3 bestp = d3(0.0, 0.0, 0.0);
     Simplex bests = s;
     bool any = false;
     tetra_face(P0, P1, P2, P3, flat, best, bestp, bests, any);
@@ -260,7 +288,7 @@ __device__ __forceinline__ V3 closest_tetra(Simplex& s, bool* inside) {
 }
 
 // Reduces s to the feature closest to the origin and returns that point.
-__device__ __forceinline__ V3 simplex_closest(Simplex& s, bool* inside) {
+__device__ __forceinline__ D3 simplex_closest(Simplex& s, bool* inside) {
     *inside = false;
     if (s.n == 1) return s.w[0].p;
     if (s.n == 2) return closest_segment(s);
@@ -272,9 +300,9 @@ __device__ __forceinline__ V3 simplex_closest(Simplex& s, bool* inside) {
 // same arithmetic regardless of the order its vertices were found in. Keys of
 // one simplex are distinct; unused entries sort last. A fixed compare-exchange
 // network keeps every index constant.
-__device__ __forceinline__ void cmpx(MPoint& a, MPoint& b, int& ka, int& kb) {
+__device__ __forceinline__ void cmpx(SPoint& a, SPoint& b, int& ka, int& kb) {
     if (ka > kb) {
-        const MPoint t = a;
+        const SPoint t = a;
         a = b;
         b = t;
         const int k = ka;
@@ -292,7 +320,7 @@ __device__ __forceinline__ void simplex_canonical(Simplex& s) {
     cmpx(s.w[1], s.w[2], k1, k2);
 }
 
-__device__ __forceinline__ void simplex_push(Simplex& s, const MPoint& w) {
+__device__ __forceinline__ void simplex_push(Simplex& s, const SPoint& w) {
     if (s.n == 0) s.w[0] = w;
     else if (s.n == 1) s.w[1] = w;
     else if (s.n == 2) s.w[2] = w;
@@ -366,7 +394,11 @@ struct GjkOut {
 __device__ inline void epa(const Shape& A, const Shape& B, const Simplex& s0, int max_iter, GjkOut& o) {
     MPoint P[EPA_MAX_V];
     int np = 0;
-    for (int i = 0; i < s0.n; ++i) P[np++] = s0.w[i];
+    for (int i = 0; i < s0.n; ++i) {
+        P[np].p = to_v3(s0.w[i].p);  // exact: simplex points are FP32 support points
+        P[np].key = s0.w[i].key;
+        ++np;
+    }
     // grow to a tetrahedron
     const V3 axes[3] = {v3(1.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), v3(0.0f, 0.0f, 1.0f)};
     if (np == 1) {
@@ -422,7 +454,7 @@ __device__ inline void epa(const Shape& A, const Shape& B, const Simplex& s0, in
         const EpaFace fb = F[best];
         const MPoint m = support_M(A, B, fb.n);
         const float gap = dot(m.p, fb.n) - fb.d;
-        if (gap <= GJK_REL_EPS * fabsf(fb.d) + 1e-9f) {
+        if (gap <= EPA_REL_EPS * fabsf(fb.d) + 1e-9f) {
             converged = true;
             break;
         }
@@ -499,45 +531,63 @@ __device__ inline void epa(const Shape& A, const Shape& B, const Simplex& s0, in
 // start or the centre difference); the first point is support_M(init_dir).
 // On return s is the final feature, v its closest point, *hit whether the
 // shapes intersect (then s encloses or touches the origin and EPA follows).
-__device__ inline void gjk_phase(const Shape& A, const Shape& B, V3 init_dir, int max_iter, Simplex& s, V3& v,
-                                 bool* hit, bool* converged, int* iters) {
+// A separation is only reported as converged when the duality gap certifies
+// it: v.v - v.w <= GJK_REL_EPS v.v on the regular exit, <= GJK_STALL_EPS v.v
+// when the support point repeats a simplex vertex or the distance stops
+// decreasing. Such an exit without a certificate and with v.w <= 0 (a point of
+// the Minkowski difference on the origin side) is *uncertain: it goes to EPA as
+// a hit, and if EPA finds no enclosing polytope the uncertified separation is
+// reported with GJK_BUDGET.
+__device__ inline void gjk_phase(const Shape& A, const Shape& B, V3 init_dir, int max_iter, Simplex& s, D3& v,
+                                 bool* hit, bool* converged, bool* uncertain, int* iters) {
     if (dot(init_dir, init_dir) == 0.0f) init_dir = v3(1.0f, 0.0f, 0.0f);
-    s.w[0] = support_M(A, B, init_dir);
+    s.w[0] = spoint(support_M(A, B, init_dir));
     s.n = 1;
     v = s.w[0].p;
     *hit = false;
     *converged = false;
+    *uncertain = false;
     int it = 0;
     while (it < max_iter) {
-        const float vv = dot(v, v);
+        const double vv = dot(v, v);
         if (vv <= GJK_TOUCH_EPS2) {
             *hit = true;
             break;
         }
-        const MPoint w = support_M(A, B, neg(v));
+        const SPoint w = spoint(support_M(A, B, to_v3(neg(v))));
         ++it;
-        const float vw = dot(v, w.p);
+        const double vw = dot(v, w.p);
         if (vv - vw <= GJK_REL_EPS * vv) {
             *converged = true;
             break;
         }
         if (simplex_has(s, w.key)) {
-            *converged = true;
+            if (vv - vw <= GJK_STALL_EPS * vv) {
+                *converged = true;
+            } else if (vw <= 0.0) {
+                *hit = true;
+                *uncertain = true;
+            }
             break;
         }
         const Simplex sp = s;
         simplex_push(s, w);
         bool inside = false;
-        const V3 vn = simplex_closest(s, &inside);
+        const D3 vn = simplex_closest(s, &inside);
         if (inside) {
             *hit = true;
             break;
         }
         if (!(dot(vn, vn) < vv)) {
-            // no progress at FP32 resolution: the previous feature is final if the
-            // duality gap certifies it to 1e-4, otherwise the query failed
+            // no progress at FP64 resolution: the previous feature is final if the
+            // duality gap certifies it to 1e-4
             s = sp;
-            *converged = vv - vw <= GJK_STALL_EPS * vv;
+            if (vv - vw <= GJK_STALL_EPS * vv) {
+                *converged = true;
+            } else if (vw <= 0.0) {
+                *hit = true;
+                *uncertain = true;
+            }
             break;
         }
         v = vn;
@@ -556,8 +606,8 @@ __device__ __forceinline__ GjkOut gjk_out_init() {
 }
 
 // Result of a separated query: distance and normal recomputed canonically from
-// the converged feature.
-__device__ inline void gjk_separated(Simplex& s, V3 v, bool converged, GjkOut& o) {
+// the converged feature (FP64, rounded to FP32 at the end).
+__device__ inline void gjk_separated(Simplex& s, D3 v, bool converged, GjkOut& o) {
     if (!converged) o.flags |= GJK_BUDGET;
     else o.flags |= GJK_CONVERGED;
     bool canon = true;
@@ -568,19 +618,36 @@ __device__ inline void gjk_separated(Simplex& s, V3 v, bool converged, GjkOut& o
         bool inside = false;
         v = simplex_closest(s, &inside);
     }
-    const float dist = sqrtf(dot(v, v));
-    o.sep = dist;
-    o.n = dist > 0.0f ? scl(v, -1.0f / dist) : v3(1.0f, 0.0f, 0.0f);
+    const double dist = sqrt(dot(v, v));
+    o.sep = (float)dist;
+    o.n = dist > 0.0 ? to_v3(scl(v, -1.0 / dist)) : v3(1.0f, 0.0f, 0.0f);
+}
+
+// EPA from the final simplex of a GJK phase; for an uncertain exit whose EPA
+// finds no enclosing polytope, the uncertified separation (GJK_BUDGET) instead.
+__device__ inline void epa_or_separated(const Shape& A, const Shape& B, const Simplex& s, bool uncertain,
+                                        int max_iter, GjkOut& o) {
+    epa(A, B, s, max_iter, o);
+    if (uncertain && (o.flags & GJK_DEGENERATE)) {
+        const int gi = o.iters, ei = o.epa_iters;
+        o = gjk_out_init();
+        o.iters = gi;
+        o.epa_iters = ei;
+        Simplex t = s;
+        bool inside = false;
+        const D3 v = simplex_closest(t, &inside);
+        gjk_separated(t, v, false, o);
+    }
 }
 
 // Complete distance / depth query (GJK, then EPA on a hit).
 __device__ inline GjkOut gjk_query(const Shape& A, const Shape& B, V3 init_dir, int max_iter) {
     GjkOut o = gjk_out_init();
     Simplex s;
-    V3 v;
-    bool hit, converged;
-    gjk_phase(A, B, init_dir, max_iter, s, v, &hit, &converged, &o.iters);
-    if (hit) epa(A, B, s, max_iter, o);
+    D3 v;
+    bool hit, converged, uncertain;
+    gjk_phase(A, B, init_dir, max_iter, s, v, &hit, &converged, &uncertain, &o.iters);
+    if (hit) epa_or_separated(A, B, s, uncertain, max_iter, o);
     else gjk_separated(s, v, converged, o);
     return o;
 }

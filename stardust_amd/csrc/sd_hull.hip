@@ -171,6 +171,135 @@ __global__ void k_sap_sweep(const float4* __restrict__ slo, const float4* __rest
     if (!WRITE) counts[t] = cnt;
 }
 
+// ------------------------------------------------------------------ batched SAP
+//
+// Many hull scenes in one pass. Hulls are concatenated scene by scene
+// (scene_first[s] .. scene_first[s+1]); every scene keeps its own sweep axis and
+// no pair crosses a scene. Keys are (scene, order-preserving float key of the
+// lower extent on the scene's axis, hull index) packed into 64 bits, so one
+// radix sort orders every scene exactly as sd_sap_pairs orders it alone, and
+// the sweep stops at the end of the scene. The per-scene axis is reduced in the
+// same FP64 order as k_axis_partial + k_axis_pick use for a scene of up to
+// AX_BLOCK * AX_BLOCKS hulls (256-element chunk sums by a tree, chunk sums in
+// order), so it is the axis sd_sap_pairs picks for that scene.
+
+constexpr int SAPB_MAX_SCENE = AX_BLOCK * AX_BLOCKS;
+
+__global__ __launch_bounds__(AX_BLOCK) void k_axis_scenes(const float4* __restrict__ lo4,
+                                                          const float4* __restrict__ hi4, int n,
+                                                          const int* __restrict__ scene_first, int nscenes,
+                                                          int* __restrict__ axis, int* __restrict__ error) {
+    __shared__ double red[6][AX_BLOCK];
+    const int sc = blockIdx.x;
+    const int b0 = scene_first[sc], b1 = scene_first[sc + 1];
+    const bool bad = b0 < 0 || b1 > n || b1 < b0 || b1 - b0 > SAPB_MAX_SCENE;
+    if (bad) {
+        if (threadIdx.x == 0) {
+            axis[sc] = 0;
+            atomicOr(error, 1);
+        }
+        return;
+    }
+    double s[6] = {0, 0, 0, 0, 0, 0};  // meaningful in thread 0
+    for (int c0 = b0; c0 < b1; c0 += AX_BLOCK) {
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        const int i = c0 + (int)threadIdx.x;
+        if (i < b1) {
+            const float4 l = lo4[i], h = hi4[i];
+            const double cx = 0.5 * ((double)l.x + (double)h.x);
+            const double cy = 0.5 * ((double)l.y + (double)h.y);
+            const double cz = 0.5 * ((double)l.z + (double)h.z);
+            acc[0] += cx;
+            acc[1] += cy;
+            acc[2] += cz;
+            acc[3] += cx * cx;
+            acc[4] += cy * cy;
+            acc[5] += cz * cz;
+        }
+        for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int h = AX_BLOCK / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h)
+                for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 6; ++k) s[k] += red[k][0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const int m = b1 - b0;
+    int best = 0;
+    double bv = -1.0;
+    for (int a = 0; a < 3; ++a) {
+        const double mean = s[a] / (double)m;
+        const double var = s[3 + a] / (double)m - mean * mean;
+        if (var > bv) {
+            bv = var;
+            best = a;
+        }
+    }
+    axis[sc] = best;
+}
+
+__global__ void k_sapb_keys(const float4* __restrict__ lo4, int n, const int* __restrict__ scene_of, int nscenes,
+                            const int* __restrict__ axis, int ibits, uint64_t* __restrict__ keys,
+                            int* __restrict__ error) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int sc = scene_of[i];
+    if (sc < 0 || sc >= nscenes) {
+        atomicOr(error, 2);
+        sc = 0;
+    }
+    const uint64_t fk = float_key(axis_of(lo4[i], axis[sc]));
+    keys[i] = ((uint64_t)sc << (32 + ibits)) | (fk << ibits) | (uint64_t)i;
+}
+
+__global__ void k_sapb_gather(const uint64_t* __restrict__ keys, int n, int ibits, const float4* __restrict__ lo4,
+                              const float4* __restrict__ hi4, const int* __restrict__ scene_of,
+                              const int* __restrict__ axis, int nscenes, float4* __restrict__ slo,
+                              float4* __restrict__ shi) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(keys[t] & ((1ull << ibits) - 1ull));
+    int sc = scene_of[i];
+    sc = sc < 0 || sc >= nscenes ? 0 : sc;
+    const int a = axis[sc];
+    const float4 l = lo4[i], h = hi4[i];
+    slo[t] = make_float4(axis_of(l, a), axis_of(l, (a + 1) % 3), axis_of(l, (a + 2) % 3), __int_as_float(i));
+    shi[t] = make_float4(axis_of(h, a), axis_of(h, (a + 1) % 3), axis_of(h, (a + 2) % 3), __int_as_float(sc));
+}
+
+// As k_sap_sweep, stopping at the end of the scene as well.
+template <bool WRITE>
+__global__ void k_sapb_sweep(const float4* __restrict__ slo, const float4* __restrict__ shi, int n,
+                             int* __restrict__ counts, const int* __restrict__ offsets, int* __restrict__ pairs,
+                             int capacity) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float4 l = slo[t], h = shi[t];
+    const int id = __float_as_int(l.w);
+    const int sc = __float_as_int(h.w);
+    int o = WRITE ? offsets[t] : 0;
+    int cnt = 0;
+    for (int u = t + 1; u < n; ++u) {
+        const float4 lu = slo[u];
+        const float4 hu = shi[u];
+        if (__float_as_int(hu.w) != sc || lu.x > h.x) break;
+        if (lu.y <= h.y && l.y <= hu.y && lu.z <= h.z && l.z <= hu.z) {
+            if (WRITE && o < capacity) {
+                const int iu = __float_as_int(lu.w);
+                pairs[2 * (size_t)o] = id < iu ? id : iu;
+                pairs[2 * (size_t)o + 1] = id < iu ? iu : id;
+            }
+            ++o;
+            ++cnt;
+        }
+    }
+    if (!WRITE) counts[t] = cnt;
+}
+
 // ------------------------------------------------------------------ GJK
 
 struct HullSet {
@@ -210,7 +339,8 @@ __device__ __forceinline__ void make_shapes(const HullSet& h, int ia, int ib, Sh
 // finish cleanly, or that lies within near_margin of contact, is redone cold
 // with max_iter iterations.
 struct EpaEntry {
-    int4 meta;     // pair, EPA iteration limit, iterations so far, simplex size | mode << 8 (0 cold, 1 warm)
+    int4 meta;     // pair, EPA iteration limit, iterations so far,
+                   // simplex size | mode << 8 (0 cold, 1 warm) | uncertain << 16
     float4 w[4];   // simplex points (x, y, z, key bits)
 };
 
@@ -233,13 +363,14 @@ __device__ __forceinline__ void write_out(int p, const GjkOut& o, int flags, int
 // The EPA of a query runs with that query's iteration limit (the warm budget
 // for a warm-started query, max_iter for a cold one).
 __device__ __forceinline__ void push_epa(EpaEntry* __restrict__ list, int* __restrict__ count, int p, int mode,
-                                         int limit, int spent, const Simplex& s) {
+                                         bool uncertain, int limit, int spent, const Simplex& s) {
     const int k = atomicAdd(count, 1);
     EpaEntry e;
-    e.meta = make_int4(p, limit, spent, s.n | (mode << 8));
+    e.meta = make_int4(p, limit, spent, s.n | (mode << 8) | ((uncertain ? 1 : 0) << 16));
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        e.w[i] = make_float4(s.w[i].p.x, s.w[i].p.y, s.w[i].p.z, __int_as_float(i < s.n ? s.w[i].key : -1));
+        e.w[i] = make_float4((float)s.w[i].p.x, (float)s.w[i].p.y, (float)s.w[i].p.z,
+                             __int_as_float(i < s.n ? s.w[i].key : -1));  // exact: FP32 support points
     list[k] = e;
 }
 
@@ -262,15 +393,15 @@ __global__ __launch_bounds__(64) void k_gjk_phase(const int* __restrict__ pairs,
     V3 dc;
     make_shapes(h, ia, ib, A, B, dc);
     Simplex s;
-    V3 v;
-    bool hit, converged;
+    D3 v;
+    bool hit, converged, uncertain;
     int it = 0, spent = 0;
     if (warm) {
         const V3 wd = v3(warm[3 * (size_t)p], warm[3 * (size_t)p + 1], warm[3 * (size_t)p + 2]);
         const int budget = warm_budget < max_iter ? warm_budget : max_iter;
-        gjk_phase(A, B, wd, budget, s, v, &hit, &converged, &it);
+        gjk_phase(A, B, wd, budget, s, v, &hit, &converged, &uncertain, &it);
         if (hit) {
-            push_epa(list, count, p, 1, budget, it, s);
+            push_epa(list, count, p, 1, uncertain, budget, it, s);
             return;
         }
         GjkOut o = gjk_out_init();
@@ -281,9 +412,9 @@ __global__ __launch_bounds__(64) void k_gjk_phase(const int* __restrict__ pairs,
         }
         spent = it;
     }
-    gjk_phase(A, B, dc, max_iter, s, v, &hit, &converged, &it);
+    gjk_phase(A, B, dc, max_iter, s, v, &hit, &converged, &uncertain, &it);
     if (hit) {
-        push_epa(list, count, p, 0, max_iter, spent + it, s);
+        push_epa(list, count, p, 0, uncertain, max_iter, spent + it, s);
         return;
     }
     GjkOut o = gjk_out_init();
@@ -305,16 +436,17 @@ __global__ __launch_bounds__(64) void k_epa(const int* __restrict__ pairs, HullS
     make_shapes(h, ia, ib, A, B, dc);
     Simplex s;
     s.n = e.meta.w & 0xff;
+    const bool uncertain = ((e.meta.w >> 16) & 1) != 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        s.w[i].p = v3(e.w[i].x, e.w[i].y, e.w[i].z);
+        s.w[i].p = d3((double)e.w[i].x, (double)e.w[i].y, (double)e.w[i].z);
         s.w[i].key = __float_as_int(e.w[i].w);
     }
     int spent = e.meta.z;
     GjkOut o = gjk_out_init();
-    epa(A, B, s, e.meta.y, o);
+    epa_or_separated(A, B, s, uncertain, e.meta.y, o);
     spent += o.epa_iters;
-    if ((e.meta.w >> 8) == 1) {  // warm-started
+    if (((e.meta.w >> 8) & 0xff) == 1) {  // warm-started
         if ((o.flags & GJK_CONVERGED) && !(o.flags & (GJK_DEGENERATE | GJK_EPA_LIMIT)) &&
             fabsf(o.sep) > near_margin) {
             write_out(p, o, o.flags | GJK_WARM_USED, spent, sep, normal, flags, iters);
@@ -425,6 +557,88 @@ extern "C" sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int
     if (axis_out) *axis_out = h_axis;
     if (h_total > capacity)
         return set_error(SD_ERR_CAPACITY, "sd_sap_pairs: %d pairs exceed capacity %d", h_total, capacity);
+    return SD_OK;
+}
+
+static int bits_of(uint64_t v) {  // bits needed for values 0 .. v - 1
+    int b = 1;
+    while (b < 63 && (1ull << b) < v) ++b;
+    return b;
+}
+
+extern "C" sd_status sd_sap_batched_scratch_bytes(int n, int nscenes, size_t* bytes) {
+    SD_REQUIRE(bytes && n >= 0 && nscenes >= 1, "sd_sap_batched_scratch_bytes: bad argument");
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    *bytes = 2 * nn * sizeof(uint64_t) + 2 * nn * sizeof(float4) + 2 * (nn + 2) * sizeof(int) +
+             (size_t)nscenes * sizeof(int) + 64 + radix_temp_bytes(n) + scan_temp_bytes(n) + 8 * 256;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_sap_pairs_batched(const float* lo4, const float* hi4, int n, const int* scene_of,
+                                          const int* scene_first, int nscenes, int* pairs, int capacity,
+                                          int* n_pairs, int* axes, void* scratch, size_t scratch_bytes,
+                                          void* stream) {
+    SD_REQUIRE(n >= 0 && nscenes >= 1 && capacity >= 0 && n_pairs, "sd_sap_pairs_batched: bad argument");
+    if (n == 0) {
+        *n_pairs = 0;
+        return SD_OK;
+    }
+    SD_REQUIRE(lo4 && hi4 && scene_of && scene_first && scratch && (pairs || capacity == 0),
+               "sd_sap_pairs_batched: null pointer");
+    const int ibits = bits_of((uint64_t)n);
+    const int sbits = bits_of((uint64_t)nscenes);
+    SD_REQUIRE(sbits + 32 + ibits <= 64, "sd_sap_pairs_batched: %d hulls in %d scenes do not fit 64-bit keys", n,
+               nscenes);
+    size_t need;
+    sd_sap_batched_scratch_bytes(n, nscenes, &need);
+    SD_REQUIRE(scratch_bytes >= need, "sd_sap_pairs_batched: scratch too small (%zu < %zu)", scratch_bytes, need);
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)scratch;
+    uint64_t* keys = (uint64_t*)carve(p, (size_t)n * 8);
+    uint64_t* keys2 = (uint64_t*)carve(p, (size_t)n * 8);
+    float4* slo = (float4*)carve(p, (size_t)n * sizeof(float4));
+    float4* shi = (float4*)carve(p, (size_t)n * sizeof(float4));
+    int* counts = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
+    int* offsets = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
+    int* axis = (int*)carve(p, (size_t)nscenes * sizeof(int));
+    int* error = (int*)carve(p, 64);
+    void* sort_tmp = (void*)carve(p, radix_temp_bytes(n));
+    void* scan_tmp = (void*)carve(p, scan_temp_bytes(n));
+    int* total = offsets + n;
+    const float4* L = (const float4*)lo4;
+    const float4* H = (const float4*)hi4;
+    sd_status st = launch_fill_u32((uint32_t*)error, 0u, 1, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_axis_scenes, dim3(nscenes), dim3(AX_BLOCK), 0, s, L, H, n, scene_first, nscenes, axis,
+                       error);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sapb_keys, dim3(div_up(n, 256)), dim3(256), 0, s, L, n, scene_of, nscenes,
+                       (const int*)axis, ibits, keys, error);
+    SD_LAUNCH_CHECK();
+    uint64_t* K = keys;
+    uint64_t* Ka = keys2;
+    st = radix_sort_keys_u64(&K, &Ka, n, 0, sbits + 32 + ibits, sort_tmp, radix_temp_bytes(n), s, 0);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_sapb_gather, dim3(div_up(n, 256)), dim3(256), 0, s, (const uint64_t*)K, n, ibits, L, H,
+                       scene_of, (const int*)axis, nscenes, slo, shi);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sapb_sweep<false>, dim3(div_up(n, 128)), dim3(128), 0, s, slo, shi, n, counts, nullptr,
+                       nullptr, 0);
+    SD_LAUNCH_CHECK();
+    st = exclusive_scan_i32(counts, offsets, n, total, scan_tmp, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_sapb_sweep<true>, dim3(div_up(n, 128)), dim3(128), 0, s, slo, shi, n, counts, offsets,
+                       pairs, capacity);
+    SD_LAUNCH_CHECK();
+    if (axes) SD_HIP(hipMemcpyAsync(axes, axis, (size_t)nscenes * sizeof(int), hipMemcpyDeviceToDevice, s));
+    int h[2] = {0, 0};
+    SD_HIP(hipMemcpyAsync(&h[0], total, sizeof(int), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(&h[1], error, sizeof(int), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipStreamSynchronize(s));
+    *n_pairs = h[0];
+    if (h[1]) return set_error(SD_ERR_ARG, "sd_sap_pairs_batched: bad scene ranges or scene ids (bits %d)", h[1]);
+    if (h[0] > capacity)
+        return set_error(SD_ERR_CAPACITY, "sd_sap_pairs_batched: %d pairs exceed capacity %d", h[0], capacity);
     return SD_OK;
 }
 

@@ -99,3 +99,57 @@ class DeviceHulls:
                 "normal": out["normal"].download(np.float32, 3 * npairs).reshape(-1, 3),
                 "flags": out["flags"].download(np.int32, npairs),
                 "iters": out["iters"].download(np.int32, npairs)}
+
+
+def concat_hulls(sets):
+    """Concatenates hull sets (vertex ranges re-based); returns (hulls, scene_first)."""
+    out = {k: [] for k in ("pos4", "quat4", "vel4", "vstart", "nv", "verts4")}
+    first = [0]
+    vbase = 0
+    for h in sets:
+        for k in ("pos4", "quat4", "vel4", "nv", "verts4"):
+            out[k].append(h[k])
+        out["vstart"].append(h["vstart"] + vbase)
+        vbase += len(h["verts4"])
+        first.append(first[-1] + len(h["pos4"]))
+    cat = {k: np.ascontiguousarray(np.concatenate(v)) for k, v in out.items()}
+    cat["vstart"] = cat["vstart"].astype(np.int32)
+    return cat, np.array(first, np.int32)
+
+
+class BatchedHulls(DeviceHulls):
+    """Hull sets of many scenes in one device set; sweep-and-prune per scene in one pass."""
+
+    def __init__(self, sets):
+        h, first = concat_hulls(sets)
+        super().__init__(h)
+        self.nscenes = len(sets)
+        self.scene_first = first
+        scene_of = np.repeat(np.arange(self.nscenes, dtype=np.int32), np.diff(first))
+        self.d_first = L.DeviceBuffer.from_array(first)
+        self.d_scene_of = L.DeviceBuffer.from_array(scene_of)
+        self.d_axes = L.DeviceBuffer(4 * self.nscenes)
+        nb = C.c_size_t()
+        L.call("sd_sap_batched_scratch_bytes", self.n, self.nscenes, C.byref(nb))
+        self.b_scratch_bytes = nb.value
+        self.b_scratch = L.DeviceBuffer(nb.value)
+
+    def sap_batched(self, capacity=None, stream=None):
+        """Overlapping pairs of every scene (device buffer, count)."""
+        cap = capacity or max(16 * self.n, 1024)
+        for _ in range(2):
+            pairs = L.DeviceBuffer(8 * cap)
+            npairs = C.c_int()
+            st = L.load().sd_sap_pairs_batched(self.lo4.ptr, self.hi4.ptr, self.n, self.d_scene_of.ptr,
+                                               self.d_first.ptr, self.nscenes, pairs.ptr, cap, C.byref(npairs),
+                                               self.d_axes.ptr, self.b_scratch.ptr, self.b_scratch_bytes, stream)
+            if st == L.SD_OK:
+                return pairs, npairs.value
+            if st != L.SD_ERR_CAPACITY:
+                L.check(st)
+            cap = npairs.value
+        raise L.SdError(L.SD_ERR_CAPACITY, "sap_batched: pair count changed between attempts")
+
+    def axes(self):
+        L.synchronize()
+        return self.d_axes.download(np.int32, self.nscenes)

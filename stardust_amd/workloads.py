@@ -22,6 +22,9 @@ batch_c5: 512 independent scenes in one batch. Per scene (numpy PCG64 seeded
         of 8 mm, so each box is a walled column whose height is the larger of
         the side and the height that seeds the spheres at solid fraction 0.15.
         Sphere positions: sd_gen_spheres with seed 0x5EED0005 + 1 + scene.
+        A quarter of the scenes (chosen by the same generator) also carry
+        U{1,000..8,000} convex hulls of the hulls_c4 kind, uniform in the scene's
+        box, generated with seed 0x5EED0005 + 0x10000 + scene.
 Seeds are 0x5EED0000 + k for configuration k (1-based).
 """
 import math
@@ -108,6 +111,16 @@ class BatchWorkload:
         solid = self.scene_n * (4.0 / 3.0) * math.pi * self.r_med ** 3 * math.exp(4.5 * r_sigma ** 2)
         self.height = np.maximum(self.side, solid / (phi_seed * self.side ** 2))
         self.n = int(self.scene_n.sum())
+        self.hull_scenes = np.sort(rng.choice(nscenes, nscenes // 4, replace=False)) if nscenes >= 4 else \
+            np.zeros(0, np.int64)
+        self.hull_n = rng.integers(1000, 8001, len(self.hull_scenes))
+
+    def generate_hull_scene(self, j):
+        """Hulls of the j-th hull-carrying scene."""
+        from . import hulls as H
+        k = int(self.hull_scenes[j])
+        return H.generate_hulls(self.seed + 0x10000 + k, int(self.hull_n[j]), lo=(0.0, 0.0, 0.0),
+                                hi=(float(self.side[k]), float(self.side[k]), float(self.height[k])))
 
     def scene(self, k):
         lo = (0.0, 0.0, 0.0)

@@ -100,3 +100,52 @@ def test_gjk_rejects_bad_ids():
     res = d.download_gjk(d.gjk(bad, 4), 4)
     assert res["flags"][1] == -1 and res["flags"][2] == -1
     assert res["flags"][0] >= 0 and res["flags"][3] >= 0
+
+
+def test_sap_batched_equals_each_scene(oracle):
+    say("sap batched vs per scene")
+    rng = np.random.default_rng(21)
+    sets = []
+    for k in range(9):
+        n = int(rng.integers(1, 3000)) if k else 1
+        side = float(rng.uniform(0.1, 0.5))
+        sets.append(H.generate_hulls(0x5EEDC000 + k, n, hi=(side, side * rng.uniform(0.5, 2), side)))
+    b = H.BatchedHulls(sets)
+    b.aabbs()
+    pairs, npairs = b.sap_batched()
+    got = pairs.download(np.int32, 2 * npairs).reshape(-1, 2)
+    axes = b.axes()
+    lo, hi = b.download_aabbs()
+    off = 0
+    rows = []
+    for k, h in enumerate(sets):
+        d = H.DeviceHulls(h)
+        d.aabbs()
+        p, n, ax = d.sap()
+        ref = p.download(np.int32, 2 * n).reshape(-1, 2) + off
+        assert ax == axes[k], k
+        rows.append(ref)
+        # oracle on this scene's boxes
+        m = len(h["pos4"])
+        want, oax = oracle.sap_pairs(lo[off:off + m], hi[off:off + m])
+        assert oax == ax
+        assert set(map(tuple, want.tolist())) == set(map(tuple, (ref - off).tolist()))
+        off += m
+    # same pairs in the same order, scene after scene
+    assert np.array_equal(got, np.concatenate(rows))
+    # GJK on the batched pair list equals the oracle
+    hp = got
+    out = b.download_gjk(b.gjk(pairs, npairs), npairs)
+    cat, _ = H.concat_hulls(sets)
+    exp = oracle.gjk(cat, hp, nthreads=8)
+    assert np.array_equal(out["flags"], exp["flags"]) and np.array_equal(out["sep"], exp["sep"])
+
+
+def test_sap_batched_rejects_bad_ranges():
+    say("sap batched bad ranges")
+    sets = [H.generate_hulls(0x5EEDC100 + k, 100) for k in range(3)]
+    b = H.BatchedHulls(sets)
+    b.aabbs()
+    b.d_first.upload(np.array([0, 100, 50, 300], np.int32))  # decreasing range
+    with pytest.raises(L.SdError):
+        b.sap_batched()
