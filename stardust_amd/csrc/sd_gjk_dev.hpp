@@ -156,11 +156,12 @@ __device__ __forceinline__ D3 cross(D3 a, D3 b) {
     return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
+// Simplex vertex: an FP32 support point (kept in FP32, widened where used).
 struct SPoint {
-    D3 p;
+    V3 p;
     int key;
 };
-__device__ __forceinline__ SPoint spoint(const MPoint& m) { return SPoint{d3(m.p), m.key}; }
+__device__ __forceinline__ SPoint spoint(const MPoint& m) { return SPoint{m.p, m.key}; }
 
 struct Simplex {
     SPoint w[4];
@@ -169,7 +170,7 @@ struct Simplex {
 
 // Closest point of segment (a, b) to the origin; keeps the feature's vertices.
 __device__ __forceinline__ D3 closest_segment(Simplex& s) {
-    const D3 a = s.w[0].p, b = s.w[1].p;
+    const D3 a = d3(s.w[0].p), b = d3(s.w[1].p);
     const D3 ab = sub(b, a);
     const double t = -dot(a, ab);
     if (t <= 0.0) {
@@ -188,7 +189,7 @@ __device__ __forceinline__ D3 closest_segment(Simplex& s) {
 // Closest point of triangle (a, b, c) to the origin (Voronoi-region tests).
 __device__ __forceinline__ D3 closest_triangle(Simplex& s) {
     const SPoint A = s.w[0], B = s.w[1], Cc = s.w[2];
-    const D3 a = A.p, b = B.p, c = Cc.p;
+    const D3 a = d3(A.p), b = d3(B.p), c = d3(Cc.p);
     const D3 ab = sub(b, a), ac = sub(c, a);
     const D3 ap = neg(a);
     const double d1 = dot(ab, ap), d2 = dot(ac, ap);
@@ -250,47 +251,63 @@ __device__ __forceinline__ bool origin_outside(D3 a, D3 b, D3 c, D3 d) {
 // searched instead. Faces in search order: (0,1,2) opposite 3, (0,2,3)
 // opposite 1, (0,3,1) opposite 2, (1,3,2) opposite 0. Every index is a
 // compile-time constant so the simplex stays in registers.
-__device__ __forceinline__ void tetra_face(const SPoint& a, const SPoint& b, const SPoint& c, const SPoint& opp,
-                                           bool flat, double& best, D3& bestp, Simplex& bests, bool& any) {
-    if (!flat && !origin_outside(a.p, b.p, c.p, opp.p)) return;
-    any = true;
+__device__ __forceinline__ double tetra_face(const SPoint& a, const SPoint& b, const SPoint& c, const SPoint& opp,
+                                             bool flat) {
+    if (!flat && !origin_outside(d3(a.p), d3(b.p), d3(c.p), d3(opp.p))) return -1.0;
     Simplex t;
     t.w[0] = a;
     t.w[1] = b;
     t.w[2] = c;
     t.n = 3;
     const D3 q = closest_triangle(t);
-    const double dq = dot(q, q);
-    if (dq < best) {
-        best = dq;
-        bestp = q;
-        bests = t;
-    }
+    return dot(q, q);
 }
 
+// The best face is found first (squared distances only); its reduced feature
+// and closest point are then computed once more (same arithmetic).
 __device__ __forceinline__ D3 closest_tetra(Simplex& s, bool* inside) {
     const SPoint P0 = s.w[0], P1 = s.w[1], P2 = s.w[2], P3 = s.w[3];
-    const D3 ab = sub(P1.p, P0.p), ac = sub(P2.p, P0.p), ad = sub(P3.p, P0.p);
+    const D3 p0 = d3(P0.p);
+    const D3 ab = sub(d3(P1.p), p0), ac = sub(d3(P2.p), p0), ad = sub(d3(P3.p), p0);
     const double vol = dot(ad, cross(ab, ac));
     const double lim = 1e-5 * (sqrt(dot(ab, ab)) * sqrt(dot(ac, ac)) * sqrt(dot(ad, ad)));
     const bool flat = !(fabs(vol) > lim);
     double best = 1.79769313486231e308;
-    D3 bestp = d3(0.0, 0.0, 0.0);
-    Simplex bests = s;
-    bool any = false;
-    tetra_face(P0, P1, P2, P3, flat, best, bestp, bests, any);
-    tetra_face(P0, P2, P3, P1, flat, best, bestp, bests, any);
-    tetra_face(P0, P3, P1, P2, flat, best, bestp, bests, any);
-    tetra_face(P1, P3, P2, P0, flat, best, bestp, bests, any);
-    *inside = !any;
-    if (any) s = bests;
-    return bestp;
+    int bi = -1;
+    double dq;
+    dq = tetra_face(P0, P1, P2, P3, flat);
+    if (dq >= 0.0 && dq < best) {
+        best = dq;
+        bi = 0;
+    }
+    dq = tetra_face(P0, P2, P3, P1, flat);
+    if (dq >= 0.0 && dq < best) {
+        best = dq;
+        bi = 1;
+    }
+    dq = tetra_face(P0, P3, P1, P2, flat);
+    if (dq >= 0.0 && dq < best) {
+        best = dq;
+        bi = 2;
+    }
+    dq = tetra_face(P1, P3, P2, P0, flat);
+    if (dq >= 0.0 && dq < best) {
+        best = dq;
+        bi = 3;
+    }
+    *inside = bi < 0;
+    if (bi < 0) return d3(0.0, 0.0, 0.0);
+    s.w[0] = bi == 3 ? P1 : P0;
+    s.w[1] = bi == 0 ? P1 : (bi == 1 ? P2 : P3);
+    s.w[2] = bi == 0 ? P2 : (bi == 1 ? P3 : (bi == 2 ? P1 : P2));
+    s.n = 3;
+    return closest_triangle(s);
 }
 
 // Reduces s to the feature closest to the origin and returns that point.
 __device__ __forceinline__ D3 simplex_closest(Simplex& s, bool* inside) {
     *inside = false;
-    if (s.n == 1) return s.w[0].p;
+    if (s.n == 1) return d3(s.w[0].p);
     if (s.n == 2) return closest_segment(s);
     if (s.n == 3) return closest_triangle(s);
     return closest_tetra(s, inside);
@@ -320,11 +337,21 @@ __device__ __forceinline__ void simplex_canonical(Simplex& s) {
     cmpx(s.w[1], s.w[2], k1, k2);
 }
 
+// Every slot is assigned unconditionally (a select), so no store has a
+// run-time address and the simplex stays in registers.
+__device__ __forceinline__ SPoint sel(bool c, const SPoint& a, const SPoint& b) {
+    SPoint r;
+    r.p.x = c ? a.p.x : b.p.x;
+    r.p.y = c ? a.p.y : b.p.y;
+    r.p.z = c ? a.p.z : b.p.z;
+    r.key = c ? a.key : b.key;
+    return r;
+}
 __device__ __forceinline__ void simplex_push(Simplex& s, const SPoint& w) {
-    if (s.n == 0) s.w[0] = w;
-    else if (s.n == 1) s.w[1] = w;
-    else if (s.n == 2) s.w[2] = w;
-    else s.w[3] = w;
+    s.w[0] = sel(s.n == 0, w, s.w[0]);
+    s.w[1] = sel(s.n == 1, w, s.w[1]);
+    s.w[2] = sel(s.n == 2, w, s.w[2]);
+    s.w[3] = sel(s.n >= 3, w, s.w[3]);
     ++s.n;
 }
 
@@ -395,7 +422,7 @@ __device__ inline void epa(const Shape& A, const Shape& B, const Simplex& s0, in
     MPoint P[EPA_MAX_V];
     int np = 0;
     for (int i = 0; i < s0.n; ++i) {
-        P[np].p = to_v3(s0.w[i].p);  // exact: simplex points are FP32 support points
+        P[np].p = s0.w[i].p;
         P[np].key = s0.w[i].key;
         ++np;
     }
@@ -543,7 +570,7 @@ __device__ inline void gjk_phase(const Shape& A, const Shape& B, V3 init_dir, in
     if (dot(init_dir, init_dir) == 0.0f) init_dir = v3(1.0f, 0.0f, 0.0f);
     s.w[0] = spoint(support_M(A, B, init_dir));
     s.n = 1;
-    v = s.w[0].p;
+    v = d3(s.w[0].p);
     *hit = false;
     *converged = false;
     *uncertain = false;
@@ -556,7 +583,7 @@ __device__ inline void gjk_phase(const Shape& A, const Shape& B, V3 init_dir, in
         }
         const SPoint w = spoint(support_M(A, B, to_v3(neg(v))));
         ++it;
-        const double vw = dot(v, w.p);
+        const double vw = dot(v, d3(w.p));
         if (vv - vw <= GJK_REL_EPS * vv) {
             *converged = true;
             break;

@@ -369,8 +369,7 @@ __device__ __forceinline__ void push_epa(EpaEntry* __restrict__ list, int* __res
     e.meta = make_int4(p, limit, spent, s.n | (mode << 8) | ((uncertain ? 1 : 0) << 16));
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        e.w[i] = make_float4((float)s.w[i].p.x, (float)s.w[i].p.y, (float)s.w[i].p.z,
-                             __int_as_float(i < s.n ? s.w[i].key : -1));  // exact: FP32 support points
+        e.w[i] = make_float4(s.w[i].p.x, s.w[i].p.y, s.w[i].p.z, __int_as_float(i < s.n ? s.w[i].key : -1));
     list[k] = e;
 }
 
@@ -439,7 +438,7 @@ __global__ __launch_bounds__(64) void k_epa(const int* __restrict__ pairs, HullS
     const bool uncertain = ((e.meta.w >> 16) & 1) != 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        s.w[i].p = d3((double)e.w[i].x, (double)e.w[i].y, (double)e.w[i].z);
+        s.w[i].p = v3(e.w[i].x, e.w[i].y, e.w[i].z);
         s.w[i].key = __float_as_int(e.w[i].w);
     }
     int spent = e.meta.z;
