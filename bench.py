@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # runs on MI355X (FETCH x 2.0 + WRITE x 1.0 KiB, factors from the plain copy kernel);
 # the runs and the per-kernel table are in profiles/bench_hbm_counters.md.
 MEASURED_TRAFFIC = {
-    ("dem_c2_column_4194304", "contact"): (270265.7 * 2.0 + 262144.0) * 1024.0,
+    ("dem_c2_column_4194304", "contact"): (267881.0 * 2.0 + 262144.0) * 1024.0,
 }
 
 
@@ -381,7 +381,7 @@ def batch_section(args):
     # plain: every scene alone in its own context (graph-replayed steps), one after the other
     tot_t, tot_work = 0.0, 0
     firsts = np.concatenate([[0], np.cumsum([d[0] for d in scenes])])
-    for k, (n, g, c) in enumerate(scenes):
+    for k, (n, g, c) in enumerate(scenes if args.plain_steps > 0 else []):
         lo, hi = int(firsts[k]), int(firsts[k + 1])
         sc = SphereScene(g, c, key64=True, use_graph=True)
         sc.upload(s["x"][lo:hi], s["y"][lo:hi], s["z"][lo:hi], s["r"][lo:hi], s["m"][lo:hi])
@@ -393,12 +393,13 @@ def batch_section(args):
         sc.close()
         if k % 128 == 127:
             log("batch: plain form, %d scenes done" % (k + 1))
-    plain = tot_work / tot_t
-    log("batch: plain %.3e particle-steps/s, batched/plain %.1fx" % (plain, batched / plain))
+    plain = tot_work / tot_t if tot_t > 0 else None
+    if plain:
+        log("batch: plain %.3e particle-steps/s, batched/plain %.1fx" % (plain, batched / plain))
     return {"workload": w.name, "scenes": w.nscenes, "spheres": w.n, "cells": w.cells(), "steps": args.batch_steps,
             "batched_particle_steps_per_s": batched, "batched_ms_per_step": round(1000 * t / args.batch_steps, 4),
             "plain_particle_steps_per_s": plain, "plain_steps_per_scene": args.plain_steps,
-            "batched_over_plain": batched / plain, "autotune": tune, "stage_ms": per,
+            "batched_over_plain": batched / plain if plain else None, "autotune": tune, "stage_ms": per,
             "contacts_per_sphere": contacts / w.n,
             "form": "batched: one graph-replayed step for all scenes; plain: each scene alone in its own "
                     "graph-replayed context, scenes one after the other"}
@@ -438,7 +439,7 @@ def main():
                     help="extra sections: batch, gjk, batch_gjk, clumps (comma list, or none)")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-warmup", type=int, default=10)
-    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--plain-steps", type=int, default=20, help="steps per scene of the plain form (0: skip it)")
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-log-seconds", type=float, default=90.0)
     ap.add_argument("--train-epochs", type=int, default=3)
