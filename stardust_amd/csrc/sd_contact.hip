@@ -94,6 +94,7 @@ __device__ __forceinline__ void find_history(const ContactArgs& a, int os, int o
 
 constexpr int XCDS = 8;       // gfx950: 8 accelerator complex dies, one L2 each
 constexpr int CAND_UNROLL = 4; // candidate positions loaded per batch (loads in flight)
+static_assert(CAND_UNROLL - 1 <= CONTACT_POS_PAD, "sorted positions must be padded for a whole batch");
 
 // Workgroup i is dispatched to XCD i % 8. Remap so that every XCD works on a
 // contiguous range of logical blocks: blocks resident on one XCD then touch
@@ -113,6 +114,7 @@ __device__ __forceinline__ int xcd_logical_block(int nblocks) {
 template <int MODE, bool INDIRECT>
 __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     constexpr bool BATCH = MODE == CONTACT_FUSED_BATCH;
+    constexpr bool PADDED = MODE != CONTACT_FORCES;  // fused modes read a context's padded buffer
     const int lb = xcd_logical_block((a.n + (int)blockDim.x - 1) / (int)blockDim.x);
     if (lb < 0) return;
     const int s = lb * (int)blockDim.x + (int)threadIdx.x;
@@ -186,10 +188,13 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
             row_range(g, ranges, cx, y, z, lo, hi);
             for (int t0 = lo; t0 < hi; t0 += CAND_UNROLL) {
                 // issue all loads of the batch first, then test; contacts are
-                // visited in ascending position order
+                // visited in ascending position order. The context's sorted
+                // positions are padded, so a batch reads past hi without
+                // clamping (those lanes are masked out). The sphere itself
+                // has d2 == 0 and is excluded with coincident spheres.
                 float4 q[CAND_UNROLL];
 #pragma unroll
-                for (int k = 0; k < CAND_UNROLL; ++k) q[k] = a.pos[t0 + k < hi ? t0 + k : t0];
+                for (int k = 0; k < CAND_UNROLL; ++k) q[k] = a.pos[PADDED || t0 + k < hi ? t0 + k : t0];
                 uint32_t hit = 0u;
 #pragma unroll
                 for (int k = 0; k < CAND_UNROLL; ++k) {
@@ -198,7 +203,8 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
                     const float dzz = p.z - q[k].z;
                     const float d2 = dx * dx + dyy * dyy + dzz * dzz;
                     const float rs = p.w + q[k].w;
-                    const bool h = (t0 + k < hi) && (t0 + k != s) && (d2 < rs * rs) && (d2 != 0.0f);
+                    // bitwise: evaluated without branches
+                    const bool h = (t0 + k < hi) & (d2 < rs * rs) & (d2 != 0.0f);
                     hit |= h ? (1u << k) : 0u;
                 }
                 while (hit) {

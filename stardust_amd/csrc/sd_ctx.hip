@@ -346,7 +346,7 @@ sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const f
         ALLOC(c->omg[k], n);
         ALLOC(c->ids[k], n);
     }
-    ALLOC(c->posB, n);
+    ALLOC(c->posB, (size_t)n + CONTACT_POS_PAD);
     if (c->p.key64) {
         ALLOC(c->k64[0], n);
         ALLOC(c->k64[1], n);
@@ -387,6 +387,7 @@ sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const f
     SD_HIP(hipMemcpyAsync(c->ids[0], ids.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
     st = launch_fill_u32((uint32_t*)c->hcount[0], 0u, n, c->stream);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->hcount[1], 0u, n, c->stream);
+    if (st == SD_OK) st = launch_fill_u32((uint32_t*)(c->posB + n), 0u, 4 * (size_t)CONTACT_POS_PAD, c->stream);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->overflow, 0u, 1, c->stream);
     if (st != SD_OK) return st;
     c->cur_keys = 0;

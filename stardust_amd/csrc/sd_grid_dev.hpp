@@ -27,20 +27,30 @@ __device__ __forceinline__ uint32_t cell_id(const GridDev& g, float x, float y, 
 
 // Sorted-position range covering cells (cx-1 .. cx+1, y, z). The cells of one
 // row are consecutive cell ids, so their sorted runs are contiguous; empty
-// cells (start == end == 0) are skipped. Empty row: lo == hi == 0.
+// cells (start == end == 0) are skipped. Empty row: lo == hi == 0. The three
+// range entries are loaded together (a clamped x repeats a cell, which does not
+// change the union).
 __device__ __forceinline__ void row_range(const GridDev& g, const int* __restrict__ ranges, int cx, int y, int z,
                                           int& lo, int& hi) {
-    const int x0 = cx > 0 ? cx - 1 : 0;
-    const int x1 = cx < g.nx - 1 ? cx + 1 : g.nx - 1;
     const int base = (z * g.ny + y) * g.nx;
+    const int xa = cx > 0 ? cx - 1 : cx;
+    const int xb = cx < g.nx - 1 ? cx + 1 : cx;
+    const int2 ra = *reinterpret_cast<const int2*>(ranges + 2 * (size_t)(base + xa));
+    const int2 rb = *reinterpret_cast<const int2*>(ranges + 2 * (size_t)(base + cx));
+    const int2 rc = *reinterpret_cast<const int2*>(ranges + 2 * (size_t)(base + xb));
     lo = 0x7fffffff;
     hi = 0;
-    for (int x = x0; x <= x1; ++x) {
-        const int2 r = *reinterpret_cast<const int2*>(ranges + 2 * (size_t)(base + x));
-        if (r.y > r.x) {
-            lo = r.x < lo ? r.x : lo;
-            hi = r.y > hi ? r.y : hi;
-        }
+    if (ra.y > ra.x) {
+        lo = ra.x < lo ? ra.x : lo;
+        hi = ra.y > hi ? ra.y : hi;
+    }
+    if (rb.y > rb.x) {
+        lo = rb.x < lo ? rb.x : lo;
+        hi = rb.y > hi ? rb.y : hi;
+    }
+    if (rc.y > rc.x) {
+        lo = rc.x < lo ? rc.x : lo;
+        hi = rc.y > hi ? rc.y : hi;
     }
     if (hi == 0) lo = 0;
 }

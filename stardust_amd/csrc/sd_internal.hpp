@@ -71,6 +71,10 @@ struct ContactDev {
 ContactDev to_dev(const sd_contact_params& c);
 sd_status check_contact(const sd_contact_params* c);
 
+// Entries past n that the sorted-position buffer of a context carries, so the
+// contact kernel's batched candidate loads need no clamping.
+constexpr int CONTACT_POS_PAD = 4;
+
 // Fused step kernel modes. CONTACT_FUSED_BATCH: packed u64 keys, every sphere
 // takes its grid, contact law and cell-id base from its scene's SceneDev.
 enum { CONTACT_FORCES = 0, CONTACT_FUSED_U32 = 1, CONTACT_FUSED_U64 = 2, CONTACT_FUSED_BATCH = 3 };
