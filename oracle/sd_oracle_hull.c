@@ -508,30 +508,34 @@ static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_i
         }
         const int iv = np;
         P[np++] = m;
+        /* faces that see the new point; the horizon is every edge (a, b), (b, c),
+         * (c, a) of a visible face, in face order, whose reverse edge belongs to no
+         * visible face; the other faces survive in order */
+        int vis[MAXF];
+        for (int f = 0; f < nf; ++f) vis[f] = vdot(F[f].n, vsub(m.p, P[F[f].a].p)) > 0.0f;
         int ea[MAXE], eb[MAXE], ne = 0, overflow = 0, w = 0;
         for (int f = 0; f < nf; ++f) {
-            if (vdot(F[f].n, vsub(m.p, P[F[f].a].p)) > 0.0f) {
-                const int e[3][2] = {{F[f].a, F[f].b}, {F[f].b, F[f].c}, {F[f].c, F[f].a}};
-                for (int k = 0; k < 3; ++k) {
-                    int found = -1;
-                    for (int q = 0; q < ne; ++q)
-                        if (ea[q] == e[k][1] && eb[q] == e[k][0]) found = q;
-                    if (found >= 0) {
-                        ea[found] = ea[ne - 1];
-                        eb[found] = eb[ne - 1];
-                        --ne;
-                    } else if (ne < MAXE) {
-                        ea[ne] = e[k][0];
-                        eb[ne] = e[k][1];
-                        ++ne;
-                    } else {
-                        overflow = 1;
-                    }
+            if (!vis[f]) continue;
+            const int e[3][2] = {{F[f].a, F[f].b}, {F[f].b, F[f].c}, {F[f].c, F[f].a}};
+            for (int k = 0; k < 3; ++k) {
+                int twin = 0;
+                for (int g = 0; g < nf && !twin; ++g) {
+                    if (!vis[g]) continue;
+                    twin = (F[g].a == e[k][1] && F[g].b == e[k][0]) || (F[g].b == e[k][1] && F[g].c == e[k][0]) ||
+                           (F[g].c == e[k][1] && F[g].a == e[k][0]);
                 }
-            } else {
-                F[w++] = F[f];
+                if (twin) continue;
+                if (ne < MAXE) {
+                    ea[ne] = e[k][0];
+                    eb[ne] = e[k][1];
+                    ++ne;
+                } else {
+                    overflow = 1;
+                }
             }
         }
+        for (int f = 0; f < nf; ++f)
+            if (!vis[f]) F[w++] = F[f];
         nf = w;
         if (overflow || nf + ne > MAXF) {
             o->flags |= ELIMIT;

@@ -1,5 +1,6 @@
-// Device-side convex-hull geometry: quaternion rotation, support mapping,
-// GJK distance with Voronoi-region simplex reduction, and EPA penetration depth.
+// Device-side convex-hull geometry: quaternion rotation, support mapping and
+// the GJK distance phase with Voronoi-region simplex reduction (EPA, run by a
+// whole wavefront per pair, is in sd_epa_wave.hpp).
 //
 // Replaces the boolean GJK of the reference sandbox (sandbox/testGJK.cu:52-315:
 // supportPointSoupNaive, computeMinkowskiDifference, expandSimplex3/4, gjkCUDA),
@@ -313,6 +314,10 @@ __device__ __forceinline__ D3 simplex_closest(Simplex& s, bool* inside) {
     return closest_tetra(s, inside);
 }
 
+// The same, as a call: for the rare recomputations of kernels whose code
+// should not hold a second inlined copy of the simplex code.
+__device__ __noinline__ D3 simplex_closest_call(Simplex& s, bool* inside) { return simplex_closest(s, inside); }
+
 // Sorts the simplex by vertex key so that the same feature always yields the
 // same arithmetic regardless of the order its vertices were found in. Keys of
 // one simplex are distinct; unused entries sort last. A fixed compare-exchange
@@ -362,52 +367,7 @@ __device__ __forceinline__ bool simplex_has(const Simplex& s, int key) {
     return dup;
 }
 
-// ---------------------------------------------------------------- EPA
-
-struct EpaFace {
-    int a, b, c;
-    V3 n;
-    float d;
-};
-
-// Face (a, b, c) oriented away from the origin side interior point `inner`.
-__device__ __forceinline__ bool epa_make_face(const MPoint* P, int a, int b, int c, V3 inner, EpaFace& f) {
-    V3 nrm = cross(sub(P[b].p, P[a].p), sub(P[c].p, P[a].p));
-    const float l2 = dot(nrm, nrm);
-    if (!(l2 > 0.0f)) return false;
-    if (dot(nrm, sub(P[a].p, inner)) < 0.0f) {
-        const int t = b;
-        b = c;
-        c = t;
-        nrm = neg(nrm);
-    }
-    const float inv = 1.0f / sqrtf(l2);
-    f.a = a;
-    f.b = b;
-    f.c = c;
-    f.n = scl(nrm, inv);
-    f.d = dot(f.n, P[a].p);
-    return true;
-}
-
-// Canonical depth / normal of a face: vertices ordered by key.
-__device__ __forceinline__ void epa_face_canonical(const MPoint* P, const EpaFace& f, V3 inner, V3& n, float& d) {
-    MPoint q[3] = {P[f.a], P[f.b], P[f.c]};
-    for (int i = 1; i < 3; ++i) {
-        MPoint x = q[i];
-        int j = i - 1;
-        while (j >= 0 && q[j].key > x.key) {
-            q[j + 1] = q[j];
-            --j;
-        }
-        q[j + 1] = x;
-    }
-    V3 nrm = cross(sub(q[1].p, q[0].p), sub(q[2].p, q[0].p));
-    const float l2 = dot(nrm, nrm);
-    if (dot(nrm, sub(q[0].p, inner)) < 0.0f) nrm = neg(nrm);
-    n = scl(nrm, 1.0f / sqrtf(l2));
-    d = dot(n, q[0].p);
-}
+// ---------------------------------------------------------------- results
 
 struct GjkOut {
     float sep;
@@ -416,143 +376,6 @@ struct GjkOut {
     int iters;
     int epa_iters;
 };
-
-// EPA from a GJK simplex that encloses (or touches) the origin.
-__device__ inline void epa(const Shape& A, const Shape& B, const Simplex& s0, int max_iter, GjkOut& o) {
-    MPoint P[EPA_MAX_V];
-    int np = 0;
-    for (int i = 0; i < s0.n; ++i) {
-        P[np].p = s0.w[i].p;
-        P[np].key = s0.w[i].key;
-        ++np;
-    }
-    // grow to a tetrahedron
-    const V3 axes[3] = {v3(1.0f, 0.0f, 0.0f), v3(0.0f, 1.0f, 0.0f), v3(0.0f, 0.0f, 1.0f)};
-    if (np == 1) {
-        for (int k = 0; k < 6 && np == 1; ++k) {
-            const V3 u = (k & 1) ? neg(axes[k >> 1]) : axes[k >> 1];
-            const MPoint m = support_M(A, B, u);
-            const V3 dd = sub(m.p, P[0].p);
-            if (dot(dd, dd) > 1e-20f) P[np++] = m;
-        }
-    }
-    if (np == 2) {
-        const V3 ab = sub(P[1].p, P[0].p);
-        const float ax = fabsf(ab.x), ay = fabsf(ab.y), az = fabsf(ab.z);
-        const V3 e = (ax <= ay && ax <= az) ? axes[0] : (ay <= az ? axes[1] : axes[2]);
-        const V3 u = cross(ab, e);
-        for (int k = 0; k < 2 && np == 2; ++k) {
-            const MPoint m = support_M(A, B, k ? neg(u) : u);
-            const V3 c = cross(ab, sub(m.p, P[0].p));
-            if (dot(c, c) > 1e-24f) P[np++] = m;
-        }
-    }
-    if (np == 3) {
-        const V3 nrm = cross(sub(P[1].p, P[0].p), sub(P[2].p, P[0].p));
-        for (int k = 0; k < 2 && np == 3; ++k) {
-            const MPoint m = support_M(A, B, k ? neg(nrm) : nrm);
-            const float vol = dot(nrm, sub(m.p, P[0].p));
-            if (fabsf(vol) > 1e-24f) P[np++] = m;
-        }
-    }
-    if (np < 4) {
-        o.flags |= GJK_HIT | GJK_DEGENERATE;
-        o.sep = 0.0f;
-        return;
-    }
-    const V3 inner = scl(add(add(P[0].p, P[1].p), add(P[2].p, P[3].p)), 0.25f);
-    EpaFace F[EPA_MAX_F];
-    int nf = 0;
-    const int T[4][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 3}, {1, 3, 2}};
-    for (int k = 0; k < 4; ++k)
-        if (epa_make_face(P, T[k][0], T[k][1], T[k][2], inner, F[nf])) ++nf;
-    if (nf < 4) {
-        o.flags |= GJK_HIT | GJK_DEGENERATE;
-        o.sep = 0.0f;
-        return;
-    }
-    int best = 0;
-    int it = 0;
-    bool converged = false;
-    for (; it < max_iter; ++it) {
-        best = 0;
-        for (int f = 1; f < nf; ++f)
-            if (F[f].d < F[best].d) best = f;
-        const EpaFace fb = F[best];
-        const MPoint m = support_M(A, B, fb.n);
-        const float gap = dot(m.p, fb.n) - fb.d;
-        if (gap <= EPA_REL_EPS * fabsf(fb.d) + 1e-9f) {
-            converged = true;
-            break;
-        }
-        bool dup = false;
-        for (int i = 0; i < np; ++i) dup = dup || (m.key >= 0 && P[i].key == m.key);
-        if (dup || np >= EPA_MAX_V) {
-            converged = dup;
-            if (!dup) o.flags |= GJK_EPA_LIMIT;
-            break;
-        }
-        const int iv = np;
-        P[np++] = m;
-        // remove faces that see the new point, keep the horizon edges
-        int ea[EPA_MAX_E], eb[EPA_MAX_E];
-        int ne = 0;
-        bool overflow = false;
-        int w = 0;
-        for (int f = 0; f < nf; ++f) {
-            if (dot(F[f].n, sub(m.p, P[F[f].a].p)) > 0.0f) {
-                const int e[3][2] = {{F[f].a, F[f].b}, {F[f].b, F[f].c}, {F[f].c, F[f].a}};
-                for (int k = 0; k < 3; ++k) {
-                    int found = -1;
-                    for (int q = 0; q < ne; ++q)
-                        if (ea[q] == e[k][1] && eb[q] == e[k][0]) found = q;
-                    if (found >= 0) {
-                        ea[found] = ea[ne - 1];
-                        eb[found] = eb[ne - 1];
-                        --ne;
-                    } else if (ne < EPA_MAX_E) {
-                        ea[ne] = e[k][0];
-                        eb[ne] = e[k][1];
-                        ++ne;
-                    } else {
-                        overflow = true;
-                    }
-                }
-            } else {
-                F[w++] = F[f];
-            }
-        }
-        nf = w;
-        if (overflow || nf + ne > EPA_MAX_F) {
-            o.flags |= GJK_EPA_LIMIT;
-            break;
-        }
-        for (int q = 0; q < ne; ++q)
-            if (epa_make_face(P, ea[q], eb[q], iv, inner, F[nf])) ++nf;
-        if (nf == 0) {
-            o.flags |= GJK_DEGENERATE;
-            break;
-        }
-    }
-    if (nf == 0) {
-        o.flags |= GJK_HIT | GJK_DEGENERATE;
-        o.sep = 0.0f;
-        o.epa_iters = it;
-        return;
-    }
-    best = 0;
-    for (int f = 1; f < nf; ++f)
-        if (F[f].d < F[best].d) best = f;
-    V3 n;
-    float d;
-    epa_face_canonical(P, F[best], inner, n, d);
-    o.flags |= GJK_HIT | (converged ? GJK_CONVERGED : 0);
-    if (!converged && it >= max_iter) o.flags |= GJK_BUDGET;
-    if (d < 0.0f) o.flags |= GJK_DEGENERATE;  // polytope does not enclose the origin
-    o.sep = -d;
-    o.n = n;
-    o.epa_iters = it;
-}
 
 // GJK phase of a distance query. init_dir: initial search direction (warm
 // start or the centre difference); the first point is support_M(init_dir).
@@ -633,7 +456,9 @@ __device__ __forceinline__ GjkOut gjk_out_init() {
 }
 
 // Result of a separated query: distance and normal recomputed canonically from
-// the converged feature (FP64, rounded to FP32 at the end).
+// the converged feature (FP64, rounded to FP32 at the end). CALL: through
+// simplex_closest_call instead of inline.
+template <bool CALL = false>
 __device__ inline void gjk_separated(Simplex& s, D3 v, bool converged, GjkOut& o) {
     if (!converged) o.flags |= GJK_BUDGET;
     else o.flags |= GJK_CONVERGED;
@@ -643,40 +468,11 @@ __device__ inline void gjk_separated(Simplex& s, D3 v, bool converged, GjkOut& o
     if (canon) {
         simplex_canonical(s);
         bool inside = false;
-        v = simplex_closest(s, &inside);
+        v = CALL ? simplex_closest_call(s, &inside) : simplex_closest(s, &inside);
     }
     const double dist = sqrt(dot(v, v));
     o.sep = (float)dist;
     o.n = dist > 0.0 ? to_v3(scl(v, -1.0 / dist)) : v3(1.0f, 0.0f, 0.0f);
-}
-
-// EPA from the final simplex of a GJK phase; for an uncertain exit whose EPA
-// finds no enclosing polytope, the uncertified separation (GJK_BUDGET) instead.
-__device__ inline void epa_or_separated(const Shape& A, const Shape& B, const Simplex& s, bool uncertain,
-                                        int max_iter, GjkOut& o) {
-    epa(A, B, s, max_iter, o);
-    if (uncertain && (o.flags & GJK_DEGENERATE)) {
-        const int gi = o.iters, ei = o.epa_iters;
-        o = gjk_out_init();
-        o.iters = gi;
-        o.epa_iters = ei;
-        Simplex t = s;
-        bool inside = false;
-        const D3 v = simplex_closest(t, &inside);
-        gjk_separated(t, v, false, o);
-    }
-}
-
-// Complete distance / depth query (GJK, then EPA on a hit).
-__device__ inline GjkOut gjk_query(const Shape& A, const Shape& B, V3 init_dir, int max_iter) {
-    GjkOut o = gjk_out_init();
-    Simplex s;
-    D3 v;
-    bool hit, converged, uncertain;
-    gjk_phase(A, B, init_dir, max_iter, s, v, &hit, &converged, &uncertain, &o.iters);
-    if (hit) epa_or_separated(A, B, s, uncertain, max_iter, o);
-    else gjk_separated(s, v, converged, o);
-    return o;
 }
 
 }  // namespace sd

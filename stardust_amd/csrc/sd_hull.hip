@@ -10,6 +10,7 @@
 // (count, scan, write) instead of 16 fixed slots per object.
 //
 // GJK/EPA replaces gjkCUDA (sandbox/testGJK.cu:237-315); see sd_gjk_dev.hpp.
+#include "sd_epa_wave.hpp"
 #include "sd_gjk_dev.hpp"
 #include "sd_internal.hpp"
 
@@ -326,14 +327,16 @@ __device__ __forceinline__ void make_shapes(const HullSet& h, int ia, int ib, Sh
 }
 
 // GJK/EPA runs as two kernels so that the GJK kernel keeps its simplex in
-// registers at full occupancy and the polytope arrays of EPA (kept in private
-// memory) are paid only by intersecting pairs:
+// registers at full occupancy and the polytope of EPA is paid only by
+// intersecting pairs:
 //   k_gjk_phase  one thread per pair: the GJK phase; separated pairs are
 //                finished, intersecting pairs append their final simplex to a
 //                list (order of the list is irrelevant: every entry is
 //                finished independently and written by pair index)
-//   k_epa        one thread per listed pair: EPA from that simplex, and for a
-//                warm-started entry whose result is rejected, the cold query.
+//   k_epa_wave   one 16-lane group per listed pair: EPA from that simplex
+//                with the polytope in LDS; a warm-started entry whose result
+//                is rejected goes to a retry list, whose pairs get the cold
+//                query (k_gjk_phase<true>, then k_epa_wave again).
 // warm (nullable): per-pair initial direction (A -> B). A warm-started result
 // that is not converged within warm_budget iterations, that EPA could not
 // finish cleanly, or that lies within near_margin of contact, is redone cold
@@ -373,92 +376,126 @@ __device__ __forceinline__ void push_epa(EpaEntry* __restrict__ list, int* __res
     list[k] = e;
 }
 
-__global__ __launch_bounds__(64) void k_gjk_phase(const int* __restrict__ pairs, int npairs, HullSet h, int n_hulls,
+// GJK phase of every pair (RETRY = false, one thread per pair: warm-started
+// when warm directions are given, else cold), or the cold query of the
+// warm-started pairs whose result was rejected (RETRY = true, threads striding
+// over the retry list of (pair, iterations so far)). Hits go to the EPA list;
+// a rejected warm separation goes to the retry list.
+template <bool RETRY>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_gjk_phase(const int* __restrict__ pairs, int npairs, HullSet h, int n_hulls,
                                                   int n_verts, const float* __restrict__ warm, int warm_budget,
                                                   int max_iter, float near_margin, float* __restrict__ sep,
                                                   float* __restrict__ normal, int* __restrict__ flags,
                                                   int* __restrict__ iters, EpaEntry* __restrict__ list,
-                                                  int* __restrict__ count) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
-    const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
-    if (pair_is_bad(h, ia, ib, n_hulls, n_verts)) {  // reported, never dereferenced
-        sep[p] = 0.0f;
-        flags[p] = -1;
-        iters[p] = 0;
-        return;
-    }
-    Shape A, B;
-    V3 dc;
-    make_shapes(h, ia, ib, A, B, dc);
-    Simplex s;
-    D3 v;
-    bool hit, converged, uncertain;
-    int it = 0, spent = 0;
-    if (warm) {
-        const V3 wd = v3(warm[3 * (size_t)p], warm[3 * (size_t)p + 1], warm[3 * (size_t)p + 2]);
-        const int budget = warm_budget < max_iter ? warm_budget : max_iter;
-        gjk_phase(A, B, wd, budget, s, v, &hit, &converged, &uncertain, &it);
+                                                  int* __restrict__ count, int2* __restrict__ retry,
+                                                  int* __restrict__ retry_count) {
+    const int n = RETRY ? *retry_count : npairs;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += RETRY ? gridDim.x * blockDim.x : n) {
+        const int p = RETRY ? retry[t].x : t;
+        const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
+        if (!RETRY && pair_is_bad(h, ia, ib, n_hulls, n_verts)) {  // reported, never dereferenced
+            sep[p] = 0.0f;
+            flags[p] = -1;
+            iters[p] = 0;
+            continue;
+        }
+        Shape A, B;
+        V3 dc;
+        make_shapes(h, ia, ib, A, B, dc);
+        const bool warm_try = !RETRY && warm != nullptr;
+        const int spent = RETRY ? retry[t].y : 0;
+        V3 dir = dc;
+        int limit = max_iter;
+        if (warm_try) {
+            dir = v3(warm[3 * (size_t)p], warm[3 * (size_t)p + 1], warm[3 * (size_t)p + 2]);
+            limit = warm_budget < max_iter ? warm_budget : max_iter;
+        }
+        Simplex s;
+        D3 v;
+        bool hit, converged, uncertain;
+        int it = 0;
+        gjk_phase(A, B, dir, limit, s, v, &hit, &converged, &uncertain, &it);
         if (hit) {
-            push_epa(list, count, p, 1, uncertain, budget, it, s);
-            return;
+            push_epa(list, count, p, warm_try ? 1 : 0, uncertain, limit, spent + it, s);
+            continue;
         }
         GjkOut o = gjk_out_init();
         gjk_separated(s, v, converged, o);
-        if (converged && fabsf(o.sep) > near_margin) {
-            write_out(p, o, o.flags | GJK_WARM_USED, it, sep, normal, flags, iters);
-            return;
-        }
-        spent = it;
+        if (!warm_try) write_out(p, o, o.flags, spent + it, sep, normal, flags, iters);
+        else if (converged && fabsf(o.sep) > near_margin) write_out(p, o, o.flags | GJK_WARM_USED, it, sep, normal, flags, iters);
+        else retry[atomicAdd(retry_count, 1)] = make_int2(p, it);
     }
-    gjk_phase(A, B, dc, max_iter, s, v, &hit, &converged, &uncertain, &it);
-    if (hit) {
-        push_epa(list, count, p, 0, uncertain, max_iter, spent + it, s);
-        return;
-    }
-    GjkOut o = gjk_out_init();
-    gjk_separated(s, v, converged, o);
-    write_out(p, o, o.flags, spent + it, sep, normal, flags, iters);
 }
 
-__global__ __launch_bounds__(64) void k_epa(const int* __restrict__ pairs, HullSet h, int max_iter, float near_margin,
-                                            float* __restrict__ sep, float* __restrict__ normal,
-                                            int* __restrict__ flags, int* __restrict__ iters,
-                                            const EpaEntry* __restrict__ list, const int* __restrict__ count) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= *count) return;
-    const EpaEntry e = list[k];
-    const int p = e.meta.x;
-    const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
-    Shape A, B;
-    V3 dc;
-    make_shapes(h, ia, ib, A, B, dc);
-    Simplex s;
-    s.n = e.meta.w & 0xff;
-    const bool uncertain = ((e.meta.w >> 16) & 1) != 0;
+constexpr int EPA_WAVES_PER_BLOCK = 2;
+constexpr int EPA_GROUPS_PER_BLOCK = EPA_WAVES_PER_BLOCK * EPA_GPW;
+constexpr int GJK_RETRY_MAX_BLOCKS = 1024;
+
+// One 16-lane group per listed pair, persistent over the list (its length is
+// only known on the device); the grid is the number of blocks the device holds
+// at once. The polytope is in the group's LDS region. A warm-started entry
+// whose EPA result is rejected goes to the retry list with the iterations
+// spent so far.
+__global__ __launch_bounds__(64 * EPA_WAVES_PER_BLOCK) void k_epa_wave(
+    const int* __restrict__ pairs, HullSet h, float near_margin, float* __restrict__ sep, float* __restrict__ normal,
+    int* __restrict__ flags, int* __restrict__ iters, const EpaEntry* __restrict__ list,
+    const int* __restrict__ count, int2* __restrict__ retry, int* __restrict__ retry_count) {
+    __shared__ EpaShared sh[EPA_GROUPS_PER_BLOCK];
+    const int gl = threadIdx.x & (EPA_G - 1);
+    const int base = threadIdx.x & (64 - EPA_G);  // first lane of the group within its wave
+    const int grp = threadIdx.x / EPA_G;
+    EpaShared& S = sh[grp];
+    const int n = *count;
+    for (int k = blockIdx.x * EPA_GROUPS_PER_BLOCK + grp; k < n; k += gridDim.x * EPA_GROUPS_PER_BLOCK) {
+        const EpaEntry e = list[k];
+        const int p = e.meta.x;
+        const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
+        Shape A, B;
+        V3 dc;
+        make_shapes(h, ia, ib, A, B, dc);
+        const HullRegs hA = load_hull_regs(A, gl), hB = load_hull_regs(B, gl);
+        Simplex s;
+        s.n = e.meta.w & 0xff;
+        const bool uncertain = ((e.meta.w >> 16) & 1) != 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        s.w[i].p = v3(e.w[i].x, e.w[i].y, e.w[i].z);
-        s.w[i].key = __float_as_int(e.w[i].w);
-    }
-    int spent = e.meta.z;
-    GjkOut o = gjk_out_init();
-    epa_or_separated(A, B, s, uncertain, e.meta.y, o);
-    spent += o.epa_iters;
-    if (((e.meta.w >> 8) & 0xff) == 1) {  // warm-started
-        if ((o.flags & GJK_CONVERGED) && !(o.flags & (GJK_DEGENERATE | GJK_EPA_LIMIT)) &&
-            fabsf(o.sep) > near_margin) {
-            write_out(p, o, o.flags | GJK_WARM_USED, spent, sep, normal, flags, iters);
-            return;
+        for (int i = 0; i < 4; ++i) {
+            s.w[i].p = v3(e.w[i].x, e.w[i].y, e.w[i].z);
+            s.w[i].key = __float_as_int(e.w[i].w);
         }
-        o = gjk_query(A, B, dc, max_iter);
-        spent += o.iters + o.epa_iters;
+        GjkOut o = gjk_out_init();
+        grp_epa_or_separated(A, hA, B, hB, s, uncertain, e.meta.y, o, S, gl, base);
+        const int spent = e.meta.z + o.epa_iters;
+        const bool warm_started = ((e.meta.w >> 8) & 0xff) == 1;
+        const bool accept = !warm_started || ((o.flags & GJK_CONVERGED) &&
+                                              !(o.flags & (GJK_DEGENERATE | GJK_EPA_LIMIT)) &&
+                                              fabsf(o.sep) > near_margin);
+        if (gl == 0) {
+            if (accept) write_out(p, o, o.flags | (warm_started ? GJK_WARM_USED : 0), spent, sep, normal, flags, iters);
+            else retry[atomicAdd(retry_count, 1)] = make_int2(p, spent);
+        }
+        grp_sync();  // the next entry reuses this group's LDS region
     }
-    write_out(p, o, o.flags, spent, sep, normal, flags, iters);
+}
+
+// Blocks of k_epa_wave resident on the current device at once.
+static int epa_resident_blocks() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+    if (cached[dev] == 0) {
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_epa_wave, 64 * EPA_WAVES_PER_BLOCK, 0) !=
+                hipSuccess ||
+            cus < 1 || per < 1)
+            return 1024;
+        cached[dev] = cus * per;
+    }
+    return cached[dev];
 }
 
 size_t gjk_scratch_bytes(int npairs) {
-    return 256 + (size_t)(npairs > 0 ? npairs : 1) * sizeof(EpaEntry);
+    return 256 + (size_t)(npairs > 0 ? npairs : 1) * (sizeof(EpaEntry) + sizeof(int2));
 }
 
 sd_status check_hulls(const float* pos4, const float* quat4, const int* vstart, const int* nv, const float* verts4) {
@@ -662,16 +699,29 @@ extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, co
     SD_REQUIRE(warm_dir == nullptr || warm_budget >= 1, "sd_gjk: warm_budget must be >= 1");
     HullSet h{(const float4*)pos4, (const float4*)quat4, vstart, nv, (const float4*)verts4};
     hipStream_t s = (hipStream_t)stream;
+    // scratch: counters (EPA list, retry list, second EPA list), the EPA list,
+    // the retry list; the second EPA list reuses the first one's memory
     int* count = (int*)scratch;
     EpaEntry* list = (EpaEntry*)((char*)scratch + 256);
-    st = launch_fill_u32((uint32_t*)count, 0u, 1, s);
+    int2* retry = (int2*)((char*)scratch + 256 + (size_t)n_pairs * sizeof(EpaEntry));
+    st = launch_fill_u32((uint32_t*)count, 0u, 3, s);
     if (st != SD_OK) return st;
-    const dim3 grid(div_up(n_pairs, 64));
-    hipLaunchKernelGGL(k_gjk_phase, grid, dim3(64), 0, s, pairs, n_pairs, h, n_hulls, n_verts, warm_dir, warm_budget,
-                       max_iter, near_margin, sep, normal3, flags, iters, list, count);
+    const int epa_blocks = std::min(div_up(n_pairs, EPA_GROUPS_PER_BLOCK), epa_resident_blocks());
+    const dim3 epa_block(64 * EPA_WAVES_PER_BLOCK);
+    hipLaunchKernelGGL(k_gjk_phase<false>, dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h, n_hulls,
+                       n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters, list, count,
+                       retry, count + 1);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_epa, grid, dim3(64), 0, s, pairs, h, max_iter, near_margin, sep, normal3, flags, iters,
-                       (const EpaEntry*)list, (const int*)count);
+    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, near_margin, sep, normal3, flags,
+                       iters, (const EpaEntry*)list, (const int*)count, retry, count + 1);
+    SD_LAUNCH_CHECK();
+    if (warm_dir == nullptr) return SD_OK;  // cold entries are never rejected
+    hipLaunchKernelGGL(k_gjk_phase<true>, dim3(std::min(div_up(n_pairs, 64), GJK_RETRY_MAX_BLOCKS)), dim3(64), 0, s,
+                       pairs, n_pairs, h, n_hulls, n_verts, nullptr, 1, max_iter, near_margin, sep, normal3, flags,
+                       iters, list, count + 2, retry, count + 1);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, near_margin, sep, normal3, flags,
+                       iters, (const EpaEntry*)list, (const int*)(count + 2), retry, count + 1);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
