@@ -93,16 +93,26 @@ __global__ __launch_bounds__(AX_BLOCK) void k_axis_partial(const float4* __restr
 }
 
 // Sums the partials in block order and picks the axis of largest variance.
-__global__ void k_axis_pick(const double* __restrict__ part, int nparts, int n, int* __restrict__ axis) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < nparts; ++b)
-        for (int k = 0; k < 6; ++k) s[k] += part[b * 6 + k];
+// The partials are staged in LDS by the whole block; one thread per component
+// then adds them in block order.
+__global__ __launch_bounds__(AX_BLOCK) void k_axis_pick(const double* __restrict__ part, int nparts, int n,
+                                                        int* __restrict__ axis) {
+    __shared__ double sp[AX_BLOCKS * 6];
+    __shared__ double sum[6];
+    for (int i = threadIdx.x; i < nparts * 6; i += AX_BLOCK) sp[i] = part[i];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double acc = 0.0;
+        for (int b = 0; b < nparts; ++b) acc += sp[b * 6 + threadIdx.x];
+        sum[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     int best = 0;
     double bv = -1.0;
     for (int a = 0; a < 3; ++a) {
-        const double m = s[a] / (double)n;
-        const double var = s[3 + a] / (double)n - m * m;
+        const double m = sum[a] / (double)n;
+        const double var = sum[3 + a] / (double)n - m * m;
         if (var > bv) {
             bv = var;
             best = a;
@@ -143,33 +153,161 @@ __global__ void k_sap_gather(const uint32_t* __restrict__ order, int n, const fl
     shi[t] = make_float4(h0, h1, h2, 0.0f);
 }
 
-// WRITE = false: count the overlapping partners after t; WRITE = true: emit
-// (min id, max id) pairs at offsets[t].
-template <bool WRITE>
-__global__ void k_sap_sweep(const float4* __restrict__ slo, const float4* __restrict__ shi, int n,
-                            int* __restrict__ counts, const int* __restrict__ offsets, int* __restrict__ pairs,
-                            int capacity) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const float4 l = slo[t], h = shi[t];
-    const int id = __float_as_int(l.w);
-    int o = WRITE ? offsets[t] : 0;
+// Sweep over the objects sorted by lower extent on the sweep axis (slo/shi:
+// sweep axis first, then the other two): the number of overlapping partners
+// after t, and the sorted positions of the first SAP_SLOTS of them. A partner
+// search stops at the first lower extent beyond t's upper one (and, with
+// SCENES, at the first object of another scene: shi[].w holds the scene). A
+// block stages consecutive SAP_TILE-object slices of the sorted extents in LDS
+// and its threads scan them from there, until no thread of the block is still
+// searching; a thread tests SAP_CHUNK candidates per step with all their loads
+// in flight at once.
+constexpr int SAP_TILE = 256;
+constexpr int SAP_CHUNK = 8;
+constexpr int SAP_SLOTS = 16;
+
+__device__ __forceinline__ bool sap_stop(const float4& lu, const float4& hu, const float4& h, int sc, bool scenes) {
+    return (scenes & (__float_as_int(hu.w) != sc)) | (lu.x > h.x);
+}
+__device__ __forceinline__ bool sap_overlap(const float4& lu, const float4& hu, const float4& l, const float4& h) {
+    return (lu.y <= h.y) & (l.y <= hu.y) & (lu.z <= h.z) & (l.z <= hu.z);
+}
+
+template <bool SCENES>
+__global__ __launch_bounds__(SAP_TILE) void k_sap_sweep(const float4* __restrict__ slo,
+                                                        const float4* __restrict__ shi, int n,
+                                                        int* __restrict__ counts, int* __restrict__ slots,
+                                                        int* __restrict__ ovf, int* __restrict__ ovf_count) {
+    __shared__ float4 tl[SAP_TILE], th[SAP_TILE];
+    const int t0 = blockIdx.x * SAP_TILE;
+    const int t = t0 + (int)threadIdx.x;
+    float4 l = make_float4(0.0f, 0.0f, 0.0f, 0.0f), h = l;
+    bool active = t < n;
     int cnt = 0;
-    for (int u = t + 1; u < n; ++u) {
-        const float4 lu = slo[u];
-        if (lu.x > h.x) break;  // sorted by lower extent: nothing further overlaps
-        const float4 hu = shi[u];
-        if (lu.y <= h.y && l.y <= hu.y && lu.z <= h.z && l.z <= hu.z) {
-            if (WRITE && o < capacity) {
-                const int iu = __float_as_int(lu.w);
-                pairs[2 * (size_t)o] = id < iu ? id : iu;
-                pairs[2 * (size_t)o + 1] = id < iu ? iu : id;
+    if (active) {
+        l = slo[t];
+        h = shi[t];
+    }
+    const int sc = __float_as_int(h.w);
+    int* my_slots = slots + (size_t)(active ? t : 0) * SAP_SLOTS;
+    for (int base = t0 + 1; base < n; base += SAP_TILE) {
+        if (!__syncthreads_or(active)) break;  // also: every thread is done with the previous slice
+        const int u = base + (int)threadIdx.x;
+        if (u < n) {
+            tl[threadIdx.x] = slo[u];
+            th[threadIdx.x] = shi[u];
+        }
+        __syncthreads();
+        if (active) {
+            const int kend = min(SAP_TILE, n - base);
+            for (int k = max(0, t + 1 - base); k < kend; k += SAP_CHUNK) {
+                float4 lu[SAP_CHUNK], hu[SAP_CHUNK];
+#pragma unroll
+                for (int i = 0; i < SAP_CHUNK; ++i) {
+                    const int kk = min(k + i, SAP_TILE - 1);
+                    lu[i] = tl[kk];
+                    hu[i] = th[kk];
+                }
+                uint32_t stop = 0u, hit = 0u;
+#pragma unroll
+                for (int i = 0; i < SAP_CHUNK; ++i) {  // bitwise: no branch per candidate
+                    const bool in = k + i < kend;
+                    stop |= (uint32_t)(in & sap_stop(lu[i], hu[i], h, sc, SCENES)) << i;
+                    hit |= (uint32_t)(in & sap_overlap(lu[i], hu[i], l, h)) << i;
+                }
+                if (stop) hit &= (1u << __builtin_ctz(stop)) - 1u;  // the first stop ends the search
+                int c = cnt;
+                for (uint32_t m = hit; m && c < SAP_SLOTS; m &= m - 1u) my_slots[c++] = base + k + __builtin_ctz(m);
+                cnt += __popc(hit);
+                if (stop) {
+                    active = false;
+                    break;
+                }
             }
-            ++o;
-            ++cnt;
         }
     }
-    if (!WRITE) counts[t] = cnt;
+    if (t < n) {
+        counts[t] = cnt;
+        if (cnt > SAP_SLOTS) ovf[atomicAdd(ovf_count, 1)] = t;  // finished by k_sap_overflow
+    }
+}
+
+// The first min(count, SAP_SLOTS) pairs of object t at offsets[t], partners in
+// sorted order.
+__global__ void k_sap_emit(const float4* __restrict__ slo, int n, const int* __restrict__ counts,
+                           const int* __restrict__ offsets, const int* __restrict__ slots, int* __restrict__ pairs,
+                           int capacity) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int c = min(counts[t], SAP_SLOTS);
+    if (c == 0) return;
+    const int id = __float_as_int(slo[t].w);
+    const int* my_slots = slots + (size_t)t * SAP_SLOTS;
+    const int o = offsets[t];
+    for (int i = 0; i < c; ++i) {
+        if (o + i < capacity) {
+            const int iu = __float_as_int(slo[my_slots[i]].w);
+            pairs[2 * (size_t)(o + i)] = id < iu ? id : iu;
+            pairs[2 * (size_t)(o + i) + 1] = id < iu ? iu : id;
+        }
+    }
+}
+
+// Objects with more than SAP_SLOTS partners: one wavefront per listed object
+// resumes the search after the last recorded partner, SAP_OVF_STEPS * 64
+// candidates per step (one per lane and sub-step, loads in flight together),
+// and writes the remaining pairs in sorted order (ballot prefix counts).
+constexpr int SAP_OVF_BLOCKS = 64;
+constexpr int SAP_OVF_STEPS = 4;
+
+template <bool SCENES>
+__global__ __launch_bounds__(256) void k_sap_overflow(const float4* __restrict__ slo, const float4* __restrict__ shi,
+                                                      int n, const int* __restrict__ counts,
+                                                      const int* __restrict__ offsets, const int* __restrict__ slots,
+                                                      const int* __restrict__ ovf, const int* __restrict__ ovf_count,
+                                                      int* __restrict__ pairs, int capacity) {
+    const int lane = threadIdx.x & 63;
+    const int nov = *ovf_count;
+    for (int w = blockIdx.x * 4 + (int)threadIdx.x / 64; w < nov; w += gridDim.x * 4) {
+        const int t = ovf[w];
+        const float4 l = slo[t], h = shi[t];
+        const int id = __float_as_int(l.w);
+        const int sc = __float_as_int(h.w);
+        int o = offsets[t] + SAP_SLOTS;
+        int left = counts[t] - SAP_SLOTS;
+        for (int u0 = slots[(size_t)t * SAP_SLOTS + SAP_SLOTS - 1] + 1; left > 0 && u0 < n;
+             u0 += 64 * SAP_OVF_STEPS) {
+            float4 lu[SAP_OVF_STEPS], hu[SAP_OVF_STEPS];
+#pragma unroll
+            for (int j = 0; j < SAP_OVF_STEPS; ++j) {
+                const int u = min(u0 + 64 * j + lane, n - 1);
+                lu[j] = slo[u];
+                hu[j] = shi[u];
+            }
+            bool done = false;
+#pragma unroll
+            for (int j = 0; j < SAP_OVF_STEPS; ++j) {
+                if (done) break;
+                const int u = u0 + 64 * j + lane;
+                const bool in = u < n;
+                const uint64_t stop = __ballot(in && sap_stop(lu[j], hu[j], h, sc, SCENES));
+                uint64_t hit = __ballot(in && sap_overlap(lu[j], hu[j], l, h));
+                if (stop) hit &= (1ull << __builtin_ctzll(stop)) - 1ull;
+                if ((hit >> lane) & 1ull) {
+                    const int q = o + __popcll(hit & ((1ull << lane) - 1ull));
+                    if (q < capacity) {
+                        const int iu = __float_as_int(lu[j].w);
+                        pairs[2 * (size_t)q] = id < iu ? id : iu;
+                        pairs[2 * (size_t)q + 1] = id < iu ? iu : id;
+                    }
+                }
+                o += __popcll(hit);
+                left -= __popcll(hit);
+                done = stop != 0ull || !(u0 + 64 * (j + 1) < n);
+            }
+            if (done) break;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ batched SAP
@@ -270,35 +408,6 @@ __global__ void k_sapb_gather(const uint64_t* __restrict__ keys, int n, int ibit
     const float4 l = lo4[i], h = hi4[i];
     slo[t] = make_float4(axis_of(l, a), axis_of(l, (a + 1) % 3), axis_of(l, (a + 2) % 3), __int_as_float(i));
     shi[t] = make_float4(axis_of(h, a), axis_of(h, (a + 1) % 3), axis_of(h, (a + 2) % 3), __int_as_float(sc));
-}
-
-// As k_sap_sweep, stopping at the end of the scene as well.
-template <bool WRITE>
-__global__ void k_sapb_sweep(const float4* __restrict__ slo, const float4* __restrict__ shi, int n,
-                             int* __restrict__ counts, const int* __restrict__ offsets, int* __restrict__ pairs,
-                             int capacity) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const float4 l = slo[t], h = shi[t];
-    const int id = __float_as_int(l.w);
-    const int sc = __float_as_int(h.w);
-    int o = WRITE ? offsets[t] : 0;
-    int cnt = 0;
-    for (int u = t + 1; u < n; ++u) {
-        const float4 lu = slo[u];
-        const float4 hu = shi[u];
-        if (__float_as_int(hu.w) != sc || lu.x > h.x) break;
-        if (lu.y <= h.y && l.y <= hu.y && lu.z <= h.z && l.z <= hu.z) {
-            if (WRITE && o < capacity) {
-                const int iu = __float_as_int(lu.w);
-                pairs[2 * (size_t)o] = id < iu ? id : iu;
-                pairs[2 * (size_t)o + 1] = id < iu ? iu : id;
-            }
-            ++o;
-            ++cnt;
-        }
-    }
-    if (!WRITE) counts[t] = cnt;
 }
 
 // ------------------------------------------------------------------ GJK
@@ -520,14 +629,20 @@ extern "C" sd_status sd_hull_aabbs(const float* pos4, const float* quat4, const 
     return SD_OK;
 }
 
+// Scratch regions are carved at 256-byte boundaries; sizes below add each
+// region rounded up the same way.
+static size_t pad256(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
 extern "C" sd_status sd_sap_scratch_bytes(int n, size_t* bytes) {
     SD_REQUIRE(bytes && n >= 0, "sd_sap_scratch_bytes: bad argument");
     const size_t nn = (size_t)(n > 0 ? n : 1);
-    *bytes = 4 * nn * sizeof(uint32_t)        // keys, vals, alt keys, alt vals
-             + 2 * nn * sizeof(float4)        // sorted lower / upper boxes
-             + 2 * (nn + 2) * sizeof(int)     // counts, offsets (+ total)
-             + (size_t)AX_BLOCKS * 6 * sizeof(double) + 64  // axis partials + axis
-             + radix_temp_bytes(n) + scan_temp_bytes(n) + 8 * 256;
+    *bytes = 4 * pad256(nn * sizeof(uint32_t))           // keys, vals, alt keys, alt vals
+             + 2 * pad256(nn * sizeof(float4))           // sorted lower / upper boxes
+             + 2 * pad256((nn + 2) * sizeof(int))        // counts, offsets (+ total)
+             + pad256(nn * SAP_SLOTS * sizeof(int))      // first partners of each object
+             + pad256((nn + 1) * sizeof(int))            // objects with more partners (+ count)
+             + pad256((size_t)AX_BLOCKS * 6 * sizeof(double)) + pad256(64)  // axis partials, axis
+             + pad256(radix_temp_bytes(n)) + pad256(scan_temp_bytes(n));
     return SD_OK;
 }
 
@@ -558,6 +673,9 @@ extern "C" sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int
     float4* shi = (float4*)carve(p, (size_t)n * sizeof(float4));
     int* counts = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
     int* offsets = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
+    int* slots = (int*)carve(p, (size_t)n * SAP_SLOTS * sizeof(int));
+    int* ovf = (int*)carve(p, (size_t)(n + 1) * sizeof(int));
+    int* ovf_count = ovf + n;
     double* part = (double*)carve(p, (size_t)AX_BLOCKS * 6 * sizeof(double));
     int* axis = (int*)carve(p, 64);
     void* sort_tmp = (void*)carve(p, radix_temp_bytes(n));
@@ -569,7 +687,7 @@ extern "C" sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int
     const int nb = div_up(n, AX_BLOCK) < AX_BLOCKS ? div_up(n, AX_BLOCK) : AX_BLOCKS;
     hipLaunchKernelGGL(k_axis_partial, dim3(nb), dim3(AX_BLOCK), 0, s, L, H, n, part);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_axis_pick, dim3(1), dim3(64), 0, s, part, nb, n, axis);
+    hipLaunchKernelGGL(k_axis_pick, dim3(1), dim3(AX_BLOCK), 0, s, part, nb, n, axis);
     SD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sap_keys, dim3(div_up(n, 256)), dim3(256), 0, s, L, n, axis, keys, vals);
     SD_LAUNCH_CHECK();
@@ -577,13 +695,18 @@ extern "C" sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int
     if (st != SD_OK) return st;
     hipLaunchKernelGGL(k_sap_gather, dim3(div_up(n, 256)), dim3(256), 0, s, vals, n, L, H, axis, slo, shi);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sap_sweep<false>, dim3(div_up(n, 128)), dim3(128), 0, s, slo, shi, n, counts, nullptr,
-                       nullptr, 0);
+    st = launch_fill_u32((uint32_t*)ovf_count, 0u, 1, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_sap_sweep<false>, dim3(div_up(n, SAP_TILE)), dim3(SAP_TILE), 0, s, slo, shi, n, counts,
+                       slots, ovf, ovf_count);
     SD_LAUNCH_CHECK();
     st = exclusive_scan_i32(counts, offsets, n, total, scan_tmp, s);
     if (st != SD_OK) return st;
-    hipLaunchKernelGGL(k_sap_sweep<true>, dim3(div_up(n, 128)), dim3(128), 0, s, slo, shi, n, counts, offsets, pairs,
+    hipLaunchKernelGGL(k_sap_emit, dim3(div_up(n, 256)), dim3(256), 0, s, slo, n, counts, offsets, slots, pairs,
                        capacity);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sap_overflow<false>, dim3(SAP_OVF_BLOCKS), dim3(256), 0, s, slo, shi, n, counts, offsets,
+                       slots, ovf, ovf_count, pairs, capacity);
     SD_LAUNCH_CHECK();
     int h_total = 0, h_axis = 0;
     SD_HIP(hipMemcpyAsync(&h_total, total, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -605,8 +728,13 @@ static int bits_of(uint64_t v) {  // bits needed for values 0 .. v - 1
 extern "C" sd_status sd_sap_batched_scratch_bytes(int n, int nscenes, size_t* bytes) {
     SD_REQUIRE(bytes && n >= 0 && nscenes >= 1, "sd_sap_batched_scratch_bytes: bad argument");
     const size_t nn = (size_t)(n > 0 ? n : 1);
-    *bytes = 2 * nn * sizeof(uint64_t) + 2 * nn * sizeof(float4) + 2 * (nn + 2) * sizeof(int) +
-             (size_t)nscenes * sizeof(int) + 64 + radix_temp_bytes(n) + scan_temp_bytes(n) + 8 * 256;
+    *bytes = 2 * pad256(nn * sizeof(uint64_t))         // keys, alt keys
+             + 2 * pad256(nn * sizeof(float4))         // sorted lower / upper boxes
+             + 2 * pad256((nn + 2) * sizeof(int))      // counts, offsets (+ total)
+             + pad256(nn * SAP_SLOTS * sizeof(int))    // first partners of each object
+             + pad256((nn + 1) * sizeof(int))          // objects with more partners (+ count)
+             + pad256((size_t)nscenes * sizeof(int)) + pad256(64)  // axes, error flag
+             + pad256(radix_temp_bytes(n)) + pad256(scan_temp_bytes(n));
     return SD_OK;
 }
 
@@ -636,6 +764,9 @@ extern "C" sd_status sd_sap_pairs_batched(const float* lo4, const float* hi4, in
     float4* shi = (float4*)carve(p, (size_t)n * sizeof(float4));
     int* counts = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
     int* offsets = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
+    int* slots = (int*)carve(p, (size_t)n * SAP_SLOTS * sizeof(int));
+    int* ovf = (int*)carve(p, (size_t)(n + 1) * sizeof(int));
+    int* ovf_count = ovf + n;
     int* axis = (int*)carve(p, (size_t)nscenes * sizeof(int));
     int* error = (int*)carve(p, 64);
     void* sort_tmp = (void*)carve(p, radix_temp_bytes(n));
@@ -658,13 +789,18 @@ extern "C" sd_status sd_sap_pairs_batched(const float* lo4, const float* hi4, in
     hipLaunchKernelGGL(k_sapb_gather, dim3(div_up(n, 256)), dim3(256), 0, s, (const uint64_t*)K, n, ibits, L, H,
                        scene_of, (const int*)axis, nscenes, slo, shi);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sapb_sweep<false>, dim3(div_up(n, 128)), dim3(128), 0, s, slo, shi, n, counts, nullptr,
-                       nullptr, 0);
+    st = launch_fill_u32((uint32_t*)ovf_count, 0u, 1, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_sap_sweep<true>, dim3(div_up(n, SAP_TILE)), dim3(SAP_TILE), 0, s, slo, shi, n, counts,
+                       slots, ovf, ovf_count);
     SD_LAUNCH_CHECK();
     st = exclusive_scan_i32(counts, offsets, n, total, scan_tmp, s);
     if (st != SD_OK) return st;
-    hipLaunchKernelGGL(k_sapb_sweep<true>, dim3(div_up(n, 128)), dim3(128), 0, s, slo, shi, n, counts, offsets,
-                       pairs, capacity);
+    hipLaunchKernelGGL(k_sap_emit, dim3(div_up(n, 256)), dim3(256), 0, s, slo, n, counts, offsets, slots, pairs,
+                       capacity);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sap_overflow<true>, dim3(SAP_OVF_BLOCKS), dim3(256), 0, s, slo, shi, n, counts, offsets,
+                       slots, ovf, ovf_count, pairs, capacity);
     SD_LAUNCH_CHECK();
     if (axes) SD_HIP(hipMemcpyAsync(axes, axis, (size_t)nscenes * sizeof(int), hipMemcpyDeviceToDevice, s));
     int h[2] = {0, 0};

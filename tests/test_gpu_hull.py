@@ -149,3 +149,57 @@ def test_sap_batched_rejects_bad_ranges():
     b.d_first.upload(np.array([0, 100, 50, 300], np.int32))  # decreasing range
     with pytest.raises(L.SdError):
         b.sap_batched()
+
+
+def crowded_boxes(rng, n, n_big):
+    """n boxes in the unit cube: small ones, and n_big large ones that overlap
+    hundreds of others (partner lists beyond the per-object slots)."""
+    c = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    r = rng.uniform(0.002, 0.02, (n, 3)).astype(np.float32)
+    big = rng.choice(n, n_big, replace=False)
+    r[big] = rng.uniform(0.2, 0.45, (n_big, 3)).astype(np.float32)
+    lo = np.zeros((n, 4), np.float32)
+    hi = np.zeros((n, 4), np.float32)
+    lo[:, :3] = c - r
+    hi[:, :3] = c + r
+    return lo, hi
+
+
+def test_sap_many_partners_match_oracle(oracle):
+    say("sap crowded boxes")
+    rng = np.random.default_rng(77)
+    n = 6000
+    lo, hi = crowded_boxes(rng, n, 12)
+    d = H.DeviceHulls(H.generate_hulls(0x5EEDC200, n))
+    d.lo4.upload(lo)
+    d.hi4.upload(hi)
+    pairs, npairs, axis = d.sap()
+    got = pairs.download(np.int32, 2 * npairs).reshape(-1, 2)
+    want, oax = oracle.sap_pairs(lo, hi)
+    assert axis == oax
+    pos = np.empty(n, np.int64)
+    pos[np.argsort(lo[:, oax], kind="stable")] = np.arange(n)
+    first = np.where(pos[want[:, 0]] < pos[want[:, 1]], want[:, 0], want[:, 1])
+    assert np.bincount(first, minlength=n).max() > 16 + 4 * 64  # searches resumed over several steps
+    assert np.array_equal(got, want)
+
+
+def test_sap_batched_many_partners_match_oracle(oracle):
+    say("sap batched crowded boxes")
+    rng = np.random.default_rng(78)
+    sizes = [3000, 1, 5000, 2500]
+    sets = [H.generate_hulls(0x5EEDC300 + k, m) for k, m in enumerate(sizes)]
+    b = H.BatchedHulls(sets)
+    boxes = [crowded_boxes(rng, m, max(1, m // 400)) for m in sizes]
+    b.lo4.upload(np.concatenate([x[0] for x in boxes]))
+    b.hi4.upload(np.concatenate([x[1] for x in boxes]))
+    pairs, npairs = b.sap_batched()
+    got = pairs.download(np.int32, 2 * npairs).reshape(-1, 2)
+    axes = b.axes()
+    rows, off = [], 0
+    for k, (lo, hi) in enumerate(boxes):
+        want, oax = oracle.sap_pairs(lo, hi)
+        assert oax == axes[k], k
+        rows.append(want + off)
+        off += len(lo)
+    assert np.array_equal(got, np.concatenate(rows))
