@@ -260,7 +260,25 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
             a.keys_o32[s] = cell_id(g, p.x, p.y, p.z);
             a.vals_o32[s] = (uint32_t)s;
         } else {
-            a.keys_o64[s] = ((uint64_t)(cbase + cell_id(g, p.x, p.y, p.z)) << 32) | (uint64_t)(uint32_t)s;
+            const uint32_t cell_new = cbase + cell_id(g, p.x, p.y, p.z);
+            const uint64_t key_new = ((uint64_t)cell_new << 32) | (uint64_t)(uint32_t)s;
+            a.keys_o64[s] = key_new;
+            // a sphere that left the cell it was sorted in: listed for the next
+            // step's re-sort (one atomic per wave)
+            const uint32_t cell_old = cbase + (uint32_t)((cz * g.ny + cy) * g.nx + cx);
+            const bool moved = a.mcount != nullptr && cell_new != cell_old;
+            const uint64_t m = __ballot(moved);
+            if (moved) {
+                const int leader = __ffsll((long long)m) - 1;
+                int first = 0;
+                if (lane_id() == leader) first = atomicAdd(a.mcount, __popcll(m));
+                first = __shfl(first, leader);
+                const int i = first + __popcll(m & lanemask_lt());
+                if (i < a.mcap) {
+                    a.mnew[i] = key_new;
+                    a.mold[i] = ((uint64_t)cell_old << 32) | (uint64_t)(uint32_t)s;
+                }
+            }
         }
     }
 }

@@ -102,6 +102,13 @@ struct sd_ctx {
     float4* hist[2] = {nullptr, nullptr};
     int cur_hist = 0;
     int* overflow = nullptr;
+    // spheres that changed cell in the last step (u64 keys): count (set above
+    // RESORT_CAP to force the full sort), new and old keys, and the same sorted
+    int* mcount = nullptr;
+    uint64_t* mnew = nullptr;
+    uint64_t* mold = nullptr;
+    uint64_t* msn = nullptr;
+    uint64_t* mso = nullptr;
     double* diag = nullptr;
     int diag_blocks = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -130,7 +137,7 @@ void release(sd_ctx* c) {
     void* bufs[] = {c->pos[0], c->pos[1], c->vel[0], c->vel[1], c->omg[0], c->omg[1], c->ids[0], c->ids[1],
                     c->posB, c->k32[0], c->k32[1], c->v32[0], c->v32[1], c->k64[0], c->k64[1], c->sort_tmp,
                     c->ranges, c->hcount[0], c->hcount[1], c->hist[0], c->hist[1], c->overflow, c->diag,
-                    c->scenes_d, c->scene_of};
+                    c->scenes_d, c->scene_of, c->mcount, c->mnew, c->mold, c->msn, c->mso};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < 2; ++k) {
@@ -145,6 +152,8 @@ void release(sd_ctx* c) {
     c->hcount[0] = c->hcount[1] = nullptr;
     c->hist[0] = c->hist[1] = nullptr;
     c->overflow = nullptr;
+    c->mcount = nullptr;
+    c->mnew = c->mold = c->msn = c->mso = nullptr;
     c->diag = nullptr;
     c->scenes_d = nullptr;
     c->scene_of = nullptr;
@@ -189,19 +198,34 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
     sd_status st;
     MARK(0);
     if (c->p.key64) {
+        // sorted keys from the previous order and the spheres that changed cell
+        // (positions permuted in the same pass), or, when more than RESORT_CAP
+        // did (and after upload), the full radix sort and permutation; the
+        // count decides on the device, so both are in every captured step
         uint64_t* K = c->k64[c->cur_keys];
         uint64_t* Ka = c->k64[1 - c->cur_keys];
-        st = radix_sort_keys_u64(&K, &Ka, n, 32, 32 + c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s, c->items);
+        const uint64_t* unsorted = K;
+        st = radix_sort_keys_u64_gated(&K, &Ka, n, 32, 32 + c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s, c->items,
+                                       c->mcount, RESORT_CAP);
+        if (st != SD_OK) return st;
+        st = launch_resort_u64(c->mcount, c->mnew, c->mold, c->msn, c->mso, unsorted, n, c->ranges, to_dev(c->p.grid),
+                               c->batch ? c->scenes_d : nullptr, c->scene_of, c->pos[so], c->posB, K, s);
         if (st != SD_OK) return st;
         MARK(1);
         st = launch_cell_ranges_u64(K, n, c->n_cells, c->ranges, s);
         if (st != SD_OK) return st;
         MARK(2);
-        st = launch_permute_pos(nullptr, K, n, c->pos[so], c->posB, s);
+        st = launch_permute_pos(nullptr, K, n, c->pos[so], c->posB, s, c->mcount, RESORT_CAP);
         if (st != SD_OK) return st;
         MARK(3);
+        st = launch_fill_u32((uint32_t*)c->mcount, 0u, 1, s);
+        if (st != SD_OK) return st;
         a.old_slot_u64 = K;
         a.keys_o64 = Ka;
+        a.mcount = c->mcount;
+        a.mnew = c->mnew;
+        a.mold = c->mold;
+        a.mcap = RESORT_CAP;
         a.scenes = c->scenes_d;
         a.scene_of = c->scene_of;
         st = launch_contacts(c->batch ? CONTACT_FUSED_BATCH : CONTACT_FUSED_U64, a, s, c->contact_block);
@@ -327,6 +351,10 @@ sd_status check_spheres(const char* who, int n, const float* x, const float* y, 
     return SD_OK;
 }
 
+// The next step's keys do not come from a step of this order (upload,
+// restore): the next sort is the full one.
+sd_status force_full_sort(sd_ctx* c) { return launch_fill_u32((uint32_t*)c->mcount, 0x7fffffffu, 1, c->stream); }
+
 // Allocates the device state for n spheres and uploads it (slot = particle id);
 // keys are computed by the caller.
 sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const float* z, const float* vx,
@@ -364,6 +392,11 @@ sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const f
     ALLOC(c->hist[0], (size_t)maxc * n);
     ALLOC(c->hist[1], (size_t)maxc * n);
     ALLOC(c->overflow, 1);
+    ALLOC(c->mcount, 1);
+    ALLOC(c->mnew, RESORT_CAP);
+    ALLOC(c->mold, RESORT_CAP);
+    ALLOC(c->msn, RESORT_CAP);
+    ALLOC(c->mso, RESORT_CAP);
     c->diag_blocks = div_up(n, DIAG_BLOCK) < 1024 ? div_up(n, DIAG_BLOCK) : 1024;
     ALLOC(c->diag, (size_t)c->diag_blocks * DIAG_VALUES);
 #undef ALLOC
@@ -389,6 +422,7 @@ sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const f
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->hcount[1], 0u, n, c->stream);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)(c->posB + n), 0u, 4 * (size_t)CONTACT_POS_PAD, c->stream);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->overflow, 0u, 1, c->stream);
+    if (st == SD_OK) st = force_full_sort(c);
     if (st != SD_OK) return st;
     c->cur_keys = 0;
     c->cur_hist = 0;
@@ -636,6 +670,8 @@ sd_status snapshot_restore(sd_ctx* c, const Snapshot& sn) {
     role_buffers(c, b, sz);
     for (int k = 0; k < 8; ++k)
         if (sz[k]) SD_HIP(hipMemcpyAsync(b[k], sn.buf + sn.off[k], sz[k], hipMemcpyDeviceToDevice, c->stream));
+    sd_status st = force_full_sort(c);
+    if (st != SD_OK) return st;
     return launch_fill_u32((uint32_t*)c->overflow, 0u, 1, c->stream);
 }
 
@@ -861,6 +897,7 @@ extern "C" sd_status sd_ctx_load(sd_ctx* c, const void* buffer, size_t bytes) {
     }
     if (st == SD_OK) st = h2d(c->hcount[0], n * sizeof(int));
     if (st == SD_OK) st = h2d(c->hist[0], (size_t)c->p.contact.max_contacts * n * sizeof(float4));
+    if (st == SD_OK) st = force_full_sort(c);
     if (st != SD_OK) return st;
     SD_HIP(hipStreamSynchronize(c->stream));
     c->steps_done = h.steps_done;

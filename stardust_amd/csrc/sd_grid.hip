@@ -178,7 +178,9 @@ sd_status launch_permute(const uint32_t* perm32, const uint64_t* perm64, int n, 
 }
 
 __global__ void k_permute_pos(const uint32_t* __restrict__ perm32, const uint64_t* __restrict__ perm64, int n,
-                              const float4* __restrict__ pos, float4* __restrict__ pos_o) {
+                              const float4* __restrict__ pos, float4* __restrict__ pos_o,
+                              const int* __restrict__ gate, int gate_cap) {
+    if (gate && *gate <= gate_cap) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t src = perm32 ? perm32[t] : (uint32_t)perm64[t];
@@ -187,9 +189,10 @@ __global__ void k_permute_pos(const uint32_t* __restrict__ perm32, const uint64_
 }
 
 sd_status launch_permute_pos(const uint32_t* perm32, const uint64_t* perm64, int n, const float4* pos, float4* pos_o,
-                             hipStream_t s) {
+                             hipStream_t s, const int* gate, int gate_cap) {
     if (n <= 0) return SD_OK;
-    hipLaunchKernelGGL(k_permute_pos, dim3(div_up(n, 256)), dim3(256), 0, s, perm32, perm64, n, pos, pos_o);
+    hipLaunchKernelGGL(k_permute_pos, dim3(div_up(n, 256)), dim3(256), 0, s, perm32, perm64, n, pos, pos_o, gate,
+                       gate_cap);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

@@ -20,6 +20,12 @@ sd_status radix_sort_pairs_u32(uint32_t** keys, uint32_t** vals, uint32_t** keys
                                hipStream_t s, int items);
 sd_status radix_sort_keys_u64(uint64_t** keys, uint64_t** keys_alt, int n, int begin_bit, int end_bit,
                               void* temp, size_t temp_bytes, hipStream_t s, int items);
+// As radix_sort_keys_u64, with an odd number of passes (the result always ends
+// in *keys_alt's original buffer) and every kernel skipped on the device while
+// *gate <= gate_cap.
+sd_status radix_sort_keys_u64_gated(uint64_t** keys, uint64_t** keys_alt, int n, int begin_bit, int end_bit,
+                                    void* temp, size_t temp_bytes, hipStream_t s, int items, const int* gate,
+                                    int gate_cap);
 // Number of LSD passes the sort makes over a bit range.
 int radix_passes(int begin_bit, int end_bit);
 
@@ -57,8 +63,9 @@ sd_status launch_permute(const uint32_t* perm32, const uint64_t* perm64, int n, 
                          const float4* omg, const int* ids, float4* pos_o, float4* vel_o, float4* omg_o, int* ids_o,
                          hipStream_t s);
 
+// gate (nullable): skipped on the device while *gate <= gate_cap.
 sd_status launch_permute_pos(const uint32_t* perm32, const uint64_t* perm64, int n, const float4* pos, float4* pos_o,
-                             hipStream_t s);
+                             hipStream_t s, const int* gate = nullptr, int gate_cap = 0);
 
 // ------------------------------------------------------------------ contacts
 struct ContactDev {
@@ -120,11 +127,27 @@ struct ContactArgs {
     uint32_t* keys_o32;
     uint32_t* vals_o32;
     uint64_t* keys_o64;
+    // fused u64 modes: spheres whose cell changed this step, appended as
+    // (new key, old key) while *mcount < mcap (nullable: not recorded)
+    int* mcount;
+    uint64_t* mnew;
+    uint64_t* mold;
+    int mcap;
     // CONTACT_FUSED_BATCH: scene table and scene of every slot
     const SceneDev* scenes;
     const int* scene_of;
 };
 sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int block);
+
+// Sorted order of the next step's u64 keys from the previous sorted order and
+// the spheres that changed cell (see sd_resort.hip). Runs only while
+// *mcount <= RESORT_CAP (the full sort covers the other case): writes the
+// sorted keys to keys_out and the positions in that order to posB.
+constexpr int RESORT_CAP = 1024;
+sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint64_t* mold, uint64_t* msorted_new,
+                            uint64_t* msorted_old, const uint64_t* keys, int n, const int* ranges_prev,
+                            const GridDev& g, const SceneDev* scenes, const int* scene_of, const float4* pos_state,
+                            float4* posB, uint64_t* keys_out, hipStream_t s);
 sd_status launch_integrate(float4* pos, float4* vel, float4* omg, const float4* f, const float4* t, int n,
                            const ContactDev& c, hipStream_t s);
 

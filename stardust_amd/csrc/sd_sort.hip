@@ -41,7 +41,9 @@ __device__ __forceinline__ uint64_t rs_match(uint32_t d, bool valid, int nbits) 
 
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const K* __restrict__ keys, int n, int bit, int nbits,
-                                                      uint32_t* __restrict__ hist, int ntiles) {
+                                                      uint32_t* __restrict__ hist, int ntiles,
+                                                      const int* __restrict__ gate, int gate_cap) {
+    if (gate && *gate <= gate_cap) return;
     __shared__ uint32_t cnt[RS_RADIX];
     const int radix = 1 << nbits;
     const uint32_t mask = (uint32_t)radix - 1u;
@@ -69,7 +71,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const K* __restrict__ keys
 }
 
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scan(uint32_t* __restrict__ hist, int ntiles,
-                                                      uint32_t* __restrict__ totals) {
+                                                      uint32_t* __restrict__ totals, const int* __restrict__ gate,
+                                                      int gate_cap) {
+    if (gate && *gate <= gate_cap) return;
     __shared__ uint32_t wsum[RS_WAVES];
     uint32_t* row = hist + (long)blockIdx.x * ntiles;
     uint32_t carry = 0;
@@ -88,7 +92,9 @@ template <typename K, bool HASV, int ITEMS>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                          K* __restrict__ kout, uint32_t* __restrict__ vout, int n,
                                                          int bit, int nbits, const uint32_t* __restrict__ hist,
-                                                         const uint32_t* __restrict__ totals, int ntiles) {
+                                                         const uint32_t* __restrict__ totals, int ntiles,
+                                                         const int* __restrict__ gate, int gate_cap) {
+    if (gate && *gate <= gate_cap) return;
     constexpr int TILE = RS_BLOCK * ITEMS;
     __shared__ uint32_t wcnt[RS_WAVES][RS_RADIX];
     __shared__ uint32_t tpre[RS_RADIX];
@@ -198,21 +204,26 @@ size_t radix_temp_bytes(int n) {
 
 template <typename K, bool HASV, int ITEMS>
 static sd_status rs_pass(const K* kin, const uint32_t* vin, K* kout, uint32_t* vout, int n, int bit, int nbits,
-                         uint32_t* hist, uint32_t* totals, hipStream_t s) {
+                         uint32_t* hist, uint32_t* totals, hipStream_t s, const int* gate, int gate_cap) {
     const int ntiles = div_up(n, RS_BLOCK * ITEMS);
-    hipLaunchKernelGGL((k_rs_hist<K, ITEMS>), dim3(ntiles), dim3(RS_BLOCK), 0, s, kin, n, bit, nbits, hist, ntiles);
+    hipLaunchKernelGGL((k_rs_hist<K, ITEMS>), dim3(ntiles), dim3(RS_BLOCK), 0, s, kin, n, bit, nbits, hist, ntiles,
+                       gate, gate_cap);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_rs_scan, dim3(1 << nbits), dim3(RS_BLOCK), 0, s, hist, ntiles, totals);
+    hipLaunchKernelGGL(k_rs_scan, dim3(1 << nbits), dim3(RS_BLOCK), 0, s, hist, ntiles, totals, gate, gate_cap);
     SD_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_rs_scatter<K, HASV, ITEMS>), dim3(ntiles), dim3(RS_BLOCK), 0, s, kin, vin, kout, vout, n,
-                       bit, nbits, hist, totals, ntiles);
+                       bit, nbits, hist, totals, ntiles, gate, gate_cap);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
 
+// gate (nullable): every kernel returns at once while *gate <= gate_cap, so a
+// captured sort can be switched off on the device. odd_passes: at least one and
+// an odd number of passes, so the result always ends in the alternate buffer.
 template <typename K, bool HASV>
 static sd_status rs_sort(K** keys, uint32_t** vals, K** keys_alt, uint32_t** vals_alt, int n, int begin_bit,
-                         int end_bit, void* temp, size_t temp_bytes, hipStream_t s, int items) {
+                         int end_bit, void* temp, size_t temp_bytes, hipStream_t s, int items,
+                         const int* gate = nullptr, int gate_cap = 0, bool odd_passes = false) {
     SD_REQUIRE(n >= 0, "radix sort: n < 0");
     SD_REQUIRE(begin_bit >= 0 && end_bit <= (int)(8 * sizeof(K)) && begin_bit <= end_bit,
                "radix sort: bad bit range [%d, %d)", begin_bit, end_bit);
@@ -225,7 +236,8 @@ static sd_status rs_sort(K** keys, uint32_t** vals, K** keys_alt, uint32_t** val
     SD_REQUIRE(items == 4 || items == 8 || items == 16, "radix sort: items must be 4, 8 or 16");
 
     const int bits = end_bit - begin_bit;
-    const int passes = radix_passes(begin_bit, end_bit);
+    int passes = radix_passes(begin_bit, end_bit);
+    if (odd_passes && passes % 2 == 0) ++passes;  // (a zero-bit pass is a stable copy)
     uint32_t* hist = (uint32_t*)temp;
     const long ntiles_max = div_up(n, RS_BLOCK * 4);
     uint32_t* totals = hist + RS_RADIX * ntiles_max;
@@ -239,11 +251,11 @@ static sd_status rs_sort(K** keys, uint32_t** vals, K** keys_alt, uint32_t** val
         uint32_t* vout = HASV ? *vals_alt : nullptr;
         sd_status st;
         if (items == 4)
-            st = rs_pass<K, HASV, 4>(kin, vin, kout, vout, n, bit, nbits, hist, totals, s);
+            st = rs_pass<K, HASV, 4>(kin, vin, kout, vout, n, bit, nbits, hist, totals, s, gate, gate_cap);
         else if (items == 8)
-            st = rs_pass<K, HASV, 8>(kin, vin, kout, vout, n, bit, nbits, hist, totals, s);
+            st = rs_pass<K, HASV, 8>(kin, vin, kout, vout, n, bit, nbits, hist, totals, s, gate, gate_cap);
         else
-            st = rs_pass<K, HASV, 16>(kin, vin, kout, vout, n, bit, nbits, hist, totals, s);
+            st = rs_pass<K, HASV, 16>(kin, vin, kout, vout, n, bit, nbits, hist, totals, s, gate, gate_cap);
         if (st != SD_OK) return st;
         *keys = kout;
         *keys_alt = kin;
@@ -266,6 +278,13 @@ sd_status radix_sort_keys_u64(uint64_t** keys, uint64_t** keys_alt, int n, int b
                               size_t temp_bytes, hipStream_t s, int items) {
     return rs_sort<uint64_t, false>(keys, nullptr, keys_alt, nullptr, n, begin_bit, end_bit, temp, temp_bytes, s,
                                     items);
+}
+
+sd_status radix_sort_keys_u64_gated(uint64_t** keys, uint64_t** keys_alt, int n, int begin_bit, int end_bit,
+                                    void* temp, size_t temp_bytes, hipStream_t s, int items, const int* gate,
+                                    int gate_cap) {
+    return rs_sort<uint64_t, false>(keys, nullptr, keys_alt, nullptr, n, begin_bit, end_bit, temp, temp_bytes, s,
+                                    items, gate, gate_cap, true);
 }
 
 }  // namespace sd

@@ -384,3 +384,32 @@ def test_diagnostics_momentum_energy(oracle):
     assert np.all(np.abs(d1["angular_momentum"] - d0["angular_momentum"]) < 1e-5 * scale)
     assert d1["kinetic"] <= d0["kinetic"] * (1 + 1e-4)
     sc.close()
+
+
+@pytest.mark.parametrize("n,speed", [(6000, 150.0), (30000, 150.0), (6000, 2.0)])
+def test_step_many_cell_changes(oracle, n, speed):
+    """Fast spheres change cell every few steps: hundreds per step at n = 6000
+    (the step re-sorts from the previous order and the changed spheres), a few
+    thousand at n = 30000 (the full radix sort), none to a few at the low speed;
+    every case equals the oracle bit for bit."""
+    say("many cell changes", n, speed)
+    rng = np.random.default_rng(41)
+    box, r = 0.2, 0.004
+    pos = rng.uniform(r, box - r, (n, 3)).astype(np.float32)
+    rad = rng.uniform(0.5 * r, r, n).astype(np.float32)
+    m = W.sphere_mass(rad)
+    vel = rng.normal(0, speed, (n, 3)).astype(np.float32)
+    init = {"x": pos[:, 0].copy(), "y": pos[:, 1].copy(), "z": pos[:, 2].copy(), "r": rad, "m": m,
+            "vx": vel[:, 0].copy(), "vy": vel[:, 1].copy(), "vz": vel[:, 2].copy()}
+    st = oracle.SphereState(pos[:, 0], pos[:, 1], pos[:, 2], rad, m, vx=vel[:, 0], vy=vel[:, 1], vz=vel[:, 2],
+                            max_contacts=32)
+    nc = int(math.ceil(box / (2 * r)))
+    g = L.Grid.make((0.0, 0.0, 0.0), 2 * r, (nc, nc, nc))
+    c = L.ContactParams.make(1e5, 0.3, 0.5, 2e-6, gravity=(0.0, 0.0, -9.81),
+                             walls=((0.0, 0.0, 0.0), (box, box, box)), max_contacts=32)
+    sc = make_scene(init, g, c, key64=True, graph=True)
+    st.step(g, c, nsteps=12, nthreads=8)
+    sc.step(12)
+    compare_scene(sc, st)
+    assert sc.contact_count() == int(st.hcount.sum())
+    sc.close()
