@@ -61,13 +61,19 @@ def stage_bytes(w, n, contacts_per_sphere, n_cells):
     bits = max(1, int(math.ceil(math.log2(n_cells))))
     passes = (bits + 7) // 8
     c = contacts_per_sphere
+    if w.key64:
+        # re-sort from the previous order: read keys and pos4, write sorted keys
+        # and pos4 (the permutation rides on it; the changed-cell lists are tiny)
+        sort_b, permute_b = n * 2 * (key_b + 16), 0
+    else:
+        # one histogram read of the keys, then per pass read + write of keys and
+        # values; permute: read the permutation, gather pos4, write sorted pos4
+        sort_b, permute_b = n * (key_b + passes * 2 * (key_b + val_b)), n * ((key_b + val_b) + 2 * 16)
     return {
-        # one histogram read of the keys, then per pass read + write of keys (and values)
-        "sort": n * (key_b + passes * 2 * (key_b + val_b)),
+        "sort": sort_b,
         # read every sorted key once, clear + write start/end of every cell
         "ranges": n * key_b + 8 * n_cells,
-        # read the permutation (key/value), gather pos4, write sorted pos4
-        "permute": n * ((key_b + val_b) + 2 * 16),
+        "permute": permute_b,
         # own state 52 B + old slot + history count in/out 8 B + state out 52 B + next key;
         # per contact: old history row 16 B read + new row 16 B written
         "contact": n * (52 + (key_b + val_b) + 8 + 52 + (key_b + val_b)) + n * c * 32,
@@ -493,6 +499,7 @@ def main():
     value = w.n * steps / t
     log("timed: wall %.3f s, events %.3f s -> %.3e particle-steps/s" % (wall, gpu_s, value))
     contacts = sc.contact_count()
+    changes, full_next = sc.cell_changes()
     diag = sc.diagnostics()
 
     prof = sc.profile(args.profile_steps)
@@ -502,7 +509,8 @@ def main():
     dominant = max(per_launch, key=per_launch.get)
     achieved = sbytes[dominant] / (per_launch[dominant] / 1000.0) / 1e9
     stages = {k: {"ms": round(per_launch[k], 5), "bytes": int(sbytes[k]),
-                  "GBs": round(sbytes[k] / (per_launch[k] / 1000.0) / 1e9, 1)} for k in per_launch}
+                  "GBs": round(sbytes[k] / (per_launch[k] / 1000.0) / 1e9, 1) if per_launch[k] > 0 else None}
+              for k in per_launch}
     sc.close()
 
     cpu = None
@@ -537,6 +545,7 @@ def main():
                      "traffic": MEASURED_TRAFFIC.get((w.name, dominant)), "algorithmic_bytes": sbytes[dominant]},
         "stages": stages,
         "contacts_per_sphere": contacts / w.n,
+        "cell_changes_last_timed_step": {"spheres": changes, "full_sort_next": full_next},
         "energy_J": {"kinetic": diag["kinetic"], "potential": diag["potential"]},
         "cpu_baseline": cpu,
     }

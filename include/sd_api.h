@@ -184,9 +184,17 @@ sd_status sd_ctx_upload_spheres(sd_ctx* ctx, int n, const float* x, const float*
 sd_status sd_ctx_step(sd_ctx* ctx, int nsteps);
 /* Runs nsteps steps without graph replay, recording HIP events on the context
  * stream around each stage, and adds the time of each stage summed over the
- * steps to stage_ms: [0] radix sort (all passes), [1] cell ranges (clear + mark),
- * [2] permute, [3] fused contact + integrate + next keys. */
+ * steps to stage_ms: [0] sort, [1] cell ranges (clear + mark), [2] permute,
+ * [3] fused contact + integrate + next keys. With u64 keys, [0] is the re-sort
+ * from the previous order (positions permuted in the same pass) when at most
+ * 1024 spheres changed cell in the previous step, else the radix sort (all
+ * passes), after which [2] permutes the positions. */
 sd_status sd_ctx_profile(sd_ctx* ctx, int nsteps, float* stage_ms, int nstages);
+/* Spheres that changed cell in the last step (u64 keys; 0 with u32 keys):
+ * *full_next = 1 if the next step uses the full radix sort (more than 1024
+ * changes, or after upload / load), else 0. Diagnostic with no counterpart in
+ * the reference, whose sort does not depend on the previous order. */
+sd_status sd_ctx_cell_changes(sd_ctx* ctx, int* count, int* full_next);
 /* Host arrays in particle-id order (any pointer may be NULL). */
 sd_status sd_ctx_download_spheres(sd_ctx* ctx, float* x, float* y, float* z,
                                   float* vx, float* vy, float* vz, float* wx, float* wy, float* wz);

@@ -122,6 +122,7 @@ _SIGS = {
     "sd_ctx_profile": [_P, _I, _P, _I],
     "sd_ctx_download_spheres": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "sd_ctx_contact_count": [_P, C.POINTER(C.c_longlong)],
+    "sd_ctx_cell_changes": [_P, C.POINTER(_I), C.POINTER(_I)],
     "sd_ctx_diagnostics": [_P, _P],
     "sd_ctx_stream": [_P, C.POINTER(C.c_void_p)],
     "sd_ctx_state_bytes": [_P, C.POINTER(_SZ)],

@@ -802,6 +802,20 @@ extern "C" sd_status sd_ctx_contact_count(sd_ctx* c, long long* count) {
     return SD_OK;
 }
 
+extern "C" sd_status sd_ctx_cell_changes(sd_ctx* c, int* count, int* full_next) {
+    SD_REQUIRE(c && count && full_next, "sd_ctx_cell_changes: NULL");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_cell_changes: no spheres uploaded");
+    *count = 0;
+    *full_next = c->p.key64 ? 0 : 1;
+    if (!c->p.key64) return SD_OK;
+    int m = 0;
+    SD_HIP(hipMemcpyAsync(&m, c->mcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipStreamSynchronize(c->stream));
+    *full_next = m > RESORT_CAP ? 1 : 0;
+    *count = m == 0x7fffffff ? 0 : m;
+    return SD_OK;
+}
+
 // ------------------------------------------------------------------ checkpoint
 
 namespace {

@@ -39,7 +39,9 @@ class SphereScene:
 
     def profile(self, nsteps):
         """Per-stage GPU time in ms summed over nsteps eager steps:
-        sort, ranges, permute, contact (HIP events on the context stream)."""
+        sort, ranges, permute, contact (HIP events on the context stream). With
+        u64 keys "sort" is the re-sort, positions permuted in the same pass,
+        unless the full radix sort ran (then "permute" is the permutation)."""
         out = np.zeros(4, np.float32)
         L.call("sd_ctx_profile", self.h, int(nsteps), L.ptr(out), 4)
         return {"sort": float(out[0]), "ranges": float(out[1]), "permute": float(out[2]),
@@ -51,6 +53,13 @@ class SphereScene:
         L.call("sd_ctx_download_spheres", self.h, *[L.ptr(out[k]) for k in
                                                     ("x", "y", "z", "vx", "vy", "vz", "wx", "wy", "wz")])
         return out
+
+    def cell_changes(self):
+        """(spheres that changed cell in the last step, whether the next step
+        uses the full radix sort)."""
+        m, full = C.c_int(), C.c_int()
+        L.call("sd_ctx_cell_changes", self.h, C.byref(m), C.byref(full))
+        return m.value, bool(full.value)
 
     def contact_count(self):
         c = C.c_longlong()

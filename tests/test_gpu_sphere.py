@@ -412,4 +412,8 @@ def test_step_many_cell_changes(oracle, n, speed):
     sc.step(12)
     compare_scene(sc, st)
     assert sc.contact_count() == int(st.hcount.sum())
+    changes, full_next = sc.cell_changes()
+    say("cell changes in the last step", changes, "full sort next:", full_next)
+    if speed > 100.0:  # the path this case is meant to exercise was taken
+        assert full_next == (n > 1024 * 10) and changes > 100
     sc.close()
