@@ -188,7 +188,7 @@ sd_status sd_ctx_step(sd_ctx* ctx, int nsteps);
  * [3] fused contact + integrate + next keys. With u64 keys, [0] is the re-sort
  * from the previous order (positions permuted in the same pass) when at most
  * 1024 spheres changed cell in the previous step, else the radix sort (all
- * passes), after which [2] permutes the positions. */
+ * passes), and the positions are then permuted within [1]; [2] is empty. */
 sd_status sd_ctx_profile(sd_ctx* ctx, int nsteps, float* stage_ms, int nstages);
 /* Spheres that changed cell in the last step (u64 keys; 0 with u32 keys):
  * *full_next = 1 if the next step uses the full radix sort (more than 1024

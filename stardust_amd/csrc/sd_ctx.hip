@@ -205,24 +205,26 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
         uint64_t* K = c->k64[c->cur_keys];
         uint64_t* Ka = c->k64[1 - c->cur_keys];
         const uint64_t* unsorted = K;
+        // changed-cell counters alternate with the history buffers: this step
+        // reads [ho] (written by the previous step) and appends to [hn]
+        int* mc_read = c->mcount + ho;
+        int* mc_write = c->mcount + hn;
         st = radix_sort_keys_u64_gated(&K, &Ka, n, 32, 32 + c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s, c->items,
-                                       c->mcount, RESORT_CAP);
+                                       mc_read, RESORT_CAP);
         if (st != SD_OK) return st;
-        st = launch_resort_u64(c->mcount, c->mnew, c->mold, c->msn, c->mso, unsorted, n, c->ranges, to_dev(c->p.grid),
+        st = launch_resort_u64(mc_read, c->mnew, c->mold, c->msn, c->mso, unsorted, n, c->ranges, to_dev(c->p.grid),
                                c->batch ? c->scenes_d : nullptr, c->scene_of, c->pos[so], c->posB, K, s);
         if (st != SD_OK) return st;
         MARK(1);
-        st = launch_cell_ranges_u64(K, n, c->n_cells, c->ranges, s);
+        // ranges; the permutation when the full sort ran; mc_write = 0
+        st = launch_step_ranges_u64(K, n, c->n_cells, c->ranges, mc_read, RESORT_CAP, c->pos[so], c->posB, mc_write,
+                                    s);
         if (st != SD_OK) return st;
         MARK(2);
-        st = launch_permute_pos(nullptr, K, n, c->pos[so], c->posB, s, c->mcount, RESORT_CAP);
-        if (st != SD_OK) return st;
         MARK(3);
-        st = launch_fill_u32((uint32_t*)c->mcount, 0u, 1, s);
-        if (st != SD_OK) return st;
         a.old_slot_u64 = K;
         a.keys_o64 = Ka;
-        a.mcount = c->mcount;
+        a.mcount = mc_write;
         a.mnew = c->mnew;
         a.mold = c->mold;
         a.mcap = RESORT_CAP;
@@ -353,7 +355,9 @@ sd_status check_spheres(const char* who, int n, const float* x, const float* y, 
 
 // The next step's keys do not come from a step of this order (upload,
 // restore): the next sort is the full one.
-sd_status force_full_sort(sd_ctx* c) { return launch_fill_u32((uint32_t*)c->mcount, 0x7fffffffu, 1, c->stream); }
+sd_status force_full_sort(sd_ctx* c) {
+    return launch_fill_u32((uint32_t*)(c->mcount + c->cur_hist), 0x7fffffffu, 1, c->stream);
+}
 
 // Allocates the device state for n spheres and uploads it (slot = particle id);
 // keys are computed by the caller.
@@ -392,7 +396,7 @@ sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const f
     ALLOC(c->hist[0], (size_t)maxc * n);
     ALLOC(c->hist[1], (size_t)maxc * n);
     ALLOC(c->overflow, 1);
-    ALLOC(c->mcount, 1);
+    ALLOC(c->mcount, 2);
     ALLOC(c->mnew, RESORT_CAP);
     ALLOC(c->mold, RESORT_CAP);
     ALLOC(c->msn, RESORT_CAP);
@@ -422,13 +426,12 @@ sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const f
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->hcount[1], 0u, n, c->stream);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)(c->posB + n), 0u, 4 * (size_t)CONTACT_POS_PAD, c->stream);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->overflow, 0u, 1, c->stream);
-    if (st == SD_OK) st = force_full_sort(c);
     if (st != SD_OK) return st;
     c->cur_keys = 0;
     c->cur_hist = 0;
     c->cur_state = 0;
     c->steps_done = 0;
-    return SD_OK;
+    return force_full_sort(c);  // (reads cur_hist)
 }
 
 }  // namespace
@@ -808,8 +811,8 @@ extern "C" sd_status sd_ctx_cell_changes(sd_ctx* c, int* count, int* full_next) 
     *count = 0;
     *full_next = c->p.key64 ? 0 : 1;
     if (!c->p.key64) return SD_OK;
-    int m = 0;
-    SD_HIP(hipMemcpyAsync(&m, c->mcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int m = 0;  // written by the last step, read by the next one
+    SD_HIP(hipMemcpyAsync(&m, c->mcount + c->cur_hist, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SD_HIP(hipStreamSynchronize(c->stream));
     *full_next = m > RESORT_CAP ? 1 : 0;
     *count = m == 0x7fffffff ? 0 : m;

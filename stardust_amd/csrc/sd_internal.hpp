@@ -58,6 +58,10 @@ sd_status launch_hash_keys_batch(const float4* pos4, int n, const SceneDev* scen
                                  uint64_t* keys, hipStream_t s);
 sd_status launch_cell_ranges_u32(const uint32_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
 sd_status launch_cell_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
+// Cell ranges of the step's sorted u64 keys; gathers pos into sorted order
+// (pos_o) while *gate > gate_cap; sets *next_count = 0.
+sd_status launch_step_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, const int* gate,
+                                 int gate_cap, const float4* pos, float4* pos_o, int* next_count, hipStream_t s);
 // perm is either u32 values (perm32) or the low halves of u64 keys (perm64).
 sd_status launch_permute(const uint32_t* perm32, const uint64_t* perm64, int n, const float4* pos, const float4* vel,
                          const float4* omg, const int* ids, float4* pos_o, float4* vel_o, float4* omg_o, int* ids_o,
