@@ -115,6 +115,10 @@ template <int MODE, bool INDIRECT>
 __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     constexpr bool BATCH = MODE == CONTACT_FUSED_BATCH;
     constexpr bool PADDED = MODE != CONTACT_FORCES;  // fused modes read a context's padded buffer
+    if (a.step_ctl) {
+        if (a.step_ctl[0]) return;  // the step was given up
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.step_ctl[1] += 1;
+    }
     const int lb = xcd_logical_block((a.n + (int)blockDim.x - 1) / (int)blockDim.x);
     if (lb < 0) return;
     const int s = lb * (int)blockDim.x + (int)threadIdx.x;

@@ -109,6 +109,9 @@ struct sd_ctx {
     uint64_t* mold = nullptr;
     uint64_t* msn = nullptr;
     uint64_t* mso = nullptr;
+    // [0] != 0: a re-sort step was given up; [1] steps run since the last reset
+    int* step_ctl = nullptr;
+    bool need_full = true;  // the next step's keys are not from a step of this order
     double* diag = nullptr;
     int diag_blocks = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -137,7 +140,7 @@ void release(sd_ctx* c) {
     void* bufs[] = {c->pos[0], c->pos[1], c->vel[0], c->vel[1], c->omg[0], c->omg[1], c->ids[0], c->ids[1],
                     c->posB, c->k32[0], c->k32[1], c->v32[0], c->v32[1], c->k64[0], c->k64[1], c->sort_tmp,
                     c->ranges, c->hcount[0], c->hcount[1], c->hist[0], c->hist[1], c->overflow, c->diag,
-                    c->scenes_d, c->scene_of, c->mcount, c->mnew, c->mold, c->msn, c->mso};
+                    c->scenes_d, c->scene_of, c->mcount, c->mnew, c->mold, c->msn, c->mso, c->step_ctl};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < 2; ++k) {
@@ -154,6 +157,7 @@ void release(sd_ctx* c) {
     c->overflow = nullptr;
     c->mcount = nullptr;
     c->mnew = c->mold = c->msn = c->mso = nullptr;
+    c->step_ctl = nullptr;
     c->diag = nullptr;
     c->scenes_d = nullptr;
     c->scene_of = nullptr;
@@ -170,7 +174,11 @@ sd_status dalloc(T** p, size_t count) {
 
 // Enqueues one step on the context stream. If ev is given, ev[0..4] are
 // recorded before the sort and after the sort, ranges, permute and contact stages.
-sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
+// full (u64 keys): the radix sort instead of the re-sort from the previous
+// order. A re-sort step that finds more than RESORT_CAP changed cells is given
+// up on the device (step_ctl[0]), together with every later step of the same
+// launch sequence; the host then redoes it as a full step (run_steps).
+sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
     hipStream_t s = c->stream;
 #define MARK(i) \
     if (ev) SD_HIP(hipEventRecord(ev[i], s))
@@ -198,26 +206,30 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
     sd_status st;
     MARK(0);
     if (c->p.key64) {
-        // sorted keys from the previous order and the spheres that changed cell
-        // (positions permuted in the same pass), or, when more than RESORT_CAP
-        // did (and after upload), the full radix sort and permutation; the
-        // count decides on the device, so both are in every captured step
+        // sorted keys: the radix sort (full), or the re-sort from the previous
+        // order and the spheres that changed cell, with the positions permuted
+        // in the same pass. Either leaves them in the other key buffer.
         uint64_t* K = c->k64[c->cur_keys];
         uint64_t* Ka = c->k64[1 - c->cur_keys];
-        const uint64_t* unsorted = K;
         // changed-cell counters alternate with the history buffers: this step
         // reads [ho] (written by the previous step) and appends to [hn]
         int* mc_read = c->mcount + ho;
         int* mc_write = c->mcount + hn;
-        st = radix_sort_keys_u64_gated(&K, &Ka, n, 32, 32 + c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s, c->items,
-                                       mc_read, RESORT_CAP);
-        if (st != SD_OK) return st;
-        st = launch_resort_u64(mc_read, c->mnew, c->mold, c->msn, c->mso, unsorted, n, c->ranges, to_dev(c->p.grid),
-                               c->batch ? c->scenes_d : nullptr, c->scene_of, c->pos[so], c->posB, K, s);
+        if (full) {
+            st = radix_sort_keys_u64_gated(&K, &Ka, n, 32, 32 + c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s,
+                                           c->items, nullptr, 0);
+        } else {
+            st = launch_resort_u64(mc_read, c->mnew, c->mold, c->msn, c->mso, K, n, c->ranges, to_dev(c->p.grid),
+                                   c->batch ? c->scenes_d : nullptr, c->scene_of, c->pos[so], c->posB, Ka,
+                                   c->step_ctl, s);
+            uint64_t* t = K;
+            K = Ka;
+            Ka = t;
+        }
         if (st != SD_OK) return st;
         MARK(1);
-        // ranges; the permutation when the full sort ran; mc_write = 0
-        st = launch_step_ranges_u64(K, n, c->n_cells, c->ranges, mc_read, RESORT_CAP, c->pos[so], c->posB, mc_write,
+        // ranges; after the full sort, the permutation; mc_write = 0
+        st = launch_step_ranges_u64(K, n, c->n_cells, c->ranges, c->step_ctl, full, c->pos[so], c->posB, mc_write,
                                     s);
         if (st != SD_OK) return st;
         MARK(2);
@@ -228,6 +240,7 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr) {
         a.mnew = c->mnew;
         a.mold = c->mold;
         a.mcap = RESORT_CAP;
+        a.step_ctl = c->step_ctl;
         a.scenes = c->scenes_d;
         a.scene_of = c->scene_of;
         st = launch_contacts(c->batch ? CONTACT_FUSED_BATCH : CONTACT_FUSED_U64, a, s, c->contact_block);
@@ -296,6 +309,64 @@ sd_status run_graph_pair(sd_ctx* c) {
     return SD_OK;
 }
 
+// Runs nsteps steps. u64 keys: re-sort steps (graph-replayed in pairs when
+// use_graph), and a full step first whenever the keys are not from a step of
+// this order, or a re-sort step was given up. The device counts the steps that
+// ran, which sets the buffer roles; one synchronisation per launch sequence.
+sd_status run_steps(sd_ctx* c, int nsteps) {
+    sd_status st;
+    if (!c->p.key64) {
+        int left = nsteps;
+        if (c->p.use_graph) {
+            while (left >= 2) {
+                st = run_graph_pair(c);
+                if (st != SD_OK) return st;
+                left -= 2;
+            }
+        }
+        for (; left > 0; --left) {
+            st = enqueue_step(c);
+            if (st != SD_OK) return st;
+        }
+        return SD_OK;
+    }
+    int left = nsteps;
+    while (left > 0) {
+        st = launch_fill_u32((uint32_t*)c->step_ctl, 0u, 2, c->stream);
+        if (st != SD_OK) return st;
+        const int h0 = c->cur_hist, s0 = c->cur_state;
+        if (c->need_full) {
+            st = enqueue_step(c, nullptr, true);
+            if (st != SD_OK) return st;
+            c->need_full = false;
+        } else {
+            int l = left;
+            if (c->p.use_graph) {
+                while (l >= 2) {
+                    st = run_graph_pair(c);
+                    if (st != SD_OK) return st;
+                    l -= 2;
+                }
+            }
+            for (; l > 0; --l) {
+                st = enqueue_step(c);
+                if (st != SD_OK) return st;
+            }
+        }
+        int ctl[2] = {0, 0};
+        SD_HIP(hipMemcpyAsync(ctl, c->step_ctl, sizeof ctl, hipMemcpyDeviceToHost, c->stream));
+        SD_HIP(hipStreamSynchronize(c->stream));
+        const int ran = ctl[1];
+        if (ran < 0 || ran > left || (ran == 0 && ctl[0] == 0))
+            return set_error(SD_ERR_STATE, "step: %d of %d steps ran", ran, left);
+        c->cur_hist = h0 ^ (ran & 1);  // the roles of the steps that ran
+        c->cur_state = s0 ^ (ran & 1);
+        left -= ran;
+        if (ctl[0]) c->need_full = true;  // redo the given-up step with the full sort
+    }
+    return SD_OK;
+}
+
 }  // namespace
 
 extern "C" sd_status sd_ctx_create(const sd_sphere_params* p, sd_ctx** out) {
@@ -356,6 +427,9 @@ sd_status check_spheres(const char* who, int n, const float* x, const float* y, 
 // The next step's keys do not come from a step of this order (upload,
 // restore): the next sort is the full one.
 sd_status force_full_sort(sd_ctx* c) {
+    c->need_full = true;
+    sd_status st = launch_fill_u32((uint32_t*)c->step_ctl, 0u, 2, c->stream);
+    if (st != SD_OK) return st;
     return launch_fill_u32((uint32_t*)(c->mcount + c->cur_hist), 0x7fffffffu, 1, c->stream);
 }
 
@@ -401,6 +475,7 @@ sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const f
     ALLOC(c->mold, RESORT_CAP);
     ALLOC(c->msn, RESORT_CAP);
     ALLOC(c->mso, RESORT_CAP);
+    ALLOC(c->step_ctl, 2);
     c->diag_blocks = div_up(n, DIAG_BLOCK) < 1024 ? div_up(n, DIAG_BLOCK) : 1024;
     ALLOC(c->diag, (size_t)c->diag_blocks * DIAG_VALUES);
 #undef ALLOC
@@ -553,19 +628,8 @@ extern "C" sd_status sd_ctx_step(sd_ctx* c, int nsteps) {
     if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_step: no spheres uploaded");
     sd_status st = launch_fill_u32((uint32_t*)c->overflow, 0u, 1, c->stream);
     if (st != SD_OK) return st;
-    int left = nsteps;
-    if (c->p.use_graph) {
-        while (left >= 2) {
-            st = run_graph_pair(c);
-            if (st != SD_OK) return st;
-            left -= 2;
-        }
-    }
-    while (left > 0) {
-        st = enqueue_step(c);
-        if (st != SD_OK) return st;
-        --left;
-    }
+    st = run_steps(c, nsteps);
+    if (st != SD_OK) return st;
     int ovf = 0;
     SD_HIP(hipMemcpyAsync(&ovf, c->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SD_HIP(hipStreamSynchronize(c->stream));
@@ -585,7 +649,19 @@ extern "C" sd_status sd_ctx_profile(sd_ctx* c, int nsteps, float* stage_ms, int 
     for (int k = 0; k < 5; ++k) SD_HIP(hipEventCreate(&ev[k]));
     sd_status st = SD_OK;
     for (int i = 0; i < nsteps && st == SD_OK; ++i) {
-        st = enqueue_step(c, ev);
+        bool full = c->need_full;
+        if (c->p.key64 && !full) {  // full sort when the re-sort would give the step up
+            int m = 0;
+            if (hipMemcpyAsync(&m, c->mcount + c->cur_hist, sizeof(int), hipMemcpyDeviceToHost, c->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                st = set_error(SD_ERR_HIP, "sd_ctx_profile: counter read failed");
+                break;
+            }
+            full = m > RESORT_CAP;
+        }
+        c->need_full = false;
+        st = enqueue_step(c, ev, full);
         if (st != SD_OK) break;
         hipError_t e = hipEventSynchronize(ev[4]);
         if (e != hipSuccess) {
@@ -702,7 +778,8 @@ extern "C" sd_status sd_ctx_autotune(sd_ctx* c, int trial_steps, int* tile_items
             c->items = items_c[i];
             c->contact_block = block_c[j];
             st = snapshot_restore(c, sn);
-            if (st == SD_OK) st = enqueue_step(c);  // warm-up
+            if (st == SD_OK) st = enqueue_step(c, nullptr, c->need_full);  // warm-up (a full sort after the restore)
+            c->need_full = false;
             if (st != SD_OK) break;
             if (hipEventRecord(e0, c->stream) != hipSuccess) {
                 st = set_error(SD_ERR_HIP, "sd_ctx_autotune: event record failed");
@@ -715,6 +792,16 @@ extern "C" sd_status sd_ctx_autotune(sd_ctx* c, int trial_steps, int* tile_items
                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
                 st = set_error(SD_ERR_HIP, "sd_ctx_autotune: event timing failed");
                 break;
+            }
+            if (c->p.key64) {  // a given-up step makes the trial's time meaningless: keep the current shapes
+                int gave_up = 0;
+                if (hipMemcpyAsync(&gave_up, c->step_ctl, sizeof(int), hipMemcpyDeviceToHost, c->stream) !=
+                        hipSuccess ||
+                    hipStreamSynchronize(c->stream) != hipSuccess) {
+                    st = set_error(SD_ERR_HIP, "sd_ctx_autotune: read failed");
+                    break;
+                }
+                if (gave_up) ms = 3.4e38f;
             }
             if (ms < best) {
                 best = ms;

@@ -152,16 +152,18 @@ sd_status launch_cell_ranges_u64(const uint64_t* keys, int n, int n_cells, int* 
 }
 
 // The step's ranges with u64 keys, fused with what else reads the sorted keys:
-// positions gathered into sorted order while *gate > gate_cap (the re-sort did
-// it otherwise), and the next step's changed-cell counter cleared.
+// the positions gathered into sorted order (permute; the re-sort does it
+// otherwise) and the next step's changed-cell counter cleared. Returns at once
+// while *skip (the step was given up).
 __global__ void k_step_ranges_u64(const uint64_t* __restrict__ keys, int n, int n_cells, int* __restrict__ ranges,
-                                  const int* __restrict__ gate, int gate_cap, const float4* __restrict__ pos,
+                                  const int* __restrict__ skip, int permute, const float4* __restrict__ pos,
                                   float4* __restrict__ pos_o, int* __restrict__ next_count) {
+    if (skip && *skip) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *next_count = 0;
     if (i >= n) return;
     const uint64_t k = keys[i];
-    if (*gate > gate_cap) {
+    if (permute) {
         const uint32_t src = (uint32_t)k;
         if (src < (uint32_t)n) pos_o[i] = pos[src];
     }
@@ -171,12 +173,12 @@ __global__ void k_step_ranges_u64(const uint64_t* __restrict__ keys, int n, int 
     if (i == n - 1 || (uint32_t)(keys[i + 1] >> 32) != c) ranges[2 * (size_t)c + 1] = i + 1;
 }
 
-sd_status launch_step_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, const int* gate,
-                                 int gate_cap, const float4* pos, float4* pos_o, int* next_count, hipStream_t s) {
+sd_status launch_step_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, const int* skip, bool permute,
+                                 const float4* pos, float4* pos_o, int* next_count, hipStream_t s) {
     sd_status st = launch_fill_u32((uint32_t*)ranges, 0u, 2 * (size_t)n_cells, s);
     if (st != SD_OK || n <= 0) return st;
-    hipLaunchKernelGGL(k_step_ranges_u64, dim3(div_up(n, 256)), dim3(256), 0, s, keys, n, n_cells, ranges, gate,
-                       gate_cap, pos, pos_o, next_count);
+    hipLaunchKernelGGL(k_step_ranges_u64, dim3(div_up(n, 256)), dim3(256), 0, s, keys, n, n_cells, ranges, skip,
+                       permute ? 1 : 0, pos, pos_o, next_count);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

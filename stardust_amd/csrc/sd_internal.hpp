@@ -58,10 +58,10 @@ sd_status launch_hash_keys_batch(const float4* pos4, int n, const SceneDev* scen
                                  uint64_t* keys, hipStream_t s);
 sd_status launch_cell_ranges_u32(const uint32_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
 sd_status launch_cell_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
-// Cell ranges of the step's sorted u64 keys; gathers pos into sorted order
-// (pos_o) while *gate > gate_cap; sets *next_count = 0.
-sd_status launch_step_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, const int* gate,
-                                 int gate_cap, const float4* pos, float4* pos_o, int* next_count, hipStream_t s);
+// Cell ranges of the step's sorted u64 keys; with permute, gathers pos into
+// sorted order (pos_o); sets *next_count = 0. Does nothing while *skip.
+sd_status launch_step_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, const int* skip, bool permute,
+                                 const float4* pos, float4* pos_o, int* next_count, hipStream_t s);
 // perm is either u32 values (perm32) or the low halves of u64 keys (perm64).
 sd_status launch_permute(const uint32_t* perm32, const uint64_t* perm64, int n, const float4* pos, const float4* vel,
                          const float4* omg, const int* ids, float4* pos_o, float4* vel_o, float4* omg_o, int* ids_o,
@@ -137,6 +137,9 @@ struct ContactArgs {
     uint64_t* mnew;
     uint64_t* mold;
     int mcap;
+    // nullable: [0] != 0 -> the step was given up, return at once; [1] counts
+    // the steps that ran
+    int* step_ctl;
     // CONTACT_FUSED_BATCH: scene table and scene of every slot
     const SceneDev* scenes;
     const int* scene_of;
@@ -144,14 +147,14 @@ struct ContactArgs {
 sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int block);
 
 // Sorted order of the next step's u64 keys from the previous sorted order and
-// the spheres that changed cell (see sd_resort.hip). Runs only while
-// *mcount <= RESORT_CAP (the full sort covers the other case): writes the
-// sorted keys to keys_out and the positions in that order to posB.
+// the spheres that changed cell (see sd_resort.hip): writes the sorted keys to
+// keys_out and the positions in that order to posB. With *mcount > RESORT_CAP
+// it gives the step up instead (step_ctl[0] = 1).
 constexpr int RESORT_CAP = 1024;
 sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint64_t* mold, uint64_t* msorted_new,
                             uint64_t* msorted_old, const uint64_t* keys, int n, const int* ranges_prev,
                             const GridDev& g, const SceneDev* scenes, const int* scene_of, const float4* pos_state,
-                            float4* posB, uint64_t* keys_out, hipStream_t s);
+                            float4* posB, uint64_t* keys_out, int* step_ctl, hipStream_t s);
 sd_status launch_integrate(float4* pos, float4* vel, float4* omg, const float4* f, const float4* t, int n,
                            const ContactDev& c, hipStream_t s);
 

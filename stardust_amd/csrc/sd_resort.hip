@@ -15,12 +15,15 @@
 //              the first read off the previous step's cell ranges (a binary
 //              search over the previous sorted positions when x's cell was empty).
 // The result is the same array a stable radix sort of the keys produces, and
-// the positions are permuted into that order in the same pass.
+// the positions are permuted into that order in the same pass. With more than
+// RESORT_CAP changes the step is given up: step_ctl[0] is set, every later
+// kernel of the launch sequence returns at once (the state is untouched), and
+// the host redoes that step with the full radix sort.
 //
 // Replaces nothing in the reference (which re-sorts from scratch every step,
 // src/engine/cuda/sh_collision_detection.cu:310-360); the full radix sort runs
 // instead whenever more than RESORT_CAP spheres changed cell, and after every
-// upload or restore.
+// upload, load or restore.
 #include "sd_grid_dev.hpp"
 #include "sd_internal.hpp"
 
@@ -49,9 +52,13 @@ __global__ __launch_bounds__(RESORT_CAP) void k_resort_movers(
     const int* __restrict__ mcount, const uint64_t* __restrict__ mnew, const uint64_t* __restrict__ mold,
     uint64_t* __restrict__ msn, uint64_t* __restrict__ mso, int n, const int* __restrict__ ranges_prev, GridDev g,
     const SceneDev* __restrict__ scenes, const int* __restrict__ scene_of, const float4* __restrict__ pos_state,
-    float4* __restrict__ posB, uint64_t* __restrict__ keys_out) {
+    float4* __restrict__ posB, uint64_t* __restrict__ keys_out, int* __restrict__ step_ctl) {
+    if (step_ctl[0]) return;  // an earlier step of this launch sequence was given up
     const int M = *mcount;
-    if (M > RESORT_CAP) return;
+    if (M > RESORT_CAP) {  // too many changes: this step is given up (the host redoes it with the full sort)
+        if (threadIdx.x == 0) step_ctl[0] = 1;
+        return;
+    }
     __shared__ uint64_t sn[RESORT_CAP], so[RESORT_CAP];
     const int i = threadIdx.x;
     if (i < M) {
@@ -101,7 +108,8 @@ __global__ __launch_bounds__(RESORT_CAP) void k_resort_movers(
 __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
     const int* __restrict__ mcount, const uint64_t* __restrict__ msn, const uint64_t* __restrict__ mso,
     const uint64_t* __restrict__ keys, int n, const float4* __restrict__ pos_state, float4* __restrict__ posB,
-    uint64_t* __restrict__ keys_out) {
+    uint64_t* __restrict__ keys_out, const int* __restrict__ step_ctl) {
+    if (step_ctl[0]) return;
     const int M = *mcount;
     if (M > RESORT_CAP) return;
     __shared__ uint64_t sn[RESORT_CAP], so[RESORT_CAP];
@@ -136,13 +144,14 @@ __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
 sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint64_t* mold, uint64_t* msorted_new,
                             uint64_t* msorted_old, const uint64_t* keys, int n, const int* ranges_prev,
                             const GridDev& g, const SceneDev* scenes, const int* scene_of, const float4* pos_state,
-                            float4* posB, uint64_t* keys_out, hipStream_t s) {
+                            float4* posB, uint64_t* keys_out, int* step_ctl, hipStream_t s) {
     if (n <= 0) return SD_OK;
     hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(RESORT_CAP), 0, s, mcount, mnew, mold, msorted_new,
-                       msorted_old, n, ranges_prev, g, scenes, scene_of, pos_state, posB, keys_out);
+                       msorted_old, n, ranges_prev, g, scenes, scene_of, pos_state, posB, keys_out, step_ctl);
     SD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_resort_merge, dim3(div_up(n, RESORT_MERGE_BLOCK)), dim3(RESORT_MERGE_BLOCK), 0, s, mcount,
-                       (const uint64_t*)msorted_new, (const uint64_t*)msorted_old, keys, n, pos_state, posB, keys_out);
+                       (const uint64_t*)msorted_new, (const uint64_t*)msorted_old, keys, n, pos_state, posB, keys_out,
+                       (const int*)step_ctl);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
