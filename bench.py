@@ -347,7 +347,8 @@ def clump_section(args):
     return {"workload": "clumps_c3", "bodies": sc.nb, "spheres": sc.ns, "steps": args.clump_steps,
             "ms_per_step": round(ms, 4), "sphere_steps_per_s": sc.ns / (ms / 1000.0),
             "body_steps_per_s": sc.nb / (ms / 1000.0), "stage_ms": {k: round(v, 4) for k, v in per.items()},
-            "broadphase_pairs": npairs, "form": "eager step (pair count read back each step)"}
+            "broadphase_pairs": npairs,
+            "form": "graph-replayed step pairs; partners found in the LBVH inside the contact kernel"}
 
 
 def batch_section(args):

@@ -362,10 +362,13 @@ sd_status sd_clump_upload(sd_clump_ctx* ctx, int nb, const float* pos4, const fl
                           const float* local4);
 sd_status sd_clump_step(sd_clump_ctx* ctx, int nsteps);
 /* Stage times (ms, summed over nsteps eager steps): [0] prep + pose + body AABBs,
- * [1] domain + Morton + sort, [2] LBVH build, [3] query, [4] adjacency,
- * [5] contacts + integration. */
+ * [1] domain + Morton + sort, [2] LBVH build, [3] and [4] empty (kept for the
+ * layout), [5] partner search in the tree + contacts + integration. */
 sd_status sd_clump_profile(sd_clump_ctx* ctx, int nsteps, float* stage_ms, int nstages);
 sd_status sd_clump_download(sd_clump_ctx* ctx, float* pos4, float* vel4, float* quat4, float* angmom4);
+/* Unique candidate body pairs (a < b, AABBs overlap) of the last step's tree,
+ * built on this call (the step itself finds partners inside its contact
+ * kernel); SD_ERR_CAPACITY with *n_pairs set when capacity is too small. */
 sd_status sd_clump_pairs(sd_clump_ctx* ctx, int* pairs, int capacity, int* n_pairs);
 sd_status sd_clump_spheres(sd_clump_ctx* ctx, float* pos4);
 /* Clumps of count U{c_min..c_max} spheres with centres N(0, sigma) about the

@@ -46,7 +46,9 @@ def contact_c3():
 
 
 class ClumpScene:
-    STAGES = ("pose", "morton_sort", "lbvh_build", "lbvh_query", "adjacency", "contacts")
+    # stage times of sd_clump_profile; [3] and [4] are empty (the partner search
+    # runs inside the contact kernel)
+    STAGES = ("pose", "morton_sort", "lbvh_build", None, None, "partners_contacts")
 
     def __init__(self, contact):
         L.require_gpu()
@@ -69,7 +71,7 @@ class ClumpScene:
     def profile(self, n):
         out = np.zeros(6, np.float32)
         L.call("sd_clump_profile", self.h, int(n), L.ptr(out), 6)
-        return {k: float(v) for k, v in zip(self.STAGES, out)}
+        return {k: float(v) for k, v in zip(self.STAGES, out) if k}
 
     def download(self):
         out = {k: np.zeros((self.nb, 4), np.float32) for k in ("pos4", "vel4", "quat4", "angm4")}

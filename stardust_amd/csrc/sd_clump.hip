@@ -11,10 +11,12 @@
 // of its contacts instead of racing scatters (Q7).
 //
 // Per step: body prep (world angular velocity), sphere pose, body AABBs, global
-// domain, Morton keys, radix sort, LBVH build + query (unique body pairs),
-// canonical both-way adjacency (sorted by (body, partner)), then one kernel per
-// body sums its sphere-sphere contacts (partners ascending, own spheres
-// ascending, partner spheres ascending) and integrates it.
+// domain, Morton keys, radix sort, LBVH build, then one kernel per body finds
+// its partners in the tree and sums its sphere-sphere contacts (partners
+// ascending, own spheres ascending, partner spheres ascending) and integrates
+// it. Every launch has a fixed size, so steps are captured into a hipGraph per
+// buffer parity and replayed; the candidate pair list (unique a < b) is built
+// from the last step's tree only when asked for (sd_clump_pairs).
 //
 // Integration of a body with 1/m > 0 (FP32, fixed operation order):
 //   v += (F/m + g) dt;  x += v dt;  L += T dt
@@ -118,48 +120,39 @@ __global__ void k_body_aabb(const float4* __restrict__ sx, const int* __restrict
     hi4[b] = make_float4(hx, hy, hz, 0.0f);
 }
 
-// Both directions of every pair as keys body * nb + partner; padding keys are ~0.
-__global__ void k_adj_keys(const int* __restrict__ pairs, int npairs, int cap2, int nb, uint64_t* __restrict__ keys) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= cap2) return;
-    uint64_t k = ~0ull;
-    if (e < 2 * npairs) {
-        const int p = e >> 1;
-        const int a = pairs[2 * (size_t)p], b = pairs[2 * (size_t)p + 1];
-        k = (e & 1) ? (uint64_t)b * (uint64_t)nb + (uint64_t)a : (uint64_t)a * (uint64_t)nb + (uint64_t)b;
-    }
-    keys[e] = k;
+constexpr int CL_BLOCK = 128;
+constexpr int CL_PCAP = 32;  // partners listed per body and pass
+constexpr int CL_SENTINEL = -1;
+
+__device__ __forceinline__ bool cl_overlap(float4 al, float4 ah, float4 bl, float4 bh) {
+    return al.x <= bh.x && bl.x <= ah.x && al.y <= bh.y && bl.y <= ah.y && al.z <= bh.z && bl.z <= ah.z;
 }
 
-// ranges[2b], ranges[2b+1]: the adjacency entries of body b (empty: 0, 0).
-__global__ void k_adj_ranges(const uint64_t* __restrict__ keys, int n, int nb, int* __restrict__ ranges) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const uint64_t k = keys[e];
-    const int b = (int)(k / (uint64_t)nb);
-    if (e == 0 || (int)(keys[e - 1] / (uint64_t)nb) != b) ranges[2 * (size_t)b] = e;
-    if (e == n - 1 || (int)(keys[e + 1] / (uint64_t)nb) != b) ranges[2 * (size_t)b + 1] = e + 1;
-}
-
+// One body per thread: its partners from the LBVH (every body whose AABB
+// overlaps its own), visited in ascending body id, then integration. A pass
+// walks the tree (stackless, ropes) and keeps the CL_PCAP smallest partner ids
+// above the last pass's largest, sorted in LDS; a body with more partners takes
+// more passes. The walk's bound turns a malformed tree into an error flag.
 template <bool INTEGRATE>
-__global__ __launch_bounds__(128) void k_clump_contacts(ClumpState st, ClumpState out, const float4* __restrict__ omega,
-                                                        const float4* __restrict__ i0, const float4* __restrict__ i1,
-                                                        const float4* __restrict__ sx, const float4* __restrict__ sv,
-                                                        const int* __restrict__ bfirst, const int* __restrict__ bcount,
-                                                        int nb, const uint64_t* __restrict__ adj,
-                                                        const int* __restrict__ ranges, ContactDev c,
-                                                        float4* __restrict__ force, float4* __restrict__ torque) {
+__global__ __launch_bounds__(CL_BLOCK) void k_clump_contacts(
+    ClumpState st, ClumpState out, const float4* __restrict__ omega, const float4* __restrict__ i0,
+    const float4* __restrict__ i1, const float4* __restrict__ sx, const float4* __restrict__ sv,
+    const int* __restrict__ bfirst, const int* __restrict__ bcount, int nb, const float4* __restrict__ blo,
+    const float4* __restrict__ bhi, const uint64_t* __restrict__ sorted_keys, const int* __restrict__ left,
+    const int* __restrict__ rope, const float4* __restrict__ nlo, const float4* __restrict__ nhi, ContactDev c,
+    int* __restrict__ error, float4* __restrict__ force, float4* __restrict__ torque) {
+    __shared__ int plist[CL_BLOCK][CL_PCAP + 1];  // (+1: rows start in different banks)
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
+    int* pl = plist[threadIdx.x];  // this body's partner list
     const float4 X = st.pos[b];
     const float4 wb4 = omega[b];
     const int f0 = bfirst[b], fn = bcount[b];
+    const float4 ql = blo[b], qh = bhi[b];
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
-    const int e0 = ranges[2 * (size_t)b], e1 = ranges[2 * (size_t)b + 1];
-    for (int e = e0; e < e1; ++e) {
-        const int p = (int)(adj[e] % (uint64_t)nb);
+    auto partner = [&](int p) {
         const float inv_mp = st.pos[p].w;
-        if (X.w + inv_mp == 0.0f) continue;  // two static bodies
+        if (X.w + inv_mp == 0.0f) return;  // two static bodies
         const float4 wp4 = omega[p];
         const int g0 = bfirst[p], gn = bcount[p];
         for (int i = f0; i < f0 + fn; ++i) {
@@ -186,6 +179,41 @@ __global__ __launch_bounds__(128) void k_clump_contacts(ClumpState st, ClumpStat
                 Tz = Tz + ((ax * f.fy - ay * f.fx) + f.tz);
             }
         }
+    };
+    for (int lower = -1;;) {
+        int np = 0;
+        bool more = false;
+        int node = nb == 1 ? CL_SENTINEL : 0;
+        for (int guard = 0; node != CL_SENTINEL; ++guard) {
+            if (guard > 2 * nb + 2 || node < 0 || node >= 2 * nb - 1) {
+                atomicOr(error, 4);
+                break;
+            }
+            if (!cl_overlap(ql, qh, nlo[node], nhi[node])) {
+                node = rope[node];
+                continue;
+            }
+            if (node >= nb - 1) {  // a leaf: one body
+                const int p = (int)(uint32_t)sorted_keys[node - (nb - 1)];
+                if (p != b && p > lower && p < nb) {
+                    if (np == CL_PCAP) more = true;  // partners beyond this pass's list
+                    if (np < CL_PCAP || p < pl[CL_PCAP - 1]) {
+                        int k = np < CL_PCAP ? np++ : CL_PCAP - 1;  // the largest drops out of a full list
+                        while (k > 0 && pl[k - 1] > p) {
+                            pl[k] = pl[k - 1];
+                            --k;
+                        }
+                        pl[k] = p;
+                    }
+                }
+                node = rope[node];
+            } else {
+                node = left[node];
+            }
+        }
+        for (int e = 0; e < np; ++e) partner(pl[e]);
+        if (!more) break;
+        lower = pl[CL_PCAP - 1];
     }
     if (!INTEGRATE) {
         force[b] = make_float4(Fx, Fy, Fz, 0.0f);
@@ -256,13 +284,12 @@ struct sd_clump_ctx {
     int* pairs = nullptr;
     int pair_cap = 0;
     int npairs = 0;
-    uint64_t *adj = nullptr, *adj2 = nullptr;
-    void* adj_tmp = nullptr;
-    size_t adj_tmp_bytes = 0;
-    int* ranges = nullptr;
+    const uint64_t* ksorted = nullptr;  // sorted Morton keys of the last step's tree
+    bool have_tree = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps from buffer parity 0 / 1
+    hipGraph_t graph_obj[2] = {nullptr, nullptr};
     float4 *force = nullptr, *torque = nullptr;
     long long steps_done = 0;
-    long long pair_total = 0;
 };
 
 namespace {
@@ -273,13 +300,23 @@ sd_status dalloc(T** p, size_t count) {
     return SD_OK;
 }
 
+void drop_clump_graphs(sd_clump_ctx* c) {
+    for (int k = 0; k < 2; ++k) {
+        if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+        if (c->graph_obj[k]) (void)hipGraphDestroy(c->graph_obj[k]);
+        c->graph[k] = nullptr;
+        c->graph_obj[k] = nullptr;
+    }
+}
+
 void cfree(sd_clump_ctx* c) {
     void* bufs[] = {c->pos[0],  c->pos[1],    c->vel[0],     c->vel[1],   c->quat[0], c->quat[1],  c->angm[0],
                     c->angm[1], c->i0,        c->i1,         c->omega,    c->bfirst,  c->bcount,   c->sbody,
                     c->slocal,  c->sx,        c->sv,         c->blo,      c->bhi,     c->domain,   c->dom_scratch,
                     c->mkeys,   c->mkeys2,    c->sort_tmp,   c->left,     c->right,   c->parent,   c->first,
                     c->last,    c->rope,      c->nlo,        c->nhi,      c->counters, c->error,   c->q_scratch,
-                    c->pairs,   c->adj,       c->adj2,       c->adj_tmp,  c->ranges,  c->force,    c->torque};
+                    c->pairs,   c->force,     c->torque};
+    drop_clump_graphs(c);
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     sd_clump_params keep = c->p;
@@ -293,17 +330,8 @@ sd_status grow_pairs(sd_clump_ctx* c, int need) {
     if (need <= c->pair_cap) return SD_OK;
     int cap = need + need / 2 + 1024;
     if (c->pairs) (void)hipFree(c->pairs);
-    if (c->adj) (void)hipFree(c->adj);
-    if (c->adj2) (void)hipFree(c->adj2);
-    if (c->adj_tmp) (void)hipFree(c->adj_tmp);
     c->pairs = nullptr;
-    c->adj = c->adj2 = nullptr;
-    c->adj_tmp = nullptr;
     sd_status st = dalloc(&c->pairs, 2 * (size_t)cap);
-    if (st == SD_OK) st = dalloc(&c->adj, 2 * (size_t)cap);
-    if (st == SD_OK) st = dalloc(&c->adj2, 2 * (size_t)cap);
-    c->adj_tmp_bytes = radix_temp_bytes(2 * cap);
-    if (st == SD_OK) st = dalloc((char**)&c->adj_tmp, c->adj_tmp_bytes);
     if (st != SD_OK) return st;
     c->pair_cap = cap;
     return SD_OK;
@@ -318,7 +346,7 @@ int bits_for(uint64_t v) {
 ClumpState state_of(sd_clump_ctx* c, int k) { return ClumpState{c->pos[k], c->vel[k], c->quat[k], c->angm[k]}; }
 
 // One step. ev (optional, 6 events): prep+pose+aabb | domain+morton+sort | lbvh build |
-// query | adjacency | contacts+integrate.
+// (empty) | (empty) | partner search + contacts + integrate.
 sd_status clump_step(sd_clump_ctx* c, hipEvent_t* ev) {
     hipStream_t s = c->stream;
     const int nb = c->nb, ns = c->ns;
@@ -344,46 +372,16 @@ sd_status clump_step(sd_clump_ctx* c, hipEvent_t* ev) {
     st = radix_sort_keys_u64(&K, &Ka, nb, 32, 62, c->sort_tmp, c->sort_tmp_bytes, s, 0);
     if (st != SD_OK) return st;
     MARK(2);
-    st = launch_fill_u32((uint32_t*)c->error, 0u, 1, s);
-    if (st != SD_OK) return st;
     st = sd_lbvh_build(K, nb, (const float*)c->blo, (const float*)c->bhi, c->left, c->right, c->parent, c->first,
                        c->last, c->rope, (float*)c->nlo, (float*)c->nhi, c->counters, c->error, s);
     if (st != SD_OK) return st;
+    c->ksorted = K;
     MARK(3);
-    int npairs = 0;
-    st = sd_lbvh_query(K, nb, c->left, c->last, c->rope, (const float*)c->nlo, (const float*)c->nhi, c->pairs,
-                       c->pair_cap, &npairs, c->error, c->q_scratch, c->q_scratch_bytes, s);
-    if (st == SD_ERR_CAPACITY) {
-        st = grow_pairs(c, npairs);
-        if (st != SD_OK) return st;
-        st = sd_lbvh_query(K, nb, c->left, c->last, c->rope, (const float*)c->nlo, (const float*)c->nhi, c->pairs,
-                           c->pair_cap, &npairs, c->error, c->q_scratch, c->q_scratch_bytes, s);
-    }
-    if (st != SD_OK) return st;
-    c->npairs = npairs;
-    c->pair_total += npairs;
-    MARK(4);
-    // canonical adjacency: (body, partner) ascending
-    const int n2 = 2 * npairs;
-    st = launch_fill_u32((uint32_t*)c->ranges, 0u, 2 * (size_t)nb, s);
-    if (st != SD_OK) return st;
-    if (n2 > 0) {
-        hipLaunchKernelGGL(k_adj_keys, dim3(div_up(n2, 256)), dim3(256), 0, s, c->pairs, npairs, n2, nb, c->adj);
-        SD_LAUNCH_CHECK();
-        uint64_t *A = c->adj, *Aa = c->adj2;
-        st = radix_sort_keys_u64(&A, &Aa, n2, 0, bits_for((uint64_t)nb * (uint64_t)nb), c->adj_tmp, c->adj_tmp_bytes,
-                                 s, 0);
-        if (st != SD_OK) return st;
-        if (A != c->adj) {
-            c->adj2 = c->adj;
-            c->adj = A;
-        }
-        hipLaunchKernelGGL(k_adj_ranges, dim3(div_up(n2, 256)), dim3(256), 0, s, c->adj, n2, nb, c->ranges);
-        SD_LAUNCH_CHECK();
-    }
+    MARK(4);  // (partner search is part of the contact kernel)
     MARK(5);
-    hipLaunchKernelGGL(k_clump_contacts<true>, dim3(div_up(nb, 128)), dim3(128), 0, s, cs, nx, c->omega, c->i0,
-                       c->i1, c->sx, c->sv, c->bfirst, c->bcount, nb, c->adj, c->ranges, cd, nullptr, nullptr);
+    hipLaunchKernelGGL(k_clump_contacts<true>, dim3(div_up(nb, CL_BLOCK)), dim3(CL_BLOCK), 0, s, cs, nx, c->omega,
+                       c->i0, c->i1, c->sx, c->sv, c->bfirst, c->bcount, nb, c->blo, c->bhi, (const uint64_t*)K,
+                       c->left, c->rope, c->nlo, c->nhi, cd, c->error, nullptr, nullptr);
     SD_LAUNCH_CHECK();
     MARK(6);
 #undef MARK
@@ -420,6 +418,7 @@ extern "C" sd_status sd_clump_set_params(sd_clump_ctx* c, const sd_clump_params*
     if (st != SD_OK) return st;
     (void)hipStreamSynchronize(c->stream);
     c->p = *p;
+    drop_clump_graphs(c);  // the contact law is a launch argument of the captured steps
     return SD_OK;
 }
 
@@ -496,7 +495,6 @@ extern "C" sd_status sd_clump_upload(sd_clump_ctx* c, int nb, const float* pos4,
     ALLOC(c->error, 1);
     sd_lbvh_query_scratch_bytes(nb, &c->q_scratch_bytes);
     ALLOC(*(char**)&c->q_scratch, c->q_scratch_bytes);
-    ALLOC(c->ranges, 2 * (size_t)nb);
     ALLOC(c->force, nb);
     ALLOC(c->torque, nb);
 #undef ALLOC
@@ -563,19 +561,62 @@ extern "C" sd_status sd_clump_upload(sd_clump_ctx* c, int nb, const float* pos4,
     SD_HIP(hipStreamSynchronize(s));
     c->cur = 0;
     c->steps_done = 0;
-    c->pair_total = 0;
+    c->have_tree = false;
+    c->npairs = -1;
+    drop_clump_graphs(c);
+    return SD_OK;
+}
+
+// Two steps from the current buffer parity, captured once per parity.
+static sd_status clump_graph_pair(sd_clump_ctx* c) {
+    const int k = c->cur;
+    if (!c->graph[k]) {
+        SD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        sd_status st = clump_step(c, nullptr);
+        if (st == SD_OK) st = clump_step(c, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->cur = k;  // capture advanced the parity twice: back where it was
+        if (st != SD_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        SD_HIP(e);
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            SD_HIP(e);
+        }
+        c->graph[k] = ge;
+        c->graph_obj[k] = g;
+    }
+    SD_HIP(hipGraphLaunch(c->graph[k], c->stream));
     return SD_OK;
 }
 
 extern "C" sd_status sd_clump_step(sd_clump_ctx* c, int nsteps) {
     SD_REQUIRE(c && nsteps >= 0, "sd_clump_step: bad argument");
     if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_step: no bodies uploaded");
-    for (int i = 0; i < nsteps; ++i) {
-        sd_status st = clump_step(c, nullptr);
+    if (nsteps == 0) return SD_OK;
+    sd_status st = launch_fill_u32((uint32_t*)c->error, 0u, 1, c->stream);
+    if (st != SD_OK) return st;
+    int left = nsteps;
+    for (; left >= 2; left -= 2) {
+        st = clump_graph_pair(c);
         if (st != SD_OK) return st;
     }
+    if (left) {
+        st = clump_step(c, nullptr);
+        if (st != SD_OK) return st;
+    }
+    int err = 0;
+    SD_HIP(hipMemcpyAsync(&err, c->error, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SD_HIP(hipStreamSynchronize(c->stream));
     c->steps_done += nsteps;
+    c->have_tree = true;
+    c->npairs = -1;  // the pair list of the new last step is built when asked for
+    if (err) return set_error(SD_ERR_STATE, "sd_clump_step: malformed tree (error bits %d)", err);
     return SD_OK;
 }
 
@@ -601,7 +642,11 @@ extern "C" sd_status sd_clump_profile(sd_clump_ctx* c, int nsteps, float* stage_
         }
     }
     for (int k = 0; k < 7; ++k) (void)hipEventDestroy(ev[k]);
-    if (st == SD_OK) c->steps_done += nsteps;
+    if (st == SD_OK) {
+        c->steps_done += nsteps;
+        c->have_tree = true;
+        c->npairs = -1;
+    }
     return st;
 }
 
@@ -624,6 +669,22 @@ extern "C" sd_status sd_clump_download(sd_clump_ctx* c, float* pos4, float* vel4
 // Unique candidate body pairs (a < b) of the last step's broadphase.
 extern "C" sd_status sd_clump_pairs(sd_clump_ctx* c, int* pairs, int capacity, int* n_pairs) {
     SD_REQUIRE(c && n_pairs, "sd_clump_pairs: NULL");
+    if (!c->have_tree) return set_error(SD_ERR_STATE, "sd_clump_pairs: no step has run");
+    if (c->npairs < 0) {  // query the last step's tree
+        int np = 0;
+        sd_status st = sd_lbvh_query(c->ksorted, c->nb, c->left, c->last, c->rope, (const float*)c->nlo,
+                                     (const float*)c->nhi, c->pairs, c->pair_cap, &np, c->error, c->q_scratch,
+                                     c->q_scratch_bytes, c->stream);
+        if (st == SD_ERR_CAPACITY) {
+            st = grow_pairs(c, np);
+            if (st != SD_OK) return st;
+            st = sd_lbvh_query(c->ksorted, c->nb, c->left, c->last, c->rope, (const float*)c->nlo,
+                               (const float*)c->nhi, c->pairs, c->pair_cap, &np, c->error, c->q_scratch,
+                               c->q_scratch_bytes, c->stream);
+        }
+        if (st != SD_OK) return st;
+        c->npairs = np;
+    }
     *n_pairs = c->npairs;
     if (pairs && capacity > 0) {
         const int k = c->npairs < capacity ? c->npairs : capacity;

@@ -280,3 +280,25 @@ def test_clump_upload_rejects_bad_ranges():
         sc.upload(bad)
     sc.upload(s)
     sc.step(2)
+
+
+def test_clump_steps_crowded_bit_exact(oracle):
+    """Bodies 12 mm apart with AABBs about 6 cm wide: each has over a hundred
+    candidate partners, so the step's partner search takes several passes of
+    its 32-entry list; every step equals the oracle bit for bit."""
+    say("clump steps crowded")
+    s = clump_set(729, 0xC7, spacing=0.012, v_sigma=0.05)
+    c = CL.contact_c3()
+    o = oracle.ClumpState(s)
+    sx0, sv0, lo0, hi0 = o.pose()
+    per = np.bincount(oracle.brute_aabb_pairs(lo0, hi0).ravel(), minlength=729)
+    assert per.max() > 3 * 32
+    sc = CL.ClumpScene(c)
+    sc.upload(s)
+    total = 0
+    for k in (1, 2, 3):
+        sc.step(k)
+        o.step(c, k)
+        total += k
+        compare_state(sc.download(), o, "step %d" % total)
+    sc.close()
