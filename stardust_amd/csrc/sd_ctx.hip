@@ -228,9 +228,10 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
         }
         if (st != SD_OK) return st;
         MARK(1);
-        // ranges; after the full sort, the permutation; mc_write = 0
-        st = launch_step_ranges_u64(K, n, c->n_cells, c->ranges, c->step_ctl, full, c->pos[so], c->posB, mc_write,
-                                    s);
+        // ranges (after the re-sort only the cells left by changed spheres
+        // needed clearing); after the full sort, the permutation; mc_write = 0
+        st = launch_step_ranges_u64(K, n, c->n_cells, c->ranges, c->step_ctl, full, full, c->pos[so], c->posB,
+                                    mc_write, s);
         if (st != SD_OK) return st;
         MARK(2);
         MARK(3);

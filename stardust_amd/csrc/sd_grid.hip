@@ -174,9 +174,12 @@ __global__ void k_step_ranges_u64(const uint64_t* __restrict__ keys, int n, int 
 }
 
 sd_status launch_step_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, const int* skip, bool permute,
-                                 const float4* pos, float4* pos_o, int* next_count, hipStream_t s) {
-    sd_status st = launch_fill_u32((uint32_t*)ranges, 0u, 2 * (size_t)n_cells, s);
-    if (st != SD_OK || n <= 0) return st;
+                                 bool clear_all, const float4* pos, float4* pos_o, int* next_count, hipStream_t s) {
+    if (clear_all) {
+        sd_status st = launch_fill_u32((uint32_t*)ranges, 0u, 2 * (size_t)n_cells, s);
+        if (st != SD_OK) return st;
+    }
+    if (n <= 0) return SD_OK;
     hipLaunchKernelGGL(k_step_ranges_u64, dim3(div_up(n, 256)), dim3(256), 0, s, keys, n, n_cells, ranges, skip,
                        permute ? 1 : 0, pos, pos_o, next_count);
     SD_LAUNCH_CHECK();

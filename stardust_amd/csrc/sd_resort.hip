@@ -50,7 +50,7 @@ __device__ __forceinline__ uint64_t old_key_at(const float4* __restrict__ posB, 
 // distinct) and places every mover.
 __global__ __launch_bounds__(RESORT_CAP) void k_resort_movers(
     const int* __restrict__ mcount, const uint64_t* __restrict__ mnew, const uint64_t* __restrict__ mold,
-    uint64_t* __restrict__ msn, uint64_t* __restrict__ mso, int n, const int* __restrict__ ranges_prev, GridDev g,
+    uint64_t* __restrict__ msn, uint64_t* __restrict__ mso, int n, int* ranges, GridDev g,
     const SceneDev* __restrict__ scenes, const int* __restrict__ scene_of, const float4* __restrict__ pos_state,
     float4* __restrict__ posB, uint64_t* __restrict__ keys_out, int* __restrict__ step_ctl) {
     if (step_ctl[0]) return;  // an earlier step of this launch sequence was given up
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(RESORT_CAP) void k_resort_movers(
         mso[ro] = o;
         const uint32_t cell = (uint32_t)(x >> 32);  // < the cell count: written from a clamped cell
         const int sx = (int)(uint32_t)x;
-        const int2 r = *reinterpret_cast<const int2*>(ranges_prev + 2 * (size_t)cell);
+        const int2 r = *reinterpret_cast<const int2*>(ranges + 2 * (size_t)cell);  // the previous step's
         int all_less;
         if (r.y > r.x) {
             all_less = r.x + (sx < r.x ? 0 : (sx > r.y ? r.y - r.x : sx - r.x));
@@ -97,10 +97,15 @@ __global__ __launch_bounds__(RESORT_CAP) void k_resort_movers(
         }
         dest = all_less - old_less + rn;
     }
-    __syncthreads();  // every search is done with posB
+    __syncthreads();  // every search is done with posB and the ranges
     if (i < M && dest >= 0 && dest < n && (uint32_t)x < (uint32_t)n) {  // (always, for consistent keys)
         keys_out[dest] = x;
         posB[dest] = pos_state[(uint32_t)x];
+        // a cell left by a mover may be empty now; the step's range marking
+        // rewrites every cell that is not, so only these need clearing
+        const uint32_t cell_old = (uint32_t)(so[i] >> 32);
+        ranges[2 * (size_t)cell_old] = 0;
+        ranges[2 * (size_t)cell_old + 1] = 0;
     }
 }
 
@@ -142,12 +147,12 @@ __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
 }
 
 sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint64_t* mold, uint64_t* msorted_new,
-                            uint64_t* msorted_old, const uint64_t* keys, int n, const int* ranges_prev,
+                            uint64_t* msorted_old, const uint64_t* keys, int n, int* ranges,
                             const GridDev& g, const SceneDev* scenes, const int* scene_of, const float4* pos_state,
                             float4* posB, uint64_t* keys_out, int* step_ctl, hipStream_t s) {
     if (n <= 0) return SD_OK;
     hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(RESORT_CAP), 0, s, mcount, mnew, mold, msorted_new,
-                       msorted_old, n, ranges_prev, g, scenes, scene_of, pos_state, posB, keys_out, step_ctl);
+                       msorted_old, n, ranges, g, scenes, scene_of, pos_state, posB, keys_out, step_ctl);
     SD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_resort_merge, dim3(div_up(n, RESORT_MERGE_BLOCK)), dim3(RESORT_MERGE_BLOCK), 0, s, mcount,
                        (const uint64_t*)msorted_new, (const uint64_t*)msorted_old, keys, n, pos_state, posB, keys_out,
