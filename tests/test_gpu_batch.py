@@ -1,6 +1,7 @@
 """Batched multi-scene stepping on the GPU: every scene of a batch is
 bit-identical to the same scene stepped alone (sd_ctx, packed u64 keys) and to
 the oracle; the autotuner changes launch shapes but no result."""
+import math
 import sys
 
 import numpy as np
@@ -147,4 +148,58 @@ def test_batch_c5_full_size_spot_checks():
         sc.step(steps)
         assert_same({f: got[f][lo:hi] for f in FIELDS}, sc.download(), "c5 scene %d" % k)
         sc.close()
+    b.close()
+
+
+@pytest.mark.parametrize("scale,speed", [(1, 150.0), (4, 150.0), (1, 2.0)])
+def test_batch_many_cell_changes(oracle, scale, speed):
+    """Fast spheres in scenes of different origins and cell sizes: the re-sort
+    steps place hundreds of changed spheres across scene cell blocks (scale 1),
+    or give up and redo with the full radix sort (scale 4, a few thousand
+    changes a step); every scene equals the oracle bit for bit."""
+    from stardust_amd import _lib as L
+    say("batch many cell changes", scale, speed)
+    rng = np.random.default_rng(43)
+    spec = [(1500, 0.2, 0.004, (0.0, 0.0, 0.0)), (250, 0.1, 0.003, (1.0, -2.0, 0.5)),
+            (2000, 0.25, 0.005, (-3.0, 0.0, 0.0)), (750, 0.15, 0.0035, (0.0, 4.0, -1.0))]
+    scenes, cols, states = [], {k: [] for k in ("x", "y", "z", "r", "m", "vx", "vy", "vz")}, []
+    for n0, box, r, org in spec:
+        n = n0 * scale
+        box = box * scale ** (1.0 / 3.0)
+        lo = np.array(org, np.float32)
+        pos = (lo + rng.uniform(r, box - r, (n, 3))).astype(np.float32)
+        rad = rng.uniform(0.5 * r, r, n).astype(np.float32)
+        m = W.sphere_mass(rad)
+        vel = rng.normal(0, speed, (n, 3)).astype(np.float32)
+        nc = int(math.ceil(box / (2 * r)))
+        g = L.Grid.make(tuple(float(v) for v in lo), 2 * r, (nc, nc, nc))
+        c = L.ContactParams.make(1e5, 0.3, 0.5, 2e-6, gravity=(0.0, 0.0, -9.81),
+                                 walls=(tuple(float(v) for v in lo), tuple(float(v) + box for v in lo)),
+                                 max_contacts=32)
+        scenes.append((n, g, c))
+        for k, a in (("x", pos[:, 0]), ("y", pos[:, 1]), ("z", pos[:, 2]), ("r", rad), ("m", m),
+                     ("vx", vel[:, 0]), ("vy", vel[:, 1]), ("vz", vel[:, 2])):
+            cols[k].append(np.ascontiguousarray(a))
+        st = oracle.SphereState(pos[:, 0], pos[:, 1], pos[:, 2], rad, m, vx=vel[:, 0], vy=vel[:, 1],
+                                vz=vel[:, 2], max_contacts=32)
+        st.step(g, c, nsteps=12, nthreads=8)
+        states.append(st)
+    s = {k: np.concatenate(v) for k, v in cols.items()}
+    b = batch_of(None, scenes, s, [s["vx"], s["vy"], s["vz"]])
+    for k in (5, 4, 3):
+        b.step(k)
+    got = b.download()
+    changes, full_next = b.cell_changes()
+    say("cell changes in the last step", changes, "full sort next:", full_next)
+    off = 0
+    for k, ((n, g, c), st) in enumerate(zip(scenes, states)):
+        sl = slice(off, off + n)
+        e = st.by_id()
+        for f, (arr, col) in {"x": ("pos4", 0), "y": ("pos4", 1), "z": ("pos4", 2), "vx": ("vel4", 0),
+                              "vy": ("vel4", 1), "vz": ("vel4", 2), "wx": ("omg4", 0), "wy": ("omg4", 1),
+                              "wz": ("omg4", 2)}.items():
+            assert np.array_equal(got[f][sl], e[arr][:, col]), ("scene", k, f)
+        off += n
+    if speed > 100.0:
+        assert changes > 100 and full_next == (scale > 1)
     b.close()
