@@ -95,6 +95,14 @@ __device__ __forceinline__ void find_history(const ContactArgs& a, int os, int o
 constexpr int XCDS = 8;       // gfx950: 8 accelerator complex dies, one L2 each
 constexpr int CAND_UNROLL = 4; // candidate positions loaded per batch (loads in flight)
 static_assert(CAND_UNROLL - 1 <= CONTACT_POS_PAD, "sorted positions must be padded for a whole batch");
+constexpr int CONTACT_STAGE = 256;  // staged candidate positions per wave (4 KiB of LDS)
+
+// Orders one wave's LDS writes and reads (the lanes of a wave only).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Workgroup i is dispatched to XCD i % 8. Remap so that every XCD works on a
 // contiguous range of logical blocks: blocks resident on one XCD then touch
@@ -152,7 +160,8 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
     int nc = 0;
 
-    // one sphere-sphere contact with the sphere at sorted position t
+    // one sphere-sphere contact with the sphere at sorted position t; the
+    // sphere itself and coincident spheres (d2 == 0) are not contacts
     auto visit = [&](int t) {
         const float4 q = a.pos[t];
         const float dx = p.x - q.x;
@@ -160,6 +169,7 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
         const float dzz = p.z - q.z;
         const float d2 = dx * dx + dyy * dyy + dzz * dzz;
         const float rs = p.w + q.w;
+        if (!(d2 < rs * rs) || d2 == 0.0f) return;
         const int ot = INDIRECT ? old_slot_of(a, t) : t;
         const float4 vq = a.vel[ot];
         if (v.w + vq.w == 0.0f) return;  // two static spheres
@@ -182,39 +192,96 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
         ++nc;
     };
 
+    // Distance test of the candidates at sorted positions t0 .. t0+3 (q), then
+    // the contacts among them in ascending position order. Only the distance
+    // is tested per candidate; a batch with a possible contact in any lane
+    // (rare, and always the batch holding the sphere itself) then drops
+    // positions past hi, and visit() drops the sphere itself and coincident
+    // spheres (d2 == 0).
+    auto test_batch = [&](const float4 (&q)[CAND_UNROLL], int t0, int hi) {
+        bool h[CAND_UNROLL];
+        uint64_t any = 0u;  // lanes with a possible contact (scalar masks)
+#pragma unroll
+        for (int k = 0; k < CAND_UNROLL; ++k) {
+            const float dx = p.x - q[k].x;
+            const float dyy = p.y - q[k].y;
+            const float dzz = p.z - q[k].z;
+            const float d2 = dx * dx + dyy * dyy + dzz * dzz;
+            const float rs = p.w + q[k].w;
+            h[k] = d2 < rs * rs;
+            any |= __builtin_amdgcn_ballot_w64(h[k]);
+        }
+        if (any) {
+            uint32_t hit = 0u;
+#pragma unroll
+            for (int k = 0; k < CAND_UNROLL; ++k) hit |= h[k] ? (1u << k) : 0u;
+            while (hit) {
+                const int k = __builtin_ctz(hit);
+                hit &= hit - 1u;
+                if (t0 + k < hi) visit(t0 + k);
+            }
+        }
+    };
+
+    // Candidate staging. The lanes of a wave are consecutive sorted spheres, so
+    // for one stencil row offset (dz, dy) their windows ascend with the lane
+    // (rows and cells are in sorted order). The lanes in the first active
+    // lane's grid row (segment A) and the others (segment B, normally the next
+    // row) each have a compact union of windows; both unions are copied into
+    // the wave's LDS region with coalesced loads and the candidates are then
+    // read from LDS. A wave whose unions do not fit (a wave over several
+    // sparse rows) reads its candidates from global memory for that offset.
+    extern __shared__ float4 cstage[];
+    float4* const stg = cstage + (threadIdx.x >> 6) * CONTACT_STAGE;
+    const uint64_t amask = __ballot(1);  // active lanes: a prefix of the wave
+    const int arank = __popcll(amask & lanemask_lt());
+    const int nact = __popcll(amask);
+    const uint32_t rowid = cbase + (uint32_t)((cz * g.ny + cy) * g.nx);
+    const bool segB = rowid != (uint32_t)__builtin_amdgcn_readfirstlane((int)rowid);
+    const uint64_t bmask = __ballot(segB);
+    constexpr int PAD = CAND_UNROLL - 1;  // a batch reads up to 3 positions past its window
+
     for (int dz = -1; dz <= 1; ++dz) {
         const int z = cz + dz;
-        if (z < 0 || z >= g.nz) continue;
         for (int dy = -1; dy <= 1; ++dy) {
             const int y = cy + dy;
-            if (y < 0 || y >= g.ny) continue;
-            int lo, hi;
-            row_range(g, ranges, cx, y, z, lo, hi);
-            for (int t0 = lo; t0 < hi; t0 += CAND_UNROLL) {
-                // issue all loads of the batch first, then test; contacts are
-                // visited in ascending position order. The context's sorted
-                // positions are padded, so a batch reads past hi without
-                // clamping (those lanes are masked out). The sphere itself
-                // has d2 == 0 and is excluded with coincident spheres.
-                float4 q[CAND_UNROLL];
+            int lo = 0, hi = 0;
+            if (z >= 0 && z < g.nz && y >= 0 && y < g.ny) row_range(g, ranges, cx, y, z, lo, hi);
+            const uint64_t ne = __ballot(hi > lo);
+            if (ne == 0u) continue;
+            const uint64_t ma = ne & ~bmask, mb = ne & bmask;
+            int la = 0, na = 0, lb = 0, nb = 0;
+            if (ma) {
+                la = __builtin_amdgcn_readlane(lo, __builtin_ctzll(ma));
+                na = __builtin_amdgcn_readlane(hi, 63 - __builtin_clzll(ma)) - la;
+            }
+            if (mb) {
+                lb = __builtin_amdgcn_readlane(lo, __builtin_ctzll(mb));
+                nb = __builtin_amdgcn_readlane(hi, 63 - __builtin_clzll(mb)) - lb;
+            }
+            const int offb = na + PAD;
+            if (offb + nb + PAD <= CONTACT_STAGE) {
+                for (int i = arank; i < na + PAD; i += nact) stg[i] = a.pos[min(la + i, a.n - 1)];
+                for (int i = arank; i < nb + PAD; i += nact) stg[offb + i] = a.pos[min(lb + i, a.n - 1)];
+                wave_lds_sync();
+                const int off = segB ? offb - lb : -la;  // LDS slot of sorted position t: t + off
+                for (int t0 = lo; t0 < hi; t0 += CAND_UNROLL) {
+                    float4 q[CAND_UNROLL];
 #pragma unroll
-                for (int k = 0; k < CAND_UNROLL; ++k) q[k] = a.pos[PADDED || t0 + k < hi ? t0 + k : t0];
-                uint32_t hit = 0u;
-#pragma unroll
-                for (int k = 0; k < CAND_UNROLL; ++k) {
-                    const float dx = p.x - q[k].x;
-                    const float dyy = p.y - q[k].y;
-                    const float dzz = p.z - q[k].z;
-                    const float d2 = dx * dx + dyy * dyy + dzz * dzz;
-                    const float rs = p.w + q[k].w;
-                    // bitwise: evaluated without branches
-                    const bool h = (t0 + k < hi) & (d2 < rs * rs) & (d2 != 0.0f);
-                    hit |= h ? (1u << k) : 0u;
+                    for (int k = 0; k < CAND_UNROLL; ++k) q[k] = stg[t0 + off + k];
+                    test_batch(q, t0, hi);
                 }
-                while (hit) {
-                    const int k = __builtin_ctz(hit);
-                    hit &= hit - 1u;
-                    visit(t0 + k);
+                wave_lds_sync();  // every lane's reads precede the next row's copies
+            } else {
+                for (int t0 = lo; t0 < hi; t0 += CAND_UNROLL) {
+                    // the context's sorted positions are padded, so a batch
+                    // reads past hi without clamping
+                    const float4* qb = reinterpret_cast<const float4*>(
+                        reinterpret_cast<const char*>(a.pos) + (size_t)((uint32_t)t0 * (uint32_t)sizeof(float4)));
+                    float4 q[CAND_UNROLL];
+#pragma unroll
+                    for (int k = 0; k < CAND_UNROLL; ++k) q[k] = PADDED || t0 + k < hi ? qb[k] : qb[0];
+                    test_batch(q, t0, hi);
                 }
             }
         }
@@ -305,14 +372,15 @@ sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int blo
     SD_REQUIRE(block % WAVE == 0 && block <= 256, "contact kernel: block must be a multiple of 64, <= 256");
     const int nblocks = div_up(a.n, block);
     const dim3 grid(div_up(nblocks, XCDS) * XCDS);  // padded for the XCD remap
+    const size_t lds = (size_t)(block / WAVE) * CONTACT_STAGE * sizeof(float4);
     if (mode == CONTACT_FORCES)
-        hipLaunchKernelGGL((k_contacts<CONTACT_FORCES, false>), grid, dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_contacts<CONTACT_FORCES, false>), grid, dim3(block), lds, s, a);
     else if (mode == CONTACT_FUSED_U32)
-        hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U32, true>), grid, dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U32, true>), grid, dim3(block), lds, s, a);
     else if (mode == CONTACT_FUSED_U64)
-        hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U64, true>), grid, dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U64, true>), grid, dim3(block), lds, s, a);
     else
-        hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_BATCH, true>), grid, dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_BATCH, true>), grid, dim3(block), lds, s, a);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
