@@ -227,10 +227,12 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     // for one stencil row offset (dz, dy) their windows ascend with the lane
     // (rows and cells are in sorted order). The lanes in the first active
     // lane's grid row (segment A) and the others (segment B, normally the next
-    // row) each have a compact union of windows; both unions are copied into
-    // the wave's LDS region with coalesced loads and the candidates are then
-    // read from LDS. A wave whose unions do not fit (a wave over several
-    // sparse rows) reads its candidates from global memory for that offset.
+    // row) each have a compact union of windows. The two unions, each followed
+    // by the 3 positions a batch may read past a window, form one staged
+    // sequence: sorted position t of a lane is entry t + off of it. The wave
+    // copies the sequence into its LDS region with coalesced loads (in chunks
+    // of CONTACT_STAGE entries; one chunk unless the wave spans several sparse
+    // rows) and every lane tests its candidates in the chunk from LDS.
     extern __shared__ float4 cstage[];
     float4* const stg = cstage + (threadIdx.x >> 6) * CONTACT_STAGE;
     const uint64_t amask = __ballot(1);  // active lanes: a prefix of the wave
@@ -239,7 +241,9 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     const uint32_t rowid = cbase + (uint32_t)((cz * g.ny + cy) * g.nx);
     const bool segB = rowid != (uint32_t)__builtin_amdgcn_readfirstlane((int)rowid);
     const uint64_t bmask = __ballot(segB);
-    constexpr int PAD = CAND_UNROLL - 1;  // a batch reads up to 3 positions past its window
+    constexpr int PAD = CAND_UNROLL - 1;        // a batch reads up to 3 positions past its window
+    constexpr int CHUNK = CONTACT_STAGE - PAD;  // sequence entries tested per chunk
+    const char* const pos_bytes = reinterpret_cast<const char*>(a.pos);
 
     for (int dz = -1; dz <= 1; ++dz) {
         const int z = cz + dz;
@@ -259,30 +263,35 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
                 lb = __builtin_amdgcn_readlane(lo, __builtin_ctzll(mb));
                 nb = __builtin_amdgcn_readlane(hi, 63 - __builtin_clzll(mb)) - lb;
             }
-            const int offb = na + PAD;
-            if (offb + nb + PAD <= CONTACT_STAGE) {
-                for (int i = arank; i < na + PAD; i += nact) stg[i] = a.pos[min(la + i, a.n - 1)];
-                for (int i = arank; i < nb + PAD; i += nact) stg[offb + i] = a.pos[min(lb + i, a.n - 1)];
+            const int offb = na + PAD;                 // sequence entry of position lb
+            const int seq = offb + nb + PAD;           // sequence length
+            const int off = segB ? offb - lb : -la;    // sequence entry of sorted position t: t + off
+            const int lb_b = lb - offb;                // source of entry e >= offb: lb_b + e
+            for (int e0 = 0; e0 < seq - PAD; e0 += CHUNK) {
+                const int m = min(CONTACT_STAGE, seq - e0);
+                for (int i = arank; i < m; i += 2 * nact) {
+                    const int e1 = e0 + i, e2 = e1 + nact;
+                    const int s1 = min((e1 < offb ? la : lb_b) + e1, a.n - 1);
+                    const int s2 = min((e2 < offb ? la : lb_b) + e2, a.n - 1);
+                    const float4 v1 = *reinterpret_cast<const float4*>(pos_bytes + (size_t)((uint32_t)s1 * 16u));
+                    if (i + nact < m) {
+                        const float4 v2 = *reinterpret_cast<const float4*>(pos_bytes + (size_t)((uint32_t)s2 * 16u));
+                        stg[i + nact] = v2;
+                    }
+                    stg[i] = v1;
+                }
                 wave_lds_sync();
-                const int off = segB ? offb - lb : -la;  // LDS slot of sorted position t: t + off
-                for (int t0 = lo; t0 < hi; t0 += CAND_UNROLL) {
+                // this lane's positions in the chunk; a batch may read 3 entries
+                // past them (still inside the chunk's copy)
+                const int tlo = max(lo, e0 - off), thi = min(hi, e0 + CHUNK - off);
+                const float4* const sl = stg + (off - e0);
+                for (int t0 = tlo; t0 < thi; t0 += CAND_UNROLL) {
                     float4 q[CAND_UNROLL];
 #pragma unroll
-                    for (int k = 0; k < CAND_UNROLL; ++k) q[k] = stg[t0 + off + k];
-                    test_batch(q, t0, hi);
+                    for (int k = 0; k < CAND_UNROLL; ++k) q[k] = sl[t0 + k];
+                    test_batch(q, t0, thi);
                 }
-                wave_lds_sync();  // every lane's reads precede the next row's copies
-            } else {
-                for (int t0 = lo; t0 < hi; t0 += CAND_UNROLL) {
-                    // the context's sorted positions are padded, so a batch
-                    // reads past hi without clamping
-                    const float4* qb = reinterpret_cast<const float4*>(
-                        reinterpret_cast<const char*>(a.pos) + (size_t)((uint32_t)t0 * (uint32_t)sizeof(float4)));
-                    float4 q[CAND_UNROLL];
-#pragma unroll
-                    for (int k = 0; k < CAND_UNROLL; ++k) q[k] = PADDED || t0 + k < hi ? qb[k] : qb[0];
-                    test_batch(q, t0, hi);
-                }
+                wave_lds_sync();  // every lane's reads precede the next copy
             }
         }
     }

@@ -203,3 +203,49 @@ def test_batch_many_cell_changes(oracle, scale, speed):
     if speed > 100.0:
         assert changes > 100 and full_next == (scale > 1)
     b.close()
+
+
+def test_batch_tiny_scenes_equal_oracle(oracle):
+    """Forty scenes of 6 to 40 spheres: every wave spans several scenes (with
+    their own grids), and each scene equals the oracle bit for bit."""
+    from stardust_amd import _lib as L
+    say("batch of tiny scenes")
+    rng = np.random.default_rng(59)
+    scenes, cols, states = [], {k: [] for k in ("x", "y", "z", "r", "m", "vx", "vy", "vz")}, []
+    for k in range(40):
+        n = int(rng.integers(6, 41))
+        r_max = float(rng.uniform(0.002, 0.004))
+        box = 6 * r_max
+        lo = np.array([k * 0.1, 0.0, 0.0], np.float32)
+        pos = (lo + rng.uniform(r_max, box - r_max, (n, 3))).astype(np.float32)
+        rad = rng.uniform(0.5 * r_max, r_max, n).astype(np.float32)
+        m = W.sphere_mass(rad)
+        vel = rng.normal(0, 0.05, (n, 3)).astype(np.float32)
+        nc = 3
+        g = L.Grid.make(tuple(float(v) for v in lo), 2 * r_max, (nc, nc, nc))
+        c = L.ContactParams.make(1e4, 0.3, 0.5, 1e-5, gravity=(0.0, 0.0, -9.81),
+                                 walls=(tuple(float(v) for v in lo), tuple(float(v) + box for v in lo)),
+                                 max_contacts=32)
+        scenes.append((n, g, c))
+        for key, a in (("x", pos[:, 0]), ("y", pos[:, 1]), ("z", pos[:, 2]), ("r", rad), ("m", m),
+                       ("vx", vel[:, 0]), ("vy", vel[:, 1]), ("vz", vel[:, 2])):
+            cols[key].append(np.ascontiguousarray(a))
+        st = oracle.SphereState(pos[:, 0], pos[:, 1], pos[:, 2], rad, m, vx=vel[:, 0], vy=vel[:, 1],
+                                vz=vel[:, 2], max_contacts=32)
+        st.step(g, c, nsteps=8, nthreads=1)
+        states.append(st)
+    s = {key: np.concatenate(v) for key, v in cols.items()}
+    b = batch_of(None, scenes, s, [s["vx"], s["vy"], s["vz"]])
+    b.step(8)
+    got = b.download()
+    assert b.contact_count() > 0
+    off = 0
+    for k, ((n, g, c), st) in enumerate(zip(scenes, states)):
+        sl = slice(off, off + n)
+        e = st.by_id()
+        for f, (arr, col) in {"x": ("pos4", 0), "y": ("pos4", 1), "z": ("pos4", 2), "vx": ("vel4", 0),
+                              "vy": ("vel4", 1), "vz": ("vel4", 2), "wx": ("omg4", 0), "wy": ("omg4", 1),
+                              "wz": ("omg4", 2)}.items():
+            assert np.array_equal(got[f][sl], e[arr][:, col]), ("scene", k, f)
+        off += n
+    b.close()

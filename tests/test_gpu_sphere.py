@@ -420,3 +420,35 @@ def test_step_many_cell_changes(oracle, n, speed):
     if speed > 100.0 and n != 11400:  # the path this case is meant to exercise was taken
         assert full_next == (n > 1024 * 10) and changes > 100
     sc.close()
+
+
+@pytest.mark.parametrize("key64", [False, True])
+def test_step_sparse_plane_under_dense_plane(oracle, key64):
+    """One sphere per grid row in the bottom plane and a dense plane above:
+    the first wave's 64 lanes lie in 64 different rows, so the candidates of
+    their upper-plane rows (about 1,000 positions) are staged and tested in
+    several chunks. Bit-identical to the oracle."""
+    say("sparse plane under a dense plane", key64)
+    rng = np.random.default_rng(53)
+    cell, nx, ny = 0.01, 4, 64
+    n0, n1 = ny, 1024
+    x = np.concatenate([rng.uniform(0.004, 0.036, n0), rng.uniform(0.004, 0.036, n1)])
+    y = np.concatenate([(np.arange(n0) + 0.5) * cell, rng.uniform(0.004, ny * cell - 0.004, n1)])
+    z = np.concatenate([np.full(n0, 0.0075), rng.uniform(0.0105, 0.0195, n1)])
+    r = np.concatenate([np.full(n0, 0.003), rng.uniform(0.002, 0.004, n1)])
+    init = {"x": x.astype(np.float32), "y": y.astype(np.float32), "z": z.astype(np.float32),
+            "r": r.astype(np.float32)}
+    init["m"] = W.sphere_mass(init["r"])
+    vel = rng.normal(0, 0.05, (n0 + n1, 3)).astype(np.float32)
+    init.update(vx=vel[:, 0].copy(), vy=vel[:, 1].copy(), vz=vel[:, 2].copy())
+    g = L.Grid.make((0.0, 0.0, 0.0), cell, (nx, ny, 3))
+    c = L.ContactParams.make(1e4, 0.3, 0.5, 1e-5, gravity=(0.0, 0.0, -9.81),
+                             walls=((0.0, 0.0, 0.0), (nx * cell, ny * cell, 3 * cell)), max_contacts=32)
+    st = oracle.SphereState(init["x"], init["y"], init["z"], init["r"], init["m"], vx=init["vx"], vy=init["vy"],
+                            vz=init["vz"], max_contacts=32)
+    sc = make_scene(init, g, c, key64=key64, graph=True)
+    st.step(g, c, nsteps=6, nthreads=8)
+    sc.step(6)
+    compare_scene(sc, st)
+    assert sc.contact_count() == int(st.hcount.sum()) > 0
+    sc.close()
