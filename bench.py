@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # runs on MI355X (FETCH x 2.0 + WRITE x 1.0 KiB, factors from the plain copy kernel);
 # the runs and the per-kernel table are in profiles/bench_hbm_counters.md.
 MEASURED_TRAFFIC = {
-    ("dem_c2_column_4194304", "contact"): (267881.0 * 2.0 + 262144.0) * 1024.0,
+    ("dem_c2_column_4194304", "contact"): (270612.5 * 2.0 + 262144.0) * 1024.0,
 }
 
 
@@ -71,8 +71,8 @@ def stage_bytes(w, n, contacts_per_sphere, n_cells):
         sort_b, permute_b = n * (key_b + passes * 2 * (key_b + val_b)), n * ((key_b + val_b) + 2 * 16)
     return {
         "sort": sort_b,
-        # read every sorted key once, clear + write start/end of every cell
-        "ranges": n * key_b + 8 * n_cells,
+        # read every sorted key once, write the start of every cell
+        "ranges": n * key_b + 4 * (n_cells + 1),
         "permute": permute_b,
         # own state 52 B + old slot + history count in/out 8 B + state out 52 B + next key;
         # per contact: old history row 16 B read + new row 16 B written

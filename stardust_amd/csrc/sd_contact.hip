@@ -133,18 +133,18 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     if (s >= a.n) return;
     GridDev g;
     ContactDev c;
-    const int* ranges;
+    const int* ranges;  // CONTACT_FORCES: (start, end) per cell; fused modes: cell starts
     uint32_t cbase = 0u;
     if constexpr (BATCH) {
         const SceneDev& S = a.scenes[a.scene_of[s]];
         g = S.g;
         c = S.c;
         cbase = S.cbase;
-        ranges = a.ranges + 2 * (size_t)cbase;
+        ranges = a.cell_start + cbase;
     } else {
         g = a.g;
         c = a.c;
-        ranges = a.ranges;
+        ranges = MODE == CONTACT_FORCES ? a.ranges : a.cell_start;
     }
     float4 p = a.pos[s];
     const int os = old_slot_of(a, s);
@@ -250,7 +250,10 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
         for (int dy = -1; dy <= 1; ++dy) {
             const int y = cy + dy;
             int lo = 0, hi = 0;
-            if (z >= 0 && z < g.nz && y >= 0 && y < g.ny) row_range(g, ranges, cx, y, z, lo, hi);
+            if (z >= 0 && z < g.nz && y >= 0 && y < g.ny) {
+                if constexpr (MODE == CONTACT_FORCES) row_range(g, ranges, cx, y, z, lo, hi);
+                else row_window(g, ranges, cx, y, z, lo, hi);
+            }
             const uint64_t ne = __ballot(hi > lo);
             if (ne == 0u) continue;
             const uint64_t ma = ne & ~bmask, mb = ne & bmask;

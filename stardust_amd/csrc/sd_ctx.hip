@@ -5,8 +5,10 @@
 // (src/engine/entities/entity_handler.cpp:196-472) for sphere scenes.
 //
 // One step, all on the context stream, no host synchronisation:
-//   sort(keys)            keys were written by the previous step (or upload)
-//   cell ranges           clear (fill kernel) + mark runs
+//   sort(keys)            keys were written by the previous step (or upload):
+//                         the radix sort, or with u64 keys the re-sort from
+//                         the previous order (which also permutes positions)
+//   cell starts           first sorted position of every cell
 //   permute positions     sorted positions gathered from the current state
 //   fused contact kernel  reads sorted positions, the current state (through the
 //                         sort permutation) and the old history; writes the
@@ -97,7 +99,7 @@ struct sd_ctx {
     int cur_keys = 0;
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
-    int* ranges = nullptr;
+    int* cell_start = nullptr;  // n_cells + 1 entries
     int* hcount[2] = {nullptr, nullptr};
     float4* hist[2] = {nullptr, nullptr};
     int cur_hist = 0;
@@ -139,7 +141,7 @@ void release(sd_ctx* c) {
     drop_graphs(c);
     void* bufs[] = {c->pos[0], c->pos[1], c->vel[0], c->vel[1], c->omg[0], c->omg[1], c->ids[0], c->ids[1],
                     c->posB, c->k32[0], c->k32[1], c->v32[0], c->v32[1], c->k64[0], c->k64[1], c->sort_tmp,
-                    c->ranges, c->hcount[0], c->hcount[1], c->hist[0], c->hist[1], c->overflow, c->diag,
+                    c->cell_start, c->hcount[0], c->hcount[1], c->hist[0], c->hist[1], c->overflow, c->diag,
                     c->scenes_d, c->scene_of, c->mcount, c->mnew, c->mold, c->msn, c->mso, c->step_ctl};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -151,7 +153,7 @@ void release(sd_ctx* c) {
     c->k32[0] = c->k32[1] = c->v32[0] = c->v32[1] = nullptr;
     c->k64[0] = c->k64[1] = nullptr;
     c->sort_tmp = nullptr;
-    c->ranges = nullptr;
+    c->cell_start = nullptr;
     c->hcount[0] = c->hcount[1] = nullptr;
     c->hist[0] = c->hist[1] = nullptr;
     c->overflow = nullptr;
@@ -192,7 +194,7 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
     a.ids = c->ids[so];
     a.n = n;
     a.g = to_dev(c->p.grid);
-    a.ranges = c->ranges;
+    a.cell_start = c->cell_start;
     a.c = to_dev(c->p.contact);
     a.hcount_old = c->hcount[ho];
     a.hist_old = c->hist[ho];
@@ -219,18 +221,16 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
             st = radix_sort_keys_u64_gated(&K, &Ka, n, 32, 32 + c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s,
                                            c->items, nullptr, 0);
         } else {
-            st = launch_resort_u64(mc_read, c->mnew, c->mold, c->msn, c->mso, K, n, c->ranges, to_dev(c->p.grid),
-                                   c->batch ? c->scenes_d : nullptr, c->scene_of, c->pos[so], c->posB, Ka,
-                                   c->step_ctl, s);
+            st = launch_resort_u64(mc_read, c->mnew, c->mold, c->msn, c->mso, K, n, c->cell_start, c->pos[so],
+                                   c->posB, Ka, c->step_ctl, s);
             uint64_t* t = K;
             K = Ka;
             Ka = t;
         }
         if (st != SD_OK) return st;
         MARK(1);
-        // ranges (after the re-sort only the cells left by changed spheres
-        // needed clearing); after the full sort, the permutation; mc_write = 0
-        st = launch_step_ranges_u64(K, n, c->n_cells, c->ranges, c->step_ctl, full, full, c->pos[so], c->posB,
+        // cell starts; after the full sort, the permutation; mc_write = 0
+        st = launch_step_starts_u64(K, n, c->n_cells, c->cell_start, c->step_ctl, full, c->pos[so], c->posB,
                                     mc_write, s);
         if (st != SD_OK) return st;
         MARK(2);
@@ -256,7 +256,7 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
         st = radix_sort_pairs_u32(&K, &V, &Ka, &Va, n, 0, c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s, c->items);
         if (st != SD_OK) return st;
         MARK(1);
-        st = launch_cell_ranges_u32(K, n, c->n_cells, c->ranges, s);
+        st = launch_cell_starts_u32(K, n, c->n_cells, c->cell_start, s);
         if (st != SD_OK) return st;
         MARK(2);
         st = launch_permute_pos(V, nullptr, n, c->pos[so], c->posB, s);
@@ -465,7 +465,7 @@ sd_status upload_state(sd_ctx* c, int n, const float* x, const float* y, const f
     }
     c->sort_tmp_bytes = radix_temp_bytes(n);
     ALLOC(*(char**)&c->sort_tmp, c->sort_tmp_bytes);
-    ALLOC(c->ranges, 2 * (size_t)c->n_cells);
+    ALLOC(c->cell_start, (size_t)c->n_cells + 1);
     ALLOC(c->hcount[0], n);
     ALLOC(c->hcount[1], n);
     ALLOC(c->hist[0], (size_t)maxc * n);

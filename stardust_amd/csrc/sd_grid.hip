@@ -151,39 +151,78 @@ sd_status launch_cell_ranges_u64(const uint64_t* keys, int n, int n_cells, int* 
     return SD_OK;
 }
 
-// The step's ranges with u64 keys, fused with what else reads the sorted keys:
-// the positions gathered into sorted order (permute; the re-sort does it
-// otherwise) and the next step's changed-cell counter cleared. Returns at once
-// while *skip (the step was given up).
-__global__ void k_step_ranges_u64(const uint64_t* __restrict__ keys, int n, int n_cells, int* __restrict__ ranges,
-                                  const int* __restrict__ skip, int permute, const float4* __restrict__ pos,
-                                  float4* __restrict__ pos_o, int* __restrict__ next_count) {
-    if (skip && *skip) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *next_count = 0;
-    if (i >= n) return;
-    const uint64_t k = keys[i];
-    if (permute) {
-        const uint32_t src = (uint32_t)k;
-        if (src < (uint32_t)n) pos_o[i] = pos[src];
+// First sorted position of every cell: start[c] = the number of keys whose
+// cell is below c, for c = 0 .. n_cells (start[n_cells] = n), so that cells
+// a .. b hold the sorted positions [start[a], start[b + 1]) whether or not
+// they are empty. Key i writes the entries of the cells after key i-1's cell
+// up to its own; the last key also those after its cell. Every entry is
+// written once per step. Up to CELL_GAP_INLINE entries are written by the
+// key's lane, longer gaps (empty stretches of a sparse grid) by its whole wave.
+constexpr int CELL_GAP_INLINE = 8;
+
+__device__ __forceinline__ void fill_cell_starts(int* __restrict__ start, int lo, int hi, int val) {
+    const int e = min(hi, lo + CELL_GAP_INLINE - 1);
+    for (int c = lo; c <= e; ++c) start[c] = val;
+    uint64_t m = __ballot(hi > e);  // lanes with a longer gap (all lanes take part)
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1u;
+        const int a = __builtin_amdgcn_readlane(e + 1, l);
+        const int b = __builtin_amdgcn_readlane(hi, l);
+        const int v = __builtin_amdgcn_readlane(val, l);
+        for (int c = a + lane_id(); c <= b; c += WAVE) start[c] = v;
     }
-    const uint32_t c = (uint32_t)(k >> 32);
-    if (c >= (uint32_t)n_cells) return;  // not a cell of this grid
-    if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != c) ranges[2 * (size_t)c] = i;
-    if (i == n - 1 || (uint32_t)(keys[i + 1] >> 32) != c) ranges[2 * (size_t)c + 1] = i + 1;
 }
 
-sd_status launch_step_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, const int* skip, bool permute,
-                                 bool clear_all, const float4* pos, float4* pos_o, int* next_count, hipStream_t s) {
-    if (clear_all) {
-        sd_status st = launch_fill_u32((uint32_t*)ranges, 0u, 2 * (size_t)n_cells, s);
-        if (st != SD_OK) return st;
+// The cell starts of a step's sorted keys, fused with what else reads them:
+// with permute, the positions gathered into sorted order (the re-sort does it
+// otherwise); next_count (if given) set to 0. Does nothing while *skip (the
+// step was given up). Cell ids at or past n_cells (never written by this
+// library) count as n_cells.
+template <typename K, int SHIFT>
+__global__ void k_cell_starts(const K* __restrict__ keys, int n, int n_cells, int* __restrict__ start,
+                              const int* __restrict__ skip, int permute, const float4* __restrict__ pos,
+                              float4* __restrict__ pos_o, int* __restrict__ next_count) {
+    if (skip && *skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && next_count) *next_count = 0;
+    int lo = 0, hi = -1, tail_lo = 0, tail_hi = -1;  // every lane stays for the wave-wide fills
+    if (i < n) {
+        const K k = keys[i];
+        if (permute) {
+            const uint32_t src = (uint32_t)k;
+            if (src < (uint32_t)n) pos_o[i] = pos[src];
+        }
+        const uint32_t c = min((uint32_t)(k >> SHIFT), (uint32_t)n_cells);
+        const uint32_t prev = i == 0 ? 0u : min((uint32_t)(keys[i - 1] >> SHIFT), (uint32_t)n_cells) + 1u;
+        lo = (int)prev;
+        hi = (int)c;
+        if (i == n - 1) {
+            tail_lo = (int)c + 1;
+            tail_hi = n_cells;
+        }
     }
-    if (n <= 0) return SD_OK;
-    hipLaunchKernelGGL(k_step_ranges_u64, dim3(div_up(n, 256)), dim3(256), 0, s, keys, n, n_cells, ranges, skip,
-                       permute ? 1 : 0, pos, pos_o, next_count);
+    fill_cell_starts(start, lo, hi, i);
+    fill_cell_starts(start, tail_lo, tail_hi, n);
+}
+
+template <typename K, int SHIFT>
+static sd_status launch_cell_starts(const K* keys, int n, int n_cells, int* start, const int* skip, bool permute,
+                                    const float4* pos, float4* pos_o, int* next_count, hipStream_t s) {
+    if (n <= 0) return launch_fill_u32((uint32_t*)start, 0u, (size_t)n_cells + 1, s);
+    hipLaunchKernelGGL((k_cell_starts<K, SHIFT>), dim3(div_up(n, 256)), dim3(256), 0, s, keys, n, n_cells, start,
+                       skip, permute ? 1 : 0, pos, pos_o, next_count);
     SD_LAUNCH_CHECK();
     return SD_OK;
+}
+
+sd_status launch_cell_starts_u32(const uint32_t* keys, int n, int n_cells, int* start, hipStream_t s) {
+    return launch_cell_starts<uint32_t, 0>(keys, n, n_cells, start, nullptr, false, nullptr, nullptr, nullptr, s);
+}
+
+sd_status launch_step_starts_u64(const uint64_t* keys, int n, int n_cells, int* start, const int* skip, bool permute,
+                                 const float4* pos, float4* pos_o, int* next_count, hipStream_t s) {
+    return launch_cell_starts<uint64_t, 32>(keys, n, n_cells, start, skip, permute, pos, pos_o, next_count, s);
 }
 
 // ------------------------------------------------------------------ permute

@@ -55,4 +55,16 @@ __device__ __forceinline__ void row_range(const GridDev& g, const int* __restric
     if (hi == 0) lo = 0;
 }
 
+// Sorted-position window of cells (cx-1 .. cx+1, y, z) from the cell starts
+// (every cell has one, empty or not): [start of the first cell, start of the
+// cell after the last). Empty window: lo == hi.
+__device__ __forceinline__ void row_window(const GridDev& g, const int* __restrict__ start, int cx, int y, int z,
+                                           int& lo, int& hi) {
+    const int base = (z * g.ny + y) * g.nx;
+    const int xa = cx > 0 ? cx - 1 : cx;
+    const int xe = cx < g.nx - 1 ? cx + 2 : cx + 1;
+    lo = start[(size_t)(base + xa)];
+    hi = start[(size_t)(base + xe)];
+}
+
 }  // namespace sd

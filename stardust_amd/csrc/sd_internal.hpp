@@ -58,12 +58,13 @@ sd_status launch_hash_keys_batch(const float4* pos4, int n, const SceneDev* scen
                                  uint64_t* keys, hipStream_t s);
 sd_status launch_cell_ranges_u32(const uint32_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
 sd_status launch_cell_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, hipStream_t s);
-// Cell ranges of the step's sorted u64 keys (after clearing every range when
-// clear_all; otherwise every cell that may have emptied is clear already); with
-// permute, gathers pos into sorted order (pos_o); sets *next_count = 0. Does
-// nothing while *skip.
-sd_status launch_step_ranges_u64(const uint64_t* keys, int n, int n_cells, int* ranges, const int* skip, bool permute,
-                                 bool clear_all, const float4* pos, float4* pos_o, int* next_count, hipStream_t s);
+// Cell starts of sorted keys: start[c] = number of keys whose cell is below c,
+// c = 0 .. n_cells (n_cells + 1 entries), every entry written. The u64 form
+// also gathers pos into sorted order (pos_o) with permute and sets
+// *next_count = 0; it does nothing while *skip.
+sd_status launch_cell_starts_u32(const uint32_t* keys, int n, int n_cells, int* start, hipStream_t s);
+sd_status launch_step_starts_u64(const uint64_t* keys, int n, int n_cells, int* start, const int* skip, bool permute,
+                                 const float4* pos, float4* pos_o, int* next_count, hipStream_t s);
 // perm is either u32 values (perm32) or the low halves of u64 keys (perm64).
 sd_status launch_permute(const uint32_t* perm32, const uint64_t* perm64, int n, const float4* pos, const float4* vel,
                          const float4* omg, const int* ids, float4* pos_o, float4* vel_o, float4* omg_o, int* ids_o,
@@ -111,7 +112,8 @@ struct ContactArgs {
     const int* ids;
     int n;
     GridDev g;
-    const int* ranges;
+    const int* ranges;      // CONTACT_FORCES: (start, end) per cell, empty cells (0, 0)
+    const int* cell_start;  // fused modes: first sorted position of every cell (n_cells + 1 entries)
     ContactDev c;
     // old order: either an int array, u32 sort values or packed u64 keys
     const int* old_slot_i32;
@@ -150,15 +152,13 @@ sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int blo
 
 // Sorted order of the next step's u64 keys from the previous sorted order and
 // the spheres that changed cell (see sd_resort.hip): writes the sorted keys to
-// keys_out and the positions in that order to posB, and clears the ranges of
-// the cells the changed spheres left (ranges holds the previous step's cell
-// ranges). With *mcount > RESORT_CAP it gives the step up instead
-// (step_ctl[0] = 1).
+// keys_out and the positions in that order to posB (cell_start holds the
+// previous order's cell starts). With *mcount > RESORT_CAP it gives the step
+// up instead (step_ctl[0] = 1).
 constexpr int RESORT_CAP = 1024;
 sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint64_t* mold, uint64_t* msorted_new,
-                            uint64_t* msorted_old, const uint64_t* keys, int n, int* ranges,
-                            const GridDev& g, const SceneDev* scenes, const int* scene_of, const float4* pos_state,
-                            float4* posB, uint64_t* keys_out, int* step_ctl, hipStream_t s);
+                            uint64_t* msorted_old, const uint64_t* keys, int n, const int* cell_start,
+                            const float4* pos_state, float4* posB, uint64_t* keys_out, int* step_ctl, hipStream_t s);
 sd_status launch_integrate(float4* pos, float4* vel, float4* omg, const float4* f, const float4* t, int n,
                            const ContactDev& c, hipStream_t s);
 
