@@ -299,6 +299,9 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
         }
     }
 
+    // walls: the touched ones first, so that the contact-law set-up of a wall
+    // contact runs only where some lane of the wave touches a wall
+    uint32_t touch = 0u;
     if (c.walls && v.w > 0.0f) {
 #pragma unroll
         for (int wl = 0; wl < 6; ++wl) {
@@ -306,7 +309,17 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
             const bool upper = wl & 1;
             const float pc = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
             const float dist = upper ? c.whi[ax] - pc : pc - c.wlo[ax];
-            if (!(dist < p.w)) continue;
+            touch |= dist < p.w ? (1u << wl) : 0u;
+        }
+    }
+    if (touch) {
+#pragma unroll
+        for (int wl = 0; wl < 6; ++wl) {
+            if (!((touch >> wl) & 1u)) continue;
+            const int ax = wl >> 1;
+            const bool upper = wl & 1;
+            const float pc = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
+            const float dist = upper ? c.whi[ax] - pc : pc - c.wlo[ax];
             const float sgn = upper ? -1.0f : 1.0f;
             const float nx = ax == 0 ? sgn : 0.0f;
             const float ny = ax == 1 ? sgn : 0.0f;
