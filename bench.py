@@ -63,16 +63,19 @@ def stage_bytes(w, n, contacts_per_sphere, n_cells):
     c = contacts_per_sphere
     if w.key64:
         # re-sort from the previous order: read keys and pos4, write sorted keys
-        # and pos4 (the permutation rides on it; the changed-cell lists are tiny)
-        sort_b, permute_b = n * 2 * (key_b + 16), 0
+        # and pos4; the permutation and the cell-start update ride on it (the
+        # changed-cell lists are tiny and only cells between a changed sphere's
+        # old and new cell are rewritten)
+        sort_b, ranges_b, permute_b = n * 2 * (key_b + 16), 0, 0
     else:
         # one histogram read of the keys, then per pass read + write of keys and
-        # values; permute: read the permutation, gather pos4, write sorted pos4
-        sort_b, permute_b = n * (key_b + passes * 2 * (key_b + val_b)), n * ((key_b + val_b) + 2 * 16)
+        # values; cell starts: read every sorted key once, write the start of
+        # every cell; permute: read the permutation, gather pos4, write sorted pos4
+        sort_b = n * (key_b + passes * 2 * (key_b + val_b))
+        ranges_b, permute_b = n * key_b + 4 * (n_cells + 1), n * ((key_b + val_b) + 2 * 16)
     return {
         "sort": sort_b,
-        # read every sorted key once, write the start of every cell
-        "ranges": n * key_b + 4 * (n_cells + 1),
+        "ranges": ranges_b,
         "permute": permute_b,
         # own state 52 B + old slot + history count in/out 8 B + state out 52 B + next key;
         # per contact: old history row 16 B read + new row 16 B written

@@ -221,17 +221,18 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
             st = radix_sort_keys_u64_gated(&K, &Ka, n, 32, 32 + c->key_bits, c->sort_tmp, c->sort_tmp_bytes, s,
                                            c->items, nullptr, 0);
         } else {
-            st = launch_resort_u64(mc_read, c->mnew, c->mold, c->msn, c->mso, K, n, c->cell_start, c->pos[so],
-                                   c->posB, Ka, c->step_ctl, s);
+            // also updates the cell starts and sets mc_write = 0
+            st = launch_resort_u64(mc_read, c->mnew, c->mold, c->msn, c->mso, K, n, c->cell_start, c->n_cells,
+                                   c->pos[so], c->posB, Ka, mc_write, c->step_ctl, s);
             uint64_t* t = K;
             K = Ka;
             Ka = t;
         }
         if (st != SD_OK) return st;
         MARK(1);
-        // cell starts; after the full sort, the permutation; mc_write = 0
-        st = launch_step_starts_u64(K, n, c->n_cells, c->cell_start, c->step_ctl, full, c->pos[so], c->posB,
-                                    mc_write, s);
+        // after the full sort: cell starts, the permutation, mc_write = 0
+        if (full) st = launch_step_starts_u64(K, n, c->n_cells, c->cell_start, c->step_ctl, true, c->pos[so],
+                                              c->posB, mc_write, s);
         if (st != SD_OK) return st;
         MARK(2);
         MARK(3);

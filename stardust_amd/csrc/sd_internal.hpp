@@ -152,13 +152,15 @@ sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int blo
 
 // Sorted order of the next step's u64 keys from the previous sorted order and
 // the spheres that changed cell (see sd_resort.hip): writes the sorted keys to
-// keys_out and the positions in that order to posB (cell_start holds the
-// previous order's cell starts). With *mcount > RESORT_CAP it gives the step
-// up instead (step_ctl[0] = 1).
+// keys_out and the positions in that order to posB, turns cell_start (the
+// previous order's cell starts, n_cells + 1 entries) into the new order's and
+// sets *next_count = 0. With *mcount > RESORT_CAP it gives the step up instead
+// (step_ctl[0] = 1).
 constexpr int RESORT_CAP = 1024;
 sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint64_t* mold, uint64_t* msorted_new,
-                            uint64_t* msorted_old, const uint64_t* keys, int n, const int* cell_start,
-                            const float4* pos_state, float4* posB, uint64_t* keys_out, int* step_ctl, hipStream_t s);
+                            uint64_t* msorted_old, const uint64_t* keys, int n, int* cell_start, int n_cells,
+                            const float4* pos_state, float4* posB, uint64_t* keys_out, int* next_count,
+                            int* step_ctl, hipStream_t s);
 sd_status launch_integrate(float4* pos, float4* vel, float4* omg, const float4* f, const float4* t, int n,
                            const ContactDev& c, hipStream_t s);
 

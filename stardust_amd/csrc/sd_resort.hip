@@ -75,11 +75,26 @@ __global__ __launch_bounds__(RESORT_CAP) void k_resort_movers(
     }
 }
 
-// Every stayer: its place, key and position.
+__device__ __forceinline__ int lower_bound_u64(const uint64_t* a, int lo, int hi, uint64_t key) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Every stayer: its place, key and position. Also the new order's cell starts,
+// in place: start[c] counts the keys below c << 32, so it changes by the
+// movers' new keys below c << 32 minus their old keys below c << 32. Each
+// block updates cells_per_block consecutive cells, RESORT_MERGE_BLOCK at a
+// time; a run of cells that holds no mover key has one change for all its
+// cells (most runs: none). Sets *next_count = 0 for the next step's movers.
 __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
     const int* __restrict__ mcount, const uint64_t* __restrict__ msn, const uint64_t* __restrict__ mso,
     const uint64_t* __restrict__ keys, int n, const float4* __restrict__ pos_state, float4* __restrict__ posB,
-    uint64_t* __restrict__ keys_out, const int* __restrict__ step_ctl) {
+    uint64_t* __restrict__ keys_out, int* __restrict__ cell_start, int n_cells, int cells_per_block,
+    int* __restrict__ next_count, const int* __restrict__ step_ctl) {
     if (step_ctl[0]) return;
     const int M = *mcount;
     if (M > RESORT_CAP) return;
@@ -89,6 +104,33 @@ __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
         so[j] = mso[j];
     }
     __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x == 0) *next_count = 0;
+    const int c0 = blockIdx.x * cells_per_block;
+    const int c1 = min(c0 + cells_per_block, n_cells);
+    if (M > 0 && c0 < c1) {
+        // mover keys below the run start (new: alo, old: blo); the runs are
+        // consecutive, so each next bound is a short scan from the last one
+        int alo = lower_bound_u64(sn, 0, M, (uint64_t)c0 << 32);
+        int blo = lower_bound_u64(so, 0, M, (uint64_t)c0 << 32);
+        for (int cs = c0; cs < c1; cs += RESORT_MERGE_BLOCK) {
+            const int ce = min(cs + RESORT_MERGE_BLOCK, c1);
+            const uint64_t khi = (uint64_t)ce << 32;
+            int ahi = alo, bhi = blo;
+            while (ahi < M && sn[ahi] < khi) ++ahi;
+            while (bhi < M && so[bhi] < khi) ++bhi;
+            const int c = cs + (int)threadIdx.x;
+            if (c < ce) {
+                int d = alo - blo;
+                if (alo != ahi || blo != bhi) {
+                    const uint64_t k = (uint64_t)c << 32;
+                    d = lower_bound_u64(sn, alo, ahi, k) - lower_bound_u64(so, blo, bhi, k);
+                }
+                if (d != 0) cell_start[c] += d;
+            }
+            alo = ahi;
+            blo = bhi;
+        }
+    }
     const int s = blockIdx.x * RESORT_MERGE_BLOCK + (int)threadIdx.x;
     if (s >= n) return;
     // movers before s: old keys ascend with their positions (the low halves)
@@ -100,12 +142,7 @@ __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
     }
     if (lo < M && (uint32_t)so[lo] == (uint32_t)s) return;  // a mover, placed by k_resort_movers
     const uint64_t y = keys[s];
-    int a = 0, b = M;
-    while (a < b) {
-        const int mid = (a + b) >> 1;
-        if (sn[mid] < y) a = mid + 1;
-        else b = mid;
-    }
+    const int a = lower_bound_u64(sn, 0, M, y);
     const int d = s - lo + a;
     if (d < 0 || d >= n) return;  // (never, for consistent keys)
     keys_out[d] = y;
@@ -113,15 +150,18 @@ __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
 }
 
 sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint64_t* mold, uint64_t* msorted_new,
-                            uint64_t* msorted_old, const uint64_t* keys, int n, const int* cell_start,
-                            const float4* pos_state, float4* posB, uint64_t* keys_out, int* step_ctl, hipStream_t s) {
+                            uint64_t* msorted_old, const uint64_t* keys, int n, int* cell_start, int n_cells,
+                            const float4* pos_state, float4* posB, uint64_t* keys_out, int* next_count,
+                            int* step_ctl, hipStream_t s) {
     if (n <= 0) return SD_OK;
     hipLaunchKernelGGL(k_resort_movers, dim3(1), dim3(RESORT_CAP), 0, s, mcount, mnew, mold, msorted_new,
-                       msorted_old, n, cell_start, pos_state, posB, keys_out, step_ctl);
+                       msorted_old, n, (const int*)cell_start, pos_state, posB, keys_out, step_ctl);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_resort_merge, dim3(div_up(n, RESORT_MERGE_BLOCK)), dim3(RESORT_MERGE_BLOCK), 0, s, mcount,
+    const int nblocks = div_up(n, RESORT_MERGE_BLOCK);
+    const int cells_per_block = div_up(div_up(n_cells, nblocks), RESORT_MERGE_BLOCK) * RESORT_MERGE_BLOCK;
+    hipLaunchKernelGGL(k_resort_merge, dim3(nblocks), dim3(RESORT_MERGE_BLOCK), 0, s, mcount,
                        (const uint64_t*)msorted_new, (const uint64_t*)msorted_old, keys, n, pos_state, posB, keys_out,
-                       (const int*)step_ctl);
+                       cell_start, n_cells, cells_per_block, next_count, (const int*)step_ctl);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
