@@ -159,7 +159,7 @@ sd_status sd_permute_spheres(const int* perm, int n,
 
 /* ------------------------------------------------------- scene contexts
  * A context owns the device state of one sphere scene and runs whole steps:
- *   keys -> sort -> cell ranges -> permute -> fused contact + integrate + next keys,
+ *   keys -> sort -> cell starts -> permute -> fused contact + integrate + next keys,
  * replayed from a hipGraph. Replaces DEMEngine::update
  * (src/engine/cuda/simulation_kernel.cu:52-153) and cleanBuffers (:30-50). */
 typedef struct sd_sphere_params {
@@ -184,11 +184,12 @@ sd_status sd_ctx_upload_spheres(sd_ctx* ctx, int n, const float* x, const float*
 sd_status sd_ctx_step(sd_ctx* ctx, int nsteps);
 /* Runs nsteps steps without graph replay, recording HIP events on the context
  * stream around each stage, and adds the time of each stage summed over the
- * steps to stage_ms: [0] sort, [1] cell ranges (clear + mark), [2] permute,
- * [3] fused contact + integrate + next keys. With u64 keys, [0] is the re-sort
- * from the previous order (positions permuted in the same pass) when at most
- * 1024 spheres changed cell in the previous step, else the radix sort (all
- * passes), and the positions are then permuted within [1]; [2] is empty. */
+ * steps to stage_ms: [0] sort, [1] cell starts (first sorted position of
+ * every cell), [2] permute, [3] fused contact + integrate + next keys. With u64
+ * keys, [0] is the re-sort from the previous order (positions permuted and
+ * cell starts updated in the same pass; [1] is empty) when at most 1024
+ * spheres changed cell in the previous step, else the radix sort (all passes),
+ * and the positions are then permuted within [1]; [2] is empty. */
 sd_status sd_ctx_profile(sd_ctx* ctx, int nsteps, float* stage_ms, int nstages);
 /* Spheres that changed cell in the last step (u64 keys; 0 with u32 keys):
  * *full_next = 1 if the next step uses the full radix sort (more than 1024

@@ -39,9 +39,10 @@ class SphereScene:
 
     def profile(self, nsteps):
         """Per-stage GPU time in ms summed over nsteps eager steps:
-        sort, ranges, permute, contact (HIP events on the context stream). With
-        u64 keys "sort" is the re-sort, positions permuted in the same pass,
-        unless the full radix sort ran (then "permute" is the permutation)."""
+        sort, ranges (the cell starts), permute, contact (HIP events on the
+        context stream). With u64 keys "sort" is the re-sort, positions permuted
+        and cell starts updated in the same pass, unless the full radix sort ran
+        (then "ranges" holds the cell starts and the permutation)."""
         out = np.zeros(4, np.float32)
         L.call("sd_ctx_profile", self.h, int(nsteps), L.ptr(out), 4)
         return {"sort": float(out[0]), "ranges": float(out[1]), "permute": float(out[2]),
