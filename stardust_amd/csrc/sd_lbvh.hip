@@ -47,12 +47,27 @@ __global__ void k_domain_partial(const float4* __restrict__ lo4, const float4* _
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_domain_final(const float* __restrict__ part, int nb, float* __restrict__ dom) {
-    if (threadIdx.x >= 6) return;
-    const int k = threadIdx.x;
-    float v = part[k];
-    for (int b = 1; b < nb; ++b) v = k < 3 ? fminf(v, part[b * 6 + k]) : fmaxf(v, part[b * 6 + k]);
-    dom[k] = v;
+// One thread per partial (nb <= DOMAIN_PARTS), then a tree reduction in LDS;
+// min and max are exact, so the order does not matter (threads past nb repeat
+// partial 0).
+constexpr int DOMAIN_PARTS = 256;
+__global__ __launch_bounds__(DOMAIN_PARTS) void k_domain_final(const float* __restrict__ part, int nb,
+                                                               float* __restrict__ dom) {
+    __shared__ float s[6][DOMAIN_PARTS];
+    const int t = threadIdx.x;
+    const int b = t < nb ? t : 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k][t] = part[b * 6 + k];
+    __syncthreads();
+    for (int h = DOMAIN_PARTS / 2; h > 0; h >>= 1) {
+        if (t < h) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                s[k][t] = k < 3 ? fminf(s[k][t], s[k][t + h]) : fmaxf(s[k][t], s[k][t + h]);
+        }
+        __syncthreads();
+    }
+    if (t < 6) dom[t] = s[t][0];
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t x) {
@@ -152,12 +167,16 @@ __global__ void k_lbvh_leaves(const uint64_t* __restrict__ keys, int n, const fl
 //                    so its arrival counter lives in LDS at i - chunk start); the
 //                    second child to arrive computes the box
 //  k_refit_mark      nodes whose range crosses a chunk boundary are listed with
-//                    the number of their children already final
+//                    the number of their children already final (and their
+//                    place in the list, both in ready[])
 //  k_refit_boundary  one workgroup finishes the listed nodes bottom-up with
-//                    arrival counters (workgroup-scope ordering only)
+//                    arrival counters (workgroup-scope ordering only); up to
+//                    RF_LDS_CAP listed nodes are finished in LDS (boxes,
+//                    counters and parent/child places copied in first)
 // Every walk is bounded; a malformed tree sets *error.
-constexpr int RF_CHUNK = 256;
+constexpr int RF_CHUNK = 1024;
 constexpr int RF_FINISH_THREADS = 1024;
+constexpr int RF_LDS_CAP = 2048;  // listed nodes finished in LDS (48 B each)
 
 __device__ __forceinline__ bool rf_crosses(const int* __restrict__ first, const int* __restrict__ last, int node) {
     return first[node] / RF_CHUNK != last[node] / RF_CHUNK;
@@ -204,7 +223,8 @@ __global__ __launch_bounds__(RF_CHUNK) void k_refit_chunks(int n, const int* __r
     }
 }
 
-// scratch: cnt[n], list[n], ready[n], count[1] (ints)
+// scratch: cnt[n], list[n], ready[n], count[1] (ints). ready[i] of a listed
+// node: (its place in list) << 2 | (children already final).
 __global__ void k_refit_mark(int n, const int* __restrict__ left, const int* __restrict__ right,
                              const int* __restrict__ first, const int* __restrict__ last, int* __restrict__ cnt,
                              int* __restrict__ list, int* __restrict__ ready, int* __restrict__ count) {
@@ -214,8 +234,101 @@ __global__ void k_refit_mark(int n, const int* __restrict__ left, const int* __r
     const int done = (a >= n - 1 || !rf_crosses(first, last, a) ? 1 : 0) +
                      (b >= n - 1 || !rf_crosses(first, last, b) ? 1 : 0);
     cnt[i] = done;
-    ready[i] = done;
-    list[atomicAdd(count, 1)] = i;
+    const int e = atomicAdd(count, 1);
+    ready[i] = (e << 2) | done;
+    list[e] = i;
+}
+
+// Finishes the m <= RF_LDS_CAP listed nodes in LDS. A listed node's entry
+// holds the union of its final (non-listed) children's boxes, its counter
+// (children final), the entry of its parent and those of its listed
+// children; the second child to arrive completes the node.
+__device__ void refit_listed_lds(int n, int m, const int* __restrict__ left, const int* __restrict__ right,
+                                 const int* __restrict__ parent, const int* __restrict__ first,
+                                 const int* __restrict__ last, float4* nlo, float4* nhi,
+                                 const int* __restrict__ list, const int* __restrict__ ready, int* error) {
+    __shared__ float4 blo[RF_LDS_CAP], bhi[RF_LDS_CAP];
+    __shared__ int lcnt[RF_LDS_CAP], lpar[RF_LDS_CAP];
+    __shared__ int2 lch[RF_LDS_CAP];
+    const float inf = __int_as_float(0x7f800000);
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+        const int node = list[e];
+        if (node < 0 || node >= n - 1) {
+            atomicOr(error, 1);
+            lcnt[e] = 0;
+            lpar[e] = -1;
+            continue;
+        }
+        float4 lo = make_float4(inf, inf, inf, 0.0f), hi = make_float4(-inf, -inf, -inf, 0.0f);
+        int ch[2] = {-1, -1};
+        const int kids[2] = {left[node], right[node]};
+        for (int j = 0; j < 2; ++j) {
+            const int k = kids[j];
+            if (k < n - 1 && rf_crosses(first, last, k)) {
+                const int ek = ready[k] >> 2;
+                if (ek < 0 || ek >= m) atomicOr(error, 1);
+                else ch[j] = ek;
+            } else {
+                const float4 a = nlo[k], b = nhi[k];
+                lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.0f);
+                hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), 0.0f);
+            }
+        }
+        blo[e] = lo;
+        bhi[e] = hi;
+        lch[e] = make_int2(ch[0], ch[1]);
+        lcnt[e] = ready[node] & 3;
+        int ep = -1;
+        const int p = node == 0 ? -1 : parent[node];
+        if (p >= 0) {
+            if (p >= n - 1 || !rf_crosses(first, last, p)) atomicOr(error, 1);  // a crossing node's parent crosses
+            else ep = ready[p] >> 2;
+            if (ep >= m) {
+                atomicOr(error, 1);
+                ep = -1;
+            }
+        }
+        lpar[e] = ep;
+    }
+    __syncthreads();
+    for (int e0 = threadIdx.x; e0 < m; e0 += blockDim.x) {
+        if ((ready[list[e0]] & 3) != 2) continue;  // waits for a listed child (read where nothing changes it)
+        int e = e0;
+        for (int guard = 0;; ++guard) {
+            if (guard > m) {
+                atomicOr(error, 1);
+                break;
+            }
+            float4 lo = blo[e], hi = bhi[e];
+            const int2 ch = lch[e];
+            if (ch.x >= 0) {
+                const float4 a = blo[ch.x], b = bhi[ch.x];
+                lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.0f);
+                hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), 0.0f);
+            }
+            if (ch.y >= 0) {
+                const float4 a = blo[ch.y], b = bhi[ch.y];
+                lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.0f);
+                hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), 0.0f);
+            }
+            blo[e] = lo;
+            bhi[e] = hi;
+            const int ep = lpar[e];
+            if (ep < 0) break;
+            __threadfence_block();  // (only LDS accesses are outstanding here)
+            if (atomicAdd(&lcnt[ep], 1) + 1 < 2) break;  // the other child is not final yet
+            __threadfence_block();
+            e = ep;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < m; e += blockDim.x) {
+        const int node = list[e];
+        if (node >= 0 && node < n - 1) {
+            nlo[node] = blo[e];
+            nhi[node] = bhi[e];
+        }
+    }
 }
 
 __global__ __launch_bounds__(RF_FINISH_THREADS) void k_refit_boundary(int n, const int* __restrict__ left,
@@ -228,9 +341,13 @@ __global__ __launch_bounds__(RF_FINISH_THREADS) void k_refit_boundary(int n, con
                                                                        const int* __restrict__ ready,
                                                                        const int* __restrict__ count, int* error) {
     const int m = *count;
+    if (m <= RF_LDS_CAP) {
+        refit_listed_lds(n, m, left, right, parent, first, last, nlo, nhi, list, ready, error);
+        return;
+    }
     for (int e = threadIdx.x; e < m; e += blockDim.x) {
         int node = list[e];
-        if (ready[node] != 2) continue;  // waits for a crossing child
+        if ((ready[node] & 3) != 2) continue;  // waits for a crossing child
         for (int guard = 0;; ++guard) {
             if (guard > n || node < 0 || node >= n - 1) {
                 atomicOr(error, 1);
@@ -333,12 +450,12 @@ using namespace sd;
 extern "C" sd_status sd_global_domain(const float* lo4, const float* hi4, int n, float* domain6, void* scratch,
                                       void* stream) {
     SD_REQUIRE(n >= 1 && lo4 && hi4 && domain6 && scratch, "sd_global_domain: bad argument");
-    const int nb = div_up(n, 256) < 256 ? div_up(n, 256) : 256;
+    const int nb = div_up(n, 256) < DOMAIN_PARTS ? div_up(n, 256) : DOMAIN_PARTS;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_domain_partial, dim3(nb), dim3(256), 0, s, (const float4*)lo4, (const float4*)hi4, n,
                        (float*)scratch);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_domain_final, dim3(1), dim3(64), 0, s, (const float*)scratch, nb, domain6);
+    hipLaunchKernelGGL(k_domain_final, dim3(1), dim3(DOMAIN_PARTS), 0, s, (const float*)scratch, nb, domain6);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

@@ -184,6 +184,28 @@ def test_lbvh_tree_and_pairs(oracle, n, case):
         assert cnt == n * (n - 1) // 2
 
 
+def test_lbvh_many_crossing_nodes(oracle):
+    """1,048,576 random boxes: each of the 4,095 refit chunk boundaries has its
+    own crossing node (the split between its two leaves), more than the 2,048
+    the refit finishes in LDS, so the top of the tree is finished by the
+    global-memory walk; the smaller trees of the other tests finish it in LDS."""
+    say("lbvh many crossing nodes")
+    L.require_gpu()
+    rng = np.random.default_rng(91)
+    n = 1 << 20
+    lo, hi = random_boxes(rng, n, size=0.001)
+    bv = Lbvh(lo, hi)
+    bv.domain()
+    bv.morton()
+    skeys = bv.sort()
+    t = bv.build()
+    check_tree(t, skeys, lo, hi)
+    want = oracle.sap_pairs(lo, hi)[0]
+    code, cnt, pairs = bv.query(capacity=len(want) + 16)
+    assert code == L.SD_OK and cnt == len(want)
+    assert pair_set(pairs) == pair_set(want)
+
+
 def test_lbvh_query_capacity_error(oracle):
     say("lbvh capacity")
     L.require_gpu()
