@@ -90,17 +90,29 @@ struct Shape {
     V3 off;  // centre in the query frame
 };
 
-// Index of the vertex with the largest projection on u (first one on ties).
-__device__ __forceinline__ int support_index(const Shape& s, V3 u) {
+// Index of the vertex with the largest projection on u (first one on ties),
+// and that vertex. Vertices are read SUPPORT_BATCH at a time so that their
+// loads are in flight together; indices past the last vertex read the last
+// vertex again, which never wins (ties keep the first index).
+constexpr int SUPPORT_BATCH = 16;
+__device__ __forceinline__ int support_index(const Shape& s, V3 u, float4& bp) {
     const V3 l = rot_tmul(s.R, u);
     int best = 0;
     float bs = -3.402823466e38f;
-    for (int k = 0; k < s.nv; ++k) {
-        const float4 p = s.verts[k];
-        const float d = p.x * l.x + p.y * l.y + p.z * l.z;
-        if (d > bs) {
-            bs = d;
-            best = k;
+    bp = s.verts[0];
+    const int last = s.nv - 1;
+    for (int k = 0; k < s.nv; k += SUPPORT_BATCH) {
+        float4 p[SUPPORT_BATCH];
+#pragma unroll
+        for (int j = 0; j < SUPPORT_BATCH; ++j) p[j] = s.verts[min(k + j, last)];
+#pragma unroll
+        for (int j = 0; j < SUPPORT_BATCH; ++j) {
+            const float d = p[j].x * l.x + p[j].y * l.y + p[j].z * l.z;
+            if (d > bs) {
+                bs = d;
+                best = k + j;
+                bp = p[j];
+            }
         }
     }
     return best;
@@ -114,8 +126,8 @@ __device__ __forceinline__ V3 support_point(const Shape& s, V3 u, int& idx) {
         const float k = s.radius / sqrtf(uu);
         return add(s.off, scl(u, k));
     }
-    idx = support_index(s, u);
-    const float4 p = s.verts[idx];
+    float4 p;
+    idx = support_index(s, u, p);
     return add(rot_mul(s.R, v3(p.x, p.y, p.z)), s.off);
 }
 
