@@ -422,13 +422,11 @@ def test_step_many_cell_changes(oracle, n, speed):
     sc.close()
 
 
-@pytest.mark.parametrize("key64", [False, True])
-def test_step_sparse_plane_under_dense_plane(oracle, key64):
-    """One sphere per grid row in the bottom plane and a dense plane above:
-    the first wave's 64 lanes lie in 64 different rows, so the candidates of
-    their upper-plane rows (about 1,000 positions) are staged and tested in
-    several chunks. Bit-identical to the oracle."""
-    say("sparse plane under a dense plane", key64)
+def sparse_plane_scene():
+    """One sphere per grid row in the bottom plane (64 rows) and 1,024 spheres in
+    the plane above: the first wave's 64 lanes lie in 64 different rows, so the
+    candidates of their upper-plane rows (about 1,000 positions) are staged and
+    tested in several chunks."""
     rng = np.random.default_rng(53)
     cell, nx, ny = 0.01, 4, 64
     n0, n1 = ny, 1024
@@ -444,11 +442,44 @@ def test_step_sparse_plane_under_dense_plane(oracle, key64):
     g = L.Grid.make((0.0, 0.0, 0.0), cell, (nx, ny, 3))
     c = L.ContactParams.make(1e4, 0.3, 0.5, 1e-5, gravity=(0.0, 0.0, -9.81),
                              walls=((0.0, 0.0, 0.0), (nx * cell, ny * cell, 3 * cell)), max_contacts=32)
-    st = oracle.SphereState(init["x"], init["y"], init["z"], init["r"], init["m"], vx=init["vx"], vy=init["vy"],
-                            vz=init["vz"], max_contacts=32)
+    return init, g, c
+
+
+def oracle_state(oracle, init):
+    return oracle.SphereState(init["x"], init["y"], init["z"], init["r"], init["m"], vx=init["vx"], vy=init["vy"],
+                              vz=init["vz"], max_contacts=32)
+
+
+@pytest.mark.parametrize("key64", [False, True])
+def test_step_sparse_plane_under_dense_plane(oracle, key64):
+    """Stepping with the multi-chunk candidate staging: bit-identical to the oracle."""
+    say("sparse plane under a dense plane", key64)
+    init, g, c = sparse_plane_scene()
+    st = oracle_state(oracle, init)
     sc = make_scene(init, g, c, key64=key64, graph=True)
     st.step(g, c, nsteps=6, nthreads=8)
     sc.step(6)
     compare_scene(sc, st)
     assert sc.contact_count() == int(st.hcount.sum()) > 0
     sc.close()
+
+
+def test_contact_forces_sparse_plane(oracle):
+    """sd_contact_forces (unpadded positions, caller's (start, end) ranges) with
+    the multi-chunk candidate staging: forces, torques and history equal the
+    oracle's."""
+    say("contact_forces sparse plane")
+    init, g, c = sparse_plane_scene()
+    st = oracle_state(oracle, init)
+    st.step(g, c, nsteps=3, nthreads=8)
+    s = sorted_state(oracle, st, g)
+    exp = oracle.contact_forces(s["pos4"], s["vel4"], s["omg4"], s["ids"], g, s["ranges"], c, s["old_slot"],
+                                st.hcount, st.hist, nthreads=8)
+    got = gpu_contact_forces(s, st, g, c)
+    assert exp["hcount"].sum() > 0
+    assert np.array_equal(got["hcount"], exp["hcount"])
+    eh = exp["hist"].reshape(c.max_contacts, -1, 4)
+    assert np.array_equal(valid_hist(got["hist"], got["hcount"]).view(np.uint32),
+                          valid_hist(eh, exp["hcount"]).view(np.uint32))
+    assert np.array_equal(got["force"], exp["force"])
+    assert np.array_equal(got["torque"], exp["torque"])
