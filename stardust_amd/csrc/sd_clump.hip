@@ -128,6 +128,20 @@ __device__ __forceinline__ bool cl_overlap(float4 al, float4 ah, float4 bl, floa
     return al.x <= bh.x && bl.x <= ah.x && al.y <= bh.y && bl.y <= ah.y && al.z <= bh.z && bl.z <= ah.z;
 }
 
+// The tree in the form the partner walk reads: node i is pnode[2i] =
+// (box low corner, left child or a leaf's body as int bits) and pnode[2i+1] =
+// (box high corner, rope as int bits), one 32-byte load step per node.
+__global__ void k_pack_nodes(int nb, const int* __restrict__ left, const int* __restrict__ rope,
+                             const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                             const uint64_t* __restrict__ sorted_keys, float4* __restrict__ pnode) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= 2 * nb - 1) return;
+    const int down = node < nb - 1 ? left[node] : (int)(uint32_t)sorted_keys[node - (nb - 1)];
+    const float4 l = nlo[node], h = nhi[node];
+    pnode[2 * (size_t)node] = make_float4(l.x, l.y, l.z, __int_as_float(down));
+    pnode[2 * (size_t)node + 1] = make_float4(h.x, h.y, h.z, __int_as_float(rope[node]));
+}
+
 // One body per thread: its partners from the LBVH (every body whose AABB
 // overlaps its own), visited in ascending body id, then integration. A pass
 // walks the tree (stackless, ropes) and keeps the CL_PCAP smallest partner ids
@@ -138,9 +152,8 @@ __global__ __launch_bounds__(CL_BLOCK) void k_clump_contacts(
     ClumpState st, ClumpState out, const float4* __restrict__ omega, const float4* __restrict__ i0,
     const float4* __restrict__ i1, const float4* __restrict__ sx, const float4* __restrict__ sv,
     const int* __restrict__ bfirst, const int* __restrict__ bcount, int nb, const float4* __restrict__ blo,
-    const float4* __restrict__ bhi, const uint64_t* __restrict__ sorted_keys, const int* __restrict__ left,
-    const int* __restrict__ rope, const float4* __restrict__ nlo, const float4* __restrict__ nhi, ContactDev c,
-    int* __restrict__ error, float4* __restrict__ force, float4* __restrict__ torque) {
+    const float4* __restrict__ bhi, const float4* __restrict__ pnode, ContactDev c, int* __restrict__ error,
+    float4* __restrict__ force, float4* __restrict__ torque) {
     __shared__ int plist[CL_BLOCK][CL_PCAP + 1];  // (+1: rows start in different banks)
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
@@ -189,12 +202,15 @@ __global__ __launch_bounds__(CL_BLOCK) void k_clump_contacts(
                 atomicOr(error, 4);
                 break;
             }
-            if (!cl_overlap(ql, qh, nlo[node], nhi[node])) {
-                node = rope[node];
+            // one load step per node: box, the next node if the boxes
+            // overlap (left child, or a leaf's body), and the rope
+            const float4 nl = pnode[2 * (size_t)node], nh = pnode[2 * (size_t)node + 1];
+            if (!cl_overlap(ql, qh, nl, nh)) {
+                node = __float_as_int(nh.w);
                 continue;
             }
             if (node >= nb - 1) {  // a leaf: one body
-                const int p = (int)(uint32_t)sorted_keys[node - (nb - 1)];
+                const int p = __float_as_int(nl.w);
                 if (p != b && p > lower && p < nb) {
                     if (np == CL_PCAP) more = true;  // partners beyond this pass's list
                     if (np < CL_PCAP || p < pl[CL_PCAP - 1]) {
@@ -206,9 +222,9 @@ __global__ __launch_bounds__(CL_BLOCK) void k_clump_contacts(
                         pl[k] = p;
                     }
                 }
-                node = rope[node];
+                node = __float_as_int(nh.w);
             } else {
-                node = left[node];
+                node = __float_as_int(nl.w);
             }
         }
         for (int e = 0; e < np; ++e) partner(pl[e]);
@@ -278,6 +294,7 @@ struct sd_clump_ctx {
     size_t sort_tmp_bytes = 0;
     int *left = nullptr, *right = nullptr, *parent = nullptr, *first = nullptr, *last = nullptr, *rope = nullptr;
     float4 *nlo = nullptr, *nhi = nullptr;
+    float4* pnode = nullptr;  // 2 per node: the walk's packed tree
     int *counters = nullptr, *error = nullptr;
     void* q_scratch = nullptr;
     size_t q_scratch_bytes = 0;
@@ -315,7 +332,7 @@ void cfree(sd_clump_ctx* c) {
                     c->slocal,  c->sx,        c->sv,         c->blo,      c->bhi,     c->domain,   c->dom_scratch,
                     c->mkeys,   c->mkeys2,    c->sort_tmp,   c->left,     c->right,   c->parent,   c->first,
                     c->last,    c->rope,      c->nlo,        c->nhi,      c->counters, c->error,   c->q_scratch,
-                    c->pairs,   c->force,     c->torque};
+                    c->pairs,   c->force,     c->torque,     c->pnode};
     drop_clump_graphs(c);
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -376,12 +393,16 @@ sd_status clump_step(sd_clump_ctx* c, hipEvent_t* ev) {
                        c->last, c->rope, (float*)c->nlo, (float*)c->nhi, c->counters, c->error, s);
     if (st != SD_OK) return st;
     c->ksorted = K;
+    hipLaunchKernelGGL(k_pack_nodes, dim3(div_up(2 * nb - 1, 256)), dim3(256), 0, s, nb, (const int*)c->left,
+                       (const int*)c->rope, (const float4*)c->nlo, (const float4*)c->nhi, (const uint64_t*)K,
+                       c->pnode);
+    SD_LAUNCH_CHECK();
     MARK(3);
     MARK(4);  // (partner search is part of the contact kernel)
     MARK(5);
     hipLaunchKernelGGL(k_clump_contacts<true>, dim3(div_up(nb, CL_BLOCK)), dim3(CL_BLOCK), 0, s, cs, nx, c->omega,
-                       c->i0, c->i1, c->sx, c->sv, c->bfirst, c->bcount, nb, c->blo, c->bhi, (const uint64_t*)K,
-                       c->left, c->rope, c->nlo, c->nhi, cd, c->error, nullptr, nullptr);
+                       c->i0, c->i1, c->sx, c->sv, c->bfirst, c->bcount, nb, c->blo, c->bhi,
+                       (const float4*)c->pnode, cd, c->error, nullptr, nullptr);
     SD_LAUNCH_CHECK();
     MARK(6);
 #undef MARK
@@ -491,6 +512,7 @@ extern "C" sd_status sd_clump_upload(sd_clump_ctx* c, int nb, const float* pos4,
     ALLOC(c->rope, 2 * (size_t)nb);
     ALLOC(c->nlo, 2 * (size_t)nb);
     ALLOC(c->nhi, 2 * (size_t)nb);
+    ALLOC(c->pnode, 4 * (size_t)nb);
     ALLOC(c->counters, 3 * (size_t)nb + 8);
     ALLOC(c->error, 1);
     sd_lbvh_query_scratch_bytes(nb, &c->q_scratch_bytes);
