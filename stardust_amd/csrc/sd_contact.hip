@@ -245,15 +245,24 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     constexpr int CHUNK = CONTACT_STAGE - PAD;  // sequence entries tested per chunk
     const char* const pos_bytes = reinterpret_cast<const char*>(a.pos);
 
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int z = cz + dz;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = cy + dy;
-            int lo = 0, hi = 0;
-            if (z >= 0 && z < g.nz && y >= 0 && y < g.ny) {
-                if constexpr (MODE == CONTACT_FORCES) row_range(g, ranges, cx, y, z, lo, hi);
-                else row_window(g, ranges, cx, y, z, lo, hi);
-            }
+    // window of stencil row r (rows (dz, dy) in ascending order); the next
+    // row's window is loaded while a row is copied and tested
+    auto window = [&](int r, int& lo, int& hi) {
+        const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
+        lo = 0;
+        hi = 0;
+        if (z >= 0 && z < g.nz && y >= 0 && y < g.ny) {
+            if constexpr (MODE == CONTACT_FORCES) row_range(g, ranges, cx, y, z, lo, hi);
+            else row_window(g, ranges, cx, y, z, lo, hi);
+        }
+    };
+    int next_lo, next_hi;
+    window(0, next_lo, next_hi);
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+        {
+            const int lo = next_lo, hi = next_hi;
+            if (r + 1 < 9) window(r + 1, next_lo, next_hi);
             const uint64_t ne = __ballot(hi > lo);
             if (ne == 0u) continue;
             const uint64_t ma = ne & ~bmask, mb = ne & bmask;
