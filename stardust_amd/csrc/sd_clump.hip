@@ -107,14 +107,21 @@ __global__ void k_body_aabb(const float4* __restrict__ sx, const int* __restrict
     float lx = 3.402823466e38f, ly = 3.402823466e38f, lz = 3.402823466e38f;
     float hx = -3.402823466e38f, hy = -3.402823466e38f, hz = -3.402823466e38f;
     const int k0 = bfirst[b], kn = bcount[b];
-    for (int k = 0; k < kn; ++k) {
-        const float4 p = sx[k0 + k];
-        lx = fminf(lx, p.x - p.w);
-        ly = fminf(ly, p.y - p.w);
-        lz = fminf(lz, p.z - p.w);
-        hx = fmaxf(hx, p.x + p.w);
-        hy = fmaxf(hy, p.y + p.w);
-        hz = fmaxf(hz, p.z + p.w);
+    // 8 spheres per step with their loads in flight; indices past the last
+    // sphere repeat it (min and max do not change)
+    for (int k = 0; k < kn; k += 8) {
+        float4 p[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) p[j] = sx[k0 + min(k + j, kn - 1)];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            lx = fminf(lx, p[j].x - p[j].w);
+            ly = fminf(ly, p[j].y - p[j].w);
+            lz = fminf(lz, p[j].z - p[j].w);
+            hx = fmaxf(hx, p[j].x + p[j].w);
+            hy = fmaxf(hy, p[j].y + p[j].w);
+            hz = fmaxf(hz, p[j].z + p[j].w);
+        }
     }
     lo4[b] = make_float4(lx, ly, lz, 0.0f);
     hi4[b] = make_float4(hx, hy, hz, 0.0f);
