@@ -38,15 +38,22 @@ __global__ void k_hull_aabb(const float4* __restrict__ pos, const float4* __rest
     } else {
         lx = ly = lz = 3.402823466e38f;
         hx = hy = hz = -3.402823466e38f;
-        for (int k = 0; k < kn; ++k) {
-            const float4 p = verts[k0 + k];
-            const V3 w = rot_mul(R, v3(p.x, p.y, p.z));
-            lx = fminf(lx, w.x);
-            ly = fminf(ly, w.y);
-            lz = fminf(lz, w.z);
-            hx = fmaxf(hx, w.x);
-            hy = fmaxf(hy, w.y);
-            hz = fmaxf(hz, w.z);
+        // 8 vertices per step with their loads in flight; indices past the
+        // last vertex repeat it (min and max do not change)
+        for (int k = 0; k < kn; k += 8) {
+            float4 p[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) p[j] = verts[k0 + min(k + j, kn - 1)];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const V3 w = rot_mul(R, v3(p[j].x, p[j].y, p[j].z));
+                lx = fminf(lx, w.x);
+                ly = fminf(ly, w.y);
+                lz = fminf(lz, w.z);
+                hx = fmaxf(hx, w.x);
+                hy = fmaxf(hy, w.y);
+                hz = fmaxf(hz, w.z);
+            }
         }
         lx = lx + c.x;
         ly = ly + c.y;
