@@ -1,5 +1,7 @@
-// Uniform-grid kernels: home-cell hash keys, cell start/end ranges, state
-// permutation and neighbour-pair generation, plus a device-wide int scan.
+// Uniform-grid kernels: home-cell hash keys, cell start/end ranges (the public
+// form) and cell starts (every cell's first sorted position, the form the
+// stepping contexts keep), state permutation and neighbour-pair generation,
+// plus a device-wide int scan.
 //
 // Replaces constructCollisionListCUDA (src/engine/cuda/sh_collision_detection.cu:27-127),
 // the run detection of tranverseCollisionListCUDA (src/engine/cuda/collision_resolution.cu:100-199)
