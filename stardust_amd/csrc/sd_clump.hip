@@ -130,6 +130,7 @@ __global__ void k_body_aabb(const float4* __restrict__ sx, const int* __restrict
 constexpr int CL_BLOCK = 128;
 constexpr int CL_PCAP = 32;  // partners listed per body and pass
 constexpr int CL_SENTINEL = -1;
+constexpr int CL_MORTON_DROP = 6;  // lowest Morton code bits not sorted on (24 of 30 kept)
 
 __device__ __forceinline__ bool cl_overlap(float4 al, float4 ah, float4 bl, float4 bh) {
     return al.x <= bh.x && bl.x <= ah.x && al.y <= bh.y && bl.y <= ah.y && al.z <= bh.z && bl.z <= ah.z;
@@ -390,10 +391,14 @@ sd_status clump_step(sd_clump_ctx* c, hipEvent_t* ev) {
     MARK(1);
     sd_status st = sd_global_domain((const float*)c->blo, (const float*)c->bhi, nb, c->domain, c->dom_scratch, s);
     if (st != SD_OK) return st;
-    st = sd_morton30((const float*)c->blo, (const float*)c->bhi, nb, c->domain, c->mkeys, s);
+    // Morton codes with their 6 lowest bits cleared (8 bits per axis), sorted
+    // in 3 passes: partner sets are exact and contacts are gathered in
+    // ascending partner id, so the results do not depend on the tree, and a
+    // 256-cell axis is far finer than the bodies' spacing
+    st = launch_morton((const float*)c->blo, (const float*)c->bhi, nb, c->domain, c->mkeys, CL_MORTON_DROP, s);
     if (st != SD_OK) return st;
     uint64_t *K = c->mkeys, *Ka = c->mkeys2;
-    st = radix_sort_keys_u64(&K, &Ka, nb, 32, 62, c->sort_tmp, c->sort_tmp_bytes, s, 0);
+    st = radix_sort_keys_u64(&K, &Ka, nb, 32 + CL_MORTON_DROP, 62, c->sort_tmp, c->sort_tmp_bytes, s, 0);
     if (st != SD_OK) return st;
     MARK(2);
     st = sd_lbvh_build(K, nb, (const float*)c->blo, (const float*)c->bhi, c->left, c->right, c->parent, c->first,

@@ -157,6 +157,9 @@ sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int blo
 // sets *next_count = 0. With *mcount > RESORT_CAP it gives the step up instead
 // (step_ctl[0] = 1).
 constexpr int RESORT_CAP = 1024;
+// keys[i] = (morton30 of box i's centroid with its drop_bits lowest bits cleared) << 32 | i
+sd_status launch_morton(const float* lo4, const float* hi4, int n, const float* domain6, uint64_t* keys,
+                        int drop_bits, hipStream_t s);
 sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint64_t* mold, uint64_t* msorted_new,
                             uint64_t* msorted_old, const uint64_t* keys, int n, int* cell_start, int n_cells,
                             const float4* pos_state, float4* posB, uint64_t* keys_out, int* next_count,

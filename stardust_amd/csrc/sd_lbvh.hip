@@ -88,9 +88,10 @@ __device__ __forceinline__ uint32_t morton_axis(float c, float lo, float hi) {
     return (uint32_t)f;
 }
 
-// keys[i] = morton30(centroid of box i) << 32 | i
+// keys[i] = morton30(centroid of box i) << 32 | i, with the code's drop_bits
+// lowest bits cleared (0 for the public Morton codes)
 __global__ void k_morton(const float4* __restrict__ lo4, const float4* __restrict__ hi4, int n,
-                         const float* __restrict__ dom, uint64_t* __restrict__ keys) {
+                         const float* __restrict__ dom, uint64_t* __restrict__ keys, int drop_bits) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 l = lo4[i], h = hi4[i];
@@ -98,7 +99,7 @@ __global__ void k_morton(const float4* __restrict__ lo4, const float4* __restric
     const uint32_t mx = morton_axis(cx, dom[0], dom[3]);
     const uint32_t my = morton_axis(cy, dom[1], dom[4]);
     const uint32_t mz = morton_axis(cz, dom[2], dom[5]);
-    const uint32_t code = spread10(mx) | (spread10(my) << 1) | (spread10(mz) << 2);
+    const uint32_t code = (spread10(mx) | (spread10(my) << 1) | (spread10(mz) << 2)) >> drop_bits << drop_bits;
     keys[i] = ((uint64_t)code << 32) | (uint64_t)(uint32_t)i;
 }
 
@@ -443,6 +444,15 @@ size_t lbvh_scratch_bytes(int n) {
     return (2 * nn + 8) * sizeof(int) * 2 + 256 * 6 * sizeof(float) + 64 + scan_temp_bytes(n) + 1024;
 }
 
+sd_status launch_morton(const float* lo4, const float* hi4, int n, const float* domain6, uint64_t* keys,
+                        int drop_bits, hipStream_t s) {
+    if (n <= 0) return SD_OK;
+    hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, s, (const float4*)lo4, (const float4*)hi4, n,
+                       domain6, keys, drop_bits);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
 }  // namespace sd
 
 using namespace sd;
@@ -464,10 +474,7 @@ extern "C" sd_status sd_morton30(const float* lo4, const float* hi4, int n, cons
                                  void* stream) {
     SD_REQUIRE(n >= 0 && (n == 0 || (lo4 && hi4 && domain6 && keys)), "sd_morton30: bad argument");
     if (n == 0) return SD_OK;
-    hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, (const float4*)lo4,
-                       (const float4*)hi4, n, domain6, keys);
-    SD_LAUNCH_CHECK();
-    return SD_OK;
+    return sd::launch_morton(lo4, hi4, n, domain6, keys, 0, (hipStream_t)stream);
 }
 
 extern "C" sd_status sd_lbvh_build(const uint64_t* sorted_keys, int n, const float* lo4, const float* hi4, int* left,
