@@ -136,20 +136,6 @@ __device__ __forceinline__ bool cl_overlap(float4 al, float4 ah, float4 bl, floa
     return al.x <= bh.x && bl.x <= ah.x && al.y <= bh.y && bl.y <= ah.y && al.z <= bh.z && bl.z <= ah.z;
 }
 
-// The tree in the form the partner walk reads: node i is pnode[2i] =
-// (box low corner, left child or a leaf's body as int bits) and pnode[2i+1] =
-// (box high corner, rope as int bits), one 32-byte load step per node.
-__global__ void k_pack_nodes(int nb, const int* __restrict__ left, const int* __restrict__ rope,
-                             const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                             const uint64_t* __restrict__ sorted_keys, float4* __restrict__ pnode) {
-    const int node = blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= 2 * nb - 1) return;
-    const int down = node < nb - 1 ? left[node] : (int)(uint32_t)sorted_keys[node - (nb - 1)];
-    const float4 l = nlo[node], h = nhi[node];
-    pnode[2 * (size_t)node] = make_float4(l.x, l.y, l.z, __int_as_float(down));
-    pnode[2 * (size_t)node + 1] = make_float4(h.x, h.y, h.z, __int_as_float(rope[node]));
-}
-
 // One body per thread: its partners from the LBVH (every body whose AABB
 // overlaps its own), visited in ascending body id, then integration. A pass
 // walks the tree (stackless, ropes) and keeps the CL_PCAP smallest partner ids
@@ -401,14 +387,10 @@ sd_status clump_step(sd_clump_ctx* c, hipEvent_t* ev) {
     st = radix_sort_keys_u64(&K, &Ka, nb, 32 + CL_MORTON_DROP, 62, c->sort_tmp, c->sort_tmp_bytes, s, 0);
     if (st != SD_OK) return st;
     MARK(2);
-    st = sd_lbvh_build(K, nb, (const float*)c->blo, (const float*)c->bhi, c->left, c->right, c->parent, c->first,
-                       c->last, c->rope, (float*)c->nlo, (float*)c->nhi, c->counters, c->error, s);
+    st = lbvh_build(K, nb, (const float*)c->blo, (const float*)c->bhi, c->left, c->right, c->parent, c->first,
+                    c->last, c->rope, (float*)c->nlo, (float*)c->nhi, c->counters, c->error, c->pnode, s);
     if (st != SD_OK) return st;
     c->ksorted = K;
-    hipLaunchKernelGGL(k_pack_nodes, dim3(div_up(2 * nb - 1, 256)), dim3(256), 0, s, nb, (const int*)c->left,
-                       (const int*)c->rope, (const float4*)c->nlo, (const float4*)c->nhi, (const uint64_t*)K,
-                       c->pnode);
-    SD_LAUNCH_CHECK();
     MARK(3);
     MARK(4);  // (partner search is part of the contact kernel)
     MARK(5);

@@ -157,6 +157,11 @@ sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int blo
 // sets *next_count = 0. With *mcount > RESORT_CAP it gives the step up instead
 // (step_ctl[0] = 1).
 constexpr int RESORT_CAP = 1024;
+// sd_lbvh_build, and with pnode (2 per node) also the packed tree of a
+// stackless walk: (box low corner, left child or leaf index), (box high corner, rope)
+sd_status lbvh_build(const uint64_t* sorted_keys, int n, const float* lo4, const float* hi4, int* left, int* right,
+                     int* parent, int* first, int* last, int* rope, float* node_lo4, float* node_hi4, int* counters,
+                     int* error, float4* pnode, hipStream_t s);
 // keys[i] = (morton30 of box i's centroid with its drop_bits lowest bits cleared) << 32 | i
 sd_status launch_morton(const float* lo4, const float* hi4, int n, const float* domain6, uint64_t* keys,
                         int drop_bits, hipStream_t s);

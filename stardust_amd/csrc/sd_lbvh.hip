@@ -371,8 +371,13 @@ __global__ __launch_bounds__(RF_FINISH_THREADS) void k_refit_boundary(int n, con
 
 // Rope of a node: the right sibling of its nearest ancestor-or-self that is a
 // left child (the next node in depth-first order after its subtree).
+// With pnode, also the tree in the form a stackless walk reads in one load
+// step per node: pnode[2i] = (box low corner, left child or a leaf's index
+// from its sorted key, as int bits), pnode[2i+1] = (box high corner, rope).
 __global__ void k_lbvh_ropes(int n, const int* __restrict__ left, const int* __restrict__ right,
-                             const int* __restrict__ parent, int* __restrict__ rope, int* __restrict__ error) {
+                             const int* __restrict__ parent, int* __restrict__ rope, int* __restrict__ error,
+                             const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                             const uint64_t* __restrict__ sorted_keys, float4* __restrict__ pnode) {
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= 2 * n - 1) return;
     int c = node;
@@ -390,6 +395,12 @@ __global__ void k_lbvh_ropes(int n, const int* __restrict__ left, const int* __r
         c = p;
     }
     rope[node] = r;
+    if (pnode) {
+        const int down = node < n - 1 ? left[node] : (int)(uint32_t)sorted_keys[node - (n - 1)];
+        const float4 l = nlo[node], h = nhi[node];
+        pnode[2 * (size_t)node] = make_float4(l.x, l.y, l.z, __int_as_float(down));
+        pnode[2 * (size_t)node + 1] = make_float4(h.x, h.y, h.z, __int_as_float(r));
+    }
 }
 
 __device__ __forceinline__ bool box_overlap(float4 al, float4 ah, float4 bl, float4 bh) {
@@ -477,14 +488,13 @@ extern "C" sd_status sd_morton30(const float* lo4, const float* hi4, int n, cons
     return sd::launch_morton(lo4, hi4, n, domain6, keys, 0, (hipStream_t)stream);
 }
 
-extern "C" sd_status sd_lbvh_build(const uint64_t* sorted_keys, int n, const float* lo4, const float* hi4, int* left,
-                                   int* right, int* parent, int* first, int* last, int* rope, float* node_lo4,
-                                   float* node_hi4, int* counters, int* error, void* stream) {
+sd_status sd::lbvh_build(const uint64_t* sorted_keys, int n, const float* lo4, const float* hi4, int* left, int* right,
+                         int* parent, int* first, int* last, int* rope, float* node_lo4, float* node_hi4, int* counters,
+                         int* error, float4* pnode, hipStream_t s) {
     SD_REQUIRE(n >= 1, "sd_lbvh_build: n must be >= 1");
     SD_REQUIRE(sorted_keys && lo4 && hi4 && left && right && parent && first && last && rope && node_lo4 && node_hi4 &&
                    counters && error,
                "sd_lbvh_build: null pointer");
-    hipStream_t s = (hipStream_t)stream;
     if (n > 1) {
         hipLaunchKernelGGL(k_lbvh_build, dim3(div_up(n - 1, 256)), dim3(256), 0, s, sorted_keys, n, left, right,
                            parent, first, last);
@@ -510,9 +520,16 @@ extern "C" sd_status sd_lbvh_build(const uint64_t* sorted_keys, int n, const flo
         SD_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_lbvh_ropes, dim3(div_up(2 * n - 1, 256)), dim3(256), 0, s, n, left, right, parent, rope,
-                       error);
+                       error, (const float4*)node_lo4, (const float4*)node_hi4, sorted_keys, pnode);
     SD_LAUNCH_CHECK();
     return SD_OK;
+}
+
+extern "C" sd_status sd_lbvh_build(const uint64_t* sorted_keys, int n, const float* lo4, const float* hi4, int* left,
+                                   int* right, int* parent, int* first, int* last, int* rope, float* node_lo4,
+                                   float* node_hi4, int* counters, int* error, void* stream) {
+    return lbvh_build(sorted_keys, n, lo4, hi4, left, right, parent, first, last, rope, node_lo4, node_hi4, counters,
+                      error, nullptr, (hipStream_t)stream);
 }
 
 extern "C" sd_status sd_lbvh_query_scratch_bytes(int n, size_t* bytes) {
