@@ -116,7 +116,7 @@ __device__ __forceinline__ int xcd_logical_block(int nblocks) {
 // MODE: CONTACT_FORCES writes force/torque; CONTACT_FUSED_* integrates and writes
 // the next state and keys. INDIRECT: vel/omg/ids are in the previous step's order
 // and are read through old_slot (positions are always in sorted order).
-// CONTACT_FUSED_BATCH: the grid, contact law and cell ranges come from the
+// CONTACT_FUSED_BATCH: the grid, contact law and cell starts come from the
 // sphere's scene; the arithmetic is the same code, so a scene stepped in a
 // batch is bit-identical to the same scene stepped alone.
 template <int MODE, bool INDIRECT>

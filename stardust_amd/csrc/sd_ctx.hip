@@ -175,7 +175,7 @@ sd_status dalloc(T** p, size_t count) {
 }
 
 // Enqueues one step on the context stream. If ev is given, ev[0..4] are
-// recorded before the sort and after the sort, ranges, permute and contact stages.
+// recorded before the sort and after the sort, cell-start, permute and contact stages.
 // full (u64 keys): the radix sort instead of the re-sort from the previous
 // order. A re-sort step that finds more than RESORT_CAP changed cells is given
 // up on the device (step_ctl[0]), together with every later step of the same
