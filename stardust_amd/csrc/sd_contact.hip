@@ -196,8 +196,8 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     // the contacts among them in ascending position order. Only the distance
     // is tested per candidate; a batch with a possible contact in any lane
     // (rare, and always the batch holding the sphere itself) then drops
-    // positions past hi, and visit() drops the sphere itself and coincident
-    // spheres (d2 == 0).
+    // positions past hi and the sphere itself (position s, without a memory
+    // access), and visit() drops coincident spheres (d2 == 0).
     auto test_batch = [&](const float4 (&q)[CAND_UNROLL], int t0, int hi) {
         bool h[CAND_UNROLL];
         uint64_t any = 0u;  // lanes with a possible contact (scalar masks)
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
             while (hit) {
                 const int k = __builtin_ctz(hit);
                 hit &= hit - 1u;
-                if (t0 + k < hi) visit(t0 + k);
+                if (t0 + k < hi && t0 + k != s) visit(t0 + k);
             }
         }
     };
