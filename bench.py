@@ -14,7 +14,7 @@ dominant kernel (its time measured with HIP events on the scene stream in an
 eager profiling pass right after the timed region) and a CPU baseline (the
 oracle's full step on the host cores, bounded sample of the same workload).
 
-Further sections ride on the same line (--sections, default "batch,gjk,clumps"):
+Further sections ride on the same line (--sections, default "batch,gjk,batch_gjk,clumps"):
   batch  — batch_c5 (512 independent scenes, 7.7M spheres): particle-steps/s of
            the batched graph-replayed step (after the autotuner picks the launch
            shapes) vs the plain form (every scene alone in its own context).
