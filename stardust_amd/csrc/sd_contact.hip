@@ -119,24 +119,56 @@ __device__ __forceinline__ int xcd_logical_block(int nblocks) {
 // CONTACT_FUSED_BATCH: the grid, contact law and cell starts come from the
 // sphere's scene; the arithmetic is the same code, so a scene stepped in a
 // batch is bit-identical to the same scene stepped alone.
-template <int MODE, bool INDIRECT>
+// a wave-uniform struct in global memory, word by word into scalar registers
+// (the compiler does not use scalar loads for memory the kernel also writes)
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T& src) {
+    static_assert(sizeof(T) % 4 == 0, "uniform_copy: whole 32-bit words");
+    T dst;
+    const int* const w = reinterpret_cast<const int*>(&src);
+    int* const d = reinterpret_cast<int*>(&dst);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    return dst;
+}
+
+// STRADDLE (batch mode only): one 64-thread block per listed wave whose slots
+// span more than one scene; every lane then takes its own scene's parameters.
+template <int MODE, bool INDIRECT, bool STRADDLE = false>
 __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     constexpr bool BATCH = MODE == CONTACT_FUSED_BATCH;
     constexpr bool PADDED = MODE != CONTACT_FORCES;  // fused modes read a context's padded buffer
+    static_assert(!STRADDLE || BATCH, "k_contacts: straddling waves exist in batch mode only");
     if (a.step_ctl) {
         if (a.step_ctl[0]) return;  // the step was given up
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.step_ctl[1] += 1;
+        if (!STRADDLE && blockIdx.x == 0 && threadIdx.x == 0) a.step_ctl[1] += 1;
     }
-    const int lb = xcd_logical_block((a.n + (int)blockDim.x - 1) / (int)blockDim.x);
-    if (lb < 0) return;
-    const int s = lb * (int)blockDim.x + (int)threadIdx.x;
+    int s;
+    if constexpr (STRADDLE) {
+        s = a.straddle[blockIdx.x] * WAVE + (int)threadIdx.x;
+    } else {
+        const int lb = xcd_logical_block((a.n + (int)blockDim.x - 1) / (int)blockDim.x);
+        if (lb < 0) return;
+        s = lb * (int)blockDim.x + (int)threadIdx.x;
+    }
     if (s >= a.n) return;
     GridDev g;
     ContactDev c;
     const int* ranges;  // CONTACT_FORCES: (start, end) per cell; fused modes: cell starts
     uint32_t cbase = 0u;
-    if constexpr (BATCH) {
+    if constexpr (STRADDLE) {
         const SceneDev& S = a.scenes[a.scene_of[s]];
+        g = S.g;
+        c = S.c;
+        cbase = S.cbase;
+        ranges = a.cell_start + cbase;
+    } else if constexpr (BATCH) {
+        // one scene in the wave (the waves spanning two are in a.straddle and
+        // run by the straddle launch): its parameters are wave-uniform
+        const int my_scene = a.scene_of[s];
+        const int sid = __builtin_amdgcn_readfirstlane(my_scene);
+        if (__ballot(my_scene != sid) != 0u) return;
+        const SceneDev S = uniform_copy(a.scenes[sid]);
         g = S.g;
         c = S.c;
         cbase = S.cbase;
@@ -415,6 +447,15 @@ sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int blo
         hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U64, true>), grid, dim3(block), lds, s, a);
     else
         hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_BATCH, true>), grid, dim3(block), lds, s, a);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+sd_status launch_contacts_straddle(const ContactArgs& a, hipStream_t s) {
+    if (a.n <= 0 || a.n_straddle <= 0) return SD_OK;
+    SD_REQUIRE(a.straddle && a.scenes && a.scene_of, "contact kernel: straddle launch without its lists");
+    const size_t lds = (size_t)CONTACT_STAGE * sizeof(float4);
+    hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_BATCH, true, true>), dim3(a.n_straddle), dim3(WAVE), lds, s, a);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

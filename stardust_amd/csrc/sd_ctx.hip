@@ -19,6 +19,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <vector>
 
@@ -126,6 +127,10 @@ struct sd_ctx {
     std::vector<SceneDev> scenes_h;
     SceneDev* scenes_d = nullptr;
     int* scene_of = nullptr;
+    // waves (64 consecutive slots from a multiple of 64) spanning two or more
+    // scenes, run by a launch of their own
+    int* straddle = nullptr;
+    int n_straddle = 0;
 };
 
 namespace {
@@ -142,7 +147,8 @@ void release(sd_ctx* c) {
     void* bufs[] = {c->pos[0], c->pos[1], c->vel[0], c->vel[1], c->omg[0], c->omg[1], c->ids[0], c->ids[1],
                     c->posB, c->k32[0], c->k32[1], c->v32[0], c->v32[1], c->k64[0], c->k64[1], c->sort_tmp,
                     c->cell_start, c->hcount[0], c->hcount[1], c->hist[0], c->hist[1], c->overflow, c->diag,
-                    c->scenes_d, c->scene_of, c->mcount, c->mnew, c->mold, c->msn, c->mso, c->step_ctl};
+                    c->scenes_d, c->scene_of, c->straddle, c->mcount, c->mnew, c->mold, c->msn, c->mso,
+                    c->step_ctl};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < 2; ++k) {
@@ -163,6 +169,8 @@ void release(sd_ctx* c) {
     c->diag = nullptr;
     c->scenes_d = nullptr;
     c->scene_of = nullptr;
+    c->straddle = nullptr;
+    c->n_straddle = 0;
     c->scenes_h.clear();
     c->nscenes = 0;
     c->n = 0;
@@ -245,7 +253,10 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
         a.step_ctl = c->step_ctl;
         a.scenes = c->scenes_d;
         a.scene_of = c->scene_of;
+        a.straddle = c->straddle;
+        a.n_straddle = c->n_straddle;
         st = launch_contacts(c->batch ? CONTACT_FUSED_BATCH : CONTACT_FUSED_U64, a, s, c->contact_block);
+        if (st == SD_OK && c->batch) st = launch_contacts_straddle(a, s);
         if (st != SD_OK) return st;
         MARK(4);
         c->cur_keys = (Ka == c->k64[0]) ? 0 : 1;
@@ -616,6 +627,23 @@ extern "C" sd_status sd_batch_upload(sd_ctx* c, int nscenes, const sd_scene_desc
     std::vector<int> sof((size_t)n);
     for (int k = 0; k < nscenes; ++k)
         for (int i = 0; i < table[k].n; ++i) sof[(size_t)table[k].first + i] = k;
+    // scenes are consecutive, so a wave spans two scenes exactly when its first
+    // and last slot differ in scene
+    std::vector<int> straddle;
+    for (long long w = 0; w * WAVE < n; ++w) {
+        const long long last = std::min(w * WAVE + WAVE - 1, n - 1);
+        if (sof[(size_t)(w * WAVE)] != sof[(size_t)last]) straddle.push_back((int)w);
+    }
+    if (!straddle.empty()) {
+        st = dalloc(&c->straddle, straddle.size());
+        if (st != SD_OK) {
+            release(c);
+            return st;
+        }
+        SD_HIP(hipMemcpyAsync(c->straddle, straddle.data(), straddle.size() * sizeof(int), hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    c->n_straddle = (int)straddle.size();
     SD_HIP(hipMemcpyAsync(c->scenes_d, table.data(), nscenes * sizeof(SceneDev), hipMemcpyHostToDevice, c->stream));
     SD_HIP(hipMemcpyAsync(c->scene_of, sof.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
     st = launch_hash_keys_batch(c->pos[0], (int)n, c->scenes_d, c->scene_of, c->k64[0], c->stream);

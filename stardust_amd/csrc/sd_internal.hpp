@@ -147,8 +147,15 @@ struct ContactArgs {
     // CONTACT_FUSED_BATCH: scene table and scene of every slot
     const SceneDev* scenes;
     const int* scene_of;
+    // CONTACT_FUSED_BATCH: the waves (64 consecutive slots from a multiple of
+    // 64) whose slots span more than one scene, n_straddle of them. The main
+    // launch leaves them out (its waves take their scene's parameters in scalar
+    // registers); launch_contacts_straddle runs them with per-lane parameters.
+    const int* straddle;
+    int n_straddle;
 };
 sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int block);
+sd_status launch_contacts_straddle(const ContactArgs& a, hipStream_t s);
 
 // Sorted order of the next step's u64 keys from the previous sorted order and
 // the spheres that changed cell (see sd_resort.hip): writes the sorted keys to
