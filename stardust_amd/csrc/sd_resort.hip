@@ -98,6 +98,15 @@ __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
     if (step_ctl[0]) return;
     const int M = *mcount;
     if (M > RESORT_CAP) return;
+    // this thread's key and position, loaded first: their latency overlaps the
+    // movers' staging and the searches below
+    const int s = blockIdx.x * RESORT_MERGE_BLOCK + (int)threadIdx.x;
+    uint64_t y = 0u;
+    float4 ps = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (s < n) {
+        y = keys[s];
+        ps = pos_state[s];
+    }
     __shared__ uint64_t sn[RESORT_CAP], so[RESORT_CAP];
     for (int j = threadIdx.x; j < M; j += RESORT_MERGE_BLOCK) {
         sn[j] = msn[j];
@@ -131,7 +140,6 @@ __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
             blo = bhi;
         }
     }
-    const int s = blockIdx.x * RESORT_MERGE_BLOCK + (int)threadIdx.x;
     if (s >= n) return;
     // movers before s: old keys ascend with their positions (the low halves)
     int lo = 0, hi = M;
@@ -141,12 +149,11 @@ __global__ __launch_bounds__(RESORT_MERGE_BLOCK) void k_resort_merge(
         else hi = mid;
     }
     if (lo < M && (uint32_t)so[lo] == (uint32_t)s) return;  // a mover, placed by k_resort_movers
-    const uint64_t y = keys[s];
     const int a = lower_bound_u64(sn, 0, M, y);
     const int d = s - lo + a;
     if (d < 0 || d >= n) return;  // (never, for consistent keys)
     keys_out[d] = y;
-    posB[d] = pos_state[s];
+    posB[d] = ps;
 }
 
 sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint64_t* mold, uint64_t* msorted_new,
