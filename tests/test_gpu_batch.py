@@ -72,6 +72,40 @@ def test_batch_equals_each_scene_alone():
     b.close()
 
 
+def test_batch_wave_boundaries():
+    """Scene sizes that put some scene boundaries exactly on wave boundaries
+    (multiples of 64 slots) and others inside a wave: single-scene waves take
+    their scene's parameters as wave-uniform values, waves spanning two scenes
+    run in a launch of their own; every scene is bit-identical to it alone."""
+    sizes = [64, 128, 200, 56, 64, 192, 65, 63]  # boundaries 64 128+64 392 448 512 704 769 832
+    say("batch vs solo, scene sizes", sizes)
+    w = small_batch(nscenes=len(sizes), seed=0x5EEDB0A1)
+    s_full, scenes_full = w.generate()
+    keep, scenes, off = [], [], 0
+    for (n, g, c), m in zip(scenes_full, sizes):
+        assert n >= m
+        keep.append(np.arange(off, off + m))
+        scenes.append((m, g, c))
+        off += n
+    idx = np.concatenate(keep)
+    s = {k: np.ascontiguousarray(s_full[k][idx]) for k in ("x", "y", "z", "r", "m")}
+    v = velocities(len(idx), 3)
+    b = batch_of(w, scenes, s, v)
+    steps = 120
+    b.step(steps)
+    got = b.download()
+    off = 0
+    for k, sc_desc in enumerate(scenes):
+        n = sc_desc[0]
+        sc = solo(sc_desc, s, v, off, off + n)
+        sc.step(steps)
+        ref = sc.download()
+        assert_same({f: got[f][off:off + n] for f in FIELDS}, ref, "scene %d" % k)
+        sc.close()
+        off += n
+    b.close()
+
+
 def test_batch_scene_equals_oracle(oracle):
     say("batch vs oracle, 3 scenes")
     w = small_batch(nscenes=3, seed=0x5EEDB002)
