@@ -445,6 +445,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes of the C2 context")
     ap.add_argument("--sections", default="batch,gjk,batch_gjk,clumps",
                     help="extra sections: batch, gjk, batch_gjk, clumps (comma list, or none)")
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -482,6 +483,11 @@ def main():
     g, c = w.grid(), w.contact()
     sc = SphereScene(g, c, key64=w.key64, use_graph=True)
     sc.upload(s["x"], s["y"], s["z"], s["r"], s["m"])
+    tune = None
+    if not args.no_autotune:
+        # launch shapes only (sort tile, contact block); the state is unchanged
+        tune = sc.autotune(trial_steps=8)
+        log("autotuned", tune)
     log("uploaded", w.n, "spheres; warmup", args.warmup)
     sc.step(args.warmup)
     stream = sc.stream()
@@ -543,7 +549,8 @@ def main():
         "data": "synthetic (seeded SplitMix64 scene, generated on the host)",
         "config": {"workload": w.name, "spheres": w.n, "steps_per_run": w.steps, "dt": w.dt,
                    "keys": "u64 (cell<<32|index)" if w.key64 else "u32 cell + u32 index",
-                   "cells": g.n_cells, "form": "plain single-scene, graph-replayed step"},
+                   "cells": g.n_cells, "form": "plain single-scene, graph-replayed step",
+                   "launch_shapes": tune if tune else "defaults (sort tile by key count, contact_block 128)"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((w.name, dominant)), "algorithmic_bytes": sbytes[dominant]},
