@@ -131,6 +131,11 @@ struct sd_ctx {
     // scenes, run by a launch of their own
     int* straddle = nullptr;
     int n_straddle = 0;
+    // the straddling waves' launch runs on a side stream, forked from and
+    // joined back into the context's stream around the main contact launch
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join = nullptr;
 };
 
 namespace {
@@ -255,9 +260,19 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
         a.scene_of = c->scene_of;
         a.straddle = c->straddle;
         a.n_straddle = c->n_straddle;
+        // waves spanning two scenes: their own launch, beside the main one
+        // (disjoint spheres; shared counters are atomics)
+        const bool fork = c->batch && c->n_straddle > 0;
+        if (fork) {
+            SD_HIP(hipEventRecord(c->ev_fork, s));
+            SD_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            st = launch_contacts_straddle(a, c->side);
+            if (st != SD_OK) return st;
+            SD_HIP(hipEventRecord(c->ev_join, c->side));
+        }
         st = launch_contacts(c->batch ? CONTACT_FUSED_BATCH : CONTACT_FUSED_U64, a, s, c->contact_block);
-        if (st == SD_OK && c->batch) st = launch_contacts_straddle(a, s);
         if (st != SD_OK) return st;
+        if (fork) SD_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
         MARK(4);
         c->cur_keys = (Ka == c->k64[0]) ? 0 : 1;
     } else {
@@ -411,6 +426,10 @@ extern "C" sd_status sd_ctx_destroy(sd_ctx* c) {
     if (!c) return SD_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     release(c);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SD_OK;
@@ -635,6 +654,11 @@ extern "C" sd_status sd_batch_upload(sd_ctx* c, int nscenes, const sd_scene_desc
         if (sof[(size_t)(w * WAVE)] != sof[(size_t)last]) straddle.push_back((int)w);
     }
     if (!straddle.empty()) {
+        if (!c->side) {
+            SD_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+            SD_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+            SD_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        }
         st = dalloc(&c->straddle, straddle.size());
         if (st != SD_OK) {
             release(c);
