@@ -315,8 +315,11 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
                 const int m = min(CONTACT_STAGE, seq - e0);
                 for (int i = arank; i < m; i += 2 * nact) {
                     const int e1 = e0 + i, e2 = e1 + nact;
-                    const int s1 = min((e1 < offb ? la : lb_b) + e1, a.n - 1);
-                    const int s2 = min((e2 < offb ? la : lb_b) + e2, a.n - 1);
+                    // sources reach at most n + 2 (3 padding entries past a
+                    // window that ends at n): inside a context's padded buffer,
+                    // clamped for a caller's unpadded one
+                    const int s1 = PADDED ? (e1 < offb ? la : lb_b) + e1 : min((e1 < offb ? la : lb_b) + e1, a.n - 1);
+                    const int s2 = PADDED ? (e2 < offb ? la : lb_b) + e2 : min((e2 < offb ? la : lb_b) + e2, a.n - 1);
                     const float4 v1 = *reinterpret_cast<const float4*>(pos_bytes + (size_t)((uint32_t)s1 * 16u));
                     if (i + nact < m) {
                         const float4 v2 = *reinterpret_cast<const float4*>(pos_bytes + (size_t)((uint32_t)s2 * 16u));
