@@ -442,19 +442,49 @@ constexpr int INF_ROWS = 16;  // samples per wave per tile
 //   A: row l % 16, k = 4 (l / 16) .. + 3      B: k = 4 (l / 16) .. + 3, column l % 16
 //   C/D: rows 4 (l / 16) .. + 3, column l % 16
 // One wave pushes 16 samples through the whole network; activations go through
-// LDS (as BF16) to turn the D layout into the next layer's A layout.
+// LDS (as BF16) to turn the D layout into the next layer's A layout. The block
+// keeps the weights in LDS transposed, [n][k] with rows padded by 4, so that a
+// lane's four k values of A and of B are each one 8-byte LDS read.
+constexpr int WT_K1 = K1 + 4, WT_NH = NH + 4;  // transposed row lengths (bf16)
+constexpr int WT_1 = 0, WT_2 = WT_1 + NH * WT_K1, WT_3 = WT_2 + NH * WT_NH, WT_4 = WT_3 + NH * WT_NH,
+              WT_END = WT_4 + N4 * WT_NH;
+static_assert(PK_END % 8 == 0 && (3 * NH + N4) % 4 == 0, "weights are staged in 16-byte pieces");
+
+// SiLU of the inference path: hardware exp and reciprocal (a few ulp of FP32,
+// far below the BF16 rounding of the activations it feeds)
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
 __global__ __launch_bounds__(INF_WAVES * 64) void k_mlp_infer(const float* __restrict__ X, int n,
                                                               const unsigned short* __restrict__ wb_g,
                                                               const float* __restrict__ bias_g,
                                                               const int* __restrict__ pairs,
                                                               const float4* __restrict__ quat,
                                                               float* __restrict__ out_dir) {
-    __shared__ unsigned short wb[PK_END];
-    __shared__ float bias[3 * NH + N4];
-    __shared__ unsigned short act[INF_WAVES][INF_ROWS][NH + 8];
+    __shared__ __attribute__((aligned(16))) unsigned short wt[WT_END];
+    __shared__ __attribute__((aligned(16))) float bias[3 * NH + N4];
+    __shared__ __attribute__((aligned(16))) unsigned short act[INF_WAVES][INF_ROWS][NH + 8];
     __shared__ float yout[INF_WAVES][INF_ROWS][N4];
-    for (int e = threadIdx.x; e < PK_END; e += blockDim.x) wb[e] = wb_g[e];
-    for (int e = threadIdx.x; e < 3 * NH + N4; e += blockDim.x) bias[e] = bias_g[e];
+    // packed [K][N] rows of 8 bf16 (one 16-byte load) -> transposed rows
+    for (int c = threadIdx.x; c < PK_END / 8; c += blockDim.x) {
+        const uint4 v = reinterpret_cast<const uint4*>(wb_g)[c];
+        const int e = 8 * c;
+        int base, row, k, n0;
+        if (e < PK_W2) {
+            base = WT_1, row = WT_K1, k = e / NH, n0 = e % NH;
+        } else if (e < PK_W3) {
+            base = WT_2, row = WT_NH, k = (e - PK_W2) / NH, n0 = (e - PK_W2) % NH;
+        } else if (e < PK_W4) {
+            base = WT_3, row = WT_NH, k = (e - PK_W3) / NH, n0 = (e - PK_W3) % NH;
+        } else {
+            base = WT_4, row = WT_NH, k = (e - PK_W4) / N4, n0 = (e - PK_W4) % N4;
+        }
+        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            wt[base + (n0 + j) * row + k] = (unsigned short)(w4[j >> 1] >> (16 * (j & 1)));
+    }
+    for (int c = threadIdx.x; c < (3 * NH + N4) / 4; c += blockDim.x)
+        reinterpret_cast<float4*>(bias)[c] = reinterpret_cast<const float4*>(bias_g)[c];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, q4 = lane >> 4;
@@ -472,16 +502,14 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_mlp_infer(const float* __res
         // three hidden layers
         for (int layer = 0; layer < 3; ++layer) {
             const int K = layer == 0 ? K1 : NH;
-            const unsigned short* W = wb + (layer == 0 ? PK_W1 : (layer == 1 ? PK_W2 : PK_W3));
+            const int row = layer == 0 ? WT_K1 : WT_NH;
+            const unsigned short* W = wt + (layer == 0 ? WT_1 : (layer == 1 ? WT_2 : WT_3));
             f32x4 accs[NH / 16];
             for (int nt = 0; nt < NH / 16; ++nt) {
                 f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
                 for (int k0 = 0; k0 < K; k0 += 16) {
-                    bf16x4 a, b;
-                    for (int t = 0; t < 4; ++t) {
-                        a[t] = (short)act[wave][r16][k0 + 4 * q4 + t];
-                        b[t] = (short)W[(k0 + 4 * q4 + t) * NH + nt * 16 + r16];
-                    }
+                    const bf16x4 a = *reinterpret_cast<const bf16x4*>(&act[wave][r16][k0 + 4 * q4]);
+                    const bf16x4 b = *reinterpret_cast<const bf16x4*>(W + (nt * 16 + r16) * row + k0 + 4 * q4);
                     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
                 }
                 accs[nt] = acc;
@@ -491,8 +519,7 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_mlp_infer(const float* __res
             for (int nt = 0; nt < NH / 16; ++nt) {
                 for (int t = 0; t < 4; ++t) {
                     const int r = 4 * q4 + t, c = nt * 16 + r16;
-                    const float z = accs[nt][t] + bias[layer * NH + c];
-                    act[wave][r][c] = to_bf16(z * sigm(z));
+                    act[wave][r][c] = to_bf16(silu_fast(accs[nt][t] + bias[layer * NH + c]));
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -501,11 +528,8 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_mlp_infer(const float* __res
         // output layer: N padded to 16
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int k0 = 0; k0 < NH; k0 += 16) {
-            bf16x4 a, b;
-            for (int t = 0; t < 4; ++t) {
-                a[t] = (short)act[wave][r16][k0 + 4 * q4 + t];
-                b[t] = (short)wb[PK_W4 + (k0 + 4 * q4 + t) * N4 + r16];
-            }
+            const bf16x4 a = *reinterpret_cast<const bf16x4*>(&act[wave][r16][k0 + 4 * q4]);
+            const bf16x4 b = *reinterpret_cast<const bf16x4*>(wt + WT_4 + r16 * WT_NH + k0 + 4 * q4);
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
         }
         for (int t = 0; t < 4; ++t) yout[wave][4 * q4 + t][r16] = acc[t] + bias[3 * NH + r16];
