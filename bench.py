@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # runs on MI355X (FETCH x 2.0 + WRITE x 1.0 KiB, factors from the plain copy kernel);
 # the runs and the per-kernel table are in profiles/bench_hbm_counters.md.
 MEASURED_TRAFFIC = {
-    ("dem_c2_column_4194304", "contact"): (250593.2 * 2.0 + 262144.0) * 1024.0,
+    ("dem_c2_column_4194304", "contact"): (250667.7 * 2.0 + 262144.0) * 1024.0,
 }
 
 
