@@ -90,6 +90,19 @@ sd_status sd_radix_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys
                                   void* stream);
 sd_status sd_radix_sort_keys_u64(uint64_t* keys, uint64_t* keys_alt, int n, int begin_bit, int end_bit,
                                  void* temp, size_t temp_bytes, void* stream);
+/* As sd_radix_sort_pairs_u32, with no temp argument and no allocation: the
+ * histogram workspace is a static device array of each GPU (about 8.4 MB,
+ * held until the process ends), so n may not exceed SD_SORT_NOALLOC_MAX_N
+ * (SD_ERR_ARG otherwise). No hipMalloc, hipFree or allocating library call is
+ * made. Calls that overlap on one GPU (on different streams or host threads)
+ * share that workspace and race; serialise them. The workspace needs no
+ * clearing (every entry is written before it is read).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). */
+#define SD_SORT_NOALLOC_MAX_N 8388608
+sd_status sd_radix_sort_pairs_u32_noalloc(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                                          int n, int begin_bit, int end_bit, void* stream);
 
 /* --------------------------------------------------- cell ranges / pairs
  * ranges[2*c+0] = first sorted position whose cell is c, ranges[2*c+1] = one

@@ -107,6 +107,7 @@ _SIGS = {
     "sd_radix_sort_temp_bytes": [_I, C.POINTER(_SZ)],
     "sd_radix_sort_pairs_u32": [_P, _P, _P, _P, _I, _I, _I, _P, _SZ, _P],
     "sd_radix_sort_keys_u64": [_P, _P, _I, _I, _I, _P, _SZ, _P],
+    "sd_radix_sort_pairs_u32_noalloc": [_P, _P, _P, _P, _I, _I, _I, _P],
     "sd_cell_ranges_u32": [_P, _I, _I, _P, _P],
     "sd_cell_ranges_u64": [_P, _I, _I, _P, _P],
     "sd_neighbor_pairs": [_P, _I, C.POINTER(Grid), _P, _F, _P, _I, C.POINTER(C.c_int), _P, _SZ, _P],

@@ -15,6 +15,8 @@
 //                   consecutive addresses of each digit bucket.
 // Digit width per pass is ceil(bits / passes) with passes = ceil(bits / 8),
 // so a 17-bit key costs 3 passes of 6/6/5 bits.
+#include <mutex>
+
 #include "sd_internal.hpp"
 
 namespace sd {
@@ -319,4 +321,60 @@ extern "C" sd_status sd_radix_sort_keys_u64(uint64_t* keys, uint64_t* keys_alt, 
     if (st != SD_OK) return st;
     if (k != keys) SD_HIP(hipMemcpyAsync(keys, k, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
     return SD_OK;
+}
+
+// ------------------------------------------------------------------ no-allocation form
+//
+// The histogram workspace of sd_radix_sort_pairs_u32_noalloc is a file-scope
+// device array sized for SD_SORT_NOALLOC_MAX_N elements. Every GPU the library
+// runs on holds its own copy until the process ends. Host code never passes
+// the array's name to a kernel (in host code it is only a stand-in): its
+// device address is looked up once per device with hipGetSymbolAddress and
+// kept here.
+namespace sd {
+constexpr size_t SORT_NOALLOC_BYTES =
+    ((size_t)RS_RADIX * (size_t)((SD_SORT_NOALLOC_MAX_N + RS_BLOCK * 4 - 1) / (RS_BLOCK * 4)) + RS_RADIX) *
+        sizeof(uint32_t) +
+    256;
+static __device__ __align__(256) unsigned char scratch[SORT_NOALLOC_BYTES];
+
+constexpr int SORT_NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[SORT_NOALLOC_MAX_DEVICES];
+
+// Device address of this device's workspace (looked up on the first call per
+// device, with that device current).
+static sd_status noalloc_scratch(void** out) {
+    int dev = 0;
+    SD_HIP(hipGetDevice(&dev));
+    SD_REQUIRE(dev >= 0 && dev < SORT_NOALLOC_MAX_DEVICES, "sort workspace: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        SD_REQUIRE(p != nullptr, "sort workspace: no device address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SD_OK;
+}
+}  // namespace sd
+
+extern "C" sd_status sd_radix_sort_pairs_u32_noalloc(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                                                     uint32_t* vals_alt, int n, int begin_bit, int end_bit,
+                                                     void* stream) {
+    SD_REQUIRE(n >= 0, "sd_radix_sort_pairs_u32_noalloc: n < 0");
+    SD_REQUIRE(n <= SD_SORT_NOALLOC_MAX_N,
+               "sd_radix_sort_pairs_u32_noalloc: n = %d exceeds the static workspace limit %d "
+               "(use sd_radix_sort_pairs_u32 with a caller-provided temp buffer)",
+               n, SD_SORT_NOALLOC_MAX_N);
+    static_assert(sd::SORT_NOALLOC_BYTES >= 256, "workspace size");
+    if (n <= 1) return SD_OK;
+    void* temp = nullptr;
+    const sd_status st = sd::noalloc_scratch(&temp);
+    if (st != SD_OK) return st;
+    // radix_temp_bytes(n) <= SORT_NOALLOC_BYTES for every n <= SD_SORT_NOALLOC_MAX_N
+    return sd_radix_sort_pairs_u32(keys, vals, keys_alt, vals_alt, n, begin_bit, end_bit, temp,
+                                   sd::radix_temp_bytes(n), stream);
 }
