@@ -143,7 +143,8 @@ def trained_model(args, n_hulls):
     log_s = time.perf_counter() - t0
     model = WS.WarmStartModel(seed=11)
     t0 = time.perf_counter()
-    hist = model.train(data, epochs=args.train_epochs, batch=8192, lr=3e-3, log=lambda *a: log("gjk:", *a))
+    hist = model.train(data, epochs=args.train_epochs, batch=8192, lr=3e-3, lr_final=3e-4,
+                       log=lambda *a: log("gjk:", *a))
     train_s = time.perf_counter() - t0
     log("gjk: trained on %d samples (%d scenes, logged in %.1f s), %.1f s" % (data.n, seed - 0x5EED1000, log_s,
                                                                                train_s))
@@ -276,14 +277,28 @@ def gjk_section(args):
     infer_ms = timed(lambda: model.infer(X, npairs, pairs=pairs, quat=d.buf["quat4"], out=warm), reps)
     warm_ms = timed(lambda: d.gjk(pairs, npairs, warm=warm, out=out_w), reps)
     learned_ms = feat_ms + infer_ms + warm_ms
+    # ablation: the same seeded warm query started from the centre difference
+    # (no network): what the seeding gains without the learned direction
+    hp = d_pairs_host(pairs, npairs)
+    dcv = (h["pos4"][hp[:, 1], :3] - h["pos4"][hp[:, 0], :3]).astype(np.float32)
+    cdir = L.DeviceBuffer.from_array(np.ascontiguousarray(dcv))
+    out_s = d.gjk(pairs, npairs, warm=cdir)
+    seeded_ms = timed(lambda: d.gjk(pairs, npairs, warm=cdir, out=out_s), reps)
     c = d.download_gjk(out_c, npairs)
     w = d.download_gjk(out_w, npairs)
-    ok = ((c["flags"] | w["flags"]) & L.GJK_BUDGET) == 0
+    cs = d.download_gjk(out_s, npairs)
+    L.synchronize()
+    wdir = warm.download(np.float32, 3 * npairs).reshape(-1, 3)
     same_verdict = bool(np.array_equal(c["flags"] & 1, w["flags"] & 1))
-    rel = np.abs(w["sep"][ok] - c["sep"][ok]) / np.maximum(np.abs(c["sep"][ok]), 1e-3)
+    a64, b64 = w["sep"].astype(np.float64), c["sep"].astype(np.float64)
+    rel = np.where(a64 == b64, 0.0, np.abs(a64 - b64) / np.maximum(np.abs(b64), 1e-300))
+    dcn = dcv / np.linalg.norm(dcv, axis=1, keepdims=True)
+    ang = lambda u: float(np.degrees(np.arccos(np.clip((u * c["normal"]).sum(1), -1, 1))).mean())
     hmax = int(max(c["iters"].max(), w["iters"].max())) + 1
-    log("gjk: cold %.3f ms (%.2f it), learned %.3f ms = feat %.3f + infer %.3f + warm %.3f (%.2f it)" % (
-        cold_ms, c["iters"].mean(), learned_ms, feat_ms, infer_ms, warm_ms, w["iters"].mean()))
+    log("gjk: cold %.3f ms (%.2f it), learned %.3f ms = feat %.3f + infer %.3f + warm %.3f (%.2f it); "
+        "centre-seeded %.3f ms (%.2f it); max rel sep diff %.2e" % (
+            cold_ms, c["iters"].mean(), learned_ms, feat_ms, infer_ms, warm_ms, w["iters"].mean(), seeded_ms,
+            cs["iters"].mean(), rel.max()))
 
     cpu = None
     if not args.no_cpu:
@@ -317,6 +332,12 @@ def gjk_section(args):
                        "learned": np.bincount(w["iters"], minlength=hmax).tolist()},
         "warm_used_frac": float(((w["flags"] & L.GJK_WARM_USED) != 0).mean()),
         "verdicts_identical": same_verdict, "max_rel_sep_diff": float(rel.max()) if len(rel) else 0.0,
+        "sep_bit_identical_frac": float((w["sep"] == c["sep"]).mean()),
+        "ablation_centre_seeded": {
+            "note": "the same seeded warm query from the centre difference instead of the network",
+            "pairs_per_s": npairs / (seeded_ms / 1000.0), "ms": round(seeded_ms, 4),
+            "iters_mean": float(cs["iters"].mean()),
+            "mean_angle_deg_to_converged_normal": {"network": ang(wdir), "centre_difference": ang(dcn)}},
         "hit_frac": float((c["flags"] & 1).mean()),
         "train": train,
         "cpu": cpu,
@@ -453,7 +474,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20, help="steps per scene of the plain form (0: skip it)")
     ap.add_argument("--train-samples", type=int, default=10_000_000)
     ap.add_argument("--train-log-seconds", type=float, default=90.0)
-    ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--gjk-reps", type=int, default=5)
     ap.add_argument("--cpu-gjk-pairs", type=int, default=0, help="CPU GJK sample size (0: every SAP pair)")
     ap.add_argument("--clump-warmup", type=int, default=5)

@@ -45,6 +45,7 @@ _SIGS = {
     "or_sap_pairs": ([_I, _P, _P, _I, _P, C.c_long], C.c_long),
     "or_brute_aabb_pairs": ([_I, _P, _P, _P, C.c_long], C.c_long),
     "or_gjk": ([_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, C.c_float, _P, _P, _P, _P, _I], None),
+    "or_gjk_ex": ([_I, _P, _P, _P, _P, _P, _P, _P, _I, _I, C.c_float, _P, _P, _P, _P, _P, _I], None),
     "or_clump_prepare": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
     "or_clump_pose": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], None),
     "or_global_domain": ([_I, _P, _P, _P], None),
@@ -156,9 +157,10 @@ def gjk(h, pairs, warm=None, warm_budget=16, max_iter=64, near_margin=1e-6, nthr
     iters = np.zeros(n, np.int32)
     a = [np.ascontiguousarray(h[k]) for k in ("pos4", "quat4", "vstart", "nv", "verts4")]
     w = None if warm is None else np.ascontiguousarray(warm, np.float32)
-    load().or_gjk(n, p(pairs), *[p(x) for x in a], None if w is None else p(w), warm_budget, max_iter, near_margin,
-                  p(sep), p(nrm), p(flags), p(iters), nthreads)
-    return {"sep": sep, "normal": nrm, "flags": flags, "iters": iters}
+    small = np.zeros(n, np.int32)
+    load().or_gjk_ex(n, p(pairs), *[p(x) for x in a], None if w is None else p(w), warm_budget, max_iter,
+                     near_margin, p(sep), p(nrm), p(flags), p(iters), p(small), nthreads)
+    return {"sep": sep, "normal": nrm, "flags": flags, "iters": iters, "small_iters": small}
 
 _lib = None
 

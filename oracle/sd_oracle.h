@@ -102,6 +102,11 @@ long or_brute_aabb_pairs(int n, const float* lo4, const float* hi4, int* pairs, 
 void or_gjk(int npairs, const int* pairs, const float* pos4, const float* quat4, const int* vstart, const int* nv,
             const float* verts4, const float* warm, int warm_budget, int max_iter, float near_margin, float* sep,
             float* normal3, int* flags, int* iters, int nthreads);
+/* As or_gjk; small_iters (nullable) receives the iterations each warm query
+ * spent on its seed problem (the top-3 supports of both hulls). */
+void or_gjk_ex(int npairs, const int* pairs, const float* pos4, const float* quat4, const int* vstart, const int* nv,
+               const float* verts4, const float* warm, int warm_budget, int max_iter, float near_margin, float* sep,
+               float* normal3, int* flags, int* iters, int* small_iters, int nthreads);
 
 /* ------------------------------------------------------------------ clumps
  * (sd_oracle_clump.c) body state P (x, y, z, 1/m), V, Q (w, x, y, z), L (world

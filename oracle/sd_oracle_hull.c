@@ -108,18 +108,41 @@ static uint32_t fkey(float f) {
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+/* Axis of largest box-centre variance, in FP32 and in the order of the GPU's
+ * reduction: centres relative to the first box's centre, per-thread sums over
+ * a grid stride of nb = min(ceil(n / 256), 256) blocks of 256 threads, a tree
+ * over each block's 256 sums (halving strides), block sums added in order,
+ * var = S2 / n - (S1 / n)^2, the first axis of largest variance. */
 int or_sap_axis(int n, const float* lo4, const float* hi4) {
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) {
-            const double c = 0.5 * ((double)lo4[4 * (long)i + a] + (double)hi4[4 * (long)i + a]);
-            s[a] += c;
-            s[3 + a] += c * c;
+    if (n <= 0) return 0;
+    const int blk = 256;
+    int nb = (n + blk - 1) / blk;
+    if (nb > 256) nb = 256;
+    float r[3];
+    for (int a = 0; a < 3; ++a) r[a] = 0.5f * (lo4[a] + hi4[a]);
+    float s[6] = {0, 0, 0, 0, 0, 0};
+    float red[6][256];
+    for (int b = 0; b < nb; ++b) {
+        for (int t = 0; t < blk; ++t) {
+            float acc[6] = {0, 0, 0, 0, 0, 0};
+            for (long i = (long)b * blk + t; i < n; i += (long)nb * blk)
+                for (int a = 0; a < 3; ++a) {
+                    const float d = 0.5f * (lo4[4 * i + a] + hi4[4 * i + a]) - r[a];
+                    acc[a] += d;
+                    acc[3 + a] += d * d;
+                }
+            for (int k = 0; k < 6; ++k) red[k][t] = acc[k];
         }
+        for (int h = blk / 2; h > 0; h >>= 1)
+            for (int t = 0; t < h; ++t)
+                for (int k = 0; k < 6; ++k) red[k][t] += red[k][t + h];
+        for (int k = 0; k < 6; ++k) s[k] += red[k][0];
+    }
     int best = 0;
-    double bv = -1.0;
+    float bv = -1.0f;
     for (int a = 0; a < 3; ++a) {
-        const double m = s[a] / n, var = s[3 + a] / n - m * m;
+        const float m = s[a] / (float)n;
+        const float var = s[3 + a] / (float)n - m * m;
         if (var > bv) {
             bv = var;
             best = a;
@@ -182,13 +205,39 @@ long or_brute_aabb_pairs(int n, const float* lo4, const float* hi4, int* pairs, 
     return np;
 }
 
-/* ------------------------------------------------------------------ GJK / EPA */
+/* ------------------------------------------------------------------ GJK / EPA
+ *
+ * All arithmetic is FP32, written out in the order the GPU kernels use
+ * (compiled without contraction). M = A - B in the query frame: A's centre at
+ * the origin, B's centre at dc = c_B - c_A. Vertex keys are ia * 1024 + ib.
+ *
+ * GJK ends on an exact exit (the support point of M along -v is already a
+ * simplex vertex), on a stall (adding it does not decrease |v|), inside (hit),
+ * or on the iteration budget. The reported separation / depth and normal are
+ * a function of the closest features alone: the sorted vertex sets IA, IB of
+ * the final simplex (or EPA face) give vertex-vertex, vertex-edge, edge-vertex
+ * distances, or the plane distance of a vertex-face, face-vertex or edge-edge
+ * feature. Two queries that end on the same features report the same bits,
+ * whatever their start.
+ *
+ * A warm-started query seeds its simplex from a predicted direction u: the
+ * eight vertices of A furthest along u and of B furthest along -u form a small
+ * problem (64 points of M) solved first, and GJK on the full hulls continues
+ * from its simplex. A warm result is accepted only with a certificate that its
+ * features are the unique closest ones: every vertex outside the features lies
+ * a margin below the supporting plane along the normal, and the closest points
+ * lie inside the features. Otherwise the pair is solved cold.
+ */
 
 #define KEYMUL 1024
-#define REL_EPS 1e-6      /* GJK (double) */
-#define TOUCH_EPS2 1e-14
-#define STALL_EPS 1e-4
-#define EPA_REL_EPS 1e-6f /* EPA (float) */
+#define TOUCH_EPS2 1e-14f   /* |v|^2 below this: origin on the boundary (hit) */
+#define REL_EPS 1e-6f       /* sphere shapes: v.v - v.w <= eps v.v ends the query */
+#define STALL_EPS 1e-4f     /* gap accepted when the distance stops decreasing */
+#define EPA_REL_EPS 1e-6f
+#define CERT_REL 1.52587890625e-05f /* 2^-16: support margin relative to the features' vertex radii */
+#define CERT_TAU 9.765625e-04f      /* 2^-10: closest points this far inside their features */
+#define WARM_K 8           /* warm seed: vertices of each hull furthest along the predicted direction */
+#define SMALL_MAX_ITER 16
 #define MAXV 48
 #define MAXF 96
 #define MAXE 48
@@ -205,6 +254,12 @@ typedef struct {
     V3 p;
     int key;
 } MP;
+
+/* vertex k of s in the query frame */
+static V3 vert_at(const Shape* s, int k) {
+    const float* p = s->verts + 4 * (long)k;
+    return vadd(mvec(s->m, mk(p[0], p[1], p[2])), s->off);
+}
 
 static V3 support(const Shape* s, V3 u, int* idx) {
     if (s->nv == 0) {
@@ -226,8 +281,7 @@ static V3 support(const Shape* s, V3 u, int* idx) {
         }
     }
     *idx = best;
-    const float* p = s->verts + 4 * (long)best;
-    return vadd(mvec(s->m, mk(p[0], p[1], p[2])), s->off);
+    return vert_at(s, best);
 }
 
 static MP supM(const Shape* A, const Shape* B, V3 u) {
@@ -240,149 +294,181 @@ static MP supM(const Shape* A, const Shape* B, V3 u) {
     return r;
 }
 
-/* The GJK simplex and everything derived from it are evaluated in double (the
- * support points themselves are FP32 values, stored exactly). */
-typedef struct {
-    double x, y, z;
-} D3;
-static D3 dk(double x, double y, double z) {
-    D3 r = {x, y, z};
-    return r;
-}
-static D3 d_of(V3 a) { return dk((double)a.x, (double)a.y, (double)a.z); }
-static V3 f_of(D3 a) { return mk((float)a.x, (float)a.y, (float)a.z); }
-static D3 dadd(D3 a, D3 b) { return dk(a.x + b.x, a.y + b.y, a.z + b.z); }
-static D3 dsub(D3 a, D3 b) { return dk(a.x - b.x, a.y - b.y, a.z - b.z); }
-static D3 dneg(D3 a) { return dk(-a.x, -a.y, -a.z); }
-static D3 dscl(D3 a, double s) { return dk(a.x * s, a.y * s, a.z * s); }
-static double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-static D3 dcross(D3 a, D3 b) { return dk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
-
-typedef struct {
-    D3 p;
-    int key;
-} SP;
-static SP sp_of(MP m) {
-    SP r;
-    r.p = d_of(m.p);
-    r.key = m.key;
-    return r;
+/* The K vertices of s furthest along u (larger projection first, lower index on
+ * ties), in the query frame. Returns their number (min(K, nv)). */
+static int top_k(const Shape* s, V3 u, int* idx, V3* pts) {
+    const V3 l = mtvec(s->m, u);
+    float val[WARM_K];
+    int n = 0;
+    for (int k = 0; k < s->nv; ++k) {
+        const float* p = s->verts + 4 * (long)k;
+        const float d = p[0] * l.x + p[1] * l.y + p[2] * l.z;
+        int pos = n < WARM_K ? n : WARM_K;
+        while (pos > 0 && d > val[pos - 1]) --pos;
+        if (pos >= WARM_K) continue;
+        const int last = n < WARM_K ? n : WARM_K - 1;
+        for (int j = last; j > pos; --j) {
+            val[j] = val[j - 1];
+            idx[j] = idx[j - 1];
+        }
+        val[pos] = d;
+        idx[pos] = k;
+        if (n < WARM_K) ++n;
+    }
+    for (int j = 0; j < n; ++j) pts[j] = vert_at(s, idx[j]);
+    return n;
 }
 
+/* Support over the full shapes, or over two small vertex sets (the warm seed). */
 typedef struct {
-    SP w[4];
+    const Shape* A;
+    const Shape* B;
+    int small;
+    int na, nb;
+    int ia[WARM_K], ib[WARM_K];
+    V3 pa[WARM_K], pb[WARM_K];
+} Sup;
+
+static MP sup_eval(const Sup* S, V3 u) {
+    if (!S->small) return supM(S->A, S->B, u);
+    /* larger projection, lower vertex index on ties (independent of the order of the sets) */
+    int ka = 0, kb = 0;
+    float ba = vdot(S->pa[0], u), bb = -vdot(S->pb[0], u);
+    for (int j = 1; j < S->na; ++j) {
+        const float d = vdot(S->pa[j], u);
+        if (d > ba || (d == ba && S->ia[j] < S->ia[ka])) {
+            ba = d;
+            ka = j;
+        }
+    }
+    for (int j = 1; j < S->nb; ++j) {
+        const float d = -vdot(S->pb[j], u);
+        if (d > bb || (d == bb && S->ib[j] < S->ib[kb])) {
+            bb = d;
+            kb = j;
+        }
+    }
+    MP r;
+    r.p = vsub(S->pa[ka], S->pb[kb]);
+    r.key = S->ia[ka] * KEYMUL + S->ib[kb];
+    return r;
+}
+
+typedef struct {
+    MP w[4];
     int n;
 } Simplex;
 
-static D3 cl_segment(Simplex* s) {
-    const D3 a = s->w[0].p, b = s->w[1].p;
-    const D3 ab = dsub(b, a);
-    const double t = -ddot(a, ab);
-    if (t <= 0.0) {
+static V3 cl_segment(Simplex* s) {
+    const V3 a = s->w[0].p, b = s->w[1].p;
+    const V3 ab = vsub(b, a);
+    const float t = -vdot(a, ab);
+    if (t <= 0.0f) {
         s->n = 1;
         return a;
     }
-    const double den = ddot(ab, ab);
+    const float den = vdot(ab, ab);
     if (t >= den) {
         s->w[0] = s->w[1];
         s->n = 1;
         return b;
     }
-    return dadd(a, dscl(ab, t / den));
+    return vadd(a, vscl(ab, t / den));
 }
 
 /* closest point of a triangle to the origin by Voronoi regions (vertex, edge, face) */
-static D3 cl_triangle(Simplex* s) {
-    const SP A = s->w[0], B = s->w[1], C = s->w[2];
-    const D3 a = A.p, b = B.p, c = C.p;
-    const D3 ab = dsub(b, a), ac = dsub(c, a), ap = dneg(a);
-    const double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
-    if (d1 <= 0.0 && d2 <= 0.0) {
+static V3 cl_triangle(Simplex* s) {
+    const MP A = s->w[0], B = s->w[1], C = s->w[2];
+    const V3 a = A.p, b = B.p, c = C.p;
+    const V3 ab = vsub(b, a), ac = vsub(c, a), ap = vneg(a);
+    const float d1 = vdot(ab, ap), d2 = vdot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) {
         s->n = 1;
         return a;
     }
-    const D3 bp = dneg(b);
-    const double d3 = ddot(ab, bp), d4 = ddot(ac, bp);
-    if (d3 >= 0.0 && d4 <= d3) {
+    const V3 bp = vneg(b);
+    const float d3 = vdot(ab, bp), d4 = vdot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) {
         s->w[0] = B;
         s->n = 1;
         return b;
     }
-    const double vc = d1 * d4 - d3 * d2;
-    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
-        const double t = d1 / (d1 - d3);
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        const float t = d1 / (d1 - d3);
         s->n = 2;
-        return dadd(a, dscl(ab, t));
+        return vadd(a, vscl(ab, t));
     }
-    const D3 cp = dneg(c);
-    const double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
-    if (d6 >= 0.0 && d5 <= d6) {
+    const V3 cp = vneg(c);
+    const float d5 = vdot(ab, cp), d6 = vdot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) {
         s->w[0] = C;
         s->n = 1;
         return c;
     }
-    const double vb = d5 * d2 - d1 * d6;
-    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
-        const double t = d2 / (d2 - d6);
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        const float t = d2 / (d2 - d6);
         s->w[1] = C;
         s->n = 2;
-        return dadd(a, dscl(ac, t));
+        return vadd(a, vscl(ac, t));
     }
-    const double va = d3 * d6 - d5 * d4;
-    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
-        const double t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
         s->w[0] = B;
         s->w[1] = C;
         s->n = 2;
-        return dadd(b, dscl(dsub(c, b), t));
+        return vadd(b, vscl(vsub(c, b), t));
     }
-    const double den = 1.0 / (va + vb + vc);
-    const double v = vb * den, w = vc * den;
-    return dadd(dadd(a, dscl(ab, v)), dscl(ac, w));
+    const float den = 1.0f / (va + vb + vc);
+    const float v = vb * den, w = vc * den;
+    return vadd(vadd(a, vscl(ab, v)), vscl(ac, w));
 }
 
-static int outside(D3 a, D3 b, D3 c, D3 d) {
-    const D3 nrm = dcross(dsub(b, a), dsub(c, a));
-    const double so = -ddot(a, nrm);
-    const double sd = ddot(dsub(d, a), nrm);
-    return so * sd < 0.0;
+static int outside(V3 a, V3 b, V3 c, V3 d) {
+    const V3 nrm = vcross(vsub(b, a), vsub(c, a));
+    const float so = -vdot(a, nrm);
+    const float sd = vdot(vsub(d, a), nrm);
+    return so * sd < 0.0f;
 }
 
 /* A near-flat tetrahedron (|6V| <= 1e-5 |ab||ac||ad|) never counts as enclosing
- * the origin; then every face is searched. */
-static D3 cl_tetra(Simplex* s, int* inside) {
-    const SP P[4] = {s->w[0], s->w[1], s->w[2], s->w[3]};
+ * the origin; then every face is searched. Faces in search order: (0,1,2),
+ * (0,2,3), (0,3,1), (1,3,2); the first face of smallest distance wins. */
+static V3 cl_tetra(Simplex* s, int* inside) {
+    const MP P[4] = {s->w[0], s->w[1], s->w[2], s->w[3]};
     static const int F[4][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 3, 2, 0}};
-    const D3 ab = dsub(P[1].p, P[0].p), ac = dsub(P[2].p, P[0].p), ad = dsub(P[3].p, P[0].p);
-    const double vol = ddot(ad, dcross(ab, ac));
-    const double lim = 1e-5 * (sqrt(ddot(ab, ab)) * sqrt(ddot(ac, ac)) * sqrt(ddot(ad, ad)));
-    const int flat = !(fabs(vol) > lim);
-    double best = 1.79769313486231e308;
-    D3 bestp = dk(0.0, 0.0, 0.0);
-    Simplex bests = *s;
-    int any = 0;
+    const V3 ab = vsub(P[1].p, P[0].p), ac = vsub(P[2].p, P[0].p), ad = vsub(P[3].p, P[0].p);
+    const float vol = vdot(ad, vcross(ab, ac));
+    const float lim = 1e-5f * (sqrtf(vdot(ab, ab)) * sqrtf(vdot(ac, ac)) * sqrtf(vdot(ad, ad)));
+    const int flat = !(fabsf(vol) > lim);
+    float best = 3.402823466e38f;
+    int bi = -1;
     for (int f = 0; f < 4; ++f) {
         if (!flat && !outside(P[F[f][0]].p, P[F[f][1]].p, P[F[f][2]].p, P[F[f][3]].p)) continue;
-        any = 1;
-        Simplex t = *s;
+        Simplex t;
         t.w[0] = P[F[f][0]];
         t.w[1] = P[F[f][1]];
         t.w[2] = P[F[f][2]];
         t.n = 3;
-        const D3 q = cl_triangle(&t);
-        const double dq = ddot(q, q);
+        const V3 q = cl_triangle(&t);
+        const float dq = vdot(q, q);
         if (dq < best) {
             best = dq;
-            bestp = q;
-            bests = t;
+            bi = f;
         }
     }
-    *inside = !any;
-    if (any) *s = bests;
-    return bestp;
+    *inside = bi < 0;
+    if (bi < 0) return mk(0.0f, 0.0f, 0.0f);
+    s->w[0] = P[F[bi][0]];
+    s->w[1] = P[F[bi][1]];
+    s->w[2] = P[F[bi][2]];
+    s->n = 3;
+    return cl_triangle(s);
 }
 
-static D3 cl_simplex(Simplex* s, int* inside) {
+static V3 cl_simplex(Simplex* s, int* inside) {
     *inside = 0;
     if (s->n == 1) return s->w[0].p;
     if (s->n == 2) return cl_segment(s);
@@ -390,9 +476,9 @@ static D3 cl_simplex(Simplex* s, int* inside) {
     return cl_tetra(s, inside);
 }
 
-static void sort_by_key(SP* w, int n) {
+static void sort_by_key(MP* w, int n) {
     for (int i = 1; i < n; ++i) {
-        SP x = w[i];
+        MP x = w[i];
         int j = i - 1;
         while (j >= 0 && w[j].key > x.key) {
             w[j + 1] = w[j];
@@ -400,6 +486,232 @@ static void sort_by_key(SP* w, int n) {
         }
         w[j + 1] = x;
     }
+}
+
+typedef struct {
+    int hit, converged, uncertain, it;
+} GjkState;
+
+/* GJK iterations from simplex s with closest point *vp, at most max_iter
+ * support evaluations (counted in g->it). */
+static void gjk_loop(const Sup* S, Simplex* s, V3* vp, int max_iter, GjkState* g) {
+    V3 v = *vp;
+    int it = 0;
+    while (it < max_iter) {
+        const float vv = vdot(v, v);
+        if (vv <= TOUCH_EPS2) {
+            g->hit = 1;
+            break;
+        }
+        const MP w = sup_eval(S, vneg(v));
+        ++it;
+        const float vw = vdot(v, w.p);
+        const float gap = vv - vw;
+        int dup = 0;
+        for (int i = 0; i < s->n; ++i) dup = dup || (w.key >= 0 && s->w[i].key == w.key);
+        /* exact exit: the support point is already a simplex vertex, so in
+         * exact arithmetic v.w >= v.v and any gap is rounding; v.w <= 0 then
+         * means |v| is at rounding level: an uncertain hit for EPA. A sphere
+         * shape (keyless) ends on the relative gap instead. */
+        if (dup || (w.key < 0 && gap <= REL_EPS * vv)) {
+            if (vw > 0.0f) {
+                g->converged = 1;
+            } else {
+                g->hit = 1;
+                g->uncertain = 1;
+            }
+            break;
+        }
+        const Simplex sp = *s;
+        s->w[s->n++] = w;
+        int inside = 0;
+        const V3 vn = cl_simplex(s, &inside);
+        if (inside) {
+            g->hit = 1;
+            break;
+        }
+        if (!(vdot(vn, vn) < vv)) {
+            /* stalled at FP32 resolution: the previous feature is final if the
+             * duality gap is within 1e-4 */
+            *s = sp;
+            if (gap <= STALL_EPS * vv) {
+                g->converged = 1;
+            } else if (vw <= 0.0f) {
+                g->hit = 1;
+                g->uncertain = 1;
+            }
+            break;
+        }
+        v = vn;
+    }
+    g->it += it;
+    *vp = v;
+}
+
+/* ---------------------------------------------------------- canonical features */
+
+typedef struct {
+    float sep;
+    V3 n;
+    int ok;   /* a recognised feature pair with its closest points inside it */
+    int kind; /* 0 degenerate, else na * 4 + nb */
+} Canon;
+
+static V3 unit_or_x(V3 d, float l) { return l > 0.0f ? vscl(d, 1.0f / l) : mk(1.0f, 0.0f, 0.0f); }
+
+/* barycentric (u, v, w) of p in triangle (a, b, c), all >= tau */
+static int inside_tri(V3 p, V3 a, V3 b, V3 c) {
+    const V3 e1 = vsub(b, a), e2 = vsub(c, a), r = vsub(p, a);
+    const float d00 = vdot(e1, e1), d01 = vdot(e1, e2), d11 = vdot(e2, e2);
+    const float d20 = vdot(r, e1), d21 = vdot(r, e2);
+    const float den = d00 * d11 - d01 * d01;
+    if (!(den > 0.0f)) return 0;
+    const float v = (d11 * d20 - d01 * d21) / den;
+    const float w = (d00 * d21 - d01 * d20) / den;
+    const float u = 1.0f - v - w;
+    return u >= CERT_TAU && v >= CERT_TAU && w >= CERT_TAU;
+}
+
+/* Distance of the features conv(pa[0..na)) of A and conv(pb[0..nb)) of B,
+ * vertices ordered by index. hit: the shapes intersect (the plane distance is
+ * then a depth, reported negative). The normal n points from A to B. */
+static Canon canon_feature(const V3* pa, int na, const V3* pb, int nb, int hit) {
+    Canon c;
+    c.ok = 0;
+    c.kind = 0;
+    c.sep = 0.0f;
+    c.n = mk(1.0f, 0.0f, 0.0f);
+    if (!hit && na == 1 && nb == 1) {
+        const V3 d = vsub(pb[0], pa[0]);
+        const float l = sqrtf(vdot(d, d));
+        c.sep = l;
+        c.n = unit_or_x(d, l);
+        c.ok = l > 0.0f;
+        c.kind = 5;
+        return c;
+    }
+    if (!hit && ((na == 1 && nb == 2) || (na == 2 && nb == 1))) {
+        /* point p against segment (s0, s1); d points from A to B */
+        const int pa_is_point = na == 1;
+        const V3 p = pa_is_point ? pa[0] : pb[0];
+        const V3 s0 = pa_is_point ? pb[0] : pa[0];
+        const V3 s1 = pa_is_point ? pb[1] : pa[1];
+        const V3 e = vsub(s1, s0);
+        const float ee = vdot(e, e);
+        if (!(ee > 0.0f)) return c;
+        const float t = vdot(vsub(p, s0), e) / ee;
+        const V3 q = vadd(s0, vscl(e, t));
+        const V3 d = pa_is_point ? vsub(q, p) : vsub(p, q);
+        const float l = sqrtf(vdot(d, d));
+        c.sep = l;
+        c.n = unit_or_x(d, l);
+        c.ok = l > 0.0f && t >= CERT_TAU && t <= 1.0f - CERT_TAU;
+        c.kind = na * 4 + nb;
+        return c;
+    }
+    V3 e1, e2, ref_a = pa[0], ref_b = pb[0];
+    if (na == 1 && nb == 3) {
+        e1 = vsub(pb[1], pb[0]);
+        e2 = vsub(pb[2], pb[0]);
+    } else if (na == 3 && nb == 1) {
+        e1 = vsub(pa[1], pa[0]);
+        e2 = vsub(pa[2], pa[0]);
+    } else if (na == 2 && nb == 2) {
+        e1 = vsub(pa[1], pa[0]);
+        e2 = vsub(pb[1], pb[0]);
+    } else {
+        return c;
+    }
+    const V3 cr = vcross(e1, e2);
+    const float l = sqrtf(vdot(cr, cr));
+    if (!(l > 0.0f)) return c;
+    V3 m = vscl(cr, 1.0f / l);
+    float h = vdot(vsub(ref_b, ref_a), m);
+    if (hit ? h > 0.0f : h < 0.0f) {
+        m = vneg(m);
+        h = -h;
+    }
+    c.sep = h;
+    c.n = m;
+    c.kind = na * 4 + nb;
+    if (na == 1) {
+        /* a projected onto B's face plane */
+        c.ok = inside_tri(vadd(pa[0], vscl(m, h)), pb[0], pb[1], pb[2]);
+    } else if (nb == 1) {
+        c.ok = inside_tri(vsub(pb[0], vscl(m, h)), pa[0], pa[1], pa[2]);
+    } else {
+        /* closest parameters of the two edge lines */
+        const V3 r = vsub(pa[0], pb[0]);
+        const float a = vdot(e1, e1), b = vdot(e1, e2), e = vdot(e2, e2);
+        const float cc = vdot(e1, r), f = vdot(e2, r);
+        const float den = a * e - b * b;
+        if (den > 0.0f) {
+            const float s = (b * f - cc * e) / den;
+            const float t = (a * f - b * cc) / den;
+            c.ok = s >= CERT_TAU && s <= 1.0f - CERT_TAU && t >= CERT_TAU && t <= 1.0f - CERT_TAU;
+        }
+    }
+    return c;
+}
+
+/* Sorted distinct vertex indices of A (key / 1024) and B (key % 1024) of n
+ * keyed points; returns 0 if a point has no key (sphere shape). */
+static int feature_sets(const MP* w, int n, int* IA, int* na, int* IB, int* nb) {
+    *na = 0;
+    *nb = 0;
+    for (int i = 0; i < n; ++i) {
+        if (w[i].key < 0) return 0;
+        const int a = w[i].key / KEYMUL, b = w[i].key % KEYMUL;
+        int da = 0, db = 0;
+        for (int j = 0; j < *na; ++j) da = da || IA[j] == a;
+        for (int j = 0; j < *nb; ++j) db = db || IB[j] == b;
+        if (!da) IA[(*na)++] = a;
+        if (!db) IB[(*nb)++] = b;
+    }
+    for (int i = 1; i < *na; ++i)
+        for (int j = i; j > 0 && IA[j - 1] > IA[j]; --j) {
+            const int t = IA[j];
+            IA[j] = IA[j - 1];
+            IA[j - 1] = t;
+        }
+    for (int i = 1; i < *nb; ++i)
+        for (int j = i; j > 0 && IB[j - 1] > IB[j]; --j) {
+            const int t = IB[j];
+            IB[j] = IB[j - 1];
+            IB[j - 1] = t;
+        }
+    return 1;
+}
+
+/* Margin certificate: along n, every vertex of A outside IA lies more than
+ * delta below the lowest vertex of IA, every vertex of B outside IB more than
+ * delta above the highest of IB; delta = 2^-16 (|a0| + |b0|) of the features'
+ * first vertices (local frames). */
+static int certify(const Shape* A, const int* IA, int na, const Shape* B, const int* IB, int nb, V3 n) {
+    const V3 la = mtvec(A->m, n), lb = mtvec(B->m, n);
+    const float* a0 = A->verts + 4 * (long)IA[0];
+    const float* b0 = B->verts + 4 * (long)IB[0];
+    const float delta =
+        CERT_REL * (sqrtf(a0[0] * a0[0] + a0[1] * a0[1] + a0[2] * a0[2]) + sqrtf(b0[0] * b0[0] + b0[1] * b0[1] + b0[2] * b0[2]));
+    float in_a = 3.402823466e38f, out_a = -3.402823466e38f;
+    for (int k = 0; k < A->nv; ++k) {
+        const float* p = A->verts + 4 * (long)k;
+        const float d = p[0] * la.x + p[1] * la.y + p[2] * la.z;
+        int mem = 0;
+        for (int j = 0; j < na; ++j) mem = mem || IA[j] == k;
+        if (mem) in_a = fminf(in_a, d);
+        else out_a = fmaxf(out_a, d);
+    }
+    float in_b = -3.402823466e38f, out_b = 3.402823466e38f;
+    for (int k = 0; k < B->nv; ++k) {
+        const float* p = B->verts + 4 * (long)k;
+        const float d = p[0] * lb.x + p[1] * lb.y + p[2] * lb.z;
+        int mem = 0;
+        for (int j = 0; j < nb; ++j) mem = mem || IB[j] == k;
+        if (mem) in_b = fmaxf(in_b, d);
+        else out_b = fminf(out_b, d);
+    }
+    return out_a < in_a - delta && out_b > in_b + delta;
 }
 
 typedef struct {
@@ -431,18 +743,56 @@ typedef struct {
     float sep;
     V3 n;
     int flags, iters, epa_iters;
+    int certified; /* closest features certified unique (computed on request) */
 } Out;
 
 enum { HIT = 1, CONV = 2, BUDGET = 4, ELIMIT = 8, DEGEN = 16, WARM = 32 };
 
-static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_iter, Out* o) {
+/* Result of a feature (simplex points or an EPA face), canonical when every
+ * point has a key; *ok receives whether the features were recognised with
+ * interior closest points; cert: also run the margin certificate. */
+static void feature_result(const Shape* A, const Shape* B, MP* w, int n, int hit, int cert, Out* o, V3 inner) {
+    int IA[4], IB[4], na, nb;
+    o->certified = 0;
+    if (feature_sets(w, n, IA, &na, IB, &nb)) {
+        V3 pa[3], pb[3];
+        if (na <= 3 && nb <= 3) {
+            for (int k = 0; k < na; ++k) pa[k] = vert_at(A, IA[k]);
+            for (int k = 0; k < nb; ++k) pb[k] = vert_at(B, IB[k]);
+            const Canon c = canon_feature(pa, na, pb, nb, hit);
+            if (c.kind) {
+                o->sep = c.sep;
+                o->n = c.n;
+                o->certified = c.ok && cert && certify(A, IA, na, B, IB, nb, c.n);
+                return;
+            }
+        }
+        /* degenerate features: the points ordered by key */
+        sort_by_key(w, n);
+    }
+    if (!hit) {
+        Simplex s;
+        for (int i = 0; i < n; ++i) s.w[i] = w[i];
+        s.n = n;
+        int inside = 0;
+        const V3 v = n == 1 ? w[0].p : cl_simplex(&s, &inside);
+        const float l = sqrtf(vdot(v, v));
+        o->sep = l;
+        o->n = l > 0.0f ? vscl(v, -1.0f / l) : mk(1.0f, 0.0f, 0.0f);
+        return;
+    }
+    V3 nrm = vcross(vsub(w[1].p, w[0].p), vsub(w[2].p, w[0].p));
+    const float l2 = vdot(nrm, nrm);
+    if (vdot(nrm, vsub(w[0].p, inner)) < 0.0f) nrm = vneg(nrm);
+    const V3 nn = vscl(nrm, 1.0f / sqrtf(l2));
+    o->sep = -vdot(nn, w[0].p);
+    o->n = nn;
+}
+
+static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_iter, int cert, Out* o) {
     MP P[MAXV];
     int np = 0;
-    for (int i = 0; i < s0->n; ++i) {
-        P[np].p = f_of(s0->w[i].p); /* exact: FP32 support points */
-        P[np].key = s0->w[i].key;
-        ++np;
-    }
+    for (int i = 0; i < s0->n; ++i) P[np++] = s0->w[i];
     const V3 axes[3] = {mk(1.0f, 0.0f, 0.0f), mk(0.0f, 1.0f, 0.0f), mk(0.0f, 0.0f, 1.0f)};
     if (np == 1) {
         for (int k = 0; k < 6 && np == 1; ++k) {
@@ -474,6 +824,7 @@ static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_i
     if (np < 4) {
         o->flags |= HIT | DEGEN;
         o->sep = 0.0f;
+        o->certified = 0;
         return;
     }
     const V3 inner = vscl(vadd(vadd(P[0].p, P[1].p), vadd(P[2].p, P[3].p)), 0.25f);
@@ -485,6 +836,7 @@ static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_i
     if (nf < 4) {
         o->flags |= HIT | DEGEN;
         o->sep = 0.0f;
+        o->certified = 0;
         return;
     }
     int best = 0, it = 0, converged = 0;
@@ -548,154 +900,144 @@ static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_i
             break;
         }
     }
+    o->epa_iters = it;
     if (nf == 0) {
         o->flags |= HIT | DEGEN;
         o->sep = 0.0f;
-        o->epa_iters = it;
+        o->certified = 0;
         return;
     }
     best = 0;
     for (int f = 1; f < nf; ++f)
         if (F[f].d < F[best].d) best = f;
-    /* canonical depth / normal: the face's vertices ordered by key */
     MP q[3] = {P[F[best].a], P[F[best].b], P[F[best].c]};
-    for (int i = 1; i < 3; ++i) {
-        MP x = q[i];
-        int j = i - 1;
-        while (j >= 0 && q[j].key > x.key) {
-            q[j + 1] = q[j];
-            --j;
-        }
-        q[j + 1] = x;
-    }
-    V3 nrm = vcross(vsub(q[1].p, q[0].p), vsub(q[2].p, q[0].p));
-    const float l2 = vdot(nrm, nrm);
-    if (vdot(nrm, vsub(q[0].p, inner)) < 0.0f) nrm = vneg(nrm);
-    const V3 n = vscl(nrm, 1.0f / sqrtf(l2));
-    const float d = vdot(n, q[0].p);
     o->flags |= HIT | (converged ? CONV : 0);
     if (!converged && it >= max_iter) o->flags |= BUDGET;
-    if (d < 0.0f) o->flags |= DEGEN; /* polytope does not enclose the origin */
-    o->sep = -d;
-    o->n = n;
-    o->epa_iters = it;
+    if (F[best].d < 0.0f) o->flags |= DEGEN; /* polytope does not enclose the origin */
+    feature_result(A, B, q, 3, 1, cert, o, inner);
 }
 
-static Out query(const Shape* A, const Shape* B, V3 init, int max_iter) {
+/* Finishes a query from its GJK state: EPA for a hit, the feature result for a
+ * separation. An uncertain hit for which EPA finds no enclosing polytope
+ * reports the uncertified separation. */
+static void finish(const Shape* A, const Shape* B, Simplex* s, const GjkState* g, int max_iter, int cert, Out* o) {
+    int converged = g->converged;
+    if (g->hit) {
+        run_epa(A, B, s, max_iter, cert, o);
+        if (!(g->uncertain && (o->flags & DEGEN))) return;
+        const int ei = o->epa_iters;
+        o->flags = 0;
+        o->epa_iters = ei;
+        converged = 0;
+    }
+    o->flags |= converged ? CONV : BUDGET;
+    feature_result(A, B, s->w, s->n, 0, cert, o, mk(0.0f, 0.0f, 0.0f));
+}
+
+static Out out_init(void) {
     Out o;
     o.flags = 0;
     o.iters = 0;
     o.epa_iters = 0;
     o.sep = 0.0f;
     o.n = mk(1.0f, 0.0f, 0.0f);
-    if (vdot(init, init) == 0.0f) init = mk(1.0f, 0.0f, 0.0f);
-    Simplex s;
-    s.w[0] = sp_of(supM(A, B, init));
-    s.n = 1;
-    D3 v = s.w[0].p;
-    int hit = 0, converged = 0, uncertain = 0, it = 0;
-    while (it < max_iter) {
-        const double vv = ddot(v, v);
-        if (vv <= TOUCH_EPS2) {
-            hit = 1;
-            break;
-        }
-        const SP w = sp_of(supM(A, B, f_of(dneg(v))));
-        ++it;
-        const double vw = ddot(v, w.p);
-        if (vv - vw <= REL_EPS * vv) {
-            converged = 1;
-            break;
-        }
-        /* a repeated support vertex or a stalled distance ends the query: a
-         * separation only when the duality gap certifies it to 1e-4; without a
-         * certificate and with v.w <= 0 the exit is uncertain and goes to EPA */
-        int dup = 0;
-        for (int i = 0; i < s.n; ++i) dup = dup || (w.key >= 0 && s.w[i].key == w.key);
-        if (dup) {
-            if (vv - vw <= STALL_EPS * vv) converged = 1;
-            else if (vw <= 0.0) hit = uncertain = 1;
-            break;
-        }
-        const Simplex sp = s;
-        s.w[s.n++] = w;
-        int inside = 0;
-        const D3 vn = cl_simplex(&s, &inside);
-        if (inside) {
-            hit = 1;
-            break;
-        }
-        /* the distance must strictly decrease; otherwise the previous feature is
-         * final when the duality gap certifies it to 1e-4 */
-        if (!(ddot(vn, vn) < vv)) {
-            s = sp;
-            if (vv - vw <= STALL_EPS * vv) converged = 1;
-            else if (vw <= 0.0) hit = uncertain = 1;
-            break;
-        }
-        v = vn;
-    }
-    o.iters = it;
-    if (hit) {
-        run_epa(A, B, &s, max_iter, &o);
-        if (!(uncertain && (o.flags & DEGEN))) return o;
-        /* no enclosing polytope after an uncertain exit: the uncertified separation */
-        const int ei = o.epa_iters;
-        o.flags = 0;
-        o.epa_iters = ei;
-        o.sep = 0.0f;
-        o.n = mk(1.0f, 0.0f, 0.0f);
-        int inside = 0;
-        v = cl_simplex(&s, &inside);
-        converged = 0;
-    }
-    o.flags |= converged ? CONV : BUDGET;
-    int canon = 1;
-    for (int i = 0; i < s.n; ++i) canon = canon && s.w[i].key >= 0;
-    if (canon) {
-        sort_by_key(s.w, s.n);
-        int inside = 0;
-        v = cl_simplex(&s, &inside);
-    }
-    const double dist = sqrt(ddot(v, v));
-    o.sep = (float)dist;
-    o.n = dist > 0.0 ? f_of(dscl(v, -1.0 / dist)) : mk(1.0f, 0.0f, 0.0f);
+    o.certified = 0;
     return o;
 }
 
-/* Per pair (a, b): same outputs and warm/cold policy as the GPU entry point. */
-void or_gjk(int npairs, const int* pairs, const float* pos4, const float* quat4, const int* vstart, const int* nv,
-            const float* verts4, const float* warm, int warm_budget, int max_iter, float near_margin, float* sep,
-            float* normal3, int* flags, int* iters, int nthreads) {
+/* Cold query: GJK from the support point along init (the centre difference). */
+static Out query_cold(const Shape* A, const Shape* B, V3 init, int max_iter) {
+    Out o = out_init();
+    if (vdot(init, init) == 0.0f) init = mk(1.0f, 0.0f, 0.0f);
+    Sup S;
+    S.A = A;
+    S.B = B;
+    S.small = 0;
+    Simplex s;
+    s.w[0] = supM(A, B, init);
+    s.n = 1;
+    V3 v = s.w[0].p;
+    GjkState g = {0, 0, 0, 0};
+    gjk_loop(&S, &s, &v, max_iter, &g);
+    o.iters = g.it;
+    finish(A, B, &s, &g, max_iter, 0, &o);
+    return o;
+}
+
+/* Warm query from the predicted direction u (A -> B): seed from the top-K
+ * supports (one pass over both hulls, counted as one iteration), the small
+ * problem from the seed support along u, then GJK on the full hulls within
+ * budget iterations, then the certificate (one more pass, counted). */
+static Out query_warm(const Shape* A, const Shape* B, V3 u, int budget, int* small_it) {
+    Out o = out_init();
+    *small_it = 0;
+    if (vdot(u, u) == 0.0f) u = mk(1.0f, 0.0f, 0.0f);
+    Sup S;
+    S.A = A;
+    S.B = B;
+    S.small = 1;
+    S.na = top_k(A, u, S.ia, S.pa);
+    S.nb = top_k(B, vneg(u), S.ib, S.pb);
+    Simplex s;
+    s.w[0] = sup_eval(&S, u);
+    s.n = 1;
+    V3 v = s.w[0].p;
+    GjkState g = {0, 0, 0, 0};
+    gjk_loop(&S, &s, &v, SMALL_MAX_ITER, &g);
+    *small_it = g.it;
+    GjkState gf = {0, 0, 0, 0};
+    if (g.hit && !g.uncertain) {
+        gf.hit = 1;
+    } else {
+        S.small = 0;
+        gjk_loop(&S, &s, &v, budget, &gf);
+    }
+    o.iters = 1 + gf.it + 1;
+    finish(A, B, &s, &gf, budget, 1, &o);
+    return o;
+}
+
+static void make_pair_shapes(const float* pos4, const float* quat4, const int* vstart, const int* nv,
+                             const float* verts4, int ia, int ib, Shape* A, Shape* B, V3* dc) {
+    const float* ca = pos4 + 4 * (long)ia;
+    const float* cb = pos4 + 4 * (long)ib;
+    A->verts = verts4 + 4 * (long)vstart[ia];
+    A->nv = nv[ia];
+    A->radius = ca[3];
+    qmat(quat4 + 4 * (long)ia, A->m);
+    A->off = mk(0.0f, 0.0f, 0.0f);
+    B->verts = verts4 + 4 * (long)vstart[ib];
+    B->nv = nv[ib];
+    B->radius = cb[3];
+    qmat(quat4 + 4 * (long)ib, B->m);
+    *dc = mk(cb[0] - ca[0], cb[1] - ca[1], cb[2] - ca[2]);
+    B->off = *dc;
+}
+
+/* Per pair (a, b): same outputs and warm/cold policy as the GPU entry point.
+ * small_iters (nullable) receives the small-problem iterations of warm queries. */
+void or_gjk_ex(int npairs, const int* pairs, const float* pos4, const float* quat4, const int* vstart, const int* nv,
+               const float* verts4, const float* warm, int warm_budget, int max_iter, float near_margin, float* sep,
+               float* normal3, int* flags, int* iters, int* small_iters, int nthreads) {
     if (nthreads < 1) nthreads = 1;
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
     for (int p = 0; p < npairs; ++p) {
         const int ia = pairs[2 * (long)p], ib = pairs[2 * (long)p + 1];
-        const float* ca = pos4 + 4 * (long)ia;
-        const float* cb = pos4 + 4 * (long)ib;
         Shape A, B;
-        A.verts = verts4 + 4 * (long)vstart[ia];
-        A.nv = nv[ia];
-        A.radius = ca[3];
-        qmat(quat4 + 4 * (long)ia, A.m);
-        A.off = mk(0.0f, 0.0f, 0.0f);
-        B.verts = verts4 + 4 * (long)vstart[ib];
-        B.nv = nv[ib];
-        B.radius = cb[3];
-        qmat(quat4 + 4 * (long)ib, B.m);
-        const V3 dc = mk(cb[0] - ca[0], cb[1] - ca[1], cb[2] - ca[2]);
-        B.off = dc;
+        V3 dc;
+        make_pair_shapes(pos4, quat4, vstart, nv, verts4, ia, ib, &A, &B, &dc);
         Out o;
-        int spent = 0, done = 0;
-        if (warm) {
+        int spent = 0, done = 0, sit = 0;
+        if (warm && A.nv > 0 && B.nv > 0) {
             const V3 wd = mk(warm[3 * (long)p], warm[3 * (long)p + 1], warm[3 * (long)p + 2]);
-            o = query(&A, &B, wd, warm_budget < max_iter ? warm_budget : max_iter);
+            o = query_warm(&A, &B, wd, warm_budget < max_iter ? warm_budget : max_iter, &sit);
             spent = o.iters + o.epa_iters;
-            done = (o.flags & CONV) && !(o.flags & (DEGEN | ELIMIT)) && fabsf(o.sep) > near_margin;
+            done = (o.flags & CONV) && !(o.flags & (DEGEN | ELIMIT | BUDGET)) && o.certified &&
+                   fabsf(o.sep) > near_margin;
             if (done) o.flags |= WARM;
         }
         if (!done) {
-            o = query(&A, &B, dc, max_iter);
+            o = query_cold(&A, &B, dc, max_iter);
             spent += o.iters + o.epa_iters;
         }
         sep[p] = o.sep;
@@ -704,5 +1046,13 @@ void or_gjk(int npairs, const int* pairs, const float* pos4, const float* quat4,
         normal3[3 * (long)p + 2] = o.n.z;
         flags[p] = o.flags;
         iters[p] = spent;
+        if (small_iters) small_iters[p] = sit;
     }
+}
+
+void or_gjk(int npairs, const int* pairs, const float* pos4, const float* quat4, const int* vstart, const int* nv,
+            const float* verts4, const float* warm, int warm_budget, int max_iter, float near_margin, float* sep,
+            float* normal3, int* flags, int* iters, int nthreads) {
+    or_gjk_ex(npairs, pairs, pos4, quat4, vstart, nv, verts4, warm, warm_budget, max_iter, near_margin, sep, normal3,
+              flags, iters, NULL, nthreads);
 }

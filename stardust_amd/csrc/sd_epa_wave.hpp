@@ -220,8 +220,25 @@ __device__ __forceinline__ bool has_edge(int x, int v, int u) {
     return (ga == v && gb == u) || (gb == v && gc == u) || (gc == v && ga == u);
 }
 
+// EPA's outcome: the closest face of the final polytope (its three points
+// with their keys, in face order) and the interior point that orients it, or
+// none (a degenerate start); flags and iterations.
+struct EpaFace {
+    MPoint q[3];
+    V3 inner;
+    float d;  // plane distance of the face in the polytope
+    bool has_face;
+    int flags;
+    int iters;
+};
+
 __device__ inline void grp_epa(const Shape& A, const HullRegs& hA, const Shape& B, const HullRegs& hB,
-                               const Simplex& s0, int max_iter, GjkOut& o, EpaShared& S, int gl, int base) {
+                               const Simplex& s0, int max_iter, EpaFace& o, EpaShared& S, int gl, int base) {
+    o.has_face = false;
+    o.flags = 0;
+    o.iters = 0;
+    o.d = 0.0f;
+    o.inner = v3(0.0f, 0.0f, 0.0f);
     // grow the simplex to a tetrahedron
     MPoint T0, T1, T2, T3;
     T0.p = s0.w[0].p;
@@ -274,7 +291,6 @@ __device__ inline void grp_epa(const Shape& A, const HullRegs& hA, const Shape& 
     }
     if (np < 4) {
         o.flags |= GJK_HIT | GJK_DEGENERATE;
-        o.sep = 0.0f;
         return;
     }
     const V3 inner = scl(add(add(T0.p, T1.p), add(T2.p, T3.p)), 0.25f);
@@ -305,7 +321,6 @@ __device__ inline void grp_epa(const Shape& A, const HullRegs& hA, const Shape& 
     grp_sync();
     if (nf < 4) {
         o.flags |= GJK_HIT | GJK_DEGENERATE;
-        o.sep = 0.0f;
         return;
     }
     int it = 0;
@@ -459,54 +474,59 @@ __device__ inline void grp_epa(const Shape& A, const HullRegs& hA, const Shape& 
             break;
         }
     }
+    o.iters = it;
     if (nf == 0) {
         o.flags |= GJK_HIT | GJK_DEGENERATE;
-        o.sep = 0.0f;
-        o.epa_iters = it;
         return;
     }
     const int best = grp_closest_face(S, nf, gl);
     const int abc = S.fabc[best];
-    MPoint q0, q1, q2;
     const float4 pa = S.P[abc & 0xff], pb = S.P[(abc >> 8) & 0xff], pc = S.P[(abc >> 16) & 0xff];
-    q0.p = pv(pa);
-    q0.key = __float_as_int(pa.w);
-    q1.p = pv(pb);
-    q1.key = __float_as_int(pb.w);
-    q2.p = pv(pc);
-    q2.key = __float_as_int(pc.w);
-    cmpx3(q0, q1);  // stable for equal keys: the order of a serial insertion sort
-    cmpx3(q1, q2);
-    cmpx3(q0, q1);
-    V3 nrm = cross(sub(q1.p, q0.p), sub(q2.p, q0.p));
-    const float l2 = dot(nrm, nrm);
-    if (dot(nrm, sub(q0.p, inner)) < 0.0f) nrm = neg(nrm);
-    const V3 n = scl(nrm, 1.0f / sqrtf(l2));
-    const float d = dot(n, q0.p);
+    o.q[0].p = pv(pa);
+    o.q[0].key = __float_as_int(pa.w);
+    o.q[1].p = pv(pb);
+    o.q[1].key = __float_as_int(pb.w);
+    o.q[2].p = pv(pc);
+    o.q[2].key = __float_as_int(pc.w);
+    o.inner = inner;
+    o.d = S.fn[best].w;
+    o.has_face = true;
     o.flags |= GJK_HIT | (converged ? GJK_CONVERGED : 0);
     if (!converged && it >= max_iter) o.flags |= GJK_BUDGET;
-    if (d < 0.0f) o.flags |= GJK_DEGENERATE;  // polytope does not enclose the origin
-    o.sep = -d;
-    o.n = n;
-    o.epa_iters = it;
+    if (o.d < 0.0f) o.flags |= GJK_DEGENERATE;  // polytope does not enclose the origin
 }
 
-// EPA after a GJK phase; for an uncertain exit whose EPA finds no enclosing
-// polytope, the uncertified separation (GJK_BUDGET) instead.
-__device__ inline void grp_epa_or_separated(const Shape& A, const HullRegs& hA, const Shape& B, const HullRegs& hB,
-                                            const Simplex& s, bool uncertain, int max_iter, GjkOut& o, EpaShared& S,
-                                            int gl, int base) {
-    grp_epa(A, hA, B, hB, s, max_iter, o, S, gl, base);
-    if (uncertain && (o.flags & GJK_DEGENERATE)) {
-        const int gi = o.iters, ei = o.epa_iters;
-        o = gjk_out_init();
-        o.iters = gi;
-        o.epa_iters = ei;
-        Simplex t = s;
-        bool inside = false;
-        const D3 v = simplex_closest_call(t, &inside);
-        gjk_separated<true>(t, v, false, o);
+// Depth and normal of EPA's closest face: canonical from its closest features,
+// else the face's plane with its points ordered by key. CERT: also evaluate
+// the certificate (o.certified).
+template <bool CERT>
+__device__ __forceinline__ void hit_result(const Shape& A, const Shape& B, MPoint* q, V3 inner, GjkOut& o) {
+    Features f;
+    o.certified = false;
+    if (feature_sets(q, 3, f) && f.na <= 3 && f.nb <= 3) {
+        V3 fa[3], fb[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            fa[k] = k < f.na ? vert_at(A, f.ia[k]) : v3(0.0f, 0.0f, 0.0f);
+            fb[k] = k < f.nb ? vert_at(B, f.ib[k]) : v3(0.0f, 0.0f, 0.0f);
+        }
+        const Canon c = canon_feature(fa, f.na, fb, f.nb, true);
+        if (c.kind != 0) {
+            o.sep = c.sep;
+            o.n = c.n;
+            if (CERT) o.certified = c.ok && certify(A, B, f, c.n);
+            return;
+        }
     }
+    cmpx_key(q[0], q[1]);  // stable for equal keys: the order of a serial insertion sort
+    cmpx_key(q[1], q[2]);
+    cmpx_key(q[0], q[1]);
+    V3 nrm = cross(sub(q[1].p, q[0].p), sub(q[2].p, q[0].p));
+    const float l2 = dot(nrm, nrm);
+    if (dot(nrm, sub(q[0].p, inner)) < 0.0f) nrm = neg(nrm);
+    const V3 n = scl(nrm, 1.0f / sqrtf(l2));
+    o.sep = -dot(n, q[0].p);
+    o.n = n;
 }
 
 }  // namespace sd

@@ -71,61 +71,77 @@ __global__ void k_hull_aabb(const float4* __restrict__ pos, const float4* __rest
 constexpr int AX_BLOCK = 256;
 constexpr int AX_BLOCKS = 256;
 
-// Per-block sums of centre and centre^2 per axis (FP64), grid-stride.
-__global__ __launch_bounds__(AX_BLOCK) void k_axis_partial(const float4* __restrict__ lo4,
-                                                           const float4* __restrict__ hi4, int n,
-                                                           double* __restrict__ part) {
-    __shared__ double red[6][AX_BLOCK];
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = blockIdx.x * AX_BLOCK + threadIdx.x; i < n; i += gridDim.x * AX_BLOCK) {
-        const float4 l = lo4[i], h = hi4[i];
-        const double cx = 0.5 * ((double)l.x + (double)h.x);
-        const double cy = 0.5 * ((double)l.y + (double)h.y);
-        const double cz = 0.5 * ((double)l.z + (double)h.z);
-        acc[0] += cx;
-        acc[1] += cy;
-        acc[2] += cz;
-        acc[3] += cx * cx;
-        acc[4] += cy * cy;
-        acc[5] += cz * cz;
-    }
-    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int s = AX_BLOCK / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+// The sweep axis is the one of largest variance of the box centres, computed
+// in FP32 in a fixed order: centres relative to the first box's centre (which
+// keeps the sums well conditioned), per-thread sums over a grid stride of nb
+// blocks of 256 threads, a tree over each block's 256 sums (halving strides),
+// the block sums added in block order, var = S2 / n - (S1 / n)^2 per axis,
+// the first axis of largest variance. The order is fixed for a given n, so
+// the axis does not depend on the launch.
+__device__ __forceinline__ void box_centre(const float4& l, const float4& h, float& cx, float& cy, float& cz) {
+    cx = 0.5f * (l.x + h.x);
+    cy = 0.5f * (l.y + h.y);
+    cz = 0.5f * (l.z + h.z);
 }
 
-// Sums the partials in block order and picks the axis of largest variance.
-// The partials are staged in LDS by the whole block; one thread per component
-// then adds them in block order.
-__global__ __launch_bounds__(AX_BLOCK) void k_axis_pick(const double* __restrict__ part, int nparts, int n,
-                                                        int* __restrict__ axis) {
-    __shared__ double sp[AX_BLOCKS * 6];
-    __shared__ double sum[6];
-    for (int i = threadIdx.x; i < nparts * 6; i += AX_BLOCK) sp[i] = part[i];
+// Tree sum of red[k][0..AX_BLOCK) into red[k][0] (halving strides).
+__device__ __forceinline__ void axis_tree(float (*red)[AX_BLOCK]) {
     __syncthreads();
-    if (threadIdx.x < 6) {
-        double acc = 0.0;
-        for (int b = 0; b < nparts; ++b) acc += sp[b * 6 + threadIdx.x];
-        sum[threadIdx.x] = acc;
+    for (int h = AX_BLOCK / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
+}
+
+__device__ __forceinline__ int axis_of_sums(const float* s, int n) {
     int best = 0;
-    double bv = -1.0;
+    float bv = -1.0f;
     for (int a = 0; a < 3; ++a) {
-        const double m = sum[a] / (double)n;
-        const double var = sum[3 + a] / (double)n - m * m;
+        const float m = s[a] / (float)n;
+        const float var = s[3 + a] / (float)n - m * m;
         if (var > bv) {
             bv = var;
             best = a;
         }
     }
-    *axis = best;
+    return best;
+}
+
+__global__ __launch_bounds__(AX_BLOCK) void k_axis_partial(const float4* __restrict__ lo4,
+                                                           const float4* __restrict__ hi4, int n,
+                                                           float* __restrict__ part) {
+    __shared__ float red[6][AX_BLOCK];
+    float rx, ry, rz;
+    box_centre(lo4[0], hi4[0], rx, ry, rz);
+    float acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * AX_BLOCK + threadIdx.x; i < n; i += gridDim.x * AX_BLOCK) {
+        float cx, cy, cz;
+        box_centre(lo4[i], hi4[i], cx, cy, cz);
+        const float dx = cx - rx, dy = cy - ry, dz = cz - rz;
+        acc[0] += dx;
+        acc[1] += dy;
+        acc[2] += dz;
+        acc[3] += dx * dx;
+        acc[4] += dy * dy;
+        acc[5] += dz * dz;
+    }
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = acc[k];
+    axis_tree(red);
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Adds the partials in block order and picks the axis.
+__global__ __launch_bounds__(64) void k_axis_pick(const float* __restrict__ part, int nparts, int n,
+                                                  int* __restrict__ axis) {
+    __shared__ float sum[6];
+    if (threadIdx.x < 6) {
+        float acc = 0.0f;
+        for (int b = 0; b < nparts; ++b) acc += part[b * 6 + threadIdx.x];
+        sum[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *axis = axis_of_sums(sum, n);
 }
 
 // Order-preserving map of a float to u32 (negative values flip all bits).
@@ -325,9 +341,10 @@ __global__ __launch_bounds__(256) void k_sap_overflow(const float4* __restrict__
 // lower extent on the scene's axis, hull index) packed into 64 bits, so one
 // radix sort orders every scene exactly as sd_sap_pairs orders it alone, and
 // the sweep stops at the end of the scene. The per-scene axis is reduced in the
-// same FP64 order as k_axis_partial + k_axis_pick use for a scene of up to
-// AX_BLOCK * AX_BLOCKS hulls (256-element chunk sums by a tree, chunk sums in
-// order), so it is the axis sd_sap_pairs picks for that scene.
+// same FP32 order as k_axis_partial + k_axis_pick use for a scene of up to
+// AX_BLOCK * AX_BLOCKS hulls (centres relative to the scene's first box,
+// 256-element chunk sums by a tree, chunk sums in order), so it is the axis
+// sd_sap_pairs picks for that scene.
 
 constexpr int SAPB_MAX_SCENE = AX_BLOCK * AX_BLOCKS;
 
@@ -335,7 +352,7 @@ __global__ __launch_bounds__(AX_BLOCK) void k_axis_scenes(const float4* __restri
                                                           const float4* __restrict__ hi4, int n,
                                                           const int* __restrict__ scene_first, int nscenes,
                                                           int* __restrict__ axis, int* __restrict__ error) {
-    __shared__ double red[6][AX_BLOCK];
+    __shared__ float red[6][AX_BLOCK];
     const int sc = blockIdx.x;
     const int b0 = scene_first[sc], b1 = scene_first[sc + 1];
     const bool bad = b0 < 0 || b1 > n || b1 < b0 || b1 - b0 > SAPB_MAX_SCENE;
@@ -346,46 +363,34 @@ __global__ __launch_bounds__(AX_BLOCK) void k_axis_scenes(const float4* __restri
         }
         return;
     }
-    double s[6] = {0, 0, 0, 0, 0, 0};  // meaningful in thread 0
+    if (b1 == b0) {
+        if (threadIdx.x == 0) axis[sc] = 0;
+        return;
+    }
+    float rx, ry, rz;
+    box_centre(lo4[b0], hi4[b0], rx, ry, rz);
+    float s[6] = {0, 0, 0, 0, 0, 0};  // meaningful in thread 0
     for (int c0 = b0; c0 < b1; c0 += AX_BLOCK) {
-        double acc[6] = {0, 0, 0, 0, 0, 0};
+        float acc[6] = {0, 0, 0, 0, 0, 0};
         const int i = c0 + (int)threadIdx.x;
         if (i < b1) {
-            const float4 l = lo4[i], h = hi4[i];
-            const double cx = 0.5 * ((double)l.x + (double)h.x);
-            const double cy = 0.5 * ((double)l.y + (double)h.y);
-            const double cz = 0.5 * ((double)l.z + (double)h.z);
-            acc[0] += cx;
-            acc[1] += cy;
-            acc[2] += cz;
-            acc[3] += cx * cx;
-            acc[4] += cy * cy;
-            acc[5] += cz * cz;
+            float cx, cy, cz;
+            box_centre(lo4[i], hi4[i], cx, cy, cz);
+            const float dx = cx - rx, dy = cy - ry, dz = cz - rz;
+            acc[0] += dx;
+            acc[1] += dy;
+            acc[2] += dz;
+            acc[3] += dx * dx;
+            acc[4] += dy * dy;
+            acc[5] += dz * dz;
         }
         for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = acc[k];
-        __syncthreads();
-        for (int h = AX_BLOCK / 2; h > 0; h >>= 1) {
-            if (threadIdx.x < h)
-                for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
-            __syncthreads();
-        }
+        axis_tree(red);
         if (threadIdx.x == 0)
             for (int k = 0; k < 6; ++k) s[k] += red[k][0];
         __syncthreads();
     }
-    if (threadIdx.x != 0) return;
-    const int m = b1 - b0;
-    int best = 0;
-    double bv = -1.0;
-    for (int a = 0; a < 3; ++a) {
-        const double mean = s[a] / (double)m;
-        const double var = s[3 + a] / (double)m - mean * mean;
-        if (var > bv) {
-            bv = var;
-            best = a;
-        }
-    }
-    axis[sc] = best;
+    if (threadIdx.x == 0) axis[sc] = axis_of_sums(s, b1 - b0);
 }
 
 __global__ void k_sapb_keys(const float4* __restrict__ lo4, int n, const int* __restrict__ scene_of, int nscenes,
@@ -452,15 +457,22 @@ __device__ __forceinline__ void make_shapes(const HullSet& h, int ia, int ib, Sh
 //   k_epa_wave   one 16-lane group per listed pair: EPA from that simplex
 //                with the polytope in LDS; a warm-started entry whose result
 //                is rejected goes to a retry list, whose pairs get the cold
-//                query (k_gjk_phase<true>, then k_epa_wave again).
-// warm (nullable): per-pair initial direction (A -> B). A warm-started result
-// that is not converged within warm_budget iterations, that EPA could not
-// finish cleanly, or that lies within near_margin of contact, is redone cold
-// with max_iter iterations.
+//                query (k_gjk_phase<true, false>, then k_epa_wave again).
+// warm (nullable): per-pair predicted direction (A -> B). A warm-started query
+// selects the WARM_K vertices of each hull furthest along it (one pass over
+// both hulls), solves GJK over those seed sets, continues on the full hulls
+// within warm_budget iterations, and is accepted only when it converged, its
+// closest features carry the margin certificate (one more pass), and it lies
+// beyond near_margin of contact; otherwise the pair is redone cold with
+// max_iter iterations. Pairs involving a sphere shape are always cold.
 struct EpaEntry {
     int4 meta;     // pair, EPA iteration limit, iterations so far,
                    // simplex size | mode << 8 (0 cold, 1 warm) | uncertain << 16
     float4 w[4];   // simplex points (x, y, z, key bits)
+    // written by k_epa_wave, read by k_epa_finish:
+    float4 f[3];   // closest face of the final polytope (x, y, z, key bits)
+    float4 inner;  // interior point of the polytope (x, y, z), face plane distance
+    int4 res;      // flags, EPA iterations, has_face, 0
 };
 
 __device__ __forceinline__ bool pair_is_bad(const HullSet& h, int ia, int ib, int n_hulls, int n_verts) {
@@ -492,13 +504,22 @@ __device__ __forceinline__ void push_epa(EpaEntry* __restrict__ list, int* __res
     list[k] = e;
 }
 
+// A warm result is kept when it converged cleanly, its features are certified
+// unique, and it is not within near_margin of contact.
+__device__ __forceinline__ bool warm_accepted(const GjkOut& o, float near_margin) {
+    return (o.flags & GJK_CONVERGED) && !(o.flags & (GJK_DEGENERATE | GJK_EPA_LIMIT | GJK_BUDGET)) && o.certified &&
+           fabsf(o.sep) > near_margin;
+}
+
 // GJK phase of every pair (RETRY = false, one thread per pair: warm-started
-// when warm directions are given, else cold), or the cold query of the
+// with WARM = true, else cold), or the cold query of the
 // warm-started pairs whose result was rejected (RETRY = true, threads striding
 // over the retry list of (pair, iterations so far)). Hits go to the EPA list;
-// a rejected warm separation goes to the retry list.
-template <bool RETRY>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_gjk_phase(const int* __restrict__ pairs, int npairs, HullSet h, int n_hulls,
+// a rejected warm separation goes to the retry list. Iterations count support
+// passes over the full hulls: a warm query adds one for the seed selection and
+// one for the certificate.
+template <bool RETRY, bool WARM>
+__global__ __launch_bounds__(64) void k_gjk_phase(const int* __restrict__ pairs, int npairs, HullSet h, int n_hulls,
                                                   int n_verts, const float* __restrict__ warm, int warm_budget,
                                                   int max_iter, float near_margin, float* __restrict__ sep,
                                                   float* __restrict__ normal, int* __restrict__ flags,
@@ -518,28 +539,56 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         Shape A, B;
         V3 dc;
         make_shapes(h, ia, ib, A, B, dc);
-        const bool warm_try = !RETRY && warm != nullptr;
+        const bool warm_try = WARM && !RETRY && A.nv > 0 && B.nv > 0;
         const int spent = RETRY ? retry[t].y : 0;
-        V3 dir = dc;
-        int limit = max_iter;
-        if (warm_try) {
-            dir = v3(warm[3 * (size_t)p], warm[3 * (size_t)p + 1], warm[3 * (size_t)p + 2]);
-            limit = warm_budget < max_iter ? warm_budget : max_iter;
-        }
         Simplex s;
-        D3 v;
-        bool hit, converged, uncertain;
-        int it = 0;
-        gjk_phase(A, B, dir, limit, s, v, &hit, &converged, &uncertain, &it);
-        if (hit) {
-            push_epa(list, count, p, warm_try ? 1 : 0, uncertain, limit, spent + it, s);
+        V3 v;
+        GjkState g = gjk_state();
+        int limit = max_iter, extra = 0;
+        bool full = true;
+        if (WARM && warm_try) {
+            // seed sets along the prediction u, the seed problem from the seed
+            // support along u; a hit of the seed problem is a hit of the hulls
+            // (the seed sets are subsets), else GJK continues on the full hulls
+            limit = warm_budget < max_iter ? warm_budget : max_iter;
+            extra = 2;
+            V3 u = v3(warm[3 * (size_t)p], warm[3 * (size_t)p + 1], warm[3 * (size_t)p + 2]);
+            if (dot(u, u) == 0.0f) u = v3(1.0f, 0.0f, 0.0f);
+            SeedSet sa, sb;
+            seed_select(A, u, sa);
+            seed_select(B, neg(u), sb);
+            s.w[0] = seed_support(sa, sb, u);
+            s.n = 1;
+            v = s.w[0].p;
+            GjkState gs = gjk_state();
+            gjk_loop<true>(A, B, &sa, &sb, s, v, SMALL_MAX_ITER, gs);
+            if (gs.hit && !gs.uncertain) {
+                g.hit = true;
+                full = false;
+            }
+        } else {
+            V3 dir = dc;
+            if (dot(dir, dir) == 0.0f) dir = v3(1.0f, 0.0f, 0.0f);
+            s.w[0] = support_M(A, B, dir);
+            s.n = 1;
+            v = s.w[0].p;
+        }
+        if (full) gjk_loop<false>(A, B, nullptr, nullptr, s, v, limit, g);
+        const int it = g.it + extra;
+        if (g.hit) {
+            push_epa(list, count, p, warm_try ? 1 : 0, g.uncertain, limit, spent + it, s);
             continue;
         }
         GjkOut o = gjk_out_init();
-        gjk_separated(s, v, converged, o);
-        if (!warm_try) write_out(p, o, o.flags, spent + it, sep, normal, flags, iters);
-        else if (converged && fabsf(o.sep) > near_margin) write_out(p, o, o.flags | GJK_WARM_USED, it, sep, normal, flags, iters);
-        else retry[atomicAdd(retry_count, 1)] = make_int2(p, it);
+        o.flags = g.converged ? GJK_CONVERGED : GJK_BUDGET;
+        if (WARM && warm_try) {
+            separated_result<true>(A, B, s, o);
+            if (warm_accepted(o, near_margin)) write_out(p, o, o.flags | GJK_WARM_USED, it, sep, normal, flags, iters);
+            else retry[atomicAdd(retry_count, 1)] = make_int2(p, it);
+        } else {
+            separated_result<false>(A, B, s, o);
+            write_out(p, o, o.flags, spent + it, sep, normal, flags, iters);
+        }
     }
 }
 
@@ -549,13 +598,11 @@ constexpr int GJK_RETRY_MAX_BLOCKS = 1024;
 
 // One 16-lane group per listed pair, persistent over the list (its length is
 // only known on the device); the grid is the number of blocks the device holds
-// at once. The polytope is in the group's LDS region. A warm-started entry
-// whose EPA result is rejected goes to the retry list with the iterations
-// spent so far.
-__global__ __launch_bounds__(64 * EPA_WAVES_PER_BLOCK) void k_epa_wave(
-    const int* __restrict__ pairs, HullSet h, float near_margin, float* __restrict__ sep, float* __restrict__ normal,
-    int* __restrict__ flags, int* __restrict__ iters, const EpaEntry* __restrict__ list,
-    const int* __restrict__ count, int2* __restrict__ retry, int* __restrict__ retry_count) {
+// at once. The polytope is in the group's LDS region. The group stores EPA's
+// final face in the entry; k_epa_finish turns it into the result.
+__global__ __launch_bounds__(64 * EPA_WAVES_PER_BLOCK) void k_epa_wave(const int* __restrict__ pairs, HullSet h,
+                                                                       EpaEntry* __restrict__ list,
+                                                                       const int* __restrict__ count) {
     __shared__ EpaShared sh[EPA_GROUPS_PER_BLOCK];
     const int gl = threadIdx.x & (EPA_G - 1);
     const int base = threadIdx.x & (64 - EPA_G);  // first lane of the group within its wave
@@ -563,33 +610,91 @@ __global__ __launch_bounds__(64 * EPA_WAVES_PER_BLOCK) void k_epa_wave(
     EpaShared& S = sh[grp];
     const int n = *count;
     for (int k = blockIdx.x * EPA_GROUPS_PER_BLOCK + grp; k < n; k += gridDim.x * EPA_GROUPS_PER_BLOCK) {
-        const EpaEntry e = list[k];
-        const int p = e.meta.x;
+        const int4 meta = list[k].meta;
+        const int p = meta.x;
         const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
         Shape A, B;
         V3 dc;
         make_shapes(h, ia, ib, A, B, dc);
         const HullRegs hA = load_hull_regs(A, gl), hB = load_hull_regs(B, gl);
         Simplex s;
-        s.n = e.meta.w & 0xff;
-        const bool uncertain = ((e.meta.w >> 16) & 1) != 0;
+        s.n = meta.w & 0xff;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            s.w[i].p = v3(e.w[i].x, e.w[i].y, e.w[i].z);
-            s.w[i].key = __float_as_int(e.w[i].w);
+            const float4 w = list[k].w[i];
+            s.w[i].p = v3(w.x, w.y, w.z);
+            s.w[i].key = __float_as_int(w.w);
         }
-        GjkOut o = gjk_out_init();
-        grp_epa_or_separated(A, hA, B, hB, s, uncertain, e.meta.y, o, S, gl, base);
-        const int spent = e.meta.z + o.epa_iters;
-        const bool warm_started = ((e.meta.w >> 8) & 0xff) == 1;
-        const bool accept = !warm_started || ((o.flags & GJK_CONVERGED) &&
-                                              !(o.flags & (GJK_DEGENERATE | GJK_EPA_LIMIT)) &&
-                                              fabsf(o.sep) > near_margin);
+        EpaFace r;
+        grp_epa(A, hA, B, hB, s, meta.y, r, S, gl, base);
         if (gl == 0) {
-            if (accept) write_out(p, o, o.flags | (warm_started ? GJK_WARM_USED : 0), spent, sep, normal, flags, iters);
-            else retry[atomicAdd(retry_count, 1)] = make_int2(p, spent);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                list[k].f[i] = make_float4(r.q[i].p.x, r.q[i].p.y, r.q[i].p.z, __int_as_float(r.q[i].key));
+            list[k].inner = make_float4(r.inner.x, r.inner.y, r.inner.z, r.d);
+            list[k].res = make_int4(r.flags, r.iters, r.has_face ? 1 : 0, 0);
         }
         grp_sync();  // the next entry reuses this group's LDS region
+    }
+}
+
+// Result of every listed pair from EPA's face (one thread per entry): depth
+// and normal of the closest features, the certificate of a warm-started entry;
+// an uncertain GJK exit whose EPA found no enclosing polytope reports the
+// uncertified separation of its GJK simplex (GJK_BUDGET). A warm-started
+// entry whose result is rejected goes to the retry list with the iterations
+// spent so far.
+__global__ __launch_bounds__(64) void k_epa_finish(const int* __restrict__ pairs, HullSet h, float near_margin,
+                                                   float* __restrict__ sep, float* __restrict__ normal,
+                                                   int* __restrict__ flags, int* __restrict__ iters,
+                                                   const EpaEntry* __restrict__ list, const int* __restrict__ count,
+                                                   int2* __restrict__ retry, int* __restrict__ retry_count) {
+    const int n = *count;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const EpaEntry& e = list[k];
+        const int4 meta = e.meta, res = e.res;
+        const int p = meta.x;
+        const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
+        Shape A, B;
+        V3 dc;
+        make_shapes(h, ia, ib, A, B, dc);
+        const bool warm_started = ((meta.w >> 8) & 0xff) == 1;
+        const bool uncertain = ((meta.w >> 16) & 1) != 0;
+        GjkOut o = gjk_out_init();
+        o.flags = res.x;
+        o.epa_iters = res.y;
+        if (res.z) {
+            MPoint q[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const float4 f = e.f[i];
+                q[i].p = v3(f.x, f.y, f.z);
+                q[i].key = __float_as_int(f.w);
+            }
+            const float4 in = e.inner;
+            if (warm_started) hit_result<true>(A, B, q, v3(in.x, in.y, in.z), o);
+            else hit_result<false>(A, B, q, v3(in.x, in.y, in.z), o);
+        }
+        if (uncertain && (o.flags & GJK_DEGENERATE)) {
+            Simplex s;
+            s.n = meta.w & 0xff;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float4 w = e.w[i];
+                s.w[i].p = v3(w.x, w.y, w.z);
+                s.w[i].key = __float_as_int(w.w);
+            }
+            const int ei = o.epa_iters;
+            o = gjk_out_init();
+            o.epa_iters = ei;
+            o.flags = GJK_BUDGET;  // never accepted for a warm query: no certificate needed
+            separated_result<false, true>(A, B, s, o);
+        }
+        const int spent = meta.z + o.epa_iters;
+        if (!warm_started || warm_accepted(o, near_margin))
+            write_out(p, o, o.flags | (warm_started ? GJK_WARM_USED : 0), spent, sep, normal, flags, iters);
+        else
+            retry[atomicAdd(retry_count, 1)] = make_int2(p, spent);
     }
 }
 
@@ -648,7 +753,7 @@ extern "C" sd_status sd_sap_scratch_bytes(int n, size_t* bytes) {
              + 2 * pad256((nn + 2) * sizeof(int))        // counts, offsets (+ total)
              + pad256(nn * SAP_SLOTS * sizeof(int))      // first partners of each object
              + pad256((nn + 1) * sizeof(int))            // objects with more partners (+ count)
-             + pad256((size_t)AX_BLOCKS * 6 * sizeof(double)) + pad256(64)  // axis partials, axis
+             + pad256((size_t)AX_BLOCKS * 6 * sizeof(float)) + pad256(64)  // axis partials, axis
              + pad256(radix_temp_bytes(n)) + pad256(scan_temp_bytes(n));
     return SD_OK;
 }
@@ -683,7 +788,7 @@ extern "C" sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int
     int* slots = (int*)carve(p, (size_t)n * SAP_SLOTS * sizeof(int));
     int* ovf = (int*)carve(p, (size_t)(n + 1) * sizeof(int));
     int* ovf_count = ovf + n;
-    double* part = (double*)carve(p, (size_t)AX_BLOCKS * 6 * sizeof(double));
+    float* part = (float*)carve(p, (size_t)AX_BLOCKS * 6 * sizeof(float));
     int* axis = (int*)carve(p, 64);
     void* sort_tmp = (void*)carve(p, radix_temp_bytes(n));
     void* scan_tmp = (void*)carve(p, scan_temp_bytes(n));
@@ -694,7 +799,7 @@ extern "C" sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int
     const int nb = div_up(n, AX_BLOCK) < AX_BLOCKS ? div_up(n, AX_BLOCK) : AX_BLOCKS;
     hipLaunchKernelGGL(k_axis_partial, dim3(nb), dim3(AX_BLOCK), 0, s, L, H, n, part);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_axis_pick, dim3(1), dim3(AX_BLOCK), 0, s, part, nb, n, axis);
+    hipLaunchKernelGGL(k_axis_pick, dim3(1), dim3(64), 0, s, part, nb, n, axis);
     SD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sap_keys, dim3(div_up(n, 256)), dim3(256), 0, s, L, n, axis, keys, vals);
     SD_LAUNCH_CHECK();
@@ -851,19 +956,29 @@ extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, co
     if (st != SD_OK) return st;
     const int epa_blocks = std::min(div_up(n_pairs, EPA_GROUPS_PER_BLOCK), epa_resident_blocks());
     const dim3 epa_block(64 * EPA_WAVES_PER_BLOCK);
-    hipLaunchKernelGGL(k_gjk_phase<false>, dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h, n_hulls,
-                       n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters, list, count,
-                       retry, count + 1);
+    if (warm_dir != nullptr)
+        hipLaunchKernelGGL((k_gjk_phase<false, true>), dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h,
+                           n_hulls, n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters,
+                           list, count, retry, count + 1);
+    else
+        hipLaunchKernelGGL((k_gjk_phase<false, false>), dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h,
+                           n_hulls, n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters,
+                           list, count, retry, count + 1);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, near_margin, sep, normal3, flags,
+    const int fin_blocks = std::min(div_up(n_pairs, 64), GJK_RETRY_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)count);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_epa_finish, dim3(fin_blocks), dim3(64), 0, s, pairs, h, near_margin, sep, normal3, flags,
                        iters, (const EpaEntry*)list, (const int*)count, retry, count + 1);
     SD_LAUNCH_CHECK();
     if (warm_dir == nullptr) return SD_OK;  // cold entries are never rejected
-    hipLaunchKernelGGL(k_gjk_phase<true>, dim3(std::min(div_up(n_pairs, 64), GJK_RETRY_MAX_BLOCKS)), dim3(64), 0, s,
-                       pairs, n_pairs, h, n_hulls, n_verts, nullptr, 1, max_iter, near_margin, sep, normal3, flags,
-                       iters, list, count + 2, retry, count + 1);
+    hipLaunchKernelGGL((k_gjk_phase<true, false>), dim3(fin_blocks), dim3(64), 0, s, pairs, n_pairs, h, n_hulls,
+                       n_verts, nullptr, 1, max_iter, near_margin, sep, normal3, flags, iters, list, count + 2, retry,
+                       count + 1);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, near_margin, sep, normal3, flags,
+    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)(count + 2));
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_epa_finish, dim3(fin_blocks), dim3(64), 0, s, pairs, h, near_margin, sep, normal3, flags,
                        iters, (const EpaEntry*)list, (const int*)(count + 2), retry, count + 1);
     SD_LAUNCH_CHECK();
     return SD_OK;

@@ -83,8 +83,10 @@ class WarmStartModel:
         L.synchronize()
         return self.g.download(np.float32, self.np), float(self.loss.download(np.float32, 1)[0])
 
-    def train(self, data, epochs=1, batch=8192, lr=3e-3, seed=1, log=None):
-        """Adam over shuffled mini-batches; returns the mean loss of each epoch."""
+    def train(self, data, epochs=1, batch=8192, lr=3e-3, seed=1, log=None, lr_final=None):
+        """Adam over shuffled mini-batches; returns the mean loss of each epoch.
+        lr_final: the learning rate decays geometrically from lr to lr_final over
+        the epochs (constant when None)."""
         nb = C.c_size_t()
         L.call("sd_mlp_train_scratch_bytes", batch, C.byref(nb))
         scratch = L.DeviceBuffer(nb.value)
@@ -93,6 +95,7 @@ class WarmStartModel:
         history = []
         for ep in range(epochs):
             t0 = time.perf_counter()
+            lr_ep = lr if lr_final is None or epochs < 2 else lr * (lr_final / lr) ** (ep / (epochs - 1))
             idx.upload(rng.permutation(data.n).astype(np.int32))
             losses = []
             for b0 in range(0, data.n - batch + 1, batch):
@@ -100,7 +103,7 @@ class WarmStartModel:
                 L.call("sd_mlp_forward_backward", data.X.ptr, data.T.ptr, sub, batch, self.w.ptr, self.g.ptr,
                        self.loss.ptr, scratch.ptr, nb.value, None)
                 self.step += 1
-                L.call("sd_adam_step", self.w.ptr, self.g.ptr, self.m.ptr, self.v.ptr, self.np, lr, 0.9, 0.999,
+                L.call("sd_adam_step", self.w.ptr, self.g.ptr, self.m.ptr, self.v.ptr, self.np, lr_ep, 0.9, 0.999,
                        1e-8, self.step, None)
                 if (b0 // batch) % 64 == 0:
                     L.synchronize()

@@ -15,6 +15,13 @@ from stardust_amd import warmstart as WS
 pytestmark = pytest.mark.gpu
 
 
+def rel_diff(a, b):
+    """|a - b| / |b| per element; 0 where the two are equal (both zero included)."""
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return np.where(a == b, 0.0, np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
 def say(*a):
     print("[gpu-case]", *a, flush=True)
     sys.stdout.flush()
@@ -95,18 +102,24 @@ def test_bf16_mfma_inference_matches_emulation():
         assert (got * full).sum(1).mean() > 0.99
 
 
-def test_trained_network_reduces_gjk_iterations():
-    say("mlp train + warm gjk")
-    data = WS.PairData(600000)
+def test_trained_network_reduces_gjk_iterations(oracle):
+    """The trained network, as the warm start of full-size configuration-4
+    queries (131,072 hulls, 445,828 pairs): identical verdicts and depths within
+    1e-6 relative of plain cold GJK on every pair, bit-identical to the
+    oracle's warm query from the same directions, and fewer iterations than
+    plain GJK. The network's own share of the gain is reported against the
+    same seeded query started from the centre difference."""
+    say("mlp train + warm gjk at full C4")
+    data = WS.PairData(3_000_000)
     seed = 0x5EED1000
-    while data.n < 400000:
-        data.append_scene(H.generate_hulls(seed, 40000))
+    while data.n < 2_500_000:
+        data.append_scene(H.generate_hulls(seed, 131072))
         seed += 1
     model = WS.WarmStartModel(seed=11)
-    hist = model.train(data, epochs=3, batch=8192, lr=3e-3, log=lambda *a: say(*a))
+    hist = model.train(data, epochs=8, batch=8192, lr=3e-3, lr_final=3e-4, log=lambda *a: say(*a))
     assert hist[-1] < hist[0] and hist[-1] < 0.25, hist
-    # held-out scene
-    h = H.generate_hulls(0x5EED2000, 40000)
+    # held-out scene: the full configuration-4 set
+    h = H.hulls_c4()
     d = H.DeviceHulls(h)
     d.aabbs()
     pairs, npairs, _ = d.sap()
@@ -114,21 +127,37 @@ def test_trained_network_reduces_gjk_iterations():
     warm = model.infer(X, npairs, pairs=pairs, quat=d.buf["quat4"])
     cold = d.download_gjk(d.gjk(pairs, npairs), npairs)
     ws = d.download_gjk(d.gjk(pairs, npairs, warm=warm, warm_budget=16), npairs)
-    ok = ((cold["flags"] | ws["flags"]) & L.GJK_BUDGET) == 0
-    assert np.array_equal(cold["flags"][ok] & 1, ws["flags"][ok] & 1)
-    diff = np.abs(ws["sep"][ok] - cold["sep"][ok]) / np.maximum(np.abs(cold["sep"][ok]), 1e-3)
-    assert diff.max() <= 1e-5
+    L.synchronize()
+    wdir = warm.download(np.float32, 3 * npairs).reshape(-1, 3)
+    hp = pairs.download(np.int32, 2 * npairs).reshape(-1, 2)
+    # every verdict identical, every depth within 1e-6 relative (no floor)
+    assert np.array_equal(cold["flags"] & 1, ws["flags"] & 1)
+    diff = rel_diff(ws["sep"], cold["sep"])
+    assert diff.max() <= 1e-6, diff.max()
+    # the GPU's warm queries equal the oracle's from the same directions
+    exp = oracle.gjk(h, hp, warm=wdir, warm_budget=16, nthreads=16)
+    assert np.array_equal(ws["flags"], exp["flags"]) and np.array_equal(ws["iters"], exp["iters"])
+    assert np.array_equal(ws["sep"].view(np.uint32), exp["sep"].view(np.uint32))
     sep = (cold["flags"] & 1) == 0
-    say("iterations cold %.3f warm %.3f (separated: %.3f -> %.3f), warm used %.3f" % (
-        cold["iters"].mean(), ws["iters"].mean(), cold["iters"][sep].mean(), ws["iters"][sep].mean(),
-        float(((ws["flags"] & L.GJK_WARM_USED) != 0).mean())))
+    used = float(((ws["flags"] & L.GJK_WARM_USED) != 0).mean())
+    say("pairs %d: iterations cold %.3f warm %.3f (separated: %.3f -> %.3f), warm kept %.4f, bit-identical %.5f, "
+        "max rel diff %.2e" % (npairs, cold["iters"].mean(), ws["iters"].mean(), cold["iters"][sep].mean(),
+                               ws["iters"][sep].mean(), used, float((ws["sep"] == cold["sep"]).mean()), diff.max()))
     say("cold histogram", np.bincount(cold["iters"])[:30].tolist())
     say("warm histogram", np.bincount(ws["iters"])[:30].tolist())
-    # the learned start may not cost more than 2 % of the cold iterations
-    assert ws["iters"].mean() <= 1.02 * cold["iters"].mean()
-    # the mechanism itself: starting from the converged direction saves iterations
-    exact = L.DeviceBuffer.from_array(np.ascontiguousarray(cold["normal"]))
-    ex = d.download_gjk(d.gjk(pairs, npairs, warm=exact, warm_budget=16), npairs)
-    say("iterations with the converged direction as the start: %.3f (separated %.3f)" % (
-        ex["iters"].mean(), ex["iters"][sep].mean()))
-    assert ex["iters"][sep].mean() < 0.95 * cold["iters"][sep].mean()
+    assert used > 0.95
+    assert ws["iters"].mean() < 0.85 * cold["iters"].mean()
+    assert ws["iters"][sep].mean() < 0.75 * cold["iters"][sep].mean()
+    # what the network adds over its input pose: its directions are closer to the
+    # converged normals than the centre difference is
+    dc = (h["pos4"][hp[:, 1], :3] - h["pos4"][hp[:, 0], :3]).astype(np.float64)
+    dc /= np.linalg.norm(dc, axis=1, keepdims=True)
+    ang_net = np.degrees(np.arccos(np.clip((wdir * cold["normal"]).sum(1), -1, 1)))
+    ang_dc = np.degrees(np.arccos(np.clip((dc * cold["normal"]).sum(1), -1, 1)))
+    say("mean angle to the converged normal: network %.2f deg, centre difference %.2f deg" % (ang_net.mean(),
+                                                                                                ang_dc.mean()))
+    assert ang_net.mean() < ang_dc.mean()
+    cdir = L.DeviceBuffer.from_array(np.ascontiguousarray(dc.astype(np.float32)))
+    cs = d.download_gjk(d.gjk(pairs, npairs, warm=cdir, warm_budget=16), npairs)
+    say("same seeded query from the centre difference: iterations %.3f (network %.3f)" % (cs["iters"].mean(),
+                                                                                          ws["iters"].mean()))
