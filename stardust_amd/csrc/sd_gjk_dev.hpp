@@ -115,7 +115,7 @@ __device__ __forceinline__ V3 vert_at(const Shape& s, int k) {
 // loads are in flight together; indices past the last vertex read the last
 // vertex again, which never wins (ties keep the first index).
 constexpr int SUPPORT_BATCH = 16;
-constexpr int SCAN_BATCH = 8;  // the seed selection and the certificate scan (more state per vertex)
+constexpr int SCAN_BATCH = 8;  // the seed selection and the certificate scan
 __device__ __forceinline__ int support_index(const Shape& s, V3 u, float4& bp) {
     const V3 l = rot_tmul(s.R, u);
     int best = 0;

@@ -131,13 +131,18 @@ __global__ __launch_bounds__(AX_BLOCK) void k_axis_partial(const float4* __restr
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Adds the partials in block order and picks the axis.
-__global__ __launch_bounds__(64) void k_axis_pick(const float* __restrict__ part, int nparts, int n,
-                                                  int* __restrict__ axis) {
+// Adds the partials in block order and picks the axis. The partials are
+// staged in LDS by the whole block first (one coalesced read), so the six
+// serial sums read LDS, not a chain of dependent global loads.
+__global__ __launch_bounds__(AX_BLOCK) void k_axis_pick(const float* __restrict__ part, int nparts, int n,
+                                                        int* __restrict__ axis) {
+    __shared__ float sp[AX_BLOCKS * 6];
     __shared__ float sum[6];
+    for (int i = threadIdx.x; i < nparts * 6; i += AX_BLOCK) sp[i] = part[i];
+    __syncthreads();
     if (threadIdx.x < 6) {
         float acc = 0.0f;
-        for (int b = 0; b < nparts; ++b) acc += part[b * 6 + threadIdx.x];
+        for (int b = 0; b < nparts; ++b) acc += sp[b * 6 + threadIdx.x];
         sum[threadIdx.x] = acc;
     }
     __syncthreads();
@@ -511,6 +516,51 @@ __device__ __forceinline__ bool warm_accepted(const GjkOut& o, float near_margin
            fabsf(o.sep) > near_margin;
 }
 
+// The seed problem of a warm-started pair, solved by k_gjk_seed and continued
+// on the full hulls by k_gjk_phase<false, true>: the simplex (a seed hit may
+// hold four points), its closest point, its size and whether it is a hit.
+struct SeedState {
+    float4 w[4];  // simplex points (x, y, z, key bits)
+    float4 v;     // closest point (x, y, z); w: simplex size | hit << 8 (int bits)
+};
+
+// Seed problem of every warm-started pair (one thread per pair): the WARM_K
+// vertices of each hull furthest along the prediction u (a pass over both
+// hulls), then GJK over those seed sets from their support along u. A hit of
+// the seed problem is a hit of the hulls (the seed sets are subsets). Pairs
+// with a sphere shape or invalid ids are not seeded (the continuation kernel
+// does not read their state).
+__global__ __launch_bounds__(64) void k_gjk_seed(const int* __restrict__ pairs, int npairs, HullSet h, int n_hulls,
+                                                 int n_verts, const float* __restrict__ warm,
+                                                 SeedState* __restrict__ state) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
+    if (pair_is_bad(h, ia, ib, n_hulls, n_verts)) return;
+    Shape A, B;
+    V3 dc;
+    make_shapes(h, ia, ib, A, B, dc);
+    if (A.nv == 0 || B.nv == 0) return;
+    V3 u = v3(warm[3 * (size_t)p], warm[3 * (size_t)p + 1], warm[3 * (size_t)p + 2]);
+    if (dot(u, u) == 0.0f) u = v3(1.0f, 0.0f, 0.0f);
+    SeedSet sa, sb;
+    seed_select(A, u, sa);
+    seed_select(B, neg(u), sb);
+    Simplex s;
+    s.w[0] = seed_support(sa, sb, u);
+    s.n = 1;
+    V3 v = s.w[0].p;
+    GjkState gs = gjk_state();
+    gjk_loop<true>(A, B, &sa, &sb, s, v, SMALL_MAX_ITER, gs);
+    const bool hit = gs.hit && !gs.uncertain;
+    SeedState st;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        st.w[i] = make_float4(s.w[i].p.x, s.w[i].p.y, s.w[i].p.z, __int_as_float(i < s.n ? s.w[i].key : -1));
+    st.v = make_float4(v.x, v.y, v.z, __int_as_float(s.n | (hit ? 1 << 8 : 0)));
+    state[p] = st;
+}
+
 // GJK phase of every pair (RETRY = false, one thread per pair: warm-started
 // with WARM = true, else cold), or the cold query of the
 // warm-started pairs whose result was rejected (RETRY = true, threads striding
@@ -525,7 +575,7 @@ __global__ __launch_bounds__(64) void k_gjk_phase(const int* __restrict__ pairs,
                                                   float* __restrict__ normal, int* __restrict__ flags,
                                                   int* __restrict__ iters, EpaEntry* __restrict__ list,
                                                   int* __restrict__ count, int2* __restrict__ retry,
-                                                  int* __restrict__ retry_count) {
+                                                  int* __restrict__ retry_count, const SeedState* __restrict__ seed) {
     const int n = RETRY ? *retry_count : npairs;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += RETRY ? gridDim.x * blockDim.x : n) {
         const int p = RETRY ? retry[t].x : t;
@@ -547,22 +597,20 @@ __global__ __launch_bounds__(64) void k_gjk_phase(const int* __restrict__ pairs,
         int limit = max_iter, extra = 0;
         bool full = true;
         if (WARM && warm_try) {
-            // seed sets along the prediction u, the seed problem from the seed
-            // support along u; a hit of the seed problem is a hit of the hulls
-            // (the seed sets are subsets), else GJK continues on the full hulls
+            // continue the seed problem of k_gjk_seed on the full hulls; a seed
+            // hit goes to EPA as it is
             limit = warm_budget < max_iter ? warm_budget : max_iter;
             extra = 2;
-            V3 u = v3(warm[3 * (size_t)p], warm[3 * (size_t)p + 1], warm[3 * (size_t)p + 2]);
-            if (dot(u, u) == 0.0f) u = v3(1.0f, 0.0f, 0.0f);
-            SeedSet sa, sb;
-            seed_select(A, u, sa);
-            seed_select(B, neg(u), sb);
-            s.w[0] = seed_support(sa, sb, u);
-            s.n = 1;
-            v = s.w[0].p;
-            GjkState gs = gjk_state();
-            gjk_loop<true>(A, B, &sa, &sb, s, v, SMALL_MAX_ITER, gs);
-            if (gs.hit && !gs.uncertain) {
+            const SeedState st = seed[p];
+            const int meta = __float_as_int(st.v.w);
+            s.n = meta & 0xff;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                s.w[i].p = v3(st.w[i].x, st.w[i].y, st.w[i].z);
+                s.w[i].key = __float_as_int(st.w[i].w);
+            }
+            v = v3(st.v.x, st.v.y, st.v.z);
+            if (meta >> 8) {
                 g.hit = true;
                 full = false;
             }
@@ -716,7 +764,7 @@ static int epa_resident_blocks() {
 }
 
 size_t gjk_scratch_bytes(int npairs) {
-    return 256 + (size_t)(npairs > 0 ? npairs : 1) * (sizeof(EpaEntry) + sizeof(int2));
+    return 256 + (size_t)(npairs > 0 ? npairs : 1) * (sizeof(EpaEntry) + sizeof(int2) + sizeof(SeedState));
 }
 
 sd_status check_hulls(const float* pos4, const float* quat4, const int* vstart, const int* nv, const float* verts4) {
@@ -799,7 +847,7 @@ extern "C" sd_status sd_sap_pairs(const float* lo4, const float* hi4, int n, int
     const int nb = div_up(n, AX_BLOCK) < AX_BLOCKS ? div_up(n, AX_BLOCK) : AX_BLOCKS;
     hipLaunchKernelGGL(k_axis_partial, dim3(nb), dim3(AX_BLOCK), 0, s, L, H, n, part);
     SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_axis_pick, dim3(1), dim3(64), 0, s, part, nb, n, axis);
+    hipLaunchKernelGGL(k_axis_pick, dim3(1), dim3(AX_BLOCK), 0, s, part, nb, n, axis);
     SD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sap_keys, dim3(div_up(n, 256)), dim3(256), 0, s, L, n, axis, keys, vals);
     SD_LAUNCH_CHECK();
@@ -948,22 +996,28 @@ extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, co
     HullSet h{(const float4*)pos4, (const float4*)quat4, vstart, nv, (const float4*)verts4};
     hipStream_t s = (hipStream_t)stream;
     // scratch: counters (EPA list, retry list, second EPA list), the EPA list,
-    // the retry list; the second EPA list reuses the first one's memory
+    // the retry list, the seed states; the second EPA list reuses the first
+    // one's memory
     int* count = (int*)scratch;
     EpaEntry* list = (EpaEntry*)((char*)scratch + 256);
     int2* retry = (int2*)((char*)scratch + 256 + (size_t)n_pairs * sizeof(EpaEntry));
+    SeedState* seed = (SeedState*)((char*)retry + (size_t)n_pairs * sizeof(int2));
     st = launch_fill_u32((uint32_t*)count, 0u, 3, s);
     if (st != SD_OK) return st;
     const int epa_blocks = std::min(div_up(n_pairs, EPA_GROUPS_PER_BLOCK), epa_resident_blocks());
     const dim3 epa_block(64 * EPA_WAVES_PER_BLOCK);
-    if (warm_dir != nullptr)
+    if (warm_dir != nullptr) {
+        hipLaunchKernelGGL(k_gjk_seed, dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h, n_hulls, n_verts,
+                           warm_dir, seed);
+        SD_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_gjk_phase<false, true>), dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h,
                            n_hulls, n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters,
-                           list, count, retry, count + 1);
-    else
+                           list, count, retry, count + 1, (const SeedState*)seed);
+    } else {
         hipLaunchKernelGGL((k_gjk_phase<false, false>), dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h,
                            n_hulls, n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters,
-                           list, count, retry, count + 1);
+                           list, count, retry, count + 1, (const SeedState*)nullptr);
+    }
     SD_LAUNCH_CHECK();
     const int fin_blocks = std::min(div_up(n_pairs, 64), GJK_RETRY_MAX_BLOCKS);
     hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)count);
@@ -974,7 +1028,7 @@ extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, co
     if (warm_dir == nullptr) return SD_OK;  // cold entries are never rejected
     hipLaunchKernelGGL((k_gjk_phase<true, false>), dim3(fin_blocks), dim3(64), 0, s, pairs, n_pairs, h, n_hulls,
                        n_verts, nullptr, 1, max_iter, near_margin, sep, normal3, flags, iters, list, count + 2, retry,
-                       count + 1);
+                       count + 1, (const SeedState*)nullptr);
     SD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)(count + 2));
     SD_LAUNCH_CHECK();
