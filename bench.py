@@ -14,7 +14,10 @@ dominant kernel (its time measured with HIP events on the scene stream in an
 eager profiling pass right after the timed region) and a CPU baseline (the
 oracle's full step on the host cores, bounded sample of the same workload).
 
-Further sections ride on the same line (--sections, default "batch,gjk,batch_gjk,clumps"):
+Further sections ride on the same line (--sections, default "c2_dense,batch,gjk,batch_gjk,clumps"):
+  c2_dense — dem_c2_dense: the configuration-2 law on a contact-dense FCC packing of
+           the same 4,194,304 spheres (about 3 contacts per sphere): particle-steps/s
+           and the contact kernel's roofline, beside the dilute headline.
   batch  — batch_c5 (512 independent scenes, 7.7M spheres): particle-steps/s of
            the batched graph-replayed step (after the autotuner picks the launch
            shapes) vs the plain form (every scene alone in its own context).
@@ -457,6 +460,84 @@ def calibrate_copy(nbytes=256 << 20, reps=5):
                       "copy_GBs": 2 * nbytes / (ms.value / 1000.0) / 1e9}), flush=True)
 
 
+def sphere_run(w, steps, warmup, autotune, profile_steps, s=None):
+    """Times `steps` graph-replayed steps of sphere workload w (after the
+    autotuner and `warmup` steps), then profiles the stages eagerly. Returns
+    the measurements and the generated arrays."""
+    import ctypes
+
+    from stardust_amd import _lib as L
+    from stardust_amd.scene import SphereScene
+    if s is None:
+        log("generating", w.name)
+        t0 = time.perf_counter()
+        s = w.generate()
+        log("generated in %.1f s" % (time.perf_counter() - t0))
+    g, c = w.grid(), w.contact()
+    sc = SphereScene(g, c, key64=w.key64, use_graph=True)
+    sc.upload(s["x"], s["y"], s["z"], s["r"], s["m"])
+    tune = None
+    if autotune:
+        # launch shapes only (sort tile, contact block); the state is unchanged
+        tune = sc.autotune(trial_steps=8)
+        log("autotuned", tune)
+    log("uploaded", w.n, "spheres; warmup", warmup)
+    sc.step(warmup)
+    stream = sc.stream()
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    L.call("sd_event_create", ctypes.byref(ev0))
+    L.call("sd_event_create", ctypes.byref(ev1))
+    L.synchronize(stream)
+    log("timed region:", steps, "steps")
+    t0 = time.perf_counter()
+    L.call("sd_event_record", ev0, stream)
+    sc.step(steps)  # returns after the stream is synchronised
+    L.call("sd_event_record", ev1, stream)
+    L.synchronize(stream)
+    wall = time.perf_counter() - t0
+    ms = ctypes.c_float()
+    L.call("sd_event_elapsed_ms", ev0, ev1, ctypes.byref(ms))
+    gpu_s = ms.value / 1000.0
+    t = max(wall, gpu_s)
+    value = w.n * steps / t
+    log("timed: wall %.3f s, events %.3f s -> %.3e particle-steps/s" % (wall, gpu_s, value))
+    contacts = sc.contact_count()
+    changes, full_next = sc.cell_changes()
+    diag = sc.diagnostics()
+    prof = sc.profile(profile_steps)
+    per_launch = {k: v / profile_steps for k, v in prof.items()}
+    log("per-step stage ms:", {k: round(v, 4) for k, v in per_launch.items()})
+    sbytes = stage_bytes(w, w.n, contacts / w.n, g.n_cells)
+    dominant = max(per_launch, key=per_launch.get)
+    achieved = sbytes[dominant] / (per_launch[dominant] / 1000.0) / 1e9
+    stages = {k: {"ms": round(per_launch[k], 5), "bytes": int(sbytes[k]),
+                  "GBs": round(sbytes[k] / (per_launch[k] / 1000.0) / 1e9, 1) if per_launch[k] > 0 else None}
+              for k in per_launch}
+    sc.close()
+    r = {"value": value, "t": t, "tune": tune, "grid": g, "contacts": contacts, "changes": changes,
+         "full_next": full_next, "diag": diag, "stages": stages, "dominant": dominant, "achieved": achieved,
+         "sbytes": sbytes}
+    return r, s
+
+
+def c2_dense_section(args):
+    """dem_c2_dense: the configuration-2 law on a contact-dense packing of the same
+    sphere count (about 3 contacts per sphere), particle-steps/s and the contact
+    kernel's roofline."""
+    from stardust_amd import workloads as W
+    w = W.dem_c2_dense()
+    steps = args.steps if args.steps is not None else 20
+    r, _ = sphere_run(w, steps, args.warmup, not args.no_autotune, args.profile_steps)
+    dom = r["dominant"]
+    return {"workload": w.name, "spheres": w.n, "steps": steps, "ms_per_step": round(1000.0 * r["t"] / steps, 4),
+            "particle_steps_per_s": r["value"], "contacts_per_sphere": r["contacts"] / w.n,
+            "cells": r["grid"].n_cells, "launch_shapes": r["tune"], "stages": r["stages"],
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes": r["sbytes"][dom]},
+            "packing": w.describe()["packing"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,8 +548,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes of the C2 context")
-    ap.add_argument("--sections", default="batch,gjk,batch_gjk,clumps",
-                    help="extra sections: batch, gjk, batch_gjk, clumps (comma list, or none)")
+    ap.add_argument("--sections", default="c2_dense,batch,gjk,batch_gjk,clumps",
+                    help="extra sections: c2_dense, batch, gjk, batch_gjk, clumps (comma list, or none)")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-warmup", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=20, help="steps per scene of the plain form (0: skip it)")
@@ -497,53 +578,8 @@ def main():
     L.require_gpu()
     w = W.dem_c2() if args.workload == "c2" else W.dem_c1()
     steps = args.steps if args.steps is not None else w.steps
-    log("generating", w.name)
-    t0 = time.perf_counter()
-    s = w.generate()
-    log("generated in %.1f s" % (time.perf_counter() - t0))
-    g, c = w.grid(), w.contact()
-    sc = SphereScene(g, c, key64=w.key64, use_graph=True)
-    sc.upload(s["x"], s["y"], s["z"], s["r"], s["m"])
-    tune = None
-    if not args.no_autotune:
-        # launch shapes only (sort tile, contact block); the state is unchanged
-        tune = sc.autotune(trial_steps=8)
-        log("autotuned", tune)
-    log("uploaded", w.n, "spheres; warmup", args.warmup)
-    sc.step(args.warmup)
-    stream = sc.stream()
-    ev0, ev1 = __import__("ctypes").c_void_p(), __import__("ctypes").c_void_p()
-    L.call("sd_event_create", __import__("ctypes").byref(ev0))
-    L.call("sd_event_create", __import__("ctypes").byref(ev1))
-    L.synchronize(stream)
-    log("timed region:", steps, "steps")
-    t0 = time.perf_counter()
-    L.call("sd_event_record", ev0, stream)
-    sc.step(steps)  # returns after the stream is synchronised
-    L.call("sd_event_record", ev1, stream)
-    L.synchronize(stream)
-    wall = time.perf_counter() - t0
-    ms = __import__("ctypes").c_float()
-    L.call("sd_event_elapsed_ms", ev0, ev1, __import__("ctypes").byref(ms))
-    gpu_s = ms.value / 1000.0
-    t = max(wall, gpu_s)
-    value = w.n * steps / t
-    log("timed: wall %.3f s, events %.3f s -> %.3e particle-steps/s" % (wall, gpu_s, value))
-    contacts = sc.contact_count()
-    changes, full_next = sc.cell_changes()
-    diag = sc.diagnostics()
-
-    prof = sc.profile(args.profile_steps)
-    per_launch = {k: v / args.profile_steps for k, v in prof.items()}
-    log("per-step stage ms:", {k: round(v, 4) for k, v in per_launch.items()})
-    sbytes = stage_bytes(w, w.n, contacts / w.n, g.n_cells)
-    dominant = max(per_launch, key=per_launch.get)
-    achieved = sbytes[dominant] / (per_launch[dominant] / 1000.0) / 1e9
-    stages = {k: {"ms": round(per_launch[k], 5), "bytes": int(sbytes[k]),
-                  "GBs": round(sbytes[k] / (per_launch[k] / 1000.0) / 1e9, 1) if per_launch[k] > 0 else None}
-              for k in per_launch}
-    sc.close()
-
+    r, s = sphere_run(w, steps, args.warmup, not args.no_autotune, args.profile_steps)
+    value, t, tune, g = r["value"], r["t"], r["tune"], r["grid"]
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -572,16 +608,19 @@ def main():
                    "keys": "u64 (cell<<32|index)" if w.key64 else "u32 cell + u32 index",
                    "cells": g.n_cells, "form": "plain single-scene, graph-replayed step",
                    "launch_shapes": tune if tune else "defaults (sort tile by key count, contact_block 128)"},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((w.name, dominant)), "algorithmic_bytes": sbytes[dominant]},
-        "stages": stages,
-        "contacts_per_sphere": contacts / w.n,
-        "cell_changes_last_timed_step": {"spheres": changes, "full_sort_next": full_next},
-        "energy_J": {"kinetic": diag["kinetic"], "potential": diag["potential"]},
+        "roofline": {"bound": "hbm", "kernel": r["dominant"], "achieved": round(r["achieved"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((w.name, r["dominant"])),
+                     "algorithmic_bytes": r["sbytes"][r["dominant"]]},
+        "stages": r["stages"],
+        "contacts_per_sphere": r["contacts"] / w.n,
+        "cell_changes_last_timed_step": {"spheres": r["changes"], "full_sort_next": r["full_next"]},
+        "energy_J": {"kinetic": r["diag"]["kinetic"], "potential": r["diag"]["potential"]},
         "cpu_baseline": cpu,
     }
     sections = [x for x in args.sections.split(",") if x and x != "none"]
+    if "c2_dense" in sections:
+        out["c2_dense"] = c2_dense_section(args)
     if "batch" in sections:
         out["batch"] = batch_section(args)
     if "gjk" in sections:

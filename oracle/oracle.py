@@ -340,6 +340,17 @@ class SphereState:
                                                p(self.omg4), p(self.ids), p(self.hcount), p(self.hist), nthreads))
         return mx
 
+    def reorder_by_id(self):
+        """Permutes every row (and each sphere's contact history) into particle-id
+        order. Stepping from id order sorts each cell's spheres by id, the order
+        the reference's id-ordered arrays and a stable sort give."""
+        inv = np.empty(self.n, np.int64)
+        inv[self.ids] = np.arange(self.n)
+        self.pos4, self.vel4, self.omg4 = self.pos4[inv], self.vel4[inv], self.omg4[inv]
+        self.hcount = self.hcount[inv]
+        self.hist = np.ascontiguousarray(self.hist.reshape(self.maxc, self.n, 4)[:, inv].reshape(-1, 4))
+        self.ids = np.arange(self.n, dtype=np.int32)
+
     def by_id(self):
         """State rows reordered to particle-id order."""
         inv = np.empty(self.n, np.int64)

@@ -13,6 +13,14 @@ dem_c2: plain DEM at scale. 4,194,304 polydisperse spheres, radius lognormal
         widened to 2.0 x 2.0 m (solid fraction 0.18 at seeding, where random
         sequential addition still finishes in about 20 s); the walled domain is
         the whole column.
+dem_c2_dense: the dem_c2 law (kn = 2e5 N/m, dt = 5e-6 s, zeta = 0.3, mu = 0.5,
+        gravity, six walls) on a contact-dense packing of the same 4,194,304
+        spheres: an FCC lattice (128 x 128 x 64 cubic cells of 4 sites) with
+        nearest-neighbour spacing 10 mm, 5.05 mm from the walls, radii lognormal
+        (median 5 mm, sigma 0.25, numpy PCG64 seeded with 0x5EED0102) clipped
+        to [2, 5.05] mm. Neighbours whose radii sum past 10 mm overlap by at
+        most 0.1 mm, about 3 contacts per sphere from the first step; the
+        walled box (1.81 x 1.81 x 0.91 m) holds the packing together.
 batch_c5: 512 independent scenes in one batch. Per scene (numpy PCG64 seeded
         with 0x5EED0005): sphere count log-uniform in [1,024, 65,536]; box
         footprint side U[0.2, 1] m; radius lognormal with median U[2, 8] mm,
@@ -93,6 +101,44 @@ def dem_c1(n=65536):
 def dem_c2(n=4194304):
     return Workload("dem_c2_column_4194304", n, (0.0, 0.0, 0.0), (2.0, 2.0, 4.0), 0.005, 0.25, 0.002, 0.012, 2e5,
                     5e-6, 2000, True, 0x5EED0002, 24)
+
+
+class LatticeWorkload(Workload):
+    """Spheres on an FCC lattice (a contact-dense packing) under a sphere law."""
+
+    def __init__(self, name, cells, spacing, r_med, r_sigma, r_min, r_max, kn, dt, steps, seed, max_contacts):
+        self.cells3 = tuple(int(c) for c in cells)
+        self.spacing = spacing
+        self.a = spacing * math.sqrt(2.0)  # cubic cell edge of the FCC lattice
+        self.margin = r_max
+        n = 4 * self.cells3[0] * self.cells3[1] * self.cells3[2]
+        hi = tuple(2.0 * self.margin + (c - 0.5) * self.a for c in self.cells3)
+        super().__init__(name, n, (0.0, 0.0, 0.0), hi, r_med, r_sigma, r_min, r_max, kn, dt, steps, True, seed,
+                         max_contacts)
+
+    def generate(self, n=None):
+        assert n is None or n == self.n, "the lattice fixes the sphere count"
+        nx, ny, nz = self.cells3
+        basis = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]])
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        cell = np.stack([i.ravel(), j.ravel(), k.ravel()], 1).astype(np.float64)
+        pts = (cell[:, None, :] + basis[None, :, :]).reshape(-1, 3) * self.a + self.margin
+        rng = np.random.Generator(np.random.PCG64(self.seed))
+        r = np.exp(math.log(self.r_med) + self.r_sigma * rng.standard_normal(self.n))
+        r = np.clip(r, self.r_min, self.r_max).astype(np.float32)
+        p = pts.astype(np.float32)
+        return {"x": p[:, 0].copy(), "y": p[:, 1].copy(), "z": p[:, 2].copy(), "r": r, "m": sphere_mass(r)}
+
+    def describe(self):
+        d = super().describe()
+        d.update({"packing": "FCC lattice, nearest-neighbour spacing %g m" % self.spacing,
+                  "lattice_cells": list(self.cells3)})
+        return d
+
+
+def dem_c2_dense(cells=(128, 128, 64)):
+    return LatticeWorkload("dem_c2_dense_%d" % (4 * cells[0] * cells[1] * cells[2]), cells, 0.01, 0.005, 0.25,
+                           0.002, 0.00505, 2e5, 5e-6, 2000, 0x5EED0102, 24)
 
 
 class BatchWorkload:
