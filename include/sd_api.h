@@ -90,6 +90,17 @@ sd_status sd_radix_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys
                                   void* stream);
 sd_status sd_radix_sort_keys_u64(uint64_t* keys, uint64_t* keys_alt, int n, int begin_bit, int end_bit,
                                  void* temp, size_t temp_bytes, void* stream);
+/* Stable sort of (key, value) pairs within each segment of a CSR partition
+ * (device seg_first[0] = 0 <= ... <= seg_first[nseg] = n): every segment ends
+ * sorted by the key bits [0, key_bits), equal keys in input order. temp must
+ * hold sd_segmented_sort_temp_bytes(n) bytes. Replaces a sortCollisionList
+ * call per scene (sh_collision_detection.cu:310-360) for callers that hold
+ * segments; the batched step sorts its scenes in one sd_radix_sort_keys_u64
+ * pass over scene-major packed keys instead. */
+sd_status sd_segmented_sort_temp_bytes(int n, size_t* bytes);
+sd_status sd_segmented_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                                      int n, const int* seg_first, int nseg, int key_bits, void* temp,
+                                      size_t temp_bytes, void* stream);
 /* As sd_radix_sort_pairs_u32, with no temp argument and no allocation: the
  * histogram workspace is a static device array of each GPU (about 8.4 MB,
  * held until the process ends), so n may not exceed SD_SORT_NOALLOC_MAX_N
@@ -158,6 +169,26 @@ sd_status sd_contact_forces(const float* pos4, const float* vel4, const float* o
                             int* hist_count_new, float* hist4_new,
                             float* force4, float* torque4, int* overflow, void* stream);
 
+/* Per-body force and torque from per-sphere forces: F_b = sum f_s and
+ * T_b = sum ((x_s - X_b) x f_s + tau_s) over the body's spheres
+ * [first[b], first[b] + count[b]) in sphere order (body_pos4 (X, *), sphere_pos4
+ * (x, *), force4 / torque4 rows (., 0)). Replaces computeForcesAndTorquesCUDA
+ * (src/engine/cuda/physics_update.cu:144-195, declared physics_update.cuh:19-59);
+ * gravity and the static rule are applied by sd_integrate_bodies. The clump
+ * step gathers each body's contacts directly and does not go through it. */
+sd_status sd_body_reduce(const float* body_pos4, const int* first, const int* count, int nb,
+                         const float* sphere_pos4, const float* sphere_force4, const float* sphere_torque4,
+                         float* body_force4, float* body_torque4, void* stream);
+/* One semi-implicit rigid-body step per body, in place: v += (F/m + g) dt,
+ * x += v dt, L += T dt, w = R I^-1 R^T L, q = normalise(dq(w dt) q) with
+ * pos4 = (x, y, z, 1/m) (1/m = 0: static), quat4 = (w, x, y, z), angmom4 =
+ * world L, inv_inertia6 = body-frame inverse inertia (xx, yy, zz, xy, xz, yz);
+ * dt and gravity from cp. The same arithmetic as the clump step's
+ * integration. Replaces advectParticlesCUDA (physics_update.cu:232-310). */
+sd_status sd_integrate_bodies(float* pos4, float* vel4, float* quat4, float* angmom4, const float* inv_inertia6,
+                              const float* force4, const float* torque4, int nb, const sd_contact_params* cp,
+                              void* stream);
+
 /* Semi-implicit Euler for free spheres: v += (F/m + g) dt; x += v dt; w += T/I dt.
  * Spheres with 1/m == 0 are static. Replaces advectParticlesCUDA
  * (src/engine/cuda/physics_update.cu:232-310) for single-sphere bodies. */
@@ -187,6 +218,11 @@ typedef struct sd_ctx sd_ctx;
 
 sd_status sd_ctx_create(const sd_sphere_params* p, sd_ctx** ctx);
 sd_status sd_ctx_destroy(sd_ctx* ctx);
+/* New contact law, dt, gravity or walls for the following steps (state and
+ * contact history kept; max_contacts must stay the same). Lets
+ * DEMEngine::update(float dt) (simulation_kernel.cu:52) take dt per call.
+ * Not for batch contexts. */
+sd_status sd_ctx_set_contact(sd_ctx* ctx, const sd_contact_params* cp);
 /* Host arrays in particle-id order. mass may not be NULL; mass[i] == 0 marks a static sphere. */
 sd_status sd_ctx_upload_spheres(sd_ctx* ctx, int n, const float* x, const float* y, const float* z,
                                 const float* vx, const float* vy, const float* vz,
@@ -421,6 +457,9 @@ sd_status sd_mlp_forward_backward(const float* X, const float* T, const int* idx
 /* Adam with bias correction; step counts from 1. */
 sd_status sd_adam_step(float* params, const float* grad, float* m, float* v, int n, float lr, float beta1,
                        float beta2, float eps, int step, void* stream);
+/* FP32 forward pass: unit directions of n feature rows from the FP32 master
+ * parameters (the reference-precision form of sd_mlp_infer_bf16). */
+sd_status sd_mlp_forward(const float* X, int n, const float* params, float* out_dir3, void* stream);
 /* BF16 copy of the weights in the MFMA inference layout (sd_mlp_packed_bytes() bytes). */
 sd_status sd_mlp_pack_bf16(const float* params, void* packed, void* stream);
 /* Unit directions for n feature rows (BF16 MFMA GEMMs, FP32 accumulation).

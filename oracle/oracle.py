@@ -52,6 +52,8 @@ _SIGS = {
     "or_morton30": ([_I, _P, _P, _P, _P], None),
     "or_clump_contacts_integrate": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_long, _P, C.POINTER(OrContact),
                                      _P, _P, _I], _I),
+    "or_body_reduce": ([_I, _P, _P, _P, _P, _P, _P, _P, _P], None),
+    "or_integrate_bodies": ([_I, _P, _P, _P, _P, _P, _P, _P, C.POINTER(OrContact)], None),
     "or_clump_step": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, C.POINTER(OrContact)], C.c_long),
 }
 
@@ -356,3 +358,20 @@ class SphereState:
         inv = np.empty(self.n, np.int64)
         inv[self.ids] = np.arange(self.n)
         return {"pos4": self.pos4[inv], "vel4": self.vel4[inv], "omg4": self.omg4[inv]}
+
+
+def body_reduce(bpos4, first, count, spos4, sf4, st4):
+    a = [np.ascontiguousarray(x, t) for x, t in ((bpos4, np.float32), (first, np.int32), (count, np.int32),
+                                                  (spos4, np.float32), (sf4, np.float32), (st4, np.float32))]
+    nb = len(a[0])
+    bf, bt = np.zeros((nb, 4), np.float32), np.zeros((nb, 4), np.float32)
+    load().or_body_reduce(nb, *[p(x) for x in a], p(bf), p(bt))
+    return bf, bt
+
+
+def integrate_bodies(P, V, Q, L, inv6, f4, t4, c):
+    P, V, Q, L = (np.array(x, np.float32) for x in (P, V, Q, L))
+    a = [np.ascontiguousarray(x, np.float32) for x in (inv6, f4, t4)]
+    oc = contact(c)
+    load().or_integrate_bodies(len(P), p(P), p(V), p(Q), p(L), p(a[0]), p(a[1]), p(a[2]), C.byref(oc))
+    return P, V, Q, L

@@ -121,6 +121,10 @@ void or_morton30(int n, const float* lo4, const float* hi4, const float* dom6, u
 int or_clump_contacts_integrate(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1,
                                 const int* first, const int* count, const float* sx, const float* sv, long npairs,
                                 const int* pairs, const or_contact* c, float* force4, float* torque4, int integrate);
+void or_body_reduce(int nb, const float* bpos4, const int* first, const int* count, const float* spos4,
+                    const float* sf4, const float* st4, float* bf4, float* bt4);
+void or_integrate_bodies(int nb, float* P, float* V, float* Q, float* L, const float* inv6, const float* f4,
+                         const float* t4, const or_contact* c);
 long or_clump_step(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const int* first,
                    const int* count, const float* local4, int ns, const or_contact* c);
 

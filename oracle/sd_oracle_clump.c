@@ -204,6 +204,84 @@ static int cmp_u64(const void* a, const void* b) {
 /* Contacts + integration of every body given the unique pair set (a, b).
  * Adjacency: (body, partner) ascending; per contact the sphere law with no
  * history; torque about the centre of mass. State arrays updated in place. */
+/* Semi-implicit step of one body (P, V, q, L rows; P[3] = 1/m, 0 = static) under
+ * force F and torque T: v += (F/m + g) dt, x += v dt, L += T dt, w = R I^-1 R^T L,
+ * q = normalise(dq q) (physics_update.cu:232-310 with quirks Q10-Q12 corrected). */
+static void integrate_body(float* Pn, float* Vn, float* q, float* Ln, const float* i0, const float* i1,
+                           const float* F, const float* T, const or_contact* c) {
+    if (!(Pn[3] > 0.0f)) return;
+    Vn[0] = Vn[0] + (F[0] * Pn[3] + c->gravity[0]) * c->dt;
+    Vn[1] = Vn[1] + (F[1] * Pn[3] + c->gravity[1]) * c->dt;
+    Vn[2] = Vn[2] + (F[2] * Pn[3] + c->gravity[2]) * c->dt;
+    Pn[0] = Pn[0] + Vn[0] * c->dt;
+    Pn[1] = Pn[1] + Vn[1] * c->dt;
+    Pn[2] = Pn[2] + Vn[2] * c->dt;
+    Ln[0] = Ln[0] + T[0] * c->dt;
+    Ln[1] = Ln[1] + T[1] * c->dt;
+    Ln[2] = Ln[2] + T[2] * c->dt;
+    float m[9];
+    qmat(q, m);
+    const V3 w = inv_inertia(m, i0, i1, mk(Ln[0], Ln[1], Ln[2]));
+    const float wn = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+    if (wn > 0.0f) {
+        float sa, ca;
+        sincos_small(wn * (0.5f * c->dt), &sa, &ca);
+        const float k = sa / wn;
+        const float dw = ca, dx = w.x * k, dy = w.y * k, dz = w.z * k;
+        const float nw = dw * q[0] - (dx * q[1] + dy * q[2] + dz * q[3]);
+        const float nx = dw * q[1] + q[0] * dx + (dy * q[3] - dz * q[2]);
+        const float ny = dw * q[2] + q[0] * dy + (dz * q[1] - dx * q[3]);
+        const float nz = dw * q[3] + q[0] * dz + (dx * q[2] - dy * q[1]);
+        const float inv = 1.0f / sqrtf(nw * nw + nx * nx + ny * ny + nz * nz);
+        q[0] = nw * inv;
+        q[1] = nx * inv;
+        q[2] = ny * inv;
+        q[3] = nz * inv;
+    }
+}
+
+/* Per-body sums of per-sphere forces (computeForcesAndTorquesCUDA,
+ * physics_update.cu:144-195, without its gravity and static rule, which the
+ * integration applies): F = sum f, T = sum ((x_s - X) x f + tau_s), spheres
+ * of [first, first + count) in order. */
+void or_body_reduce(int nb, const float* bpos4, const int* first, const int* count, const float* spos4,
+                    const float* sf4, const float* st4, float* bf4, float* bt4) {
+    for (int b = 0; b < nb; ++b) {
+        const float* X = bpos4 + 4 * (long)b;
+        float F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+        for (int i = first[b]; i < first[b] + count[b]; ++i) {
+            const float* x = spos4 + 4 * (long)i;
+            const float* f = sf4 + 4 * (long)i;
+            const float* t = st4 + 4 * (long)i;
+            F[0] = F[0] + f[0];
+            F[1] = F[1] + f[1];
+            F[2] = F[2] + f[2];
+            const float ax = x[0] - X[0], ay = x[1] - X[1], az = x[2] - X[2];
+            T[0] = T[0] + ((ay * f[2] - az * f[1]) + t[0]);
+            T[1] = T[1] + ((az * f[0] - ax * f[2]) + t[1]);
+            T[2] = T[2] + ((ax * f[1] - ay * f[0]) + t[2]);
+        }
+        for (int a = 0; a < 3; ++a) {
+            bf4[4 * (long)b + a] = F[a];
+            bt4[4 * (long)b + a] = T[a];
+        }
+        bf4[4 * (long)b + 3] = 0.0f;
+        bt4[4 * (long)b + 3] = 0.0f;
+    }
+}
+
+/* One step of every body under given forces and torques; inverse inertia in
+ * the body frame (xx, yy, zz, xy, xz, yz). */
+void or_integrate_bodies(int nb, float* P, float* V, float* Q, float* L, const float* inv6, const float* f4,
+                         const float* t4, const or_contact* c) {
+    for (int b = 0; b < nb; ++b) {
+        const float* I = inv6 + 6 * (long)b;
+        const float i0[4] = {I[0], I[1], I[2], I[3]}, i1[4] = {I[4], I[5], 0.0f, 0.0f};
+        integrate_body(P + 4 * (long)b, V + 4 * (long)b, Q + 4 * (long)b, L + 4 * (long)b, i0, i1,
+                       f4 + 4 * (long)b, t4 + 4 * (long)b, c);
+    }
+}
+
 int or_clump_contacts_integrate(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1,
                                 const int* first, const int* count, const float* sx, const float* sv, long npairs,
                                 const int* pairs, const or_contact* c, float* force4, float* torque4, int integrate) {
@@ -308,40 +386,7 @@ int or_clump_contacts_integrate(int nb, float* P, float* V, float* Q, float* L, 
         memcpy(o + 4, V + 4 * (long)b, 16);
         memcpy(o + 8, Q + 4 * (long)b, 16);
         memcpy(o + 12, L + 4 * (long)b, 16);
-        if (integrate && X[3] > 0.0f) {
-            float* Pn = o;
-            float* Vn = o + 4;
-            float* q = o + 8;
-            float* Ln = o + 12;
-            Vn[0] = Vn[0] + (F[0] * X[3] + c->gravity[0]) * c->dt;
-            Vn[1] = Vn[1] + (F[1] * X[3] + c->gravity[1]) * c->dt;
-            Vn[2] = Vn[2] + (F[2] * X[3] + c->gravity[2]) * c->dt;
-            Pn[0] = Pn[0] + Vn[0] * c->dt;
-            Pn[1] = Pn[1] + Vn[1] * c->dt;
-            Pn[2] = Pn[2] + Vn[2] * c->dt;
-            Ln[0] = Ln[0] + T[0] * c->dt;
-            Ln[1] = Ln[1] + T[1] * c->dt;
-            Ln[2] = Ln[2] + T[2] * c->dt;
-            float m[9];
-            qmat(q, m);
-            const V3 w = inv_inertia(m, I0 + 4 * (long)b, I1 + 4 * (long)b, mk(Ln[0], Ln[1], Ln[2]));
-            const float wn = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
-            if (wn > 0.0f) {
-                float sa, ca;
-                sincos_small(wn * (0.5f * c->dt), &sa, &ca);
-                const float k = sa / wn;
-                const float dw = ca, dx = w.x * k, dy = w.y * k, dz = w.z * k;
-                const float nw = dw * q[0] - (dx * q[1] + dy * q[2] + dz * q[3]);
-                const float nx = dw * q[1] + q[0] * dx + (dy * q[3] - dz * q[2]);
-                const float ny = dw * q[2] + q[0] * dy + (dz * q[1] - dx * q[3]);
-                const float nz = dw * q[3] + q[0] * dz + (dx * q[2] - dy * q[1]);
-                const float inv = 1.0f / sqrtf(nw * nw + nx * nx + ny * ny + nz * nz);
-                q[0] = nw * inv;
-                q[1] = nx * inv;
-                q[2] = ny * inv;
-                q[3] = nz * inv;
-            }
-        }
+        if (integrate) integrate_body(o, o + 4, o + 8, o + 12, I0 + 4 * (long)b, I1 + 4 * (long)b, F, T, c);
     }
     if (integrate)
         for (int b = 0; b < nb; ++b) {

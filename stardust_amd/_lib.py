@@ -108,6 +108,11 @@ _SIGS = {
     "sd_radix_sort_pairs_u32": [_P, _P, _P, _P, _I, _I, _I, _P, _SZ, _P],
     "sd_radix_sort_keys_u64": [_P, _P, _I, _I, _I, _P, _SZ, _P],
     "sd_radix_sort_pairs_u32_noalloc": [_P, _P, _P, _P, _I, _I, _I, _P],
+    "sd_segmented_sort_temp_bytes": [_I, C.POINTER(_SZ)],
+    "sd_segmented_sort_pairs_u32": [_P, _P, _P, _P, _I, _P, _I, _I, _P, _SZ, _P],
+    "sd_body_reduce": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P],
+    "sd_integrate_bodies": [_P, _P, _P, _P, _P, _P, _P, _I, C.POINTER(ContactParams), _P],
+    "sd_mlp_forward": [_P, _I, _P, _P, _P],
     "sd_cell_ranges_u32": [_P, _I, _I, _P, _P],
     "sd_cell_ranges_u64": [_P, _I, _I, _P, _P],
     "sd_neighbor_pairs": [_P, _I, C.POINTER(Grid), _P, _F, _P, _I, C.POINTER(C.c_int), _P, _SZ, _P],
@@ -118,6 +123,7 @@ _SIGS = {
     "sd_permute_spheres": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "sd_ctx_create": [C.POINTER(SphereParams), C.POINTER(C.c_void_p)],
     "sd_ctx_destroy": [_P],
+    "sd_ctx_set_contact": [_P, C.POINTER(ContactParams)],
     "sd_ctx_upload_spheres": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "sd_ctx_step": [_P, _I],
     "sd_ctx_profile": [_P, _I, _P, _I],

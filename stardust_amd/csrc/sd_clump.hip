@@ -67,6 +67,39 @@ __device__ __forceinline__ V3 inv_inertia_apply(const Rot& R, float4 i0, float4 
     return rot_mul(R, w);
 }
 
+// Semi-implicit step of one body with 1/m > 0 under force F and torque T
+// (the integration above); a static body (1/m = 0) is left as it is.
+__device__ __forceinline__ void integrate_body(float4& P, float4& V, float4& q, float4& L, float4 i0, float4 i1,
+                                               float Fx, float Fy, float Fz, float Tx, float Ty, float Tz,
+                                               const ContactDev& c) {
+    if (!(P.w > 0.0f)) return;
+    V.x = V.x + (Fx * P.w + c.gx) * c.dt;
+    V.y = V.y + (Fy * P.w + c.gy) * c.dt;
+    V.z = V.z + (Fz * P.w + c.gz) * c.dt;
+    P.x = P.x + V.x * c.dt;
+    P.y = P.y + V.y * c.dt;
+    P.z = P.z + V.z * c.dt;
+    L.x = L.x + Tx * c.dt;
+    L.y = L.y + Ty * c.dt;
+    L.z = L.z + Tz * c.dt;
+    const Rot R = quat_rot(q);
+    const V3 w = inv_inertia_apply(R, i0, i1, v3(L.x, L.y, L.z));
+    const float wn = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+    if (wn > 0.0f) {
+        float sa, ca;
+        sincos_small(wn * (0.5f * c.dt), sa, ca);
+        const float k = sa / wn;
+        const float dw = ca, dx = w.x * k, dy = w.y * k, dz = w.z * k;
+        // dq * q
+        const float nw = dw * q.x - (dx * q.y + dy * q.z + dz * q.w);
+        const float nx = dw * q.y + q.x * dx + (dy * q.w - dz * q.z);
+        const float ny = dw * q.z + q.x * dy + (dz * q.y - dx * q.w);
+        const float nz = dw * q.w + q.x * dz + (dx * q.z - dy * q.y);
+        const float inv = 1.0f / sqrtf(nw * nw + nx * nx + ny * ny + nz * nz);
+        q = make_float4(nw * inv, nx * inv, ny * inv, nz * inv);
+    }
+}
+
 struct ClumpState {
     const float4* pos;   // (x, y, z, 1/m)
     const float4* vel;   // (vx, vy, vz, 0)
@@ -232,42 +265,92 @@ __global__ __launch_bounds__(CL_BLOCK) void k_clump_contacts(
     }
     float4 V = st.vel[b], q = st.quat[b], L = st.angm[b];
     float4 P = X;
-    if (X.w > 0.0f) {
-        V.x = V.x + (Fx * X.w + c.gx) * c.dt;
-        V.y = V.y + (Fy * X.w + c.gy) * c.dt;
-        V.z = V.z + (Fz * X.w + c.gz) * c.dt;
-        P.x = P.x + V.x * c.dt;
-        P.y = P.y + V.y * c.dt;
-        P.z = P.z + V.z * c.dt;
-        L.x = L.x + Tx * c.dt;
-        L.y = L.y + Ty * c.dt;
-        L.z = L.z + Tz * c.dt;
-        const Rot R = quat_rot(q);
-        const V3 w = inv_inertia_apply(R, i0[b], i1[b], v3(L.x, L.y, L.z));
-        const float wn = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
-        if (wn > 0.0f) {
-            float sa, ca;
-            sincos_small(wn * (0.5f * c.dt), sa, ca);
-            const float k = sa / wn;
-            const float dw = ca, dx = w.x * k, dy = w.y * k, dz = w.z * k;
-            // dq * q
-            const float nw = dw * q.x - (dx * q.y + dy * q.z + dz * q.w);
-            const float nx = dw * q.y + q.x * dx + (dy * q.w - dz * q.z);
-            const float ny = dw * q.z + q.x * dy + (dz * q.y - dx * q.w);
-            const float nz = dw * q.w + q.x * dz + (dx * q.z - dy * q.y);
-            const float inv = 1.0f / sqrtf(nw * nw + nx * nx + ny * ny + nz * nz);
-            q = make_float4(nw * inv, nx * inv, ny * inv, nz * inv);
-        }
-    }
+    integrate_body(P, V, q, L, i0[b], i1[b], Fx, Fy, Fz, Tx, Ty, Tz, c);
     ((float4*)out.pos)[b] = P;
     ((float4*)out.vel)[b] = V;
     ((float4*)out.quat)[b] = q;
     ((float4*)out.angm)[b] = L;
 }
 
+// Per-body sums of per-sphere forces over the body's sphere range, in sphere
+// order: F = sum f, T = sum ((x_s - X) x f + tau_s).
+__global__ void k_body_reduce(const float4* __restrict__ bpos, const int* __restrict__ first,
+                              const int* __restrict__ count, int nb, const float4* __restrict__ spos,
+                              const float4* __restrict__ sf, const float4* __restrict__ st,
+                              float4* __restrict__ bf, float4* __restrict__ bt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const float4 X = bpos[b];
+    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
+    const int f0 = first[b], fn = count[b];
+    for (int i = f0; i < f0 + fn; ++i) {
+        const float4 x = spos[i], f = sf[i], t = st[i];
+        Fx = Fx + f.x;
+        Fy = Fy + f.y;
+        Fz = Fz + f.z;
+        const float ax = x.x - X.x, ay = x.y - X.y, az = x.z - X.z;
+        Tx = Tx + ((ay * f.z - az * f.y) + t.x);
+        Ty = Ty + ((az * f.x - ax * f.z) + t.y);
+        Tz = Tz + ((ax * f.y - ay * f.x) + t.z);
+    }
+    bf[b] = make_float4(Fx, Fy, Fz, 0.0f);
+    bt[b] = make_float4(Tx, Ty, Tz, 0.0f);
+}
+
+// One semi-implicit step per body (integrate_body), in place.
+__global__ void k_integrate_bodies(float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ quat,
+                                   float4* __restrict__ angm, const float* __restrict__ inv6,
+                                   const float4* __restrict__ force, const float4* __restrict__ torque, int nb,
+                                   ContactDev c) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    float4 P = pos[b], V = vel[b], q = quat[b], L = angm[b];
+    const float* I = inv6 + 6 * (size_t)b;
+    const float4 i0 = make_float4(I[0], I[1], I[2], I[3]), i1 = make_float4(I[4], I[5], 0.0f, 0.0f);
+    const float4 F = force[b], T = torque[b];
+    integrate_body(P, V, q, L, i0, i1, F.x, F.y, F.z, T.x, T.y, T.z, c);
+    pos[b] = P;
+    vel[b] = V;
+    quat[b] = q;
+    angm[b] = L;
+}
+
 }  // namespace sd
 
 using namespace sd;
+
+extern "C" sd_status sd_body_reduce(const float* body_pos4, const int* first, const int* count, int nb,
+                                    const float* sphere_pos4, const float* sphere_force4,
+                                    const float* sphere_torque4, float* body_force4, float* body_torque4,
+                                    void* stream) {
+    SD_REQUIRE(nb >= 0, "sd_body_reduce: nb < 0");
+    if (nb == 0) return SD_OK;
+    SD_REQUIRE(body_pos4 && first && count && sphere_pos4 && sphere_force4 && sphere_torque4 && body_force4 &&
+                   body_torque4,
+               "sd_body_reduce: null pointer");
+    hipLaunchKernelGGL(k_body_reduce, dim3(div_up(nb, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float4*)body_pos4, first, count, nb, (const float4*)sphere_pos4,
+                       (const float4*)sphere_force4, (const float4*)sphere_torque4, (float4*)body_force4,
+                       (float4*)body_torque4);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+extern "C" sd_status sd_integrate_bodies(float* pos4, float* vel4, float* quat4, float* angmom4,
+                                         const float* inv_inertia6, const float* force4, const float* torque4, int nb,
+                                         const sd_contact_params* cp, void* stream) {
+    SD_REQUIRE(nb >= 0 && cp, "sd_integrate_bodies: bad argument");
+    if (nb == 0) return SD_OK;
+    SD_REQUIRE(pos4 && vel4 && quat4 && angmom4 && inv_inertia6 && force4 && torque4,
+               "sd_integrate_bodies: null pointer");
+    sd_status st = check_contact(cp);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_integrate_bodies, dim3(div_up(nb, 256)), dim3(256), 0, (hipStream_t)stream, (float4*)pos4,
+                       (float4*)vel4, (float4*)quat4, (float4*)angmom4, inv_inertia6, (const float4*)force4,
+                       (const float4*)torque4, nb, to_dev(*cp));
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
 
 // ------------------------------------------------------------------ clump scene context
 

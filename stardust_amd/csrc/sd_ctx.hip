@@ -422,6 +422,21 @@ extern "C" sd_status sd_ctx_create(const sd_sphere_params* p, sd_ctx** out) {
     return SD_OK;
 }
 
+extern "C" sd_status sd_ctx_set_contact(sd_ctx* c, const sd_contact_params* cp) {
+    SD_REQUIRE(c && cp, "sd_ctx_set_contact: NULL argument");
+    if (c->batch) return set_error(SD_ERR_STATE, "sd_ctx_set_contact: a batch context keeps its scenes' laws");
+    sd_status st = check_contact(cp);
+    if (st != SD_OK) return st;
+    SD_REQUIRE(cp->max_contacts == c->p.contact.max_contacts,
+               "sd_ctx_set_contact: max_contacts cannot change (%d != %d)", cp->max_contacts,
+               c->p.contact.max_contacts);
+    if (c->stream && hipStreamSynchronize(c->stream) != hipSuccess)
+        return set_error(SD_ERR_HIP, "sd_ctx_set_contact: stream synchronisation failed");
+    c->p.contact = *cp;
+    drop_graphs(c);  // captured with the old law
+    return SD_OK;
+}
+
 extern "C" sd_status sd_ctx_destroy(sd_ctx* c) {
     if (!c) return SD_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);

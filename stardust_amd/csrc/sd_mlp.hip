@@ -554,6 +554,63 @@ __global__ __launch_bounds__(INF_WAVES * 64) void k_mlp_infer(const float* __res
 
 using namespace sd;
 
+// FP32 forward pass of n feature rows (one thread per row, parameters in LDS,
+// hidden activations in registers): the unit direction u = y / |y|. The
+// reference-precision form of k_mlp_infer (BF16 MFMA); the training kernel
+// runs the same arithmetic with the backward pass.
+constexpr int FW_BLOCK = 128;
+__global__ __launch_bounds__(FW_BLOCK) void k_mlp_forward(const float* __restrict__ X, int n,
+                                                          const float* __restrict__ w, float* __restrict__ out) {
+    __shared__ float sw[NPARAM];
+    for (int i = threadIdx.x; i < NPARAM; i += FW_BLOCK) sw[i] = w[i];
+    __syncthreads();
+    const int r = blockIdx.x * FW_BLOCK + threadIdx.x;
+    if (r >= n) return;
+    float h0[NH], h1[NH];
+    const float* x = X + (size_t)r * NIN;
+#pragma unroll
+    for (int j = 0; j < NH; ++j) h0[j] = sw[OFF_B1 + j];
+    for (int i = 0; i < NIN; ++i) {
+        const float xi = x[i];
+#pragma unroll
+        for (int j = 0; j < NH; ++j) h0[j] = h0[j] + xi * sw[OFF_W1 + i * NH + j];
+    }
+#pragma unroll
+    for (int j = 0; j < NH; ++j) h0[j] = h0[j] * sigm(h0[j]);
+#pragma unroll
+    for (int j = 0; j < NH; ++j) h1[j] = sw[OFF_B2 + j];
+#pragma unroll
+    for (int i = 0; i < NH; ++i) {
+        const float a = h0[i];
+#pragma unroll
+        for (int j = 0; j < NH; ++j) h1[j] = h1[j] + a * sw[OFF_W2 + i * NH + j];
+    }
+#pragma unroll
+    for (int j = 0; j < NH; ++j) h1[j] = h1[j] * sigm(h1[j]);
+#pragma unroll
+    for (int j = 0; j < NH; ++j) h0[j] = sw[OFF_B3 + j];
+#pragma unroll
+    for (int i = 0; i < NH; ++i) {
+        const float a = h1[i];
+#pragma unroll
+        for (int j = 0; j < NH; ++j) h0[j] = h0[j] + a * sw[OFF_W3 + i * NH + j];
+    }
+    float y[NOUT];
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) y[k] = sw[OFF_B4 + k];
+#pragma unroll
+    for (int i = 0; i < NH; ++i) {
+        const float a = h0[i] * sigm(h0[i]);
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) y[k] = y[k] + a * sw[OFF_W4 + i * NOUT + k];
+    }
+    const float l = sqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+    const float il = l > 0.0f ? 1.0f / l : 0.0f;
+    out[3 * (size_t)r] = y[0] * il;
+    out[3 * (size_t)r + 1] = y[1] * il;
+    out[3 * (size_t)r + 2] = y[2] * il;
+}
+
 extern "C" int sd_mlp_param_count(void) { return NPARAM; }
 extern "C" int sd_mlp_packed_bytes(void) { return PK_BYTES; }
 
@@ -653,6 +710,15 @@ extern "C" sd_status sd_mlp_infer_bf16(const float* X, int n, const void* packed
     const float* bias = (const float*)((const char*)packed + PK_END * 2);
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(INF_WAVES * 64), 0, (hipStream_t)stream, X, n, wb, bias, pairs,
                        (const float4*)quat4, out_dir3);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+extern "C" sd_status sd_mlp_forward(const float* X, int n, const float* params, float* out_dir3, void* stream) {
+    SD_REQUIRE(n >= 0 && (n == 0 || (X && params && out_dir3)), "sd_mlp_forward: bad argument");
+    if (n == 0) return SD_OK;
+    hipLaunchKernelGGL(k_mlp_forward, dim3(div_up(n, FW_BLOCK)), dim3(FW_BLOCK), 0, (hipStream_t)stream, X, n, params,
+                       out_dir3);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

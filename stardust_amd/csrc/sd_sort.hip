@@ -323,6 +323,99 @@ extern "C" sd_status sd_radix_sort_keys_u64(uint64_t* keys, uint64_t* keys_alt, 
     return SD_OK;
 }
 
+// ------------------------------------------------------------------ segmented sort
+//
+// Stable sort of (key, value) pairs within each segment of a CSR partition
+// (seg_first[0] = 0 <= ... <= seg_first[nseg] = n): a stable LSD sort by key
+// over the whole array, then a stable sort of that order by segment, then one
+// gather. The batched step does not call it: there the segment (scene) rides
+// in the high bits of one packed u64 key, which sorts every scene in one pass.
+namespace sd {
+
+__global__ void k_seg_init(const uint32_t* __restrict__ keys, int n, uint32_t* __restrict__ k,
+                           uint32_t* __restrict__ v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    k[i] = keys[i];
+    v[i] = (uint32_t)i;
+}
+
+// Segment of element index e: the last s with seg_first[s] <= e.
+__global__ void k_seg_keys(const uint32_t* __restrict__ perm, int n, const int* __restrict__ seg_first, int nseg,
+                           uint32_t* __restrict__ sk, uint32_t* __restrict__ sv) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int e = (int)perm[j];
+    int lo = 0, hi = nseg;  // seg_first[lo] <= e < seg_first[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (seg_first[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    sk[j] = (uint32_t)lo;
+    sv[j] = (uint32_t)e;
+}
+
+__global__ void k_seg_gather(const uint32_t* __restrict__ perm, int n, const uint32_t* __restrict__ keys,
+                             const uint32_t* __restrict__ vals, uint32_t* __restrict__ ko,
+                             uint32_t* __restrict__ vo) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t e = perm[t];
+    ko[t] = keys[e];
+    vo[t] = vals[e];
+}
+
+size_t segmented_temp_bytes(int n) {
+    return radix_temp_bytes(n) + 4 * ((size_t)(n > 0 ? n : 1) * sizeof(uint32_t) + 256);
+}
+
+}  // namespace sd
+
+extern "C" sd_status sd_segmented_sort_temp_bytes(int n, size_t* bytes) {
+    SD_REQUIRE(bytes != nullptr && n >= 0, "sd_segmented_sort_temp_bytes: bad argument");
+    *bytes = sd::segmented_temp_bytes(n);
+    return SD_OK;
+}
+
+extern "C" sd_status sd_segmented_sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                                                 uint32_t* vals_alt, int n, const int* seg_first, int nseg,
+                                                 int key_bits, void* temp, size_t temp_bytes, void* stream) {
+    using namespace sd;
+    SD_REQUIRE(n >= 0 && nseg >= 1, "sd_segmented_sort_pairs_u32: bad sizes");
+    SD_REQUIRE(key_bits >= 1 && key_bits <= 32, "sd_segmented_sort_pairs_u32: key_bits must be in [1, 32]");
+    if (n <= 1) return SD_OK;
+    SD_REQUIRE(keys && vals && keys_alt && vals_alt && seg_first && temp, "sd_segmented_sort_pairs_u32: null pointer");
+    SD_REQUIRE(temp_bytes >= segmented_temp_bytes(n), "sd_segmented_sort_pairs_u32: temp too small (%zu < %zu)",
+               temp_bytes, segmented_temp_bytes(n));
+    hipStream_t s = (hipStream_t)stream;
+    char* p = (char*)temp;
+    const size_t rt = radix_temp_bytes(n), nb = ((size_t)n * sizeof(uint32_t) + 255) & ~(size_t)255;
+    void* rtemp = p;
+    uint32_t* b0 = (uint32_t*)(p + rt);
+    uint32_t* b1 = (uint32_t*)(p + rt + nb);
+    uint32_t* b2 = (uint32_t*)(p + rt + 2 * nb);
+    uint32_t* b3 = (uint32_t*)(p + rt + 3 * nb);
+    const dim3 grid(div_up(n, 256)), block(256);
+    hipLaunchKernelGGL(k_seg_init, grid, block, 0, s, keys, n, b0, b1);
+    SD_LAUNCH_CHECK();
+    uint32_t *k = b0, *v = b1, *ka = b2, *va = b3;
+    sd_status st = radix_sort_pairs_u32(&k, &v, &ka, &va, n, 0, key_bits, rtemp, rt, s, 0);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_seg_keys, grid, block, 0, s, v, n, seg_first, nseg, ka, va);
+    SD_LAUNCH_CHECK();
+    int seg_bits = 1;
+    while (seg_bits < 31 && (1 << seg_bits) < nseg) ++seg_bits;
+    uint32_t *k2 = ka, *v2 = va, *ka2 = k, *va2 = v;
+    st = radix_sort_pairs_u32(&k2, &v2, &ka2, &va2, n, 0, seg_bits, rtemp, rt, s, 0);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_seg_gather, grid, block, 0, s, v2, n, keys, vals, keys_alt, vals_alt);
+    SD_LAUNCH_CHECK();
+    SD_HIP(hipMemcpyAsync(keys, keys_alt, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    SD_HIP(hipMemcpyAsync(vals, vals_alt, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    return SD_OK;
+}
+
 // ------------------------------------------------------------------ no-allocation form
 //
 // The histogram workspace of sd_radix_sort_pairs_u32_noalloc is a file-scope

@@ -2,6 +2,7 @@
 // src/engine/cuda/simulation_kernel.cu:52-153).
 #include "engine.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <sstream>
@@ -28,6 +29,7 @@ DEMEngine::DEMEngine(engineConfig config, Scalar domain) : engineConfiguration(c
 
 DEMEngine::~DEMEngine() {
     if (m_ctx) sd_clump_destroy(m_ctx);
+    if (m_sctx) sd_ctx_destroy(m_sctx);
 }
 
 void DEMEngine::add(DEMParticle particle) {
@@ -59,14 +61,124 @@ void DEMEngine::setStaticMassThreshold(float threshold) {
     m_static_mass = threshold;
 }
 
-// Packs bodies for sd_clump_upload: centre = entity position, spheres relative
-// to it with radius diameter / 2, inertia about the centre from the member
-// spheres (own 2/5 m r^2 plus the parallel-axis term, in double).
 void DEMEngine::transfer() {
     EntityHandler& h = entity_handler;
     if (!h.prepared) throw std::runtime_error("DEMEngine::transfer: call prep() first");
-    const int nb = h.getNumberOfParticles(), ns = h.getNumberOfSpheres();
+    const int nb = h.getNumberOfParticles();
     if (nb == 0) throw std::runtime_error("DEMEngine::transfer: no entities");
+    bool single = true;
+    for (int b = 0; b < nb && single; ++b) single = h.entity_length[b] == 1;
+    m_sphere_mode = single;
+    if (single) transfer_spheres();
+    else transfer_clumps();
+    m_uploaded = true;
+    m_dt = 0.0f;
+    m_steps = 0;
+}
+
+namespace {
+
+sd_contact_params sphere_law(float kn, float zeta, float mu, float kt_ratio, float dt, const float* g,
+                             const DEMEngine::SphereGrid& sg) {
+    sd_contact_params cp{};
+    cp.kn = kn;
+    cp.damping_ratio = zeta;
+    cp.kt_ratio = kt_ratio;
+    cp.mu = mu;
+    cp.dt = dt;
+    for (int a = 0; a < 3; ++a) cp.gravity[a] = g[a];
+    cp.walls = sg.walls ? 1 : 0;
+    for (int a = 0; a < 3; ++a) {
+        cp.wall_lo[a] = sg.wall_lo[a];
+        cp.wall_hi[a] = sg.wall_hi[a];
+    }
+    cp.max_contacts = 24;
+    return cp;
+}
+
+}  // namespace
+
+// Single-sphere entities to a sphere context: radius = diameter / 2, the
+// entity's mass (0 = static when it reaches the static threshold), grid cell
+// = the largest diameter; walls on the faces of [0, domain]^3 when every
+// sphere lies inside it, else an unbounded grid over the spheres' extent.
+void DEMEngine::transfer_spheres() {
+    EntityHandler& h = entity_handler;
+    const int n = h.getNumberOfParticles();
+    std::vector<float> x(n), y(n), z(n), vx(n), vy(n), vz(n), r(n), m(n);
+    float rmax = 0.0f;
+    bool inside = m_domain > 0.0f;
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (int b = 0; b < n; ++b) {
+        const int k = h.entity_start[b];
+        const float4 X = h.rb_pos[b], V = h.rb_vel[b];
+        const float mass = h.rb_mass[b];
+        x[b] = X.x;
+        y[b] = X.y;
+        z[b] = X.z;
+        vx[b] = V.x;
+        vy[b] = V.y;
+        vz[b] = V.z;
+        r[b] = 0.5f * h.p_size[k];
+        m[b] = mass >= m_static_mass ? 0.0f : mass;
+        if (!(r[b] > 0.0f)) throw std::runtime_error("DEMEngine::transfer: sphere diameter must be positive");
+        rmax = std::max(rmax, r[b]);
+        const float p[3] = {X.x, X.y, X.z};
+        for (int a = 0; a < 3; ++a) {
+            inside = inside && p[a] >= r[b] && p[a] <= m_domain - r[b];
+            lo[a] = std::min(lo[a], p[a] - r[b]);
+            hi[a] = std::max(hi[a], p[a] + r[b]);
+        }
+    }
+    SphereGrid& sg = m_sgrid;
+    sg.cell = 2.0f * rmax;
+    sg.walls = inside;
+    for (int a = 0; a < 3; ++a) {
+        if (inside) {
+            sg.lo[a] = 0.0f;
+            sg.wall_lo[a] = 0.0f;
+            sg.wall_hi[a] = m_domain;
+        } else {
+            sg.lo[a] = lo[a] - 2.0f * sg.cell;
+            hi[a] = hi[a] + 2.0f * sg.cell;
+            sg.wall_lo[a] = sg.wall_hi[a] = 0.0f;
+        }
+    }
+    const float ext[3] = {inside ? m_domain : hi[0] - sg.lo[0], inside ? m_domain : hi[1] - sg.lo[1],
+                          inside ? m_domain : hi[2] - sg.lo[2]};
+    for (;;) {  // at most 2^26 cells: a sparse scene gets coarser cells (any cell >= the largest diameter works)
+        double cells = 1.0;
+        for (int a = 0; a < 3; ++a) {
+            sg.n[a] = std::max(1, (int)std::ceil(ext[a] / sg.cell));
+            cells *= sg.n[a];
+        }
+        if (cells <= (double)(1 << 26)) break;
+        sg.cell *= 2.0f;
+    }
+    sd_sphere_params p{};
+    for (int a = 0; a < 3; ++a) {
+        p.grid.lo[a] = sg.lo[a];
+        p.grid.n[a] = sg.n[a];
+    }
+    p.grid.cell = sg.cell;
+    p.contact = sphere_law(m_kn, m_zeta, m_mu, m_kt_ratio, 1e-5f, m_g, sg);  // dt of the first update replaces it
+    p.key64 = 1;
+    p.use_graph = 1;
+    p.tile_items = 0;
+    if (m_sctx) sd_ctx_destroy(m_sctx);
+    m_sctx = nullptr;
+    check(sd_ctx_create(&p, &m_sctx), "DEMEngine::transfer");
+    check(sd_ctx_upload_spheres(m_sctx, n, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(), r.data(),
+                                m.data()),
+          "DEMEngine::transfer");
+}
+
+// Packs bodies for sd_clump_upload: centre = entity position, spheres relative
+// to it with radius diameter / 2, inertia about the centre from the member
+// spheres (own 2/5 m r^2 plus the parallel-axis term, in double).
+void DEMEngine::transfer_clumps() {
+    EntityHandler& h = entity_handler;
+    const int nb = h.getNumberOfParticles(), ns = h.getNumberOfSpheres();
     up_pos4.assign(4 * (size_t)nb, 0.0f);
     up_vel4.assign(4 * (size_t)nb, 0.0f);
     up_quat4.assign(4 * (size_t)nb, 0.0f);
@@ -123,13 +235,16 @@ void DEMEngine::transfer() {
     check(sd_clump_upload(m_ctx, nb, up_pos4.data(), up_vel4.data(), up_quat4.data(), up_angvel4.data(),
                           up_inertia6.data(), up_first.data(), up_count.data(), ns, up_local4.data()),
           "DEMEngine::transfer");
-    m_uploaded = true;
-    m_dt = 0.0f;
-    m_steps = 0;
 }
 
 void DEMEngine::ensure_context(float dt) {
     if (dt == m_dt) return;
+    if (m_sphere_mode) {
+        const sd_contact_params cp = sphere_law(m_kn, m_zeta, m_mu, m_kt_ratio, dt, m_g, m_sgrid);
+        check(sd_ctx_set_contact(m_sctx, &cp), "DEMEngine::update");
+        m_dt = dt;
+        return;
+    }
     sd_clump_params p{};
     p.contact.kn = m_kn;
     p.contact.damping_ratio = m_zeta;
@@ -146,7 +261,8 @@ void DEMEngine::update(float dt) {
     if (!m_uploaded) throw std::runtime_error("DEMEngine::update: call prep() and transfer() first");
     if (!(dt > 0.0f)) throw std::invalid_argument("DEMEngine::update: dt must be positive");
     ensure_context(dt);
-    check(sd_clump_step(m_ctx, 1), "DEMEngine::update");
+    if (m_sphere_mode) check(sd_ctx_step(m_sctx, 1), "DEMEngine::update");
+    else check(sd_clump_step(m_ctx, 1), "DEMEngine::update");
     ++m_steps;
     is_first_step = false;
 }
@@ -155,6 +271,24 @@ void DEMEngine::sync() {
     if (!m_uploaded) throw std::runtime_error("DEMEngine::sync: nothing transferred");
     EntityHandler& h = entity_handler;
     const int nb = h.getNumberOfParticles(), ns = h.getNumberOfSpheres();
+    if (m_sphere_mode) {
+        std::vector<float> x(nb), y(nb), z(nb), vx(nb), vy(nb), vz(nb), wx(nb), wy(nb), wz(nb);
+        check(sd_ctx_download_spheres(m_sctx, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
+                                      wx.data(), wy.data(), wz.data()),
+              "DEMEngine::sync");
+        for (int b = 0; b < nb; ++b) {
+            const int k = h.entity_start[b];
+            const float m = h.rb_mass[b], r = 0.5f * h.p_size[k];
+            const float I = 0.4f * m * r * r;
+            h.rb_pos[b] = make_float4(x[b], y[b], z[b], 0.0f);
+            h.rb_vel[b] = make_float4(vx[b], vy[b], vz[b], 0.0f);
+            h.rb_linmom[b] = h.rb_vel[b] * m;
+            h.rb_angmom[b] = make_float4(I * wx[b], I * wy[b], I * wz[b], 0.0f);
+            h.p_pos[k] = h.rb_pos[b];
+            h.p_rel[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        return;
+    }
     std::vector<float> pos4(4 * (size_t)nb), vel4(4 * (size_t)nb), quat4(4 * (size_t)nb), angm4(4 * (size_t)nb);
     std::vector<float> sph(4 * (size_t)ns);
     check(sd_clump_download(m_ctx, pos4.data(), vel4.data(), quat4.data(), angm4.data()), "DEMEngine::sync");

@@ -10,11 +10,23 @@
 //     engine.transfer();
 //     for (int i = 0; i < 300; ++i) engine.update(0.0005f);
 //
-// Every entity is a rigid body made of its spheres. transfer() uploads the
-// bodies to a clump context of libstardust_hip.so (sd_clump_*), and update(dt)
-// advances one step there: sphere pose, LBVH broadphase over body boxes,
-// sphere-sphere contacts with the linear spring-dashpot law, per-body force
-// and torque gather, quaternion integration. Differences from the reference,
+// transfer() picks one of two step paths of libstardust_hip.so:
+//  - every entity a single sphere (the reference driver's 1,000,000-sphere
+//    scene, testDEM.cpp:80-128): a sphere context (sd_ctx_*): hash grid of
+//    cell = the largest diameter, packed 64-bit keys re-sorted every step,
+//    the sphere law with tangential history, semi-implicit Euler. When every
+//    sphere lies inside the cube [0, domain]^3 (the constructor's domain, in
+//    metres, as the reference's scene files state it), its six faces are
+//    walls; otherwise the domain is treated as unbounded (the reference's
+//    "TODO: Add option for unbounded", engine.cpp:73): no walls, a grid over
+//    the spheres' extent (spheres leaving it are hashed to its border cells).
+//    A sphere's orientation is not integrated (its quaternion stays the
+//    identity); its spin is.
+//  - otherwise: every entity is a rigid body made of its spheres, in a clump
+//    context (sd_clump_*): sphere pose, LBVH broadphase over body boxes,
+//    sphere-sphere contacts with the linear spring-dashpot law, per-body force
+//    and torque gather, quaternion integration.
+// update(dt) advances one step of that path. Differences from the reference,
 // each one a correction of a documented defect:
 //  - inverse inertia is R I^-1 R^T with I the member spheres' own 2/5 m r^2
 //    plus the parallel-axis term (the reference inverts only det >= 1e-4 and
@@ -38,6 +50,7 @@
 #include "entity_handler.hpp"
 
 struct sd_clump_ctx;
+struct sd_ctx;
 
 namespace STARDUST {
 
@@ -75,6 +88,17 @@ public:
 
     // Copies the current body and sphere state into the EntityHandler host arrays.
     void sync();
+
+    // The step path transfer() chose, and (sphere path) its grid and walls.
+    struct SphereGrid {
+        float lo[3];
+        float cell;
+        int n[3];
+        bool walls;
+        float wall_lo[3], wall_hi[3];
+    };
+    bool sphereMode() const { return m_sphere_mode; }
+    const SphereGrid& sphereGrid() const { return m_sgrid; }
     long long stepsDone() const { return m_steps; }
 
     // Contact law and options (defaults: kn = 2e6 N/m as the reference's
@@ -87,11 +111,16 @@ public:
 
 private:
     void ensure_context(float dt);
+    void transfer_spheres();
+    void transfer_clumps();
 
     engineConfig engineConfiguration;
     Scalar m_domain;
     EntityHandler entity_handler;
     sd_clump_ctx* m_ctx = nullptr;
+    sd_ctx* m_sctx = nullptr;
+    bool m_sphere_mode = false;
+    SphereGrid m_sgrid{};
     bool m_uploaded = false;
     float m_dt = 0.0f;
     float m_kn = 2e6f, m_zeta = 0.3f, m_mu = 0.5f, m_kt_ratio = 2.0f / 7.0f;

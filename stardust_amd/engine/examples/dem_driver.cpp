@@ -10,6 +10,14 @@
 //                                           approaching lattice bodies) on the GPU
 //                                           and prints the packed input and the
 //                                           final body state
+//   sd_dem_driver run-spheres <steps> <dt> <n> <domain> <size> <lo> <hi> <seed>
+//                                           testDEM.cpp:80-128's pattern: n single
+//                                           spheres of diameter size at seeded
+//                                           positions uniform in [lo, hi]^3 (masses
+//                                           1..9, every 97th 100: static), engine
+//                                           domain <domain>; steps on the GPU and
+//                                           prints the packed input, the step path
+//                                           with its grid, and the final state
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -121,6 +129,50 @@ int main(int argc, char** argv) {
             print_f4("final_position", h.h_rigid_body_position_ptr, nb);
             print_f4("final_velocity", h.h_rigid_body_velocity_ptr, nb);
             print_f4("final_quaternion", h.h_rigid_body_quaternion_ptr, nb);
+            print_f4("final_angular_momentum", h.h_rigid_body_angular_momentum_ptr, nb, true);
+            std::printf("}\n");
+            return 0;
+        }
+        if (mode == "run-spheres" && argc >= 10) {
+            const int steps = std::atoi(argv[2]);
+            const float dt = (float)std::atof(argv[3]);
+            const int n = std::atoi(argv[4]);
+            const float domain = (float)std::atof(argv[5]), size = (float)std::atof(argv[6]);
+            const float lo = (float)std::atof(argv[7]), hi = (float)std::atof(argv[8]);
+            unsigned long long state = std::strtoull(argv[9], nullptr, 10);
+            auto uniform = [&state]() {  // SplitMix64, 24-bit uniform in [0, 1)
+                state += 0x9E3779B97F4A7C15ull;
+                unsigned long long z = state;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                return (float)(z >> 40) * (1.0f / 16777216.0f);
+            };
+            DEMEngine engine(config, domain);
+            for (int i = 0; i < n; ++i) {
+                const float4 pos = make_float4(lo + (hi - lo) * uniform(), lo + (hi - lo) * uniform(),
+                                               lo + (hi - lo) * uniform(), 0.0f);
+                const float4 vel = make_float4(uniform() - 0.5f, uniform() - 0.5f, uniform() - 0.5f, 0.0f);
+                const float mass = (i % 97 == 0) ? 100.0f : 1.0f + 8.0f * uniform();
+                engine.add(DEMParticle((unsigned)i, 1, size, size, mass, pos, vel));
+            }
+            engine.prep();
+            std::printf("{\n");
+            print_pack(engine.getEntityHandler());
+            engine.transfer();
+            const DEMEngine::SphereGrid& g = engine.sphereGrid();
+            std::printf("\"sphere_mode\": %s,\n", engine.sphereMode() ? "true" : "false");
+            std::printf("\"grid\": {\"lo\": [%.9g, %.9g, %.9g], \"cell\": %.9g, \"n\": [%d, %d, %d], "
+                        "\"walls\": %s, \"wall_lo\": [%.9g, %.9g, %.9g], \"wall_hi\": [%.9g, %.9g, %.9g]},\n",
+                        g.lo[0], g.lo[1], g.lo[2], g.cell, g.n[0], g.n[1], g.n[2], g.walls ? "true" : "false",
+                        g.wall_lo[0], g.wall_lo[1], g.wall_lo[2], g.wall_hi[0], g.wall_hi[1], g.wall_hi[2]);
+            for (int i = 0; i < steps; ++i) engine.update(dt);
+            engine.sync();
+            EntityHandler& h = engine.getEntityHandler();
+            const int nb = h.getNumberOfParticles();
+            std::printf("\"steps\": %lld,\n", engine.stepsDone());
+            print_f4("final_position", h.h_rigid_body_position_ptr, nb);
+            print_f4("final_velocity", h.h_rigid_body_velocity_ptr, nb);
             print_f4("final_angular_momentum", h.h_rigid_body_angular_momentum_ptr, nb, true);
             std::printf("}\n");
             return 0;

@@ -135,3 +135,48 @@ def test_engine_update_star_test_scene(oracle):
     print("[gpu-case] engine star_test.json", flush=True)
     d, st = run_and_compare(oracle, 3, 1e-5, STAR)
     assert np.all(np.isfinite(np.array(d["final_position"])))
+
+
+def run_spheres_and_compare(oracle, steps, dt, n, domain, size, lo, hi, seed):
+    """testDEM.cpp's pattern (single-sphere entities) through DEMEngine: the sphere
+    path, bit-identical to the sphere oracle stepped from the same packed input
+    (radius = diameter / 2, mass >= 10 static, the engine's default law)."""
+    d = drive("run-spheres", steps, dt, n, domain, size, lo, hi, seed)
+    assert d["sphere_mode"] is True and d["steps"] == steps
+    gr = d["grid"]
+    g = L.Grid.make(gr["lo"], np.float32(gr["cell"]), gr["n"])
+    walls = (gr["wall_lo"], gr["wall_hi"]) if gr["walls"] else None
+    c = L.ContactParams.make(2e6, 0.3, 0.5, np.float32(dt), gravity=(0.0, 0.0, -9.81), walls=walls,
+                             kt_ratio=2.0 / 7.0, max_contacts=24)
+    pos = np.array(d["rigid_body_position"], np.float32)
+    vel = np.array(d["rigid_body_velocity"], np.float32)
+    mass = np.array(d["rigid_body_mass"], np.float32)
+    r = (np.float32(0.5) * np.array(d["particle_size"], np.float32)).astype(np.float32)
+    m = np.where(mass >= 10.0, np.float32(0.0), mass).astype(np.float32)
+    st = oracle.SphereState(pos[:, 0], pos[:, 1], pos[:, 2], r, m, vx=vel[:, 0], vy=vel[:, 1], vz=vel[:, 2],
+                            max_contacts=24)
+    st.step(g, c, nsteps=steps, nthreads=8)
+    exp = st.by_id()
+    assert np.array_equal(np.array(d["final_position"], np.float32)[:, :3], exp["pos4"][:, :3])
+    assert np.array_equal(np.array(d["final_velocity"], np.float32)[:, :3], exp["vel4"][:, :3])
+    I = (np.float32(0.4) * mass * r * r).astype(np.float32)
+    L_exp = (I[:, None] * exp["omg4"][:, :3]).astype(np.float32)
+    assert np.array_equal(np.array(d["final_angular_momentum"], np.float32)[:, :3], L_exp)
+    return d, st
+
+
+@pytest.mark.gpu
+def test_engine_single_spheres_walled_equals_sphere_oracle(oracle):
+    print("[gpu-case] engine single spheres, walled domain", flush=True)
+    d, st = run_spheres_and_compare(oracle, 40, 1e-4, 6000, 1.0, 0.05, 0.05, 0.95, 11)
+    assert d["grid"]["walls"] is True
+    assert int(st.hcount.sum()) > 0  # contacts formed
+
+
+@pytest.mark.gpu
+def test_engine_single_spheres_unbounded_equals_sphere_oracle(oracle):
+    # the reference driver's own numbers: diameter 0.5, positions uniform in
+    # [1, 40]^3, engine domain 0.005 (smaller than the scene: unbounded)
+    print("[gpu-case] engine single spheres, unbounded (testDEM.cpp sizes)", flush=True)
+    d, st = run_spheres_and_compare(oracle, 30, 5e-4, 100000, 0.005, 0.5, 1.0, 40.0, 5)
+    assert d["grid"]["walls"] is False

@@ -5,6 +5,7 @@ pinned sphere law, and momentum conservation."""
 import math
 
 import numpy as np
+
 import pytest
 
 from stardust_amd import _lib as L
@@ -148,3 +149,55 @@ def test_clump_cluster_conserves_momentum(oracle):
     p1 = (m[:, None] * st.V[:, :3]).sum(0)
     scale = (m[:, None] * np.abs(st.V[:, :3])).sum()
     assert np.all(np.abs(p1 - p0) < 1e-6 * scale)
+
+
+def test_body_reduce_against_float64_sums(oracle):
+    """or_body_reduce: per-body sums of sphere forces and of (x - X) x f + tau,
+    against float64 sums (FP32 rounding only)."""
+    rng = np.random.default_rng(12)
+    nb = 50
+    count = rng.integers(1, 9, nb).astype(np.int32)
+    first = np.concatenate([[0], np.cumsum(count)[:-1]]).astype(np.int32)
+    ns = int(count.sum())
+    bpos = np.zeros((nb, 4), np.float32)
+    bpos[:, :3] = rng.uniform(-1, 1, (nb, 3))
+    owner = np.repeat(np.arange(nb), count)
+    spos = np.zeros((ns, 4), np.float32)
+    spos[:, :3] = bpos[owner, :3] + rng.normal(0, 0.01, (ns, 3))
+    sf = np.zeros((ns, 4), np.float32)
+    st = np.zeros((ns, 4), np.float32)
+    sf[:, :3] = rng.normal(0, 10, (ns, 3))
+    st[:, :3] = rng.normal(0, 0.1, (ns, 3))
+    bf, bt = oracle.body_reduce(bpos, first, count, spos, sf, st)
+    a = spos[:, :3].astype(np.float64) - bpos[owner, :3]
+    tq = np.cross(a, sf[:, :3].astype(np.float64)) + st[:, :3]
+    F64 = np.zeros((nb, 3))
+    T64 = np.zeros((nb, 3))
+    np.add.at(F64, owner, sf[:, :3].astype(np.float64))
+    np.add.at(T64, owner, tq)
+    assert np.allclose(bf[:, :3], F64, rtol=1e-5, atol=1e-4)
+    assert np.allclose(bt[:, :3], T64, rtol=1e-5, atol=1e-6)
+
+
+def test_integrate_bodies_free_spin_and_fall(oracle):
+    """or_integrate_bodies: a free spin about a principal axis follows
+    q(t) = exp(w t / 2) q0 and the body falls as z0 - g dt^2 n (n + 1) / 2."""
+    nb = 3
+    P = np.array([[0, 0, 1, 1.0], [1, 2, 3, 0.5], [0, 0, 0, 0.0]], np.float32)  # last: static
+    V = np.zeros((nb, 4), np.float32)
+    Q = np.tile(np.array([1, 0, 0, 0], np.float32), (nb, 1))
+    I = np.array([0.2, 0.3, 0.4], np.float64)  # principal moments (body frame = world at t = 0)
+    inv6 = np.zeros((nb, 6), np.float32)
+    inv6[:, :3] = 1.0 / I
+    w = np.array([0.0, 0.0, 3.0])  # about z
+    Lm = np.zeros((nb, 4), np.float32)
+    Lm[:, :3] = (I * w).astype(np.float32)
+    f = np.zeros((nb, 4), np.float32)
+    c = L.ContactParams.make(1e5, 0.3, 0.5, 1e-3, gravity=(0.0, 0.0, -9.81), walls=None)
+    n = 200
+    for _ in range(n):
+        P, V, Q, Lm = oracle.integrate_bodies(P, V, Q, Lm, inv6, f, f, c)
+    t = n * 1e-3
+    assert abs(Q[0, 0] - math.cos(1.5 * t)) < 2e-5 and abs(Q[0, 3] - math.sin(1.5 * t)) < 2e-5
+    assert abs(P[0, 2] - (1.0 - 9.81 * 1e-3 ** 2 * n * (n + 1) / 2)) < 2e-5
+    assert np.array_equal(P[2], np.array([0, 0, 0, 0], np.float32)) and np.array_equal(Q[2], [1, 0, 0, 0])
