@@ -14,7 +14,18 @@ dominant kernel (its time measured with HIP events on the scene stream in an
 eager profiling pass right after the timed region) and a CPU baseline (the
 oracle's full step on the host cores, bounded sample of the same workload).
 
-Further sections ride on the same line (--sections, default "c2_dense,batch,gjk,batch_gjk,clumps"):
+Two host-side legs ride inside "cpu_baseline" (off with --no-cpu):
+  fp64   — the GPU's FP32 step against FP64, per post-step quantity: spheres of
+           the contact-dense packing (32x32x16 lattice cells) after 3 steps, and
+           one rigid-clump step (13,824 clumps) integrated in float64.
+  cpu_plan — the C oracle on the host: configuration 1 (65,536 spheres) for
+           1,000 steps on all host threads (median of 3) and on one thread;
+           serial sweep-and-prune on the C1 sphere boxes and the C4 hull boxes;
+           cold GJK/EPA on the C4 pairs on one and on all threads; nproc and the
+           CPU model.
+
+Further sections ride on the same line (--sections, default
+"c2_dense,fp64,batch,gjk,batch_gjk,clumps,cpu_plan"):
   c2_dense — dem_c2_dense: the configuration-2 law on a contact-dense FCC packing of
            the same 4,194,304 spheres (about 3 contacts per sphere): particle-steps/s
            and the contact kernel's roofline, beside the dilute headline.
@@ -310,7 +321,7 @@ def gjk_section(args):
         t0 = time.perf_counter()
         cpairs, _ = O.sap_pairs(lo, hi, axis)
         sap_cpu_s = time.perf_counter() - t0
-        threads = min(os.cpu_count() or 1, 16)
+        threads = host_cpu()["threads_all"]
         pr = d_pairs_host(pairs, npairs)
         pr = np.ascontiguousarray(pr[:args.cpu_gjk_pairs] if args.cpu_gjk_pairs > 0 else pr)
         t0 = time.perf_counter()
@@ -538,6 +549,185 @@ def c2_dense_section(args):
             "packing": w.describe()["packing"]}
 
 
+def host_cpu():
+    """nproc, the threads this process may use, and the CPU model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    # the host threads the run may use: the process's CPU share (OMP_NUM_THREADS
+    # on a GPU box, which shares its host between GPUs), else every core it sees
+    share = int(os.environ.get("OMP_NUM_THREADS", avail) or avail)
+    return {"nproc": os.cpu_count(), "affinity": avail, "threads_all": max(1, min(share, avail)), "model": model}
+
+
+def cpu_plan_section(args):
+    """The CPU baselines the plan names (the reference has no CPU path; the C
+    oracle restates it): the full configuration-1 DEM step for
+    --cpu-c1-steps steps on all host threads (median of --cpu-reps runs) and on
+    one thread (median of --cpu-reps-1t runs), serial sweep-and-prune on the C1
+    sphere boxes and the C4 hull boxes, and cold GJK/EPA on the C4 pairs on one
+    thread and on all threads."""
+    import numpy as np
+    from oracle import oracle as O
+    from stardust_amd import hulls as H
+    from stardust_amd import workloads as W
+    hc = host_cpu()
+    thr = hc["threads_all"]
+    w = W.dem_c1()
+    s = w.generate()
+    g, c = w.grid(), w.contact()
+
+    def c1_run(threads, steps):
+        st = O.SphereState(s["x"], s["y"], s["z"], s["r"], s["m"], max_contacts=c.max_contacts)
+        t0 = time.perf_counter()
+        st.step(g, c, nsteps=steps, nthreads=threads)
+        return time.perf_counter() - t0
+
+    steps = args.cpu_c1_steps
+    all_t = sorted(c1_run(thr, steps) for _ in range(args.cpu_reps))
+    one_t = sorted(c1_run(1, steps) for _ in range(args.cpu_reps_1t))
+    med = lambda v: v[len(v) // 2]
+    log("cpu plan: C1 %d steps: %d threads %.2f s, 1 thread %.2f s (median)" % (steps, thr, med(all_t), med(one_t)))
+    lo = np.stack([s["x"] - s["r"], s["y"] - s["r"], s["z"] - s["r"], 0 * s["r"]], 1).astype(np.float32)
+    hi = np.stack([s["x"] + s["r"], s["y"] + s["r"], s["z"] + s["r"], 0 * s["r"]], 1).astype(np.float32)
+    t0 = time.perf_counter()
+    p1, _ = O.sap_pairs(lo, hi)
+    sap_c1 = time.perf_counter() - t0
+    h = H.hulls_c4()
+    hlo, hhi = O.hull_aabbs(h)
+    t0 = time.perf_counter()
+    p4, _ = O.sap_pairs(hlo, hhi)
+    sap_c4 = time.perf_counter() - t0
+    sample = p4[:args.cpu_gjk_1t_pairs]
+    t0 = time.perf_counter()
+    O.gjk(h, sample, nthreads=1)
+    gjk_1t = len(sample) / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    O.gjk(h, p4, nthreads=thr)
+    gjk_all = len(p4) / (time.perf_counter() - t0)
+    log("cpu plan: serial SAP C1 %.0f ms, C4 %.0f ms; GJK %.3e pairs/s (1 thread), %.3e (%d threads)" % (
+        1000 * sap_c1, 1000 * sap_c4, gjk_1t, gjk_all, thr))
+    return {"host": hc,
+            "note": "C oracle (gcc -O3 -ffp-contract=off, OpenMP); threads_all is the host share of this process "
+                    "(OMP_NUM_THREADS on a GPU box), nproc what the OS reports",
+            "dem_c1": {"steps": steps, "spheres": w.n,
+                       "all_threads": {"threads": thr, "runs_s": [round(x, 3) for x in all_t],
+                                       "particle_steps_per_s": w.n * steps / med(all_t)},
+                       "one_thread": {"threads": 1, "runs_s": [round(x, 3) for x in one_t],
+                                      "particle_steps_per_s": w.n * steps / med(one_t)}},
+            "sap_serial": {"c1_spheres": {"boxes": w.n, "pairs": int(len(p1)), "ms": round(1000 * sap_c1, 2)},
+                           "c4_hulls": {"boxes": len(hlo), "pairs": int(len(p4)), "ms": round(1000 * sap_c4, 2)},
+                           "threads": 1},
+            "gjk_cold_c4": {"one_thread_pairs_per_s": gjk_1t, "one_thread_sample_pairs": int(len(sample)),
+                            "all_threads_pairs_per_s": gjk_all, "all_threads": thr, "pairs": int(len(p4))}}
+
+
+def fp64_error_section(args):
+    """Error of the GPU's FP32 step against FP64, per post-step quantity
+    (norm-relative: max |a - b| over max |b|, and the 99.9th percentile of the
+    per-element error): spheres from the contact-dense packing (an FP64 step of
+    the same state: FP64 contact law and FP64 integration), rigid clumps from
+    the configuration-3 generator (FP64 integration of the same step's forces
+    and torques)."""
+    import math
+
+    import numpy as np
+    from oracle import oracle as O
+    from stardust_amd import _lib as L
+    from stardust_amd import clumps as CL
+    from stardust_amd import workloads as W
+    from stardust_amd.scene import SphereScene
+
+    def err(a, b):
+        e = np.abs(np.asarray(a, np.float64) - b) / max(float(np.abs(b).max()), 1e-300)
+        return {"max": float(e.max()), "p99_9": float(np.quantile(e, 0.999))}
+
+    w = W.dem_c2_dense(tuple(args.fp64_lattice))
+    s = w.generate()
+    g, c = w.grid(), w.contact()
+    sc = SphereScene(g, c, key64=True, use_graph=True)
+    sc.upload(s["x"], s["y"], s["z"], s["r"], s["m"])
+    sc.step(3)
+    got = sc.download()
+    sc.close()
+    st = O.SphereState(s["x"], s["y"], s["z"], s["r"], s["m"], max_contacts=c.max_contacts)
+    st.step(g, c, nsteps=2, nthreads=host_cpu()["threads_all"])
+    keys, vals = O.hash_keys(st.pos4[:, 0], st.pos4[:, 1], st.pos4[:, 2], g)
+    bits = max(1, int(math.ceil(math.log2(g.n_cells))))
+    k, v = O.sort_pairs_u32(keys, vals, 0, bits)
+    ranges = O.cell_ranges_u32(k, g.n_cells)
+    pos, vel, omg = (st.pos4[v].astype(np.float64), st.vel4[v].astype(np.float64), st.omg4[v].astype(np.float64))
+    h64 = st.hist.astype(np.float64)
+    h64[:, 3] = st.hist[:, 3].view(np.int32)
+    r = O.contact_forces_f64(pos, vel, omg, st.ids[v], g, ranges, c, v.astype(np.int32), st.hcount, h64)
+    grav = np.array([c.gravity[0], c.gravity[1], c.gravity[2]], np.float64)
+    live = vel[:, 3] > 0
+    vel[live, :3] += (r["force"][live, :3] * vel[live, 3:4] + grav) * float(c.dt)
+    pos[live, :3] += vel[live, :3] * float(c.dt)
+    omg[live, :3] += (r["torque"][live, :3] * omg[live, 3:4]) * float(c.dt)
+    inv = np.empty(st.n, np.int64)
+    inv[st.ids[v]] = np.arange(st.n)
+    pos, vel, omg = pos[inv], vel[inv], omg[inv]
+    spheres = {k2: err(got[k2], arr[:, col]) for k2, (arr, col) in
+               {"x": (pos, 0), "y": (pos, 1), "z": (pos, 2), "vx": (vel, 0), "vy": (vel, 1), "vz": (vel, 2),
+                "wx": (omg, 0), "wy": (omg, 1), "wz": (omg, 2)}.items()}
+    # clumps: one step of the GPU, against float64 integration of the oracle's
+    # forces and torques of that step (quaternion by the exact exponential)
+    cs = CL.generate_clumps(0x5EED0003, args.fp64_clumps)
+    cc = CL.contact_c3()
+    ost = O.ClumpState(cs)
+    scn = CL.ClumpScene(cc)
+    scn.upload(cs)
+    scn.step(1)
+    d = scn.download()
+    pairs = scn.pairs()
+    scn.close()
+    gp, gv, gq = d["pos4"], d["vel4"], d["quat4"]
+    F, T = ost.forces(cc, pairs)
+    P0, V0, Q0, L0 = (np.array(x, np.float64) for x in (ost.P, ost.V, ost.Q, ost.L))
+    dt = float(cc.dt)
+    gvec = np.array([cc.gravity[0], cc.gravity[1], cc.gravity[2]], np.float64)
+    im = P0[:, 3:4]
+    V1 = V0[:, :3] + (F[:, :3] * im + gvec) * dt * (im > 0)
+    P1 = P0[:, :3] + V1 * dt * (im > 0)
+    L1 = L0[:, :3] + T[:, :3] * dt * (im > 0)
+    q = Q0 / np.linalg.norm(Q0, axis=1, keepdims=True)
+    w0, x0, y0, z0 = q.T
+    Rm = np.stack([np.stack([1 - 2 * (y0 * y0 + z0 * z0), 2 * (x0 * y0 - w0 * z0), 2 * (x0 * z0 + w0 * y0)], -1),
+                   np.stack([2 * (x0 * y0 + w0 * z0), 1 - 2 * (x0 * x0 + z0 * z0), 2 * (y0 * z0 - w0 * x0)], -1),
+                   np.stack([2 * (x0 * z0 - w0 * y0), 2 * (y0 * z0 + w0 * x0), 1 - 2 * (x0 * x0 + y0 * y0)], -1)], 1)
+    I0, I1 = np.asarray(ost.I0, np.float64), np.asarray(ost.I1, np.float64)
+    Iinv = np.stack([np.stack([I0[:, 0], I0[:, 3], I1[:, 0]], -1), np.stack([I0[:, 3], I0[:, 1], I1[:, 1]], -1),
+                     np.stack([I1[:, 0], I1[:, 1], I0[:, 2]], -1)], 1)
+    wv = np.einsum("nij,njk,nlk,nl->ni", Rm, Iinv, Rm, L1)
+    wn = np.linalg.norm(wv, axis=1)
+    half = 0.5 * wn * dt
+    k = np.where(wn > 0, np.sin(half) / np.where(wn > 0, wn, 1), 0)
+    dq = np.concatenate([np.cos(half)[:, None], wv * k[:, None]], 1)
+    dw, dx, dy, dz = dq.T
+    Q1 = np.stack([dw * w0 - (dx * x0 + dy * y0 + dz * z0), dw * x0 + w0 * dx + (dy * z0 - dz * y0),
+                   dw * y0 + w0 * dy + (dz * x0 - dx * z0), dw * z0 + w0 * dz + (dx * y0 - dy * x0)], 1)
+    Q1 = np.where((im > 0), Q1 / np.linalg.norm(Q1, axis=1, keepdims=True), Q0)
+    clumps = {"x": err(gp[:, 0], P1[:, 0]), "vx": err(gv[:, 0], V1[:, 0]), "qw": err(gq[:, 0], Q1[:, 0]),
+              "qx": err(gq[:, 1], Q1[:, 1]), "qy": err(gq[:, 2], Q1[:, 2]), "qz": err(gq[:, 3], Q1[:, 3])}
+    log("fp64: spheres %s" % {k2: "%.1e" % v2["max"] for k2, v2 in spheres.items()})
+    log("fp64: clumps %s" % {k2: "%.1e" % v2["max"] for k2, v2 in clumps.items()})
+    return {"spheres": {"workload": w.name, "spheres": w.n, "after_steps": 3, "per_quantity": spheres,
+                        "reference": "FP64 contact law and integration of the same state (oracle twin)"},
+            "clumps": {"bodies": int(len(cs["pos4"])), "contact_pairs": int(len(pairs)), "per_quantity": clumps,
+                       "reference": "FP64 integration (exact quaternion exponential) of the step's FP32 forces "
+                                    "and torques"},
+            "note": "max is set by pairs at contact onset (the overlap of a just-touching pair rounds to either "
+                    "side of zero and the unclamped damping switches a finite force); p99_9 is the bulk"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -548,8 +738,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes of the C2 context")
-    ap.add_argument("--sections", default="c2_dense,batch,gjk,batch_gjk,clumps",
-                    help="extra sections: c2_dense, batch, gjk, batch_gjk, clumps (comma list, or none)")
+    ap.add_argument("--sections", default="c2_dense,fp64,batch,gjk,batch_gjk,clumps,cpu_plan",
+                    help="extra sections: c2_dense, fp64, batch, gjk, batch_gjk, clumps, cpu_plan (comma list, "
+                         "or none)")
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-warmup", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=20, help="steps per scene of the plain form (0: skip it)")
@@ -560,6 +751,12 @@ def main():
     ap.add_argument("--cpu-gjk-pairs", type=int, default=0, help="CPU GJK sample size (0: every SAP pair)")
     ap.add_argument("--clump-warmup", type=int, default=5)
     ap.add_argument("--clump-steps", type=int, default=50)
+    ap.add_argument("--cpu-c1-steps", type=int, default=1000)
+    ap.add_argument("--cpu-reps", type=int, default=3, help="runs of the all-threads C1 baseline (median)")
+    ap.add_argument("--cpu-reps-1t", type=int, default=1, help="runs of the one-thread C1 baseline (median)")
+    ap.add_argument("--cpu-gjk-1t-pairs", type=int, default=100000)
+    ap.add_argument("--fp64-lattice", type=int, nargs=3, default=[32, 32, 16])
+    ap.add_argument("--fp64-clumps", type=int, default=13824)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the plain copy kernel on 256 MiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -582,7 +779,7 @@ def main():
     value, t, tune, g = r["value"], r["t"], r["tune"], r["grid"]
     cpu = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        threads = host_cpu()["threads_all"]
         log("cpu baseline: oracle full step, %d steps, %d threads" % (args.cpu_steps, threads))
         v_cpu, dt_cpu = cpu_baseline(w, s, args.cpu_steps, threads)
         cpu = {"value": v_cpu, "unit": "particle-steps/s", "cores": threads, "kind": "port",
@@ -621,6 +818,12 @@ def main():
     sections = [x for x in args.sections.split(",") if x and x != "none"]
     if "c2_dense" in sections:
         out["c2_dense"] = c2_dense_section(args)
+    # the host-side legs: the oracle runs here only as the CPU baseline and as
+    # the FP64 checker of the GPU's post-step quantities
+    if cpu is not None and "fp64" in sections:
+        cpu["fp64_error"] = fp64_error_section(args)
+    if cpu is not None and "cpu_plan" in sections:
+        cpu["plan"] = cpu_plan_section(args)
     if "batch" in sections:
         out["batch"] = batch_section(args)
     if "gjk" in sections:

@@ -467,6 +467,33 @@ sd_status sd_mlp_pack_bf16(const float* params, void* packed, void* stream);
 sd_status sd_mlp_infer_bf16(const float* X, int n, const void* packed, const int* pairs, const float* quat4,
                             float* out_dir3, void* stream);
 
+/* ------------------------------------------------------- compat restatements
+ * The reference's own arithmetic, quirks included, for the formulas the
+ * library replaces (no step calls these): parity tests use them to show the
+ * reference's formulas are reproduced bit for bit.
+ *
+ * Pair force of computeParticleToParticleCUDA
+ * (src/engine/cuda/collisions/SAP/sap_reaction.cu:20-70) for each pair
+ * (home, phantom): separation = size[home] + size[phantom] (the reference
+ * stores diameters there), relative position / velocity as float4 (w lanes
+ * included), tangential velocity v - (v * n) * n component-wise, total =
+ * -kn (sep - d) n + damping v + kt vt, multiplied component-wise by the signs
+ * of v (zero counts as +1): force_home4[k] = sgn * total, force_phantom4[k] =
+ * -sgn * total (the reference adds them unsynchronised into both spheres). */
+sd_status sd_compat_pair_forces(const float* pos4, const float* vel4, const float* size, const int* pairs,
+                                int npairs, float kn, float kt, float damping, float* force_home4,
+                                float* force_phantom4, void* stream);
+/* compute3x3Inverse (src/engine/cuda/cuda_helpers.cuh:65-77) of n row-ordered
+ * float9 matrices: the input itself when the determinant is below 1e-4. */
+sd_status sd_compat_inverse3x3(const float* m9, int n, float* out9, void* stream);
+/* World angular velocity of advectParticlesCUDA (physics_update.cu:271-277):
+ * w = M L with M = (R inv(I)) R^T evaluated by the column-major helpers
+ * (cuda_helpers.cuh:79-112) on the row-major quatToRotationCUDA matrix
+ * (:114-128), inv() the compat inverse; quat4 = (s, vx, vy, vz), inertia9 the
+ * body's float9 inertia tensor, angmom4 world L. */
+sd_status sd_compat_angular_velocity(const float* quat4, const float* inertia9, const float* angmom4, int nb,
+                                     float* omega4, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

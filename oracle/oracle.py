@@ -52,6 +52,9 @@ _SIGS = {
     "or_morton30": ([_I, _P, _P, _P, _P], None),
     "or_clump_contacts_integrate": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_long, _P, C.POINTER(OrContact),
                                      _P, _P, _I], _I),
+    "or_compat_pair_forces": ([_I, _P, _P, _P, _P, C.c_float, C.c_float, C.c_float, _P, _P], None),
+    "or_compat_inverse3x3": ([_I, _P, _P], None),
+    "or_compat_angular_velocity": ([_I, _P, _P, _P, _P], None),
     "or_body_reduce": ([_I, _P, _P, _P, _P, _P, _P, _P, _P], None),
     "or_integrate_bodies": ([_I, _P, _P, _P, _P, _P, _P, _P, C.POINTER(OrContact)], None),
     "or_clump_step": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, C.POINTER(OrContact)], C.c_long),
@@ -375,3 +378,26 @@ def integrate_bodies(P, V, Q, L, inv6, f4, t4, c):
     oc = contact(c)
     load().or_integrate_bodies(len(P), p(P), p(V), p(Q), p(L), p(a[0]), p(a[1]), p(a[2]), C.byref(oc))
     return P, V, Q, L
+
+
+def compat_pair_forces(pairs, pos4, vel4, size, kn, kt, damping):
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (pos4, vel4, size)]
+    n = len(pairs)
+    fh, fp = np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)
+    load().or_compat_pair_forces(n, p(pairs), p(a[0]), p(a[1]), p(a[2]), kn, kt, damping, p(fh), p(fp))
+    return fh, fp
+
+
+def compat_inverse3x3(m9):
+    m = np.ascontiguousarray(m9, np.float32).reshape(-1, 9)
+    o = np.zeros_like(m)
+    load().or_compat_inverse3x3(len(m), p(m), p(o))
+    return o
+
+
+def compat_angular_velocity(quat4, inertia9, angmom4):
+    a = [np.ascontiguousarray(x, np.float32) for x in (quat4, inertia9, angmom4)]
+    o = np.zeros((len(a[0]), 4), np.float32)
+    load().or_compat_angular_velocity(len(a[0]), p(a[0]), p(a[1]), p(a[2]), p(o))
+    return o

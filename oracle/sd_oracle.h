@@ -128,6 +128,16 @@ void or_integrate_bodies(int nb, float* P, float* V, float* Q, float* L, const f
 long or_clump_step(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const int* first,
                    const int* count, const float* local4, int ns, const or_contact* c);
 
+/* ------------------------------------------------------------------ compat
+ * (sd_oracle_compat.c) the reference's own arithmetic, quirks included:
+ * sap_reaction.cu:20-70 pair force, cuda_helpers.cuh:65-77 inverse,
+ * physics_update.cu:271-277 angular velocity with cuda_helpers.cuh:79-128. */
+void or_compat_pair_forces(int npairs, const int* pairs, const float* pos4, const float* vel4, const float* size,
+                           float kn, float kt, float damping, float* fh4, float* fp4);
+void or_compat_inverse3x3(int n, const float* m9, float* o9);
+void or_compat_angular_velocity(int nb, const float* quat4, const float* inertia9, const float* angmom4,
+                                float* omega4);
+
 #ifdef __cplusplus
 }
 #endif

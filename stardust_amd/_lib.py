@@ -113,6 +113,9 @@ _SIGS = {
     "sd_body_reduce": [_P, _P, _P, _I, _P, _P, _P, _P, _P, _P],
     "sd_integrate_bodies": [_P, _P, _P, _P, _P, _P, _P, _I, C.POINTER(ContactParams), _P],
     "sd_mlp_forward": [_P, _I, _P, _P, _P],
+    "sd_compat_pair_forces": [_P, _P, _P, _P, _I, _F, _F, _F, _P, _P, _P],
+    "sd_compat_inverse3x3": [_P, _I, _P, _P],
+    "sd_compat_angular_velocity": [_P, _P, _P, _I, _P, _P],
     "sd_cell_ranges_u32": [_P, _I, _I, _P, _P],
     "sd_cell_ranges_u64": [_P, _I, _I, _P, _P],
     "sd_neighbor_pairs": [_P, _I, C.POINTER(Grid), _P, _F, _P, _I, C.POINTER(C.c_int), _P, _SZ, _P],
