@@ -256,10 +256,27 @@ sd_status sd_ctx_diagnostics(sd_ctx* ctx, double out[8]);
 /* Stream the context launches on (hipStream_t as void*). */
 sd_status sd_ctx_stream(sd_ctx* ctx, void** stream);
 /* Save / restore the complete device state (positions, velocities, contact history,
- * order) to a host byte buffer. sd_ctx_state_bytes gives the size. */
+ * order) to a host byte buffer. sd_ctx_state_bytes gives the size. The buffer
+ * (format version 2) records the sizes, the number of scenes and a hash of the
+ * scene table (batch context: every scene's grid, contact law, cell and slot
+ * range) or of the grid (single scene); sd_ctx_load rejects a buffer whose
+ * sizes, scene count or hash differ from the context's with SD_ERR_ARG. */
 sd_status sd_ctx_state_bytes(sd_ctx* ctx, size_t* bytes);
 sd_status sd_ctx_save(sd_ctx* ctx, void* buffer, size_t bytes);
 sd_status sd_ctx_load(sd_ctx* ctx, const void* buffer, size_t bytes);
+/* Runtime add / remove of spheres (single-scene contexts; the reference's
+ * EntityHandler::addEntity / removeEntity are stubs, entity_handler.cpp:108-119).
+ * sd_ctx_remove_spheres drops the spheres with the given particle ids and
+ * renumbers the rest in id order (new id = old id minus the removed ids below
+ * it); the state rows keep their slot order, the contact history keeps every
+ * entry whose partner stays (partner ids renumbered). sd_ctx_add_spheres
+ * appends count spheres with ids n .. n + count - 1 (no history; 1/m and 1/I
+ * from mass as at upload). Both reallocate the context, hash the keys from the
+ * positions, make the next sort the full one and drop the captured step
+ * graphs (the next sd_ctx_step captures new ones). */
+sd_status sd_ctx_remove_spheres(sd_ctx* ctx, const int* ids, int count);
+sd_status sd_ctx_add_spheres(sd_ctx* ctx, int count, const float* x, const float* y, const float* z, const float* vx,
+                             const float* vy, const float* vz, const float* radius, const float* mass);
 /* Autotuner: times trial_steps eager steps for every radix-sort tile size
  * (4, 8, 16 items per thread) and contact block size (64, 128, 256), keeps the
  * fastest pair and returns it with its time per step. Results of a step do not
@@ -421,6 +438,27 @@ sd_status sd_clump_download(sd_clump_ctx* ctx, float* pos4, float* vel4, float* 
  * kernel); SD_ERR_CAPACITY with *n_pairs set when capacity is too small. */
 sd_status sd_clump_pairs(sd_clump_ctx* ctx, int* pairs, int capacity, int* n_pairs);
 sd_status sd_clump_spheres(sd_clump_ctx* ctx, float* pos4);
+/* Checkpoint of a clump context: body positions, velocities, orientations and
+ * angular momenta plus the step count (sd_clump_state_bytes gives the size).
+ * sd_clump_load restores it into a context holding the same bodies (same
+ * counts and the same inertia, sphere ranges and offsets, checked by a hash;
+ * SD_ERR_ARG otherwise); stepping on from it is bit-identical to stepping on
+ * from the saved context. Replaces the position read-back of the reference's
+ * GL interop path (src/engine/engine.cpp:36-62) for resume and file output. */
+sd_status sd_clump_state_bytes(sd_clump_ctx* ctx, size_t* bytes);
+sd_status sd_clump_save(sd_clump_ctx* ctx, void* buffer, size_t bytes);
+sd_status sd_clump_load(sd_clump_ctx* ctx, const void* buffer, size_t bytes);
+/* Runtime add / remove of bodies (the reference's EntityHandler::addEntity /
+ * removeEntity are stubs, entity_handler.cpp:108-119). sd_clump_remove_bodies
+ * drops the listed bodies and their spheres; the rest keep their order and are
+ * renumbered (new id = old id minus the removed ids below it).
+ * sd_clump_add_bodies appends bodies given as in sd_clump_upload (first[] is
+ * relative to the added local4 rows); they get ids nb, nb + 1, .... Both keep
+ * the state of the other bodies, reallocate and drop the captured graphs. */
+sd_status sd_clump_remove_bodies(sd_clump_ctx* ctx, const int* ids, int count);
+sd_status sd_clump_add_bodies(sd_clump_ctx* ctx, int nb, const float* pos4, const float* vel4, const float* quat4,
+                              const float* angvel4, const float* inertia6, const int* first, const int* count,
+                              int ns, const float* local4);
 /* Clumps of count U{c_min..c_max} spheres with centres N(0, sigma) about the
  * body centre, radii U[r_min, r_max], density rho; bodies on a jittered cubic
  * lattice (spacing, jitter) with uniform orientations and velocities

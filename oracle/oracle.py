@@ -76,6 +76,32 @@ class ClumpState:
         self.count = np.ascontiguousarray(s["count"], np.int32)
         self.local4 = np.ascontiguousarray(s["local4"], np.float32)
 
+    def remove(self, ids):
+        """Runtime removal (EntityHandler::removeEntity is a stub in the
+        reference, src/engine/entities/entity_handler.cpp:108-119): drops the
+        bodies and their sphere rows; the rest keep their order."""
+        keep = np.ones(self.nb, bool)
+        keep[np.asarray(ids, np.int64)] = False
+        rows = np.concatenate([np.arange(f, f + c) for f, c in zip(self.first[keep], self.count[keep])])
+        for k in ("P", "V", "Q", "L", "I0", "I1"):
+            setattr(self, k, np.ascontiguousarray(getattr(self, k)[keep]))
+        self.count = np.ascontiguousarray(self.count[keep])
+        self.first = np.ascontiguousarray(np.concatenate([[0], np.cumsum(self.count)[:-1]]).astype(np.int32))
+        self.local4 = np.ascontiguousarray(self.local4[rows])
+        self.nb, self.ns = int(keep.sum()), len(rows)
+
+    def add(self, s):
+        """Runtime addition: appends the bodies of s (prepared as at
+        construction) after the current ones."""
+        new = ClumpState(s)
+        for k in ("P", "V", "Q", "L", "I0", "I1"):
+            setattr(self, k, np.ascontiguousarray(np.concatenate([getattr(self, k), getattr(new, k)])))
+        self.first = np.ascontiguousarray(np.concatenate([self.first, new.first + self.ns]).astype(np.int32))
+        self.count = np.ascontiguousarray(np.concatenate([self.count, new.count]).astype(np.int32))
+        self.local4 = np.ascontiguousarray(np.concatenate([self.local4, new.local4]))
+        self.nb += new.nb
+        self.ns += new.ns
+
     def pose(self):
         sx = np.zeros((self.ns, 4), np.float32)
         sv = np.zeros((self.ns, 4), np.float32)
@@ -355,6 +381,56 @@ class SphereState:
         self.hcount = self.hcount[inv]
         self.hist = np.ascontiguousarray(self.hist.reshape(self.maxc, self.n, 4)[:, inv].reshape(-1, 4))
         self.ids = np.arange(self.n, dtype=np.int32)
+
+    def remove(self, ids):
+        """Runtime removal (the reference's EntityHandler::removeEntity is a
+        stub, src/engine/entities/entity_handler.cpp:108-119): drops the spheres
+        with these ids, renumbers the rest in id order, keeps the slot order of
+        the remaining rows and every history entry whose partner stays (sphere
+        partners renumbered, walls kept)."""
+        ids = np.asarray(ids, np.int64)
+        gone = np.zeros(self.n, bool)
+        gone[ids] = True
+        newid = np.cumsum(~gone) - 1
+        newid[gone] = -1
+        keep = ~gone[self.ids]
+        m = self.n - len(ids)
+        hist = self.hist.reshape(self.maxc, self.n, 4)
+        hid = hist[:, :, 3].view(np.int32)
+        nh = np.zeros((self.maxc, m, 4), np.float32)
+        nc = np.zeros(m, np.int32)
+        for j, i in enumerate(np.nonzero(keep)[0]):
+            h = 0
+            for k in range(min(int(self.hcount[i]), self.maxc)):
+                pid = int(hid[k, i])
+                e = hist[k, i].copy()
+                if pid >= 0:
+                    if newid[pid] < 0:
+                        continue
+                    e[3] = np.array([newid[pid]], np.int32).view(np.float32)[0]
+                nh[h, j] = e
+                h += 1
+            nc[j] = h
+        self.pos4, self.vel4, self.omg4 = self.pos4[keep], self.vel4[keep], self.omg4[keep]
+        self.ids = newid[self.ids[keep]].astype(np.int32)
+        self.hcount = nc
+        self.hist = nh.reshape(-1, 4)
+        self.n = m
+
+    def add(self, x, y, z, r, m, vx=None, vy=None, vz=None):
+        """Runtime addition: appends spheres with ids n, n + 1, ... and no
+        contact history (1/m, 1/I as at construction)."""
+        new = SphereState(x, y, z, r, m, vx, vy, vz, max_contacts=self.maxc)
+        n, k = self.n, new.n
+        hist = np.zeros((self.maxc, n + k, 4), np.float32)
+        hist[:, :n] = self.hist.reshape(self.maxc, n, 4)
+        self.pos4 = np.concatenate([self.pos4, new.pos4])
+        self.vel4 = np.concatenate([self.vel4, new.vel4])
+        self.omg4 = np.concatenate([self.omg4, new.omg4])
+        self.ids = np.concatenate([self.ids, new.ids + n]).astype(np.int32)
+        self.hcount = np.concatenate([self.hcount, new.hcount])
+        self.hist = hist.reshape(-1, 4)
+        self.n = n + k
 
     def by_id(self):
         """State rows reordered to particle-id order."""

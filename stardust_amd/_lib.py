@@ -138,6 +138,8 @@ _SIGS = {
     "sd_ctx_state_bytes": [_P, C.POINTER(_SZ)],
     "sd_ctx_save": [_P, _P, _SZ],
     "sd_ctx_load": [_P, _P, _SZ],
+    "sd_ctx_remove_spheres": [_P, _P, _I],
+    "sd_ctx_add_spheres": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P],
     "sd_ctx_autotune": [_P, _I, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)],
     "sd_batch_create": [C.POINTER(BatchParams), C.POINTER(C.c_void_p)],
     "sd_batch_upload": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P],
@@ -164,6 +166,11 @@ _SIGS = {
     "sd_clump_download": [_P, _P, _P, _P, _P],
     "sd_clump_pairs": [_P, _P, _I, C.POINTER(C.c_int)],
     "sd_clump_spheres": [_P, _P],
+    "sd_clump_state_bytes": [_P, C.POINTER(_SZ)],
+    "sd_clump_save": [_P, _P, _SZ],
+    "sd_clump_load": [_P, _P, _SZ],
+    "sd_clump_remove_bodies": [_P, _P, _I],
+    "sd_clump_add_bodies": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P],
     "sd_gen_clumps": [C.c_uint64, _I, _I, _F, _F, _I, _I, _F, _F, _F, _F, _F, _P, _P, _P, _P, _P, _P, _P, _P,
                       C.POINTER(C.c_int)],
     "sd_mlp_param_count": [],

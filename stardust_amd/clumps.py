@@ -61,6 +61,7 @@ class ClumpScene:
 
     def upload(self, s):
         self.nb, self.ns = len(s["pos4"]), len(s["local4"])
+        self.counts = np.asarray(s["count"], np.int64).copy()
         a = [np.ascontiguousarray(s[k]) for k in ("pos4", "vel4", "quat4", "angvel4", "inertia6", "first", "count")]
         loc = np.ascontiguousarray(s["local4"], np.float32)
         L.call("sd_clump_upload", self.h, self.nb, *[L.ptr(x) for x in a], self.ns, L.ptr(loc))
@@ -90,6 +91,35 @@ class ClumpScene:
         out = np.zeros((self.ns, 4), np.float32)
         L.call("sd_clump_spheres", self.h, L.ptr(out))
         return out
+
+    def save(self):
+        nb = C.c_size_t()
+        L.call("sd_clump_state_bytes", self.h, C.byref(nb))
+        buf = np.empty(nb.value, np.uint8)
+        L.call("sd_clump_save", self.h, L.ptr(buf), nb.value)
+        return buf
+
+    def load(self, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        L.call("sd_clump_load", self.h, L.ptr(buf), buf.nbytes)
+
+    def remove(self, ids):
+        """Drops these bodies and their spheres; the rest are renumbered in order."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        L.call("sd_clump_remove_bodies", self.h, L.ptr(ids), len(ids))
+        self.counts = np.delete(self.counts, ids)
+        self.nb, self.ns = len(self.counts), int(self.counts.sum())
+
+    def add(self, s):
+        """Appends the bodies of a generate_clumps-style dict; returns their ids."""
+        a = [np.ascontiguousarray(s[k]) for k in ("pos4", "vel4", "quat4", "angvel4", "inertia6", "first", "count")]
+        loc = np.ascontiguousarray(s["local4"], np.float32)
+        k = len(s["pos4"])
+        L.call("sd_clump_add_bodies", self.h, k, *[L.ptr(x) for x in a], len(loc), L.ptr(loc))
+        first = self.nb
+        self.counts = np.concatenate([self.counts, np.asarray(s["count"], np.int64)])
+        self.nb, self.ns = len(self.counts), int(self.counts.sum())
+        return np.arange(first, first + k)
 
     def close(self):
         if self.h is not None and self.h.value:

@@ -24,6 +24,7 @@
 //   dq = (cos a, w sin(a)/|w|), q = normalise(dq q)
 // sin and cos come from fixed Taylor polynomials for a < 0.25 (exact to FP32)
 // so the CPU restatement reproduces the bits; larger angles use sinf/cosf.
+#include <cstring>
 #include <vector>
 
 #include "sd_contact_law.hpp"
@@ -384,6 +385,7 @@ struct sd_clump_ctx {
     hipGraph_t graph_obj[2] = {nullptr, nullptr};
     float4 *force = nullptr, *torque = nullptr;
     long long steps_done = 0;
+    uint64_t static_hash = 0;  // of the fixed rows (inverse inertia, sphere ranges and offsets)
 };
 
 namespace {
@@ -532,30 +534,124 @@ extern "C" sd_status sd_clump_destroy(sd_clump_ctx* c) {
 // Bodies: pos4 (x, y, z, mass), vel4 (v, 0), quat4 (w, x, y, z), angvel4 (w, 0) world,
 // inertia6 (body frame xx, yy, zz, xy, xz, yz), first/count sphere ranges.
 // Spheres: local4 (body-frame offset from the centre of mass, radius).
-extern "C" sd_status sd_clump_upload(sd_clump_ctx* c, int nb, const float* pos4, const float* vel4,
-                                     const float* quat4, const float* angvel4, const float* inertia6,
-                                     const int* first, const int* count, int ns, const float* local4) {
-    SD_REQUIRE(c && nb >= 1 && ns >= 1, "sd_clump_upload: bad sizes");
-    SD_REQUIRE(pos4 && vel4 && quat4 && angvel4 && inertia6 && first && count && local4, "sd_clump_upload: NULL array");
-    SD_REQUIRE((uint64_t)nb * (uint64_t)nb < (1ull << 62), "sd_clump_upload: too many bodies");
-    std::vector<int> sbody(ns, -1);
+namespace {
+
+// Body rows as the context keeps them: P = (x, y, z, 1/m), V = (v, 0),
+// Q = (w, x, y, z), L = world angular momentum, I0/I1 = the body-frame inverse
+// inertia (xx, yy, zz, xy) / (xz, yz, 0, 0); first/count index the sphere
+// rows local4 = (body-frame offset, r).
+struct ClumpRows {
+    int nb = 0, ns = 0;
+    std::vector<float4> P, V, Q, L, I0, I1, loc;
+    std::vector<int> first, count;
+    void resize(int b, int sp) {
+        nb = b;
+        ns = sp;
+        P.resize(b);
+        V.resize(b);
+        Q.resize(b);
+        L.resize(b);
+        I0.resize(b);
+        I1.resize(b);
+        first.resize(b);
+        count.resize(b);
+        loc.resize(sp);
+    }
+};
+
+// Checks the body -> sphere ranges and fills sbody (every sphere in exactly one body).
+sd_status body_ranges(const char* who, int nb, const int* first, const int* count, int ns, std::vector<int>& sbody) {
+    sbody.assign(ns, -1);
     for (int b = 0; b < nb; ++b) {
         if (count[b] < 1 || first[b] < 0 || first[b] + count[b] > ns)
-            return set_error(SD_ERR_ARG, "sd_clump_upload: body %d has a bad sphere range", b);
-        if (!(pos4[4 * (size_t)b + 3] >= 0.0f))
-            return set_error(SD_ERR_ARG, "sd_clump_upload: body %d has a negative mass", b);
+            return set_error(SD_ERR_ARG, "%s: body %d has a bad sphere range", who, b);
         for (int k = first[b]; k < first[b] + count[b]; ++k) {
-            if (sbody[k] != -1) return set_error(SD_ERR_ARG, "sd_clump_upload: sphere %d belongs to two bodies", k);
+            if (sbody[k] != -1) return set_error(SD_ERR_ARG, "%s: sphere %d belongs to two bodies", who, k);
             sbody[k] = b;
         }
     }
     for (int k = 0; k < ns; ++k)
-        if (sbody[k] < 0) return set_error(SD_ERR_ARG, "sd_clump_upload: sphere %d belongs to no body", k);
+        if (sbody[k] < 0) return set_error(SD_ERR_ARG, "%s: sphere %d belongs to no body", who, k);
+    return SD_OK;
+}
+
+// The rows of nb bodies from the upload arguments (inverse inertia and world
+// angular momentum in double precision on the host), written at rows[at ..].
+sd_status prepare_bodies(const char* who, int nb, const float* pos4, const float* vel4, const float* quat4,
+                         const float* angvel4, const float* inertia6, ClumpRows& r, int at) {
+    for (int b = 0; b < nb; ++b) {
+        const float* p = pos4 + 4 * (size_t)b;
+        const float m = p[3];
+        if (!(m >= 0.0f)) return set_error(SD_ERR_ARG, "%s: body %d has a negative mass", who, b);
+        r.P[at + b] = make_float4(p[0], p[1], p[2], m > 0.0f ? 1.0f / m : 0.0f);
+        const float* v = vel4 + 4 * (size_t)b;
+        r.V[at + b] = m > 0.0f ? make_float4(v[0], v[1], v[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float* q = quat4 + 4 * (size_t)b;
+        r.Q[at + b] = make_float4(q[0], q[1], q[2], q[3]);
+        // inverse of the symmetric body inertia (double precision on the host)
+        const float* I = inertia6 + 6 * (size_t)b;
+        const double a = I[0], e = I[1], f = I[2], xy = I[3], xz = I[4], yz = I[5];
+        const double c00 = e * f - yz * yz, c01 = xz * yz - xy * f, c02 = xy * yz - xz * e;
+        const double det = a * c00 + xy * c01 + xz * c02;
+        if (m > 0.0f && !(det > 0.0))
+            return set_error(SD_ERR_ARG, "%s: body %d has a singular inertia tensor", who, b);
+        double inv[6] = {0, 0, 0, 0, 0, 0};
+        if (m > 0.0f) {
+            inv[0] = c00 / det;
+            inv[1] = (a * f - xz * xz) / det;
+            inv[2] = (a * e - xy * xy) / det;
+            inv[3] = c01 / det;
+            inv[4] = c02 / det;
+            inv[5] = (xz * xy - a * yz) / det;
+        }
+        r.I0[at + b] = make_float4((float)inv[0], (float)inv[1], (float)inv[2], (float)inv[3]);
+        r.I1[at + b] = make_float4((float)inv[4], (float)inv[5], 0.0f, 0.0f);
+        // L = R I R^T w (world), in double
+        const float* w = angvel4 + 4 * (size_t)b;
+        const double qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+        const double R[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy),
+                             2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx),
+                             2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)};
+        const double Ib[9] = {a, xy, xz, xy, e, yz, xz, yz, f};
+        double u[3], t[3], Lw[3];
+        for (int k = 0; k < 3; ++k) u[k] = R[0 + k] * w[0] + R[3 + k] * w[1] + R[6 + k] * w[2];
+        for (int k = 0; k < 3; ++k) t[k] = Ib[3 * k] * u[0] + Ib[3 * k + 1] * u[1] + Ib[3 * k + 2] * u[2];
+        for (int k = 0; k < 3; ++k) Lw[k] = R[3 * k] * t[0] + R[3 * k + 1] * t[1] + R[3 * k + 2] * t[2];
+        r.L[at + b] = m > 0.0f ? make_float4((float)Lw[0], (float)Lw[1], (float)Lw[2], 0.0f)
+                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    return SD_OK;
+}
+
+// FNV-1a 64 over the rows that stay fixed while stepping (inverse inertia,
+// sphere ranges and offsets): a checkpoint is loaded only into a context
+// holding the same bodies.
+uint64_t static_hash(const ClumpRows& r) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&](const void* p, size_t sz) {
+        const unsigned char* b = (const unsigned char*)p;
+        for (size_t i = 0; i < sz; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+    };
+    mix(r.I0.data(), r.I0.size() * sizeof(float4));
+    mix(r.I1.data(), r.I1.size() * sizeof(float4));
+    mix(r.first.data(), r.first.size() * sizeof(int));
+    mix(r.count.data(), r.count.size() * sizeof(int));
+    mix(r.loc.data(), r.loc.size() * sizeof(float4));
+    return h;
+}
+
+// (Re)allocates the context for r.nb bodies and r.ns spheres and uploads the
+// rows at buffer parity 0; drops the captured graphs.
+sd_status write_clump_rows(sd_clump_ctx* c, const ClumpRows& r) {
+    const int nb = r.nb, ns = r.ns;
+    std::vector<int> sbody;
+    sd_status st = body_ranges("sd_clump", nb, r.first.data(), r.count.data(), ns, sbody);
+    if (st != SD_OK) return st;
     (void)hipStreamSynchronize(c->stream);
+    drop_clump_graphs(c);
     cfree(c);
     c->nb = nb;
     c->ns = ns;
-    sd_status st = SD_OK;
 #define ALLOC(ptr, cnt)                  \
     if (st == SD_OK) st = dalloc(&(ptr), (size_t)(cnt))
     for (int k = 0; k < 2; ++k) {
@@ -602,67 +698,69 @@ extern "C" sd_status sd_clump_upload(sd_clump_ctx* c, int nb, const float* pos4,
         cfree(c);
         return st;
     }
-    std::vector<float4> P(nb), V(nb), Q(nb), Lm(nb), I0(nb), I1(nb), loc(ns);
-    for (int b = 0; b < nb; ++b) {
-        const float* p = pos4 + 4 * (size_t)b;
-        const float m = p[3];
-        P[b] = make_float4(p[0], p[1], p[2], m > 0.0f ? 1.0f / m : 0.0f);
-        const float* v = vel4 + 4 * (size_t)b;
-        V[b] = m > 0.0f ? make_float4(v[0], v[1], v[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const float* q = quat4 + 4 * (size_t)b;
-        Q[b] = make_float4(q[0], q[1], q[2], q[3]);
-        // inverse of the symmetric body inertia (double precision on the host)
-        const float* I = inertia6 + 6 * (size_t)b;
-        const double a = I[0], e = I[1], f = I[2], xy = I[3], xz = I[4], yz = I[5];
-        const double c00 = e * f - yz * yz, c01 = xz * yz - xy * f, c02 = xy * yz - xz * e;
-        const double det = a * c00 + xy * c01 + xz * c02;
-        if (m > 0.0f && !(det > 0.0))
-            return set_error(SD_ERR_ARG, "sd_clump_upload: body %d has a singular inertia tensor", b);
-        double inv[6] = {0, 0, 0, 0, 0, 0};
-        if (m > 0.0f) {
-            inv[0] = c00 / det;
-            inv[1] = (a * f - xz * xz) / det;
-            inv[2] = (a * e - xy * xy) / det;
-            inv[3] = c01 / det;
-            inv[4] = c02 / det;
-            inv[5] = (xz * xy - a * yz) / det;
-        }
-        I0[b] = make_float4((float)inv[0], (float)inv[1], (float)inv[2], (float)inv[3]);
-        I1[b] = make_float4((float)inv[4], (float)inv[5], 0.0f, 0.0f);
-        // L = R I R^T w (world), in double
-        const float* w = angvel4 + 4 * (size_t)b;
-        const double qw = q[0], qx = q[1], qy = q[2], qz = q[3];
-        const double R[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy),
-                             2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx),
-                             2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)};
-        const double Ib[9] = {a, xy, xz, xy, e, yz, xz, yz, f};
-        double u[3], t[3], Lw[3];
-        for (int r = 0; r < 3; ++r) u[r] = R[0 + r] * w[0] + R[3 + r] * w[1] + R[6 + r] * w[2];
-        for (int r = 0; r < 3; ++r) t[r] = Ib[3 * r] * u[0] + Ib[3 * r + 1] * u[1] + Ib[3 * r + 2] * u[2];
-        for (int r = 0; r < 3; ++r) Lw[r] = R[3 * r] * t[0] + R[3 * r + 1] * t[1] + R[3 * r + 2] * t[2];
-        Lm[b] = m > 0.0f ? make_float4((float)Lw[0], (float)Lw[1], (float)Lw[2], 0.0f)
-                         : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    for (int k = 0; k < ns; ++k)
-        loc[k] = make_float4(local4[4 * (size_t)k], local4[4 * (size_t)k + 1], local4[4 * (size_t)k + 2],
-                             local4[4 * (size_t)k + 3]);
     hipStream_t s = c->stream;
-    SD_HIP(hipMemcpyAsync(c->pos[0], P.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(c->vel[0], V.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(c->quat[0], Q.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(c->angm[0], Lm.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(c->i0, I0.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(c->i1, I1.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(c->bfirst, first, nb * sizeof(int), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(c->bcount, count, nb * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->pos[0], r.P.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->vel[0], r.V.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->quat[0], r.Q.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->angm[0], r.L.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->i0, r.I0.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->i1, r.I1.data(), nb * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->bfirst, r.first.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->bcount, r.count.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
     SD_HIP(hipMemcpyAsync(c->sbody, sbody.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
-    SD_HIP(hipMemcpyAsync(c->slocal, loc.data(), ns * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->slocal, r.loc.data(), ns * sizeof(float4), hipMemcpyHostToDevice, s));
     SD_HIP(hipStreamSynchronize(s));
     c->cur = 0;
-    c->steps_done = 0;
     c->have_tree = false;
     c->npairs = -1;
-    drop_clump_graphs(c);
+    c->static_hash = static_hash(r);
+    return SD_OK;
+}
+
+// The context's rows (the current buffer parity) on the host.
+sd_status read_clump_rows(sd_clump_ctx* c, ClumpRows& r) {
+    r.resize(c->nb, c->ns);
+    const size_t b4 = (size_t)c->nb * sizeof(float4);
+    hipStream_t s = c->stream;
+    const int k = c->cur;
+    SD_HIP(hipMemcpyAsync(r.P.data(), c->pos[k], b4, hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.V.data(), c->vel[k], b4, hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.Q.data(), c->quat[k], b4, hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.L.data(), c->angm[k], b4, hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.I0.data(), c->i0, b4, hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.I1.data(), c->i1, b4, hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.first.data(), c->bfirst, (size_t)c->nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.count.data(), c->bcount, (size_t)c->nb * sizeof(int), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.loc.data(), c->slocal, (size_t)c->ns * sizeof(float4), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipStreamSynchronize(s));
+    return SD_OK;
+}
+
+}  // namespace
+
+extern "C" sd_status sd_clump_upload(sd_clump_ctx* c, int nb, const float* pos4, const float* vel4,
+                                     const float* quat4, const float* angvel4, const float* inertia6,
+                                     const int* first, const int* count, int ns, const float* local4) {
+    SD_REQUIRE(c && nb >= 1 && ns >= 1, "sd_clump_upload: bad sizes");
+    SD_REQUIRE(pos4 && vel4 && quat4 && angvel4 && inertia6 && first && count && local4, "sd_clump_upload: NULL array");
+    SD_REQUIRE((uint64_t)nb * (uint64_t)nb < (1ull << 62), "sd_clump_upload: too many bodies");
+    std::vector<int> sbody;
+    sd_status st = body_ranges("sd_clump_upload", nb, first, count, ns, sbody);
+    if (st != SD_OK) return st;
+    ClumpRows r;
+    r.resize(nb, ns);
+    st = prepare_bodies("sd_clump_upload", nb, pos4, vel4, quat4, angvel4, inertia6, r, 0);
+    if (st != SD_OK) return st;
+    for (int b = 0; b < nb; ++b) {
+        r.first[b] = first[b];
+        r.count[b] = count[b];
+    }
+    for (int k = 0; k < ns; ++k)
+        r.loc[k] = make_float4(local4[4 * (size_t)k], local4[4 * (size_t)k + 1], local4[4 * (size_t)k + 2],
+                               local4[4 * (size_t)k + 3]);
+    st = write_clump_rows(c, r);
+    if (st != SD_OK) return st;
+    c->steps_done = 0;
     return SD_OK;
 }
 
@@ -801,4 +899,153 @@ extern "C" sd_status sd_clump_spheres(sd_clump_ctx* c, float* pos4) {
     SD_HIP(hipMemcpyAsync(pos4, c->sx, (size_t)c->ns * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     SD_HIP(hipStreamSynchronize(c->stream));
     return SD_OK;
+}
+
+// ------------------------------------------------------------------ checkpoint, add / remove
+
+namespace {
+struct ClumpStateHeader {
+    uint32_t magic;
+    uint32_t version;
+    int32_t nb;
+    int32_t ns;
+    int64_t steps_done;
+    uint64_t static_hash;
+};
+constexpr uint32_t CLUMP_MAGIC = 0x5344434Bu;  // "SDCK"
+
+size_t clump_state_bytes(const sd_clump_ctx* c) {
+    return sizeof(ClumpStateHeader) + 4 * (size_t)c->nb * sizeof(float4);
+}
+}  // namespace
+
+extern "C" sd_status sd_clump_state_bytes(sd_clump_ctx* c, size_t* bytes) {
+    SD_REQUIRE(c && bytes, "sd_clump_state_bytes: NULL");
+    if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_state_bytes: no bodies uploaded");
+    *bytes = clump_state_bytes(c);
+    return SD_OK;
+}
+
+extern "C" sd_status sd_clump_save(sd_clump_ctx* c, void* buffer, size_t bytes) {
+    SD_REQUIRE(c && buffer, "sd_clump_save: NULL");
+    if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_save: no bodies uploaded");
+    SD_REQUIRE(bytes >= clump_state_bytes(c), "sd_clump_save: buffer too small (%zu < %zu)", bytes,
+               clump_state_bytes(c));
+    ClumpStateHeader h{CLUMP_MAGIC, 1u, c->nb, c->ns, c->steps_done, c->static_hash};
+    char* b = (char*)buffer;
+    memcpy(b, &h, sizeof h);
+    b += sizeof h;
+    const size_t b4 = (size_t)c->nb * sizeof(float4);
+    float4* src[4] = {c->pos[c->cur], c->vel[c->cur], c->quat[c->cur], c->angm[c->cur]};
+    for (int k = 0; k < 4; ++k) {
+        SD_HIP(hipMemcpyAsync(b, src[k], b4, hipMemcpyDeviceToHost, c->stream));
+        b += b4;
+    }
+    SD_HIP(hipStreamSynchronize(c->stream));
+    return SD_OK;
+}
+
+extern "C" sd_status sd_clump_load(sd_clump_ctx* c, const void* buffer, size_t bytes) {
+    SD_REQUIRE(c && buffer, "sd_clump_load: NULL");
+    if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_load: upload the same bodies first");
+    SD_REQUIRE(bytes >= sizeof(ClumpStateHeader), "sd_clump_load: buffer too small");
+    ClumpStateHeader h;
+    memcpy(&h, buffer, sizeof h);
+    SD_REQUIRE(h.magic == CLUMP_MAGIC && h.version == 1u, "sd_clump_load: not a clump state buffer");
+    SD_REQUIRE(h.nb == c->nb && h.ns == c->ns, "sd_clump_load: state was saved from %d bodies / %d spheres, "
+               "the context holds %d / %d", h.nb, h.ns, c->nb, c->ns);
+    SD_REQUIRE(h.static_hash == c->static_hash,
+               "sd_clump_load: state was saved from other bodies (inertia, sphere ranges or offsets differ)");
+    SD_REQUIRE(bytes >= clump_state_bytes(c), "sd_clump_load: buffer too small");
+    (void)hipStreamSynchronize(c->stream);
+    const char* b = (const char*)buffer + sizeof h;
+    const size_t b4 = (size_t)c->nb * sizeof(float4);
+    c->cur = 0;
+    float4* dst[4] = {c->pos[0], c->vel[0], c->quat[0], c->angm[0]};
+    for (int k = 0; k < 4; ++k) {
+        SD_HIP(hipMemcpyAsync(dst[k], b, b4, hipMemcpyHostToDevice, c->stream));
+        b += b4;
+    }
+    SD_HIP(hipStreamSynchronize(c->stream));
+    c->steps_done = h.steps_done;
+    c->have_tree = false;
+    c->npairs = -1;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_clump_remove_bodies(sd_clump_ctx* c, const int* ids, int count) {
+    SD_REQUIRE(c, "sd_clump_remove_bodies: NULL context");
+    if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_remove_bodies: no bodies uploaded");
+    SD_REQUIRE(count >= 0 && (count == 0 || ids), "sd_clump_remove_bodies: bad id list");
+    if (count == 0) return SD_OK;
+    std::vector<char> gone(c->nb, 0);
+    for (int k = 0; k < count; ++k) {
+        SD_REQUIRE(ids[k] >= 0 && ids[k] < c->nb, "sd_clump_remove_bodies: id %d out of range [0, %d)", ids[k],
+                   c->nb);
+        SD_REQUIRE(!gone[ids[k]], "sd_clump_remove_bodies: id %d listed twice", ids[k]);
+        gone[ids[k]] = 1;
+    }
+    SD_REQUIRE(count < c->nb, "sd_clump_remove_bodies: removing every body (destroy the context instead)");
+    (void)hipStreamSynchronize(c->stream);
+    ClumpRows r, o;
+    sd_status st = read_clump_rows(c, r);
+    if (st != SD_OK) return st;
+    int ns = 0;
+    for (int b = 0; b < r.nb; ++b)
+        if (!gone[b]) ns += r.count[b];
+    o.resize(r.nb - count, ns);
+    int j = 0, sp = 0;
+    for (int b = 0; b < r.nb; ++b) {
+        if (gone[b]) continue;
+        o.P[j] = r.P[b];
+        o.V[j] = r.V[b];
+        o.Q[j] = r.Q[b];
+        o.L[j] = r.L[b];
+        o.I0[j] = r.I0[b];
+        o.I1[j] = r.I1[b];
+        o.first[j] = sp;
+        o.count[j] = r.count[b];
+        for (int k = 0; k < r.count[b]; ++k) o.loc[sp + k] = r.loc[r.first[b] + k];
+        sp += r.count[b];
+        ++j;
+    }
+    const long long steps = c->steps_done;
+    st = write_clump_rows(c, o);
+    c->steps_done = steps;
+    return st;
+}
+
+extern "C" sd_status sd_clump_add_bodies(sd_clump_ctx* c, int nb, const float* pos4, const float* vel4,
+                                         const float* quat4, const float* angvel4, const float* inertia6,
+                                         const int* first, const int* count, int ns, const float* local4) {
+    SD_REQUIRE(c, "sd_clump_add_bodies: NULL context");
+    if (c->nb == 0) return set_error(SD_ERR_STATE, "sd_clump_add_bodies: no bodies uploaded (use sd_clump_upload)");
+    SD_REQUIRE(nb >= 0 && ns >= 0, "sd_clump_add_bodies: bad sizes");
+    if (nb == 0) return SD_OK;
+    SD_REQUIRE(pos4 && vel4 && quat4 && angvel4 && inertia6 && first && count && local4,
+               "sd_clump_add_bodies: NULL array");
+    SD_REQUIRE((uint64_t)(c->nb + nb) * (uint64_t)(c->nb + nb) < (1ull << 62), "sd_clump_add_bodies: too many bodies");
+    std::vector<int> sbody;
+    sd_status st = body_ranges("sd_clump_add_bodies", nb, first, count, ns, sbody);
+    if (st != SD_OK) return st;
+    (void)hipStreamSynchronize(c->stream);
+    ClumpRows r;
+    st = read_clump_rows(c, r);
+    if (st != SD_OK) return st;
+    const int nb0 = r.nb, ns0 = r.ns;
+    ClumpRows o = r;
+    o.resize(nb0 + nb, ns0 + ns);
+    st = prepare_bodies("sd_clump_add_bodies", nb, pos4, vel4, quat4, angvel4, inertia6, o, nb0);
+    if (st != SD_OK) return st;
+    for (int b = 0; b < nb; ++b) {
+        o.first[nb0 + b] = ns0 + first[b];
+        o.count[nb0 + b] = count[b];
+    }
+    for (int k = 0; k < ns; ++k)
+        o.loc[ns0 + k] = make_float4(local4[4 * (size_t)k], local4[4 * (size_t)k + 1], local4[4 * (size_t)k + 2],
+                                     local4[4 * (size_t)k + 3]);
+    const long long steps = c->steps_done;
+    st = write_clump_rows(c, o);
+    c->steps_done = steps;
+    return st;
 }

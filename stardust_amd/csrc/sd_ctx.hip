@@ -986,8 +986,31 @@ struct StateHeader {
     int32_t key64;
     int32_t n_cells;
     int64_t steps_done;
+    int32_t nscenes;  // 0 for a single-scene context
+    int32_t reserved;
+    uint64_t layout_hash;  // scene table (batch) or grid (single scene)
 };
 constexpr uint32_t STATE_MAGIC = 0x53445354u;  // "SDST"
+constexpr uint32_t STATE_VERSION = 2u;
+
+// FNV-1a 64 over the bytes of the scene table of a batch context (grids,
+// contact laws, cell-id bases, slot ranges: 4-byte fields, no padding) or of
+// the grid of a single-scene context, so that a state saved from another
+// batch with the same totals is rejected.
+uint64_t layout_hash(const sd_ctx* c) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&](const void* p, size_t sz) {
+        const unsigned char* b = (const unsigned char*)p;
+        for (size_t i = 0; i < sz; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+    };
+    if (c->nscenes > 0) {
+        mix(c->scenes_h.data(), c->scenes_h.size() * sizeof(SceneDev));
+    } else {
+        const GridDev g = to_dev(c->p.grid);
+        mix(&g, sizeof g);
+    }
+    return h;
+}
 
 size_t state_bytes(const sd_ctx* c) {
     const size_t n = (size_t)c->n;
@@ -996,6 +1019,178 @@ size_t state_bytes(const sd_ctx* c) {
            (size_t)c->p.contact.max_contacts * n * sizeof(float4);
 }
 }  // namespace
+
+// ------------------------------------------------------------------ runtime add / remove
+
+namespace {
+
+// Host copy of the current state rows in slot order; hist holds entry k of
+// slot i at k * n + i, as on the device.
+struct HostRows {
+    int n = 0;
+    std::vector<float4> pos, vel, omg, hist;
+    std::vector<int> ids, hcount;
+};
+
+sd_status read_rows(sd_ctx* c, HostRows& r) {
+    const size_t n = (size_t)c->n, maxc = (size_t)c->p.contact.max_contacts;
+    r.n = c->n;
+    r.pos.resize(n);
+    r.vel.resize(n);
+    r.omg.resize(n);
+    r.ids.resize(n);
+    r.hcount.resize(n);
+    r.hist.resize(maxc * n);
+    const int cs = c->cur_state, ch = c->cur_hist;
+    hipStream_t s = c->stream;
+    SD_HIP(hipMemcpyAsync(r.pos.data(), c->pos[cs], n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.vel.data(), c->vel[cs], n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.omg.data(), c->omg[cs], n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.ids.data(), c->ids[cs], n * sizeof(int), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.hcount.data(), c->hcount[ch], n * sizeof(int), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipMemcpyAsync(r.hist.data(), c->hist[ch], maxc * n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    SD_HIP(hipStreamSynchronize(s));
+    return SD_OK;
+}
+
+// Reallocates the context for r.n spheres and uploads the rows; keys are
+// hashed from the positions (index = slot) and the next sort is the full one,
+// as after an upload. The step count is kept.
+sd_status write_rows(sd_ctx* c, const HostRows& r) {
+    const int n = r.n;
+    const size_t maxc = (size_t)c->p.contact.max_contacts;
+    const long long steps = c->steps_done;
+    std::vector<float> x(n), y(n), z(n), rad(n), one(n, 1.0f), zero(n, 0.0f);
+    for (int i = 0; i < n; ++i) {
+        x[i] = r.pos[i].x;
+        y[i] = r.pos[i].y;
+        z[i] = r.pos[i].z;
+        rad[i] = r.pos[i].w;
+    }
+    release(c);
+    sd_status st = upload_state(c, n, x.data(), y.data(), z.data(), zero.data(), zero.data(), zero.data(),
+                                rad.data(), one.data());
+    if (st != SD_OK) {
+        release(c);
+        return st;
+    }
+    hipStream_t s = c->stream;
+    SD_HIP(hipMemcpyAsync(c->pos[0], r.pos.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->vel[0], r.vel.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->omg[0], r.omg.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->ids[0], r.ids.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->hcount[0], r.hcount.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->hist[0], r.hist.data(), maxc * n * sizeof(float4), hipMemcpyHostToDevice, s));
+    const GridDev g = to_dev(c->p.grid);
+    st = c->p.key64 ? launch_hash_keys_u64(c->pos[0], n, g, c->k64[0], s)
+                    : launch_hash_keys_u32(c->pos[0], n, g, c->k32[0], c->v32[0], s);
+    if (st != SD_OK) return st;
+    SD_HIP(hipStreamSynchronize(s));
+    c->steps_done = steps;
+    return SD_OK;
+}
+
+}  // namespace
+
+extern "C" sd_status sd_ctx_remove_spheres(sd_ctx* c, const int* ids, int count) {
+    SD_REQUIRE(c, "sd_ctx_remove_spheres: NULL context");
+    SD_REQUIRE(!c->batch, "sd_ctx_remove_spheres: not available on a batch context");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_remove_spheres: no spheres uploaded");
+    SD_REQUIRE(count >= 0 && (count == 0 || ids), "sd_ctx_remove_spheres: bad id list");
+    if (count == 0) return SD_OK;
+    const int n = c->n;
+    std::vector<int> newid(n, 0);  // 1 = removed, then the new id of every kept one
+    for (int k = 0; k < count; ++k) {
+        SD_REQUIRE(ids[k] >= 0 && ids[k] < n, "sd_ctx_remove_spheres: id %d out of range [0, %d)", ids[k], n);
+        SD_REQUIRE(newid[ids[k]] == 0, "sd_ctx_remove_spheres: id %d listed twice", ids[k]);
+        newid[ids[k]] = 1;
+    }
+    SD_REQUIRE(count < n, "sd_ctx_remove_spheres: removing every sphere (destroy the context instead)");
+    int next = 0;
+    for (int i = 0; i < n; ++i) newid[i] = newid[i] ? -1 : next++;
+    (void)hipStreamSynchronize(c->stream);
+    HostRows r, o;
+    sd_status st = read_rows(c, r);
+    if (st != SD_OK) return st;
+    const size_t maxc = (size_t)c->p.contact.max_contacts;
+    const int m = n - count;
+    o.n = m;
+    o.pos.resize(m);
+    o.vel.resize(m);
+    o.omg.resize(m);
+    o.ids.resize(m);
+    o.hcount.assign(m, 0);
+    o.hist.assign(maxc * m, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    int j = 0;
+    for (int i = 0; i < n; ++i) {
+        const int nid = newid[r.ids[i]];
+        if (nid < 0) continue;
+        o.pos[j] = r.pos[i];
+        o.vel[j] = r.vel[i];
+        o.omg[j] = r.omg[i];
+        o.ids[j] = nid;
+        int h = 0;
+        for (int k = 0; k < r.hcount[i] && k < (int)maxc; ++k) {
+            float4 e = r.hist[(size_t)k * n + i];
+            int pid;
+            memcpy(&pid, &e.w, sizeof pid);  // partner id stored as int bits; walls are -1..-6
+            if (pid >= 0) {                   // a sphere partner: renumbered, or dropped with it
+                if (pid >= n || newid[pid] < 0) continue;
+                memcpy(&e.w, &newid[pid], sizeof pid);
+            }
+            o.hist[(size_t)h * m + j] = e;
+            ++h;
+        }
+        o.hcount[j] = h;
+        ++j;
+    }
+    return write_rows(c, o);
+}
+
+extern "C" sd_status sd_ctx_add_spheres(sd_ctx* c, int count, const float* x, const float* y, const float* z,
+                                        const float* vx, const float* vy, const float* vz, const float* radius,
+                                        const float* mass) {
+    SD_REQUIRE(c, "sd_ctx_add_spheres: NULL context");
+    SD_REQUIRE(!c->batch, "sd_ctx_add_spheres: not available on a batch context");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_ctx_add_spheres: no spheres uploaded (use sd_ctx_upload_spheres)");
+    SD_REQUIRE(count >= 0, "sd_ctx_add_spheres: count < 0");
+    if (count == 0) return SD_OK;
+    SD_REQUIRE(x && y && z && vx && vy && vz && radius && mass, "sd_ctx_add_spheres: NULL array");
+    SD_REQUIRE((long long)c->n + count < (1ll << 31), "sd_ctx_add_spheres: more than 2^31 spheres");
+    sd_status st = check_spheres("sd_ctx_add_spheres", count, x, y, z, radius, mass, c->p.grid.cell);
+    if (st != SD_OK) return st;
+    (void)hipStreamSynchronize(c->stream);
+    HostRows r, o;
+    st = read_rows(c, r);
+    if (st != SD_OK) return st;
+    const int n = c->n, m = n + count;
+    const size_t maxc = (size_t)c->p.contact.max_contacts;
+    o.n = m;
+    o.pos = r.pos;
+    o.vel = r.vel;
+    o.omg = r.omg;
+    o.ids = r.ids;
+    o.hcount = r.hcount;
+    o.pos.resize(m);
+    o.vel.resize(m);
+    o.omg.resize(m);
+    o.ids.resize(m);
+    o.hcount.resize(m, 0);
+    o.hist.assign(maxc * m, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (size_t k = 0; k < maxc; ++k)
+        for (int i = 0; i < n; ++i) o.hist[k * m + i] = r.hist[k * n + i];
+    for (int k = 0; k < count; ++k) {
+        const float ms = mass[k];
+        const float inv_m = ms > 0.0f ? 1.0f / ms : 0.0f;
+        const float I = 0.4f * ms * radius[k] * radius[k];
+        o.pos[n + k] = make_float4(x[k], y[k], z[k], radius[k]);
+        o.vel[n + k] = make_float4(ms > 0.0f ? vx[k] : 0.0f, ms > 0.0f ? vy[k] : 0.0f, ms > 0.0f ? vz[k] : 0.0f,
+                                   inv_m);
+        o.omg[n + k] = make_float4(0.0f, 0.0f, 0.0f, ms > 0.0f ? 1.0f / I : 0.0f);
+        o.ids[n + k] = n + k;
+    }
+    return write_rows(c, o);
+}
 
 extern "C" sd_status sd_ctx_state_bytes(sd_ctx* c, size_t* bytes) {
     SD_REQUIRE(c && bytes, "sd_ctx_state_bytes: NULL");
@@ -1010,7 +1205,8 @@ extern "C" sd_status sd_ctx_save(sd_ctx* c, void* buffer, size_t bytes) {
     SD_REQUIRE(bytes >= state_bytes(c), "sd_ctx_save: buffer too small (%zu < %zu)", bytes, state_bytes(c));
     const size_t n = (size_t)c->n;
     char* b = (char*)buffer;
-    StateHeader h{STATE_MAGIC, 1u, c->n, c->p.contact.max_contacts, c->p.key64, c->n_cells, c->steps_done};
+    StateHeader h{STATE_MAGIC, STATE_VERSION, c->n, c->p.contact.max_contacts, c->p.key64, c->n_cells,
+                  c->steps_done,  c->nscenes,    0,    layout_hash(c)};
     memcpy(b, &h, sizeof h);
     b += sizeof h;
     auto d2h = [&](const void* src, size_t sz) -> sd_status {
@@ -1043,10 +1239,14 @@ extern "C" sd_status sd_ctx_load(sd_ctx* c, const void* buffer, size_t bytes) {
     SD_REQUIRE(bytes >= state_bytes(c), "sd_ctx_load: buffer too small");
     StateHeader h;
     memcpy(&h, buffer, sizeof h);
-    SD_REQUIRE(h.magic == STATE_MAGIC && h.version == 1u, "sd_ctx_load: not a state buffer");
+    SD_REQUIRE(h.magic == STATE_MAGIC, "sd_ctx_load: not a state buffer");
+    SD_REQUIRE(h.version == STATE_VERSION, "sd_ctx_load: state buffer version %u (this library reads %u)", h.version,
+               STATE_VERSION);
     SD_REQUIRE(h.n == c->n && h.maxc == c->p.contact.max_contacts && h.key64 == c->p.key64 &&
                    h.n_cells == c->n_cells,
                "sd_ctx_load: state was saved from a context with different sizes");
+    SD_REQUIRE(h.nscenes == c->nscenes && h.layout_hash == layout_hash(c),
+               "sd_ctx_load: state was saved from a context with a different scene table or grid");
     const size_t n = (size_t)c->n;
     const char* b = (const char*)buffer + sizeof h;
     c->cur_keys = 0;

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
@@ -61,6 +63,70 @@ void DEMEngine::setStaticMassThreshold(float threshold) {
     m_static_mass = threshold;
 }
 
+namespace {
+
+// Upload arrays of sd_clump_upload / sd_clump_add_bodies.
+struct ClumpPack {
+    std::vector<float> pos4, vel4, quat4, angvel4, inertia6, local4;
+    std::vector<int> first, count;
+};
+
+// Packs the bodies of a prepared handler: centre = entity position, spheres
+// relative to it with radius diameter / 2, inertia about the centre from the
+// member spheres (own 2/5 m r^2 plus the parallel-axis term, in double),
+// bodies at or above the static mass threshold uploaded with mass 0.
+void pack_clumps(const EntityHandler& h, const std::vector<int>& estart, const std::vector<int>& elen,
+                 const std::vector<float4>& rb_pos, const std::vector<float4>& rb_vel,
+                 const std::vector<float>& rb_mass, const std::vector<float4>& p_rel,
+                 const std::vector<float>& p_size, const std::vector<float>& p_mass, float static_mass,
+                 ClumpPack& u) {
+    const int nb = h.getNumberOfParticles(), ns = h.getNumberOfSpheres();
+    u.pos4.assign(4 * (size_t)nb, 0.0f);
+    u.vel4.assign(4 * (size_t)nb, 0.0f);
+    u.quat4.assign(4 * (size_t)nb, 0.0f);
+    u.angvel4.assign(4 * (size_t)nb, 0.0f);
+    u.inertia6.assign(6 * (size_t)nb, 0.0f);
+    u.first.assign(nb, 0);
+    u.count.assign(nb, 0);
+    u.local4.assign(4 * (size_t)ns, 0.0f);
+    for (int b = 0; b < nb; ++b) {
+        const float4 X = rb_pos[b], V = rb_vel[b];
+        const float m = rb_mass[b];
+        const bool fixed = m >= static_mass;
+        u.pos4[4 * b] = X.x;
+        u.pos4[4 * b + 1] = X.y;
+        u.pos4[4 * b + 2] = X.z;
+        u.pos4[4 * b + 3] = fixed ? 0.0f : m;
+        u.vel4[4 * b] = V.x;
+        u.vel4[4 * b + 1] = V.y;
+        u.vel4[4 * b + 2] = V.z;
+        u.quat4[4 * b] = 1.0f;
+        u.first[b] = estart[b];
+        u.count[b] = elen[b];
+        double I[6] = {0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < elen[b]; ++j) {
+            const int k = estart[b] + j;
+            const float4 d = p_rel[k];
+            const float r = 0.5f * p_size[k];
+            u.local4[4 * (size_t)k] = d.x;
+            u.local4[4 * (size_t)k + 1] = d.y;
+            u.local4[4 * (size_t)k + 2] = d.z;
+            u.local4[4 * (size_t)k + 3] = r;
+            const double mj = p_mass[k], dx = d.x, dy = d.y, dz = d.z;
+            const double own = 0.4 * mj * (double)r * (double)r;
+            I[0] += own + mj * (dy * dy + dz * dz);
+            I[1] += own + mj * (dx * dx + dz * dz);
+            I[2] += own + mj * (dx * dx + dy * dy);
+            I[3] += -mj * dx * dy;
+            I[4] += -mj * dx * dz;
+            I[5] += -mj * dy * dz;
+        }
+        for (int q = 0; q < 6; ++q) u.inertia6[6 * (size_t)b + q] = (float)I[q];
+    }
+}
+
+}  // namespace
+
 void DEMEngine::transfer() {
     EntityHandler& h = entity_handler;
     if (!h.prepared) throw std::runtime_error("DEMEngine::transfer: call prep() first");
@@ -74,6 +140,148 @@ void DEMEngine::transfer() {
     m_uploaded = true;
     m_dt = 0.0f;
     m_steps = 0;
+    bind_runtime();
+}
+
+void DEMEngine::bind_runtime() {
+    entity_handler.runtime_add = [this](const DEMParticle& e) { addRuntime(e); };
+    entity_handler.runtime_remove = [this](int idx) { removeRuntime(idx); };
+}
+
+void DEMEngine::addRuntime(const DEMParticle& e) {
+    if (!m_uploaded) throw std::runtime_error("DEMEngine::addRuntime: call transfer() first");
+    EntityHandler one;
+    one.addEntity(e);
+    one.prepareArrays();
+    if (m_sphere_mode) {
+        if (one.getNumberOfSpheres() != 1)
+            throw std::runtime_error("DEMEngine::addRuntime: this engine steps single spheres; the entity has " +
+                                     std::to_string(one.getNumberOfSpheres()));
+        const float4 X = one.rb_pos[0], V = one.rb_vel[0];
+        const float r = 0.5f * one.p_size[0], mass = one.rb_mass[0] >= m_static_mass ? 0.0f : one.rb_mass[0];
+        check(sd_ctx_add_spheres(m_sctx, 1, &X.x, &X.y, &X.z, &V.x, &V.y, &V.z, &r, &mass), "DEMEngine::addRuntime");
+    } else {
+        ClumpPack u;
+        pack_clumps(one, one.entity_start, one.entity_length, one.rb_pos, one.rb_vel, one.rb_mass, one.p_rel,
+                    one.p_size, one.p_mass, m_static_mass, u);
+        check(sd_clump_add_bodies(m_ctx, 1, u.pos4.data(), u.vel4.data(), u.quat4.data(), u.angvel4.data(),
+                                  u.inertia6.data(), u.first.data(), u.count.data(), one.getNumberOfSpheres(),
+                                  u.local4.data()),
+              "DEMEngine::addRuntime");
+    }
+    entity_handler.addEntity(e);
+    entity_handler.prepareArrays();
+    sync();
+}
+
+void DEMEngine::removeRuntime(int idx) {
+    if (!m_uploaded) throw std::runtime_error("DEMEngine::removeRuntime: call transfer() first");
+    if (idx < 0 || idx >= entity_handler.getNumberOfParticles())
+        throw std::out_of_range("DEMEngine::removeRuntime: no entity " + std::to_string(idx));
+    if (m_sphere_mode) check(sd_ctx_remove_spheres(m_sctx, &idx, 1), "DEMEngine::removeRuntime");
+    else check(sd_clump_remove_bodies(m_ctx, &idx, 1), "DEMEngine::removeRuntime");
+    entity_handler.removeEntity(0, idx);
+    entity_handler.prepareArrays();
+    sync();
+}
+
+namespace {
+constexpr char ENGINE_MAGIC[8] = {'S', 'D', 'E', 'N', 'G', 'I', 'N', '1'};
+}
+
+void DEMEngine::saveState(const std::string& path) {
+    if (!m_uploaded) throw std::runtime_error("DEMEngine::saveState: call transfer() first");
+    size_t bytes = 0;
+    if (m_sphere_mode) check(sd_ctx_state_bytes(m_sctx, &bytes), "DEMEngine::saveState");
+    else check(sd_clump_state_bytes(m_ctx, &bytes), "DEMEngine::saveState");
+    std::vector<char> buf(bytes);
+    if (m_sphere_mode) check(sd_ctx_save(m_sctx, buf.data(), bytes), "DEMEngine::saveState");
+    else check(sd_clump_save(m_ctx, buf.data(), bytes), "DEMEngine::saveState");
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("DEMEngine::saveState: cannot open " + path);
+    const int32_t mode = m_sphere_mode ? 1 : 0;
+    const int64_t steps = m_steps, nbytes = (int64_t)bytes;
+    f.write(ENGINE_MAGIC, 8);
+    f.write((const char*)&mode, sizeof mode);
+    f.write((const char*)&steps, sizeof steps);
+    f.write((const char*)&nbytes, sizeof nbytes);
+    f.write(buf.data(), (std::streamsize)bytes);
+    if (!f) throw std::runtime_error("DEMEngine::saveState: write failed on " + path);
+}
+
+void DEMEngine::loadState(const std::string& path) {
+    if (!m_uploaded) throw std::runtime_error("DEMEngine::loadState: transfer() the same entities first");
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("DEMEngine::loadState: cannot open " + path);
+    char magic[8];
+    int32_t mode = -1;
+    int64_t steps = 0, nbytes = 0;
+    f.read(magic, 8);
+    f.read((char*)&mode, sizeof mode);
+    f.read((char*)&steps, sizeof steps);
+    f.read((char*)&nbytes, sizeof nbytes);
+    if (!f || std::memcmp(magic, ENGINE_MAGIC, 8) != 0 || nbytes < 0)
+        throw std::runtime_error("DEMEngine::loadState: " + path + " is not an engine checkpoint");
+    if (mode != (m_sphere_mode ? 1 : 0))
+        throw std::runtime_error("DEMEngine::loadState: the checkpoint is of the other step path");
+    std::vector<char> buf((size_t)nbytes);
+    f.read(buf.data(), (std::streamsize)nbytes);
+    if (!f) throw std::runtime_error("DEMEngine::loadState: " + path + " is truncated");
+    if (m_sphere_mode) check(sd_ctx_load(m_sctx, buf.data(), buf.size()), "DEMEngine::loadState");
+    else check(sd_clump_load(m_ctx, buf.data(), buf.size()), "DEMEngine::loadState");
+    m_steps = steps;
+    is_first_step = steps == 0;
+    sync();
+}
+
+namespace {
+
+// Legacy VTK binary data is big-endian.
+void put_be(std::ofstream& f, const void* p) {
+    const unsigned char* b = (const unsigned char*)p;
+    const unsigned char r[4] = {b[3], b[2], b[1], b[0]};
+    f.write((const char*)r, 4);
+}
+
+}  // namespace
+
+void DEMEngine::writeVTK(const std::string& path) {
+    if (!m_uploaded) throw std::runtime_error("DEMEngine::writeVTK: call transfer() first");
+    sync();
+    EntityHandler& h = entity_handler;
+    const int ns = h.getNumberOfSpheres();
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("DEMEngine::writeVTK: cannot open " + path);
+    f << "# vtk DataFile Version 3.0\nstardust_amd spheres, step " << m_steps << "\nBINARY\nDATASET POLYDATA\n";
+    f << "POINTS " << ns << " float\n";
+    for (int k = 0; k < ns; ++k) {
+        const float4 p = h.p_pos[k];
+        put_be(f, &p.x);
+        put_be(f, &p.y);
+        put_be(f, &p.z);
+    }
+    f << "\nVERTICES " << ns << " " << 2 * (long long)ns << "\n";
+    for (int k = 0; k < ns; ++k) {
+        const int one = 1;
+        put_be(f, &one);
+        put_be(f, &k);
+    }
+    f << "\nPOINT_DATA " << ns << "\nSCALARS radius float 1\nLOOKUP_TABLE default\n";
+    for (int k = 0; k < ns; ++k) {
+        const float r = 0.5f * h.p_size[k];
+        put_be(f, &r);
+    }
+    f << "\nVECTORS body_velocity float\n";
+    for (int k = 0; k < ns; ++k) {
+        const float4 v = h.rb_vel[h.particle_to_rigid[k]];
+        put_be(f, &v.x);
+        put_be(f, &v.y);
+        put_be(f, &v.z);
+    }
+    f << "\nSCALARS body int 1\nLOOKUP_TABLE default\n";
+    for (int k = 0; k < ns; ++k) put_be(f, &h.particle_to_rigid[k]);
+    f << "\n";
+    if (!f) throw std::runtime_error("DEMEngine::writeVTK: write failed on " + path);
 }
 
 namespace {
@@ -173,54 +381,13 @@ void DEMEngine::transfer_spheres() {
           "DEMEngine::transfer");
 }
 
-// Packs bodies for sd_clump_upload: centre = entity position, spheres relative
-// to it with radius diameter / 2, inertia about the centre from the member
-// spheres (own 2/5 m r^2 plus the parallel-axis term, in double).
+
 void DEMEngine::transfer_clumps() {
     EntityHandler& h = entity_handler;
     const int nb = h.getNumberOfParticles(), ns = h.getNumberOfSpheres();
-    up_pos4.assign(4 * (size_t)nb, 0.0f);
-    up_vel4.assign(4 * (size_t)nb, 0.0f);
-    up_quat4.assign(4 * (size_t)nb, 0.0f);
-    up_angvel4.assign(4 * (size_t)nb, 0.0f);
-    up_inertia6.assign(6 * (size_t)nb, 0.0f);
-    up_first.assign(nb, 0);
-    up_count.assign(nb, 0);
-    up_local4.assign(4 * (size_t)ns, 0.0f);
-    for (int b = 0; b < nb; ++b) {
-        const float4 X = h.rb_pos[b], V = h.rb_vel[b];
-        const float m = h.rb_mass[b];
-        const bool fixed = m >= m_static_mass;
-        up_pos4[4 * b] = X.x;
-        up_pos4[4 * b + 1] = X.y;
-        up_pos4[4 * b + 2] = X.z;
-        up_pos4[4 * b + 3] = fixed ? 0.0f : m;
-        up_vel4[4 * b] = V.x;
-        up_vel4[4 * b + 1] = V.y;
-        up_vel4[4 * b + 2] = V.z;
-        up_quat4[4 * b] = 1.0f;
-        up_first[b] = h.entity_start[b];
-        up_count[b] = h.entity_length[b];
-        double I[6] = {0, 0, 0, 0, 0, 0};
-        for (int j = 0; j < h.entity_length[b]; ++j) {
-            const int k = h.entity_start[b] + j;
-            const float4 d = h.p_rel[k];
-            const float r = 0.5f * h.p_size[k];
-            up_local4[4 * (size_t)k] = d.x;
-            up_local4[4 * (size_t)k + 1] = d.y;
-            up_local4[4 * (size_t)k + 2] = d.z;
-            up_local4[4 * (size_t)k + 3] = r;
-            const double mj = h.p_mass[k], dx = d.x, dy = d.y, dz = d.z;
-            const double own = 0.4 * mj * (double)r * (double)r;
-            I[0] += own + mj * (dy * dy + dz * dz);
-            I[1] += own + mj * (dx * dx + dz * dz);
-            I[2] += own + mj * (dx * dx + dy * dy);
-            I[3] += -mj * dx * dy;
-            I[4] += -mj * dx * dz;
-            I[5] += -mj * dy * dz;
-        }
-        for (int q = 0; q < 6; ++q) up_inertia6[6 * (size_t)b + q] = (float)I[q];
-    }
+    ClumpPack u;
+    pack_clumps(h, h.entity_start, h.entity_length, h.rb_pos, h.rb_vel, h.rb_mass, h.p_rel, h.p_size, h.p_mass,
+                m_static_mass, u);
     if (!m_ctx) {
         sd_clump_params p{};
         p.contact.kn = m_kn;
@@ -232,8 +399,8 @@ void DEMEngine::transfer_clumps() {
         p.contact.max_contacts = 16;
         check(sd_clump_create(&p, &m_ctx), "DEMEngine::transfer");
     }
-    check(sd_clump_upload(m_ctx, nb, up_pos4.data(), up_vel4.data(), up_quat4.data(), up_angvel4.data(),
-                          up_inertia6.data(), up_first.data(), up_count.data(), ns, up_local4.data()),
+    check(sd_clump_upload(m_ctx, nb, u.pos4.data(), u.vel4.data(), u.quat4.data(), u.angvel4.data(),
+                          u.inertia6.data(), u.first.data(), u.count.data(), ns, u.local4.data()),
           "DEMEngine::transfer");
 }
 

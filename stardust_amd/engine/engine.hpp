@@ -101,6 +101,25 @@ public:
     const SphereGrid& sphereGrid() const { return m_sgrid; }
     long long stepsDone() const { return m_steps; }
 
+    // Runtime add / remove (after transfer): the other entities keep their
+    // state, contact history included (EntityHandler::addEntityRuntime and
+    // removeEntityRuntime forward here). On the sphere path an added entity
+    // must be a single sphere; indices above a removed entity shift down.
+    void addRuntime(const DEMParticle& particle);
+    void removeRuntime(int idx);
+
+    // Checkpoint of the device state (step path, step count and the context's
+    // state buffer) to a file; loadState restores it into an engine that has
+    // transferred the same entities. Stepping on from a loaded state is
+    // bit-identical to stepping on from the saved one.
+    void saveState(const std::string& path);
+    void loadState(const std::string& path);
+
+    // State dump in place of the reference's GL interop (engine.cpp:36-62):
+    // legacy VTK (binary) polydata, one point per sphere with its radius, its
+    // body's velocity and its body index.
+    void writeVTK(const std::string& path);
+
     // Contact law and options (defaults: kn = 2e6 N/m as the reference's
     // constant, damping ratio 0.3, mu = 0.5, kt = 2/7 kn, gravity 9.81 along -z).
     void setContactLaw(float kn, float damping_ratio, float mu, float kt_ratio);
@@ -113,6 +132,7 @@ private:
     void ensure_context(float dt);
     void transfer_spheres();
     void transfer_clumps();
+    void bind_runtime();
 
     engineConfig engineConfiguration;
     Scalar m_domain;

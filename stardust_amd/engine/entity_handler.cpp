@@ -2,6 +2,7 @@
 #include "entity_handler.hpp"
 
 #include <stdexcept>
+#include <string>
 
 namespace STARDUST {
 
@@ -46,8 +47,25 @@ void EntityHandler::addEntity(const DEMMesh&) {
     throw std::runtime_error("EntityHandler::addEntity: triangle meshes are not supported");
 }
 
-void EntityHandler::removeEntity(int, int) {
-    throw std::runtime_error("EntityHandler::removeEntity: entities cannot be removed");
+void EntityHandler::removeEntity(int type, int idx) {
+    if (type != 0) throw std::invalid_argument("EntityHandler::removeEntity: only particles (type 0) are held");
+    if (idx < 0 || idx >= (int)particles.size())
+        throw std::out_of_range("EntityHandler::removeEntity: no entity " + std::to_string(idx));
+    n_spheres -= (int)particles[idx].getParticles().size();
+    particles.erase(particles.begin() + idx);
+    prepared = false;
+}
+
+void EntityHandler::addEntityRuntime(const DEMParticle& particle) {
+    if (!runtime_add) throw std::runtime_error("EntityHandler::addEntityRuntime: not transferred to a DEMEngine yet");
+    runtime_add(particle);
+}
+
+void EntityHandler::removeEntityRuntime(int type, int idx) {
+    if (type != 0) throw std::invalid_argument("EntityHandler::removeEntityRuntime: only particles (type 0) are held");
+    if (!runtime_remove)
+        throw std::runtime_error("EntityHandler::removeEntityRuntime: not transferred to a DEMEngine yet");
+    runtime_remove(idx);
 }
 
 void EntityHandler::bind() {

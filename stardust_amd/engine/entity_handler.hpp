@@ -11,6 +11,7 @@
 #ifndef STARDUST_AMD_ENGINE_ENTITY_HANDLER_HPP
 #define STARDUST_AMD_ENGINE_ENTITY_HANDLER_HPP
 
+#include <functional>
 #include <vector>
 
 #include "entity.hpp"
@@ -30,7 +31,16 @@ public:
 
     void addEntity(const DEMParticle& particle);
     void addEntity(const DEMMesh& mesh);  // meshes are not supported: throws
-    void removeEntity(int type, int idx);  // not supported after prep (throws), as in the reference's stub
+    // Before transfer(): drops entity idx of the given type (0 = particle) from
+    // the list (call prepareArrays again). The reference's is a stub
+    // (entity_handler.cpp:118).
+    void removeEntity(int type, int idx);
+    // After transfer(): add or remove an entity in the running simulation
+    // (forwarded to the DEMEngine that owns this handler; the reference's are
+    // stubs, entity_handler.cpp:108-119). The other entities keep their state;
+    // entity indices above a removed one shift down by one.
+    void addEntityRuntime(const DEMParticle& particle);
+    void removeEntityRuntime(int type, int idx);
 
     void prepareArrays();
     const std::vector<DEMParticle>& entities() const { return particles; }
@@ -61,6 +71,10 @@ public:
 
 private:
     void bind();
+
+    // set by the owning DEMEngine once transferred
+    std::function<void(const DEMParticle&)> runtime_add;
+    std::function<void(int)> runtime_remove;
 
     std::vector<DEMParticle> particles;
     int n_spheres = 0;

@@ -18,6 +18,21 @@
 //                                           domain <domain>; steps on the GPU and
 //                                           prints the packed input, the step path
 //                                           with its grid, and the final state
+//   sd_dem_driver edit-spheres <same arguments as run-spheres> <every>
+//                                           as run-spheres, with runtime edits:
+//                                           <steps> steps, removeEntityRuntime of
+//                                           every <every>-th entity (highest index
+//                                           first), <steps> steps, addEntityRuntime
+//                                           of 3 spheres, <steps> steps
+//   sd_dem_driver edit-clumps <steps> <dt>  the default "run" bodies plus a fourth,
+//                                           with body 2 removed and a fifth body
+//                                           added at runtime (steps between)
+//   sd_dem_driver checkpoint <steps> <dt> <file> [vtk]
+//                                           the default "run" bodies: <steps> steps,
+//                                           saveState, <steps> steps (final A); a
+//                                           second engine with the same entities,
+//                                           loadState, <steps> steps (final B);
+//                                           optionally writeVTK of the first
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -68,6 +83,16 @@ void print_pack(EntityHandler& h) {
     print_f4("particle_relative_position", h.h_particle_relative_position_ptr, ns);
 }
 
+// two 2x2x2 lattice bodies approaching along x, slightly offset, and a static sphere
+void two_bodies(DEMEngine& engine) {
+    engine.add(DEMParticle(0, 2, 0.04f, 0.02f, 0.5f, make_float4(0.0f, 0.0f, 0.0f, 0.0f),
+                           make_float4(1.0f, 0.0f, 0.0f, 0.0f)));
+    engine.add(DEMParticle(1, 2, 0.04f, 0.02f, 0.5f, make_float4(0.032f, 0.003f, 0.002f, 0.0f),
+                           make_float4(-1.0f, 0.0f, 0.0f, 0.0f)));
+    engine.add(DEMParticle(2, 1, 0.02f, 0.02f, 50.0f, make_float4(0.016f, -0.02f, 0.0f, 0.0f),
+                           make_float4(0.0f, 0.0f, 0.0f, 0.0f)));  // mass >= 10: static
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -109,13 +134,7 @@ int main(int argc, char** argv) {
             if (argc >= 5) {
                 engine.loadJSONSetup(argv[4]);
             } else {
-                // two 2x2x2 lattice bodies approaching along x, slightly offset
-                engine.add(DEMParticle(0, 2, 0.04f, 0.02f, 0.5f, make_float4(0.0f, 0.0f, 0.0f, 0.0f),
-                                       make_float4(1.0f, 0.0f, 0.0f, 0.0f)));
-                engine.add(DEMParticle(1, 2, 0.04f, 0.02f, 0.5f, make_float4(0.032f, 0.003f, 0.002f, 0.0f),
-                                       make_float4(-1.0f, 0.0f, 0.0f, 0.0f)));
-                engine.add(DEMParticle(2, 1, 0.02f, 0.02f, 50.0f, make_float4(0.016f, -0.02f, 0.0f, 0.0f),
-                                       make_float4(0.0f, 0.0f, 0.0f, 0.0f)));  // mass >= 10: static
+                two_bodies(engine);
             }
             engine.prep();
             std::printf("{\n");
@@ -133,13 +152,90 @@ int main(int argc, char** argv) {
             std::printf("}\n");
             return 0;
         }
-        if (mode == "run-spheres" && argc >= 10) {
+        if (mode == "checkpoint" && argc >= 5) {
+            const int steps = std::atoi(argv[2]);
+            const float dt = (float)std::atof(argv[3]);
+            auto build = [&](DEMEngine& e) {
+                two_bodies(e);
+                e.prep();
+                e.transfer();
+            };
+            DEMEngine a(config, 1.0f), b(config, 1.0f);
+            build(a);
+            for (int i = 0; i < steps; ++i) a.update(dt);
+            a.saveState(argv[4]);
+            for (int i = 0; i < steps; ++i) a.update(dt);
+            a.sync();
+            build(b);
+            b.loadState(argv[4]);
+            const long long resumed_at = b.stepsDone();
+            for (int i = 0; i < steps; ++i) b.update(dt);
+            b.sync();
+            if (argc >= 6) a.writeVTK(argv[5]);
+            std::printf("{\n\"resumed_at\": %lld, \"steps_a\": %lld, \"steps_b\": %lld,\n", resumed_at,
+                        a.stepsDone(), b.stepsDone());
+            EntityHandler &ha = a.getEntityHandler(), &hb = b.getEntityHandler();
+            const int nb = ha.getNumberOfParticles(), ns = ha.getNumberOfSpheres();
+            print_f4("a_position", ha.h_rigid_body_position_ptr, nb);
+            print_f4("a_quaternion", ha.h_rigid_body_quaternion_ptr, nb);
+            print_f4("a_angular_momentum", ha.h_rigid_body_angular_momentum_ptr, nb);
+            print_f4("a_velocity", ha.h_rigid_body_velocity_ptr, nb);
+            print_f4("a_sphere_position", ha.h_particle_position_ptr, ns);
+            print_f("a_sphere_size", ha.h_particle_size_ptr, ns);
+            print_i("a_particle_to_rigid", ha.h_particle_to_rigid_idx_ptr, ns);
+            print_f4("b_position", hb.h_rigid_body_position_ptr, nb);
+            print_f4("b_quaternion", hb.h_rigid_body_quaternion_ptr, nb);
+            print_f4("b_angular_momentum", hb.h_rigid_body_angular_momentum_ptr, nb);
+            print_f4("b_velocity", hb.h_rigid_body_velocity_ptr, nb, true);
+            std::printf("}\n");
+            return 0;
+        }
+        if (mode == "edit-clumps" && argc >= 4) {
+            const int steps = std::atoi(argv[2]);
+            const float dt = (float)std::atof(argv[3]);
+            DEMEngine engine(config, 1.0f);
+            two_bodies(engine);
+            engine.add(DEMParticle(3, 2, 0.04f, 0.02f, 0.5f, make_float4(0.0f, 0.05f, 0.0f, 0.0f),
+                                   make_float4(0.0f, -0.5f, 0.0f, 0.0f)));
+            engine.prep();
+            std::printf("{\n");
+            print_pack(engine.getEntityHandler());
+            engine.transfer();
+            for (int i = 0; i < steps; ++i) engine.update(dt);
+            engine.getEntityHandler().removeEntityRuntime(0, 2);
+            for (int i = 0; i < steps; ++i) engine.update(dt);
+            const DEMParticle extra(4, 2, 0.04f, 0.02f, 0.5f, make_float4(0.03f, 0.045f, 0.002f, 0.0f),
+                                    make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            engine.getEntityHandler().addEntityRuntime(extra);
+            for (int i = 0; i < steps; ++i) engine.update(dt);
+            engine.sync();
+            {
+                DEMEngine one(config, 1.0f);
+                one.add(extra);
+                one.prep();
+                std::printf("\"added\": {\n");
+                print_pack(one.getEntityHandler());
+                std::printf("\"end\": 0},\n");
+            }
+            EntityHandler& h = engine.getEntityHandler();
+            const int nb = h.getNumberOfParticles();
+            std::printf("\"steps\": %lld, \"bodies_after\": %d,\n", engine.stepsDone(), nb);
+            print_f4("final_position", h.h_rigid_body_position_ptr, nb);
+            print_f4("final_velocity", h.h_rigid_body_velocity_ptr, nb);
+            print_f4("final_quaternion", h.h_rigid_body_quaternion_ptr, nb);
+            print_f4("final_angular_momentum", h.h_rigid_body_angular_momentum_ptr, nb, true);
+            std::printf("}\n");
+            return 0;
+        }
+        if ((mode == "run-spheres" && argc >= 10) || (mode == "edit-spheres" && argc >= 11)) {
             const int steps = std::atoi(argv[2]);
             const float dt = (float)std::atof(argv[3]);
             const int n = std::atoi(argv[4]);
             const float domain = (float)std::atof(argv[5]), size = (float)std::atof(argv[6]);
             const float lo = (float)std::atof(argv[7]), hi = (float)std::atof(argv[8]);
             unsigned long long state = std::strtoull(argv[9], nullptr, 10);
+            const int n0 = n;
+            (void)n0;
             auto uniform = [&state]() {  // SplitMix64, 24-bit uniform in [0, 1)
                 state += 0x9E3779B97F4A7C15ull;
                 unsigned long long z = state;
@@ -167,6 +263,32 @@ int main(int argc, char** argv) {
                         g.lo[0], g.lo[1], g.lo[2], g.cell, g.n[0], g.n[1], g.n[2], g.walls ? "true" : "false",
                         g.wall_lo[0], g.wall_lo[1], g.wall_lo[2], g.wall_hi[0], g.wall_hi[1], g.wall_hi[2]);
             for (int i = 0; i < steps; ++i) engine.update(dt);
+            if (mode == "edit-spheres") {
+                const int every = std::atoi(argv[10]);
+                std::printf("\"removed\": [");
+                bool first = true;
+                for (int i = n - 1; i >= 0; --i) {
+                    if (i % every) continue;
+                    engine.getEntityHandler().removeEntityRuntime(0, i);
+                    std::printf("%s%d", first ? "" : ", ", i);
+                    first = false;
+                }
+                std::printf("],\n");
+                for (int i = 0; i < steps; ++i) engine.update(dt);
+                std::printf("\"added\": [");
+                for (int k = 0; k < 3; ++k) {
+                    const float4 pos = make_float4(lo + (hi - lo) * uniform(), lo + (hi - lo) * uniform(),
+                                                   lo + (hi - lo) * uniform(), 0.0f);
+                    const float4 vel = make_float4(uniform() - 0.5f, uniform() - 0.5f, uniform() - 0.5f, 0.0f);
+                    const float mass = 1.0f + 8.0f * uniform();
+                    engine.getEntityHandler().addEntityRuntime(DEMParticle((unsigned)(n + k), 1, size, size, mass,
+                                                                           pos, vel));
+                    std::printf("%s[%.9g, %.9g, %.9g, %.9g, %.9g, %.9g, %.9g]", k ? ", " : "", pos.x, pos.y, pos.z,
+                                vel.x, vel.y, vel.z, mass);
+                }
+                std::printf("],\n");
+                for (int i = 0; i < steps; ++i) engine.update(dt);
+            }
             engine.sync();
             EntityHandler& h = engine.getEntityHandler();
             const int nb = h.getNumberOfParticles();

@@ -95,6 +95,24 @@ class SphereScene:
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
         L.call("sd_ctx_load", self.h, L.ptr(buf), buf.nbytes)
 
+    def remove(self, ids):
+        """Drops the spheres with these particle ids; the others are renumbered in id order."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        L.call("sd_ctx_remove_spheres", self.h, L.ptr(ids), len(ids))
+        self.n -= len(ids)
+
+    def add(self, x, y, z, radius, mass, vx=None, vy=None, vz=None):
+        """Appends spheres with ids n, n + 1, ...; returns their ids."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        k = len(x)
+        zero = np.zeros(k, np.float32)
+        arrs = [f(x), f(y), f(z), f(vx if vx is not None else zero), f(vy if vy is not None else zero),
+                f(vz if vz is not None else zero), f(radius), f(mass)]
+        L.call("sd_ctx_add_spheres", self.h, k, *[L.ptr(a) for a in arrs])
+        first = self.n
+        self.n += k
+        return np.arange(first, first + k)
+
     def close(self):
         if self.h is not None and self.h.value:
             L.load().sd_ctx_destroy(self.h)

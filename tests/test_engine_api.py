@@ -180,3 +180,116 @@ def test_engine_single_spheres_unbounded_equals_sphere_oracle(oracle):
     print("[gpu-case] engine single spheres, unbounded (testDEM.cpp sizes)", flush=True)
     d, st = run_spheres_and_compare(oracle, 30, 5e-4, 100000, 0.005, 0.5, 1.0, 40.0, 5)
     assert d["grid"]["walls"] is False
+
+
+def sphere_oracle_from_pack(oracle, d, dt):
+    gr = d["grid"]
+    g = L.Grid.make(gr["lo"], np.float32(gr["cell"]), gr["n"])
+    walls = (gr["wall_lo"], gr["wall_hi"]) if gr["walls"] else None
+    c = L.ContactParams.make(2e6, 0.3, 0.5, np.float32(dt), gravity=(0.0, 0.0, -9.81), walls=walls,
+                             kt_ratio=2.0 / 7.0, max_contacts=24)
+    pos = np.array(d["rigid_body_position"], np.float32)
+    vel = np.array(d["rigid_body_velocity"], np.float32)
+    mass = np.array(d["rigid_body_mass"], np.float32)
+    r = (np.float32(0.5) * np.array(d["particle_size"], np.float32)).astype(np.float32)
+    m = np.where(mass >= 10.0, np.float32(0.0), mass).astype(np.float32)
+    st = oracle.SphereState(pos[:, 0], pos[:, 1], pos[:, 2], r, m, vx=vel[:, 0], vy=vel[:, 1], vz=vel[:, 2],
+                            max_contacts=24)
+    return g, c, st, r
+
+
+@pytest.mark.gpu
+def test_engine_runtime_remove_and_add_spheres(oracle):
+    """EntityHandler::removeEntityRuntime / addEntityRuntime on the sphere path:
+    the engine's state equals the oracle's, which removes the same entities and
+    appends the same spheres, bit for bit (contact history kept)."""
+    print("[gpu-case] engine runtime edit, spheres", flush=True)
+    steps, dt = 25, 1e-4
+    d = drive("edit-spheres", steps, dt, 3000, 1.0, 0.05, 0.05, 0.95, 17, 9)
+    g, c, st, r = sphere_oracle_from_pack(oracle, d, dt)
+    st.step(g, c, nsteps=steps, nthreads=8)
+    assert int(st.hcount.sum()) > 0
+    st.remove(np.array(d["removed"]))
+    st.step(g, c, nsteps=steps, nthreads=8)
+    add = np.array(d["added"], np.float32)
+    rr = np.full(len(add), np.float32(0.5) * np.float32(0.05), np.float32)
+    st.add(add[:, 0], add[:, 1], add[:, 2], rr, add[:, 6], vx=add[:, 3], vy=add[:, 4], vz=add[:, 5])
+    st.step(g, c, nsteps=steps, nthreads=8)
+    exp = st.by_id()
+    assert d["steps"] == 3 * steps
+    assert np.array_equal(np.array(d["final_position"], np.float32)[:, :3], exp["pos4"][:, :3])
+    assert np.array_equal(np.array(d["final_velocity"], np.float32)[:, :3], exp["vel4"][:, :3])
+
+
+@pytest.mark.gpu
+def test_engine_runtime_remove_and_add_clumps(oracle):
+    print("[gpu-case] engine runtime edit, clumps", flush=True)
+    steps, dt = 150, 1e-5
+    d = drive("edit-clumps", steps, dt)
+    st = oracle.ClumpState(clump_input_from_pack(d))
+    c = L.ContactParams.make(2e6, 0.3, 0.5, np.float32(dt), gravity=(0.0, 0.0, -9.81), walls=None,
+                             kt_ratio=2.0 / 7.0, max_contacts=16)
+    st.step(c, steps)
+    st.remove([2])
+    st.step(c, steps)
+    st.add(clump_input_from_pack(d["added"]))
+    st.step(c, steps)
+    assert d["bodies_after"] == 4
+    for key, arr, w in (("final_position", st.P, 3), ("final_velocity", st.V, 3), ("final_quaternion", st.Q, 4),
+                        ("final_angular_momentum", st.L, 3)):
+        got = np.array(d[key], np.float32)[:, :w]
+        assert np.array_equal(got, arr[:, :w]), (key, got, arr[:, :w])
+
+
+def read_vtk(path):
+    """Minimal reader of the legacy binary polydata DEMEngine::writeVTK writes."""
+    raw = open(path, "rb").read()
+    out, pos = {}, 0
+
+    def line():
+        nonlocal pos
+        e = raw.index(b"\n", pos)
+        t = raw[pos:e].decode()
+        pos = e + 1
+        return t
+
+    assert line().startswith("# vtk DataFile")
+    line()
+    assert line() == "BINARY" and line() == "DATASET POLYDATA"
+    _, n, _ = line().split()
+    n = int(n)
+    out["points"] = np.frombuffer(raw, ">f4", 3 * n, pos).reshape(n, 3)
+    pos += 12 * n + 1
+    assert line().startswith("VERTICES")
+    out["verts"] = np.frombuffer(raw, ">i4", 2 * n, pos).reshape(n, 2)
+    pos += 8 * n + 1
+    assert line() == "POINT_DATA %d" % n and line() == "SCALARS radius float 1" and line() == "LOOKUP_TABLE default"
+    out["radius"] = np.frombuffer(raw, ">f4", n, pos)
+    pos += 4 * n + 1
+    assert line() == "VECTORS body_velocity float"
+    out["velocity"] = np.frombuffer(raw, ">f4", 3 * n, pos).reshape(n, 3)
+    pos += 12 * n + 1
+    assert line() == "SCALARS body int 1" and line() == "LOOKUP_TABLE default"
+    out["body"] = np.frombuffer(raw, ">i4", n, pos)
+    return out
+
+
+@pytest.mark.gpu
+def test_engine_checkpoint_resume_and_vtk(tmp_path):
+    """saveState / loadState: a second engine resumed from the checkpoint ends
+    bit-identical to the first; writeVTK holds the spheres' positions, radii,
+    body velocities and owners."""
+    print("[gpu-case] engine checkpoint", flush=True)
+    ck, vtk = tmp_path / "state.bin", tmp_path / "state.vtk"
+    d = drive("checkpoint", 120, 1e-5, ck, vtk)
+    assert d["resumed_at"] == 120 and d["steps_a"] == d["steps_b"] == 240
+    for k in ("position", "quaternion", "angular_momentum", "velocity"):
+        assert np.array_equal(np.array(d["a_" + k], np.float32), np.array(d["b_" + k], np.float32)), k
+    v = read_vtk(vtk)
+    sp = np.array(d["a_sphere_position"], np.float32)
+    assert np.array_equal(v["points"], sp[:, :3])
+    assert np.array_equal(v["radius"], (np.float32(0.5) * np.array(d["a_sphere_size"], np.float32)))
+    owner = np.array(d["a_particle_to_rigid"])
+    assert np.array_equal(v["body"], owner)
+    assert np.array_equal(v["verts"][:, 1], np.arange(len(sp)))
+    assert np.array_equal(v["velocity"], np.array(d["a_velocity"], np.float32)[owner, :3])

@@ -283,3 +283,33 @@ def test_batch_tiny_scenes_equal_oracle(oracle):
             assert np.array_equal(got[f][sl], e[arr][:, col]), ("scene", k, f)
         off += n
     b.close()
+
+
+def test_batch_checkpoint_resume_and_scene_table_check():
+    """sd_ctx_save / sd_ctx_load on a batch context: a resumed batch steps
+    bit-identically to the uninterrupted one; a state saved from a batch with
+    the same totals but another scene table (one scene's stiffness changed) is
+    rejected."""
+    say("batch checkpoint")
+    w = small_batch(nscenes=5, seed=0x5EEDB004)
+    s, scenes = w.generate()
+    v = velocities(len(s["x"]), 4)
+    a = batch_of(w, scenes, s, v)
+    a.step(40)
+    buf = a.save()
+    a.step(30)
+    ref = a.download()
+    b = batch_of(w, scenes, s, v)
+    b.load(buf)
+    b.step(30)
+    assert_same(ref, b.download(), "resumed batch")
+    n0, g0, c0 = scenes[0]
+    c1 = type(c0).from_buffer_copy(c0)
+    c1.kn = 2.0 * c0.kn
+    other = [(n0, g0, c1)] + list(scenes[1:])
+    d = batch_of(w, other, s, v)
+    with pytest.raises(Exception, match="scene table"):
+        d.load(buf)
+    a.close()
+    b.close()
+    d.close()

@@ -324,3 +324,65 @@ def test_clump_steps_crowded_bit_exact(oracle):
         total += k
         compare_state(sc.download(), o, "step %d" % total)
     sc.close()
+
+
+def test_clump_checkpoint_resume_and_body_check(oracle):
+    """sd_clump_save / sd_clump_load: a context resumed from a checkpoint steps
+    bit-identically to the uninterrupted one; a checkpoint of other bodies
+    (same counts, one sphere offset changed) is rejected."""
+    say("clump checkpoint")
+    s = clump_set(1000, 0xC9, static_every=11)
+    c = CL.contact_c3()
+    a = CL.ClumpScene(c)
+    a.upload(s)
+    a.step(7)
+    buf = a.save()
+    a.step(9)
+    ref = a.download()
+    b = CL.ClumpScene(c)
+    b.upload(s)
+    b.load(buf)
+    b.step(9)
+    got = b.download()
+    for k in ("pos4", "vel4", "quat4", "angm4"):
+        assert np.array_equal(got[k], ref[k]), k
+    other = dict(s)
+    other["local4"] = s["local4"].copy()
+    other["local4"][3, 0] += 1e-4
+    d = CL.ClumpScene(c)
+    d.upload(other)
+    with pytest.raises(L.SdError, match="other bodies"):
+        d.load(buf)
+    for x in (a, b, d):
+        x.close()
+
+
+def test_clump_remove_then_add_matches_oracle(oracle):
+    """Runtime removal and addition of bodies (sd_clump_remove_bodies,
+    sd_clump_add_bodies) against the oracle's rule, bit for bit, in contact."""
+    say("clump remove / add")
+    s = clump_set(1000, 0xCA, static_every=13)
+    c = CL.contact_c3()
+    o = oracle.ClumpState(s)
+    sc = CL.ClumpScene(c)
+    sc.upload(s)
+    sc.step(5)
+    o.step(c, 5)
+    compare_state(sc.download(), o, "before")
+    rng = np.random.default_rng(5)
+    gone = rng.choice(1000, 77, replace=False)
+    sc.remove(gone)
+    o.remove(gone)
+    sc.step(6)
+    o.step(c, 6)
+    compare_state(sc.download(), o, "after remove")
+    extra = CL.generate_clumps(0xCB, 27, lattice=3, spacing=0.035, jitter=0.002, v_sigma=0.5)
+    extra["pos4"][:, :3] += np.float32(0.012)  # among the others: contacts with them
+    ids = sc.add(extra)
+    o.add(extra)
+    assert ids.tolist() == list(range(923, 950))
+    sc.step(7)
+    o.step(c, 7)
+    compare_state(sc.download(), o, "after add")
+    assert len(sc.pairs()) > 200
+    sc.close()

@@ -201,3 +201,33 @@ def test_integrate_bodies_free_spin_and_fall(oracle):
     assert abs(Q[0, 0] - math.cos(1.5 * t)) < 2e-5 and abs(Q[0, 3] - math.sin(1.5 * t)) < 2e-5
     assert abs(P[0, 2] - (1.0 - 9.81 * 1e-3 ** 2 * n * (n + 1) / 2)) < 2e-5
     assert np.array_equal(P[2], np.array([0, 0, 0, 0], np.float32)) and np.array_equal(Q[2], [1, 0, 0, 0])
+
+
+def test_clump_remove_and_add_far_bodies_change_nothing_else(oracle):
+    """Bodies 10 m away never touch the others: carrying them and removing
+    them at step k (or adding them at step k) leaves every other body
+    bit-identical to a run that never had them, and the far bodies themselves
+    move as when stepped alone."""
+    from stardust_amd import clumps as CL
+    c = CL.contact_c3()
+    near = CL.generate_clumps(0xD1, 125, lattice=5, spacing=0.035, jitter=0.002, v_sigma=0.5)
+    far = CL.generate_clumps(0xD2, 8, lattice=2, spacing=0.05, jitter=0.002, v_sigma=0.5)
+    far["pos4"][:, :3] += np.float32(10.0)
+    both = {k: np.concatenate([near[k], far[k] + (len(near["local4"]) if k == "first" else 0)])
+            for k in near}
+    a, b, f = oracle.ClumpState(both), oracle.ClumpState(near), oracle.ClumpState(far)
+    for st in (a, b, f):
+        st.step(c, 4)
+    a.remove(np.arange(125, 133))
+    for st in (a, b):
+        st.step(c, 5)
+    for k in ("P", "V", "Q", "L"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    a.add(far)
+    f2 = oracle.ClumpState(far)
+    for st in (a, b, f2):
+        st.step(c, 6)
+    for k in ("P", "V", "Q", "L"):
+        assert np.array_equal(getattr(a, k)[:125], getattr(b, k)), k
+        assert np.array_equal(getattr(a, k)[125:], getattr(f2, k)), k
+    assert a.nb == 133 and a.ns == len(both["local4"])
