@@ -411,6 +411,54 @@ sd_status sd_lbvh_query(const uint64_t* sorted_keys, int n, const int* left, con
  * history), forces and torques summed per body, quaternion integration.
  * Walls are not applied. Replaces DEMEngine::update for multi-sphere entities
  * (simulation_kernel.cu:52-153 with physics_update.cu:26-348). */
+/* ------------------------------------------------------- rigid convex bodies
+ * Convex hulls (vertices in the body frame), spheres (no vertices, a radius)
+ * and static triangles (three vertices, mass 0: the faces of a DEMMesh,
+ * src/engine/entities/entity.hpp:64-93) stepped on the GPU in independent
+ * scenes. Each step: batched sweep-and-prune over the body boxes (each scene on
+ * its own axis), cold GJK/EPA of every candidate pair (the sd_gjk results:
+ * TRIANGLE and SPHERE shapes of sandbox/primitives.hpp:11-14 included), and
+ * for each touching pair (sep < 0) a contact between the two contact
+ * features (a hull's feature: its vertices within 1e-3 of its radius of the
+ * extreme one along the normal, averaged; a sphere's surface point), placed at
+ * the centroid of the feature of smaller spread (the midpoint of the two on a
+ * tie: two points, two like faces), with
+ *   f_n = kn * depth - c_n * (v_rel . n),  c_n = 2 zeta sqrt(kn m_eff),
+ *   f_t = -min(mu * max(f_n, 0), c_n |v_t|) v_t / |v_t|   (no history),
+ * applied to both bodies (no float atomics: each body sums its contacts in
+ * pair order), then the semi-implicit integration of the clump path (1/m = 0:
+ * static). The step is captured in a hipGraph and replayed. Replaces the
+ * hull pipeline of the reference sandbox (testGJK.cu:237-315 on Hull pairs,
+ * testGridSAP.cu), which reports overlap only. */
+typedef struct sd_hull_params {
+    sd_contact_params contact; /* kn, damping_ratio, mu, dt, gravity (kt_ratio, walls, max_contacts unused) */
+    int max_iter;              /* GJK/EPA iteration limit; 0 = 64 */
+    int pairs_per_body;        /* candidate-pair capacity per body; 0 = 32 */
+    int use_graph;             /* capture the step in a hipGraph */
+} sd_hull_params;
+typedef struct sd_hull_ctx sd_hull_ctx;
+sd_status sd_hull_create(const sd_hull_params* p, sd_hull_ctx** ctx);
+sd_status sd_hull_destroy(sd_hull_ctx* ctx);
+/* n bodies: pos4 (x, y, z, mass; mass 0 = static), vel4, quat4 (w, x, y, z,
+ * unit), angvel4 (world), inertia6 (body frame about the centre: xx, yy, zz,
+ * xy, xz, yz), radius (sphere radius where nv == 0), vstart/nv into verts4
+ * (body-frame vertices, n_verts in all); scenes are consecutive body ranges
+ * (scene_first[0] = 0 ... scene_first[nscenes] = n; NULL for one scene), at
+ * most 65,536 bodies each, with no contacts across scenes. */
+sd_status sd_hull_upload(sd_hull_ctx* ctx, int n, const float* pos4, const float* vel4, const float* quat4,
+                         const float* angvel4, const float* inertia6, const float* radius, const int* vstart,
+                         const int* nv, int n_verts, const float* verts4, const int* scene_first, int nscenes);
+/* nsteps steps. SD_ERR_CAPACITY when a step had more candidate pairs than the
+ * capacity (its result is then invalid: upload again with a larger
+ * pairs_per_body). */
+sd_status sd_hull_step(sd_hull_ctx* ctx, int nsteps);
+/* Body state: pos4 (x, y, z, 0), vel4, quat4, angmom4 (world L); any may be NULL. */
+sd_status sd_hull_download(sd_hull_ctx* ctx, float* pos4, float* vel4, float* quat4, float* angmom4);
+/* Touching pairs of the last step in pair order: (a, b), the force on b
+ * (x, y, z, 1; -force on a) and the contact point (x, y, z, 0). */
+sd_status sd_hull_contacts(sd_hull_ctx* ctx, int* pairs, float* force4, float* point4, int capacity,
+                           int* n_contacts);
+
 typedef struct sd_clump_params {
     sd_contact_params contact;
 } sd_clump_params;

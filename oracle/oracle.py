@@ -57,6 +57,8 @@ _SIGS = {
     "or_compat_angular_velocity": ([_I, _P, _P, _P, _P], None),
     "or_body_reduce": ([_I, _P, _P, _P, _P, _P, _P, _P, _P], None),
     "or_integrate_bodies": ([_I, _P, _P, _P, _P, _P, _P, _P, C.POINTER(OrContact)], None),
+    "or_hull_step": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, C.POINTER(OrContact), _I, _I, _P, _P,
+                      _P, C.c_long], C.c_long),
     "or_clump_step": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, C.POINTER(OrContact)], C.c_long),
 }
 
@@ -130,6 +132,55 @@ class ClumpState:
                                            p(self.I1), p(self.first), p(self.count), p(sx), p(sv), len(pr), p(pr),
                                            C.byref(oc), p(f), p(t), 0)
         return f, t
+
+
+class HullBodyState:
+    """Rigid convex-body state as the GPU context prepares it (float4 rows);
+    step() is one sd_hull_step."""
+
+    def __init__(self, b, scene_first=None):
+        n = len(b["pos4"])
+        self.n = n
+        self.P, self.V, self.Q, self.L, self.I0, self.I1 = (np.zeros((n, 4), np.float32) for _ in range(6))
+        a = [np.ascontiguousarray(b[k], np.float32) for k in ("pos4", "vel4", "quat4", "angvel4", "inertia6")]
+        load().or_clump_prepare(n, *[p(x) for x in a], p(self.P), p(self.V), p(self.Q), p(self.L), p(self.I0),
+                                p(self.I1))
+        self.radius = np.ascontiguousarray(b["radius"], np.float32)
+        self.vstart = np.ascontiguousarray(b["vstart"], np.int32)
+        self.nv = np.ascontiguousarray(b["nv"], np.int32)
+        self.verts4 = np.ascontiguousarray(b["verts4"], np.float32)
+        sf = [0, n] if scene_first is None else scene_first
+        self.scene_first = np.ascontiguousarray(sf, np.int32)
+
+    def step(self, c, nsteps=1, max_iter=64, nthreads=1):
+        """Steps; returns the touching pairs of the last step (pairs, force on
+        b, contact point)."""
+        oc = contact(c)
+        out = None
+        for _ in range(nsteps):
+            m = load().or_hull_step(self.n, p(self.P), p(self.V), p(self.Q), p(self.L), p(self.I0), p(self.I1),
+                                    p(self.radius), p(self.vstart), p(self.nv), p(self.verts4), len(self.verts4),
+                                    p(self.scene_first), len(self.scene_first) - 1, C.byref(oc), max_iter, nthreads,
+                                    None, None, None, 0)
+            out = m
+        return out
+
+    def contacts(self, c, max_iter=64, nthreads=1):
+        """Touching pairs of one step from the current state, without
+        advancing it (the step runs on a copy)."""
+        oc = contact(c)
+        P, V, Q, L = (x.copy() for x in (self.P, self.V, self.Q, self.L))
+        m = load().or_hull_step(self.n, p(P), p(V), p(Q), p(L), p(self.I0), p(self.I1), p(self.radius),
+                                p(self.vstart), p(self.nv), p(self.verts4), len(self.verts4), p(self.scene_first),
+                                len(self.scene_first) - 1, C.byref(oc), max_iter, nthreads, None, None, None, 0)
+        pairs = np.zeros((max(m, 1), 2), np.int32)
+        f = np.zeros((max(m, 1), 4), np.float32)
+        q = np.zeros((max(m, 1), 4), np.float32)
+        P, V, Q, L = (x.copy() for x in (self.P, self.V, self.Q, self.L))
+        load().or_hull_step(self.n, p(P), p(V), p(Q), p(L), p(self.I0), p(self.I1), p(self.radius), p(self.vstart),
+                            p(self.nv), p(self.verts4), len(self.verts4), p(self.scene_first),
+                            len(self.scene_first) - 1, C.byref(oc), max_iter, nthreads, p(pairs), p(f), p(q), m)
+        return pairs[:m], f[:m], q[:m]
 
 
 def global_domain(lo, hi):

@@ -125,6 +125,13 @@ void or_body_reduce(int nb, const float* bpos4, const int* first, const int* cou
                     const float* sf4, const float* st4, float* bf4, float* bt4);
 void or_integrate_bodies(int nb, float* P, float* V, float* Q, float* L, const float* inv6, const float* f4,
                          const float* t4, const or_contact* c);
+/* One step of the rigid convex-body context (sd_hull_step): returns the
+ * number of touching pairs; the first out_capacity are written (pairs, force
+ * on b, contact point; all nullable). */
+long or_hull_step(int n, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const float* radius,
+                  const int* vstart, const int* nv, const float* verts4, int n_verts, const int* scene_first,
+                  int nscenes, const or_contact* c, int max_iter, int nthreads, int* pairs_out, float* force_out,
+                  float* point_out, long out_capacity);
 long or_clump_step(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const int* first,
                    const int* count, const float* local4, int ns, const or_contact* c);
 

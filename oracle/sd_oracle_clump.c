@@ -422,3 +422,225 @@ long or_clump_step(int nb, float* P, float* V, float* Q, float* L, const float* 
     free(pairs);
     return np;
 }
+
+/* ------------------------------------------------------------------ rigid convex bodies
+ * One step of the hull-body context (sd_hull_step), in its order: shape
+ * centres and world angular velocities, hull boxes, sweep-and-prune of every
+ * scene on its own axis (scene by scene, ids global), cold GJK/EPA, the
+ * contact of every touching pair, the per-body sums in pair order and the
+ * integration. The narrowphase replaces the boolean gjkCUDA of the reference
+ * sandbox (sandbox/testGJK.cu:237-315); the contact force and integration
+ * follow the clump path's law and advectParticlesCUDA
+ * (src/engine/cuda/physics_update.cu:232-310). */
+#define HB_FEATURE_REL 1e-3f
+
+static V3 hb_add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static V3 hb_sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static V3 hb_scl(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static float hb_dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static V3 hb_cross(V3 a, V3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+/* contact feature along u: the vertices within HB_FEATURE_REL times the
+ * shape's radius (largest vertex distance from its centre) of the extreme one
+ * along u, their mean (vertex order, world) and the
+ * largest squared distance of one of them from it; a sphere's X + r u */
+static V3 hb_feature(const float* verts4, int nv, float radius, const float m[9], V3 X, V3 u, float* spread2) {
+    *spread2 = 0.0f;
+    if (nv == 0) return hb_add(X, hb_scl(u, radius));
+    const V3 l = mtv(m, u);
+    float dmax = -3.402823466e38f, r2 = 0.0f;
+    for (int k = 0; k < nv; ++k) {
+        const float* v = verts4 + 4 * (long)k;
+        const float d = v[0] * l.x + v[1] * l.y + v[2] * l.z;
+        dmax = fmaxf(dmax, d);
+        r2 = fmaxf(r2, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    const float cut = dmax - HB_FEATURE_REL * sqrtf(r2);
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    int cnt = 0;
+    for (int k = 0; k < nv; ++k) {
+        const float* v = verts4 + 4 * (long)k;
+        const float d = v[0] * l.x + v[1] * l.y + v[2] * l.z;
+        if (d >= cut) {
+            sx = sx + v[0];
+            sy = sy + v[1];
+            sz = sz + v[2];
+            ++cnt;
+        }
+    }
+    const float inv = 1.0f / (float)cnt;
+    const float mx = sx * inv, my = sy * inv, mz = sz * inv;
+    for (int k = 0; k < nv; ++k) {
+        const float* v = verts4 + 4 * (long)k;
+        const float d = v[0] * l.x + v[1] * l.y + v[2] * l.z;
+        if (d >= cut) {
+            const float ex = v[0] - mx, ey = v[1] - my, ez = v[2] - mz;
+            *spread2 = fmaxf(*spread2, ex * ex + ey * ey + ez * ez);
+        }
+    }
+    return hb_add(mv(m, mk(mx, my, mz)), X);
+}
+
+/* the centroid of the feature of smaller spread; the midpoint on a tie */
+static V3 hb_contact_point(V3 ca, float wa, V3 cb, float wb) {
+    const float t = wa < wb ? 0.0f : (wb < wa ? 1.0f : 0.5f);
+    return hb_add(ca, hb_scl(hb_sub(cb, ca), t));
+}
+
+typedef struct {
+    int p;
+    int e;
+} HbEntry;
+static int cmp_entry(const void* a, const void* b) {
+    const int x = ((const HbEntry*)a)->e, y = ((const HbEntry*)b)->e;
+    return (x > y) - (x < y);
+}
+
+long or_hull_step(int n, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const float* radius,
+                  const int* vstart, const int* nv, const float* verts4, int n_verts, const int* scene_first,
+                  int nscenes, const or_contact* c, int max_iter, int nthreads, int* pairs_out, float* force_out,
+                  float* point_out, long out_capacity) {
+    (void)n_verts;
+    float* spos = (float*)malloc(4 * (size_t)n * sizeof(float));
+    float* om = (float*)malloc(3 * (size_t)n * sizeof(float));
+    float* lo = (float*)malloc(4 * (size_t)n * sizeof(float));
+    float* hi = (float*)malloc(4 * (size_t)n * sizeof(float));
+    for (int b = 0; b < n; ++b) {
+        spos[4 * b] = P[4 * b];
+        spos[4 * b + 1] = P[4 * b + 1];
+        spos[4 * b + 2] = P[4 * b + 2];
+        spos[4 * b + 3] = radius[b];
+        float m[9];
+        qmat(Q + 4 * (long)b, m);
+        const V3 w = inv_inertia(m, I0 + 4 * (long)b, I1 + 4 * (long)b, mk(L[4 * b], L[4 * b + 1], L[4 * b + 2]));
+        om[3 * b] = w.x;
+        om[3 * b + 1] = w.y;
+        om[3 * b + 2] = w.z;
+    }
+    or_hull_aabbs(n, spos, Q, vstart, nv, verts4, lo, hi);
+    /* candidate pairs, scene by scene */
+    long cap = 1024, np = 0;
+    int* pairs = (int*)malloc(2 * (size_t)cap * sizeof(int));
+    for (int s = 0; s < nscenes; ++s) {
+        const int f = scene_first[s], k = scene_first[s + 1] - f;
+        if (k < 2) continue;
+        const int axis = or_sap_axis(k, lo + 4 * (long)f, hi + 4 * (long)f);
+        long m = or_sap_pairs(k, lo + 4 * (long)f, hi + 4 * (long)f, axis, NULL, 0);
+        if (np + m > cap) {
+            while (np + m > cap) cap *= 2;
+            pairs = (int*)realloc(pairs, 2 * (size_t)cap * sizeof(int));
+        }
+        or_sap_pairs(k, lo + 4 * (long)f, hi + 4 * (long)f, axis, pairs + 2 * np, m);
+        for (long t = 2 * np; t < 2 * (np + m); ++t) pairs[t] += f;
+        np += m;
+    }
+    float* sep = (float*)malloc((size_t)(np + 1) * sizeof(float));
+    float* nrm = (float*)malloc(3 * (size_t)(np + 1) * sizeof(float));
+    int* flags = (int*)malloc((size_t)(np + 1) * sizeof(int));
+    int* iters = (int*)malloc((size_t)(np + 1) * sizeof(int));
+    if (np) or_gjk((int)np, pairs, spos, Q, vstart, nv, verts4, NULL, 1, max_iter, 0.0f, sep, nrm, flags, iters,
+                   nthreads);
+    float* pf = (float*)calloc(4 * (size_t)(np + 1), sizeof(float));
+    float* pp = (float*)calloc(4 * (size_t)(np + 1), sizeof(float));
+    int* cnt = (int*)calloc((size_t)n, sizeof(int));
+    const float c2z = 2.0f * c->damping_ratio;
+    long ncontacts = 0;
+    for (long t = 0; t < np; ++t) {
+        const int a = pairs[2 * t], b = pairs[2 * t + 1];
+        const float* Pa = P + 4 * (long)a;
+        const float* Pb = P + 4 * (long)b;
+        const float minv = Pa[3] + Pb[3];
+        const int fl = flags[t];
+        const float s = sep[t];
+        if (fl < 0 || !(fl & 1) || !(s < 0.0f) || !(minv > 0.0f)) continue;
+        const V3 nn = mk(nrm[3 * t], nrm[3 * t + 1], nrm[3 * t + 2]);
+        const V3 Xa = mk(Pa[0], Pa[1], Pa[2]), Xb = mk(Pb[0], Pb[1], Pb[2]);
+        float ma[9], mb[9];
+        qmat(Q + 4 * (long)a, ma);
+        qmat(Q + 4 * (long)b, mb);
+        float sa, sb;
+        const V3 ca = hb_feature(verts4 + 4 * (long)vstart[a], nv[a], radius[a], ma, Xa, nn, &sa);
+        const V3 cb = hb_feature(verts4 + 4 * (long)vstart[b], nv[b], radius[b], mb, Xb, hb_scl(nn, -1.0f), &sb);
+        const V3 pt = hb_contact_point(ca, sa, cb, sb);
+        const V3 wa = mk(om[3 * a], om[3 * a + 1], om[3 * a + 2]), wb = mk(om[3 * b], om[3 * b + 1], om[3 * b + 2]);
+        const V3 va = hb_add(mk(V[4 * a], V[4 * a + 1], V[4 * a + 2]), hb_cross(wa, hb_sub(pt, Xa)));
+        const V3 vb = hb_add(mk(V[4 * b], V[4 * b + 1], V[4 * b + 2]), hb_cross(wb, hb_sub(pt, Xb)));
+        const V3 vr = hb_sub(vb, va);
+        const float vn = hb_dot(vr, nn);
+        const float meff = 1.0f / minv;
+        const float cn = c2z * sqrtf(c->kn * meff);
+        const float fn = c->kn * (-s) - cn * vn;
+        const V3 vt = hb_sub(vr, hb_scl(nn, vn));
+        const float vtn = sqrtf(hb_dot(vt, vt));
+        V3 f = hb_scl(nn, fn);
+        if (vtn > 0.0f) {
+            const float fmag = fminf(c->mu * fmaxf(fn, 0.0f), cn * vtn);
+            f = hb_sub(f, hb_scl(vt, fmag / vtn));
+        }
+        pf[4 * t] = f.x;
+        pf[4 * t + 1] = f.y;
+        pf[4 * t + 2] = f.z;
+        pf[4 * t + 3] = 1.0f;
+        pp[4 * t] = pt.x;
+        pp[4 * t + 1] = pt.y;
+        pp[4 * t + 2] = pt.z;
+        cnt[a]++;
+        cnt[b]++;
+        if (ncontacts < out_capacity) {
+            if (pairs_out) {
+                pairs_out[2 * ncontacts] = a;
+                pairs_out[2 * ncontacts + 1] = b;
+            }
+            if (force_out) memcpy(force_out + 4 * ncontacts, pf + 4 * t, 4 * sizeof(float));
+            if (point_out) memcpy(point_out + 4 * ncontacts, pp + 4 * t, 4 * sizeof(float));
+        }
+        ++ncontacts;
+    }
+    /* per body: its contacts in entry order (2 p for a, 2 p + 1 for b) */
+    long* offs = (long*)malloc((size_t)(n + 1) * sizeof(long));
+    offs[0] = 0;
+    for (int b = 0; b < n; ++b) offs[b + 1] = offs[b] + cnt[b];
+    HbEntry* list = (HbEntry*)malloc((size_t)(offs[n] + 1) * sizeof(HbEntry));
+    long* fill = (long*)calloc((size_t)n, sizeof(long));
+    for (long t = 0; t < np; ++t) {
+        if (!(pf[4 * t + 3] > 0.0f)) continue;
+        const int a = pairs[2 * t], b = pairs[2 * t + 1];
+        list[offs[a] + fill[a]++] = (HbEntry){(int)t, (int)(2 * t)};
+        list[offs[b] + fill[b]++] = (HbEntry){(int)t, (int)(2 * t + 1)};
+    }
+    for (int b = 0; b < n; ++b) {
+        qsort(list + offs[b], (size_t)cnt[b], sizeof(HbEntry), cmp_entry);
+        float* X = P + 4 * (long)b;
+        float F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+        for (long i = offs[b]; i < offs[b + 1]; ++i) {
+            const int e = list[i].e, p = e >> 1;
+            const float sg = (e & 1) ? 1.0f : -1.0f;
+            const float fx = sg * pf[4 * (long)p], fy = sg * pf[4 * (long)p + 1], fz = sg * pf[4 * (long)p + 2];
+            F[0] = F[0] + fx;
+            F[1] = F[1] + fy;
+            F[2] = F[2] + fz;
+            const float rx = pp[4 * (long)p] - X[0], ry = pp[4 * (long)p + 1] - X[1], rz = pp[4 * (long)p + 2] - X[2];
+            T[0] = T[0] + (ry * fz - rz * fy);
+            T[1] = T[1] + (rz * fx - rx * fz);
+            T[2] = T[2] + (rx * fy - ry * fx);
+        }
+        integrate_body(X, V + 4 * (long)b, Q + 4 * (long)b, L + 4 * (long)b, I0 + 4 * (long)b, I1 + 4 * (long)b, F, T,
+                       c);
+    }
+    free(spos);
+    free(om);
+    free(lo);
+    free(hi);
+    free(pairs);
+    free(sep);
+    free(nrm);
+    free(flags);
+    free(iters);
+    free(pf);
+    free(pp);
+    free(cnt);
+    free(offs);
+    free(list);
+    free(fill);
+    return ncontacts;
+}

@@ -166,6 +166,12 @@ _SIGS = {
     "sd_clump_download": [_P, _P, _P, _P, _P],
     "sd_clump_pairs": [_P, _P, _I, C.POINTER(C.c_int)],
     "sd_clump_spheres": [_P, _P],
+    "sd_hull_create": [_P, _P],
+    "sd_hull_destroy": [_P],
+    "sd_hull_upload": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _I],
+    "sd_hull_step": [_P, _I],
+    "sd_hull_download": [_P, _P, _P, _P, _P],
+    "sd_hull_contacts": [_P, _P, _P, _P, _I, C.POINTER(C.c_int)],
     "sd_clump_state_bytes": [_P, C.POINTER(_SZ)],
     "sd_clump_save": [_P, _P, _SZ],
     "sd_clump_load": [_P, _P, _SZ],

@@ -30,6 +30,7 @@
 #include "sd_contact_law.hpp"
 #include "sd_gjk_dev.hpp"
 #include "sd_internal.hpp"
+#include "sd_rigid_dev.hpp"
 
 extern "C" sd_status sd_global_domain(const float* lo4, const float* hi4, int n, float* domain6, void* scratch,
                                       void* stream);
@@ -45,61 +46,6 @@ extern "C" sd_status sd_lbvh_query(const uint64_t* sorted_keys, int n, const int
                                    void* stream);
 
 namespace sd {
-
-// sin(a), cos(a) for 0 <= a: Taylor polynomials in Horner form below 0.25.
-__device__ __forceinline__ void sincos_small(float a, float& s, float& c) {
-    if (a < 0.25f) {
-        const float a2 = a * a;
-        s = a * (1.0f - a2 / 6.0f * (1.0f - a2 / 20.0f * (1.0f - a2 / 42.0f * (1.0f - a2 / 72.0f))));
-        c = 1.0f - a2 / 2.0f * (1.0f - a2 / 12.0f * (1.0f - a2 / 30.0f * (1.0f - a2 / 56.0f)));
-    } else {
-        s = sinf(a);
-        c = cosf(a);
-    }
-}
-
-// World inverse inertia R I^-1 R^T applied to L (I^-1 symmetric, body frame:
-// i0 = (xx, yy, zz, xy), i1 = (xz, yz, -, -)).
-__device__ __forceinline__ V3 inv_inertia_apply(const Rot& R, float4 i0, float4 i1, V3 L) {
-    // u = R^T L, then I^-1 u, then R (.)
-    const V3 u = rot_tmul(R, L);
-    const V3 w = v3(i0.x * u.x + i0.w * u.y + i1.x * u.z, i0.w * u.x + i0.y * u.y + i1.y * u.z,
-                    i1.x * u.x + i1.y * u.y + i0.z * u.z);
-    return rot_mul(R, w);
-}
-
-// Semi-implicit step of one body with 1/m > 0 under force F and torque T
-// (the integration above); a static body (1/m = 0) is left as it is.
-__device__ __forceinline__ void integrate_body(float4& P, float4& V, float4& q, float4& L, float4 i0, float4 i1,
-                                               float Fx, float Fy, float Fz, float Tx, float Ty, float Tz,
-                                               const ContactDev& c) {
-    if (!(P.w > 0.0f)) return;
-    V.x = V.x + (Fx * P.w + c.gx) * c.dt;
-    V.y = V.y + (Fy * P.w + c.gy) * c.dt;
-    V.z = V.z + (Fz * P.w + c.gz) * c.dt;
-    P.x = P.x + V.x * c.dt;
-    P.y = P.y + V.y * c.dt;
-    P.z = P.z + V.z * c.dt;
-    L.x = L.x + Tx * c.dt;
-    L.y = L.y + Ty * c.dt;
-    L.z = L.z + Tz * c.dt;
-    const Rot R = quat_rot(q);
-    const V3 w = inv_inertia_apply(R, i0, i1, v3(L.x, L.y, L.z));
-    const float wn = sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
-    if (wn > 0.0f) {
-        float sa, ca;
-        sincos_small(wn * (0.5f * c.dt), sa, ca);
-        const float k = sa / wn;
-        const float dw = ca, dx = w.x * k, dy = w.y * k, dz = w.z * k;
-        // dq * q
-        const float nw = dw * q.x - (dx * q.y + dy * q.z + dz * q.w);
-        const float nx = dw * q.y + q.x * dx + (dy * q.w - dz * q.z);
-        const float ny = dw * q.z + q.x * dy + (dz * q.y - dx * q.w);
-        const float nz = dw * q.w + q.x * dz + (dx * q.z - dy * q.y);
-        const float inv = 1.0f / sqrtf(nw * nw + nx * nx + ny * ny + nz * nz);
-        q = make_float4(nw * inv, nx * inv, ny * inv, nz * inv);
-    }
-}
 
 struct ClumpState {
     const float4* pos;   // (x, y, z, 1/m)
@@ -534,6 +480,56 @@ extern "C" sd_status sd_clump_destroy(sd_clump_ctx* c) {
 // Bodies: pos4 (x, y, z, mass), vel4 (v, 0), quat4 (w, x, y, z), angvel4 (w, 0) world,
 // inertia6 (body frame xx, yy, zz, xy, xz, yz), first/count sphere ranges.
 // Spheres: local4 (body-frame offset from the centre of mass, radius).
+// Body rows from upload arguments: P = (x, y, z, 1/m), V = (v, 0) (zero for a
+// static body), Q = q, L = R I R^T w (world angular momentum), I0/I1 = the
+// body-frame inverse inertia; inverse and L in double precision on the host.
+sd_status sd::prepare_rigid(const char* who, int nb, const float* pos4, const float* vel4, const float* quat4,
+                        const float* angvel4, const float* inertia6, float4* P, float4* V, float4* Q, float4* L,
+                        float4* I0, float4* I1) {
+    for (int b = 0; b < nb; ++b) {
+        const float* p = pos4 + 4 * (size_t)b;
+        const float m = p[3];
+        if (!(m >= 0.0f)) return set_error(SD_ERR_ARG, "%s: body %d has a negative mass", who, b);
+        P[b] = make_float4(p[0], p[1], p[2], m > 0.0f ? 1.0f / m : 0.0f);
+        const float* v = vel4 + 4 * (size_t)b;
+        V[b] = m > 0.0f ? make_float4(v[0], v[1], v[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float* q = quat4 + 4 * (size_t)b;
+        Q[b] = make_float4(q[0], q[1], q[2], q[3]);
+        // inverse of the symmetric body inertia (double precision on the host)
+        const float* I = inertia6 + 6 * (size_t)b;
+        const double a = I[0], e = I[1], f = I[2], xy = I[3], xz = I[4], yz = I[5];
+        const double c00 = e * f - yz * yz, c01 = xz * yz - xy * f, c02 = xy * yz - xz * e;
+        const double det = a * c00 + xy * c01 + xz * c02;
+        if (m > 0.0f && !(det > 0.0))
+            return set_error(SD_ERR_ARG, "%s: body %d has a singular inertia tensor", who, b);
+        double inv[6] = {0, 0, 0, 0, 0, 0};
+        if (m > 0.0f) {
+            inv[0] = c00 / det;
+            inv[1] = (a * f - xz * xz) / det;
+            inv[2] = (a * e - xy * xy) / det;
+            inv[3] = c01 / det;
+            inv[4] = c02 / det;
+            inv[5] = (xz * xy - a * yz) / det;
+        }
+        I0[b] = make_float4((float)inv[0], (float)inv[1], (float)inv[2], (float)inv[3]);
+        I1[b] = make_float4((float)inv[4], (float)inv[5], 0.0f, 0.0f);
+        // L = R I R^T w (world), in double
+        const float* w = angvel4 + 4 * (size_t)b;
+        const double qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+        const double R[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy),
+                             2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx),
+                             2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)};
+        const double Ib[9] = {a, xy, xz, xy, e, yz, xz, yz, f};
+        double u[3], t[3], Lw[3];
+        for (int k = 0; k < 3; ++k) u[k] = R[0 + k] * w[0] + R[3 + k] * w[1] + R[6 + k] * w[2];
+        for (int k = 0; k < 3; ++k) t[k] = Ib[3 * k] * u[0] + Ib[3 * k + 1] * u[1] + Ib[3 * k + 2] * u[2];
+        for (int k = 0; k < 3; ++k) Lw[k] = R[3 * k] * t[0] + R[3 * k + 1] * t[1] + R[3 * k + 2] * t[2];
+        L[b] = m > 0.0f ? make_float4((float)Lw[0], (float)Lw[1], (float)Lw[2], 0.0f)
+                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    return SD_OK;
+}
+
 namespace {
 
 // Body rows as the context keeps them: P = (x, y, z, 1/m), V = (v, 0),
@@ -575,52 +571,11 @@ sd_status body_ranges(const char* who, int nb, const int* first, const int* coun
     return SD_OK;
 }
 
-// The rows of nb bodies from the upload arguments (inverse inertia and world
-// angular momentum in double precision on the host), written at rows[at ..].
+// The rows of nb bodies from the upload arguments, written at rows[at ..].
 sd_status prepare_bodies(const char* who, int nb, const float* pos4, const float* vel4, const float* quat4,
                          const float* angvel4, const float* inertia6, ClumpRows& r, int at) {
-    for (int b = 0; b < nb; ++b) {
-        const float* p = pos4 + 4 * (size_t)b;
-        const float m = p[3];
-        if (!(m >= 0.0f)) return set_error(SD_ERR_ARG, "%s: body %d has a negative mass", who, b);
-        r.P[at + b] = make_float4(p[0], p[1], p[2], m > 0.0f ? 1.0f / m : 0.0f);
-        const float* v = vel4 + 4 * (size_t)b;
-        r.V[at + b] = m > 0.0f ? make_float4(v[0], v[1], v[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const float* q = quat4 + 4 * (size_t)b;
-        r.Q[at + b] = make_float4(q[0], q[1], q[2], q[3]);
-        // inverse of the symmetric body inertia (double precision on the host)
-        const float* I = inertia6 + 6 * (size_t)b;
-        const double a = I[0], e = I[1], f = I[2], xy = I[3], xz = I[4], yz = I[5];
-        const double c00 = e * f - yz * yz, c01 = xz * yz - xy * f, c02 = xy * yz - xz * e;
-        const double det = a * c00 + xy * c01 + xz * c02;
-        if (m > 0.0f && !(det > 0.0))
-            return set_error(SD_ERR_ARG, "%s: body %d has a singular inertia tensor", who, b);
-        double inv[6] = {0, 0, 0, 0, 0, 0};
-        if (m > 0.0f) {
-            inv[0] = c00 / det;
-            inv[1] = (a * f - xz * xz) / det;
-            inv[2] = (a * e - xy * xy) / det;
-            inv[3] = c01 / det;
-            inv[4] = c02 / det;
-            inv[5] = (xz * xy - a * yz) / det;
-        }
-        r.I0[at + b] = make_float4((float)inv[0], (float)inv[1], (float)inv[2], (float)inv[3]);
-        r.I1[at + b] = make_float4((float)inv[4], (float)inv[5], 0.0f, 0.0f);
-        // L = R I R^T w (world), in double
-        const float* w = angvel4 + 4 * (size_t)b;
-        const double qw = q[0], qx = q[1], qy = q[2], qz = q[3];
-        const double R[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy),
-                             2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx),
-                             2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)};
-        const double Ib[9] = {a, xy, xz, xy, e, yz, xz, yz, f};
-        double u[3], t[3], Lw[3];
-        for (int k = 0; k < 3; ++k) u[k] = R[0 + k] * w[0] + R[3 + k] * w[1] + R[6 + k] * w[2];
-        for (int k = 0; k < 3; ++k) t[k] = Ib[3 * k] * u[0] + Ib[3 * k + 1] * u[1] + Ib[3 * k + 2] * u[2];
-        for (int k = 0; k < 3; ++k) Lw[k] = R[3 * k] * t[0] + R[3 * k + 1] * t[1] + R[3 * k + 2] * t[2];
-        r.L[at + b] = m > 0.0f ? make_float4((float)Lw[0], (float)Lw[1], (float)Lw[2], 0.0f)
-                               : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    return SD_OK;
+    return prepare_rigid(who, nb, pos4, vel4, quat4, angvel4, inertia6, r.P.data() + at, r.V.data() + at,
+                         r.Q.data() + at, r.L.data() + at, r.I0.data() + at, r.I1.data() + at);
 }
 
 // FNV-1a 64 over the rows that stay fixed while stepping (inverse inertia,

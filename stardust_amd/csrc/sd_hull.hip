@@ -575,8 +575,11 @@ __global__ __launch_bounds__(64) void k_gjk_phase(const int* __restrict__ pairs,
                                                   float* __restrict__ normal, int* __restrict__ flags,
                                                   int* __restrict__ iters, EpaEntry* __restrict__ list,
                                                   int* __restrict__ count, int2* __restrict__ retry,
-                                                  int* __restrict__ retry_count, const SeedState* __restrict__ seed) {
-    const int n = RETRY ? *retry_count : npairs;
+                                                  int* __restrict__ retry_count, const SeedState* __restrict__ seed,
+                                                  const int* __restrict__ dcount) {
+    // dcount (nullable): pairs held on the device (the first min(*dcount,
+    // npairs) of the list), for launches sized to a capacity
+    const int n = RETRY ? *retry_count : (dcount ? min(*dcount, npairs) : npairs);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += RETRY ? gridDim.x * blockDim.x : n) {
         const int p = RETRY ? retry[t].x : t;
         const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
@@ -909,60 +912,25 @@ extern "C" sd_status sd_sap_pairs_batched(const float* lo4, const float* hi4, in
     }
     SD_REQUIRE(lo4 && hi4 && scene_of && scene_first && scratch && (pairs || capacity == 0),
                "sd_sap_pairs_batched: null pointer");
-    const int ibits = bits_of((uint64_t)n);
-    const int sbits = bits_of((uint64_t)nscenes);
-    SD_REQUIRE(sbits + 32 + ibits <= 64, "sd_sap_pairs_batched: %d hulls in %d scenes do not fit 64-bit keys", n,
-               nscenes);
-    size_t need;
-    sd_sap_batched_scratch_bytes(n, nscenes, &need);
-    SD_REQUIRE(scratch_bytes >= need, "sd_sap_pairs_batched: scratch too small (%zu < %zu)", scratch_bytes, need);
     hipStream_t s = (hipStream_t)stream;
-    char* p = (char*)scratch;
-    uint64_t* keys = (uint64_t*)carve(p, (size_t)n * 8);
-    uint64_t* keys2 = (uint64_t*)carve(p, (size_t)n * 8);
-    float4* slo = (float4*)carve(p, (size_t)n * sizeof(float4));
-    float4* shi = (float4*)carve(p, (size_t)n * sizeof(float4));
-    int* counts = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
-    int* offsets = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
-    int* slots = (int*)carve(p, (size_t)n * SAP_SLOTS * sizeof(int));
-    int* ovf = (int*)carve(p, (size_t)(n + 1) * sizeof(int));
-    int* ovf_count = ovf + n;
-    int* axis = (int*)carve(p, (size_t)nscenes * sizeof(int));
-    int* error = (int*)carve(p, 64);
-    void* sort_tmp = (void*)carve(p, radix_temp_bytes(n));
-    void* scan_tmp = (void*)carve(p, scan_temp_bytes(n));
-    int* total = offsets + n;
-    const float4* L = (const float4*)lo4;
-    const float4* H = (const float4*)hi4;
-    sd_status st = launch_fill_u32((uint32_t*)error, 0u, 1, s);
+    int *total = nullptr, *error = nullptr;
+    sd_status st = enqueue_sap_batched((const float4*)lo4, (const float4*)hi4, n, scene_of, scene_first, nscenes,
+                                       pairs, capacity, &total, &error, scratch, scratch_bytes, s);
     if (st != SD_OK) return st;
-    hipLaunchKernelGGL(k_axis_scenes, dim3(nscenes), dim3(AX_BLOCK), 0, s, L, H, n, scene_first, nscenes, axis,
-                       error);
-    SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sapb_keys, dim3(div_up(n, 256)), dim3(256), 0, s, L, n, scene_of, nscenes,
-                       (const int*)axis, ibits, keys, error);
-    SD_LAUNCH_CHECK();
-    uint64_t* K = keys;
-    uint64_t* Ka = keys2;
-    st = radix_sort_keys_u64(&K, &Ka, n, 0, sbits + 32 + ibits, sort_tmp, radix_temp_bytes(n), s, 0);
-    if (st != SD_OK) return st;
-    hipLaunchKernelGGL(k_sapb_gather, dim3(div_up(n, 256)), dim3(256), 0, s, (const uint64_t*)K, n, ibits, L, H,
-                       scene_of, (const int*)axis, nscenes, slo, shi);
-    SD_LAUNCH_CHECK();
-    st = launch_fill_u32((uint32_t*)ovf_count, 0u, 1, s);
-    if (st != SD_OK) return st;
-    hipLaunchKernelGGL(k_sap_sweep<true>, dim3(div_up(n, SAP_TILE)), dim3(SAP_TILE), 0, s, slo, shi, n, counts,
-                       slots, ovf, ovf_count);
-    SD_LAUNCH_CHECK();
-    st = exclusive_scan_i32(counts, offsets, n, total, scan_tmp, s);
-    if (st != SD_OK) return st;
-    hipLaunchKernelGGL(k_sap_emit, dim3(div_up(n, 256)), dim3(256), 0, s, slo, n, counts, offsets, slots, pairs,
-                       capacity);
-    SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sap_overflow<true>, dim3(SAP_OVF_BLOCKS), dim3(256), 0, s, slo, shi, n, counts, offsets,
-                       slots, ovf, ovf_count, pairs, capacity);
-    SD_LAUNCH_CHECK();
-    if (axes) SD_HIP(hipMemcpyAsync(axes, axis, (size_t)nscenes * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (axes) {
+        // the axes follow the keys in the scratch layout: carve again to find them
+        char* p = (char*)scratch;
+        carve(p, (size_t)n * 8);
+        carve(p, (size_t)n * 8);
+        carve(p, (size_t)n * sizeof(float4));
+        carve(p, (size_t)n * sizeof(float4));
+        carve(p, (size_t)(n + 2) * sizeof(int));
+        carve(p, (size_t)(n + 2) * sizeof(int));
+        carve(p, (size_t)n * SAP_SLOTS * sizeof(int));
+        carve(p, (size_t)(n + 1) * sizeof(int));
+        const int* axis = (const int*)p;
+        SD_HIP(hipMemcpyAsync(axes, axis, (size_t)nscenes * sizeof(int), hipMemcpyDeviceToDevice, s));
+    }
     int h[2] = {0, 0};
     SD_HIP(hipMemcpyAsync(&h[0], total, sizeof(int), hipMemcpyDeviceToHost, s));
     SD_HIP(hipMemcpyAsync(&h[1], error, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1012,11 +980,11 @@ extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, co
         SD_LAUNCH_CHECK();
         hipLaunchKernelGGL((k_gjk_phase<false, true>), dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h,
                            n_hulls, n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters,
-                           list, count, retry, count + 1, (const SeedState*)seed);
+                           list, count, retry, count + 1, (const SeedState*)seed, (const int*)nullptr);
     } else {
         hipLaunchKernelGGL((k_gjk_phase<false, false>), dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h,
                            n_hulls, n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters,
-                           list, count, retry, count + 1, (const SeedState*)nullptr);
+                           list, count, retry, count + 1, (const SeedState*)nullptr, (const int*)nullptr);
     }
     SD_LAUNCH_CHECK();
     const int fin_blocks = std::min(div_up(n_pairs, 64), GJK_RETRY_MAX_BLOCKS);
@@ -1028,7 +996,7 @@ extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, co
     if (warm_dir == nullptr) return SD_OK;  // cold entries are never rejected
     hipLaunchKernelGGL((k_gjk_phase<true, false>), dim3(fin_blocks), dim3(64), 0, s, pairs, n_pairs, h, n_hulls,
                        n_verts, nullptr, 1, max_iter, near_margin, sep, normal3, flags, iters, list, count + 2, retry,
-                       count + 1, (const SeedState*)nullptr);
+                       count + 1, (const SeedState*)nullptr, (const int*)nullptr);
     SD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)(count + 2));
     SD_LAUNCH_CHECK();
@@ -1037,3 +1005,101 @@ extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, co
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
+
+namespace sd {
+
+sd_status launch_hull_aabbs(const float4* pos4, const float4* quat4, const int* vstart, const int* nv,
+                            const float4* verts4, int n, float4* lo4, float4* hi4, hipStream_t s) {
+    if (n == 0) return SD_OK;
+    hipLaunchKernelGGL(k_hull_aabb, dim3(div_up(n, 256)), dim3(256), 0, s, pos4, quat4, vstart, nv, verts4, n, lo4,
+                       hi4);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+sd_status enqueue_sap_batched(const float4* L, const float4* H, int n, const int* scene_of, const int* scene_first,
+                              int nscenes, int* pairs, int capacity, int** total_out, int** error_out, void* scratch,
+                              size_t scratch_bytes, hipStream_t s) {
+    const int ibits = bits_of((uint64_t)n);
+    const int sbits = bits_of((uint64_t)nscenes);
+    SD_REQUIRE(sbits + 32 + ibits <= 64, "sweep-and-prune: %d hulls in %d scenes do not fit 64-bit keys", n,
+               nscenes);
+    size_t need;
+    sd_sap_batched_scratch_bytes(n, nscenes, &need);
+    SD_REQUIRE(scratch_bytes >= need, "sweep-and-prune: scratch too small (%zu < %zu)", scratch_bytes, need);
+    char* p = (char*)scratch;
+    uint64_t* keys = (uint64_t*)carve(p, (size_t)n * 8);
+    uint64_t* keys2 = (uint64_t*)carve(p, (size_t)n * 8);
+    float4* slo = (float4*)carve(p, (size_t)n * sizeof(float4));
+    float4* shi = (float4*)carve(p, (size_t)n * sizeof(float4));
+    int* counts = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
+    int* offsets = (int*)carve(p, (size_t)(n + 2) * sizeof(int));
+    int* slots = (int*)carve(p, (size_t)n * SAP_SLOTS * sizeof(int));
+    int* ovf = (int*)carve(p, (size_t)(n + 1) * sizeof(int));
+    int* ovf_count = ovf + n;
+    int* axis = (int*)carve(p, (size_t)nscenes * sizeof(int));
+    int* error = (int*)carve(p, 64);
+    void* sort_tmp = (void*)carve(p, radix_temp_bytes(n));
+    void* scan_tmp = (void*)carve(p, scan_temp_bytes(n));
+    *total_out = offsets + n;
+    *error_out = error;
+    sd_status st = launch_fill_u32((uint32_t*)error, 0u, 1, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_axis_scenes, dim3(nscenes), dim3(AX_BLOCK), 0, s, L, H, n, scene_first, nscenes, axis,
+                       error);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sapb_keys, dim3(div_up(n, 256)), dim3(256), 0, s, L, n, scene_of, nscenes,
+                       (const int*)axis, ibits, keys, error);
+    SD_LAUNCH_CHECK();
+    uint64_t* K = keys;
+    uint64_t* Ka = keys2;
+    st = radix_sort_keys_u64(&K, &Ka, n, 0, sbits + 32 + ibits, sort_tmp, radix_temp_bytes(n), s, 0);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_sapb_gather, dim3(div_up(n, 256)), dim3(256), 0, s, (const uint64_t*)K, n, ibits, L, H,
+                       scene_of, (const int*)axis, nscenes, slo, shi);
+    SD_LAUNCH_CHECK();
+    st = launch_fill_u32((uint32_t*)ovf_count, 0u, 1, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_sap_sweep<true>, dim3(div_up(n, SAP_TILE)), dim3(SAP_TILE), 0, s, slo, shi, n, counts,
+                       slots, ovf, ovf_count);
+    SD_LAUNCH_CHECK();
+    st = exclusive_scan_i32(counts, offsets, n, offsets + n, scan_tmp, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_sap_emit, dim3(div_up(n, 256)), dim3(256), 0, s, slo, n, counts, offsets, slots, pairs,
+                       capacity);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sap_overflow<true>, dim3(SAP_OVF_BLOCKS), dim3(256), 0, s, slo, shi, n, counts, offsets,
+                       slots, ovf, ovf_count, pairs, capacity);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+sd_status enqueue_gjk_cold(const int* pairs, int capacity, const int* dcount, const float4* pos4,
+                           const float4* quat4, const int* vstart, const int* nv, const float4* verts4, int n_hulls,
+                           int n_verts, int max_iter, float near_margin, float* sep, float* normal3, int* flags,
+                           int* iters, void* scratch, hipStream_t s) {
+    if (capacity == 0) return SD_OK;
+    HullSet h{pos4, quat4, vstart, nv, verts4};
+    int* count = (int*)scratch;
+    EpaEntry* list = (EpaEntry*)((char*)scratch + 256);
+    int2* retry = (int2*)((char*)scratch + 256 + (size_t)capacity * sizeof(EpaEntry));
+    sd_status st = launch_fill_u32((uint32_t*)count, 0u, 3, s);
+    if (st != SD_OK) return st;
+    const int epa_blocks = std::min(div_up(capacity, EPA_GROUPS_PER_BLOCK), epa_resident_blocks());
+    const dim3 epa_block(64 * EPA_WAVES_PER_BLOCK);
+    hipLaunchKernelGGL((k_gjk_phase<false, false>), dim3(div_up(capacity, 64)), dim3(64), 0, s, pairs, capacity, h,
+                       n_hulls, n_verts, (const float*)nullptr, 1, max_iter, near_margin, sep, normal3, flags, iters,
+                       list, count, retry, count + 1, (const SeedState*)nullptr, dcount);
+    SD_LAUNCH_CHECK();
+    const int fin_blocks = std::min(div_up(capacity, 64), GJK_RETRY_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)count);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_epa_finish, dim3(fin_blocks), dim3(64), 0, s, pairs, h, near_margin, sep, normal3, flags,
+                       iters, (const EpaEntry*)list, (const int*)count, retry, count + 1);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+size_t gjk_cold_scratch_bytes(int capacity) { return gjk_scratch_bytes(capacity); }
+
+}  // namespace sd

@@ -1,0 +1,568 @@
+// Rigid convex bodies stepped on the GPU (sd_hull_*): convex hulls given by
+// their vertices, spheres (no vertices, a radius) and static triangles (three
+// vertices, mass 0) in independent scenes, each step
+//   pose      shape centre (x, y, z, radius) and world angular velocity
+//   boxes     k_hull_aabb
+//   pairs     batched sweep-and-prune (each scene on its own axis)
+//   narrow    cold GJK/EPA of every candidate pair
+//   contacts  per touching pair (sep < 0): contact point, linear
+//             spring-dashpot normal force, Coulomb-capped viscous friction
+//   gather    per body, its contacts in pair order (no float atomics), then
+//             the semi-implicit integration of the clump path
+// with every launch sized to the pair capacity and every count read on the
+// device, so the step is captured once in a hipGraph and replayed.
+//
+// Replaces the reference sandbox's hull pipeline (sandbox/testGJK.cu:237-315
+// gjkCUDA on Hull pairs, primitives.hpp:37-57 Hull with SPHERE, TRIANGLE and
+// POLYHEDRA types), which stops at a boolean overlap: here the narrowphase
+// result drives the bodies, and triangles (the reference's DEMMesh,
+// src/engine/entities/entity.hpp:64-93) are static bodies that hulls and
+// spheres collide with.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "sd_gjk_dev.hpp"
+#include "sd_internal.hpp"
+#include "sd_rigid_dev.hpp"
+
+namespace sd {
+
+// Support-feature tolerance: vertices within this fraction of a hull's radius
+// (its largest vertex distance from the centre) of its extreme one along the
+// contact normal form the contact feature.
+constexpr float HB_FEATURE_REL = 1e-3f;
+
+__global__ void k_hb_pose(const float4* __restrict__ P, const float4* __restrict__ Q, const float4* __restrict__ L,
+                          const float4* __restrict__ I0, const float4* __restrict__ I1,
+                          const float* __restrict__ radius, int n, float4* __restrict__ spos,
+                          float4* __restrict__ omega) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const float4 p = P[b], l = L[b];
+    spos[b] = make_float4(p.x, p.y, p.z, radius[b]);
+    const V3 w = inv_inertia_apply(quat_rot(Q[b]), I0[b], I1[b], v3(l.x, l.y, l.z));
+    omega[b] = make_float4(w.x, w.y, w.z, 0.0f);
+}
+
+// Contact feature of a shape along unit direction u: for a sphere the point
+// c + r u (spread 0); for a vertex set the vertices whose projection on u is
+// within HB_FEATURE_REL times the shape's radius (largest vertex distance from
+// its centre) of the largest, their
+// mean in vertex order (world) and spread2 = the largest squared distance of
+// one of them from that mean.
+__device__ __forceinline__ V3 feature_centroid(const float4* __restrict__ verts, int nv, float radius, const Rot& R,
+                                               V3 X, V3 u, float& spread2) {
+    spread2 = 0.0f;
+    if (nv == 0) return add(X, scl(u, radius));
+    const V3 l = rot_tmul(R, u);
+    float dmax = -3.402823466e38f, r2 = 0.0f;
+    for (int k = 0; k < nv; ++k) {
+        const float4 v = verts[k];
+        const float d = v.x * l.x + v.y * l.y + v.z * l.z;
+        dmax = fmaxf(dmax, d);
+        r2 = fmaxf(r2, v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    const float cut = dmax - HB_FEATURE_REL * sqrtf(r2);
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    int cnt = 0;
+    for (int k = 0; k < nv; ++k) {
+        const float4 v = verts[k];
+        const float d = v.x * l.x + v.y * l.y + v.z * l.z;
+        if (d >= cut) {
+            sx = sx + v.x;
+            sy = sy + v.y;
+            sz = sz + v.z;
+            ++cnt;
+        }
+    }
+    const float inv = 1.0f / (float)cnt;
+    const float mx = sx * inv, my = sy * inv, mz = sz * inv;
+    for (int k = 0; k < nv; ++k) {
+        const float4 v = verts[k];
+        const float d = v.x * l.x + v.y * l.y + v.z * l.z;
+        if (d >= cut) {
+            const float ex = v.x - mx, ey = v.y - my, ez = v.z - mz;
+            spread2 = fmaxf(spread2, ex * ex + ey * ey + ez * ez);
+        }
+    }
+    return add(rot_mul(R, v3(mx, my, mz)), X);
+}
+
+// Contact point of two features: the centroid of the one of smaller spread
+// (w = squared spread), the midpoint of the two when the spreads are equal
+// (two points, two like faces), so that a point or a small face touching a
+// large face acts where it touches.
+__device__ __forceinline__ V3 contact_point(V3 ca, float wa, V3 cb, float wb) {
+    const float t = wa < wb ? 0.0f : (wb < wa ? 1.0f : 0.5f);
+    return add(ca, scl(sub(cb, ca), t));
+}
+
+struct HbBodies {
+    const float4* P;      // (x, y, z, 1/m)
+    const float4* V;      // (v, 0)
+    const float4* Q;      // (w, x, y, z)
+    const float4* omega;  // world angular velocity
+    const float* radius;
+    const int* vstart;
+    const int* nv;
+    const float4* verts;
+};
+
+// One thread per candidate pair (a, b) of the first min(*total, capacity):
+// a touching pair (GJK/EPA flags valid and HIT, sep < 0, not both static)
+// gets pf = (force on b, 1) and pp = contact point, and counts one contact
+// for each body; any other pair gets pf.w = 0.
+__global__ void k_hb_pairs(const int* __restrict__ pairs, int capacity, const int* __restrict__ total, HbBodies B,
+                           const float* __restrict__ sep, const float* __restrict__ normal,
+                           const int* __restrict__ flags, ContactDev c, float4* __restrict__ pf,
+                           float4* __restrict__ pp, int* __restrict__ cnt, int* __restrict__ err) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = *total;
+    if (t == 0 && m > capacity) atomicOr(err, 1);
+    if (t >= min(m, capacity)) return;
+    const int a = pairs[2 * (size_t)t], b = pairs[2 * (size_t)t + 1];
+    const float4 Pa = B.P[a], Pb = B.P[b];
+    const float minv = Pa.w + Pb.w;
+    const int fl = flags[t];
+    const float s = sep[t];
+    if (fl < 0 || !(fl & 1) || !(s < 0.0f) || !(minv > 0.0f)) {
+        pf[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const V3 n = v3(normal[3 * (size_t)t], normal[3 * (size_t)t + 1], normal[3 * (size_t)t + 2]);
+    const V3 Xa = v3(Pa.x, Pa.y, Pa.z), Xb = v3(Pb.x, Pb.y, Pb.z);
+    const Rot Ra = quat_rot(B.Q[a]), Rb = quat_rot(B.Q[b]);
+    float wa, wb;
+    const V3 ca = feature_centroid(B.verts + B.vstart[a], B.nv[a], B.radius[a], Ra, Xa, n, wa);
+    const V3 cb = feature_centroid(B.verts + B.vstart[b], B.nv[b], B.radius[b], Rb, Xb, scl(n, -1.0f), wb);
+    const V3 pt = contact_point(ca, wa, cb, wb);
+    const float4 va4 = B.V[a], vb4 = B.V[b], wa4 = B.omega[a], wb4 = B.omega[b];
+    const V3 va = add(v3(va4.x, va4.y, va4.z), cross(v3(wa4.x, wa4.y, wa4.z), sub(pt, Xa)));
+    const V3 vb = add(v3(vb4.x, vb4.y, vb4.z), cross(v3(wb4.x, wb4.y, wb4.z), sub(pt, Xb)));
+    const V3 vr = sub(vb, va);
+    const float vn = dot(vr, n);
+    const float meff = 1.0f / minv;
+    const float cn = c.c2z * sqrtf(c.kn * meff);
+    const float fn = c.kn * (-s) - cn * vn;
+    const V3 vt = sub(vr, scl(n, vn));
+    const float vtn = sqrtf(dot(vt, vt));
+    V3 f = scl(n, fn);
+    if (vtn > 0.0f) {
+        const float fmag = fminf(c.mu * fmaxf(fn, 0.0f), cn * vtn);
+        f = sub(f, scl(vt, fmag / vtn));
+    }
+    pf[t] = make_float4(f.x, f.y, f.z, 1.0f);
+    pp[t] = make_float4(pt.x, pt.y, pt.z, 0.0f);
+    atomicAdd(&cnt[a], 1);
+    atomicAdd(&cnt[b], 1);
+}
+
+// Lists every touching pair under both bodies: entry 2 p (body a) or 2 p + 1
+// (body b) at offs[body] + a slot; slot order is arbitrary (the gather orders
+// the entries).
+__global__ void k_hb_fill(const int* __restrict__ pairs, int capacity, const int* __restrict__ total,
+                          const float4* __restrict__ pf, const int* __restrict__ offs, int* __restrict__ fillc,
+                          int* __restrict__ list) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= min(*total, capacity) || !(pf[t].w > 0.0f)) return;
+    const int a = pairs[2 * (size_t)t], b = pairs[2 * (size_t)t + 1];
+    list[offs[a] + atomicAdd(&fillc[a], 1)] = 2 * t;
+    list[offs[b] + atomicAdd(&fillc[b], 1)] = 2 * t + 1;
+}
+
+// Per body: F = sum f, T = sum (p - X) x f over its contacts in ascending
+// entry order (f = +pf on b, -pf on a), then the integration step.
+__global__ void k_hb_gather_integrate(int n, const int* __restrict__ offs, const int* __restrict__ cnt,
+                                      const int* __restrict__ list, const float4* __restrict__ pf,
+                                      const float4* __restrict__ pp, float4* __restrict__ P, float4* __restrict__ V,
+                                      float4* __restrict__ Q, float4* __restrict__ L, const float4* __restrict__ I0,
+                                      const float4* __restrict__ I1, ContactDev c, float4* __restrict__ fout,
+                                      float4* __restrict__ tout) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    float4 X = P[b];
+    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
+    const int o = offs[b], k = cnt[b];
+    int last = -1;
+    for (int i = 0; i < k; ++i) {
+        int e = 0x7fffffff;
+        for (int j = 0; j < k; ++j) {
+            const int x = list[o + j];
+            if (x > last && x < e) e = x;
+        }
+        last = e;
+        const int p = e >> 1;
+        const float4 f4 = pf[p], q4 = pp[p];
+        const float sg = (e & 1) ? 1.0f : -1.0f;
+        const float fx = sg * f4.x, fy = sg * f4.y, fz = sg * f4.z;
+        Fx = Fx + fx;
+        Fy = Fy + fy;
+        Fz = Fz + fz;
+        const float rx = q4.x - X.x, ry = q4.y - X.y, rz = q4.z - X.z;
+        Tx = Tx + (ry * fz - rz * fy);
+        Ty = Ty + (rz * fx - rx * fz);
+        Tz = Tz + (rx * fy - ry * fx);
+    }
+    if (fout) {
+        fout[b] = make_float4(Fx, Fy, Fz, 0.0f);
+        tout[b] = make_float4(Tx, Ty, Tz, 0.0f);
+    }
+    float4 v = V[b], q = Q[b], l = L[b];
+    integrate_body(X, v, q, l, I0[b], I1[b], Fx, Fy, Fz, Tx, Ty, Tz, c);
+    P[b] = X;
+    V[b] = v;
+    Q[b] = q;
+    L[b] = l;
+}
+
+}  // namespace sd
+
+using namespace sd;
+
+struct sd_hull_ctx {
+    sd_hull_params p{};
+    ContactDev c{};
+    hipStream_t stream = nullptr;
+    int n = 0, n_verts = 0, nscenes = 0, capacity = 0;
+    float4 *P = nullptr, *V = nullptr, *Q = nullptr, *L = nullptr, *I0 = nullptr, *I1 = nullptr;
+    float4 *spos = nullptr, *omega = nullptr, *lo = nullptr, *hi = nullptr, *verts = nullptr;
+    float* radius = nullptr;
+    int *vstart = nullptr, *nv = nullptr, *scene_of = nullptr, *scene_first = nullptr;
+    int* pairs = nullptr;
+    void* sap_scratch = nullptr;
+    size_t sap_scratch_bytes = 0;
+    int *total = nullptr, *sap_error = nullptr;  // inside sap_scratch
+    float *sep = nullptr, *normal = nullptr;
+    int *flags = nullptr, *iters = nullptr;
+    void* gjk_scratch = nullptr;
+    float4 *pf = nullptr, *pp = nullptr, *F = nullptr, *T = nullptr;
+    int *cnt = nullptr, *fillc = nullptr, *offs = nullptr, *list = nullptr, *err = nullptr;
+    void* scan_tmp = nullptr;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_obj = nullptr;
+    long long steps_done = 0;
+};
+
+namespace {
+
+template <typename T>
+sd_status halloc(T** p, size_t count) {
+    SD_HIP(hipMalloc((void**)p, (count ? count : 1) * sizeof(T)));
+    return SD_OK;
+}
+
+void hb_drop_graph(sd_hull_ctx* c) {
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->graph_obj) (void)hipGraphDestroy(c->graph_obj);
+    c->graph = nullptr;
+    c->graph_obj = nullptr;
+}
+
+void hb_free(sd_hull_ctx* c) {
+    hb_drop_graph(c);
+    void* bufs[] = {c->P,        c->V,        c->Q,      c->L,     c->I0,          c->I1,    c->spos,   c->omega,
+                    c->lo,       c->hi,       c->verts,  c->radius, c->vstart,     c->nv,    c->scene_of,
+                    c->scene_first, c->pairs, c->sap_scratch, c->sep, c->normal,   c->flags, c->iters,
+                    c->gjk_scratch, c->pf,    c->pp,     c->F,     c->T,           c->cnt,   c->fillc,  c->offs,
+                    c->list,     c->err,      c->scan_tmp};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    void** ptrs[] = {(void**)&c->P,     (void**)&c->V,          (void**)&c->Q,           (void**)&c->L,
+                     (void**)&c->I0,    (void**)&c->I1,         (void**)&c->spos,        (void**)&c->omega,
+                     (void**)&c->lo,    (void**)&c->hi,         (void**)&c->verts,       (void**)&c->radius,
+                     (void**)&c->vstart, (void**)&c->nv,        (void**)&c->scene_of,    (void**)&c->scene_first,
+                     (void**)&c->pairs, (void**)&c->sap_scratch, (void**)&c->sep,        (void**)&c->normal,
+                     (void**)&c->flags, (void**)&c->iters,      (void**)&c->gjk_scratch, (void**)&c->pf,
+                     (void**)&c->pp,    (void**)&c->F,          (void**)&c->T,           (void**)&c->cnt,
+                     (void**)&c->fillc, (void**)&c->offs,       (void**)&c->list,        (void**)&c->err,
+                     (void**)&c->scan_tmp};
+    for (void** q : ptrs) *q = nullptr;
+    c->n = 0;
+}
+
+sd_status hb_check_params(const sd_hull_params* p) {
+    SD_REQUIRE(p, "sd_hull: NULL parameters");
+    const sd_contact_params& k = p->contact;
+    SD_REQUIRE(k.kn > 0.0f && k.dt > 0.0f && k.damping_ratio >= 0.0f && k.mu >= 0.0f,
+               "sd_hull: kn and dt must be positive, damping_ratio and mu non-negative");
+    SD_REQUIRE(p->max_iter >= 0 && p->max_iter <= 1024, "sd_hull: max_iter must be in [0, 1024]");
+    SD_REQUIRE(p->pairs_per_body >= 0 && p->pairs_per_body <= 1024, "sd_hull: pairs_per_body must be in [0, 1024]");
+    return SD_OK;
+}
+
+}  // namespace
+
+namespace sd {
+
+// One step of the context, enqueued on s (no host synchronisation).
+sd_status hull_enqueue_step(sd_hull_ctx* c, hipStream_t s) {
+    const int n = c->n, cap = c->capacity;
+    const int mi = c->p.max_iter ? c->p.max_iter : 64;
+    hipLaunchKernelGGL(k_hb_pose, dim3(div_up(n, 256)), dim3(256), 0, s, (const float4*)c->P, (const float4*)c->Q,
+                       (const float4*)c->L, (const float4*)c->I0, (const float4*)c->I1, (const float*)c->radius, n,
+                       c->spos, c->omega);
+    SD_LAUNCH_CHECK();
+    sd_status st = launch_hull_aabbs(c->spos, c->Q, c->vstart, c->nv, c->verts, n, c->lo, c->hi, s);
+    if (st != SD_OK) return st;
+    st = enqueue_sap_batched(c->lo, c->hi, n, c->scene_of, c->scene_first, c->nscenes, c->pairs, cap, &c->total,
+                             &c->sap_error, c->sap_scratch, c->sap_scratch_bytes, s);
+    if (st != SD_OK) return st;
+    st = enqueue_gjk_cold(c->pairs, cap, c->total, c->spos, c->Q, c->vstart, c->nv, c->verts, n, c->n_verts, mi,
+                          0.0f, c->sep, c->normal, c->flags, c->iters, c->gjk_scratch, s);
+    if (st != SD_OK) return st;
+    st = launch_fill_u32((uint32_t*)c->cnt, 0u, n, s);
+    if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->fillc, 0u, n, s);
+    if (st != SD_OK) return st;
+    const HbBodies B{c->P, c->V, c->Q, c->omega, c->radius, c->vstart, c->nv, c->verts};
+    hipLaunchKernelGGL(k_hb_pairs, dim3(div_up(cap, 256)), dim3(256), 0, s, (const int*)c->pairs, cap,
+                       (const int*)c->total, B, (const float*)c->sep, (const float*)c->normal, (const int*)c->flags,
+                       c->c, c->pf, c->pp, c->cnt, c->err);
+    SD_LAUNCH_CHECK();
+    st = exclusive_scan_i32(c->cnt, c->offs, n, nullptr, c->scan_tmp, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL(k_hb_fill, dim3(div_up(cap, 256)), dim3(256), 0, s, (const int*)c->pairs, cap,
+                       (const int*)c->total, (const float4*)c->pf, (const int*)c->offs, c->fillc, c->list);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_hb_gather_integrate, dim3(div_up(n, 256)), dim3(256), 0, s, n, (const int*)c->offs,
+                       (const int*)c->cnt, (const int*)c->list, (const float4*)c->pf, (const float4*)c->pp, c->P,
+                       c->V, c->Q, c->L, (const float4*)c->I0, (const float4*)c->I1, c->c, c->F, c->T);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+// Errors recorded on the device by the steps since the last check: the pair
+// capacity overflowed, or a scene range was malformed.
+sd_status hull_check_errors(sd_hull_ctx* c) {
+    int h[2] = {0, 0};
+    SD_HIP(hipMemcpyAsync(&h[0], c->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipMemcpyAsync(&h[1], c->sap_error, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipStreamSynchronize(c->stream));
+    if (h[1]) return set_error(SD_ERR_ARG, "sd_hull_step: bad scene ranges (bits %d)", h[1]);
+    if (h[0])
+        return set_error(SD_ERR_CAPACITY,
+                         "sd_hull_step: more candidate pairs than the capacity (%d); the state is invalid, upload "
+                         "again with a larger pairs_per_body",
+                         c->capacity);
+    return SD_OK;
+}
+
+}  // namespace sd
+
+extern "C" sd_status sd_hull_create(const sd_hull_params* p, sd_hull_ctx** out) {
+    SD_REQUIRE(out, "sd_hull_create: NULL out");
+    *out = nullptr;
+    sd_status st = hb_check_params(p);
+    if (st != SD_OK) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_error(SD_ERR_NO_DEVICE, "sd_hull_create: no HIP device");
+    sd_hull_ctx* c = new sd_hull_ctx();
+    c->p = *p;
+    c->c = to_dev(p->contact);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_error(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
+    *out = c;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_hull_destroy(sd_hull_ctx* c) {
+    if (!c) return SD_OK;
+    (void)hipStreamSynchronize(c->stream);
+    hb_free(c);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, const float* vel4, const float* quat4,
+                                    const float* angvel4, const float* inertia6, const float* radius,
+                                    const int* vstart, const int* nv, int n_verts, const float* verts4,
+                                    const int* scene_first, int nscenes) {
+    SD_REQUIRE(c, "sd_hull_upload: NULL context");
+    SD_REQUIRE(n >= 1 && n_verts >= 0 && nscenes >= 1, "sd_hull_upload: bad sizes");
+    SD_REQUIRE(pos4 && vel4 && quat4 && angvel4 && inertia6 && radius && vstart && nv && (verts4 || n_verts == 0),
+               "sd_hull_upload: NULL array");
+    SD_REQUIRE(nscenes == 1 || scene_first, "sd_hull_upload: scene_first is required for more than one scene");
+    std::vector<int> sfirst(nscenes + 1), sof(n);
+    if (scene_first) sfirst.assign(scene_first, scene_first + nscenes + 1);
+    else sfirst = {0, n};
+    SD_REQUIRE(sfirst[0] == 0 && sfirst[nscenes] == n, "sd_hull_upload: scene_first must run from 0 to n");
+    for (int k = 0; k < nscenes; ++k) {
+        SD_REQUIRE(sfirst[k + 1] >= sfirst[k], "sd_hull_upload: scene_first must not decrease");
+        SD_REQUIRE(sfirst[k + 1] - sfirst[k] <= 65536, "sd_hull_upload: scene %d has more than 65,536 bodies", k);
+        for (int i = sfirst[k]; i < sfirst[k + 1]; ++i) sof[i] = k;
+    }
+    for (int i = 0; i < n; ++i) {
+        SD_REQUIRE(vstart[i] >= 0 && nv[i] >= 0 && vstart[i] + nv[i] <= n_verts,
+                   "sd_hull_upload: body %d has a bad vertex range", i);
+        SD_REQUIRE(nv[i] > 0 || radius[i] > 0.0f, "sd_hull_upload: body %d has neither vertices nor a radius", i);
+        const float q0 = quat4[4 * (size_t)i], q1 = quat4[4 * (size_t)i + 1], q2 = quat4[4 * (size_t)i + 2],
+                    q3 = quat4[4 * (size_t)i + 3];
+        SD_REQUIRE(fabsf(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3 - 1.0f) < 1e-4f,
+                   "sd_hull_upload: body %d has a quaternion that is not of unit length", i);
+    }
+    std::vector<float4> P(n), V(n), Q(n), L(n), I0(n), I1(n);
+    sd_status st = prepare_rigid("sd_hull_upload", n, pos4, vel4, quat4, angvel4, inertia6, P.data(), V.data(),
+                                 Q.data(), L.data(), I0.data(), I1.data());
+    if (st != SD_OK) return st;
+    (void)hipStreamSynchronize(c->stream);
+    hb_free(c);
+    const int ppb = c->p.pairs_per_body ? c->p.pairs_per_body : 32;
+    const long long capl = std::max(64ll, (long long)ppb * n);
+    SD_REQUIRE(capl < (1ll << 30), "sd_hull_upload: pair capacity %lld too large", capl);
+    const int cap = (int)capl;
+    c->n = n;
+    c->n_verts = n_verts;
+    c->nscenes = nscenes;
+    c->capacity = cap;
+    sd_sap_batched_scratch_bytes(n, nscenes, &c->sap_scratch_bytes);
+#define ALLOC(ptr, cnt) \
+    if (st == SD_OK) st = halloc(&(ptr), (size_t)(cnt))
+    ALLOC(c->P, n);
+    ALLOC(c->V, n);
+    ALLOC(c->Q, n);
+    ALLOC(c->L, n);
+    ALLOC(c->I0, n);
+    ALLOC(c->I1, n);
+    ALLOC(c->spos, n);
+    ALLOC(c->omega, n);
+    ALLOC(c->lo, n);
+    ALLOC(c->hi, n);
+    ALLOC(c->verts, n_verts);
+    ALLOC(c->radius, n);
+    ALLOC(c->vstart, n);
+    ALLOC(c->nv, n);
+    ALLOC(c->scene_of, n);
+    ALLOC(c->scene_first, nscenes + 1);
+    ALLOC(c->pairs, 2 * (size_t)cap);
+    ALLOC(*(char**)&c->sap_scratch, c->sap_scratch_bytes);
+    ALLOC(c->sep, cap);
+    ALLOC(c->normal, 3 * (size_t)cap);
+    ALLOC(c->flags, cap);
+    ALLOC(c->iters, cap);
+    ALLOC(*(char**)&c->gjk_scratch, gjk_cold_scratch_bytes(cap));
+    ALLOC(c->pf, cap);
+    ALLOC(c->pp, cap);
+    ALLOC(c->F, n);
+    ALLOC(c->T, n);
+    ALLOC(c->cnt, n);
+    ALLOC(c->fillc, n);
+    ALLOC(c->offs, n + 1);
+    ALLOC(c->list, 2 * (size_t)cap);
+    ALLOC(c->err, 1);
+    ALLOC(*(char**)&c->scan_tmp, scan_temp_bytes(n));
+#undef ALLOC
+    if (st != SD_OK) {
+        hb_free(c);
+        return st;
+    }
+    hipStream_t s = c->stream;
+    std::vector<float4> vv(std::max(n_verts, 1));
+    for (int k = 0; k < n_verts; ++k)
+        vv[k] = make_float4(verts4[4 * (size_t)k], verts4[4 * (size_t)k + 1], verts4[4 * (size_t)k + 2], 0.0f);
+    SD_HIP(hipMemcpyAsync(c->P, P.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->V, V.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->Q, Q.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->L, L.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->I0, I0.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->I1, I1.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+    if (n_verts) SD_HIP(hipMemcpyAsync(c->verts, vv.data(), n_verts * sizeof(float4), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->radius, radius, n * sizeof(float), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->vstart, vstart, n * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->nv, nv, n * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->scene_of, sof.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->scene_first, sfirst.data(), (nscenes + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    st = launch_fill_u32((uint32_t*)c->err, 0u, 1, s);
+    if (st != SD_OK) return st;
+    SD_HIP(hipStreamSynchronize(s));
+    c->steps_done = 0;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_hull_step(sd_hull_ctx* c, int nsteps) {
+    SD_REQUIRE(c && nsteps >= 0, "sd_hull_step: bad argument");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_hull_step: no bodies uploaded");
+    for (int i = 0; i < nsteps; ++i) {
+        if (!c->p.use_graph) {
+            sd_status st = hull_enqueue_step(c, c->stream);
+            if (st != SD_OK) return st;
+            continue;
+        }
+        if (!c->graph) {
+            SD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            sd_status st = hull_enqueue_step(c, c->stream);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (st != SD_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            SD_HIP(e);
+            hipGraphExec_t ge = nullptr;
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                SD_HIP(e);
+            }
+            c->graph = ge;
+            c->graph_obj = g;
+        }
+        SD_HIP(hipGraphLaunch(c->graph, c->stream));
+    }
+    sd_status st = hull_check_errors(c);
+    c->steps_done += nsteps;
+    return st;
+}
+
+extern "C" sd_status sd_hull_download(sd_hull_ctx* c, float* pos4, float* vel4, float* quat4, float* angmom4) {
+    SD_REQUIRE(c, "sd_hull_download: NULL");
+    if (c->n == 0) return set_error(SD_ERR_STATE, "sd_hull_download: no bodies uploaded");
+    const size_t bytes = (size_t)c->n * sizeof(float4);
+    hipStream_t s = c->stream;
+    if (pos4) SD_HIP(hipMemcpyAsync(pos4, c->P, bytes, hipMemcpyDeviceToHost, s));
+    if (vel4) SD_HIP(hipMemcpyAsync(vel4, c->V, bytes, hipMemcpyDeviceToHost, s));
+    if (quat4) SD_HIP(hipMemcpyAsync(quat4, c->Q, bytes, hipMemcpyDeviceToHost, s));
+    if (angmom4) SD_HIP(hipMemcpyAsync(angmom4, c->L, bytes, hipMemcpyDeviceToHost, s));
+    SD_HIP(hipStreamSynchronize(s));
+    if (pos4)
+        for (int b = 0; b < c->n; ++b) pos4[4 * (size_t)b + 3] = 0.0f;
+    return SD_OK;
+}
+
+extern "C" sd_status sd_hull_contacts(sd_hull_ctx* c, int* pairs, float* force4, float* point4, int capacity,
+                                      int* n_contacts) {
+    SD_REQUIRE(c && n_contacts, "sd_hull_contacts: NULL");
+    if (c->n == 0 || c->steps_done == 0) return set_error(SD_ERR_STATE, "sd_hull_contacts: no step has run");
+    int m = 0;
+    SD_HIP(hipMemcpyAsync(&m, c->total, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipStreamSynchronize(c->stream));
+    m = std::min(m, c->capacity);
+    std::vector<int> pr(2 * (size_t)std::max(m, 1));
+    std::vector<float4> f(std::max(m, 1)), q(std::max(m, 1));
+    if (m) {
+        SD_HIP(hipMemcpyAsync(pr.data(), c->pairs, 2 * (size_t)m * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        SD_HIP(hipMemcpyAsync(f.data(), c->pf, (size_t)m * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        SD_HIP(hipMemcpyAsync(q.data(), c->pp, (size_t)m * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        SD_HIP(hipStreamSynchronize(c->stream));
+    }
+    int k = 0;
+    for (int t = 0; t < m; ++t) {
+        if (!(f[t].w > 0.0f)) continue;
+        if (k < capacity) {
+            if (pairs) {
+                pairs[2 * (size_t)k] = pr[2 * (size_t)t];
+                pairs[2 * (size_t)k + 1] = pr[2 * (size_t)t + 1];
+            }
+            if (force4) memcpy(force4 + 4 * (size_t)k, &f[t], sizeof(float4));
+            if (point4) memcpy(point4 + 4 * (size_t)k, &q[t], sizeof(float4));
+        }
+        ++k;
+    }
+    *n_contacts = k;
+    if (k > capacity) return set_error(SD_ERR_CAPACITY, "sd_hull_contacts: %d contacts", k);
+    return SD_OK;
+}

@@ -179,4 +179,31 @@ sd_status launch_resort_u64(const int* mcount, const uint64_t* mnew, const uint6
 sd_status launch_integrate(float4* pos, float4* vel, float4* omg, const float4* f, const float4* t, int n,
                            const ContactDev& c, hipStream_t s);
 
+// Rigid-body rows from upload arguments (pos4 = (x, y, z, mass), mass 0 =
+// static; quat4 = (w, x, y, z); angvel4 world; inertia6 = (xx, yy, zz, xy, xz,
+// yz) body frame): P = (x, y, z, 1/m), V, Q, L = world angular momentum,
+// I0/I1 = body-frame inverse inertia (xx, yy, zz, xy) / (xz, yz, 0, 0).
+sd_status prepare_rigid(const char* who, int nb, const float* pos4, const float* vel4, const float* quat4,
+                        const float* angvel4, const float* inertia6, float4* P, float4* V, float4* Q, float4* L,
+                        float4* I0, float4* I1);
+
+// ------------------------------------------------------------------ hull narrowphase, enqueued only
+// Hull boxes (pos4 = centre and sphere radius, quat4 = (w, x, y, z)).
+sd_status launch_hull_aabbs(const float4* pos4, const float4* quat4, const int* vstart, const int* nv,
+                            const float4* verts4, int n, float4* lo4, float4* hi4, hipStream_t s);
+// sd_sap_pairs_batched without its host read-back: *total_out points at the
+// device pair count (may exceed capacity; pairs past capacity are not
+// written), *error_out at the device error bits of the scene ranges.
+sd_status enqueue_sap_batched(const float4* lo4, const float4* hi4, int n, const int* scene_of,
+                              const int* scene_first, int nscenes, int* pairs, int capacity, int** total_out,
+                              int** error_out, void* scratch, size_t scratch_bytes, hipStream_t s);
+// Cold GJK/EPA (as sd_gjk with no warm start) of the first min(*dcount,
+// capacity) pairs; launches sized to capacity, scratch of
+// gjk_cold_scratch_bytes(capacity) bytes.
+size_t gjk_cold_scratch_bytes(int capacity);
+sd_status enqueue_gjk_cold(const int* pairs, int capacity, const int* dcount, const float4* pos4,
+                           const float4* quat4, const int* vstart, const int* nv, const float4* verts4, int n_hulls,
+                           int n_verts, int max_iter, float near_margin, float* sep, float* normal3, int* flags,
+                           int* iters, void* scratch, hipStream_t s);
+
 }  // namespace sd

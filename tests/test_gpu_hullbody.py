@@ -1,0 +1,103 @@
+"""Rigid convex bodies on the GPU (sd_hull_*) against the oracle's step, bit
+for bit: tumbling hulls, sphere bodies and the static triangles of the box
+walls in two scenes, graph-replayed and eager steps; the contacts of the last
+step (pairs, forces, points) equal; a pair capacity that is too small is
+reported."""
+import sys
+
+import numpy as np
+import pytest
+
+from stardust_amd import _lib as L
+from stardust_amd import hullbody as HB
+from stardust_amd import hulls as H
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    L.require_gpu()
+
+
+def law(dt=1e-5):
+    return L.ContactParams.make(2e4, 0.3, 0.5, dt, gravity=(0.0, 0.0, -9.81), walls=None, max_contacts=16)
+
+
+def scene(seed, n_hulls, n_spheres, side):
+    h = H.generate_hulls(seed, n_hulls, lo=(0.0, 0.0, 0.0), hi=(side, side, side), v_sigma=0.3)
+    hb = HB.make_bodies(h, seed)
+    rng = np.random.default_rng(seed)
+    r = np.float32(0.006)
+    sp = HB.triangle_bodies(np.zeros((n_spheres, 3, 3), np.float32))
+    sp["pos4"][:, :3] = rng.uniform(r, side - r, (n_spheres, 3)).astype(np.float32)
+    m = np.float32(2500 * 4 / 3 * np.pi * r ** 3)
+    sp["pos4"][:, 3] = m
+    sp["vel4"][:, :3] = rng.normal(0, 0.3, (n_spheres, 3)).astype(np.float32)
+    sp["nv"][:] = 0
+    sp["vstart"][:] = 0
+    sp["verts4"] = np.zeros((0, 4), np.float32)
+    sp["radius"][:] = r
+    sp["inertia6"][:, :3] = np.float32(0.4) * m * r * r
+    walls = HB.box_walls((0.0, 0.0, 0.0), (side, side, side))
+    b, _ = HB.concat([hb, sp, walls])
+    return b
+
+
+def compare(sc, st, what):
+    g = sc.download()
+    for key, arr in (("pos4", st.P), ("vel4", st.V), ("quat4", st.Q), ("angm4", st.L)):
+        w = 4 if key == "quat4" else 3
+        bad = np.nonzero(np.any(g[key][:, :w] != arr[:, :w], axis=1))[0]
+        assert len(bad) == 0, "%s: %s differs at %d bodies, first %d: %s vs %s" % (
+            what, key, len(bad), bad[0], g[key][bad[0]], arr[bad[0]])
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_hull_bodies_bit_exact(oracle, graph):
+    say("hull bodies", "graph" if graph else "eager")
+    a, b2 = scene(0x4B01, 300, 60, 0.12), scene(0x4B02, 200, 100, 0.10)
+    bodies, first = HB.concat([a, b2])
+    c = law()
+    st = oracle.HullBodyState(bodies, first)
+    sc = HB.HullBodyScene(c, use_graph=graph)
+    sc.upload(bodies, first)
+    total = 0
+    touched = 0
+    for k in (1, 4, 15):
+        sc.step(k)
+        touched = st.step(c, nsteps=k, nthreads=8)
+        total += k
+        compare(sc, st, "step %d" % total)
+    assert touched > 50  # hulls, spheres and walls in contact
+    sc.close()
+
+
+def test_hull_contacts_equal_oracle(oracle):
+    say("hull contacts")
+    bodies = scene(0x4B03, 250, 50, 0.11)
+    c = law()
+    st = oracle.HullBodyState(bodies)
+    op, of, oq = st.contacts(c, nthreads=8)
+    sc = HB.HullBodyScene(c)
+    sc.upload(bodies)
+    sc.step(1)
+    gp, gf, gq = sc.contacts()
+    assert len(op) > 40
+    assert np.array_equal(gp, op) and np.array_equal(gf, of) and np.array_equal(gq, oq)
+    sc.close()
+
+
+def test_hull_capacity_overflow_reported():
+    say("hull capacity")
+    bodies = scene(0x4B04, 200, 0, 0.05)  # crowded: many candidate pairs per body
+    sc = HB.HullBodyScene(law(), pairs_per_body=1)
+    sc.upload(bodies)
+    with pytest.raises(L.SdError, match="capacity"):
+        sc.step(1)
+    sc.close()
