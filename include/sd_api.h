@@ -458,6 +458,13 @@ sd_status sd_hull_download(sd_hull_ctx* ctx, float* pos4, float* vel4, float* qu
  * (x, y, z, 1; -force on a) and the contact point (x, y, z, 0). */
 sd_status sd_hull_contacts(sd_hull_ctx* ctx, int* pairs, float* force4, float* point4, int capacity,
                            int* n_contacts);
+/* Steps the bodies of hulls with the scenes of a batch context: every
+ * sd_ctx_step / sd_ctx_profile of batch then advances both, the body step on
+ * a stream forked from and joined to the sphere step inside the same captured
+ * graph (the bodies do not touch the spheres: they are separate scene sets
+ * that share the timestep). Both must have the same dt. NULL detaches; detach
+ * before destroying hulls. The autotuner's trials step the spheres only. */
+sd_status sd_batch_attach_hulls(sd_ctx* batch, sd_hull_ctx* hulls);
 
 typedef struct sd_clump_params {
     sd_contact_params contact;

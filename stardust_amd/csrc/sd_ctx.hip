@@ -136,6 +136,11 @@ struct sd_ctx {
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join = nullptr;
+    // rigid-body scenes stepped with the batch (sd_batch_attach_hulls), on a
+    // stream of their own forked from and joined to the step
+    sd_hull_ctx* hulls = nullptr;
+    hipStream_t hside = nullptr;
+    hipEvent_t hev_fork = nullptr, hev_join = nullptr;
 };
 
 namespace {
@@ -220,6 +225,13 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
     a.ids_o = c->ids[sn];
     sd_status st;
     MARK(0);
+    if (c->hulls) {  // the rigid-body scenes' step runs beside the sphere step
+        SD_HIP(hipEventRecord(c->hev_fork, s));
+        SD_HIP(hipStreamWaitEvent(c->hside, c->hev_fork, 0));
+        st = hull_enqueue_step(c->hulls, c->hside);
+        if (st != SD_OK) return st;
+        SD_HIP(hipEventRecord(c->hev_join, c->hside));
+    }
     if (c->p.key64) {
         // sorted keys: the radix sort (full), or the re-sort from the previous
         // order and the spheres that changed cell, with the positions permuted
@@ -298,6 +310,7 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
         c->cur_keys = (Ka == c->k32[0]) ? 0 : 1;
     }
 #undef MARK
+    if (c->hulls) SD_HIP(hipStreamWaitEvent(s, c->hev_join, 0));
     c->cur_hist = hn;
     c->cur_state = sn;
     return SD_OK;
@@ -437,9 +450,33 @@ extern "C" sd_status sd_ctx_set_contact(sd_ctx* c, const sd_contact_params* cp) 
     return SD_OK;
 }
 
+extern "C" sd_status sd_batch_attach_hulls(sd_ctx* c, sd_hull_ctx* h) {
+    SD_REQUIRE(c, "sd_batch_attach_hulls: NULL context");
+    SD_REQUIRE(c->batch, "sd_batch_attach_hulls: not a batch context");
+    if (h) {
+        if (c->n == 0 || hull_body_count(h) == 0)
+            return set_error(SD_ERR_STATE, "sd_batch_attach_hulls: upload the scenes and the bodies first");
+        SD_REQUIRE(hull_dt(h) == c->p.contact.dt, "sd_batch_attach_hulls: dt differs (%g vs %g)", hull_dt(h),
+                   c->p.contact.dt);
+        if (!c->hside) {
+            SD_HIP(hipStreamCreateWithFlags(&c->hside, hipStreamNonBlocking));
+            SD_HIP(hipEventCreateWithFlags(&c->hev_fork, hipEventDisableTiming));
+            SD_HIP(hipEventCreateWithFlags(&c->hev_join, hipEventDisableTiming));
+        }
+    }
+    (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);  // the step graphs are captured again with (or without) the bodies
+    c->hulls = h;
+    return SD_OK;
+}
+
 extern "C" sd_status sd_ctx_destroy(sd_ctx* c) {
     if (!c) return SD_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->hside) (void)hipStreamSynchronize(c->hside);
+    if (c->hev_fork) (void)hipEventDestroy(c->hev_fork);
+    if (c->hev_join) (void)hipEventDestroy(c->hev_join);
+    if (c->hside) (void)hipStreamDestroy(c->hside);
     release(c);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -703,6 +740,11 @@ extern "C" sd_status sd_ctx_step(sd_ctx* c, int nsteps) {
     SD_HIP(hipMemcpyAsync(&ovf, c->overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SD_HIP(hipStreamSynchronize(c->stream));
     c->steps_done += nsteps;
+    if (c->hulls) {
+        hull_count_steps(c->hulls, nsteps);
+        st = hull_check_errors(c->hulls);
+        if (st != SD_OK) return st;
+    }
     if (ovf > 0)
         return set_error(SD_ERR_CAPACITY, "sd_ctx_step: a sphere had %d contacts, capacity is %d", ovf,
                          c->p.contact.max_contacts);
@@ -749,6 +791,10 @@ extern "C" sd_status sd_ctx_profile(sd_ctx* c, int nsteps, float* stage_ms, int 
     }
     for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
     if (st == SD_OK) c->steps_done += nsteps;
+    if (st == SD_OK && c->hulls) {
+        hull_count_steps(c->hulls, nsteps);
+        st = hull_check_errors(c->hulls);
+    }
     return st;
 }
 
@@ -838,6 +884,9 @@ extern "C" sd_status sd_ctx_autotune(sd_ctx* c, int trial_steps, int* tile_items
         return st;
     }
     const long long steps0 = c->steps_done;
+    // trials step the spheres only: attached bodies are not advanced (nor restored)
+    sd_hull_ctx* const hulls = c->hulls;
+    c->hulls = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float best = 3.4e38f;
     int bi = c->items, bb = c->contact_block;
@@ -889,6 +938,7 @@ extern "C" sd_status sd_ctx_autotune(sd_ctx* c, int trial_steps, int* tile_items
     if (e1) (void)hipEventDestroy(e1);
     drop_graphs(c);  // captured with other launch shapes
     c->steps_done = steps0;
+    c->hulls = hulls;
     if (st != SD_OK) return st;
     if (st2 != SD_OK) return st2;
     if (tile_items) *tile_items = bi;

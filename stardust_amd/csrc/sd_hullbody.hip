@@ -347,6 +347,10 @@ sd_status hull_check_errors(sd_hull_ctx* c) {
     return SD_OK;
 }
 
+int hull_body_count(const sd_hull_ctx* c) { return c->n; }
+float hull_dt(const sd_hull_ctx* c) { return c->p.contact.dt; }
+void hull_count_steps(sd_hull_ctx* c, int nsteps) { c->steps_done += nsteps; }
+
 }  // namespace sd
 
 extern "C" sd_status sd_hull_create(const sd_hull_params* p, sd_hull_ctx** out) {

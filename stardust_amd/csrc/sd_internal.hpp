@@ -187,6 +187,16 @@ sd_status prepare_rigid(const char* who, int nb, const float* pos4, const float*
                         const float* angvel4, const float* inertia6, float4* P, float4* V, float4* Q, float4* L,
                         float4* I0, float4* I1);
 
+// Rigid-body context (sd_hull_ctx, sd_hullbody.hip): one step enqueued on s
+// with no host synchronisation; device errors of the steps since the last
+// check; accessors used by a batch context that steps the bodies with its
+// scenes.
+sd_status hull_enqueue_step(sd_hull_ctx* c, hipStream_t s);
+sd_status hull_check_errors(sd_hull_ctx* c);
+int hull_body_count(const sd_hull_ctx* c);
+float hull_dt(const sd_hull_ctx* c);
+void hull_count_steps(sd_hull_ctx* c, int nsteps);
+
 // ------------------------------------------------------------------ hull narrowphase, enqueued only
 // Hull boxes (pos4 = centre and sphere radius, quat4 = (w, x, y, z)).
 sd_status launch_hull_aabbs(const float4* pos4, const float4* quat4, const int* vstart, const int* nv,

@@ -26,17 +26,23 @@ def make_bodies(h, seed, density=2500.0, w_sigma=5.0):
     n = len(h["pos4"])
     rng = np.random.Generator(np.random.PCG64(seed))
     pos4 = h["pos4"].copy()
-    inertia6 = np.zeros((n, 6), np.float32)
-    verts = h["verts4"].astype(np.float64)
-    for i in range(n):
-        v = verts[h["vstart"][i]:h["vstart"][i] + h["nv"][i], :3]
-        half = 0.5 * (v.max(0) - v.min(0))
-        m = density * 4.0 / 3.0 * np.pi * half[0] * half[1] * half[2]
-        pm = m / len(v)
-        x, y, z = v[:, 0], v[:, 1], v[:, 2]
-        inertia6[i] = [pm * (y * y + z * z).sum(), pm * (x * x + z * z).sum(), pm * (x * x + y * y).sum(),
-                       -pm * (x * y).sum(), -pm * (x * z).sum(), -pm * (y * z).sum()]
-        pos4[i, 3] = m
+    vs, nv = h["vstart"].astype(np.int64), h["nv"].astype(np.int64)
+    owner = np.repeat(np.arange(n), nv)
+    rows = np.concatenate([np.arange(a, a + k) for a, k in zip(vs, nv)]) if n else np.zeros(0, np.int64)
+    v = h["verts4"][rows, :3].astype(np.float64)
+    lo = np.full((n, 3), np.inf)
+    hi = np.full((n, 3), -np.inf)
+    np.minimum.at(lo, owner, v)
+    np.maximum.at(hi, owner, v)
+    half = 0.5 * (hi - lo)
+    m = density * 4.0 / 3.0 * np.pi * half[:, 0] * half[:, 1] * half[:, 2]
+    pm = (m / nv)[owner]
+    x, y, z = v[:, 0], v[:, 1], v[:, 2]
+    terms = np.stack([y * y + z * z, x * x + z * z, x * x + y * y, -x * y, -x * z, -y * z], 1) * pm[:, None]
+    inertia = np.zeros((n, 6))
+    np.add.at(inertia, owner, terms)
+    inertia6 = inertia.astype(np.float32)
+    pos4[:, 3] = m.astype(np.float32)
     angvel4 = np.zeros((n, 4), np.float32)
     angvel4[:, :3] = rng.normal(0.0, w_sigma, (n, 3)).astype(np.float32)
     return {"pos4": pos4, "vel4": h["vel4"].copy(), "quat4": h["quat4"].copy(), "angvel4": angvel4,

@@ -167,3 +167,8 @@ class BatchScene(SphereScene):
 
     def upload(self, *a, **k):
         raise TypeError("BatchScene takes upload_scenes(scenes, x, y, z, radius, mass)")
+
+    def attach_hulls(self, hulls):
+        """Steps the bodies of a hullbody.HullBodyScene with these scenes (None detaches)."""
+        L.call("sd_batch_attach_hulls", self.h, None if hulls is None else hulls.h)
+        self.hulls = hulls

@@ -168,6 +168,36 @@ class BatchWorkload:
         return H.generate_hulls(self.seed + 0x10000 + k, int(self.hull_n[j]), lo=(0.0, 0.0, 0.0),
                                 hi=(float(self.side[k]), float(self.side[k]), float(self.height[k])))
 
+    def hull_bodies(self, j, spacing=0.032):
+        """Rigid bodies of the j-th hull-carrying scene: its hulls (8-32
+        vertices on ellipsoids with semi-axes U[5, 15] mm, seeded velocities
+        and spins) on a cubic lattice of the given spacing over the scene's
+        floor (no initial overlaps), and the twelve static triangles of the
+        walls of a box around them (the scene's footprint, tall enough for the
+        lattice)."""
+        from . import hullbody as HB
+        h = self.generate_hull_scene(j)
+        k = int(self.hull_scenes[j])
+        side = float(self.side[k])
+        m = max(1, int(side / spacing))
+        n = len(h["pos4"])
+        layers = -(-n // (m * m))
+        idx = np.arange(n)
+        h["pos4"][:, 0] = ((idx % m) + 0.5) * spacing
+        h["pos4"][:, 1] = (((idx // m) % m) + 0.5) * spacing
+        h["pos4"][:, 2] = ((idx // (m * m)) + 0.5) * spacing
+        b = HB.make_bodies(h, self.seed + 0x20000 + k)
+        top = max(float(self.height[k]), (layers + 1) * spacing)
+        walls = HB.box_walls((0.0, 0.0, 0.0), (m * spacing, m * spacing, top))
+        out, _ = HB.concat([b, walls])
+        return out
+
+    def hull_set(self):
+        """Bodies of every hull-carrying scene, one scene each, and the scene
+        boundaries."""
+        from . import hullbody as HB
+        return HB.concat([self.hull_bodies(j) for j in range(len(self.hull_scenes))])
+
     def scene(self, k):
         lo = (0.0, 0.0, 0.0)
         hi = (float(self.side[k]), float(self.side[k]), float(self.height[k]))

@@ -313,3 +313,47 @@ def test_batch_checkpoint_resume_and_scene_table_check():
     a.close()
     b.close()
     d.close()
+
+
+def test_batch_steps_attached_hull_scenes():
+    """sd_batch_attach_hulls: the rigid-body scenes step inside the batched
+    step (graph-captured, forked stream); afterwards the spheres equal the
+    batch stepped without them and the bodies equal their own context stepped
+    alone, bit for bit."""
+    say("batch + attached hull scenes")
+    from stardust_amd import hullbody as HB
+    from stardust_amd import hulls as H
+    w = small_batch(nscenes=6, seed=0x5EEDB005)
+    s, scenes = w.generate()
+    v = velocities(len(s["x"]), 5)
+    sets = []
+    for j, side in enumerate((0.1, 0.08)):
+        h = H.generate_hulls(0x4C00 + j, 300, lo=(0.0, 0.0, 0.0), hi=(side, side, side), v_sigma=0.3)
+        sets.append(HB.concat([HB.make_bodies(h, 0x4C10 + j), HB.box_walls((0, 0, 0), (side, side, side))])[0])
+    bodies, first = HB.concat(sets)
+    c = L.ContactParams.make(2e4, 0.3, 0.5, w.dt, gravity=(0.0, 0.0, -9.81), walls=None, max_contacts=16)
+    a = batch_of(w, scenes, s, v)
+    ha = HB.HullBodyScene(c)
+    ha.upload(bodies, first)
+    a.attach_hulls(ha)
+    b = batch_of(w, scenes, s, v)
+    hb = HB.HullBodyScene(c)
+    hb.upload(bodies, first)
+    for k in (1, 6, 13):
+        a.step(k)
+        b.step(k)
+        hb.step(k)
+    assert_same(a.download(), b.download(), "spheres with bodies attached")
+    ga, gb = ha.download(), hb.download()
+    for key in ("pos4", "vel4", "quat4", "angm4"):
+        assert np.array_equal(ga[key], gb[key]), key
+    assert len(ha.contacts()[0]) > 50
+    a.attach_hulls(None)
+    a.step(3)
+    b.step(3)
+    assert_same(a.download(), b.download(), "after detaching")
+    assert np.array_equal(ha.download()["pos4"], ga["pos4"])  # detached bodies no longer move
+    for x in (a, b):
+        x.close()
+    ha.close()
+    hb.close()
