@@ -321,6 +321,7 @@ def test_batch_steps_attached_hull_scenes():
     batch stepped without them and the bodies equal their own context stepped
     alone, bit for bit."""
     say("batch + attached hull scenes")
+    from stardust_amd import _lib as L
     from stardust_amd import hullbody as HB
     from stardust_amd import hulls as H
     w = small_batch(nscenes=6, seed=0x5EEDB005)
