@@ -31,7 +31,9 @@ Further sections ride on the same line (--sections, default
            and the contact kernel's roofline, beside the dilute headline.
   batch  — batch_c5 (512 independent scenes, 7.7M spheres): particle-steps/s of
            the batched graph-replayed step (after the autotuner picks the launch
-           shapes) vs the plain form (every scene alone in its own context).
+           shapes) vs the plain form (every scene alone in its own context); then
+           the 128 hull-carrying scenes' rigid bodies (544k hulls on lattices in
+           walled boxes) attached, stepped inside the same graph.
   gjk    — hulls_c4 (131,072 convex hulls): GPU sweep-and-prune, then GJK/EPA
            pairs/s cold-started ("plain") and with the learned warm start
            (features + BF16 MFMA inference + warm GJK, "learned"), after training
@@ -420,8 +422,9 @@ def batch_section(args):
     prof = b.profile(args.profile_steps)
     per = {k: round(v / args.profile_steps, 5) for k, v in prof.items()}
     contacts = b.contact_count()
-    b.close()
     log("batch: %.3f ms/step -> %.3e particle-steps/s; stages %s" % (1000 * t / args.batch_steps, batched, per))
+    hulls = batch_hulls(args, w, b, t / args.batch_steps) if args.batch_hull_steps > 0 else None
+    b.close()
 
     # plain: every scene alone in its own context (graph-replayed steps), one after the other
     tot_t, tot_work = 0.0, 0
@@ -445,9 +448,55 @@ def batch_section(args):
             "batched_particle_steps_per_s": batched, "batched_ms_per_step": round(1000 * t / args.batch_steps, 4),
             "plain_particle_steps_per_s": plain, "plain_steps_per_scene": args.plain_steps,
             "batched_over_plain": batched / plain if plain else None, "autotune": tune, "stage_ms": per,
-            "contacts_per_sphere": contacts / w.n,
+            "contacts_per_sphere": contacts / w.n, "with_hull_scenes": hulls,
             "form": "batched: one graph-replayed step for all scenes; plain: each scene alone in its own "
                     "graph-replayed context, scenes one after the other"}
+
+
+def batch_hulls(args, w, b, sphere_s):
+    """The hull-carrying scenes of batch_c5 as rigid bodies (hulls on a lattice
+    in walled boxes) attached to the batch: one graph-replayed step advances
+    the spheres and the bodies together (the body step on a forked stream);
+    the bodies' context is also timed alone."""
+    from stardust_amd import _lib as L
+    from stardust_amd import hullbody as HB
+    t0 = time.perf_counter()
+    bodies, first = w.hull_set()
+    n_hulls = int((bodies["nv"] > 3).sum())
+    log("batch: %d hull scenes, %d bodies (%d hulls) generated in %.1f s" % (len(first) - 1, len(bodies["pos4"]),
+                                                                          n_hulls, time.perf_counter() - t0))
+    c = L.ContactParams.make(2e5, 0.3, 0.5, w.dt, gravity=(0.0, 0.0, -9.81), walls=None, max_contacts=16)
+    hs = HB.HullBodyScene(c)
+    hs.upload(bodies, first)
+    hs.step(2)
+    ev = event_pair()
+    L.synchronize(None)
+    t0 = time.perf_counter()
+    hs.step(args.batch_hull_steps)
+    alone = (time.perf_counter() - t0) / args.batch_hull_steps
+    b.attach_hulls(hs)
+    b.step(2)
+    bstream = b.stream()
+    L.synchronize(bstream)
+    t0 = time.perf_counter()
+    L.call("sd_event_record", ev[0], bstream)
+    b.step(args.batch_hull_steps)
+    L.call("sd_event_record", ev[1], bstream)
+    L.synchronize(bstream)
+    wall = time.perf_counter() - t0
+    both = max(wall, elapsed_ms(ev) / 1000.0) / args.batch_hull_steps
+    ncont = len(hs.contacts()[0])
+    b.attach_hulls(None)
+    hs.close()
+    log("batch: with hull scenes %.3f ms/step (spheres alone %.3f, bodies alone %.3f); %d body contacts" % (
+        1000 * both, 1000 * sphere_s, 1000 * alone, ncont))
+    nb = len(bodies["pos4"])
+    return {"hull_scenes": len(first) - 1, "bodies": nb, "hulls": n_hulls, "steps": args.batch_hull_steps,
+            "ms_per_step": round(1000 * both, 4), "spheres_alone_ms_per_step": round(1000 * sphere_s, 4),
+            "bodies_alone_ms_per_step": round(1000 * alone, 4),
+            "particle_steps_per_s": w.n / both, "body_steps_per_s": nb / both,
+            "body_contacts_last_step": ncont,
+            "form": "the bodies' step forked onto its own stream inside the batch's captured step graph"}
 
 
 def calibrate_copy(nbytes=256 << 20, reps=5):
@@ -751,6 +800,8 @@ def main():
     ap.add_argument("--cpu-gjk-pairs", type=int, default=0, help="CPU GJK sample size (0: every SAP pair)")
     ap.add_argument("--clump-warmup", type=int, default=5)
     ap.add_argument("--clump-steps", type=int, default=50)
+    ap.add_argument("--batch-hull-steps", type=int, default=10,
+                    help="steps of batch_c5 with its hull scenes attached (0: skip)")
     ap.add_argument("--cpu-c1-steps", type=int, default=1000)
     ap.add_argument("--cpu-reps", type=int, default=3, help="runs of the all-threads C1 baseline (median)")
     ap.add_argument("--cpu-reps-1t", type=int, default=1, help="runs of the one-thread C1 baseline (median)")

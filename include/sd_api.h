@@ -439,6 +439,10 @@ typedef struct sd_hull_params {
 typedef struct sd_hull_ctx sd_hull_ctx;
 sd_status sd_hull_create(const sd_hull_params* p, sd_hull_ctx** ctx);
 sd_status sd_hull_destroy(sd_hull_ctx* ctx);
+/* New contact law / dt / gravity / iteration limit for the following steps
+ * (state kept; pairs_per_body cannot change after an upload). A context
+ * attached to a batch must keep the batch's dt. */
+sd_status sd_hull_set_params(sd_hull_ctx* ctx, const sd_hull_params* p);
 /* n bodies: pos4 (x, y, z, mass; mass 0 = static), vel4, quat4 (w, x, y, z,
  * unit), angvel4 (world), inertia6 (body frame about the centre: xx, yy, zz,
  * xy, xz, yz), radius (sphere radius where nv == 0), vstart/nv into verts4

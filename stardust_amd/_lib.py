@@ -169,6 +169,7 @@ _SIGS = {
     "sd_hull_create": [_P, _P],
     "sd_batch_attach_hulls": [_P, _P],
     "sd_hull_destroy": [_P],
+    "sd_hull_set_params": [_P, _P],
     "sd_hull_upload": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _I],
     "sd_hull_step": [_P, _I],
     "sd_hull_download": [_P, _P, _P, _P, _P],

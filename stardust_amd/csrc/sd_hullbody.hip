@@ -373,6 +373,19 @@ extern "C" sd_status sd_hull_create(const sd_hull_params* p, sd_hull_ctx** out) 
     return SD_OK;
 }
 
+extern "C" sd_status sd_hull_set_params(sd_hull_ctx* c, const sd_hull_params* p) {
+    SD_REQUIRE(c, "sd_hull_set_params: NULL context");
+    sd_status st = hb_check_params(p);
+    if (st != SD_OK) return st;
+    SD_REQUIRE(c->n == 0 || p->pairs_per_body == c->p.pairs_per_body,
+               "sd_hull_set_params: pairs_per_body cannot change after upload");
+    (void)hipStreamSynchronize(c->stream);
+    hb_drop_graph(c);
+    c->p = *p;
+    c->c = to_dev(p->contact);
+    return SD_OK;
+}
+
 extern "C" sd_status sd_hull_destroy(sd_hull_ctx* c) {
     if (!c) return SD_OK;
     (void)hipStreamSynchronize(c->stream);

@@ -32,6 +32,7 @@ DEMEngine::DEMEngine(engineConfig config, Scalar domain) : engineConfiguration(c
 DEMEngine::~DEMEngine() {
     if (m_ctx) sd_clump_destroy(m_ctx);
     if (m_sctx) sd_ctx_destroy(m_sctx);
+    if (m_hctx) sd_hull_destroy(m_hctx);
 }
 
 void DEMEngine::add(DEMParticle particle) {
@@ -39,7 +40,15 @@ void DEMEngine::add(DEMParticle particle) {
     entity_handler.addEntity(particle);
 }
 
-void DEMEngine::add(DEMMesh mesh) { entity_handler.addEntity(mesh); }
+void DEMEngine::add(DEMMesh mesh) {
+    if (m_uploaded) throw std::runtime_error("DEMEngine::add: meshes cannot be added after transfer()");
+    entity_handler.addEntity(mesh);
+}
+
+void DEMEngine::add(DEMHull hull) {
+    if (m_uploaded) throw std::runtime_error("DEMEngine::add: hulls cannot be added after transfer()");
+    entity_handler.addEntity(hull);
+}
 
 void DEMEngine::prep() { entity_handler.prepareArrays(); }
 
@@ -131,11 +140,14 @@ void DEMEngine::transfer() {
     EntityHandler& h = entity_handler;
     if (!h.prepared) throw std::runtime_error("DEMEngine::transfer: call prep() first");
     const int nb = h.getNumberOfParticles();
-    if (nb == 0) throw std::runtime_error("DEMEngine::transfer: no entities");
+    if (nb + h.getNumberOfHulls() + h.getNumberOfMeshes() == 0)
+        throw std::runtime_error("DEMEngine::transfer: no entities");
     bool single = true;
     for (int b = 0; b < nb && single; ++b) single = h.entity_length[b] == 1;
-    m_sphere_mode = single;
-    if (single) transfer_spheres();
+    m_hull_mode = h.getNumberOfHulls() + h.getNumberOfMeshes() > 0;
+    m_sphere_mode = single && !m_hull_mode;
+    if (m_hull_mode) transfer_bodies();
+    else if (single) transfer_spheres();
     else transfer_clumps();
     m_uploaded = true;
     m_dt = 0.0f;
@@ -149,6 +161,7 @@ void DEMEngine::bind_runtime() {
 }
 
 void DEMEngine::addRuntime(const DEMParticle& e) {
+    if (m_hull_mode) throw std::runtime_error("DEMEngine::addRuntime: not available on the rigid-body path");
     if (!m_uploaded) throw std::runtime_error("DEMEngine::addRuntime: call transfer() first");
     EntityHandler one;
     one.addEntity(e);
@@ -175,6 +188,7 @@ void DEMEngine::addRuntime(const DEMParticle& e) {
 }
 
 void DEMEngine::removeRuntime(int idx) {
+    if (m_hull_mode) throw std::runtime_error("DEMEngine::removeRuntime: not available on the rigid-body path");
     if (!m_uploaded) throw std::runtime_error("DEMEngine::removeRuntime: call transfer() first");
     if (idx < 0 || idx >= entity_handler.getNumberOfParticles())
         throw std::out_of_range("DEMEngine::removeRuntime: no entity " + std::to_string(idx));
@@ -190,6 +204,7 @@ constexpr char ENGINE_MAGIC[8] = {'S', 'D', 'E', 'N', 'G', 'I', 'N', '1'};
 }
 
 void DEMEngine::saveState(const std::string& path) {
+    if (m_hull_mode) throw std::runtime_error("DEMEngine::saveState: not available on the rigid-body path");
     if (!m_uploaded) throw std::runtime_error("DEMEngine::saveState: call transfer() first");
     size_t bytes = 0;
     if (m_sphere_mode) check(sd_ctx_state_bytes(m_sctx, &bytes), "DEMEngine::saveState");
@@ -210,6 +225,7 @@ void DEMEngine::saveState(const std::string& path) {
 }
 
 void DEMEngine::loadState(const std::string& path) {
+    if (m_hull_mode) throw std::runtime_error("DEMEngine::loadState: not available on the rigid-body path");
     if (!m_uploaded) throw std::runtime_error("DEMEngine::loadState: transfer() the same entities first");
     std::ifstream f(path, std::ios::binary);
     if (!f) throw std::runtime_error("DEMEngine::loadState: cannot open " + path);
@@ -246,6 +262,7 @@ void put_be(std::ofstream& f, const void* p) {
 }  // namespace
 
 void DEMEngine::writeVTK(const std::string& path) {
+    if (m_hull_mode) throw std::runtime_error("DEMEngine::writeVTK: not available on the rigid-body path");
     if (!m_uploaded) throw std::runtime_error("DEMEngine::writeVTK: call transfer() first");
     sync();
     EntityHandler& h = entity_handler;
@@ -404,8 +421,103 @@ void DEMEngine::transfer_clumps() {
           "DEMEngine::transfer");
 }
 
+namespace {
+
+sd_hull_params hull_law(float kn, float zeta, float mu, float dt, const float* g) {
+    sd_hull_params p{};
+    p.contact.kn = kn;
+    p.contact.damping_ratio = zeta;
+    p.contact.mu = mu;
+    p.contact.dt = dt;
+    for (int a = 0; a < 3; ++a) p.contact.gravity[a] = g[a];
+    p.contact.max_contacts = 16;
+    p.max_iter = 64;
+    p.pairs_per_body = 32;
+    p.use_graph = 1;
+    return p;
+}
+
+}  // namespace
+
+// Bodies in this order: single-sphere particles (radius diameter / 2, solid
+// sphere inertia), hulls (inertia of equal point masses at the vertices, in
+// double), mesh triangles (static; centre = the corners' mean, corners
+// relative to it); masses at or above the static threshold are uploaded as 0.
+void DEMEngine::transfer_bodies() {
+    EntityHandler& h = entity_handler;
+    const int np = h.getNumberOfParticles();
+    std::vector<float> pos4, vel4, quat4, angvel4, inertia6, radius, verts4;
+    std::vector<int> vstart, nv;
+    auto body = [&](float x, float y, float z, float m, const float4& v, const float4& q, const double I[6], float r,
+                    int vs, int n) {
+        const float mm = m >= m_static_mass ? 0.0f : m;
+        pos4.insert(pos4.end(), {x, y, z, mm});
+        vel4.insert(vel4.end(), {v.x, v.y, v.z, 0.0f});
+        quat4.insert(quat4.end(), {q.x, q.y, q.z, q.w});
+        angvel4.insert(angvel4.end(), {0.0f, 0.0f, 0.0f, 0.0f});
+        for (int k = 0; k < 6; ++k) inertia6.push_back((float)I[k]);
+        radius.push_back(r);
+        vstart.push_back(vs);
+        nv.push_back(n);
+    };
+    const float4 ident = make_float4(1.0f, 0.0f, 0.0f, 0.0f), zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int b = 0; b < np; ++b) {
+        if (h.entity_length[b] != 1)
+            throw std::runtime_error("DEMEngine::transfer: multi-sphere particles cannot share a scene with meshes "
+                                     "or hulls");
+        const int k = h.entity_start[b];
+        const float r = 0.5f * h.p_size[k], m = h.rb_mass[b];
+        const double Is = 0.4 * (double)m * r * r;
+        const double I[6] = {Is, Is, Is, 0.0, 0.0, 0.0};
+        body(h.rb_pos[b].x, h.rb_pos[b].y, h.rb_pos[b].z, m, h.rb_vel[b], ident, I, r, 0, 0);
+    }
+    for (const DEMHull& hl : h.getHulls()) {
+        const int vs = (int)(verts4.size() / 4), n = (int)hl.vertices.size();
+        double I[6] = {0, 0, 0, 0, 0, 0};
+        const double pm = (double)hl.mass / n;
+        for (const float4& v : hl.vertices) {
+            verts4.insert(verts4.end(), {v.x, v.y, v.z, 0.0f});
+            const double x = v.x, y = v.y, z = v.z;
+            I[0] += pm * (y * y + z * z);
+            I[1] += pm * (x * x + z * z);
+            I[2] += pm * (x * x + y * y);
+            I[3] -= pm * x * y;
+            I[4] -= pm * x * z;
+            I[5] -= pm * y * z;
+        }
+        body(hl.position.x, hl.position.y, hl.position.z, hl.mass, hl.velocity, hl.quat, I, 0.0f, vs, n);
+    }
+    const double I0[6] = {0, 0, 0, 0, 0, 0};
+    for (const DEMMesh& mesh : h.getMeshes()) {
+        for (int t = 0; t < mesh.n_triangles; ++t) {
+            float c[9];
+            mesh.worldTriangle(t, c);
+            const float cx = (float)(((double)c[0] + c[3] + c[6]) / 3.0), cy = (float)(((double)c[1] + c[4] + c[7]) / 3.0),
+                        cz = (float)(((double)c[2] + c[5] + c[8]) / 3.0);
+            const int vs = (int)(verts4.size() / 4);
+            for (int k = 0; k < 3; ++k) verts4.insert(verts4.end(), {c[3 * k] - cx, c[3 * k + 1] - cy, c[3 * k + 2] - cz, 0.0f});
+            body(cx, cy, cz, 0.0f, zero, ident, I0, 0.0f, vs, 3);
+        }
+    }
+    const int n = (int)radius.size();
+    if (!m_hctx) {
+        const sd_hull_params p = hull_law(m_kn, m_zeta, m_mu, 1e-5f, m_g);  // dt of the first update replaces it
+        check(sd_hull_create(&p, &m_hctx), "DEMEngine::transfer");
+    }
+    check(sd_hull_upload(m_hctx, n, pos4.data(), vel4.data(), quat4.data(), angvel4.data(), inertia6.data(),
+                         radius.data(), vstart.data(), nv.data(), (int)(verts4.size() / 4),
+                         verts4.empty() ? nullptr : verts4.data(), nullptr, 1),
+          "DEMEngine::transfer");
+}
+
 void DEMEngine::ensure_context(float dt) {
     if (dt == m_dt) return;
+    if (m_hull_mode) {
+        const sd_hull_params p = hull_law(m_kn, m_zeta, m_mu, dt, m_g);
+        check(sd_hull_set_params(m_hctx, &p), "DEMEngine::update");
+        m_dt = dt;
+        return;
+    }
     if (m_sphere_mode) {
         const sd_contact_params cp = sphere_law(m_kn, m_zeta, m_mu, m_kt_ratio, dt, m_g, m_sgrid);
         check(sd_ctx_set_contact(m_sctx, &cp), "DEMEngine::update");
@@ -428,7 +540,8 @@ void DEMEngine::update(float dt) {
     if (!m_uploaded) throw std::runtime_error("DEMEngine::update: call prep() and transfer() first");
     if (!(dt > 0.0f)) throw std::invalid_argument("DEMEngine::update: dt must be positive");
     ensure_context(dt);
-    if (m_sphere_mode) check(sd_ctx_step(m_sctx, 1), "DEMEngine::update");
+    if (m_hull_mode) check(sd_hull_step(m_hctx, 1), "DEMEngine::update");
+    else if (m_sphere_mode) check(sd_ctx_step(m_sctx, 1), "DEMEngine::update");
     else check(sd_clump_step(m_ctx, 1), "DEMEngine::update");
     ++m_steps;
     is_first_step = false;
@@ -438,6 +551,33 @@ void DEMEngine::sync() {
     if (!m_uploaded) throw std::runtime_error("DEMEngine::sync: nothing transferred");
     EntityHandler& h = entity_handler;
     const int nb = h.getNumberOfParticles(), ns = h.getNumberOfSpheres();
+    if (m_hull_mode) {
+        const int nh = h.getNumberOfHulls();
+        int n = nb + nh;
+        for (const DEMMesh& mesh : h.getMeshes()) n += mesh.n_triangles;
+        std::vector<float> pos4(4 * (size_t)n), vel4(4 * (size_t)n), quat4(4 * (size_t)n), angm4(4 * (size_t)n);
+        check(sd_hull_download(m_hctx, pos4.data(), vel4.data(), quat4.data(), angm4.data()), "DEMEngine::sync");
+        auto f4 = [](const std::vector<float>& a, int i) {
+            return make_float4(a[4 * (size_t)i], a[4 * (size_t)i + 1], a[4 * (size_t)i + 2], a[4 * (size_t)i + 3]);
+        };
+        for (int b = 0; b < nb; ++b) {
+            const int k = h.entity_start[b];
+            h.rb_pos[b] = f4(pos4, b);
+            h.rb_vel[b] = f4(vel4, b);
+            h.rb_quat[b] = f4(quat4, b);
+            h.rb_angmom[b] = f4(angm4, b);
+            h.rb_linmom[b] = h.rb_vel[b] * h.rb_mass[b];
+            h.p_pos[k] = h.rb_pos[b];
+        }
+        for (int j = 0; j < nh; ++j) {
+            DEMHull& hl = h.getHulls()[j];
+            hl.position = f4(pos4, nb + j);
+            hl.velocity = f4(vel4, nb + j);
+            hl.quat = f4(quat4, nb + j);
+            hl.angular_momentum = f4(angm4, nb + j);
+        }
+        return;
+    }
     if (m_sphere_mode) {
         std::vector<float> x(nb), y(nb), z(nb), vx(nb), vy(nb), vz(nb), wx(nb), wy(nb), wz(nb);
         check(sd_ctx_download_spheres(m_sctx, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),

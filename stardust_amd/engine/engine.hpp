@@ -10,7 +10,14 @@
 //     engine.transfer();
 //     for (int i = 0; i < 300; ++i) engine.update(0.0005f);
 //
-// transfer() picks one of two step paths of libstardust_hip.so:
+// transfer() picks one of three step paths of libstardust_hip.so:
+//  - any DEMMesh or DEMHull added: the rigid-body path (sd_hull_*): every
+//    single-sphere DEMParticle is a sphere body, every DEMHull a convex body,
+//    every mesh triangle a static TRIANGLE body; sweep-and-prune, GJK/EPA
+//    contacts (normal spring-dashpot plus Coulomb-capped viscous friction),
+//    the semi-implicit integration; at most 65,536 bodies; multi-sphere
+//    particles, runtime edits, checkpoints and VTK output are not available
+//    on this path;
 //  - every entity a single sphere (the reference driver's 1,000,000-sphere
 //    scene, testDEM.cpp:80-128): a sphere context (sd_ctx_*): hash grid of
 //    cell = the largest diameter, packed 64-bit keys re-sorted every step,
@@ -51,6 +58,7 @@
 
 struct sd_clump_ctx;
 struct sd_ctx;
+struct sd_hull_ctx;
 
 namespace STARDUST {
 
@@ -70,7 +78,8 @@ public:
     DEMEngine& operator=(const DEMEngine&) = delete;
 
     void add(DEMParticle particle);
-    void add(DEMMesh mesh);  // throws: triangle meshes are not supported
+    void add(DEMMesh mesh);  // static triangle walls (rigid-body path)
+    void add(DEMHull hull);  // convex hull body (rigid-body path)
     void prep();
     void transfer();
     void update(float dt);
@@ -98,6 +107,7 @@ public:
         float wall_lo[3], wall_hi[3];
     };
     bool sphereMode() const { return m_sphere_mode; }
+    bool hullMode() const { return m_hull_mode; }
     const SphereGrid& sphereGrid() const { return m_sgrid; }
     long long stepsDone() const { return m_steps; }
 
@@ -132,6 +142,7 @@ private:
     void ensure_context(float dt);
     void transfer_spheres();
     void transfer_clumps();
+    void transfer_bodies();
     void bind_runtime();
 
     engineConfig engineConfiguration;
@@ -139,7 +150,9 @@ private:
     EntityHandler entity_handler;
     sd_clump_ctx* m_ctx = nullptr;
     sd_ctx* m_sctx = nullptr;
+    sd_hull_ctx* m_hctx = nullptr;
     bool m_sphere_mode = false;
+    bool m_hull_mode = false;
     SphereGrid m_sgrid{};
     bool m_uploaded = false;
     float m_dt = 0.0f;

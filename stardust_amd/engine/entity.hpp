@@ -47,14 +47,58 @@ private:
     std::vector<DEMSphere> particles;
 };
 
-// Triangle-mesh walls (entity.hpp:64-93, mesh_loader.cpp) are not part of this
-// implementation: the constructor throws std::runtime_error.
+// A triangle mesh (entity.hpp:64-93): the triangles of an STL file (binary
+// or ASCII), placed at position and rotated by quaternion (stored as the
+// reference's float4 quaternions: x = w, y = i, z = j, w = k). Its triangles
+// are static walls: DEMEngine steps scenes with meshes on the rigid-body path,
+// where spheres and hulls collide with each triangle (GJK/EPA with the
+// TRIANGLE shape). Members as in the reference: vertices (unique, mesh frame),
+// indicies (3 per triangle), normals (per triangle, from the file), and the
+// bounding sphere of every triangle (positions, radii; computeBoundingSpheres).
 class DEMMesh {
 public:
     DEMMesh(const char* filename, float4 position, float4 quaternion);
+
+    // Per triangle: the smallest sphere through or around its corners - the
+    // longest edge as diameter for a right or obtuse triangle, else the
+    // circumscribed sphere (entity.cpp:75-120; that code's circumcentre
+    // expression is not the circumcentre, here the circumcentre is exact).
+    void computeBoundingSpheres();
+    // Corners of triangle t in world space (the mesh pose applied).
+    void worldTriangle(int t, float out[9]) const;
+
     int n_triangles = 0;
     float4 position;
     float4 quat;
+
+    std::vector<float4> positions;  // bounding sphere centres (mesh frame)
+    std::vector<float> radii;
+
+    std::vector<float4> vertices;  // unique vertices, lexicographic order
+    std::vector<float4> normals;   // one per triangle
+    std::vector<int> indicies;     // 3 per triangle
+};
+
+// A convex hull (primitives.hpp:37-57, the sandbox's Hull with POLYHEDRA
+// type; it has no entity in the reference's engine): vertices in the body
+// frame about its centre, mass (>= the static threshold: static), and the
+// pose. Inertia: equal point masses at the vertices. Stepped on the
+// rigid-body path (GJK/EPA contacts).
+class DEMHull {
+public:
+    DEMHull(unsigned int id, const std::vector<float4>& vertices, Scalar mass, float4 position, float4 velocity,
+            float4 quaternion);
+    unsigned int getId() const { return m_id; }
+
+    std::vector<float4> vertices;
+    Scalar mass;
+    float4 position;
+    float4 velocity;
+    float4 quat;             // (w, x, y, z) in (x, y, z, w)
+    float4 angular_momentum; // world, after DEMEngine::sync
+
+private:
+    unsigned int m_id;
 };
 
 }  // namespace STARDUST

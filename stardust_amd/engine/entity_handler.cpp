@@ -9,7 +9,10 @@ namespace STARDUST {
 EntityHandler& EntityHandler::operator=(const EntityHandler& o) {
     if (this == &o) return *this;
     particles = o.particles;
+    meshes = o.meshes;
+    hulls = o.hulls;
     n_spheres = o.n_spheres;
+    n_triangles = o.n_triangles;
     prepared = o.prepared;
     entity_start = o.entity_start;
     entity_length = o.entity_length;
@@ -43,9 +46,12 @@ void EntityHandler::addEntity(const DEMParticle& particle) {
     prepared = false;
 }
 
-void EntityHandler::addEntity(const DEMMesh&) {
-    throw std::runtime_error("EntityHandler::addEntity: triangle meshes are not supported");
+void EntityHandler::addEntity(const DEMMesh& mesh) {
+    meshes.push_back(mesh);
+    n_triangles += mesh.n_triangles;
 }
+
+void EntityHandler::addEntity(const DEMHull& hull) { hulls.push_back(hull); }
 
 void EntityHandler::removeEntity(int type, int idx) {
     if (type != 0) throw std::invalid_argument("EntityHandler::removeEntity: only particles (type 0) are held");

@@ -26,11 +26,16 @@ public:
 
     int getNumberOfSpheres() const { return n_spheres; }
     int getNumberOfParticles() const { return (int)particles.size(); }
-    int getNumberOfMeshes() const { return 0; }
-    int getNumberOfPrimitives() const { return n_spheres; }
+    int getNumberOfMeshes() const { return (int)meshes.size(); }
+    int getNumberOfHulls() const { return (int)hulls.size(); }
+    // spheres plus mesh triangles, as the reference counts primitives
+    int getNumberOfPrimitives() const { return n_spheres + n_triangles; }
 
     void addEntity(const DEMParticle& particle);
-    void addEntity(const DEMMesh& mesh);  // meshes are not supported: throws
+    void addEntity(const DEMMesh& mesh);
+    void addEntity(const DEMHull& hull);
+    const std::vector<DEMMesh>& getMeshes() const { return meshes; }
+    std::vector<DEMHull>& getHulls() { return hulls; }
     // Before transfer(): drops entity idx of the given type (0 = particle) from
     // the list (call prepareArrays again). The reference's is a stub
     // (entity_handler.cpp:118).
@@ -77,7 +82,10 @@ private:
     std::function<void(int)> runtime_remove;
 
     std::vector<DEMParticle> particles;
+    std::vector<DEMMesh> meshes;
+    std::vector<DEMHull> hulls;
     int n_spheres = 0;
+    int n_triangles = 0;
     bool prepared = false;
 
     std::vector<int> entity_start, entity_length, particle_idx, particle_sorted_idx, particle_to_rigid;

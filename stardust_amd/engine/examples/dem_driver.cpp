@@ -27,6 +27,13 @@
 //   sd_dem_driver edit-clumps <steps> <dt>  the default "run" bodies plus a fourth,
 //                                           with body 2 removed and a fifth body
 //                                           added at runtime (steps between)
+//   sd_dem_driver mesh-info <file.stl>      a DEMMesh at the origin: its vertices,
+//                                           indices, normals and bounding spheres
+//   sd_dem_driver mesh-drop <file.stl> <steps> <dt>
+//                                           the mesh as static walls, spheres and a
+//                                           cube hull dropped onto its top face
+//                                           (rigid-body path); prints the inputs
+//                                           and the final states
 //   sd_dem_driver checkpoint <steps> <dt> <file> [vtk]
 //                                           the default "run" bodies: <steps> steps,
 //                                           saveState, <steps> steps (final A); a
@@ -38,6 +45,7 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 
@@ -145,6 +153,56 @@ int main(int argc, char** argv) {
             EntityHandler& h = engine.getEntityHandler();
             const int nb = h.getNumberOfParticles();
             std::printf("\"steps\": %lld,\n", engine.stepsDone());
+            print_f4("final_position", h.h_rigid_body_position_ptr, nb);
+            print_f4("final_velocity", h.h_rigid_body_velocity_ptr, nb);
+            print_f4("final_quaternion", h.h_rigid_body_quaternion_ptr, nb);
+            print_f4("final_angular_momentum", h.h_rigid_body_angular_momentum_ptr, nb, true);
+            std::printf("}\n");
+            return 0;
+        }
+        if (mode == "mesh-info" && argc >= 3) {
+            const DEMMesh mesh(argv[2], make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(1.0f, 0.0f, 0.0f, 0.0f));
+            std::printf("{\n\"n_triangles\": %d,\n", mesh.n_triangles);
+            print_f4("vertices", mesh.vertices.data(), (int)mesh.vertices.size());
+            print_f4("normals", mesh.normals.data(), (int)mesh.normals.size());
+            print_i("indices", mesh.indicies.data(), (int)mesh.indicies.size());
+            print_f("radii", mesh.radii.data(), (int)mesh.radii.size());
+            print_f4("centres", mesh.positions.data(), (int)mesh.positions.size(), true);
+            std::printf("}\n");
+            return 0;
+        }
+        if (mode == "mesh-drop" && argc >= 5) {
+            const int steps = std::atoi(argv[3]);
+            const float dt = (float)std::atof(argv[4]);
+            DEMEngine engine(config, 0.0f);
+            // the box of the reference's test mesh: x in [-33, -28], y in [16, 21], z in [0, 1]
+            engine.add(DEMMesh(argv[2], make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(1.0f, 0.0f, 0.0f, 0.0f)));
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j)
+                    engine.add(DEMParticle((unsigned)(4 * i + j), 1, 0.5f, 0.5f, 1.0f,
+                                           make_float4(-32.0f + 1.1f * i, 17.0f + 1.1f * j, 1.4f + 0.05f * (i + j), 0.0f),
+                                           make_float4(0.1f * i, -0.1f * j, -1.0f, 0.0f)));
+            std::vector<float4> cube;
+            for (int k = 0; k < 8; ++k)
+                cube.push_back(make_float4((k & 1) ? 0.3f : -0.3f, (k & 2) ? 0.3f : -0.3f, (k & 4) ? 0.3f : -0.3f, 0.0f));
+            engine.add(DEMHull(100, cube, 2.0f, make_float4(-30.5f, 18.5f, 2.5f, 0.0f), make_float4(0.0f, 0.0f, -2.0f, 0.0f),
+                               make_float4(0.9659258f, 0.2588190f, 0.0f, 0.0f)));
+            engine.setContactLaw(2e4f, 0.3f, 0.5f, 2.0f / 7.0f);
+            engine.prep();
+            std::printf("{\n");
+            print_pack(engine.getEntityHandler());
+            engine.transfer();
+            std::printf("\"hull_mode\": %s,\n", engine.hullMode() ? "true" : "false");
+            for (int i = 0; i < steps; ++i) engine.update(dt);
+            engine.sync();
+            EntityHandler& h = engine.getEntityHandler();
+            const int nb = h.getNumberOfParticles();
+            std::printf("\"steps\": %lld,\n", engine.stepsDone());
+            const DEMHull& hl = h.getHulls()[0];
+            print_f4("hull_final", &hl.position, 1);
+            print_f4("hull_final_velocity", &hl.velocity, 1);
+            print_f4("hull_final_quaternion", &hl.quat, 1);
+            print_f4("hull_final_angular_momentum", &hl.angular_momentum, 1);
             print_f4("final_position", h.h_rigid_body_position_ptr, nb);
             print_f4("final_velocity", h.h_rigid_body_velocity_ptr, nb);
             print_f4("final_quaternion", h.h_rigid_body_quaternion_ptr, nb);

@@ -293,3 +293,148 @@ def test_engine_checkpoint_resume_and_vtk(tmp_path):
     assert np.array_equal(v["body"], owner)
     assert np.array_equal(v["verts"][:, 1], np.arange(len(sp)))
     assert np.array_equal(v["velocity"], np.array(d["a_velocity"], np.float32)[owner, :3])
+
+
+def stl_triangles(path):
+    """Corners and normals of a binary STL, read with numpy."""
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw, "<u4", 1, 80)[0])
+    rec = np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")])
+    t = np.frombuffer(raw, rec, n, 84)
+    return t["v"].copy(), t["n"].copy()
+
+
+MESH = os.path.join(ROOT, "tests", "golden", "test_mesh.stl")
+
+
+def test_mesh_loader_kat():
+    """DEMMesh on the reference's test mesh: unique corners in lexicographic
+    order, indices that give back every corner of the file, the file's
+    normals; bounding spheres of the (right) triangles: the hypotenuse's
+    midpoint and half its length."""
+    d = drive("mesh-info", MESH)
+    tri, nrm = stl_triangles(MESH)
+    assert d["n_triangles"] == 12
+    v = np.array(d["vertices"], np.float32)[:, :3]
+    want = np.unique(tri.reshape(-1, 3), axis=0)  # lexicographic
+    assert np.array_equal(v, want)
+    idx = np.array(d["indices"]).reshape(-1, 3)
+    assert np.array_equal(v[idx], tri)
+    assert np.array_equal(np.array(d["normals"], np.float32)[:, :3], nrm)
+    for t in range(12):
+        p = tri[t].astype(np.float64)
+        e = [np.linalg.norm(p[(k + 1) % 3] - p[(k + 2) % 3]) for k in range(3)]
+        k = int(np.argmax(e))
+        mid = 0.5 * (p[(k + 1) % 3] + p[(k + 2) % 3])
+        assert np.allclose(np.array(d["centres"][t][:3]), mid, atol=1e-6)
+        assert abs(d["radii"][t] - 0.5 * e[k]) < 1e-5
+
+
+def test_mesh_loader_ascii_and_acute_circumsphere(tmp_path):
+    """An ASCII STL of a tetrahedron with acute faces: the same vertices and
+    indices as its binary twin; every bounding sphere passes through the
+    three corners (the circumsphere)."""
+    P = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.45, 0.9, 0.05], [0.5, 0.35, 0.8]], np.float32)
+    faces = [(0, 2, 1), (0, 1, 3), (1, 2, 3), (0, 3, 2)]
+    lines = ["solid tet"]
+    for f in faces:
+        a, b, c = P[list(f)]
+        n = np.cross(b - a, c - a)
+        n = n / np.linalg.norm(n)
+        lines.append("facet normal %.9g %.9g %.9g" % tuple(n))
+        lines.append("outer loop")
+        for q in (a, b, c):
+            lines.append("vertex %.9g %.9g %.9g" % tuple(q))
+        lines += ["endloop", "endfacet"]
+    lines.append("endsolid tet")
+    asc = tmp_path / "tet_ascii.stl"
+    asc.write_text("\n".join(lines) + "\n")
+    rec = np.zeros(4, np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]))
+    for t, f in enumerate(faces):
+        rec["v"][t] = P[list(f)]
+    binf = tmp_path / "tet_binary.stl"
+    binf.write_bytes(b"\0" * 80 + np.array([4], "<u4").tobytes() + rec.tobytes())
+    da, db = drive("mesh-info", asc), drive("mesh-info", binf)
+    assert da["n_triangles"] == db["n_triangles"] == 4
+    assert da["vertices"] == db["vertices"] and da["indices"] == db["indices"]
+    v = np.array(da["vertices"], np.float64)[:, :3]
+    idx = np.array(da["indices"]).reshape(-1, 3)
+    for t in range(4):
+        c, r = np.array(da["centres"][t][:3]), da["radii"][t]
+        dist = np.linalg.norm(v[idx[t]] - c, axis=1)
+        assert np.allclose(dist, r, rtol=1e-6), (dist, r)
+
+
+def mesh_drop_bodies(d):
+    """The bodies DEMEngine::transfer builds for mesh-drop, restated: the
+    particles (from the packed arrays), the cube hull, the mesh triangles."""
+    from stardust_amd import hullbody as HB
+    pos = np.array(d["rigid_body_position"], np.float32)
+    vel = np.array(d["rigid_body_velocity"], np.float32)
+    mass = np.array(d["rigid_body_mass"], np.float32)
+    size = np.array(d["particle_size"], np.float32)
+    n = len(pos)
+    sp = HB.triangle_bodies(np.zeros((n, 3, 3), np.float32))
+    sp["pos4"][:, :3] = pos[:, :3]
+    sp["pos4"][:, 3] = np.where(mass >= 10.0, 0.0, mass)
+    sp["vel4"][:, :3] = vel[:, :3]
+    r = (np.float32(0.5) * size).astype(np.float32)
+    sp["radius"][:] = r
+    for b in range(n):
+        Is = np.float32(0.4 * float(mass[b]) * float(r[b]) * float(r[b]))
+        sp["inertia6"][b, :3] = Is
+    sp["nv"][:] = 0
+    sp["vstart"][:] = 0
+    sp["verts4"] = np.zeros((0, 4), np.float32)
+    cube = np.array([[0.3 if k & 1 else -0.3, 0.3 if k & 2 else -0.3, 0.3 if k & 4 else -0.3] for k in range(8)],
+                    np.float32)
+    hb = HB.triangle_bodies(np.zeros((1, 3, 3), np.float32))
+    hb["verts4"] = np.concatenate([cube, np.zeros((8, 1), np.float32)], 1)
+    hb["nv"][:] = 8
+    hb["pos4"][0] = [-30.5, 18.5, 2.5, 2.0]
+    hb["vel4"][0, :3] = [0.0, 0.0, -2.0]
+    hb["quat4"][0] = [0.9659258, 0.2588190, 0.0, 0.0]
+    pm = 2.0 / 8
+    I = [0.0] * 6
+    for x, y, z in cube.astype(np.float64):
+        I[0] += pm * (y * y + z * z)
+        I[1] += pm * (x * x + z * z)
+        I[2] += pm * (x * x + y * y)
+        I[3] -= pm * x * y
+        I[4] -= pm * x * z
+        I[5] -= pm * y * z
+    hb["inertia6"][0] = np.array(I, np.float32)
+    tri, _ = stl_triangles(MESH)
+    c = (tri.astype(np.float64).sum(1) / 3.0).astype(np.float32)
+    tb = HB.triangle_bodies(tri)
+    tb["pos4"][:, :3] = c
+    tb["verts4"][:, :3] = (tri - c[:, None, :]).reshape(-1, 3)
+    bodies, _ = HB.concat([sp, hb, tb])
+    return bodies
+
+
+@pytest.mark.gpu
+def test_engine_mesh_walls_and_hull_equal_oracle(oracle):
+    """DEMMesh + DEMHull + single spheres through DEMEngine (the rigid-body
+    path): spheres and a cube dropped onto the top face of the reference's
+    test mesh equal the oracle's rigid-body steps bit for bit, and come to
+    rest on it."""
+    print("[gpu-case] engine mesh walls", flush=True)
+    steps, dt = 3000, 2e-4
+    d = drive("mesh-drop", MESH, steps, dt)
+    assert d["hull_mode"] is True and d["steps"] == steps
+    bodies = mesh_drop_bodies(d)
+    st = oracle.HullBodyState(bodies)
+    c = L.ContactParams.make(2e4, 0.3, 0.5, np.float32(dt), gravity=(0.0, 0.0, -9.81), walls=None, max_contacts=16)
+    st.step(c, nsteps=steps, nthreads=8)
+    n = len(d["final_position"])
+    assert np.array_equal(np.array(d["final_position"], np.float32)[:, :3], st.P[:n, :3])
+    assert np.array_equal(np.array(d["final_velocity"], np.float32)[:, :3], st.V[:n, :3])
+    assert np.array_equal(np.array(d["hull_final"], np.float32)[0, :3], st.P[n, :3])
+    assert np.array_equal(np.array(d["hull_final_quaternion"], np.float32)[0], st.Q[n])
+    # the spheres came to rest vertically on the top face (z = 1; they may
+    # still roll: the law has no rolling resistance)
+    z = st.P[:n, 2]
+    on_top = np.abs(z - (1.0 + 0.25)) < 0.01
+    assert on_top.sum() >= 8, z
+    assert np.abs(st.V[:n][on_top, 2]).max() < 0.05
