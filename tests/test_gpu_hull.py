@@ -209,3 +209,40 @@ def test_sap_batched_many_partners_match_oracle(oracle):
         rows.append(want + off)
         off += len(lo)
     assert np.array_equal(got, np.concatenate(rows))
+
+
+def test_gjk_triangles_and_spheres_parity(oracle):
+    """TRIANGLE (three vertices) and SPHERE (no vertices) shapes beside hulls:
+    boxes, sweep-and-prune pairs and GJK/EPA results bit-exact to the oracle."""
+    say("hulls + triangles + spheres")
+    rng = np.random.default_rng(23)
+    h = H.generate_hulls(0x7A1, 3000, lo=(0.0, 0.0, 0.0), hi=(0.4, 0.4, 0.4))
+    nt, ns = 2000, 1000
+    tri = rng.normal(0, 0.01, (nt, 3, 3)).astype(np.float32)
+    tc = rng.uniform(0, 0.4, (nt, 3)).astype(np.float32)
+    pos4 = np.zeros((nt + ns, 4), np.float32)
+    pos4[:nt, :3] = tc
+    pos4[nt:, :3] = rng.uniform(0, 0.4, (ns, 3)).astype(np.float32)
+    pos4[nt:, 3] = rng.uniform(0.003, 0.012, ns).astype(np.float32)
+    quat4 = rng.normal(0, 1, (nt + ns, 4)).astype(np.float32)
+    quat4 /= np.linalg.norm(quat4, axis=1, keepdims=True)
+    v0 = len(h["verts4"])
+    tv = np.zeros((3 * nt, 4), np.float32)
+    tv[:, :3] = tri.reshape(-1, 3)
+    g = {"pos4": np.concatenate([h["pos4"], pos4]), "quat4": np.concatenate([h["quat4"], quat4]),
+         "vstart": np.concatenate([h["vstart"], v0 + 3 * np.arange(nt), np.zeros(ns)]).astype(np.int32),
+         "nv": np.concatenate([h["nv"], np.full(nt, 3), np.zeros(ns)]).astype(np.int32),
+         "verts4": np.concatenate([h["verts4"], tv])}
+    lo, hi, pairs, axis, res = run_all(g)
+    elo, ehi = oracle.hull_aabbs(g)
+    assert np.array_equal(lo, elo) and np.array_equal(hi, ehi)
+    epairs, eaxis = oracle.sap_pairs(elo, ehi)
+    assert np.array_equal(pairs, epairs)
+    n3 = (g["nv"][pairs[:, 0]] == 3) | (g["nv"][pairs[:, 1]] == 3)
+    assert n3.sum() > 500
+    exp = oracle.gjk(g, epairs, nthreads=8)
+    for k in ("flags", "iters"):
+        assert np.array_equal(res[k], exp[k]), k
+    assert np.array_equal(res["sep"].view(np.uint32), exp["sep"].view(np.uint32))
+    assert np.array_equal(res["normal"].view(np.uint32), exp["normal"].view(np.uint32))
+    assert ((res["flags"][n3] & 1) > 0).sum() > 50  # triangles hit

@@ -246,3 +246,62 @@ def test_certificate_known_answers(oracle):
     assert cold["flags"][0] & 1 and w["flags"][0] & 32
     assert w["sep"][0] == cold["sep"][0]
     assert abs(cold["sep"][0] + 0.1 / math.sqrt(3)) < 1e-6
+
+
+def closest_on_triangle(p, a, b, c):
+    """Closest point of triangle abc to p (Ericson, Real-Time Collision
+    Detection, 5.1.5: Voronoi regions of the vertices, edges and face), float64."""
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = ab @ ap, ac @ ap
+    if d1 <= 0 and d2 <= 0:
+        return a
+    bp = p - b
+    d3, d4 = ab @ bp, ac @ bp
+    if d3 >= 0 and d4 <= d3:
+        return b
+    vc = d1 * d4 - d3 * d2
+    if vc <= 0 and d1 >= 0 and d3 <= 0:
+        return a + ab * (d1 / (d1 - d3))
+    cp = p - c
+    d5, d6 = ab @ cp, ac @ cp
+    if d6 >= 0 and d5 <= d6:
+        return c
+    vb = d5 * d2 - d1 * d6
+    if vb <= 0 and d2 >= 0 and d6 <= 0:
+        return a + ac * (d2 / (d2 - d6))
+    va = d3 * d6 - d5 * d4
+    if va <= 0 and (d4 - d3) >= 0 and (d5 - d6) >= 0:
+        return b + (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6)))
+    den = 1.0 / (va + vb + vc)
+    return a + ab * (vb * den) + ac * (vc * den)
+
+
+def test_gjk_sphere_triangle_closed_form(oracle):
+    """TRIANGLE shapes (three vertices, sandbox/primitives.hpp:13): a sphere
+    against a triangle is separated by |p - q| - r with q the closest point of
+    the triangle (every Voronoi region: face, edges, vertices), and overlaps
+    by r - |p - q|; the normal is (p - q) / |p - q|."""
+    rng = np.random.default_rng(17)
+    tris, centres, radii, expect, normals = [], [], [], [], []
+    for k in range(60):
+        t = rng.normal(0, 1, (3, 3)).astype(np.float32)
+        c = (t.astype(np.float64).mean(0)).astype(np.float32)
+        p = (c + rng.normal(0, 1.2, 3)).astype(np.float64)
+        q = closest_on_triangle(p, *t.astype(np.float64))
+        d = np.linalg.norm(p - q)
+        r = float(np.float32(d * (0.5 if k % 2 == 0 else 1.3)))
+        tris.append(t - c)
+        centres += [c, p.astype(np.float32)]
+        radii += [0.0, r]
+        expect.append(d - r)
+        normals.append((p - q) / d)
+    verts = []
+    for t in tris:
+        verts += [t, []]
+    h = shapes_from(verts, centres, radii=radii)
+    pairs = np.array([[2 * k, 2 * k + 1] for k in range(60)], np.int32)
+    res = oracle.gjk(h, pairs)
+    for k in range(60):
+        assert bool(res["flags"][k] & 1) == (expect[k] < 0), k
+        assert abs(res["sep"][k] - expect[k]) < 2e-4 * max(1.0, abs(expect[k])), (k, res["sep"][k], expect[k])
+        assert np.allclose(res["normal"][k], normals[k], atol=5e-2), k
