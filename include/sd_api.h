@@ -277,6 +277,14 @@ sd_status sd_ctx_load(sd_ctx* ctx, const void* buffer, size_t bytes);
 sd_status sd_ctx_remove_spheres(sd_ctx* ctx, const int* ids, int count);
 sd_status sd_ctx_add_spheres(sd_ctx* ctx, int count, const float* x, const float* y, const float* z, const float* vx,
                              const float* vy, const float* vz, const float* radius, const float* mass);
+/* Persistent form of the fused step kernel (contexts with packed 64-bit keys,
+ * single-scene or batch): instead of one 64-slot wave per launched wave, a
+ * grid of resident workgroups (the occupancy limit times the CUs) whose waves
+ * claim 64-slot tiles from one counter per XCD (the XCD's eighth of the slots)
+ * until they run out. Results are bit-identical to the default form; the
+ * captured step graphs are dropped and captured again. on = 0 restores the
+ * default form. */
+sd_status sd_ctx_set_persistent(sd_ctx* ctx, int on);
 /* Autotuner: times trial_steps eager steps for every radix-sort tile size
  * (4, 8, 16 items per thread) and contact block size (64, 128, 256), keeps the
  * fastest pair and returns it with its time per step. Results of a step do not

@@ -168,6 +168,7 @@ _SIGS = {
     "sd_clump_spheres": [_P, _P],
     "sd_hull_create": [_P, _P],
     "sd_batch_attach_hulls": [_P, _P],
+    "sd_ctx_set_persistent": [_P, _I],
     "sd_hull_destroy": [_P],
     "sd_hull_set_params": [_P, _P],
     "sd_hull_upload": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _I],

@@ -22,6 +22,8 @@
 // Walls are partners with r = 0, 1/m = 0, v = w = 0 and ids -1..-6.
 #include "sd_contact_law.hpp"
 #include "sd_grid_dev.hpp"
+#include <algorithm>
+
 #include "sd_internal.hpp"
 
 namespace sd {
@@ -134,23 +136,13 @@ __device__ __forceinline__ T uniform_copy(const T& src) {
 
 // STRADDLE (batch mode only): one 64-thread block per listed wave whose slots
 // span more than one scene; every lane then takes its own scene's parameters.
-template <int MODE, bool INDIRECT, bool STRADDLE = false>
-__global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
+// The work of one sphere slot s of a wave (its lanes are consecutive slots
+// of one wave, or the lanes of a straddling wave): everything the kernel does
+// once the slot is known.
+template <int MODE, bool INDIRECT, bool STRADDLE>
+__device__ __forceinline__ void contact_wave(const ContactArgs& a, int s) {
     constexpr bool BATCH = MODE == CONTACT_FUSED_BATCH;
     constexpr bool PADDED = MODE != CONTACT_FORCES;  // fused modes read a context's padded buffer
-    static_assert(!STRADDLE || BATCH, "k_contacts: straddling waves exist in batch mode only");
-    if (a.step_ctl) {
-        if (a.step_ctl[0]) return;  // the step was given up
-        if (!STRADDLE && blockIdx.x == 0 && threadIdx.x == 0) a.step_ctl[1] += 1;
-    }
-    int s;
-    if constexpr (STRADDLE) {
-        s = a.straddle[blockIdx.x] * WAVE + (int)threadIdx.x;
-    } else {
-        const int lb = xcd_logical_block((a.n + (int)blockDim.x - 1) / (int)blockDim.x);
-        if (lb < 0) return;
-        s = lb * (int)blockDim.x + (int)threadIdx.x;
-    }
     if (s >= a.n) return;
     GridDev g;
     ContactDev c;
@@ -423,6 +415,50 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     }
 }
 
+
+template <int MODE, bool INDIRECT, bool STRADDLE = false>
+__global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
+    static_assert(!STRADDLE || MODE == CONTACT_FUSED_BATCH, "k_contacts: straddling waves exist in batch mode only");
+    if (a.step_ctl) {
+        if (a.step_ctl[0]) return;  // the step was given up
+        if (!STRADDLE && blockIdx.x == 0 && threadIdx.x == 0) a.step_ctl[1] += 1;
+    }
+    int s;
+    if constexpr (STRADDLE) {
+        s = a.straddle[blockIdx.x] * WAVE + (int)threadIdx.x;
+    } else {
+        const int lb = xcd_logical_block((a.n + (int)blockDim.x - 1) / (int)blockDim.x);
+        if (lb < 0) return;
+        s = lb * (int)blockDim.x + (int)threadIdx.x;
+    }
+    contact_wave<MODE, INDIRECT, STRADDLE>(a, s);
+}
+
+// Persistent form of the fused step kernel: a grid of resident blocks whose
+// waves claim 64-slot tiles from a counter per XCD (tiles of XCD x are the
+// x-th eighth of the slots, so an XCD's L2 still holds neighbouring cells)
+// until its range is exhausted; the results are those of k_contacts (every
+// tile computes the same slots the same way). counters[0..7] start at 0.
+template <int MODE, bool INDIRECT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_contacts_persistent(
+    ContactArgs a, int* __restrict__ counters) {
+    if (a.step_ctl) {
+        if (a.step_ctl[0]) return;  // the step was given up
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.step_ctl[1] += 1;
+    }
+    const int ntiles = (a.n + WAVE - 1) / WAVE;
+    const int per = (ntiles + XCDS - 1) / XCDS;
+    const int x = (int)(blockIdx.x % XCDS);
+    const int t0 = x * per, t1 = min(ntiles, t0 + per);
+    for (;;) {
+        int t = 0;
+        if (lane_id() == 0) t = atomicAdd(&counters[x], 1);
+        t = t0 + __shfl(t, 0);
+        if (t >= t1) break;
+        contact_wave<MODE, INDIRECT, false>(a, t * WAVE + lane_id());
+    }
+}
+
 __global__ void k_integrate(float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ omg,
                             const float4* __restrict__ f, const float4* __restrict__ t, int n, ContactDev c) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -450,6 +486,39 @@ sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int blo
         hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_U64, true>), grid, dim3(block), lds, s, a);
     else
         hipLaunchKernelGGL((k_contacts<CONTACT_FUSED_BATCH, true>), grid, dim3(block), lds, s, a);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+sd_status launch_contacts_persistent(int mode, const ContactArgs& a, hipStream_t s, int block, int* counters) {
+    if (a.n <= 0) return SD_OK;
+    if (block <= 0) block = 128;
+    SD_REQUIRE(block % WAVE == 0 && block <= 256, "contact kernel: block must be a multiple of 64, <= 256");
+    SD_REQUIRE(mode == CONTACT_FUSED_U64 || mode == CONTACT_FUSED_BATCH,
+               "contact kernel: the persistent form is for packed 64-bit keys");
+    const size_t lds = (size_t)(block / WAVE) * CONTACT_STAGE * sizeof(float4);
+    int dev = 0, cus = 0, per_cu = 0;
+    SD_HIP(hipGetDevice(&dev));
+    SD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (mode == CONTACT_FUSED_U64)
+        SD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&k_contacts_persistent<CONTACT_FUSED_U64, true>), block, lds));
+    else
+        SD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&k_contacts_persistent<CONTACT_FUSED_BATCH, true>), block, lds));
+    // every resident block, at most one per tile, a multiple of the XCD count
+    const int tiles = div_up(a.n, WAVE);
+    int nb = std::max(1, per_cu) * std::max(1, cus);
+    nb = std::min(nb, div_up(tiles, block / WAVE));
+    nb = std::max(XCDS, nb / XCDS * XCDS);
+    sd_status st = launch_fill_u32((uint32_t*)counters, 0u, XCDS, s);
+    if (st != SD_OK) return st;
+    if (mode == CONTACT_FUSED_U64)
+        hipLaunchKernelGGL((k_contacts_persistent<CONTACT_FUSED_U64, true>), dim3(nb), dim3(block), lds, s, a,
+                           counters);
+    else
+        hipLaunchKernelGGL((k_contacts_persistent<CONTACT_FUSED_BATCH, true>), dim3(nb), dim3(block), lds, s, a,
+                           counters);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }

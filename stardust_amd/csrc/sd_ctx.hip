@@ -139,6 +139,9 @@ struct sd_ctx {
     // rigid-body scenes stepped with the batch (sd_batch_attach_hulls), on a
     // stream of their own forked from and joined to the step
     sd_hull_ctx* hulls = nullptr;
+    // persistent form of the fused step kernel (sd_ctx_set_persistent)
+    bool persistent = false;
+    int* pcounters = nullptr;
     hipStream_t hside = nullptr;
     hipEvent_t hev_fork = nullptr, hev_join = nullptr;
 };
@@ -282,7 +285,11 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
             if (st != SD_OK) return st;
             SD_HIP(hipEventRecord(c->ev_join, c->side));
         }
-        st = launch_contacts(c->batch ? CONTACT_FUSED_BATCH : CONTACT_FUSED_U64, a, s, c->contact_block);
+        if (c->persistent)
+            st = launch_contacts_persistent(c->batch ? CONTACT_FUSED_BATCH : CONTACT_FUSED_U64, a, s,
+                                            c->contact_block, c->pcounters);
+        else
+            st = launch_contacts(c->batch ? CONTACT_FUSED_BATCH : CONTACT_FUSED_U64, a, s, c->contact_block);
         if (st != SD_OK) return st;
         if (fork) SD_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
         MARK(4);
@@ -450,6 +457,19 @@ extern "C" sd_status sd_ctx_set_contact(sd_ctx* c, const sd_contact_params* cp) 
     return SD_OK;
 }
 
+extern "C" sd_status sd_ctx_set_persistent(sd_ctx* c, int on) {
+    SD_REQUIRE(c, "sd_ctx_set_persistent: NULL context");
+    SD_REQUIRE(c->p.key64, "sd_ctx_set_persistent: the persistent step kernel needs packed 64-bit keys");
+    if (on && !c->pcounters) {
+        sd_status st = dalloc(&c->pcounters, 8);
+        if (st != SD_OK) return st;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);  // captured with the other kernel form
+    c->persistent = on != 0;
+    return SD_OK;
+}
+
 extern "C" sd_status sd_batch_attach_hulls(sd_ctx* c, sd_hull_ctx* h) {
     SD_REQUIRE(c, "sd_batch_attach_hulls: NULL context");
     SD_REQUIRE(c->batch, "sd_batch_attach_hulls: not a batch context");
@@ -477,6 +497,7 @@ extern "C" sd_status sd_ctx_destroy(sd_ctx* c) {
     if (c->hev_fork) (void)hipEventDestroy(c->hev_fork);
     if (c->hev_join) (void)hipEventDestroy(c->hev_join);
     if (c->hside) (void)hipStreamDestroy(c->hside);
+    if (c->pcounters) (void)hipFree(c->pcounters);
     release(c);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);

@@ -155,6 +155,10 @@ struct ContactArgs {
     int n_straddle;
 };
 sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int block);
+// Persistent form of the fused u64 / batch step kernel (same results): a
+// grid of resident blocks claiming 64-slot tiles from counters (XCDS ints,
+// cleared here by a fill kernel) until every tile is done.
+sd_status launch_contacts_persistent(int mode, const ContactArgs& a, hipStream_t s, int block, int* counters);
 sd_status launch_contacts_straddle(const ContactArgs& a, hipStream_t s);
 
 // Sorted order of the next step's u64 keys from the previous sorted order and

@@ -95,6 +95,10 @@ class SphereScene:
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
         L.call("sd_ctx_load", self.h, L.ptr(buf), buf.nbytes)
 
+    def set_persistent(self, on=True):
+        """Persistent form of the fused step kernel (packed 64-bit keys); same results."""
+        L.call("sd_ctx_set_persistent", self.h, 1 if on else 0)
+
     def remove(self, ids):
         """Drops the spheres with these particle ids; the others are renumbered in id order."""
         ids = np.ascontiguousarray(ids, dtype=np.int32)

@@ -358,3 +358,36 @@ def test_batch_steps_attached_hull_scenes():
         x.close()
     ha.close()
     hb.close()
+
+
+def test_persistent_step_kernel_changes_no_result():
+    """sd_ctx_set_persistent: the persistent form of the fused step kernel
+    (resident workgroups claiming 64-slot tiles per XCD) gives the default
+    form's results bit for bit, batched and single-scene, graph and eager."""
+    say("persistent step kernel")
+    w = small_batch(nscenes=8, seed=0x5EEDB006)
+    s, scenes = w.generate()
+    v = velocities(len(s["x"]), 6)
+    for graph in (True, False):
+        a = batch_of(w, scenes, s, v, use_graph=graph)
+        b = batch_of(w, scenes, s, v, use_graph=graph)
+        b.set_persistent(True)
+        for k in (1, 7, 30):
+            a.step(k)
+            b.step(k)
+            assert_same(a.download(), b.download(), "batch, graph=%s, after %d" % (graph, k))
+        b.set_persistent(False)
+        a.step(5)
+        b.step(5)
+        assert_same(a.download(), b.download(), "back to the default form")
+        a.close()
+        b.close()
+    n, g, c = scenes[0]
+    x = solo(scenes[0], s, v, 0, n)
+    y = solo(scenes[0], s, v, 0, n)
+    y.set_persistent(True)
+    x.step(40)
+    y.step(40)
+    assert_same(x.download(), y.download(), "single scene")
+    x.close()
+    y.close()
