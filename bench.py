@@ -423,6 +423,24 @@ def batch_section(args):
     per = {k: round(v / args.profile_steps, 5) for k, v in prof.items()}
     contacts = b.contact_count()
     log("batch: %.3f ms/step -> %.3e particle-steps/s; stages %s" % (1000 * t / args.batch_steps, batched, per))
+    # the persistent form of the fused step kernel, same steps (results identical)
+    b.set_persistent(True)
+    b.step(3)
+    L.synchronize(stream)
+    t0 = time.perf_counter()
+    L.call("sd_event_record", ev[0], stream)
+    b.step(args.batch_steps)
+    L.call("sd_event_record", ev[1], stream)
+    L.synchronize(stream)
+    tp = max(time.perf_counter() - t0, elapsed_ms(ev) / 1000.0)
+    pprof = b.profile(args.profile_steps)
+    b.set_persistent(False)
+    persistent = {"ms_per_step": round(1000 * tp / args.batch_steps, 4),
+                  "particle_steps_per_s": w.n * args.batch_steps / tp,
+                  "contact_ms": round(pprof["contact"] / args.profile_steps, 5),
+                  "form": "fused step kernel as resident workgroups claiming 64-slot tiles per XCD"}
+    log("batch: persistent form %.3f ms/step (contact %.4f ms)" % (persistent["ms_per_step"],
+                                                                  persistent["contact_ms"]))
     hulls = batch_hulls(args, w, b, t / args.batch_steps) if args.batch_hull_steps > 0 else None
     b.close()
 
@@ -448,7 +466,7 @@ def batch_section(args):
             "batched_particle_steps_per_s": batched, "batched_ms_per_step": round(1000 * t / args.batch_steps, 4),
             "plain_particle_steps_per_s": plain, "plain_steps_per_scene": args.plain_steps,
             "batched_over_plain": batched / plain if plain else None, "autotune": tune, "stage_ms": per,
-            "contacts_per_sphere": contacts / w.n, "with_hull_scenes": hulls,
+            "contacts_per_sphere": contacts / w.n, "persistent": persistent, "with_hull_scenes": hulls,
             "form": "batched: one graph-replayed step for all scenes; plain: each scene alone in its own "
                     "graph-replayed context, scenes one after the other"}
 

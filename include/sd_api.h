@@ -280,8 +280,8 @@ sd_status sd_ctx_add_spheres(sd_ctx* ctx, int count, const float* x, const float
 /* Persistent form of the fused step kernel (contexts with packed 64-bit keys,
  * single-scene or batch): instead of one 64-slot wave per launched wave, a
  * grid of resident workgroups (the occupancy limit times the CUs) whose waves
- * claim 64-slot tiles from one counter per XCD (the XCD's eighth of the slots)
- * until they run out. Results are bit-identical to the default form; the
+ * claim runs of eight 64-slot tiles from one counter per XCD (the XCD's eighth
+ * of the slots) until they run out. Results are bit-identical to the default form; the
  * captured step graphs are dropped and captured again. on = 0 restores the
  * default form. */
 sd_status sd_ctx_set_persistent(sd_ctx* ctx, int on);

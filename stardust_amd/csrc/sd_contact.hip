@@ -434,8 +434,11 @@ __global__ __launch_bounds__(256) void k_contacts(ContactArgs a) {
     contact_wave<MODE, INDIRECT, STRADDLE>(a, s);
 }
 
+constexpr int PERSIST_CLAIM = 8;    // tiles a wave claims at a time
+constexpr int PERSIST_STRIDE = 64;  // ints between the XCDs' counters
+
 // Persistent form of the fused step kernel: a grid of resident blocks whose
-// waves claim 64-slot tiles from a counter per XCD (tiles of XCD x are the
+// waves claim runs of PERSIST_CLAIM 64-slot tiles from a counter per XCD (tiles of XCD x are the
 // x-th eighth of the slots, so an XCD's L2 still holds neighbouring cells)
 // until its range is exhausted; the results are those of k_contacts (every
 // tile computes the same slots the same way). counters[0..7] start at 0.
@@ -450,12 +453,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int per = (ntiles + XCDS - 1) / XCDS;
     const int x = (int)(blockIdx.x % XCDS);
     const int t0 = x * per, t1 = min(ntiles, t0 + per);
+    int* const ctr = counters + PERSIST_STRIDE * x;  // one counter per XCD, each on its own 256-byte line
     for (;;) {
         int t = 0;
-        if (lane_id() == 0) t = atomicAdd(&counters[x], 1);
+        if (lane_id() == 0) t = atomicAdd(ctr, PERSIST_CLAIM);
         t = t0 + __shfl(t, 0);
         if (t >= t1) break;
-        contact_wave<MODE, INDIRECT, false>(a, t * WAVE + lane_id());
+        const int te = min(t + PERSIST_CLAIM, t1);
+        for (; t < te; ++t) contact_wave<MODE, INDIRECT, false>(a, t * WAVE + lane_id());
     }
 }
 
@@ -511,7 +516,7 @@ sd_status launch_contacts_persistent(int mode, const ContactArgs& a, hipStream_t
     int nb = std::max(1, per_cu) * std::max(1, cus);
     nb = std::min(nb, div_up(tiles, block / WAVE));
     nb = std::max(XCDS, nb / XCDS * XCDS);
-    sd_status st = launch_fill_u32((uint32_t*)counters, 0u, XCDS, s);
+    sd_status st = launch_fill_u32((uint32_t*)counters, 0u, (size_t)PERSIST_STRIDE * XCDS, s);
     if (st != SD_OK) return st;
     if (mode == CONTACT_FUSED_U64)
         hipLaunchKernelGGL((k_contacts_persistent<CONTACT_FUSED_U64, true>), dim3(nb), dim3(block), lds, s, a,

@@ -461,7 +461,7 @@ extern "C" sd_status sd_ctx_set_persistent(sd_ctx* c, int on) {
     SD_REQUIRE(c, "sd_ctx_set_persistent: NULL context");
     SD_REQUIRE(c->p.key64, "sd_ctx_set_persistent: the persistent step kernel needs packed 64-bit keys");
     if (on && !c->pcounters) {
-        sd_status st = dalloc(&c->pcounters, 8);
+        sd_status st = dalloc(&c->pcounters, 64 * 8);  // one 256-byte line per XCD
         if (st != SD_OK) return st;
     }
     (void)hipStreamSynchronize(c->stream);

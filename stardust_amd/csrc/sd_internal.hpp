@@ -156,8 +156,9 @@ struct ContactArgs {
 };
 sd_status launch_contacts(int mode, const ContactArgs& a, hipStream_t s, int block);
 // Persistent form of the fused u64 / batch step kernel (same results): a
-// grid of resident blocks claiming 64-slot tiles from counters (XCDS ints,
-// cleared here by a fill kernel) until every tile is done.
+// grid of resident blocks claiming runs of 64-slot tiles from one counter per
+// XCD (counters: 512 ints, cleared here by a fill kernel) until every tile is
+// done.
 sd_status launch_contacts_persistent(int mode, const ContactArgs& a, hipStream_t s, int block, int* counters);
 sd_status launch_contacts_straddle(const ContactArgs& a, hipStream_t s);
 
