@@ -66,12 +66,18 @@ __global__ __launch_bounds__(DIAG_BLOCK) void k_diag_partial(const float4* __res
 
 struct GraphKey {
     int keys, hist, state;
+    bool full;  // two steps with the full radix sort (dense runs)
     bool operator<(const GraphKey& o) const {
         if (keys != o.keys) return keys < o.keys;
         if (hist != o.hist) return hist < o.hist;
-        return state < o.state;
+        if (state != o.state) return state < o.state;
+        return full < o.full;
     }
 };
+
+// Full-sort steps run between two reads of the changed-cell count while a
+// scene is dense (more changed cells per step than the re-sort takes).
+constexpr int DENSE_CHUNK = 16;
 
 }  // namespace sd
 
@@ -115,6 +121,7 @@ struct sd_ctx {
     // [0] != 0: a re-sort step was given up; [1] steps run since the last reset
     int* step_ctl = nullptr;
     bool need_full = true;  // the next step's keys are not from a step of this order
+    bool dense = false;     // the last full-sort step had more changed cells than the re-sort takes
     double* diag = nullptr;
     int diag_blocks = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -323,14 +330,14 @@ sd_status enqueue_step(sd_ctx* c, hipEvent_t* ev = nullptr, bool full = false) {
     return SD_OK;
 }
 
-sd_status run_graph_pair(sd_ctx* c) {
-    const GraphKey key{c->cur_keys, c->cur_hist, c->cur_state};
+sd_status run_graph_pair(sd_ctx* c, bool full = false) {
+    const GraphKey key{c->cur_keys, c->cur_hist, c->cur_state, full};
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int k0 = c->cur_keys, h0 = c->cur_hist, s0 = c->cur_state;
         SD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        sd_status st = enqueue_step(c);
-        if (st == SD_OK) st = enqueue_step(c);
+        sd_status st = enqueue_step(c, nullptr, full);
+        if (st == SD_OK) st = enqueue_step(c, nullptr, full);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (st != SD_OK) {
@@ -383,9 +390,23 @@ sd_status run_steps(sd_ctx* c, int nsteps) {
         st = launch_fill_u32((uint32_t*)c->step_ctl, 0u, 2, c->stream);
         if (st != SD_OK) return st;
         const int h0 = c->cur_hist, s0 = c->cur_state;
-        if (c->need_full) {
-            st = enqueue_step(c, nullptr, true);
-            if (st != SD_OK) return st;
+        const bool full = c->need_full || c->dense;
+        if (full) {
+            // one full-sort step after a given-up step or an upload; while the
+            // scene is dense, DENSE_CHUNK of them (graph pairs) before the
+            // changed-cell count is read again
+            int l = c->dense ? std::min(left, DENSE_CHUNK) : 1;
+            if (c->p.use_graph) {
+                while (l >= 2) {
+                    st = run_graph_pair(c, true);
+                    if (st != SD_OK) return st;
+                    l -= 2;
+                }
+            }
+            for (; l > 0; --l) {
+                st = enqueue_step(c, nullptr, true);
+                if (st != SD_OK) return st;
+            }
             c->need_full = false;
         } else {
             int l = left;
@@ -411,6 +432,12 @@ sd_status run_steps(sd_ctx* c, int nsteps) {
         c->cur_state = s0 ^ (ran & 1);
         left -= ran;
         if (ctl[0]) c->need_full = true;  // redo the given-up step with the full sort
+        if (full) {  // still dense: the last full-sort step's changed cells exceed the re-sort's cap
+            int m = 0;
+            SD_HIP(hipMemcpyAsync(&m, c->mcount + c->cur_hist, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            SD_HIP(hipStreamSynchronize(c->stream));
+            c->dense = m > RESORT_CAP;
+        }
     }
     return SD_OK;
 }
@@ -533,6 +560,7 @@ sd_status check_spheres(const char* who, int n, const float* x, const float* y, 
 // restore): the next sort is the full one.
 sd_status force_full_sort(sd_ctx* c) {
     c->need_full = true;
+    c->dense = false;
     sd_status st = launch_fill_u32((uint32_t*)c->step_ctl, 0u, 2, c->stream);
     if (st != SD_OK) return st;
     return launch_fill_u32((uint32_t*)(c->mcount + c->cur_hist), 0x7fffffffu, 1, c->stream);
@@ -920,11 +948,24 @@ extern "C" sd_status sd_ctx_autotune(sd_ctx* c, int trial_steps, int* tile_items
             if (st == SD_OK) st = enqueue_step(c, nullptr, c->need_full);  // warm-up (a full sort after the restore)
             c->need_full = false;
             if (st != SD_OK) break;
+            // a dense scene (more changed cells than the re-sort takes) is timed
+            // with the full-sort steps it runs
+            bool full_trials = false;
+            if (c->p.key64) {
+                int m = 0;
+                if (hipMemcpyAsync(&m, c->mcount + c->cur_hist, sizeof(int), hipMemcpyDeviceToHost, c->stream) !=
+                        hipSuccess ||
+                    hipStreamSynchronize(c->stream) != hipSuccess) {
+                    st = set_error(SD_ERR_HIP, "sd_ctx_autotune: read failed");
+                    break;
+                }
+                full_trials = m > RESORT_CAP;
+            }
             if (hipEventRecord(e0, c->stream) != hipSuccess) {
                 st = set_error(SD_ERR_HIP, "sd_ctx_autotune: event record failed");
                 break;
             }
-            for (int k = 0; k < trial_steps && st == SD_OK; ++k) st = enqueue_step(c);
+            for (int k = 0; k < trial_steps && st == SD_OK; ++k) st = enqueue_step(c, nullptr, full_trials);
             if (st != SD_OK) break;
             float ms = 0.0f;
             if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
