@@ -1,6 +1,6 @@
 """Benchmark of the sphere DEM step path on one MI355X.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c2_dense] [--no-cpu]
 
 Workload (default): dem_c2 — 4,194,304 polydisperse spheres in a walled column,
 packed 64-bit (cell, index) keys sorted every step, dt = 5e-6 s (see
@@ -800,7 +800,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c2_dense"],
+                    help="headline workload: the dilute configuration-2 column (default), configuration 1, or the "
+                         "contact-dense packing of the configuration-2 law")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -842,7 +844,7 @@ def main():
     from stardust_amd.scene import SphereScene
 
     L.require_gpu()
-    w = W.dem_c2() if args.workload == "c2" else W.dem_c1()
+    w = {"c2": W.dem_c2, "c1": W.dem_c1, "c2_dense": W.dem_c2_dense}[args.workload]()
     steps = args.steps if args.steps is not None else w.steps
     r, s = sphere_run(w, steps, args.warmup, not args.no_autotune, args.profile_steps)
     value, t, tune, g = r["value"], r["t"], r["tune"], r["grid"]

@@ -77,13 +77,19 @@ __device__ __forceinline__ int old_slot_of(const ContactArgs& a, int s) {
     return (int)(uint32_t)a.old_slot_u64[s];
 }
 
-// Looks up the tangential history stored for partner pid in the old list.
-__device__ __forceinline__ void find_history(const ContactArgs& a, int os, int oc, int pid, float& hx, float& hy,
-                                             float& hz) {
+// Looks up the tangential history stored for partner pid in the old list
+// (partner ids in a list are distinct, so the entry found does not depend on
+// the search order). The search starts at entry `start`, the position the
+// contact has in the new list: contacts are found in the same order step after
+// step, so a lasting contact is usually at the same position and the first
+// probe finds it.
+__device__ __forceinline__ void find_history(const ContactArgs& a, int os, int oc, int pid, int start, float& hx,
+                                             float& hy, float& hz) {
     hx = 0.0f;
     hy = 0.0f;
     hz = 0.0f;
-    for (int k = 0; k < oc; ++k) {
+    int k = start < oc ? start : 0;
+    for (int j = 0; j < oc; ++j) {
         const float4 h = a.hist_old[(size_t)k * a.n + os];
         if (__float_as_int(h.w) == pid) {
             hx = h.x;
@@ -91,6 +97,7 @@ __device__ __forceinline__ void find_history(const ContactArgs& a, int os, int o
             hz = h.z;
             break;
         }
+        k = k + 1 < oc ? k + 1 : 0;
     }
 }
 
@@ -186,8 +193,8 @@ __device__ __forceinline__ void contact_wave(const ContactArgs& a, int s) {
 
     // one sphere-sphere contact with the sphere at sorted position t; the
     // sphere itself and coincident spheres (d2 == 0) are not contacts
-    auto visit = [&](int t) {
-        const float4 q = a.pos[t];
+    auto visit = [&](int t, const float4* sl) {
+        const float4 q = sl[t];  // the staged copy (LDS) of position t
         const float dx = p.x - q.x;
         const float dyy = p.y - q.y;
         const float dzz = p.z - q.z;
@@ -203,7 +210,7 @@ __device__ __forceinline__ void contact_wave(const ContactArgs& a, int s) {
         const float inv_d = 1.0f / d;
         const float nx = dx * inv_d, ny = dyy * inv_d, nz = dzz * inv_d;
         float hx, hy, hz;
-        find_history(a, os, oc, pid, hx, hy, hz);
+        find_history(a, os, oc, pid, nc, hx, hy, hz);
         const PairForce f = contact_law(c, nx, ny, nz, rs - d, p.w, q.w, v.w, vq.w, v.x - vq.x, v.y - vq.y,
                                         v.z - vq.z, w.x, w.y, w.z, wq.x, wq.y, wq.z, hx, hy, hz);
         fx = fx + f.fx;
@@ -222,7 +229,7 @@ __device__ __forceinline__ void contact_wave(const ContactArgs& a, int s) {
     // (rare, and always the batch holding the sphere itself) then drops
     // positions past hi and the sphere itself (position s, without a memory
     // access), and visit() drops coincident spheres (d2 == 0).
-    auto test_batch = [&](const float4 (&q)[CAND_UNROLL], int t0, int hi) {
+    auto test_batch = [&](const float4 (&q)[CAND_UNROLL], int t0, int hi, const float4* sl) {
         bool h[CAND_UNROLL];
         uint64_t any = 0u;  // lanes with a possible contact (scalar masks)
 #pragma unroll
@@ -242,7 +249,7 @@ __device__ __forceinline__ void contact_wave(const ContactArgs& a, int s) {
             while (hit) {
                 const int k = __builtin_ctz(hit);
                 hit &= hit - 1u;
-                if (t0 + k < hi && t0 + k != s) visit(t0 + k);
+                if (t0 + k < hi && t0 + k != s) visit(t0 + k, sl);
             }
         }
     };
@@ -328,7 +335,7 @@ __device__ __forceinline__ void contact_wave(const ContactArgs& a, int s) {
                     float4 q[CAND_UNROLL];
 #pragma unroll
                     for (int k = 0; k < CAND_UNROLL; ++k) q[k] = sl[t0 + k];
-                    test_batch(q, t0, thi);
+                    test_batch(q, t0, thi, sl);
                 }
                 wave_lds_sync();  // every lane's reads precede the next copy
             }
@@ -362,7 +369,7 @@ __device__ __forceinline__ void contact_wave(const ContactArgs& a, int s) {
             const float nz = ax == 2 ? sgn : 0.0f;
             const int pid = -(wl + 1);
             float hx, hy, hz;
-            find_history(a, os, oc, pid, hx, hy, hz);
+            find_history(a, os, oc, pid, nc, hx, hy, hz);
             const PairForce f = contact_law(c, nx, ny, nz, p.w - dist, p.w, 0.0f, v.w, 0.0f, v.x, v.y, v.z, w.x,
                                             w.y, w.z, 0.0f, 0.0f, 0.0f, hx, hy, hz);
             fx = fx + f.fx;
