@@ -59,9 +59,10 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs on MI355X (FETCH x 2.0 + WRITE x 1.0 KiB, factors from the plain copy kernel);
-# the runs and the per-kernel table are in profiles/bench_hbm_counters.md.
+# the runs and the counters are in profiles/contact_kernel_c2_counters.md.
 MEASURED_TRAFFIC = {
-    ("dem_c2_column_4194304", "contact"): (250667.7 * 2.0 + 262144.0) * 1024.0,
+    ("dem_c2_column_4194304", "contact"): (136485.2 * 2.0 + 262144.0) * 1024.0,
+    ("dem_c2_dense_4194304", "contact"): (1475726.9 * 2.0 + 717303.8) * 1024.0,
 }
 
 
@@ -612,7 +613,7 @@ def c2_dense_section(args):
             "cells": r["grid"].n_cells, "launch_shapes": r["tune"], "stages": r["stages"],
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
-                         "algorithmic_bytes": r["sbytes"][dom]},
+                         "traffic": MEASURED_TRAFFIC.get((w.name, dom)), "algorithmic_bytes": r["sbytes"][dom]},
             "packing": w.describe()["packing"]}
 
 
