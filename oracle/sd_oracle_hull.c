@@ -233,6 +233,8 @@ long or_brute_aabb_pairs(int n, const float* lo4, const float* hi4, int* pairs, 
 #define TOUCH_EPS2 1e-14f   /* |v|^2 below this: origin on the boundary (hit) */
 #define REL_EPS 1e-6f       /* sphere shapes: v.v - v.w <= eps v.v ends the query */
 #define STALL_EPS 1e-4f     /* gap accepted when the distance stops decreasing */
+#define DUP_REL2 0x1p-32f   /* duplicate-support exit: |v|^2 above this fraction of the
+                               largest squared simplex coordinate is a separation */
 #define EPA_REL_EPS 1e-6f
 #define CERT_REL 1.52587890625e-05f /* 2^-16: support margin relative to the features' vertex radii */
 #define CERT_TAU 9.765625e-04f      /* 2^-10: closest points this far inside their features */
@@ -514,7 +516,12 @@ static void gjk_loop(const Sup* S, Simplex* s, V3* vp, int max_iter, GjkState* g
          * means |v| is at rounding level: an uncertain hit for EPA. A sphere
          * shape (keyless) ends on the relative gap instead. */
         if (dup || (w.key < 0 && gap <= REL_EPS * vv)) {
-            if (vw > 0.0f) {
+            /* w is a simplex vertex, so v.w >= v.v exactly: v.w <= 0 is the
+             * rounding of v; |v| well above the coordinates' rounding level is
+             * the distance */
+            float scale2 = vdot(w.p, w.p);
+            for (int i = 0; i < s->n; ++i) scale2 = fmaxf(scale2, vdot(s->w[i].p, s->w[i].p));
+            if (vw > 0.0f || vv > DUP_REL2 * scale2) {
                 g->converged = 1;
             } else {
                 g->hit = 1;

@@ -31,7 +31,11 @@ namespace sd {
 constexpr int GJK_MAX_VERTS_PER_HULL = 1024;  // key = ia * 1024 + ib
 constexpr float GJK_TOUCH_EPS2 = 1e-14f;      // |v|^2 below this: origin on the boundary
 constexpr float GJK_REL_EPS = 1e-6f;          // sphere shapes: v.v - v.w <= eps v.v ends the query
-constexpr float GJK_STALL_EPS = 1e-4f;        // gap accepted when the distance stops decreasing
+constexpr float GJK_STALL_EPS = 1e-4f;
+// A duplicate-support exit with v.w <= 0 is a separation when |v|^2 exceeds
+// this fraction of the largest squared simplex coordinate ((2^-16)^2: |v| is
+// 65,536 times the coordinates' rounding unit of about 2^-24 of their size).
+constexpr float GJK_DUP_REL2 = 0x1p-32f;        // gap accepted when the distance stops decreasing
 constexpr float EPA_REL_EPS = 1e-6f;          // EPA: support gap <= eps * depth + 1e-9 -> converged
 constexpr float CERT_REL = 1.52587890625e-05f;  // 2^-16: certificate margin relative to the vertex radii
 constexpr float CERT_TAU = 9.765625e-04f;       // 2^-10: closest points this far inside their features
@@ -428,6 +432,15 @@ __device__ __forceinline__ bool simplex_has(const Simplex& s, int key) {
     return dup;
 }
 
+// Largest squared coordinate vector of the simplex points and w.
+__device__ __forceinline__ float simplex_scale2(const Simplex& s, const MPoint& w) {
+    float m = dot(w.p, w.p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < s.n) m = fmaxf(m, dot(s.w[i].p, s.w[i].p));
+    return m;
+}
+
 // ---------------------------------------------------------------- GJK loop
 
 struct GjkState {
@@ -464,7 +477,10 @@ __device__ __forceinline__ void gjk_loop(const Shape& A, const Shape& B, const S
         const float vw = dot(v, w.p);
         const float gap = vv - vw;
         if (simplex_has(s, w.key) || (w.key < 0 && gap <= GJK_REL_EPS * vv)) {
-            if (vw > 0.0f) {
+            // w is a vertex of the simplex, so v.w >= v.v in exact arithmetic:
+            // v.w <= 0 only reflects the rounding of v. |v| well above the
+            // rounding level of the simplex coordinates is the distance.
+            if (vw > 0.0f || vv > GJK_DUP_REL2 * simplex_scale2(s, w)) {
                 g.converged = true;
             } else {
                 g.hit = true;

@@ -248,7 +248,7 @@ def test_certificate_known_answers(oracle):
     assert abs(cold["sep"][0] + 0.1 / math.sqrt(3)) < 1e-6
 
 
-def closest_on_triangle(p, a, b, c):
+def closest_point_on_triangle(p, a, b, c):
     """Closest point of triangle abc to p (Ericson, Real-Time Collision
     Detection, 5.1.5: Voronoi regions of the vertices, edges and face), float64."""
     ab, ac, ap = b - a, c - a, p - a
@@ -287,7 +287,7 @@ def test_gjk_sphere_triangle_closed_form(oracle):
         t = rng.normal(0, 1, (3, 3)).astype(np.float32)
         c = (t.astype(np.float64).mean(0)).astype(np.float32)
         p = (c + rng.normal(0, 1.2, 3)).astype(np.float64)
-        q = closest_on_triangle(p, *t.astype(np.float64))
+        q = closest_point_on_triangle(p, *t.astype(np.float64))
         d = np.linalg.norm(p - q)
         r = float(np.float32(d * (0.5 if k % 2 == 0 else 1.3)))
         tris.append(t - c)
@@ -305,3 +305,46 @@ def test_gjk_sphere_triangle_closed_form(oracle):
         assert bool(res["flags"][k] & 1) == (expect[k] < 0), k
         assert abs(res["sep"][k] - expect[k]) < 2e-4 * max(1.0, abs(expect[k])), (k, res["sep"][k], expect[k])
         assert np.allclose(res["normal"][k], normals[k], atol=5e-2), k
+
+
+def test_gjk_near_touching_pair_is_separated(oracle):
+    """A pair of the configuration-5 hull scenes (scene 64, hulls 464 and 2984)
+    2.7 um apart: GJK ends on a duplicate support with v.w <= 0 from the
+    rounding of v alone (|v| is thousands of rounding units). It is a
+    separation, cold and warm alike, equal to the exact gap along the normal
+    (float64); the hulls do not intersect (linear-programming feasibility)."""
+    from scipy.optimize import linprog
+
+    from stardust_amd import workloads as W
+    h = W.batch_c5().generate_hull_scene(64)
+    sub = {"pos4": h["pos4"][[464, 2984]], "quat4": h["quat4"][[464, 2984]]}
+    vs, nv, verts = [], [], []
+    for i in (464, 2984):
+        a, n = int(h["vstart"][i]), int(h["nv"][i])
+        vs.append(sum(nv))
+        nv.append(n)
+        verts.append(h["verts4"][a:a + n])
+    sub.update({"vstart": np.array(vs, np.int32), "nv": np.array(nv, np.int32), "verts4": np.concatenate(verts)})
+    pr = np.array([[0, 1]], np.int32)
+    cold = oracle.gjk(sub, pr)
+    warm = oracle.gjk(sub, pr, warm=np.array([[0.7831531, -0.30531186, 0.54171556]], np.float32), warm_budget=16)
+    assert cold["flags"][0] & 1 == 0 and cold["flags"][0] & 2
+    assert cold["sep"][0] == warm["sep"][0] and np.array_equal(cold["normal"], warm["normal"])
+
+    def world(k):
+        w0, x, y, z = sub["quat4"][k].astype(np.float64) / np.linalg.norm(sub["quat4"][k].astype(np.float64))
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w0 * z), 2 * (x * z + w0 * y)],
+                      [2 * (x * y + w0 * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w0 * x)],
+                      [2 * (x * z - w0 * y), 2 * (y * z + w0 * x), 1 - 2 * (x * x + y * y)]])
+        v = sub["verts4"][vs[k]:vs[k] + nv[k], :3].astype(np.float64)
+        return v @ R.T + sub["pos4"][k, :3].astype(np.float64)
+
+    A, B = world(0), world(1)
+    n = cold["normal"][0].astype(np.float64)
+    gap = (B @ n).min() - (A @ n).max()
+    assert abs(cold["sep"][0] - gap) < 1e-8, (cold["sep"][0], gap)
+    m = len(A) + len(B)
+    aeq = np.zeros((5, m))
+    aeq[:3, :len(A)], aeq[:3, len(A):], aeq[3, :len(A)], aeq[4, len(A):] = A.T, -B.T, 1, 1
+    r = linprog(np.zeros(m), A_eq=aeq, b_eq=[0, 0, 0, 1, 1], bounds=[(0, None)] * m, method="highs")
+    assert r.status == 2  # infeasible: the hulls are disjoint
