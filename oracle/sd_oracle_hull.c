@@ -233,8 +233,6 @@ long or_brute_aabb_pairs(int n, const float* lo4, const float* hi4, int* pairs, 
 #define TOUCH_EPS2 1e-14f   /* |v|^2 below this: origin on the boundary (hit) */
 #define REL_EPS 1e-6f       /* sphere shapes: v.v - v.w <= eps v.v ends the query */
 #define STALL_EPS 1e-4f     /* gap accepted when the distance stops decreasing */
-#define DUP_REL2 0x1p-32f   /* duplicate-support exit: |v|^2 above this fraction of the
-                               largest squared simplex coordinate is a separation */
 #define EPA_REL_EPS 1e-6f
 #define CERT_REL 1.52587890625e-05f /* 2^-16: support margin relative to the features' vertex radii */
 #define CERT_TAU 9.765625e-04f      /* 2^-10: closest points this far inside their features */
@@ -512,16 +510,12 @@ static void gjk_loop(const Sup* S, Simplex* s, V3* vp, int max_iter, GjkState* g
         int dup = 0;
         for (int i = 0; i < s->n; ++i) dup = dup || (w.key >= 0 && s->w[i].key == w.key);
         /* exact exit: the support point is already a simplex vertex, so in
-         * exact arithmetic v.w >= v.v and any gap is rounding; v.w <= 0 then
-         * means |v| is at rounding level: an uncertain hit for EPA. A sphere
-         * shape (keyless) ends on the relative gap instead. */
+         * exact arithmetic v.w >= v.v and any gap is rounding. v.w > 0 makes
+         * v a separating axis (converged); v.w <= 0 leaves the verdict to the
+         * rounding of v's direction: an uncertain hit, which finish() settles.
+         * A sphere shape (keyless) ends on the relative gap instead. */
         if (dup || (w.key < 0 && gap <= REL_EPS * vv)) {
-            /* w is a simplex vertex, so v.w >= v.v exactly: v.w <= 0 is the
-             * rounding of v; |v| well above the coordinates' rounding level is
-             * the distance */
-            float scale2 = vdot(w.p, w.p);
-            for (int i = 0; i < s->n; ++i) scale2 = fmaxf(scale2, vdot(s->w[i].p, s->w[i].p));
-            if (vw > 0.0f || vv > DUP_REL2 * scale2) {
+            if (vw > 0.0f) {
                 g->converged = 1;
             } else {
                 g->hit = 1;
@@ -924,11 +918,32 @@ static void run_epa(const Shape* A, const Shape* B, const Simplex* s0, int max_i
     feature_result(A, B, q, 3, 1, cert, o, inner);
 }
 
+/* Gap of A and B along the unit axis n (A -> B): the lowest point of B on n
+ * less the highest point of A, from one support pass over both shapes;
+ * positive proves the shapes disjoint. */
+static float axis_gap(const Shape* A, const Shape* B, V3 n) {
+    const MP m = supM(A, B, n);
+    return -vdot(n, m.p);
+}
+
 /* Finishes a query from its GJK state: EPA for a hit, the feature result for a
- * separation. An uncertain hit for which EPA finds no enclosing polytope
- * reports the uncertified separation. */
+ * separation. An uncertain hit is first tested for a separation along the
+ * normal of its simplex's feature result: a positive gap along it is a
+ * converged separation. An uncertain hit for which EPA finds no enclosing
+ * polytope reports the uncertified separation. */
 static void finish(const Shape* A, const Shape* B, Simplex* s, const GjkState* g, int max_iter, int cert, Out* o) {
     int converged = g->converged;
+    if (g->hit && g->uncertain) {
+        Out t = *o;
+        MP w[4];
+        for (int i = 0; i < s->n; ++i) w[i] = s->w[i];
+        feature_result(A, B, w, s->n, 0, cert, &t, mk(0.0f, 0.0f, 0.0f));
+        if (axis_gap(A, B, t.n) > 0.0f) {
+            *o = t;
+            o->flags |= CONV;
+            return;
+        }
+    }
     if (g->hit) {
         run_epa(A, B, s, max_iter, cert, o);
         if (!(g->uncertain && (o->flags & DEGEN))) return;

@@ -31,11 +31,7 @@ namespace sd {
 constexpr int GJK_MAX_VERTS_PER_HULL = 1024;  // key = ia * 1024 + ib
 constexpr float GJK_TOUCH_EPS2 = 1e-14f;      // |v|^2 below this: origin on the boundary
 constexpr float GJK_REL_EPS = 1e-6f;          // sphere shapes: v.v - v.w <= eps v.v ends the query
-constexpr float GJK_STALL_EPS = 1e-4f;
-// A duplicate-support exit with v.w <= 0 is a separation when |v|^2 exceeds
-// this fraction of the largest squared simplex coordinate ((2^-16)^2: |v| is
-// 65,536 times the coordinates' rounding unit of about 2^-24 of their size).
-constexpr float GJK_DUP_REL2 = 0x1p-32f;        // gap accepted when the distance stops decreasing
+constexpr float GJK_STALL_EPS = 1e-4f;         // gap accepted when the distance stops decreasing
 constexpr float EPA_REL_EPS = 1e-6f;          // EPA: support gap <= eps * depth + 1e-9 -> converged
 constexpr float CERT_REL = 1.52587890625e-05f;  // 2^-16: certificate margin relative to the vertex radii
 constexpr float CERT_TAU = 9.765625e-04f;       // 2^-10: closest points this far inside their features
@@ -432,15 +428,6 @@ __device__ __forceinline__ bool simplex_has(const Simplex& s, int key) {
     return dup;
 }
 
-// Largest squared coordinate vector of the simplex points and w.
-__device__ __forceinline__ float simplex_scale2(const Simplex& s, const MPoint& w) {
-    float m = dot(w.p, w.p);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        if (i < s.n) m = fmaxf(m, dot(s.w[i].p, s.w[i].p));
-    return m;
-}
-
 // ---------------------------------------------------------------- GJK loop
 
 struct GjkState {
@@ -454,9 +441,11 @@ __device__ __forceinline__ GjkState gjk_state() { return GjkState{false, false, 
 // support evaluations (added to g.it). SEED: support over the seed sets
 // instead of the full shapes. Exits:
 //  - exact: the support point is already a simplex vertex; in exact
-//    arithmetic v.w >= v.v then, so any gap is rounding and the feature is
-//    final (converged), unless v.w <= 0 (|v| at rounding level: an uncertain
-//    hit for EPA); a sphere shape (keyless) ends on the relative gap instead;
+//    arithmetic v.w >= v.v then, so any gap is rounding. v.w > 0 makes v a
+//    separating axis (converged); v.w <= 0 leaves the verdict to the rounding
+//    of v's direction (an uncertain hit, settled by the caller with
+//    axis_separates or EPA); a sphere shape (keyless) ends on the relative
+//    gap instead;
 //  - inside: the simplex encloses the origin (hit);
 //  - stall: the new simplex is not strictly closer in FP32; the previous
 //    feature is final if the duality gap is within 1e-4 v.v (converged), else
@@ -477,10 +466,7 @@ __device__ __forceinline__ void gjk_loop(const Shape& A, const Shape& B, const S
         const float vw = dot(v, w.p);
         const float gap = vv - vw;
         if (simplex_has(s, w.key) || (w.key < 0 && gap <= GJK_REL_EPS * vv)) {
-            // w is a vertex of the simplex, so v.w >= v.v in exact arithmetic:
-            // v.w <= 0 only reflects the rounding of v. |v| well above the
-            // rounding level of the simplex coordinates is the distance.
-            if (vw > 0.0f || vv > GJK_DUP_REL2 * simplex_scale2(s, w)) {
+            if (vw > 0.0f) {
                 g.converged = true;
             } else {
                 g.hit = true;
@@ -745,6 +731,13 @@ __device__ __forceinline__ void cmpx_key(MPoint& a, MPoint& b) {
         a = b;
         b = t;
     }
+}
+
+// Whether the unit axis n (A -> B) separates A and B: the lowest point of B
+// on n above the highest point of A, from one support pass over both shapes.
+__device__ __forceinline__ bool axis_separates(const Shape& A, const Shape& B, V3 n) {
+    const MPoint m = support_M(A, B, n);
+    return -dot(n, m.p) > 0.0f;
 }
 
 // Result of a separated query from its final simplex: the canonical feature

@@ -626,18 +626,26 @@ __global__ __launch_bounds__(64) void k_gjk_phase(const int* __restrict__ pairs,
         }
         if (full) gjk_loop<false>(A, B, nullptr, nullptr, s, v, limit, g);
         const int it = g.it + extra;
+        GjkOut o = gjk_out_init();
+        if (!g.hit || g.uncertain) {
+            // the separated result; an uncertain hit whose feature normal
+            // separates the shapes is a converged separation, else EPA decides
+            if (WARM && warm_try) separated_result<true>(A, B, s, o);
+            else separated_result<false>(A, B, s, o);
+            if (g.hit && axis_separates(A, B, o.n)) {
+                g.hit = false;
+                g.converged = true;
+            }
+        }
         if (g.hit) {
             push_epa(list, count, p, warm_try ? 1 : 0, g.uncertain, limit, spent + it, s);
             continue;
         }
-        GjkOut o = gjk_out_init();
         o.flags = g.converged ? GJK_CONVERGED : GJK_BUDGET;
         if (WARM && warm_try) {
-            separated_result<true>(A, B, s, o);
             if (warm_accepted(o, near_margin)) write_out(p, o, o.flags | GJK_WARM_USED, it, sep, normal, flags, iters);
             else retry[atomicAdd(retry_count, 1)] = make_int2(p, it);
         } else {
-            separated_result<false>(A, B, s, o);
             write_out(p, o, o.flags, spent + it, sep, normal, flags, iters);
         }
     }

@@ -307,44 +307,69 @@ def test_gjk_sphere_triangle_closed_form(oracle):
         assert np.allclose(res["normal"][k], normals[k], atol=5e-2), k
 
 
-def test_gjk_near_touching_pair_is_separated(oracle):
-    """A pair of the configuration-5 hull scenes (scene 64, hulls 464 and 2984)
-    2.7 um apart: GJK ends on a duplicate support with v.w <= 0 from the
-    rounding of v alone (|v| is thousands of rounding units). It is a
-    separation, cold and warm alike, equal to the exact gap along the normal
-    (float64); the hulls do not intersect (linear-programming feasibility)."""
-    from scipy.optimize import linprog
-
+def c5_pair(scene_index, i, j):
+    """Hulls i and j of a configuration-5 hull scene as a two-hull set, with
+    their world vertices (float64, exact rotation of the stored quaternions)."""
     from stardust_amd import workloads as W
-    h = W.batch_c5().generate_hull_scene(64)
-    sub = {"pos4": h["pos4"][[464, 2984]], "quat4": h["quat4"][[464, 2984]]}
+    h = W.batch_c5().generate_hull_scene(scene_index)
+    sub = {"pos4": h["pos4"][[i, j]], "quat4": h["quat4"][[i, j]]}
     vs, nv, verts = [], [], []
-    for i in (464, 2984):
-        a, n = int(h["vstart"][i]), int(h["nv"][i])
+    for k in (i, j):
+        a, n = int(h["vstart"][k]), int(h["nv"][k])
         vs.append(sum(nv))
         nv.append(n)
         verts.append(h["verts4"][a:a + n])
     sub.update({"vstart": np.array(vs, np.int32), "nv": np.array(nv, np.int32), "verts4": np.concatenate(verts)})
+    return sub, world(sub, 0, np.zeros(3)), world(sub, 1, np.zeros(3))
+
+
+def hulls_intersect(A, B):
+    """Linear-programming feasibility of a convex combination of A's vertices
+    equal to one of B's."""
+    from scipy.optimize import linprog
+    m = len(A) + len(B)
+    aeq = np.zeros((5, m))
+    aeq[:3, :len(A)], aeq[:3, len(A):], aeq[3, :len(A)], aeq[4, len(A):] = A.T, -B.T, 1, 1
+    r = linprog(np.zeros(m), A_eq=aeq, b_eq=[0, 0, 0, 1, 1], bounds=[(0, None)] * m, method="highs")
+    assert r.status in (0, 2)
+    return r.status == 0
+
+
+def test_gjk_near_touching_pair_is_separated(oracle):
+    """A pair of the configuration-5 hull scenes (scene 64, hulls 464 and 2984)
+    2.7 um apart: GJK ends on a duplicate support with v.w <= 0 from the
+    rounding of v alone (|v| is thousands of rounding units). The normal of the
+    simplex's feature result separates the hulls, so it is a separation, cold
+    and warm alike, equal to the exact gap along the normal (float64); the
+    hulls do not intersect (linear-programming feasibility)."""
+    sub, A, B = c5_pair(64, 464, 2984)
     pr = np.array([[0, 1]], np.int32)
     cold = oracle.gjk(sub, pr)
     warm = oracle.gjk(sub, pr, warm=np.array([[0.7831531, -0.30531186, 0.54171556]], np.float32), warm_budget=16)
     assert cold["flags"][0] & 1 == 0 and cold["flags"][0] & 2
     assert cold["sep"][0] == warm["sep"][0] and np.array_equal(cold["normal"], warm["normal"])
-
-    def world(k):
-        w0, x, y, z = sub["quat4"][k].astype(np.float64) / np.linalg.norm(sub["quat4"][k].astype(np.float64))
-        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w0 * z), 2 * (x * z + w0 * y)],
-                      [2 * (x * y + w0 * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w0 * x)],
-                      [2 * (x * z - w0 * y), 2 * (y * z + w0 * x), 1 - 2 * (x * x + y * y)]])
-        v = sub["verts4"][vs[k]:vs[k] + nv[k], :3].astype(np.float64)
-        return v @ R.T + sub["pos4"][k, :3].astype(np.float64)
-
-    A, B = world(0), world(1)
     n = cold["normal"][0].astype(np.float64)
     gap = (B @ n).min() - (A @ n).max()
     assert abs(cold["sep"][0] - gap) < 1e-8, (cold["sep"][0], gap)
-    m = len(A) + len(B)
-    aeq = np.zeros((5, m))
-    aeq[:3, :len(A)], aeq[:3, len(A):], aeq[3, :len(A)], aeq[4, len(A):] = A.T, -B.T, 1, 1
-    r = linprog(np.zeros(m), A_eq=aeq, b_eq=[0, 0, 0, 1, 1], bounds=[(0, None)] * m, method="highs")
-    assert r.status == 2  # infeasible: the hulls are disjoint
+    assert not hulls_intersect(A, B)
+    assert abs(cold["sep"][0] - exact_signed_distance(sub, 0, 1)) < 1e-8
+
+
+def test_gjk_shallow_penetration_is_a_hit(oracle):
+    """A pair of the configuration-5 hull scenes (scene 6, hulls 1370 and 4873)
+    that overlap by 6.3 um: cold GJK ends on a duplicate support with v.w <= 0
+    and |v| = 0.79 um, a triangle that is not on the boundary of A - B; the
+    normal of its feature result does not separate the hulls, so EPA decides:
+    a hit, cold and warm alike, whose depth is the exact penetration depth
+    (Qhull facets of A - B, float64); the hulls intersect (linear-programming
+    feasibility)."""
+    sub, A, B = c5_pair(6, 1370, 4873)
+    pr = np.array([[0, 1]], np.int32)
+    cold = oracle.gjk(sub, pr)
+    warm = oracle.gjk(sub, pr, warm=np.array([[0.02021875, -0.8239702, -0.5662723]], np.float32), warm_budget=16)
+    assert cold["flags"][0] & 1 and warm["flags"][0] & 1
+    assert cold["sep"][0] == warm["sep"][0] and np.array_equal(cold["normal"], warm["normal"])
+    assert hulls_intersect(A, B)
+    exact = exact_signed_distance(sub, 0, 1)
+    assert exact < -6e-6
+    assert abs(cold["sep"][0] - exact) < 1e-8, (cold["sep"][0], exact)
