@@ -473,10 +473,14 @@ def batch_section(args):
 
 
 def batch_hulls(args, w, b, sphere_s):
-    """The hull-carrying scenes of batch_c5 as rigid bodies (hulls on a lattice
-    in walled boxes) attached to the batch: one graph-replayed step advances
-    the spheres and the bodies together (the body step on a forked stream);
-    the bodies' context is also timed alone."""
+    """The hull-carrying scenes of batch_c5 as rigid bodies (hulls on a 20 mm
+    lattice in walled boxes, about 0.5 contacts per hull) attached to the
+    batch: one graph-replayed step advances the spheres and the bodies together
+    (the body step on a forked stream), its GJK queries warm-started by the
+    learned network. The bodies' context is also timed alone, cold and warm,
+    from the same state; the warm and cold steps' narrowphase verdicts and
+    contact sets are compared."""
+    import numpy as np
     from stardust_amd import _lib as L
     from stardust_amd import hullbody as HB
     t0 = time.perf_counter()
@@ -484,38 +488,66 @@ def batch_hulls(args, w, b, sphere_s):
     n_hulls = int((bodies["nv"] > 3).sum())
     log("batch: %d hull scenes, %d bodies (%d hulls) generated in %.1f s" % (len(first) - 1, len(bodies["pos4"]),
                                                                           n_hulls, time.perf_counter() - t0))
+    model, _ = trained_model(args, 131072)
     c = L.ContactParams.make(2e5, 0.3, 0.5, w.dt, gravity=(0.0, 0.0, -9.81), walls=None, max_contacts=16)
+    nsteps = args.batch_hull_steps
+
+    def timed_alone(hs):
+        L.synchronize(None)
+        t0 = time.perf_counter()
+        hs.step(nsteps)  # returns after its stream is synchronised
+        return (time.perf_counter() - t0) / nsteps
+
+    def stats(hs):
+        st = hs.narrowphase_stats()
+        st["iters_mean"] = round(st["iters"] / max(st["pairs"], 1), 3)
+        return st
+
+    cold = HB.HullBodyScene(c)
+    cold.upload(bodies, first)
+    cold.step(1)
+    first_cold = (stats(cold), cold.contacts()[0])
+    cold.step(1)
+    alone_cold = timed_alone(cold)
+    cold.close()
     hs = HB.HullBodyScene(c)
     hs.upload(bodies, first)
-    hs.step(2)
+    hs.set_warm(model, warm_budget=16)
+    hs.step(1)
+    first_warm = (stats(hs), hs.contacts()[0])
+    same = (first_cold[0]["pairs"] == first_warm[0]["pairs"] and first_cold[0]["hits"] == first_warm[0]["hits"] and
+            np.array_equal(first_cold[1], first_warm[1]))
+    log("batch: bodies' first step cold %s, warm %s; same pairs, verdicts and contacts: %s" % (
+        first_cold[0], first_warm[0], same))
+    hs.step(1)
+    alone_warm = timed_alone(hs)
     ev = event_pair()
-    L.synchronize(None)
-    t0 = time.perf_counter()
-    hs.step(args.batch_hull_steps)
-    alone = (time.perf_counter() - t0) / args.batch_hull_steps
     b.attach_hulls(hs)
     b.step(2)
     bstream = b.stream()
     L.synchronize(bstream)
     t0 = time.perf_counter()
     L.call("sd_event_record", ev[0], bstream)
-    b.step(args.batch_hull_steps)
+    b.step(nsteps)
     L.call("sd_event_record", ev[1], bstream)
     L.synchronize(bstream)
     wall = time.perf_counter() - t0
-    both = max(wall, elapsed_ms(ev) / 1000.0) / args.batch_hull_steps
+    both = max(wall, elapsed_ms(ev) / 1000.0) / nsteps
     ncont = len(hs.contacts()[0])
+    last = stats(hs)
     b.attach_hulls(None)
     hs.close()
-    log("batch: with hull scenes %.3f ms/step (spheres alone %.3f, bodies alone %.3f); %d body contacts" % (
-        1000 * both, 1000 * sphere_s, 1000 * alone, ncont))
+    log("batch: with hull scenes %.3f ms/step (spheres alone %.3f, bodies alone cold %.3f, warm %.3f); %d body "
+        "contacts" % (1000 * both, 1000 * sphere_s, 1000 * alone_cold, 1000 * alone_warm, ncont))
     nb = len(bodies["pos4"])
-    return {"hull_scenes": len(first) - 1, "bodies": nb, "hulls": n_hulls, "steps": args.batch_hull_steps,
+    return {"hull_scenes": len(first) - 1, "bodies": nb, "hulls": n_hulls, "steps": nsteps,
             "ms_per_step": round(1000 * both, 4), "spheres_alone_ms_per_step": round(1000 * sphere_s, 4),
-            "bodies_alone_ms_per_step": round(1000 * alone, 4),
+            "bodies_alone_ms_per_step": {"cold": round(1000 * alone_cold, 4), "warm": round(1000 * alone_warm, 4)},
             "particle_steps_per_s": w.n / both, "body_steps_per_s": nb / both,
-            "body_contacts_last_step": ncont,
-            "form": "the bodies' step forked onto its own stream inside the batch's captured step graph"}
+            "body_contacts_last_step": ncont, "narrowphase_last_step": last,
+            "first_step": {"cold": first_cold[0], "warm": first_warm[0], "same_pairs_verdicts_contacts": same},
+            "form": "the bodies' step (learned warm start) forked onto its own stream inside the batch's captured "
+                    "step graph"}
 
 
 def calibrate_copy(nbytes=256 << 20, reps=5):

@@ -424,8 +424,9 @@ sd_status sd_lbvh_query(const uint64_t* sorted_keys, int n, const int* left, con
  * and static triangles (three vertices, mass 0: the faces of a DEMMesh,
  * src/engine/entities/entity.hpp:64-93) stepped on the GPU in independent
  * scenes. Each step: batched sweep-and-prune over the body boxes (each scene on
- * its own axis), cold GJK/EPA of every candidate pair (the sd_gjk results:
- * TRIANGLE and SPHERE shapes of sandbox/primitives.hpp:11-14 included), and
+ * its own axis), GJK/EPA of every candidate pair (the sd_gjk results: TRIANGLE
+ * and SPHERE shapes of sandbox/primitives.hpp:11-14 included; cold, or
+ * warm-started by a learned network, sd_hull_set_warm), and
  * for each touching pair (sep < 0) a contact between the two contact
  * features (a hull's feature: its vertices within 1e-3 of its radius of the
  * extreme one along the normal, averaged; a sphere's surface point), placed at
@@ -466,6 +467,21 @@ sd_status sd_hull_upload(sd_hull_ctx* ctx, int n, const float* pos4, const float
 sd_status sd_hull_step(sd_hull_ctx* ctx, int nsteps);
 /* Body state: pos4 (x, y, z, 0), vel4, quat4, angmom4 (world L); any may be NULL. */
 sd_status sd_hull_download(sd_hull_ctx* ctx, float* pos4, float* vel4, float* quat4, float* angmom4);
+/* Learned warm start of the step's GJK queries: packed_model is a device
+ * buffer of sd_mlp_packed_bytes() bytes written by sd_mlp_pack_bf16, kept
+ * alive by the caller while attached; NULL returns to cold queries. Each step
+ * then computes the pair features, infers the directions (BF16 MFMA) and runs
+ * the warm-started queries of sd_gjk (warm_budget iterations, near margin
+ * 1e-6 m), redoing cold every query without a converged, certified result.
+ * No counterpart in the reference (its GJK starts from the centre difference,
+ * sandbox/testGJK.cu:266). */
+sd_status sd_hull_set_warm(sd_hull_ctx* ctx, const void* packed_model, int warm_budget);
+/* Narrowphase of the last step (diagnostics, copies to the host): candidate
+ * pairs, intersecting pairs, pairs whose result came from the warm-started
+ * query, and the support passes of GJK and EPA summed over the pairs (as
+ * sd_gjk's iters); any may be NULL. */
+sd_status sd_hull_narrowphase_stats(sd_hull_ctx* ctx, int* n_pairs, int* n_hits, int* n_warm_used,
+                                    long long* iters_total);
 /* Touching pairs of the last step in pair order: (a, b), the force on b
  * (x, y, z, 1; -force on a) and the contact point (x, y, z, 0). */
 sd_status sd_hull_contacts(sd_hull_ctx* ctx, int* pairs, float* force4, float* point4, int capacity,

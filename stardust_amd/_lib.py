@@ -173,6 +173,9 @@ _SIGS = {
     "sd_hull_set_params": [_P, _P],
     "sd_hull_upload": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _I],
     "sd_hull_step": [_P, _I],
+    "sd_hull_set_warm": [_P, _P, _I],
+    "sd_hull_narrowphase_stats": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_longlong)],
     "sd_hull_download": [_P, _P, _P, _P, _P],
     "sd_hull_contacts": [_P, _P, _P, _P, _I, C.POINTER(C.c_int)],
     "sd_clump_state_bytes": [_P, C.POINTER(_SZ)],

@@ -146,6 +146,7 @@ struct sd_ctx {
     // rigid-body scenes stepped with the batch (sd_batch_attach_hulls), on a
     // stream of their own forked from and joined to the step
     sd_hull_ctx* hulls = nullptr;
+    long long hull_gen = 0;  // hull_generation(hulls) when the step graphs were captured
     // persistent form of the fused step kernel (sd_ctx_set_persistent)
     bool persistent = false;
     int* pcounters = nullptr;
@@ -370,6 +371,10 @@ sd_status run_graph_pair(sd_ctx* c, bool full = false) {
 // ran, which sets the buffer roles; one synchronisation per launch sequence.
 sd_status run_steps(sd_ctx* c, int nsteps) {
     sd_status st;
+    if (c->hulls && hull_generation(c->hulls) != c->hull_gen) {
+        drop_graphs(c);  // captured with the attached bodies' previous step
+        c->hull_gen = hull_generation(c->hulls);
+    }
     if (!c->p.key64) {
         int left = nsteps;
         if (c->p.use_graph) {
@@ -514,6 +519,7 @@ extern "C" sd_status sd_batch_attach_hulls(sd_ctx* c, sd_hull_ctx* h) {
     (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);  // the step graphs are captured again with (or without) the bodies
     c->hulls = h;
+    if (h) c->hull_gen = hull_generation(h);
     return SD_OK;
 }
 
@@ -940,48 +946,29 @@ extern "C" sd_status sd_ctx_autotune(sd_ctx* c, int trial_steps, int* tile_items
     float best = 3.4e38f;
     int bi = c->items, bb = c->contact_block;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = SD_ERR_HIP;
+    // each trial runs the steps as a run does (run_steps: graph pairs, a
+    // given-up re-sort step redone with the full sort, dense chunks of
+    // full-sort pairs) from the snapshot, after a warm-up of three steps that
+    // makes the full sort the restore asks for and captures the trial's graphs
     for (int i = 0; i < 3 && st == SD_OK; ++i) {
         for (int j = 0; j < 3 && st == SD_OK; ++j) {
             c->items = items_c[i];
             c->contact_block = block_c[j];
+            drop_graphs(c);  // captured with other launch shapes
             st = snapshot_restore(c, sn);
-            if (st == SD_OK) st = enqueue_step(c, nullptr, c->need_full);  // warm-up (a full sort after the restore)
-            c->need_full = false;
+            if (st == SD_OK) st = run_steps(c, 3);
             if (st != SD_OK) break;
-            // a dense scene (more changed cells than the re-sort takes) is timed
-            // with the full-sort steps it runs
-            bool full_trials = false;
-            if (c->p.key64) {
-                int m = 0;
-                if (hipMemcpyAsync(&m, c->mcount + c->cur_hist, sizeof(int), hipMemcpyDeviceToHost, c->stream) !=
-                        hipSuccess ||
-                    hipStreamSynchronize(c->stream) != hipSuccess) {
-                    st = set_error(SD_ERR_HIP, "sd_ctx_autotune: read failed");
-                    break;
-                }
-                full_trials = m > RESORT_CAP;
-            }
             if (hipEventRecord(e0, c->stream) != hipSuccess) {
                 st = set_error(SD_ERR_HIP, "sd_ctx_autotune: event record failed");
                 break;
             }
-            for (int k = 0; k < trial_steps && st == SD_OK; ++k) st = enqueue_step(c, nullptr, full_trials);
+            st = run_steps(c, trial_steps);
             if (st != SD_OK) break;
             float ms = 0.0f;
             if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
                 st = set_error(SD_ERR_HIP, "sd_ctx_autotune: event timing failed");
                 break;
-            }
-            if (c->p.key64) {  // a given-up step makes the trial's time meaningless: keep the current shapes
-                int gave_up = 0;
-                if (hipMemcpyAsync(&gave_up, c->step_ctl, sizeof(int), hipMemcpyDeviceToHost, c->stream) !=
-                        hipSuccess ||
-                    hipStreamSynchronize(c->stream) != hipSuccess) {
-                    st = set_error(SD_ERR_HIP, "sd_ctx_autotune: read failed");
-                    break;
-                }
-                if (gave_up) ms = 3.4e38f;
             }
             if (ms < best) {
                 best = ms;

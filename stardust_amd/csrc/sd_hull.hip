@@ -532,9 +532,9 @@ struct SeedState {
 // does not read their state).
 __global__ __launch_bounds__(64) void k_gjk_seed(const int* __restrict__ pairs, int npairs, HullSet h, int n_hulls,
                                                  int n_verts, const float* __restrict__ warm,
-                                                 SeedState* __restrict__ state) {
+                                                 SeedState* __restrict__ state, const int* __restrict__ dcount) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+    if (p >= (dcount ? min(*dcount, npairs) : npairs)) return;
     const int ia = pairs[2 * (size_t)p], ib = pairs[2 * (size_t)p + 1];
     if (pair_is_bad(h, ia, ib, n_hulls, n_verts)) return;
     Shape A, B;
@@ -950,6 +950,61 @@ extern "C" sd_status sd_sap_pairs_batched(const float* lo4, const float* hi4, in
     return SD_OK;
 }
 
+namespace {
+
+// GJK/EPA of the first min(*dcount, capacity) pairs (all capacity pairs when
+// dcount is null), warm-started from warm (world directions, A -> B) when it
+// is given; every launch is sized to capacity. scratch: counters (EPA list,
+// retry list, second EPA list), the EPA list, the retry list, the seed states
+// (gjk_scratch_bytes(capacity)); the second EPA list reuses the first one's
+// memory.
+sd_status enqueue_gjk(const int* pairs, int capacity, const int* dcount, const HullSet& h, int n_hulls,
+                      int n_verts, const float* warm, int warm_budget, int max_iter, float near_margin, float* sep,
+                      float* normal3, int* flags, int* iters, void* scratch, hipStream_t s) {
+    if (capacity == 0) return SD_OK;
+    int* count = (int*)scratch;
+    EpaEntry* list = (EpaEntry*)((char*)scratch + 256);
+    int2* retry = (int2*)((char*)scratch + 256 + (size_t)capacity * sizeof(EpaEntry));
+    SeedState* seed = (SeedState*)((char*)retry + (size_t)capacity * sizeof(int2));
+    sd_status st = launch_fill_u32((uint32_t*)count, 0u, 3, s);
+    if (st != SD_OK) return st;
+    const int epa_blocks = std::min(div_up(capacity, EPA_GROUPS_PER_BLOCK), epa_resident_blocks());
+    const dim3 epa_block(64 * EPA_WAVES_PER_BLOCK);
+    const dim3 grid(div_up(capacity, 64));
+    if (warm != nullptr) {
+        hipLaunchKernelGGL(k_gjk_seed, grid, dim3(64), 0, s, pairs, capacity, h, n_hulls, n_verts, warm, seed, dcount);
+        SD_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_gjk_phase<false, true>), grid, dim3(64), 0, s, pairs, capacity, h, n_hulls, n_verts,
+                           warm, warm_budget, max_iter, near_margin, sep, normal3, flags, iters, list, count, retry,
+                           count + 1, (const SeedState*)seed, dcount);
+    } else {
+        hipLaunchKernelGGL((k_gjk_phase<false, false>), grid, dim3(64), 0, s, pairs, capacity, h, n_hulls, n_verts,
+                           warm, warm_budget, max_iter, near_margin, sep, normal3, flags, iters, list, count, retry,
+                           count + 1, (const SeedState*)nullptr, dcount);
+    }
+    SD_LAUNCH_CHECK();
+    const int fin_blocks = std::min(div_up(capacity, 64), GJK_RETRY_MAX_BLOCKS);
+    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)count);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_epa_finish, dim3(fin_blocks), dim3(64), 0, s, pairs, h, near_margin, sep, normal3, flags,
+                       iters, (const EpaEntry*)list, (const int*)count, retry, count + 1);
+    SD_LAUNCH_CHECK();
+    if (warm == nullptr) return SD_OK;  // cold entries are never rejected
+    // the rejected warm queries, cold
+    hipLaunchKernelGGL((k_gjk_phase<true, false>), dim3(fin_blocks), dim3(64), 0, s, pairs, capacity, h, n_hulls,
+                       n_verts, nullptr, 1, max_iter, near_margin, sep, normal3, flags, iters, list, count + 2, retry,
+                       count + 1, (const SeedState*)nullptr, (const int*)nullptr);
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)(count + 2));
+    SD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_epa_finish, dim3(fin_blocks), dim3(64), 0, s, pairs, h, near_margin, sep, normal3, flags,
+                       iters, (const EpaEntry*)list, (const int*)(count + 2), retry, count + 1);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+}  // namespace
+
 extern "C" sd_status sd_gjk_scratch_bytes(int n_pairs, size_t* bytes) {
     SD_REQUIRE(bytes && n_pairs >= 0, "sd_gjk_scratch_bytes: bad argument");
     *bytes = gjk_scratch_bytes(n_pairs);
@@ -969,49 +1024,10 @@ extern "C" sd_status sd_gjk(const int* pairs, int n_pairs, const float* pos4, co
                gjk_scratch_bytes(n_pairs));
     SD_REQUIRE(max_iter >= 1 && max_iter <= 1024, "sd_gjk: max_iter must be in [1, 1024]");
     SD_REQUIRE(warm_dir == nullptr || warm_budget >= 1, "sd_gjk: warm_budget must be >= 1");
-    HullSet h{(const float4*)pos4, (const float4*)quat4, vstart, nv, (const float4*)verts4};
-    hipStream_t s = (hipStream_t)stream;
-    // scratch: counters (EPA list, retry list, second EPA list), the EPA list,
-    // the retry list, the seed states; the second EPA list reuses the first
-    // one's memory
-    int* count = (int*)scratch;
-    EpaEntry* list = (EpaEntry*)((char*)scratch + 256);
-    int2* retry = (int2*)((char*)scratch + 256 + (size_t)n_pairs * sizeof(EpaEntry));
-    SeedState* seed = (SeedState*)((char*)retry + (size_t)n_pairs * sizeof(int2));
-    st = launch_fill_u32((uint32_t*)count, 0u, 3, s);
-    if (st != SD_OK) return st;
-    const int epa_blocks = std::min(div_up(n_pairs, EPA_GROUPS_PER_BLOCK), epa_resident_blocks());
-    const dim3 epa_block(64 * EPA_WAVES_PER_BLOCK);
-    if (warm_dir != nullptr) {
-        hipLaunchKernelGGL(k_gjk_seed, dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h, n_hulls, n_verts,
-                           warm_dir, seed);
-        SD_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_gjk_phase<false, true>), dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h,
-                           n_hulls, n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters,
-                           list, count, retry, count + 1, (const SeedState*)seed, (const int*)nullptr);
-    } else {
-        hipLaunchKernelGGL((k_gjk_phase<false, false>), dim3(div_up(n_pairs, 64)), dim3(64), 0, s, pairs, n_pairs, h,
-                           n_hulls, n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters,
-                           list, count, retry, count + 1, (const SeedState*)nullptr, (const int*)nullptr);
-    }
-    SD_LAUNCH_CHECK();
-    const int fin_blocks = std::min(div_up(n_pairs, 64), GJK_RETRY_MAX_BLOCKS);
-    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)count);
-    SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_epa_finish, dim3(fin_blocks), dim3(64), 0, s, pairs, h, near_margin, sep, normal3, flags,
-                       iters, (const EpaEntry*)list, (const int*)count, retry, count + 1);
-    SD_LAUNCH_CHECK();
-    if (warm_dir == nullptr) return SD_OK;  // cold entries are never rejected
-    hipLaunchKernelGGL((k_gjk_phase<true, false>), dim3(fin_blocks), dim3(64), 0, s, pairs, n_pairs, h, n_hulls,
-                       n_verts, nullptr, 1, max_iter, near_margin, sep, normal3, flags, iters, list, count + 2, retry,
-                       count + 1, (const SeedState*)nullptr, (const int*)nullptr);
-    SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)(count + 2));
-    SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_epa_finish, dim3(fin_blocks), dim3(64), 0, s, pairs, h, near_margin, sep, normal3, flags,
-                       iters, (const EpaEntry*)list, (const int*)(count + 2), retry, count + 1);
-    SD_LAUNCH_CHECK();
-    return SD_OK;
+    return enqueue_gjk(pairs, n_pairs, nullptr,
+                       HullSet{(const float4*)pos4, (const float4*)quat4, vstart, nv, (const float4*)verts4}, n_hulls,
+                       n_verts, warm_dir, warm_budget, max_iter, near_margin, sep, normal3, flags, iters, scratch,
+                       (hipStream_t)stream);
 }
 
 namespace sd {
@@ -1086,26 +1102,16 @@ sd_status enqueue_gjk_cold(const int* pairs, int capacity, const int* dcount, co
                            const float4* quat4, const int* vstart, const int* nv, const float4* verts4, int n_hulls,
                            int n_verts, int max_iter, float near_margin, float* sep, float* normal3, int* flags,
                            int* iters, void* scratch, hipStream_t s) {
-    if (capacity == 0) return SD_OK;
-    HullSet h{pos4, quat4, vstart, nv, verts4};
-    int* count = (int*)scratch;
-    EpaEntry* list = (EpaEntry*)((char*)scratch + 256);
-    int2* retry = (int2*)((char*)scratch + 256 + (size_t)capacity * sizeof(EpaEntry));
-    sd_status st = launch_fill_u32((uint32_t*)count, 0u, 3, s);
-    if (st != SD_OK) return st;
-    const int epa_blocks = std::min(div_up(capacity, EPA_GROUPS_PER_BLOCK), epa_resident_blocks());
-    const dim3 epa_block(64 * EPA_WAVES_PER_BLOCK);
-    hipLaunchKernelGGL((k_gjk_phase<false, false>), dim3(div_up(capacity, 64)), dim3(64), 0, s, pairs, capacity, h,
-                       n_hulls, n_verts, (const float*)nullptr, 1, max_iter, near_margin, sep, normal3, flags, iters,
-                       list, count, retry, count + 1, (const SeedState*)nullptr, dcount);
-    SD_LAUNCH_CHECK();
-    const int fin_blocks = std::min(div_up(capacity, 64), GJK_RETRY_MAX_BLOCKS);
-    hipLaunchKernelGGL(k_epa_wave, dim3(epa_blocks), epa_block, 0, s, pairs, h, list, (const int*)count);
-    SD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_epa_finish, dim3(fin_blocks), dim3(64), 0, s, pairs, h, near_margin, sep, normal3, flags,
-                       iters, (const EpaEntry*)list, (const int*)count, retry, count + 1);
-    SD_LAUNCH_CHECK();
-    return SD_OK;
+    return enqueue_gjk(pairs, capacity, dcount, HullSet{pos4, quat4, vstart, nv, verts4}, n_hulls, n_verts, nullptr,
+                       1, max_iter, near_margin, sep, normal3, flags, iters, scratch, s);
+}
+
+sd_status enqueue_gjk_warm(const int* pairs, int capacity, const int* dcount, const float4* pos4,
+                           const float4* quat4, const int* vstart, const int* nv, const float4* verts4, int n_hulls,
+                           int n_verts, const float* warm_dir3, int warm_budget, int max_iter, float near_margin,
+                           float* sep, float* normal3, int* flags, int* iters, void* scratch, hipStream_t s) {
+    return enqueue_gjk(pairs, capacity, dcount, HullSet{pos4, quat4, vstart, nv, verts4}, n_hulls, n_verts,
+                       warm_dir3, warm_budget, max_iter, near_margin, sep, normal3, flags, iters, scratch, s);
 }
 
 size_t gjk_cold_scratch_bytes(int capacity) { return gjk_scratch_bytes(capacity); }

@@ -4,7 +4,9 @@
 //   pose      shape centre (x, y, z, radius) and world angular velocity
 //   boxes     k_hull_aabb
 //   pairs     batched sweep-and-prune (each scene on its own axis)
-//   narrow    cold GJK/EPA of every candidate pair
+//   narrow    GJK/EPA of every candidate pair: cold, or warm-started from
+//             the directions of an attached learned model (sd_hull_set_warm),
+//             the rejected warm queries redone cold
 //   contacts  per touching pair (sep < 0): contact point, linear
 //             spring-dashpot normal force, Coulomb-capped viscous friction
 //   gather    per body, its contacts in pair order (no float atomics), then
@@ -32,6 +34,9 @@ namespace sd {
 // (its largest vertex distance from the centre) of its extreme one along the
 // contact normal form the contact feature.
 constexpr float HB_FEATURE_REL = 1e-3f;
+// A warm-started query is kept only beyond this distance from touching (m);
+// nearer ones are redone cold.
+constexpr float HB_WARM_NEAR_MARGIN = 1e-6f;
 
 __global__ void k_hb_pose(const float4* __restrict__ P, const float4* __restrict__ Q, const float4* __restrict__ L,
                           const float4* __restrict__ I0, const float4* __restrict__ I1,
@@ -154,25 +159,29 @@ __global__ void k_hb_pairs(const int* __restrict__ pairs, int capacity, const in
     }
     pf[t] = make_float4(f.x, f.y, f.z, 1.0f);
     pp[t] = make_float4(pt.x, pt.y, pt.z, 0.0f);
-    atomicAdd(&cnt[a], 1);
-    atomicAdd(&cnt[b], 1);
+    // static bodies (1/m = 0) do not move: their contacts are not listed
+    if (Pa.w > 0.0f) atomicAdd(&cnt[a], 1);
+    if (Pb.w > 0.0f) atomicAdd(&cnt[b], 1);
 }
 
-// Lists every touching pair under both bodies: entry 2 p (body a) or 2 p + 1
-// (body b) at offs[body] + a slot; slot order is arbitrary (the gather orders
-// the entries).
+// Lists every touching pair under each of its moving bodies: entry 2 p (body
+// a) or 2 p + 1 (body b) at offs[body] + a slot; slot order is arbitrary (the
+// gather orders the entries).
 __global__ void k_hb_fill(const int* __restrict__ pairs, int capacity, const int* __restrict__ total,
-                          const float4* __restrict__ pf, const int* __restrict__ offs, int* __restrict__ fillc,
-                          int* __restrict__ list) {
+                          const float4* __restrict__ P, const float4* __restrict__ pf, const int* __restrict__ offs,
+                          int* __restrict__ fillc, int* __restrict__ list) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= min(*total, capacity) || !(pf[t].w > 0.0f)) return;
     const int a = pairs[2 * (size_t)t], b = pairs[2 * (size_t)t + 1];
-    list[offs[a] + atomicAdd(&fillc[a], 1)] = 2 * t;
-    list[offs[b] + atomicAdd(&fillc[b], 1)] = 2 * t + 1;
+    if (P[a].w > 0.0f) list[offs[a] + atomicAdd(&fillc[a], 1)] = 2 * t;
+    if (P[b].w > 0.0f) list[offs[b] + atomicAdd(&fillc[b], 1)] = 2 * t + 1;
 }
 
-// Per body: F = sum f, T = sum (p - X) x f over its contacts in ascending
-// entry order (f = +pf on b, -pf on a), then the integration step.
+// Per moving body: F = sum f, T = sum (p - X) x f over its contacts in
+// ascending entry order (f = +pf on b, -pf on a), then the integration step.
+// A static body has no listed contacts (F = T = 0) and is left as it is: a
+// wall triangle under a whole layer of hulls would otherwise order thousands
+// of entries on one thread.
 __global__ void k_hb_gather_integrate(int n, const int* __restrict__ offs, const int* __restrict__ cnt,
                                       const int* __restrict__ list, const float4* __restrict__ pf,
                                       const float4* __restrict__ pp, float4* __restrict__ P, float4* __restrict__ V,
@@ -242,6 +251,14 @@ struct sd_hull_ctx {
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_obj = nullptr;
     long long steps_done = 0;
+    // learned warm start (sd_hull_set_warm): the caller's packed network, the
+    // hull descriptors, the pair features and the predicted directions
+    const void* model = nullptr;
+    int warm_budget = 16;
+    float *desc = nullptr, *X = nullptr, *wdir = nullptr;
+    // changes whenever a captured step would be stale (upload, parameters,
+    // warm start): a batch that captured this step captures it again
+    long long generation = 0;
 };
 
 namespace {
@@ -253,14 +270,36 @@ sd_status halloc(T** p, size_t count) {
 }
 
 void hb_drop_graph(sd_hull_ctx* c) {
+    ++c->generation;
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->graph_obj) (void)hipGraphDestroy(c->graph_obj);
     c->graph = nullptr;
     c->graph_obj = nullptr;
 }
 
+void hb_free_warm(sd_hull_ctx* c) {
+    for (float** q : {&c->desc, &c->X, &c->wdir}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+}
+
+// The warm-start buffers of the uploaded bodies and their descriptors (the
+// shapes are fixed at upload).
+sd_status hb_alloc_warm(sd_hull_ctx* c) {
+    hb_free_warm(c);
+    sd_status st = halloc(&c->desc, 7 * (size_t)c->n);
+    if (st == SD_OK) st = halloc(&c->X, 22 * (size_t)c->capacity);
+    if (st == SD_OK) st = halloc(&c->wdir, 3 * (size_t)c->capacity);
+    if (st == SD_OK) st = launch_hull_descriptors(c->spos, c->vstart, c->nv, c->verts, c->n, c->desc, c->stream);
+    if (st == SD_OK) SD_HIP(hipStreamSynchronize(c->stream));
+    if (st != SD_OK) hb_free_warm(c);
+    return st;
+}
+
 void hb_free(sd_hull_ctx* c) {
     hb_drop_graph(c);
+    hb_free_warm(c);
     void* bufs[] = {c->P,        c->V,        c->Q,      c->L,     c->I0,          c->I1,    c->spos,   c->omega,
                     c->lo,       c->hi,       c->verts,  c->radius, c->vstart,     c->nv,    c->scene_of,
                     c->scene_first, c->pairs, c->sap_scratch, c->sep, c->normal,   c->flags, c->iters,
@@ -308,8 +347,16 @@ sd_status hull_enqueue_step(sd_hull_ctx* c, hipStream_t s) {
     st = enqueue_sap_batched(c->lo, c->hi, n, c->scene_of, c->scene_first, c->nscenes, c->pairs, cap, &c->total,
                              &c->sap_error, c->sap_scratch, c->sap_scratch_bytes, s);
     if (st != SD_OK) return st;
-    st = enqueue_gjk_cold(c->pairs, cap, c->total, c->spos, c->Q, c->vstart, c->nv, c->verts, n, c->n_verts, mi,
-                          0.0f, c->sep, c->normal, c->flags, c->iters, c->gjk_scratch, s);
+    if (c->model) {
+        st = enqueue_warm_dirs(c->pairs, cap, c->total, c->spos, c->Q, c->desc, c->X, c->model, c->wdir, s);
+        if (st == SD_OK)
+            st = enqueue_gjk_warm(c->pairs, cap, c->total, c->spos, c->Q, c->vstart, c->nv, c->verts, n, c->n_verts,
+                                  c->wdir, std::min(c->warm_budget, mi), mi, HB_WARM_NEAR_MARGIN, c->sep, c->normal,
+                                  c->flags, c->iters, c->gjk_scratch, s);
+    } else {
+        st = enqueue_gjk_cold(c->pairs, cap, c->total, c->spos, c->Q, c->vstart, c->nv, c->verts, n, c->n_verts, mi,
+                              0.0f, c->sep, c->normal, c->flags, c->iters, c->gjk_scratch, s);
+    }
     if (st != SD_OK) return st;
     st = launch_fill_u32((uint32_t*)c->cnt, 0u, n, s);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->fillc, 0u, n, s);
@@ -322,7 +369,8 @@ sd_status hull_enqueue_step(sd_hull_ctx* c, hipStream_t s) {
     st = exclusive_scan_i32(c->cnt, c->offs, n, nullptr, c->scan_tmp, s);
     if (st != SD_OK) return st;
     hipLaunchKernelGGL(k_hb_fill, dim3(div_up(cap, 256)), dim3(256), 0, s, (const int*)c->pairs, cap,
-                       (const int*)c->total, (const float4*)c->pf, (const int*)c->offs, c->fillc, c->list);
+                       (const int*)c->total, (const float4*)c->P, (const float4*)c->pf, (const int*)c->offs, c->fillc,
+                       c->list);
     SD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_hb_gather_integrate, dim3(div_up(n, 256)), dim3(256), 0, s, n, (const int*)c->offs,
                        (const int*)c->cnt, (const int*)c->list, (const float4*)c->pf, (const float4*)c->pp, c->P,
@@ -348,6 +396,7 @@ sd_status hull_check_errors(sd_hull_ctx* c) {
 }
 
 int hull_body_count(const sd_hull_ctx* c) { return c->n; }
+long long hull_generation(const sd_hull_ctx* c) { return c->generation; }
 float hull_dt(const sd_hull_ctx* c) { return c->p.contact.dt; }
 void hull_count_steps(sd_hull_ctx* c, int nsteps) { c->steps_done += nsteps; }
 
@@ -495,9 +544,35 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     SD_HIP(hipMemcpyAsync(c->scene_first, sfirst.data(), (nscenes + 1) * sizeof(int), hipMemcpyHostToDevice, s));
     st = launch_fill_u32((uint32_t*)c->err, 0u, 1, s);
     if (st != SD_OK) return st;
+    // spos (centre, radius) for the descriptors, as the step's pose stage writes it
+    std::vector<float4> sp(n);
+    for (int i = 0; i < n; ++i) sp[i] = make_float4(P[i].x, P[i].y, P[i].z, radius[i]);
+    SD_HIP(hipMemcpyAsync(c->spos, sp.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
     SD_HIP(hipStreamSynchronize(s));
     c->steps_done = 0;
+    if (c->model) {
+        st = hb_alloc_warm(c);
+        if (st != SD_OK) {
+            hb_free(c);
+            return st;
+        }
+    }
     return SD_OK;
+}
+
+extern "C" sd_status sd_hull_set_warm(sd_hull_ctx* c, const void* packed_model, int warm_budget) {
+    SD_REQUIRE(c, "sd_hull_set_warm: NULL context");
+    SD_REQUIRE(packed_model == nullptr || (warm_budget >= 1 && warm_budget <= 1024),
+               "sd_hull_set_warm: warm_budget must be in [1, 1024]");
+    (void)hipStreamSynchronize(c->stream);
+    hb_drop_graph(c);
+    c->model = packed_model;
+    c->warm_budget = packed_model ? warm_budget : 16;
+    if (!packed_model) {
+        hb_free_warm(c);
+        return SD_OK;
+    }
+    return c->n ? hb_alloc_warm(c) : SD_OK;
 }
 
 extern "C" sd_status sd_hull_step(sd_hull_ctx* c, int nsteps) {
@@ -581,5 +656,33 @@ extern "C" sd_status sd_hull_contacts(sd_hull_ctx* c, int* pairs, float* force4,
     }
     *n_contacts = k;
     if (k > capacity) return set_error(SD_ERR_CAPACITY, "sd_hull_contacts: %d contacts", k);
+    return SD_OK;
+}
+
+extern "C" sd_status sd_hull_narrowphase_stats(sd_hull_ctx* c, int* n_pairs, int* n_hits, int* n_warm_used,
+                                               long long* iters_total) {
+    SD_REQUIRE(c, "sd_hull_narrowphase_stats: NULL");
+    if (c->n == 0 || c->steps_done == 0) return set_error(SD_ERR_STATE, "sd_hull_narrowphase_stats: no step has run");
+    int m = 0;
+    SD_HIP(hipMemcpyAsync(&m, c->total, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SD_HIP(hipStreamSynchronize(c->stream));
+    m = std::min(m, c->capacity);
+    std::vector<int> fl(std::max(m, 1)), it(std::max(m, 1));
+    if (m) {
+        SD_HIP(hipMemcpyAsync(fl.data(), c->flags, (size_t)m * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        SD_HIP(hipMemcpyAsync(it.data(), c->iters, (size_t)m * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        SD_HIP(hipStreamSynchronize(c->stream));
+    }
+    int hits = 0, warm = 0;
+    long long total = 0;
+    for (int t = 0; t < m; ++t) {
+        hits += (fl[t] & GJK_HIT) ? 1 : 0;
+        warm += (fl[t] & GJK_WARM_USED) ? 1 : 0;
+        total += it[t];
+    }
+    if (n_pairs) *n_pairs = m;
+    if (n_hits) *n_hits = hits;
+    if (n_warm_used) *n_warm_used = warm;
+    if (iters_total) *iters_total = total;
     return SD_OK;
 }

@@ -199,6 +199,8 @@ sd_status prepare_rigid(const char* who, int nb, const float* pos4, const float*
 sd_status hull_enqueue_step(sd_hull_ctx* c, hipStream_t s);
 sd_status hull_check_errors(sd_hull_ctx* c);
 int hull_body_count(const sd_hull_ctx* c);
+// Changes whenever the context's step changes (upload, parameters, warm start).
+long long hull_generation(const sd_hull_ctx* c);
 float hull_dt(const sd_hull_ctx* c);
 void hull_count_steps(sd_hull_ctx* c, int nsteps);
 
@@ -220,5 +222,22 @@ sd_status enqueue_gjk_cold(const int* pairs, int capacity, const int* dcount, co
                            const float4* quat4, const int* vstart, const int* nv, const float4* verts4, int n_hulls,
                            int n_verts, int max_iter, float near_margin, float* sep, float* normal3, int* flags,
                            int* iters, void* scratch, hipStream_t s);
+// Warm-started GJK/EPA (as sd_gjk with warm_dir3) of the first min(*dcount,
+// capacity) pairs, the rejected ones redone cold; same scratch.
+sd_status enqueue_gjk_warm(const int* pairs, int capacity, const int* dcount, const float4* pos4,
+                           const float4* quat4, const int* vstart, const int* nv, const float4* verts4, int n_hulls,
+                           int n_verts, const float* warm_dir3, int warm_budget, int max_iter, float near_margin,
+                           float* sep, float* normal3, int* flags, int* iters, void* scratch, hipStream_t s);
+
+// ------------------------------------------------------------------ learned warm start, enqueued only
+// sd_hull_descriptors on a stream (desc7: 7 floats per hull).
+sd_status launch_hull_descriptors(const float4* pos4, const int* vstart, const int* nv, const float4* verts4, int n,
+                                  float* desc7, hipStream_t s);
+// Features (X: 22 floats per pair) and BF16 inference of the network packed
+// by sd_mlp_pack_bf16 for the first min(*dcount, capacity) pairs: world
+// directions (A -> B) in out_dir3.
+sd_status enqueue_warm_dirs(const int* pairs, int capacity, const int* dcount, const float4* pos4,
+                            const float4* quat4, const float* desc7, float* X, const void* packed, float* out_dir3,
+                            hipStream_t s);
 
 }  // namespace sd

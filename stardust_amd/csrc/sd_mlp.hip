@@ -71,11 +71,13 @@ __global__ void k_hull_desc(const float4* __restrict__ pos, const int* __restric
 //   [3..6]  q_a^-1 q_b (w, x, y, z), sign chosen so that w >= 0
 //   [7..13] descriptor of a (lengths / s), [14..20] descriptor of b,
 //   [21]    |c_b - c_a| / s
+// dcount (nullable): the pairs held on the device (the first min(*dcount,
+// npairs)), for launches sized to a capacity.
 __global__ void k_mlp_features(const int* __restrict__ pairs, int npairs, const float4* __restrict__ pos,
                                const float4* __restrict__ quat, const float* __restrict__ desc,
-                               float* __restrict__ X) {
+                               float* __restrict__ X, const int* __restrict__ dcount) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+    if (p >= (dcount ? min(*dcount, npairs) : npairs)) return;
     const int a = pairs[2 * (size_t)p], b = pairs[2 * (size_t)p + 1];
     const float4 ca = pos[a], cb = pos[b];
     const float4 qa = quat[a], qb = quat[b];
@@ -454,12 +456,15 @@ static_assert(PK_END % 8 == 0 && (3 * NH + N4) % 4 == 0, "weights are staged in 
 // far below the BF16 rounding of the activations it feeds)
 __device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
-__global__ __launch_bounds__(INF_WAVES * 64) void k_mlp_infer(const float* __restrict__ X, int n,
+// dcount (nullable): rows held on the device (the first min(*dcount, n_rows)).
+__global__ __launch_bounds__(INF_WAVES * 64) void k_mlp_infer(const float* __restrict__ X, int n_rows,
                                                               const unsigned short* __restrict__ wb_g,
                                                               const float* __restrict__ bias_g,
                                                               const int* __restrict__ pairs,
                                                               const float4* __restrict__ quat,
-                                                              float* __restrict__ out_dir) {
+                                                              float* __restrict__ out_dir,
+                                                              const int* __restrict__ dcount) {
+    const int n = dcount ? min(*dcount, n_rows) : n_rows;
     __shared__ __attribute__((aligned(16))) unsigned short wt[WT_END];
     __shared__ __attribute__((aligned(16))) float bias[3 * NH + N4];
     __shared__ __attribute__((aligned(16))) unsigned short act[INF_WAVES][INF_ROWS][NH + 8];
@@ -629,7 +634,7 @@ extern "C" sd_status sd_mlp_features(const int* pairs, int npairs, const float* 
     SD_REQUIRE(npairs >= 0 && (npairs == 0 || (pairs && pos4 && quat4 && desc7 && X)), "sd_mlp_features: bad argument");
     if (npairs == 0) return SD_OK;
     hipLaunchKernelGGL(k_mlp_features, dim3(div_up(npairs, 256)), dim3(256), 0, (hipStream_t)stream, pairs, npairs,
-                       (const float4*)pos4, (const float4*)quat4, desc7, X);
+                       (const float4*)pos4, (const float4*)quat4, desc7, X, (const int*)nullptr);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
@@ -709,10 +714,38 @@ extern "C" sd_status sd_mlp_infer_bf16(const float* X, int n, const void* packed
     const unsigned short* wb = (const unsigned short*)packed;
     const float* bias = (const float*)((const char*)packed + PK_END * 2);
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(INF_WAVES * 64), 0, (hipStream_t)stream, X, n, wb, bias, pairs,
-                       (const float4*)quat4, out_dir3);
+                       (const float4*)quat4, out_dir3, (const int*)nullptr);
     SD_LAUNCH_CHECK();
     return SD_OK;
 }
+
+namespace sd {
+
+sd_status launch_hull_descriptors(const float4* pos4, const int* vstart, const int* nv, const float4* verts4, int n,
+                                  float* desc7, hipStream_t s) {
+    if (n == 0) return SD_OK;
+    hipLaunchKernelGGL(k_hull_desc, dim3(div_up(n, 256)), dim3(256), 0, s, pos4, vstart, nv, verts4, n, desc7);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+sd_status enqueue_warm_dirs(const int* pairs, int capacity, const int* dcount, const float4* pos4,
+                            const float4* quat4, const float* desc7, float* X, const void* packed, float* out_dir3,
+                            hipStream_t s) {
+    if (capacity == 0) return SD_OK;
+    hipLaunchKernelGGL(k_mlp_features, dim3(div_up(capacity, 256)), dim3(256), 0, s, pairs, capacity, pos4, quat4,
+                       desc7, X, dcount);
+    SD_LAUNCH_CHECK();
+    const int blocks = std::min(div_up(div_up(capacity, INF_ROWS), INF_WAVES), 2048);
+    const unsigned short* wb = (const unsigned short*)packed;
+    const float* bias = (const float*)((const char*)packed + PK_END * 2);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(INF_WAVES * 64), 0, s, (const float*)X, capacity, wb, bias,
+                       pairs, quat4, out_dir3, dcount);
+    SD_LAUNCH_CHECK();
+    return SD_OK;
+}
+
+}  // namespace sd
 
 extern "C" sd_status sd_mlp_forward(const float* X, int n, const float* params, float* out_dir3, void* stream) {
     SD_REQUIRE(n >= 0 && (n == 0 || (X && params && out_dir3)), "sd_mlp_forward: bad argument");

@@ -104,6 +104,7 @@ class HullBodyScene:
         L.call("sd_hull_create", C.byref(self.params), C.byref(h))
         self.h = h
         self.n = 0
+        self.model = None  # kept alive while attached
 
     def upload(self, b, scene_first=None):
         self.n = len(b["pos4"])
@@ -119,6 +120,12 @@ class HullBodyScene:
             ns = len(sf) - 1
         L.call("sd_hull_upload", self.h, self.n, *[L.ptr(x) for x in a], L.ptr(vs), L.ptr(nv), len(verts),
                L.ptr(verts), None if sf is None else L.ptr(sf), ns)
+
+    def set_warm(self, model, warm_budget=16):
+        """Warm-starts the step's GJK queries from a trained
+        warmstart.WarmStartModel (its packed BF16 network); None: cold."""
+        self.model = model
+        L.call("sd_hull_set_warm", self.h, model.packed.ptr if model is not None else None, int(warm_budget))
 
     def step(self, n=1):
         L.call("sd_hull_step", self.h, int(n))
@@ -138,6 +145,14 @@ class HullBodyScene:
         p = np.zeros((m, 4), np.float32)
         L.call("sd_hull_contacts", self.h, L.ptr(pairs), L.ptr(f), L.ptr(p), m, C.byref(k))
         return pairs[:k.value], f[:k.value], p[:k.value]
+
+    def narrowphase_stats(self):
+        """Last step: candidate pairs, hits, warm-started results kept, support
+        passes of GJK and EPA summed over the pairs."""
+        a, b, w = C.c_int(), C.c_int(), C.c_int()
+        it = C.c_longlong()
+        L.call("sd_hull_narrowphase_stats", self.h, C.byref(a), C.byref(b), C.byref(w), C.byref(it))
+        return {"pairs": a.value, "hits": b.value, "warm_used": w.value, "iters": it.value}
 
     def close(self):
         if self.h is not None and self.h.value:

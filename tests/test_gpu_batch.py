@@ -315,11 +315,12 @@ def test_batch_checkpoint_resume_and_scene_table_check():
     d.close()
 
 
-def test_batch_steps_attached_hull_scenes():
+def test_batch_steps_attached_hull_scenes(warm_model):
     """sd_batch_attach_hulls: the rigid-body scenes step inside the batched
     step (graph-captured, forked stream); afterwards the spheres equal the
     batch stepped without them and the bodies equal their own context stepped
-    alone, bit for bit."""
+    alone, bit for bit; also after the attached context switches to the
+    learned warm start (the batch captures its step again)."""
     say("batch + attached hull scenes")
     from stardust_amd import _lib as L
     from stardust_amd import hullbody as HB
@@ -349,6 +350,17 @@ def test_batch_steps_attached_hull_scenes():
     for key in ("pos4", "vel4", "quat4", "angm4"):
         assert np.array_equal(ga[key], gb[key]), key
     assert len(ha.contacts()[0]) > 50
+    ha.set_warm(warm_model)  # while attached
+    hb.set_warm(warm_model)
+    for k in (1, 5):
+        a.step(k)
+        b.step(k)
+        hb.step(k)
+    assert ha.narrowphase_stats()["warm_used"] > 0
+    assert_same(a.download(), b.download(), "spheres with warm-started bodies attached")
+    ga, gb = ha.download(), hb.download()
+    for key in ("pos4", "vel4", "quat4", "angm4"):
+        assert np.array_equal(ga[key], gb[key]), "warm " + key
     a.attach_hulls(None)
     a.step(3)
     b.step(3)

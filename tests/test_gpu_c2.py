@@ -38,7 +38,9 @@ def upload(w, s, vel=None, autotune=True):
     v = (z, z, z) if vel is None else (vel[:, 0].copy(), vel[:, 1].copy(), vel[:, 2].copy())
     sc.upload(s["x"], s["y"], s["z"], s["r"], s["m"], vx=v[0], vy=v[1], vz=v[2])
     if autotune:
-        say("autotuned shapes", sc.autotune(trial_steps=2))  # restores the uploaded state
+        t = sc.autotune(trial_steps=2)  # restores the uploaded state
+        say("autotuned shapes", t)
+        assert 0.0 < t["ms_per_step"] < 1e3  # the pick is a timed trial
     return sc
 
 

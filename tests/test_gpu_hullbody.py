@@ -2,7 +2,8 @@
 for bit: tumbling hulls, sphere bodies and the static triangles of the box
 walls in two scenes, graph-replayed and eager steps; the contacts of the last
 step (pairs, forces, points) equal; a pair capacity that is too small is
-reported."""
+reported. The step warm-started by a trained network keeps the cold step's
+verdicts and results."""
 import sys
 
 import numpy as np
@@ -101,3 +102,44 @@ def test_hull_capacity_overflow_reported():
     with pytest.raises(L.SdError, match="capacity"):
         sc.step(1)
     sc.close()
+
+
+def test_hull_bodies_warm_start_keeps_cold_results(warm_model):
+    """sd_hull_set_warm: the step's queries warm-started from the network's
+    directions. Most hull-hull queries keep their warm result (certified
+    closest features), and a kept result is a function of those features, so
+    candidate pairs, verdicts and contacts equal the cold step's and the
+    bodies end where the cold step leaves them (bit for bit for at least 99%
+    of the bodies, within 1e-6 m / 1e-5 relative for all)."""
+    say("hull bodies warm start")
+    bodies = scene(0x4B05, 400, 40, 0.12)
+    c = law()
+    cold = HB.HullBodyScene(c)
+    cold.upload(bodies)
+    warm = HB.HullBodyScene(c)
+    warm.upload(bodies)
+    warm.set_warm(warm_model)
+    used = 0
+    for k in (1, 4, 10):
+        cold.step(k)
+        warm.step(k)
+        sc, sw = cold.narrowphase_stats(), warm.narrowphase_stats()
+        say("step +%d cold %s warm %s" % (k, sc, sw))
+        assert sc["pairs"] == sw["pairs"] and sc["hits"] == sw["hits"] and sc["warm_used"] == 0
+        used += sw["warm_used"]
+        pc, fc, _ = cold.contacts()
+        pw, fw, _ = warm.contacts()
+        assert np.array_equal(pc, pw)
+        scale = np.abs(fc[:, :3]).max()
+        assert np.abs(fw[:, :3] - fc[:, :3]).max() <= 1e-5 * scale
+    assert used > 0
+    gc, gw = cold.download(), warm.download()
+    same = np.all(gc["pos4"] == gw["pos4"], axis=1) & np.all(gc["quat4"] == gw["quat4"], axis=1)
+    assert same.mean() >= 0.99, same.mean()
+    assert np.abs(gc["pos4"][:, :3] - gw["pos4"][:, :3]).max() <= 1e-6
+    warm.set_warm(None)  # back to cold queries: the two contexts now step alike
+    cold.step(2)
+    warm.step(2)
+    assert warm.narrowphase_stats()["warm_used"] == 0
+    cold.close()
+    warm.close()
