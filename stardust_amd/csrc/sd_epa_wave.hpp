@@ -11,13 +11,17 @@
 // group. Values that steer control flow are equal on the group's lanes, so a
 // group runs uniformly; the four groups of a wave diverge freely.
 //
+// (Eight-lane groups, eight queries per wave, were slower at C4: 210 VGPRs
+// and the polytopes of 16 groups per block left 1.5 waves per SIMD.)
+//
 // The algorithm is the serial one of the CPU restatement, step for step:
 //   grow the GJK simplex to a tetrahedron; faces (0,1,2) (0,3,1) (0,2,3) (1,3,2)
 //   oriented away from the centroid; repeat: closest face (first on ties),
 //   support point along its normal (first vertex on ties), stop when the gap is
 //   <= 1e-6 |d| + 1e-9 or the point repeats a vertex; faces that see the point
 //   are removed; the horizon is every edge (a, b), (b, c), (c, a) of a visible
-//   face, in face order, whose reverse edge belongs to no visible face; one new
+//   face, in face order, whose reverse edge belongs to no visible face (the
+//   visible faces' directed edges are marked in a bitmap first); one new
 //   face per horizon edge (degenerate ones skipped), appended after the
 //   surviving faces, which keep their order.
 // Depth and normal come from the final closest face with its vertices ordered
@@ -31,8 +35,11 @@ namespace sd {
 constexpr int EPA_G = 16;                                // lanes per query (one DPP row)
 constexpr int EPA_GPW = 64 / EPA_G;                      // queries per wavefront
 constexpr int EPA_VSLOT = 2;                             // hull vertices per lane in registers
+static_assert(EPA_G == 8 || EPA_G == 16, "groups are half or whole DPP rows");
 constexpr int EPA_FJ = (EPA_MAX_F + EPA_G - 1) / EPA_G;  // faces per lane
 constexpr int EPA_EK = (EPA_MAX_E + EPA_G - 1) / EPA_G;  // horizon edges per lane
+// directed edges (a, b) of the visible faces, bit a * EPA_MAX_V + b
+constexpr int EPA_EMAP_WORDS = (EPA_MAX_V * EPA_MAX_V + 31) / 32;
 
 struct EpaShared {
     float4 P[EPA_MAX_V];   // (x, y, z, key bits)
@@ -40,7 +47,20 @@ struct EpaShared {
     int fabc[EPA_MAX_F];   // a | b << 8 | c << 16
     unsigned char ea[EPA_MAX_E];
     unsigned char eb[EPA_MAX_E];
+    uint32_t emap[EPA_EMAP_WORDS];  // all zero between iterations
 };
+
+__device__ __forceinline__ void emap_mark(EpaShared& S, int a, int b) {
+    const int e = a * EPA_MAX_V + b;
+    atomicOr(&S.emap[e >> 5], 1u << (e & 31));
+}
+__device__ __forceinline__ bool emap_has(const EpaShared& S, int a, int b) {
+    const int e = a * EPA_MAX_V + b;
+    return (S.emap[e >> 5] >> (e & 31)) & 1u;
+}
+__device__ __forceinline__ void emap_clear(EpaShared& S, int gl) {
+    for (int w = gl; w < EPA_EMAP_WORDS; w += EPA_G) S.emap[w] = 0u;
+}
 
 // Orders this wave's LDS accesses (all lanes of a group are in one wave).
 __device__ __forceinline__ void grp_sync() {
@@ -50,7 +70,7 @@ __device__ __forceinline__ void grp_sync() {
 }
 
 __device__ __forceinline__ uint32_t grp_ballot(bool p, int base) {
-    return (uint32_t)((__ballot(p) >> base) & 0xffffull);
+    return (uint32_t)((__ballot(p) >> base) & ((1ull << EPA_G) - 1ull));
 }
 __device__ __forceinline__ int grp_below(uint32_t m, int gl) { return __popc(m & ((1u << gl) - 1u)); }
 
@@ -147,7 +167,7 @@ __device__ __forceinline__ V3 grp_support_point(const Shape& s, const HullRegs& 
     amax_step<DPP_XOR1>(bv, bi, bp);
     amax_step<DPP_XOR2>(bv, bi, bp);
     amax_step<DPP_HALF_MIRROR>(bv, bi, bp);
-    amax_step<DPP_MIRROR>(bv, bi, bp);
+    if constexpr (EPA_G == 16) amax_step<DPP_MIRROR>(bv, bi, bp);
     if (bi == 0x7fffffff) {  // no ordered projection: vertex 0, as a serial scan
         bi = 0;
         const float4 q = s.verts[0];
@@ -202,7 +222,7 @@ __device__ __forceinline__ int grp_closest_face(const EpaShared& S, int nf, int 
     amin_step<DPP_XOR1>(bd, bf);
     amin_step<DPP_XOR2>(bd, bf);
     amin_step<DPP_HALF_MIRROR>(bd, bf);
-    amin_step<DPP_MIRROR>(bd, bf);
+    if constexpr (EPA_G == 16) amin_step<DPP_MIRROR>(bd, bf);
     return bf == 0x7fffffff ? 0 : bf;  // no ordered distance: the first face, as a serial scan
 }
 
@@ -212,12 +232,6 @@ __device__ __forceinline__ void cmpx3(MPoint& a, MPoint& b) {
         a = b;
         b = t;
     }
-}
-
-// true if face x = (ga, gb, gc) holds the directed edge (v, u)
-__device__ __forceinline__ bool has_edge(int x, int v, int u) {
-    const int ga = x & 0xff, gb = (x >> 8) & 0xff, gc = (x >> 16) & 0xff;
-    return (ga == v && gb == u) || (gb == v && gc == u) || (gc == v && ga == u);
 }
 
 // EPA's outcome: the closest face of the final polytope (its three points
@@ -318,6 +332,7 @@ __device__ inline void grp_epa(const Shape& A, const HullRegs& hA, const Shape& 
         }
         nf = (ok[0] ? 1 : 0) + (ok[1] ? 1 : 0) + (ok[2] ? 1 : 0) + (ok[3] ? 1 : 0);
     }
+    emap_clear(S, gl);
     grp_sync();
     if (nf < 4) {
         o.flags |= GJK_HIT | GJK_DEGENERATE;
@@ -367,20 +382,26 @@ __device__ inline void grp_epa(const Shape& A, const HullRegs& hA, const Shape& 
                 vm[j] = grp_ballot(hm[j] != 0, base);
             }
         }
-        // horizon: an edge of a visible face survives unless a visible face holds its reverse
+        // horizon: an edge of a visible face survives unless a visible face
+        // holds its reverse. The visible faces mark their directed edges in
+        // the group's edge bitmap, then test the reverses of their own.
 #pragma unroll
-        for (int jj = 0; jj < EPA_FJ; ++jj) {
-            for (uint32_t mm = vm[jj]; mm; mm &= mm - 1u) {
-                const int x = S.fabc[EPA_G * jj + __builtin_ctz(mm)];
+        for (int j = 0; j < EPA_FJ; ++j) {
+            if (hm[j]) {
+                const int a = abc[j] & 0xff, b = (abc[j] >> 8) & 0xff, c = (abc[j] >> 16) & 0xff;
+                emap_mark(S, a, b);
+                emap_mark(S, b, c);
+                emap_mark(S, c, a);
+            }
+        }
+        grp_sync();
 #pragma unroll
-                for (int j = 0; j < EPA_FJ; ++j) {
-                    if (hm[j]) {
-                        const int a = abc[j] & 0xff, b = (abc[j] >> 8) & 0xff, c = (abc[j] >> 16) & 0xff;
-                        if (has_edge(x, b, a)) hm[j] &= ~1;
-                        if (has_edge(x, c, b)) hm[j] &= ~2;
-                        if (has_edge(x, a, c)) hm[j] &= ~4;
-                    }
-                }
+        for (int j = 0; j < EPA_FJ; ++j) {
+            if (hm[j]) {
+                const int a = abc[j] & 0xff, b = (abc[j] >> 8) & 0xff, c = (abc[j] >> 16) & 0xff;
+                if (emap_has(S, b, a)) hm[j] &= ~1;
+                if (emap_has(S, c, b)) hm[j] &= ~2;
+                if (emap_has(S, a, c)) hm[j] &= ~4;
             }
         }
         // horizon edge offsets (face order) and surviving face slots (face order)
@@ -425,7 +446,8 @@ __device__ inline void grp_epa(const Shape& A, const HullRegs& hA, const Shape& 
                 }
             }
         }
-        grp_sync();  // the horizon scan has read every face
+        grp_sync();  // the horizon scan has read every face and the edge bitmap
+        emap_clear(S, gl);
         // compaction: a surviving face moves to a slot <= its own; round j reads
         // faces [EPA_G j, EPA_G (j + 1)) and writes below that range's end
 #pragma unroll
