@@ -25,7 +25,9 @@ Two host-side legs ride inside "cpu_baseline" (off with --no-cpu):
            CPU model.
 
 Further sections ride on the same line (--sections, default
-"c2_dense,fp64,batch,gjk,batch_gjk,clumps,cpu_plan"):
+"c1,c2_dense,fp64,batch,gjk,batch_gjk,clumps,cpu_plan"):
+  c1     — dem_c1 (65,536 spheres, 32-bit cell keys): particle-steps/s of 1,000
+           graph-replayed steps and their stages (the CPU-runnable configuration).
   c2_dense — dem_c2_dense: the configuration-2 law on a contact-dense FCC packing of
            the same 4,194,304 spheres (about 3 contacts per sphere): particle-steps/s
            and the contact kernel's roofline, beside the dilute headline.
@@ -631,6 +633,18 @@ def sphere_run(w, steps, warmup, autotune, profile_steps, s=None):
     return r, s
 
 
+def c1_section(args):
+    """dem_c1 (65,536 spheres, u32 cell keys, the full radix sort every step):
+    particle-steps/s of the graph-replayed step and its stages."""
+    from stardust_amd import workloads as W
+    w = W.dem_c1()
+    steps = 1000
+    r, _ = sphere_run(w, steps, args.warmup, not args.no_autotune, args.profile_steps)
+    return {"workload": w.name, "spheres": w.n, "steps": steps, "ms_per_step": round(1000.0 * r["t"] / steps, 4),
+            "particle_steps_per_s": r["value"], "contacts_per_sphere": r["contacts"] / w.n,
+            "launch_shapes": r["tune"], "stages": r["stages"]}
+
+
 def c2_dense_section(args):
     """dem_c2_dense: the configuration-2 law on a contact-dense packing of the same
     sphere count (about 3 contacts per sphere), particle-steps/s and the contact
@@ -840,7 +854,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes of the C2 context")
-    ap.add_argument("--sections", default="c2_dense,fp64,batch,gjk,batch_gjk,clumps,cpu_plan",
+    ap.add_argument("--sections", default="c1,c2_dense,fp64,batch,gjk,batch_gjk,clumps,cpu_plan",
                     help="extra sections: c2_dense, fp64, batch, gjk, batch_gjk, clumps, cpu_plan (comma list, "
                          "or none)")
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -920,6 +934,8 @@ def main():
         "cpu_baseline": cpu,
     }
     sections = [x for x in args.sections.split(",") if x and x != "none"]
+    if "c1" in sections and args.workload != "c1":
+        out["c1"] = c1_section(args)
     if "c2_dense" in sections:
         out["c2_dense"] = c2_dense_section(args)
     # the host-side legs: the oracle runs here only as the CPU baseline and as

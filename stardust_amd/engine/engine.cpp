@@ -35,6 +35,16 @@ DEMEngine::~DEMEngine() {
     if (m_hctx) sd_hull_destroy(m_hctx);
 }
 
+void DEMEngine::cleanBuffers() {
+    if (m_ctx) sd_clump_destroy(m_ctx);
+    if (m_sctx) sd_ctx_destroy(m_sctx);
+    if (m_hctx) sd_hull_destroy(m_hctx);
+    m_ctx = nullptr;
+    m_sctx = nullptr;
+    m_hctx = nullptr;
+    m_uploaded = false;
+}
+
 void DEMEngine::add(DEMParticle particle) {
     if (m_uploaded) throw std::runtime_error("DEMEngine::add: entities cannot be added after transfer()");
     entity_handler.addEntity(particle);

@@ -84,6 +84,19 @@ public:
     void transfer();
     void update(float dt);
 
+    // The reference's other names for the same steps (engine.hpp:172-185;
+    // engine.cpp:110-130,243,601-612): addParticle/addMesh add, prepArrays
+    // packs, transferDataToDevice transfers, step advances one step. The grid
+    // is set up by transfer() (setUpGrid does nothing); cleanBuffers releases
+    // the device state (a later transfer() uploads again).
+    void addParticle(DEMParticle particle) { add(particle); }
+    void addMesh(DEMMesh mesh) { add(mesh); }
+    void setUpGrid() {}
+    void prepArrays() { prep(); }
+    void transferDataToDevice() { transfer(); }
+    void step(Scalar dt) { update((float)dt); }
+    void cleanBuffers();
+
     // Scene file of the reference's format (setup/star_test.json): entities of
     // type "generic" become DEMParticle(count + 1, 10, size, size, mass, position,
     // velocity) (the mapping of the commented loader, engine.cpp:64-108, with

@@ -44,6 +44,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -153,6 +154,39 @@ int main(int argc, char** argv) {
             EntityHandler& h = engine.getEntityHandler();
             const int nb = h.getNumberOfParticles();
             std::printf("\"steps\": %lld,\n", engine.stepsDone());
+            print_f4("final_position", h.h_rigid_body_position_ptr, nb);
+            print_f4("final_velocity", h.h_rigid_body_velocity_ptr, nb);
+            print_f4("final_quaternion", h.h_rigid_body_quaternion_ptr, nb);
+            print_f4("final_angular_momentum", h.h_rigid_body_angular_momentum_ptr, nb, true);
+            std::printf("}\n");
+            return 0;
+        }
+        if (mode == "run-legacy" && argc >= 4) {
+            // the two-body scene through the reference's other names
+            const int steps = std::atoi(argv[2]);
+            const float dt = (float)std::atof(argv[3]);
+            DEMEngine engine(config, 1.0f);
+            engine.addParticle(DEMParticle(0, 2, 0.04f, 0.02f, 0.5f, make_float4(0.0f, 0.0f, 0.0f, 0.0f),
+                                           make_float4(1.0f, 0.0f, 0.0f, 0.0f)));
+            engine.addParticle(DEMParticle(1, 2, 0.04f, 0.02f, 0.5f, make_float4(0.032f, 0.003f, 0.002f, 0.0f),
+                                           make_float4(-1.0f, 0.0f, 0.0f, 0.0f)));
+            engine.addParticle(DEMParticle(2, 1, 0.02f, 0.02f, 50.0f, make_float4(0.016f, -0.02f, 0.0f, 0.0f),
+                                           make_float4(0.0f, 0.0f, 0.0f, 0.0f)));
+            engine.prepArrays();
+            engine.setUpGrid();
+            engine.transferDataToDevice();
+            for (int i = 0; i < steps; ++i) engine.step(dt);
+            engine.sync();
+            EntityHandler& h = engine.getEntityHandler();
+            const int nb = h.getNumberOfParticles();
+            engine.cleanBuffers();
+            bool cleaned = false;
+            try {
+                engine.update(dt);
+            } catch (const std::runtime_error&) {
+                cleaned = true;  // no device state after cleanBuffers
+            }
+            std::printf("{\n\"steps\": %d, \"cleaned\": %s,\n", steps, cleaned ? "true" : "false");
             print_f4("final_position", h.h_rigid_body_position_ptr, nb);
             print_f4("final_velocity", h.h_rigid_body_velocity_ptr, nb);
             print_f4("final_quaternion", h.h_rigid_body_quaternion_ptr, nb);

@@ -131,6 +131,19 @@ def test_engine_update_equals_oracle_two_bodies(oracle):
 
 
 @pytest.mark.gpu
+def test_engine_reference_other_names():
+    """addParticle / prepArrays / setUpGrid / transferDataToDevice / step (the
+    reference's other names, engine.hpp:172-185) give the run of add / prep /
+    transfer / update bit for bit; cleanBuffers releases the device state."""
+    print("[gpu-case] engine legacy names", flush=True)
+    a = drive("run", 200, 1e-5)
+    b = drive("run-legacy", 200, 1e-5)
+    assert b["cleaned"] is True
+    for key in ("final_position", "final_velocity", "final_quaternion", "final_angular_momentum"):
+        assert a[key] == b[key], key
+
+
+@pytest.mark.gpu
 def test_engine_update_star_test_scene(oracle):
     print("[gpu-case] engine star_test.json", flush=True)
     d, st = run_and_compare(oracle, 3, 1e-5, STAR)
