@@ -94,11 +94,22 @@ struct ClumpPack {
 // relative to it with radius diameter / 2, inertia about the centre from the
 // member spheres (own 2/5 m r^2 plus the parallel-axis term, in double),
 // bodies at or above the static mass threshold uploaded with mass 0.
-void pack_clumps(const EntityHandler& h, const std::vector<int>& estart, const std::vector<int>& elen,
-                 const std::vector<float4>& rb_pos, const std::vector<float4>& rb_vel,
-                 const std::vector<float>& rb_mass, const std::vector<float4>& p_rel,
-                 const std::vector<float>& p_size, const std::vector<float>& p_mass, float static_mass,
-                 ClumpPack& u) {
+struct ClumpArrays {  // a handler's body and sphere arrays (read by DEMEngine members)
+    const std::vector<int>& estart;
+    const std::vector<int>& elen;
+    const std::vector<float4>& rb_pos;
+    const std::vector<float4>& rb_vel;
+    const std::vector<float4>& rb_quat;
+    const std::vector<float4>& rb_angvel;
+    const std::vector<float>& rb_mass;
+    const std::vector<float4>& p_init_rel;
+    const std::vector<float>& p_size;
+    const std::vector<float>& p_mass;
+};
+
+void pack_clumps(const EntityHandler& h, const ClumpArrays& a, float static_mass, ClumpPack& u) {
+    const std::vector<int>& estart = a.estart;
+    const std::vector<int>& elen = a.elen;
     const int nb = h.getNumberOfParticles(), ns = h.getNumberOfSpheres();
     u.pos4.assign(4 * (size_t)nb, 0.0f);
     u.vel4.assign(4 * (size_t)nb, 0.0f);
@@ -109,8 +120,8 @@ void pack_clumps(const EntityHandler& h, const std::vector<int>& estart, const s
     u.count.assign(nb, 0);
     u.local4.assign(4 * (size_t)ns, 0.0f);
     for (int b = 0; b < nb; ++b) {
-        const float4 X = rb_pos[b], V = rb_vel[b];
-        const float m = rb_mass[b];
+        const float4 X = a.rb_pos[b], V = a.rb_vel[b], q = a.rb_quat[b], w = a.rb_angvel[b];
+        const float m = a.rb_mass[b];
         const bool fixed = m >= static_mass;
         u.pos4[4 * b] = X.x;
         u.pos4[4 * b + 1] = X.y;
@@ -119,19 +130,25 @@ void pack_clumps(const EntityHandler& h, const std::vector<int>& estart, const s
         u.vel4[4 * b] = V.x;
         u.vel4[4 * b + 1] = V.y;
         u.vel4[4 * b + 2] = V.z;
-        u.quat4[4 * b] = 1.0f;
+        u.quat4[4 * b] = q.x;  // (w, x, y, z), as the reference's float4 stores it
+        u.quat4[4 * b + 1] = q.y;
+        u.quat4[4 * b + 2] = q.z;
+        u.quat4[4 * b + 3] = q.w;
+        u.angvel4[4 * b] = w.x;
+        u.angvel4[4 * b + 1] = w.y;
+        u.angvel4[4 * b + 2] = w.z;
         u.first[b] = estart[b];
         u.count[b] = elen[b];
         double I[6] = {0, 0, 0, 0, 0, 0};
         for (int j = 0; j < elen[b]; ++j) {
             const int k = estart[b] + j;
-            const float4 d = p_rel[k];
-            const float r = 0.5f * p_size[k];
+            const float4 d = a.p_init_rel[k];  // body frame
+            const float r = 0.5f * a.p_size[k];
             u.local4[4 * (size_t)k] = d.x;
             u.local4[4 * (size_t)k + 1] = d.y;
             u.local4[4 * (size_t)k + 2] = d.z;
             u.local4[4 * (size_t)k + 3] = r;
-            const double mj = p_mass[k], dx = d.x, dy = d.y, dz = d.z;
+            const double mj = a.p_mass[k], dx = d.x, dy = d.y, dz = d.z;
             const double own = 0.4 * mj * (double)r * (double)r;
             I[0] += own + mj * (dy * dy + dz * dz);
             I[1] += own + mj * (dx * dx + dz * dz);
@@ -140,7 +157,7 @@ void pack_clumps(const EntityHandler& h, const std::vector<int>& estart, const s
             I[4] += -mj * dx * dz;
             I[5] += -mj * dy * dz;
         }
-        for (int q = 0; q < 6; ++q) u.inertia6[6 * (size_t)b + q] = (float)I[q];
+        for (int c = 0; c < 6; ++c) u.inertia6[6 * (size_t)b + c] = (float)I[c];
     }
 }
 
@@ -185,8 +202,10 @@ void DEMEngine::addRuntime(const DEMParticle& e) {
         check(sd_ctx_add_spheres(m_sctx, 1, &X.x, &X.y, &X.z, &V.x, &V.y, &V.z, &r, &mass), "DEMEngine::addRuntime");
     } else {
         ClumpPack u;
-        pack_clumps(one, one.entity_start, one.entity_length, one.rb_pos, one.rb_vel, one.rb_mass, one.p_rel,
-                    one.p_size, one.p_mass, m_static_mass, u);
+        pack_clumps(one,
+                    ClumpArrays{one.entity_start, one.entity_length, one.rb_pos, one.rb_vel, one.rb_quat, one.rb_angvel,
+                                one.rb_mass, one.p_init_rel, one.p_size, one.p_mass},
+                    m_static_mass, u);
         check(sd_clump_add_bodies(m_ctx, 1, u.pos4.data(), u.vel4.data(), u.quat4.data(), u.angvel4.data(),
                                   u.inertia6.data(), u.first.data(), u.count.data(), one.getNumberOfSpheres(),
                                   u.local4.data()),
@@ -340,6 +359,13 @@ sd_contact_params sphere_law(float kn, float zeta, float mu, float kt_ratio, flo
 void DEMEngine::transfer_spheres() {
     EntityHandler& h = entity_handler;
     const int n = h.getNumberOfParticles();
+    for (int b = 0; b < n; ++b) {
+        const float4 w = h.rb_angvel[b];
+        if (w.x != 0.0f || w.y != 0.0f || w.z != 0.0f)
+            throw std::invalid_argument("DEMEngine::transfer: entity " + std::to_string(b) +
+                                        " has an initial angular velocity; single-sphere scenes (the sphere "
+                                        "path) start every sphere without spin");
+    }
     std::vector<float> x(n), y(n), z(n), vx(n), vy(n), vz(n), r(n), m(n);
     float rmax = 0.0f;
     bool inside = m_domain > 0.0f;
@@ -413,7 +439,9 @@ void DEMEngine::transfer_clumps() {
     EntityHandler& h = entity_handler;
     const int nb = h.getNumberOfParticles(), ns = h.getNumberOfSpheres();
     ClumpPack u;
-    pack_clumps(h, h.entity_start, h.entity_length, h.rb_pos, h.rb_vel, h.rb_mass, h.p_rel, h.p_size, h.p_mass,
+    pack_clumps(h,
+                ClumpArrays{h.entity_start, h.entity_length, h.rb_pos, h.rb_vel, h.rb_quat, h.rb_angvel, h.rb_mass,
+                            h.p_init_rel, h.p_size, h.p_mass},
                 m_static_mass, u);
     if (!m_ctx) {
         sd_clump_params p{};

@@ -3,6 +3,7 @@
 
 #include "mesh_loader.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -19,6 +20,39 @@ DEMParticle::DEMParticle(unsigned int id, int grid_resolution, Scalar size, Scal
     initParticles(m_grid_resolution);
     getCenterOfMass();
     setParticlesInWorldSpace();
+}
+
+DEMParticle::DEMParticle(unsigned int id, const std::vector<DEMSphere>& spheres, float4 position_, float4 velocity_,
+                         float4 quaternion_, float4 angular_velocity_)
+    : position(position_), velocity(velocity_), COM(make_float4(0.0f, 0.0f, 0.0f, 0.0f)), mass(0.0f),
+      diameter(0.0f), quaternion(quaternion_), angular_velocity(angular_velocity_), m_id(id), m_size(0.0f),
+      m_grid_resolution(0) {
+    if (spheres.empty()) throw std::invalid_argument("DEMParticle: a clump needs at least one sphere");
+    const double qw = quaternion.x, qx = quaternion.y, qy = quaternion.z, qz = quaternion.w;
+    const double qn = qw * qw + qx * qx + qy * qy + qz * qz;
+    if (!(std::fabs(qn - 1.0) < 1e-4)) throw std::invalid_argument("DEMParticle: the quaternion must be of unit length");
+    // R(q) in double; each world coordinate rounded once
+    const double R[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz),     2 * (qx * qz + qw * qy),
+                         2 * (qx * qy + qw * qz),     1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx),
+                         2 * (qx * qz - qw * qy),     2 * (qy * qz + qw * qx),     1 - 2 * (qx * qx + qy * qy)};
+    for (const DEMSphere& b : spheres) {
+        if (!(b.size > 0.0f) || !(b.mass > 0.0f))
+            throw std::invalid_argument("DEMParticle: clump spheres need a positive diameter and mass");
+        const double x = b.position.x, y = b.position.y, z = b.position.z;
+        DEMSphere w = b;
+        w.position = make_float4((float)(position.x + (R[0] * x + R[1] * y + R[2] * z)),
+                                 (float)(position.y + (R[3] * x + R[4] * y + R[5] * z)),
+                                 (float)(position.z + (R[6] * x + R[7] * y + R[8] * z)), 0.0f);
+        particles.push_back(w);
+        m_body.push_back(make_float4(b.position.x, b.position.y, b.position.z, 0.0f));
+        mass += b.mass;
+        diameter = std::max(diameter, b.size);
+    }
+}
+
+float4 DEMParticle::bodyOffset(std::size_t j) const {
+    if (!m_body.empty()) return m_body[j];
+    return particles[j].position - position;
 }
 
 // grid_resolution^3 spheres at (i, j, k) * dx, i, j, k = 1 .. grid_resolution,

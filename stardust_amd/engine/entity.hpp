@@ -10,6 +10,7 @@
 #ifndef STARDUST_AMD_ENGINE_ENTITY_HPP
 #define STARDUST_AMD_ENGINE_ENTITY_HPP
 
+#include <cstddef>
 #include <vector>
 
 #include "types.hpp"
@@ -26,6 +27,17 @@ class DEMParticle {
 public:
     DEMParticle(unsigned int id, int grid_resolution, Scalar size, Scalar diameter, Scalar mass, float4 position,
                 float4 velocity);
+    // A rigid clump of arbitrary spheres (no counterpart in the reference,
+    // whose DEMParticle is the lattice above; the bodies of configuration 3):
+    // spheres[j].position is sphere j's centre relative to the body position
+    // in the body frame (w unused), .size its diameter, .mass its mass. The
+    // body sits at position with orientation quaternion (x = w, y = i, z = j,
+    // w = k; unit) and angular velocity angular_velocity (world, rad/s); its
+    // mass is the sum of the spheres' and its spheres' world positions are
+    // position + R(quaternion) offset. (A scene of single spheres runs on the
+    // sphere path, which takes no initial spin: transfer() rejects one.)
+    DEMParticle(unsigned int id, const std::vector<DEMSphere>& spheres, float4 position, float4 velocity,
+                float4 quaternion, float4 angular_velocity);
 
     void initParticles(int grid_resolution);
     void setParticlesInWorldSpace();
@@ -34,17 +46,24 @@ public:
     std::vector<DEMSphere> getParticles() const { return particles; }
     unsigned int getId() const { return m_id; }
 
+    // Body-frame offset of sphere j from the body position (the lattice: its
+    // world offset, the body starting unrotated).
+    float4 bodyOffset(std::size_t j) const;
+
     float4 position;  // world position of the entity (the centroid of its spheres)
     float4 velocity;
     float4 COM;       // lattice centroid in entity space
     float mass;
     float diameter;
+    float4 quaternion = make_float4(1.0f, 0.0f, 0.0f, 0.0f);        // initial orientation (x = w, y = i, z = j, w = k)
+    float4 angular_velocity = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // initial, world
 
 private:
     unsigned int m_id;
     Scalar m_size;
     int m_grid_resolution;
     std::vector<DEMSphere> particles;
+    std::vector<float4> m_body;  // body-frame offsets (general clumps only)
 };
 
 // A triangle mesh (entity.hpp:64-93): the triangles of an STL file (binary

@@ -135,6 +135,8 @@ void EntityHandler::prepareArrays() {
         rb_pos[i] = e.position;
         rb_vel[i] = e.velocity;
         rb_mass[i] = e.mass;
+        rb_quat[i] = e.quaternion;
+        rb_angvel[i] = e.angular_velocity;
         entity_start[i] = offset;
         entity_length[i] = (int)sph.size();
         float Ixx = 0.0f, Iyy = 0.0f, Izz = 0.0f, Ixy = 0.0f, Ixz = 0.0f, Iyz = 0.0f;
@@ -148,9 +150,11 @@ void EntityHandler::prepareArrays() {
             p_mass[k] = s.mass;
             p_size[k] = s.size;
             particle_to_rigid[k] = (int)i;
-            const float4 r = s.position - e.position;
+            // world offset, and the body-frame offset (the same for a lattice,
+            // which starts unrotated); the inertia is the body frame's
+            p_rel[k] = s.position - e.position;
+            const float4 r = e.bodyOffset(j);
             p_init_rel[k] = r;
-            p_rel[k] = r;
             Ixx += s.mass * (r.y * r.y + r.z * r.z);
             Iyy += s.mass * (r.x * r.x + r.z * r.z);
             Izz += s.mass * (r.x * r.x + r.y * r.y);

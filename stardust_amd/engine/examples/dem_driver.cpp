@@ -89,7 +89,39 @@ void print_pack(EntityHandler& h) {
     std::printf("],\n");
     print_f4("rigid_body_position", h.h_rigid_body_position_ptr, nb);
     print_f4("rigid_body_velocity", h.h_rigid_body_velocity_ptr, nb);
+    print_f4("rigid_body_quaternion", h.h_rigid_body_quaternion_ptr, nb);
+    print_f4("rigid_body_angular_velocity", h.h_rigid_body_angular_velocity_ptr, nb);
     print_f4("particle_relative_position", h.h_particle_relative_position_ptr, ns);
+    print_f4("particle_init_relative_position", h.h_particle_init_relative_position_ptr, ns);
+}
+
+// Clumps from a text file: the body count, then per body a line
+//   n px py pz vx vy vz qw qx qy qz wx wy wz
+// and n lines "rx ry rz diameter mass" (body-frame sphere centres).
+void read_clumps(const char* path, DEMEngine& engine) {
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + path);
+    int nb = 0;
+    if (std::fscanf(f, "%d", &nb) != 1 || nb < 1) throw std::runtime_error("bad clump file");
+    for (int b = 0; b < nb; ++b) {
+        int n = 0;
+        float v[13];
+        if (std::fscanf(f, "%d", &n) != 1 || n < 1) throw std::runtime_error("bad clump file");
+        for (float& x : v)
+            if (std::fscanf(f, "%f", &x) != 1) throw std::runtime_error("bad clump file");
+        std::vector<DEMSphere> sph(n);
+        for (DEMSphere& q : sph) {
+            float r[5];
+            for (float& x : r)
+                if (std::fscanf(f, "%f", &x) != 1) throw std::runtime_error("bad clump file");
+            q.position = make_float4(r[0], r[1], r[2], 0.0f);
+            q.size = r[3];
+            q.mass = r[4];
+        }
+        engine.add(DEMParticle((unsigned)b, sph, make_float4(v[0], v[1], v[2], 0.0f), make_float4(v[3], v[4], v[5], 0.0f),
+                               make_float4(v[6], v[7], v[8], v[9]), make_float4(v[10], v[11], v[12], 0.0f)));
+    }
+    std::fclose(f);
 }
 
 // two 2x2x2 lattice bodies approaching along x, slightly offset, and a static sphere
@@ -158,6 +190,33 @@ int main(int argc, char** argv) {
             print_f4("final_velocity", h.h_rigid_body_velocity_ptr, nb);
             print_f4("final_quaternion", h.h_rigid_body_quaternion_ptr, nb);
             print_f4("final_angular_momentum", h.h_rigid_body_angular_momentum_ptr, nb, true);
+            std::printf("}\n");
+            return 0;
+        }
+        if ((mode == "pack-clumps" && argc >= 3) || (mode == "run-clumps" && argc >= 5)) {
+            // general clumps (DEMParticle from spheres, orientation and spin)
+            const bool run = mode == "run-clumps";
+            DEMEngine engine(config, 0.0f);
+            read_clumps(argv[run ? 4 : 2], engine);
+            engine.prep();
+            std::printf("{\n");
+            print_pack(engine.getEntityHandler());
+            if (run) {
+                const int steps = std::atoi(argv[2]);
+                const float dt = (float)std::atof(argv[3]);
+                engine.transfer();
+                for (int i = 0; i < steps; ++i) engine.update(dt);
+                engine.sync();
+                EntityHandler& h = engine.getEntityHandler();
+                const int nb = h.getNumberOfParticles();
+                std::printf("\"steps\": %lld,\n", engine.stepsDone());
+                print_f4("final_position", h.h_rigid_body_position_ptr, nb);
+                print_f4("final_velocity", h.h_rigid_body_velocity_ptr, nb);
+                print_f4("final_quaternion", h.h_rigid_body_quaternion_ptr, nb);
+                print_f4("final_angular_momentum", h.h_rigid_body_angular_momentum_ptr, nb);
+            }
+            print_f4("particle_position", engine.getEntityHandler().h_particle_position_ptr,
+                     engine.getNumberOfSpheres(), true);
             std::printf("}\n");
             return 0;
         }

@@ -80,7 +80,8 @@ def clump_input_from_pack(d, static_mass=10.0):
     """Independent restatement of the engine's upload (see module docstring)."""
     nb = d["bodies"]
     start, length = d["entity_start"], d["entity_length"]
-    rel = np.array(d["particle_relative_position"], np.float32)
+    # body-frame offsets (the world ones of an unrotated lattice body)
+    rel = np.array(d.get("particle_init_relative_position", d["particle_relative_position"]), np.float32)
     pm = np.array(d["particle_mass"], np.float32)
     rad = (np.float32(0.5) * np.array(d["particle_size"], np.float32)).astype(np.float32)
     s = {"pos4": np.zeros((nb, 4), np.float32), "vel4": np.zeros((nb, 4), np.float32),
@@ -93,6 +94,9 @@ def clump_input_from_pack(d, static_mass=10.0):
         s["pos4"][b, :3] = d["rigid_body_position"][b][:3]
         s["pos4"][b, 3] = 0.0 if m >= static_mass else m
         s["vel4"][b, :3] = d["rigid_body_velocity"][b][:3]
+        if "rigid_body_quaternion" in d:
+            s["quat4"][b] = d["rigid_body_quaternion"][b]  # (w, x, y, z) as the engine's float4 holds it
+            s["angvel4"][b, :3] = d["rigid_body_angular_velocity"][b][:3]
         I = np.zeros(6)
         for k in range(start[b], start[b] + length[b]):
             dx, dy, dz = (float(v) for v in rel[k, :3])
@@ -451,3 +455,82 @@ def test_engine_mesh_walls_and_hull_equal_oracle(oracle):
     on_top = np.abs(z - (1.0 + 0.25)) < 0.01
     assert on_top.sum() >= 8, z
     assert np.abs(st.V[:n][on_top, 2]).max() < 0.05
+
+
+def write_clumps(path, c, rho=2500.0):
+    """C3-style clumps (the sd_gen_clumps generator) as the driver's clump
+    file: per-sphere masses rho 4/3 pi r^3 (float32), diameters 2 r."""
+    lines = [str(len(c["pos4"]))]
+    for b in range(len(c["pos4"])):
+        f, n = int(c["first"][b]), int(c["count"][b])
+        p, v, q, w = c["pos4"][b], c["vel4"][b], c["quat4"][b], c["angvel4"][b]
+        lines.append(" ".join(["%d" % n] + ["%.9g" % x for x in (*p[:3], *v[:3], *q, *w[:3])]))
+        for k in range(f, f + n):
+            r = np.float32(c["local4"][k, 3])
+            m = np.float32(rho * 4.0 / 3.0 * np.pi * float(r) ** 3)
+            lines.append("%.9g %.9g %.9g %.9g %.9g" % (*c["local4"][k, :3], np.float32(2.0) * r, m))
+    path.write_text("\n".join(lines) + "\n")
+
+
+def quat_rot64(q):
+    w, x, y, z = (float(v) for v in q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_engine_general_clumps_pack(tmp_path):
+    """DEMParticle from arbitrary spheres (the configuration-3 bodies: 4-16
+    spheres, Gaussian offsets, random orientation and spin): the handler holds
+    the body-frame offsets, the orientation and the spin as given, the
+    spheres' world positions are position + R(q) offset (float64, rounded
+    once), and the body mass is the float sum of the sphere masses."""
+    from stardust_amd import clumps as CL
+    c = CL.generate_clumps(0x5EED0C01, 40, lattice=4)
+    f = tmp_path / "clumps.txt"
+    write_clumps(f, c)
+    d = drive("pack-clumps", f)
+    assert d["bodies"] == 40 and d["spheres"] == int(c["count"].sum())
+    pos = np.array(d["particle_position"], np.float32)
+    init = np.array(d["particle_init_relative_position"], np.float32)
+    for b in range(40):
+        fst, n = int(c["first"][b]), int(c["count"][b])
+        assert d["entity_start"][b] == fst and d["entity_length"][b] == n
+        assert np.array_equal(np.float32(d["rigid_body_quaternion"][b]), c["quat4"][b])
+        assert np.array_equal(np.float32(d["rigid_body_angular_velocity"][b][:3]), c["angvel4"][b, :3])
+        R = quat_rot64(c["quat4"][b])
+        X = c["pos4"][b, :3].astype(np.float64)
+        m = np.float32(0.0)
+        for k in range(fst, fst + n):
+            r = c["local4"][k, :3].astype(np.float64)
+            assert np.array_equal(init[k, :3], c["local4"][k, :3])
+            assert np.array_equal(pos[k, :3], (X + R @ r).astype(np.float32)), (b, k)
+            m = np.float32(m + np.float32(d["particle_mass"][k]))
+        assert np.float32(d["rigid_body_mass"][b]) == m
+
+
+@pytest.mark.gpu
+def test_engine_general_clumps_equal_oracle(oracle, tmp_path):
+    """Configuration-3 style clumps (512 bodies of 4-16 spheres, random
+    orientations, velocities N(0, 0.5 m/s), spins) built through the engine API
+    and stepped by DEMEngine::update equal the oracle's clump steps on the
+    packed bodies, bit for bit."""
+    print("[gpu-case] engine general clumps", flush=True)
+    from stardust_amd import clumps as CL
+    c = CL.generate_clumps(0x5EED0C02, 512, lattice=8, spacing=0.035)
+    f = tmp_path / "clumps.txt"
+    write_clumps(f, c)
+    steps, dt = 60, 1e-5
+    d = drive("run-clumps", steps, dt, f)
+    assert d["steps"] == steps
+    s = clump_input_from_pack(d)
+    st = oracle.ClumpState(s)
+    cp = L.ContactParams.make(2e6, 0.3, 0.5, np.float32(dt), gravity=(0.0, 0.0, -9.81), walls=None,
+                              kt_ratio=2.0 / 7.0, max_contacts=16)
+    st.step(cp, steps)
+    for key, arr, w in (("final_position", st.P, 3), ("final_velocity", st.V, 3), ("final_quaternion", st.Q, 4),
+                        ("final_angular_momentum", st.L, 3)):
+        got = np.array(d[key], np.float32)[:, :w]
+        assert np.array_equal(got, arr[:, :w]), key
+    # the bodies turned: the final orientations differ from the initial ones
+    assert not np.array_equal(np.array(d["final_quaternion"], np.float32), c["quat4"])
