@@ -143,3 +143,26 @@ def test_hull_bodies_warm_start_keeps_cold_results(warm_model):
     assert warm.narrowphase_stats()["warm_used"] == 0
     cold.close()
     warm.close()
+
+
+def test_hull_bodies_c5_full_size(oracle):
+    """The 128 hull-carrying scenes of the configuration-5 batch as rigid
+    bodies at full size (544,353 bodies: 542,817 hulls on 20 mm lattices and
+    each scene's twelve wall triangles; about 985k candidate pairs and 300k
+    contacts per step): two graph-replayed steps equal the oracle's, bit for
+    bit, and so do the last step's contacts."""
+    from stardust_amd import workloads as W
+    say("hull bodies C5 full size")
+    w = W.batch_c5()
+    bodies, first = w.hull_set()
+    c = L.ContactParams.make(2e5, 0.3, 0.5, w.dt, gravity=(0.0, 0.0, -9.81), walls=None, max_contacts=16)
+    st = oracle.HullBodyState(bodies, first)
+    sc = HB.HullBodyScene(c)
+    sc.upload(bodies, first)
+    sc.step(2)
+    touching = st.step(c, nsteps=2, nthreads=16)
+    compare(sc, st, "C5 hull scenes, 2 steps")
+    gp, gf, _ = sc.contacts()
+    say("contacts: gpu %d, oracle %d" % (len(gp), touching))
+    assert len(gp) == touching and touching > 200000
+    sc.close()
