@@ -27,7 +27,8 @@ Two host-side legs ride inside "cpu_baseline" (off with --no-cpu):
 Further sections ride on the same line (--sections, default
 "c1,c2_dense,fp64,batch,gjk,batch_gjk,clumps,cpu_plan"):
   c1     — dem_c1 (65,536 spheres, 32-bit cell keys): particle-steps/s of 1,000
-           graph-replayed steps and their stages (the CPU-runnable configuration).
+           graph-replayed steps (--steps when given) and their stages (the
+           CPU-runnable configuration).
   c2_dense — dem_c2_dense: the configuration-2 law on a contact-dense FCC packing of
            the same 4,194,304 spheres (about 3 contacts per sphere): particle-steps/s
            and the contact kernel's roofline, beside the dilute headline.
@@ -638,7 +639,7 @@ def c1_section(args):
     particle-steps/s of the graph-replayed step and its stages."""
     from stardust_amd import workloads as W
     w = W.dem_c1()
-    steps = 1000
+    steps = args.steps if args.steps is not None else 1000
     r, _ = sphere_run(w, steps, args.warmup, not args.no_autotune, args.profile_steps)
     return {"workload": w.name, "spheres": w.n, "steps": steps, "ms_per_step": round(1000.0 * r["t"] / steps, 4),
             "particle_steps_per_s": r["value"], "contacts_per_sphere": r["contacts"] / w.n,
