@@ -368,7 +368,8 @@ sd_status sd_sap_pairs_batched(const float* lo4, const float* hi4, int n, const 
 /* GJK distance / EPA penetration for each pair (a, b) of hull ids. Outputs per
  * pair: sep (> 0 separation distance, < 0 penetration depth), normal3 (unit,
  * from a to b), flags (SD_GJK_* bits; -1 for ids or vertex ranges outside the
- * set), iters (support iterations spent, GJK + EPA, warm + cold).
+ * set, or a hull of more than 1,024 vertices), iters (support iterations spent,
+ * GJK + EPA, warm + cold).
  * warm_dir (nullable, 3 floats per pair): initial direction of a warm-started
  * query limited to warm_budget iterations; a warm query that does not converge
  * within its budget, or ends within near_margin of contact, is redone from the
@@ -455,7 +456,7 @@ sd_status sd_hull_set_params(sd_hull_ctx* ctx, const sd_hull_params* p);
 /* n bodies: pos4 (x, y, z, mass; mass 0 = static), vel4, quat4 (w, x, y, z,
  * unit), angvel4 (world), inertia6 (body frame about the centre: xx, yy, zz,
  * xy, xz, yz), radius (sphere radius where nv == 0), vstart/nv into verts4
- * (body-frame vertices, n_verts in all); scenes are consecutive body ranges
+ * (body-frame vertices, n_verts in all; at most 1,024 per body); scenes are consecutive body ranges
  * (scene_first[0] = 0 ... scene_first[nscenes] = n; NULL for one scene), at
  * most 65,536 bodies each, with no contacts across scenes. */
 sd_status sd_hull_upload(sd_hull_ctx* ctx, int n, const float* pos4, const float* vel4, const float* quat4,

@@ -480,8 +480,11 @@ struct EpaEntry {
     int4 res;      // flags, EPA iterations, has_face, 0
 };
 
+// Ids outside the set, vertex ranges outside the vertex array, or a hull of
+// more vertices than a feature key holds.
 __device__ __forceinline__ bool pair_is_bad(const HullSet& h, int ia, int ib, int n_hulls, int n_verts) {
     if (ia < 0 || ia >= n_hulls || ib < 0 || ib >= n_hulls) return true;
+    if (h.nv[ia] > GJK_MAX_VERTS_PER_HULL || h.nv[ib] > GJK_MAX_VERTS_PER_HULL) return true;
     return h.vstart[ia] + h.nv[ia] > n_verts || h.vstart[ib] + h.nv[ib] > n_verts;
 }
 

@@ -466,6 +466,8 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
         SD_REQUIRE(vstart[i] >= 0 && nv[i] >= 0 && vstart[i] + nv[i] <= n_verts,
                    "sd_hull_upload: body %d has a bad vertex range", i);
         SD_REQUIRE(nv[i] > 0 || radius[i] > 0.0f, "sd_hull_upload: body %d has neither vertices nor a radius", i);
+        SD_REQUIRE(nv[i] <= GJK_MAX_VERTS_PER_HULL, "sd_hull_upload: body %d has %d vertices (at most %d)", i, nv[i],
+                   GJK_MAX_VERTS_PER_HULL);
         const float q0 = quat4[4 * (size_t)i], q1 = quat4[4 * (size_t)i + 1], q2 = quat4[4 * (size_t)i + 2],
                     q3 = quat4[4 * (size_t)i + 3];
         SD_REQUIRE(fabsf(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3 - 1.0f) < 1e-4f,

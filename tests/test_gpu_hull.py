@@ -106,6 +106,22 @@ def test_gjk_rejects_bad_ids():
     res = d.download_gjk(d.gjk(bad, 4), 4)
     assert res["flags"][1] == -1 and res["flags"][2] == -1
     assert res["flags"][0] >= 0 and res["flags"][3] >= 0
+    # a hull of 1,025 vertices (more than a feature key holds) is rejected too
+    rng = np.random.default_rng(3)
+    big = rng.normal(size=(1025, 3))
+    big = 0.01 * big / np.linalg.norm(big, axis=1, keepdims=True)
+    h3 = {k: h[k][:2].copy() for k in ("pos4", "quat4")}
+    v0 = h["verts4"][h["vstart"][0]:h["vstart"][0] + h["nv"][0]]
+    verts = np.zeros((len(v0) + 1025, 4), np.float32)
+    verts[:len(v0)] = v0
+    verts[len(v0):, :3] = big
+    h3.update({"vstart": np.array([0, len(v0)], np.int32), "nv": np.array([len(v0), 1025], np.int32),
+               "verts4": verts})
+    h3["pos4"][1, :3] = h3["pos4"][0, :3]
+    d3 = H.DeviceHulls(h3)
+    pr = L.DeviceBuffer.from_array(np.array([[0, 1], [1, 0], [0, 0]], np.int32))
+    r3 = d3.download_gjk(d3.gjk(pr, 3), 3)
+    assert r3["flags"][0] == -1 and r3["flags"][1] == -1 and r3["flags"][2] >= 0
 
 
 def test_sap_batched_equals_each_scene(oracle):

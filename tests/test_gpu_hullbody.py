@@ -166,3 +166,19 @@ def test_hull_bodies_c5_full_size(oracle):
     say("contacts: gpu %d, oracle %d" % (len(gp), touching))
     assert len(gp) == touching and touching > 200000
     sc.close()
+
+
+def test_hull_upload_rejects_oversized_hull():
+    """A body of more than 1,024 vertices (more than a GJK feature key holds)
+    is refused at upload."""
+    say("hull upload oversized")
+    b = scene(0x4B06, 20, 0, 0.1)
+    big = np.zeros((1025, 4), np.float32)
+    big[:, :3] = np.random.default_rng(5).normal(size=(1025, 3)) * 0.005
+    b["vstart"][0] = len(b["verts4"])
+    b["nv"][0] = 1025
+    b["verts4"] = np.concatenate([b["verts4"], big])
+    sc = HB.HullBodyScene(law())
+    with pytest.raises(L.SdError, match="1,024|1024"):
+        sc.upload(b)
+    sc.close()
