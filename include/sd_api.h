@@ -424,8 +424,10 @@ sd_status sd_lbvh_query(const uint64_t* sorted_keys, int n, const int* left, con
  * Convex hulls (vertices in the body frame), spheres (no vertices, a radius)
  * and static triangles (three vertices, mass 0: the faces of a DEMMesh,
  * src/engine/entities/entity.hpp:64-93) stepped on the GPU in independent
- * scenes. Each step: batched sweep-and-prune over the body boxes (each scene on
- * its own axis), GJK/EPA of every candidate pair (the sd_gjk results: TRIANGLE
+ * scenes. Each step: a uniform grid per scene over the body boxes (fixed at
+ * upload: the cell twice the largest moving body's bounding radius; larger
+ * static bodies such as wall triangles kept out of it and tested against every
+ * body of their scene), GJK/EPA of every candidate pair (the sd_gjk results: TRIANGLE
  * and SPHERE shapes of sandbox/primitives.hpp:11-14 included; cold, or
  * warm-started by a learned network, sd_hull_set_warm), and
  * for each touching pair (sep < 0) a contact between the two contact

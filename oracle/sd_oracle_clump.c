@@ -496,12 +496,204 @@ static int cmp_entry(const void* a, const void* b) {
     return (x > y) - (x < y);
 }
 
+/* ---------------------------------------------------------- body broadphase */
+
+/* Bounding radius of a body about its centre: the largest vertex distance
+ * (body frame), or the sphere radius of a body without vertices. */
+static float hb_bound(const float* verts4, int nv, float radius) {
+    if (nv == 0) return radius;
+    float r = 0.0f;
+    for (int k = 0; k < nv; ++k) {
+        const float* v = verts4 + 4 * (long)k;
+        r = fmaxf(r, sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    }
+    return r;
+}
+
+#define HB_GRID_AXIS_MAX 512
+#define HB_GRID_CELLS_MAX 4194304L
+
+/* Per-scene uniform grid of the rigid-body broadphase, fixed from the initial
+ * bodies (sd_hull_upload). rd = the largest bounding radius of the scene's
+ * moving bodies (mass > 0; of all its bodies when none moves); a body whose
+ * bounding radius exceeds rd (a large static body such as a wall triangle)
+ * stays out of the grid and is tested against the scene's bodies directly
+ * (large[b] = 1). Cell = 2 rd (inflated by 2^-10, so that two gridded bodies
+ * whose boxes overlap have centres at most one cell apart per axis), over the
+ * box of the gridded bodies' bounding spheres; the cell grows by 1.25 until no
+ * axis has more than 512 cells and the scene no more than 2^22. grid8[8 s ..]
+ * = (lo x, y, z, 1 / cell, nx, ny, nz, first cell id) with the last four as
+ * int bits. Returns the cell count of all scenes. */
+long or_hull_grid(int n, const float* pos4, const float* radius, const int* vstart, const int* nv,
+                  const float* verts4, const int* scene_first, int nscenes, float* grid8, int* large) {
+    (void)n;
+    long cbase = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        float rd = 0.0f, ra = 0.0f;
+        int moving = 0;
+        for (int b = scene_first[s]; b < scene_first[s + 1]; ++b) {
+            const float r = hb_bound(verts4 + 4 * (long)vstart[b], nv[b], radius[b]);
+            ra = fmaxf(ra, r);
+            if (pos4[4 * (long)b + 3] > 0.0f) {
+                rd = fmaxf(rd, r);
+                moving = 1;
+            }
+        }
+        if (!moving) rd = ra;
+        float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+        int first = 1;
+        for (int b = scene_first[s]; b < scene_first[s + 1]; ++b) {
+            const float r = hb_bound(verts4 + 4 * (long)vstart[b], nv[b], radius[b]);
+            large[b] = r > rd;
+            if (large[b]) continue;
+            for (int a = 0; a < 3; ++a) {
+                const float x = pos4[4 * (long)b + a];
+                lo[a] = first ? x - r : fminf(lo[a], x - r);
+                hi[a] = first ? x + r : fmaxf(hi[a], x + r);
+            }
+            first = 0;
+        }
+        float cell = 2.0f * rd * 1.0009765625f;
+        if (!(cell > 0.0f)) cell = 1e-3f;
+        float inv;
+        int d[3];
+        for (;;) {
+            inv = 1.0f / cell;
+            long cells = 1;
+            int ok = 1;
+            for (int a = 0; a < 3; ++a) {
+                const float e = ceilf((hi[a] - lo[a]) * inv);
+                d[a] = e >= 1.0f ? (e <= (float)HB_GRID_AXIS_MAX ? (int)e : HB_GRID_AXIS_MAX + 1) : 1;
+                ok = ok && d[a] <= HB_GRID_AXIS_MAX;
+                cells *= d[a];
+            }
+            if (ok && cells <= HB_GRID_CELLS_MAX) break;
+            cell = cell * 1.25f;
+        }
+        float* g = grid8 + 8 * (long)s;
+        g[0] = lo[0];
+        g[1] = lo[1];
+        g[2] = lo[2];
+        g[3] = inv;
+        int* gi = (int*)(g + 4);
+        gi[0] = d[0];
+        gi[1] = d[1];
+        gi[2] = d[2];
+        gi[3] = (int)cbase;
+        cbase += (long)d[0] * d[1] * d[2];
+    }
+    return cbase;
+}
+
+/* cell coordinate of x on one axis of a scene grid, clamped to [0, nd - 1] */
+static int hb_cell_axis(float x, float lo, float inv, int nd) {
+    const float t = fminf(fmaxf(floorf((x - lo) * inv), 0.0f), (float)(nd - 1));
+    return (int)t;
+}
+
+static int hb_boxes_overlap(const float* l1, const float* h1, const float* l2, const float* h2) {
+    return l2[0] <= h1[0] && l1[0] <= h2[0] && l2[1] <= h1[1] && l1[1] <= h2[1] && l2[2] <= h1[2] && l1[2] <= h2[2];
+}
+
+static int hb_pair_ok(const float* P, const float* lo, const float* hi, int a, int b) {
+    if (P[4 * (long)a + 3] == 0.0f && P[4 * (long)b + 3] == 0.0f) return 0; /* two static bodies */
+    return hb_boxes_overlap(lo + 4 * (long)a, hi + 4 * (long)a, lo + 4 * (long)b, hi + 4 * (long)b);
+}
+
+static int* hb_push(int* pairs, long* cap, long np, int a, int b) {
+    if (np >= *cap) {
+        *cap *= 2;
+        pairs = (int*)realloc(pairs, 2 * (size_t)*cap * sizeof(int));
+    }
+    pairs[2 * np] = a;
+    pairs[2 * np + 1] = b;
+    return pairs;
+}
+
+/* Candidate pairs of the rigid-body step, body a by body a (index order); for
+ * each gridded body a, the bodies of its scene whose boxes overlap a's
+ * (closed intervals), except pairs of two static bodies (1/m = 0, P.w): first
+ * the gridded b > a in the 27 cells around a's own (dz, dy, dx ascending;
+ * within a cell in index order), then the scene's large bodies L in index
+ * order, as (min(a, L), max(a, L)). A large body is static (its bounding
+ * radius exceeds every moving body's), so two large bodies never pair and a
+ * large body adds no pairs of its own. Returns the pair count (pairs grown by
+ * realloc). */
+static long hb_grid_pairs(int n, const float* P, const float* lo, const float* hi, const int* scene_first,
+                          int nscenes, const float* grid8, const int* large, int** pairs_io, long* cap_io) {
+    long np = 0;
+    int* pairs = *pairs_io;
+    long cap = *cap_io;
+    int* cellof = (int*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    for (int s = 0; s < nscenes; ++s) {
+        const int f = scene_first[s], k = scene_first[s + 1] - f;
+        if (k < 2) continue;
+        const float* g = grid8 + 8 * (long)s;
+        const int* gi = (const int*)(g + 4);
+        const int nx = gi[0], ny = gi[1], nz = gi[2];
+        const long ncell = (long)nx * ny * nz;
+        int* start = (int*)calloc((size_t)ncell + 1, sizeof(int));
+        int* order = (int*)malloc((size_t)k * sizeof(int));
+        for (int b = f; b < f + k; ++b) {
+            if (large[b]) continue;
+            const int cx = hb_cell_axis(P[4 * (long)b], g[0], g[3], nx);
+            const int cy = hb_cell_axis(P[4 * (long)b + 1], g[1], g[3], ny);
+            const int cz = hb_cell_axis(P[4 * (long)b + 2], g[2], g[3], nz);
+            cellof[b] = (cz * ny + cy) * nx + cx;
+            ++start[cellof[b] + 1];
+        }
+        for (long c = 0; c < ncell; ++c) start[c + 1] += start[c];
+        int* fill = (int*)calloc((size_t)ncell, sizeof(int));
+        for (int b = f; b < f + k; ++b)
+            if (!large[b]) order[start[cellof[b]] + fill[cellof[b]]++] = b; /* stable: index order */
+        free(fill);
+        for (int a = f; a < f + k; ++a) {
+            if (large[a]) continue;
+            const int ca = cellof[a];
+            const int cx = ca % nx, cy = (ca / nx) % ny, cz = ca / (nx * ny);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int x = cx + dx, y = cy + dy, z = cz + dz;
+                        if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) continue;
+                        const int c = (z * ny + y) * nx + x;
+                        for (int t = start[c]; t < start[c + 1]; ++t) {
+                            const int b = order[t];
+                            if (b > a && hb_pair_ok(P, lo, hi, a, b)) pairs = hb_push(pairs, &cap, np++, a, b);
+                        }
+                    }
+            for (int b = f; b < f + k; ++b)
+                if (large[b] && hb_pair_ok(P, lo, hi, a, b))
+                    pairs = hb_push(pairs, &cap, np++, a < b ? a : b, a < b ? b : a);
+        }
+        free(start);
+        free(order);
+    }
+    free(cellof);
+    *pairs_io = pairs;
+    *cap_io = cap;
+    return np;
+}
+
+long or_hull_grid_pairs(int n, const float* P, const float* lo4, const float* hi4, const int* scene_first,
+                        int nscenes, const float* grid8, const int* large, int* pairs, long capacity) {
+    long cap = 1024;
+    int* all = (int*)malloc(2 * (size_t)cap * sizeof(int));
+    const long np = hb_grid_pairs(n, P, lo4, hi4, scene_first, nscenes, grid8, large, &all, &cap);
+    for (long t = 0; t < np && t < capacity; ++t) {
+        pairs[2 * t] = all[2 * t];
+        pairs[2 * t + 1] = all[2 * t + 1];
+    }
+    free(all);
+    return np;
+}
+
 long or_hull_step(int n, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const float* radius,
                   const int* vstart, const int* nv, const float* verts4, int n_verts, const int* scene_first,
-                  int nscenes, const or_contact* c, int max_iter, int nthreads, int* pairs_out, float* force_out,
-                  float* point_out, long out_capacity) {
+                  int nscenes, const float* grid8, const int* large, const or_contact* c, int max_iter, int nthreads,
+                  int* pairs_out, float* force_out, float* point_out, long out_capacity) {
     (void)n_verts;
-    float* spos = (float*)malloc(4 * (size_t)n * sizeof(float));
+    float* spos = (float*)calloc(4 * (size_t)(n > 0 ? n : 1), sizeof(float));
     float* om = (float*)malloc(3 * (size_t)n * sizeof(float));
     float* lo = (float*)malloc(4 * (size_t)n * sizeof(float));
     float* hi = (float*)malloc(4 * (size_t)n * sizeof(float));
@@ -518,22 +710,10 @@ long or_hull_step(int n, float* P, float* V, float* Q, float* L, const float* I0
         om[3 * b + 2] = w.z;
     }
     or_hull_aabbs(n, spos, Q, vstart, nv, verts4, lo, hi);
-    /* candidate pairs, scene by scene */
-    long cap = 1024, np = 0;
+    /* candidate pairs: the per-scene grids */
+    long cap = 1024;
     int* pairs = (int*)malloc(2 * (size_t)cap * sizeof(int));
-    for (int s = 0; s < nscenes; ++s) {
-        const int f = scene_first[s], k = scene_first[s + 1] - f;
-        if (k < 2) continue;
-        const int axis = or_sap_axis(k, lo + 4 * (long)f, hi + 4 * (long)f);
-        long m = or_sap_pairs(k, lo + 4 * (long)f, hi + 4 * (long)f, axis, NULL, 0);
-        if (np + m > cap) {
-            while (np + m > cap) cap *= 2;
-            pairs = (int*)realloc(pairs, 2 * (size_t)cap * sizeof(int));
-        }
-        or_sap_pairs(k, lo + 4 * (long)f, hi + 4 * (long)f, axis, pairs + 2 * np, m);
-        for (long t = 2 * np; t < 2 * (np + m); ++t) pairs[t] += f;
-        np += m;
-    }
+    const long np = hb_grid_pairs(n, P, lo, hi, scene_first, nscenes, grid8, large, &pairs, &cap);
     float* sep = (float*)malloc((size_t)(np + 1) * sizeof(float));
     float* nrm = (float*)malloc(3 * (size_t)(np + 1) * sizeof(float));
     int* flags = (int*)malloc((size_t)(np + 1) * sizeof(int));

@@ -3,7 +3,9 @@
 // vertices, mass 0) in independent scenes, each step
 //   pose      shape centre (x, y, z, radius) and world angular velocity
 //   boxes     k_hull_aabb
-//   pairs     batched sweep-and-prune (each scene on its own axis)
+//   pairs     a uniform grid per scene (fixed at upload): bodies sorted by
+//             cell, each body's overlapping partners of higher index in the
+//             27 cells around its own
 //   narrow    GJK/EPA of every candidate pair: cold, or warm-started from
 //             the directions of an attached learned model (sd_hull_set_warm),
 //             the rejected warm queries redone cold
@@ -101,6 +103,122 @@ __device__ __forceinline__ V3 feature_centroid(const float4* __restrict__ verts,
 __device__ __forceinline__ V3 contact_point(V3 ca, float wa, V3 cb, float wb) {
     const float t = wa < wb ? 0.0f : (wb < wa ? 1.0f : 0.5f);
     return add(ca, scl(sub(cb, ca), t));
+}
+
+// Broadphase grid of one scene, fixed at upload from the initial bodies:
+// origin, 1 / cell, cells per axis, the scene's first global cell id and its
+// bodies kept out of the grid (llist[lfirst .. lfirst + lcount), index
+// order). The cell is twice the largest bounding radius rd of the scene's
+// moving bodies (inflated by 2^-10), so two gridded bodies whose boxes
+// overlap have centres at most one cell apart on every axis; bodies outside
+// the grid are clamped to its border cells, which only merges cells. A body
+// whose bounding radius exceeds rd (a wall triangle) is static and is tested
+// against every gridded body of its scene instead.
+struct HbGrid {
+    float lox, loy, loz, inv;
+    int nx, ny, nz, cbase, lfirst, lcount;
+};
+
+__device__ __forceinline__ int hb_cell_axis(float x, float lo, float inv, int nd) {
+    return (int)fminf(fmaxf(floorf((x - lo) * inv), 0.0f), (float)(nd - 1));
+}
+
+// key = (global cell of the body's centre) << 32 | body index; a body kept
+// out of the grids takes cell ncells, past every scene's cells
+__global__ void k_hb_cell_keys(const float4* __restrict__ P, const int* __restrict__ scene_of,
+                               const HbGrid* __restrict__ grid, const int* __restrict__ large, int n, int ncells,
+                               uint64_t* __restrict__ keys) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    uint32_t cell = (uint32_t)ncells;
+    if (!large[b]) {
+        const HbGrid g = grid[scene_of[b]];
+        const float4 p = P[b];
+        const int cx = hb_cell_axis(p.x, g.lox, g.inv, g.nx);
+        const int cy = hb_cell_axis(p.y, g.loy, g.inv, g.ny);
+        const int cz = hb_cell_axis(p.z, g.loz, g.inv, g.nz);
+        cell = (uint32_t)g.cbase + (uint32_t)((cz * g.ny + cy) * g.nx + cx);
+    }
+    keys[b] = ((uint64_t)cell << 32) | (uint64_t)(uint32_t)b;
+}
+
+// Candidate pairs of body a (one thread per body) whose boxes overlap a's on
+// all three axes (closed intervals), except pairs of two static bodies: the 27
+// cells around its own (dz, dy, dx ascending), in each the bodies b > a in
+// index order (keys sorted by cell, then index), then the scene's large
+// bodies L in index order as (min(a, L), max(a, L)). A large body (static,
+// larger than every moving body) has none of its own. WRITE = false counts
+// them into cnt[a]; WRITE = true writes them from offs[a] (pairs past
+// capacity are dropped: the total exceeds the capacity and the step reports
+// it).
+template <bool WRITE>
+__global__ void k_hb_grid_pairs(const float4* __restrict__ P, const float4* __restrict__ lo,
+                                const float4* __restrict__ hi, const int* __restrict__ scene_of,
+                                const HbGrid* __restrict__ grid, const int* __restrict__ large,
+                                const int* __restrict__ llist, const uint64_t* __restrict__ sorted,
+                                const int* __restrict__ cell_start, int n, int* __restrict__ cnt,
+                                const int* __restrict__ offs, int* __restrict__ pairs, int capacity) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    if (large[a]) {
+        if (!WRITE) cnt[a] = 0;
+        return;
+    }
+    const HbGrid g = grid[scene_of[a]];
+    const float4 pa = P[a], la = lo[a], ha = hi[a];
+    const bool sa = pa.w == 0.0f;  // static (1/m = 0)
+    const int cx = hb_cell_axis(pa.x, g.lox, g.inv, g.nx);
+    const int cy = hb_cell_axis(pa.y, g.loy, g.inv, g.ny);
+    const int cz = hb_cell_axis(pa.z, g.loz, g.inv, g.nz);
+    int k = 0;
+    const int base = WRITE ? offs[a] : 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= g.ny) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = cx + dx;
+                if (x < 0 || x >= g.nx) continue;
+                const int c = g.cbase + (z * g.ny + y) * g.nx + x;
+                const int t1 = cell_start[c + 1];
+                for (int t = cell_start[c]; t < t1; ++t) {
+                    const int b = (int)(uint32_t)sorted[t];
+                    if (b <= a) continue;
+                    if (sa && P[b].w == 0.0f) continue;
+                    const float4 lb = lo[b], hb = hi[b];
+                    if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z &&
+                          la.z <= hb.z))
+                        continue;
+                    if (WRITE) {
+                        const int o = base + k;
+                        if (o < capacity) {
+                            pairs[2 * (size_t)o] = a;
+                            pairs[2 * (size_t)o + 1] = b;
+                        }
+                    }
+                    ++k;
+                }
+            }
+        }
+    }
+    const int lend = sa ? g.lfirst : g.lfirst + g.lcount;  // large bodies are static
+    for (int t = g.lfirst; t < lend; ++t) {
+        const int b = llist[t];
+        const float4 lb = lo[b], hb = hi[b];
+        if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z && la.z <= hb.z))
+            continue;
+        if (WRITE) {
+            const int o = base + k;
+            if (o < capacity) {
+                pairs[2 * (size_t)o] = min(a, b);
+                pairs[2 * (size_t)o + 1] = max(a, b);
+            }
+        }
+        ++k;
+    }
+    if (!WRITE) cnt[a] = k;
 }
 
 struct HbBodies {
@@ -239,9 +357,16 @@ struct sd_hull_ctx {
     float* radius = nullptr;
     int *vstart = nullptr, *nv = nullptr, *scene_of = nullptr, *scene_first = nullptr;
     int* pairs = nullptr;
-    void* sap_scratch = nullptr;
-    size_t sap_scratch_bytes = 0;
-    int *total = nullptr, *sap_error = nullptr;  // inside sap_scratch
+    // broadphase: per-scene grids, the bodies kept out of them (flag per body,
+    // per-scene lists), cell keys (two buffers), cell starts, per-body pair
+    // counts and offsets (the total at offs[n]), sort scratch
+    HbGrid* grid = nullptr;
+    int *large = nullptr, *llist = nullptr;
+    long long ncells = 0;
+    uint64_t *gkeys = nullptr, *gkeys2 = nullptr;
+    int *cell_start = nullptr, *gcnt = nullptr, *goff = nullptr;
+    void* sort_tmp = nullptr;
+    int *total = nullptr, *sap_error = nullptr;  // total = goff + n; sap_error: a zero word
     float *sep = nullptr, *normal = nullptr;
     int *flags = nullptr, *iters = nullptr;
     void* gjk_scratch = nullptr;
@@ -300,23 +425,23 @@ sd_status hb_alloc_warm(sd_hull_ctx* c) {
 void hb_free(sd_hull_ctx* c) {
     hb_drop_graph(c);
     hb_free_warm(c);
-    void* bufs[] = {c->P,        c->V,        c->Q,      c->L,     c->I0,          c->I1,    c->spos,   c->omega,
-                    c->lo,       c->hi,       c->verts,  c->radius, c->vstart,     c->nv,    c->scene_of,
-                    c->scene_first, c->pairs, c->sap_scratch, c->sep, c->normal,   c->flags, c->iters,
-                    c->gjk_scratch, c->pf,    c->pp,     c->F,     c->T,           c->cnt,   c->fillc,  c->offs,
-                    c->list,     c->err,      c->scan_tmp};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
-    void** ptrs[] = {(void**)&c->P,     (void**)&c->V,          (void**)&c->Q,           (void**)&c->L,
-                     (void**)&c->I0,    (void**)&c->I1,         (void**)&c->spos,        (void**)&c->omega,
-                     (void**)&c->lo,    (void**)&c->hi,         (void**)&c->verts,       (void**)&c->radius,
-                     (void**)&c->vstart, (void**)&c->nv,        (void**)&c->scene_of,    (void**)&c->scene_first,
-                     (void**)&c->pairs, (void**)&c->sap_scratch, (void**)&c->sep,        (void**)&c->normal,
-                     (void**)&c->flags, (void**)&c->iters,      (void**)&c->gjk_scratch, (void**)&c->pf,
-                     (void**)&c->pp,    (void**)&c->F,          (void**)&c->T,           (void**)&c->cnt,
-                     (void**)&c->fillc, (void**)&c->offs,       (void**)&c->list,        (void**)&c->err,
-                     (void**)&c->scan_tmp};
-    for (void** q : ptrs) *q = nullptr;
+    void** ptrs[] = {(void**)&c->P,          (void**)&c->V,        (void**)&c->Q,          (void**)&c->L,
+                     (void**)&c->I0,         (void**)&c->I1,       (void**)&c->spos,       (void**)&c->omega,
+                     (void**)&c->lo,         (void**)&c->hi,       (void**)&c->verts,      (void**)&c->radius,
+                     (void**)&c->vstart,     (void**)&c->nv,       (void**)&c->scene_of,   (void**)&c->scene_first,
+                     (void**)&c->pairs,      (void**)&c->grid,     (void**)&c->gkeys,      (void**)&c->gkeys2,
+                     (void**)&c->large,      (void**)&c->llist,
+                     (void**)&c->cell_start, (void**)&c->gcnt,     (void**)&c->goff,       (void**)&c->sort_tmp,
+                     (void**)&c->sap_error,  (void**)&c->sep,      (void**)&c->normal,     (void**)&c->flags,
+                     (void**)&c->iters,      (void**)&c->gjk_scratch, (void**)&c->pf,      (void**)&c->pp,
+                     (void**)&c->F,          (void**)&c->T,        (void**)&c->cnt,        (void**)&c->fillc,
+                     (void**)&c->offs,       (void**)&c->list,     (void**)&c->err,        (void**)&c->scan_tmp};
+    for (void** q : ptrs) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+    c->total = nullptr;
+    c->ncells = 0;
     c->n = 0;
 }
 
@@ -344,9 +469,32 @@ sd_status hull_enqueue_step(sd_hull_ctx* c, hipStream_t s) {
     SD_LAUNCH_CHECK();
     sd_status st = launch_hull_aabbs(c->spos, c->Q, c->vstart, c->nv, c->verts, n, c->lo, c->hi, s);
     if (st != SD_OK) return st;
-    st = enqueue_sap_batched(c->lo, c->hi, n, c->scene_of, c->scene_first, c->nscenes, c->pairs, cap, &c->total,
-                             &c->sap_error, c->sap_scratch, c->sap_scratch_bytes, s);
+    // candidate pairs: bodies sorted by grid cell (index order within a
+    // cell), cell starts, then per body its count, the offsets and the pairs
+    hipLaunchKernelGGL(k_hb_cell_keys, dim3(div_up(n, 256)), dim3(256), 0, s, (const float4*)c->P,
+                       (const int*)c->scene_of, (const HbGrid*)c->grid, (const int*)c->large, n, (int)c->ncells,
+                       c->gkeys);
+    SD_LAUNCH_CHECK();
+    uint64_t *K = c->gkeys, *Ka = c->gkeys2;
+    int cbits = 1;  // cell ids run to ncells (the bodies kept out of the grids)
+    while (cbits < 32 && (1ll << cbits) <= c->ncells) ++cbits;
+    st = radix_sort_keys_u64(&K, &Ka, n, 32, 32 + cbits, c->sort_tmp, radix_temp_bytes(n), s, 0);
+    if (st == SD_OK)
+        st = launch_step_starts_u64(K, n, (int)c->ncells, c->cell_start, nullptr, false, nullptr, nullptr, nullptr,
+                                    s);
     if (st != SD_OK) return st;
+    hipLaunchKernelGGL((k_hb_grid_pairs<false>), dim3(div_up(n, 256)), dim3(256), 0, s, (const float4*)c->P,
+                       (const float4*)c->lo, (const float4*)c->hi, (const int*)c->scene_of, (const HbGrid*)c->grid,
+                       (const int*)c->large, (const int*)c->llist, (const uint64_t*)K, (const int*)c->cell_start, n,
+                       c->gcnt, (const int*)nullptr, c->pairs, cap);
+    SD_LAUNCH_CHECK();
+    st = exclusive_scan_i32(c->gcnt, c->goff, n, c->goff + n, c->scan_tmp, s);
+    if (st != SD_OK) return st;
+    hipLaunchKernelGGL((k_hb_grid_pairs<true>), dim3(div_up(n, 256)), dim3(256), 0, s, (const float4*)c->P,
+                       (const float4*)c->lo, (const float4*)c->hi, (const int*)c->scene_of, (const HbGrid*)c->grid,
+                       (const int*)c->large, (const int*)c->llist, (const uint64_t*)K, (const int*)c->cell_start, n,
+                       c->gcnt, (const int*)c->goff, c->pairs, cap);
+    SD_LAUNCH_CHECK();
     if (c->model) {
         st = enqueue_warm_dirs(c->pairs, cap, c->total, c->spos, c->Q, c->desc, c->X, c->model, c->wdir, s);
         if (st == SD_OK)
@@ -401,6 +549,92 @@ float hull_dt(const sd_hull_ctx* c) { return c->p.contact.dt; }
 void hull_count_steps(sd_hull_ctx* c, int nsteps) { c->steps_done += nsteps; }
 
 }  // namespace sd
+
+namespace {
+
+// Bounding radius of a body about its centre: the largest vertex distance
+// (body frame), or the sphere radius of a body without vertices.
+float hb_bound(const float* verts4, int nv, float radius) {
+    if (nv == 0) return radius;
+    float r = 0.0f;
+    for (int k = 0; k < nv; ++k) {
+        const float* v = verts4 + 4 * (size_t)k;
+        r = fmaxf(r, sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    }
+    return r;
+}
+
+constexpr int HB_GRID_AXIS_MAX = 512;
+constexpr long long HB_GRID_CELLS_MAX = 1ll << 22;
+
+// The grid of every scene from the initial bodies: rd = the largest bounding
+// radius of its moving bodies (of all its bodies when none moves); a body
+// whose bounding radius exceeds rd stays out of the grid (large[b] = 1,
+// listed in llist in index order); cell = 2 rd (times 1 + 2^-10) over the box
+// of the gridded bodies' bounding spheres, grown by 1.25 until no axis has
+// more than 512 cells and the scene no more than 2^22. Returns the cell count
+// of all scenes.
+long long hb_grids(const float* pos4, const float* radius, const int* vstart, const int* nv, const float* verts4,
+                   const std::vector<int>& sfirst, std::vector<HbGrid>& grids, std::vector<int>& large,
+                   std::vector<int>& llist) {
+    const int nscenes = (int)sfirst.size() - 1;
+    grids.assign(nscenes, HbGrid{});
+    large.assign(sfirst[nscenes], 0);
+    llist.clear();
+    long long cbase = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        float rd = 0.0f, ra = 0.0f, lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+        bool moving = false;
+        for (int b = sfirst[s]; b < sfirst[s + 1]; ++b) {
+            const float r = hb_bound(verts4 + 4 * (size_t)vstart[b], nv[b], radius[b]);
+            ra = fmaxf(ra, r);
+            if (pos4[4 * (size_t)b + 3] > 0.0f) {
+                rd = fmaxf(rd, r);
+                moving = true;
+            }
+        }
+        if (!moving) rd = ra;
+        const int lfirst = (int)llist.size();
+        bool first = true;
+        for (int b = sfirst[s]; b < sfirst[s + 1]; ++b) {
+            const float r = hb_bound(verts4 + 4 * (size_t)vstart[b], nv[b], radius[b]);
+            if (r > rd) {
+                large[b] = 1;
+                llist.push_back(b);
+                continue;
+            }
+            for (int a = 0; a < 3; ++a) {
+                const float x = pos4[4 * (size_t)b + a];
+                lo[a] = first ? x - r : fminf(lo[a], x - r);
+                hi[a] = first ? x + r : fmaxf(hi[a], x + r);
+            }
+            first = false;
+        }
+        float cell = 2.0f * rd * 1.0009765625f;
+        if (!(cell > 0.0f)) cell = 1e-3f;
+        float inv;
+        int d[3];
+        for (;;) {
+            inv = 1.0f / cell;
+            long long cells = 1;
+            bool ok = true;
+            for (int a = 0; a < 3; ++a) {
+                const float e = ceilf((hi[a] - lo[a]) * inv);
+                d[a] = e >= 1.0f ? (e <= (float)HB_GRID_AXIS_MAX ? (int)e : HB_GRID_AXIS_MAX + 1) : 1;
+                ok = ok && d[a] <= HB_GRID_AXIS_MAX;
+                cells *= d[a];
+            }
+            if (ok && cells <= HB_GRID_CELLS_MAX) break;
+            cell = cell * 1.25f;
+        }
+        grids[s] = HbGrid{lo[0], lo[1], lo[2], inv, d[0], d[1], d[2], (int)cbase, lfirst,
+                          (int)llist.size() - lfirst};
+        cbase += (long long)d[0] * d[1] * d[2];
+    }
+    return cbase;
+}
+
+}  // namespace
 
 extern "C" sd_status sd_hull_create(const sd_hull_params* p, sd_hull_ctx** out) {
     SD_REQUIRE(out, "sd_hull_create: NULL out");
@@ -483,11 +717,15 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     const long long capl = std::max(64ll, (long long)ppb * n);
     SD_REQUIRE(capl < (1ll << 30), "sd_hull_upload: pair capacity %lld too large", capl);
     const int cap = (int)capl;
+    std::vector<HbGrid> grids;
+    std::vector<int> large, llist;
+    const long long ncells = hb_grids(pos4, radius, vstart, nv, verts4, sfirst, grids, large, llist);
+    SD_REQUIRE(ncells < (1ll << 31) - 1, "sd_hull_upload: %lld broadphase cells", ncells);
     c->n = n;
     c->n_verts = n_verts;
     c->nscenes = nscenes;
     c->capacity = cap;
-    sd_sap_batched_scratch_bytes(n, nscenes, &c->sap_scratch_bytes);
+    c->ncells = ncells;
 #define ALLOC(ptr, cnt) \
     if (st == SD_OK) st = halloc(&(ptr), (size_t)(cnt))
     ALLOC(c->P, n);
@@ -507,7 +745,16 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     ALLOC(c->scene_of, n);
     ALLOC(c->scene_first, nscenes + 1);
     ALLOC(c->pairs, 2 * (size_t)cap);
-    ALLOC(*(char**)&c->sap_scratch, c->sap_scratch_bytes);
+    ALLOC(c->grid, nscenes);
+    ALLOC(c->large, n);
+    ALLOC(c->llist, llist.size());
+    ALLOC(c->gkeys, n);
+    ALLOC(c->gkeys2, n);
+    ALLOC(c->cell_start, ncells + 1);
+    ALLOC(c->gcnt, n);
+    ALLOC(c->goff, n + 1);
+    ALLOC(*(char**)&c->sort_tmp, radix_temp_bytes(n));
+    ALLOC(c->sap_error, 1);
     ALLOC(c->sep, cap);
     ALLOC(c->normal, 3 * (size_t)cap);
     ALLOC(c->flags, cap);
@@ -544,7 +791,13 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     SD_HIP(hipMemcpyAsync(c->nv, nv, n * sizeof(int), hipMemcpyHostToDevice, s));
     SD_HIP(hipMemcpyAsync(c->scene_of, sof.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
     SD_HIP(hipMemcpyAsync(c->scene_first, sfirst.data(), (nscenes + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->grid, grids.data(), nscenes * sizeof(HbGrid), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->large, large.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    if (!llist.empty())
+        SD_HIP(hipMemcpyAsync(c->llist, llist.data(), llist.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    c->total = c->goff + n;
     st = launch_fill_u32((uint32_t*)c->err, 0u, 1, s);
+    if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->sap_error, 0u, 1, s);
     if (st != SD_OK) return st;
     // spos (centre, radius) for the descriptors, as the step's pose stage writes it
     std::vector<float4> sp(n);

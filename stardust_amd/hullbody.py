@@ -1,6 +1,7 @@
 """Rigid convex bodies stepped on the GPU (wraps sd_hull_*): convex hulls,
-spheres and static triangles in independent scenes; sweep-and-prune, cold
-GJK/EPA and the contact law of every touching pair each step.
+spheres and static triangles in independent scenes; a uniform grid per scene,
+GJK/EPA (cold or learned warm start) and the contact law of every touching
+pair each step.
 
 make_bodies turns a hull set of hulls.generate_hulls into bodies (mass from the
 density and the ellipsoid of the vertex box, inertia of equal point masses at

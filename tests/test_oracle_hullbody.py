@@ -142,3 +142,48 @@ def test_scenes_are_independent(oracle):
     for k in ("P", "V", "Q", "L"):
         assert np.array_equal(getattr(s_all, k)[:40], getattr(s_a, k)), k
         assert np.array_equal(getattr(s_all, k)[40:], getattr(s_b, k)), k
+
+
+def test_grid_broadphase_equals_brute_force(oracle):
+    """The body step's candidate pairs come from per-scene grids fixed at
+    upload. After the bodies have moved, some of them far outside the initial
+    grid (clamped to its border cells, several of them next to each other),
+    the pairs are exactly the pairs of overlapping boxes (closed intervals)
+    within a scene, without pairs of two static bodies, each once as (a, b)
+    with a < b (brute force). The wall triangles, larger than any moving body,
+    stay out of the grids; one scene lists them after its hulls, one before."""
+    h1 = H.generate_hulls(0x4E01, 300, lo=(0.0, 0.0, 0.0), hi=(0.12, 0.12, 0.12))
+    s1, _ = HB.concat([HB.make_bodies(h1, 1), HB.box_walls((0.0, 0.0, 0.0), (0.12, 0.12, 0.12))])
+    h2 = H.generate_hulls(0x4E02, 200, lo=(0.0, 0.0, 0.0), hi=(0.1, 0.1, 0.1))
+    sp = sphere_bodies(np.random.default_rng(3).uniform(0.0, 0.1, (40, 3)), np.float32(0.006), np.float32(0.01))
+    h3 = H.generate_hulls(0x4E03, 150, lo=(0.0, 0.0, 0.0), hi=(0.08, 0.08, 0.08))
+    s3, _ = HB.concat([HB.box_walls((0.0, 0.0, 0.0), (0.08, 0.08, 0.08)), HB.make_bodies(h3, 3)])
+    bodies, first = HB.concat([s1, HB.concat([HB.make_bodies(h2, 2), sp])[0], s3])
+    n = len(bodies["pos4"])
+    st = oracle.HullBodyState(bodies, first)
+    walls = np.r_[np.arange(300, 312), np.arange(first[2], first[2] + 12)]
+    assert np.array_equal(np.nonzero(st.large)[0], walls)
+    rng = np.random.default_rng(7)
+    st.P[:, :3] += rng.normal(0.0, 0.01, (n, 3)).astype(np.float32)
+    far = rng.choice(n, 40, replace=False)
+    st.P[far, :3] += rng.uniform(-2.0, 2.0, (40, 3)).astype(np.float32)
+    st.P[far[1::2], :3] = st.P[far[0::2], :3] + np.float32(0.003)  # far bodies in overlapping pairs
+    spos = np.concatenate([st.P[:, :3], bodies["radius"][:, None]], 1).astype(np.float32)
+    lo, hi = oracle.hull_aabbs({"pos4": spos, "quat4": st.Q, "vstart": bodies["vstart"], "nv": bodies["nv"],
+                                "verts4": bodies["verts4"]})
+    got = st.grid_pairs(lo, hi)
+    assert np.all(got[:, 0] < got[:, 1])
+    static = st.P[:, 3] == 0.0
+    want = set()
+    for s in range(len(first) - 1):
+        ids = np.arange(first[s], first[s + 1])
+        l, u = lo[ids, :3], hi[ids, :3]
+        ov = np.all((l[:, None, :] <= u[None, :, :]) & (l[None, :, :] <= u[:, None, :]), axis=2)
+        for i, j in zip(*np.nonzero(np.triu(ov, 1))):
+            a, b = int(ids[i]), int(ids[j])
+            if not (static[a] and static[b]):
+                want.add((a, b))
+    assert len(got) == len(set(map(tuple, got.tolist())))
+    assert set(map(tuple, got.tolist())) == want
+    assert len(want) > 300 and any(a in far or b in far for a, b in want)
+    assert any(a in walls[12:] for a, b in want) and any(b in walls[:12] for a, b in want)
