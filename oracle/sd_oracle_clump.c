@@ -612,10 +612,14 @@ static int* hb_push(int* pairs, long* cap, long np, int a, int b) {
 
 /* Candidate pairs of the rigid-body step, body a by body a (index order); for
  * each gridded body a, the bodies of its scene whose boxes overlap a's
- * (closed intervals), except pairs of two static bodies (1/m = 0, P.w): first
- * the gridded b > a in the 27 cells around a's own (dz, dy, dx ascending;
- * within a cell in index order), then the scene's large bodies L in index
- * order, as (min(a, L), max(a, L)). A large body is static (its bounding
+ * (closed intervals), except pairs of two static bodies (1/m = 0, P.w), each
+ * pair written as (min, max): first the gridded bodies of a's own cell after
+ * it (b > a) and of the 13 cells that follow its own in (dz, dy, dx) order
+ * (dz = 1; dz = 0, dy = 1; dz = dy = 0, dx = 1), in that order and within a
+ * cell in index order, then the scene's large bodies in index order. Two
+ * gridded bodies whose boxes overlap lie in the same cell or in neighbouring
+ * ones (clamping to the border cells keeps that), so the pair is found once,
+ * from the body whose cell comes first (the lower index in a shared cell). A large body is static (its bounding
  * radius exceeds every moving body's), so two large bodies never pair and a
  * large body adds no pairs of its own. Returns the pair count (pairs grown by
  * realloc). */
@@ -651,15 +655,18 @@ static long hb_grid_pairs(int n, const float* P, const float* lo, const float* h
             if (large[a]) continue;
             const int ca = cellof[a];
             const int cx = ca % nx, cy = (ca / nx) % ny, cz = ca / (nx * ny);
-            for (int dz = -1; dz <= 1; ++dz)
+            for (int dz = 0; dz <= 1; ++dz)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
+                        if (dz == 0 && (dy < 0 || (dy == 0 && dx < 0))) continue; /* the other half */
                         const int x = cx + dx, y = cy + dy, z = cz + dz;
                         if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) continue;
                         const int c = (z * ny + y) * nx + x;
+                        const int own = dz == 0 && dy == 0 && dx == 0;
                         for (int t = start[c]; t < start[c + 1]; ++t) {
                             const int b = order[t];
-                            if (b > a && hb_pair_ok(P, lo, hi, a, b)) pairs = hb_push(pairs, &cap, np++, a, b);
+                            if ((!own || b > a) && hb_pair_ok(P, lo, hi, a, b))
+                                pairs = hb_push(pairs, &cap, np++, a < b ? a : b, a < b ? b : a);
                         }
                     }
             for (int b = f; b < f + k; ++b)

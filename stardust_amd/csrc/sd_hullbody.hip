@@ -4,8 +4,9 @@
 //   pose      shape centre (x, y, z, radius) and world angular velocity
 //   boxes     k_hull_aabb
 //   pairs     a uniform grid per scene (fixed at upload): bodies sorted by
-//             cell, each body's overlapping partners of higher index in the
-//             27 cells around its own
+//             cell, each body's overlapping partners in the later part of its
+//             own cell and the 13 cells after it; large static bodies (walls)
+//             tested against every body of their scene
 //   narrow    GJK/EPA of every candidate pair: cold, or warm-started from
 //             the directions of an attached learned model (sd_hull_set_warm),
 //             the rejected warm queries redone cold
@@ -52,27 +53,40 @@ __global__ void k_hb_pose(const float4* __restrict__ P, const float4* __restrict
     omega[b] = make_float4(w.x, w.y, w.z, 0.0f);
 }
 
+// Support-feature tolerance of every body: HB_FEATURE_REL times its radius
+// (largest vertex distance from its centre); 0 for a sphere. Fixed at upload.
+__global__ void k_hb_feature_tol(const float4* __restrict__ verts, const int* __restrict__ vstart,
+                                 const int* __restrict__ nv, int n, float* __restrict__ tol) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const float4* vb = verts + vstart[b];
+    float r2 = 0.0f;
+    for (int k = 0; k < nv[b]; ++k) {
+        const float4 v = vb[k];
+        r2 = fmaxf(r2, v.x * v.x + v.y * v.y + v.z * v.z);
+    }
+    tol[b] = HB_FEATURE_REL * sqrtf(r2);
+}
+
 // Contact feature of a shape along unit direction u: for a sphere the point
 // c + r u (spread 0); for a vertex set the vertices whose projection on u is
-// within HB_FEATURE_REL times the shape's radius (largest vertex distance from
-// its centre) of the largest, their
-// mean in vertex order (world) and spread2 = the largest squared distance of
-// one of them from that mean.
-__device__ __forceinline__ V3 feature_centroid(const float4* __restrict__ verts, int nv, float radius, const Rot& R,
-                                               V3 X, V3 u, float& spread2) {
+// within tol (k_hb_feature_tol) of the largest, their mean in vertex order
+// (world) and spread2 = the largest squared distance of one of them from that
+// mean. The vertices of a hull of up to 32 are selected once, as a bit mask.
+__device__ __forceinline__ V3 feature_centroid(const float4* __restrict__ verts, int nv, float radius, float tol,
+                                               const Rot& R, V3 X, V3 u, float& spread2) {
     spread2 = 0.0f;
     if (nv == 0) return add(X, scl(u, radius));
     const V3 l = rot_tmul(R, u);
-    float dmax = -3.402823466e38f, r2 = 0.0f;
+    float dmax = -3.402823466e38f;
     for (int k = 0; k < nv; ++k) {
         const float4 v = verts[k];
-        const float d = v.x * l.x + v.y * l.y + v.z * l.z;
-        dmax = fmaxf(dmax, d);
-        r2 = fmaxf(r2, v.x * v.x + v.y * v.y + v.z * v.z);
+        dmax = fmaxf(dmax, v.x * l.x + v.y * l.y + v.z * l.z);
     }
-    const float cut = dmax - HB_FEATURE_REL * sqrtf(r2);
+    const float cut = dmax - tol;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     int cnt = 0;
+    uint32_t sel = 0u;
     for (int k = 0; k < nv; ++k) {
         const float4 v = verts[k];
         const float d = v.x * l.x + v.y * l.y + v.z * l.z;
@@ -81,16 +95,25 @@ __device__ __forceinline__ V3 feature_centroid(const float4* __restrict__ verts,
             sy = sy + v.y;
             sz = sz + v.z;
             ++cnt;
+            if (k < 32) sel |= 1u << k;
         }
     }
     const float inv = 1.0f / (float)cnt;
     const float mx = sx * inv, my = sy * inv, mz = sz * inv;
-    for (int k = 0; k < nv; ++k) {
-        const float4 v = verts[k];
-        const float d = v.x * l.x + v.y * l.y + v.z * l.z;
-        if (d >= cut) {
+    if (nv <= 32) {
+        for (uint32_t m = sel; m; m &= m - 1u) {
+            const float4 v = verts[__builtin_ctz(m)];
             const float ex = v.x - mx, ey = v.y - my, ez = v.z - mz;
             spread2 = fmaxf(spread2, ex * ex + ey * ey + ez * ez);
+        }
+    } else {
+        for (int k = 0; k < nv; ++k) {
+            const float4 v = verts[k];
+            const float d = v.x * l.x + v.y * l.y + v.z * l.z;
+            if (d >= cut) {
+                const float ex = v.x - mx, ey = v.y - my, ez = v.z - mz;
+                spread2 = fmaxf(spread2, ex * ex + ey * ey + ez * ez);
+            }
         }
     }
     return add(rot_mul(R, v3(mx, my, mz)), X);
@@ -143,14 +166,16 @@ __global__ void k_hb_cell_keys(const float4* __restrict__ P, const int* __restri
 }
 
 // Candidate pairs of body a (one thread per body) whose boxes overlap a's on
-// all three axes (closed intervals), except pairs of two static bodies: the 27
-// cells around its own (dz, dy, dx ascending), in each the bodies b > a in
-// index order (keys sorted by cell, then index), then the scene's large
-// bodies L in index order as (min(a, L), max(a, L)). A large body (static,
-// larger than every moving body) has none of its own. WRITE = false counts
-// them into cnt[a]; WRITE = true writes them from offs[a] (pairs past
-// capacity are dropped: the total exceeds the capacity and the step reports
-// it).
+// all three axes (closed intervals), except pairs of two static bodies, each
+// written as (min, max): the bodies after a in its own cell, then those of the
+// 13 cells that follow its own in (dz, dy, dx) order, within a cell in index
+// order (keys sorted by cell, then index), then the scene's large bodies in
+// index order. Overlapping gridded bodies lie in the same or neighbouring
+// cells, so each such pair is found once, from the body whose cell comes
+// first. A large body (static, larger than every moving body) has none of its
+// own. WRITE = false counts them into cnt[a]; WRITE = true writes them from
+// offs[a] (pairs past capacity are dropped: the total exceeds the capacity and
+// the step reports it).
 template <bool WRITE>
 __global__ void k_hb_grid_pairs(const float4* __restrict__ P, const float4* __restrict__ lo,
                                 const float4* __restrict__ hi, const int* __restrict__ scene_of,
@@ -172,20 +197,21 @@ __global__ void k_hb_grid_pairs(const float4* __restrict__ P, const float4* __re
     const int cz = hb_cell_axis(pa.z, g.loz, g.inv, g.nz);
     int k = 0;
     const int base = WRITE ? offs[a] : 0;
-    for (int dz = -1; dz <= 1; ++dz) {
+    for (int dz = 0; dz <= 1; ++dz) {
         const int z = cz + dz;
-        if (z < 0 || z >= g.nz) continue;
+        if (z >= g.nz) continue;
         for (int dy = -1; dy <= 1; ++dy) {
             const int y = cy + dy;
-            if (y < 0 || y >= g.ny) continue;
+            if (y < 0 || y >= g.ny || (dz == 0 && dy < 0)) continue;
             for (int dx = -1; dx <= 1; ++dx) {
                 const int x = cx + dx;
-                if (x < 0 || x >= g.nx) continue;
+                if (x < 0 || x >= g.nx || (dz == 0 && dy == 0 && dx < 0)) continue;
+                const bool own = dz == 0 && dy == 0 && dx == 0;
                 const int c = g.cbase + (z * g.ny + y) * g.nx + x;
                 const int t1 = cell_start[c + 1];
                 for (int t = cell_start[c]; t < t1; ++t) {
                     const int b = (int)(uint32_t)sorted[t];
-                    if (b <= a) continue;
+                    if (own && b <= a) continue;
                     if (sa && P[b].w == 0.0f) continue;
                     const float4 lb = lo[b], hb = hi[b];
                     if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z &&
@@ -194,8 +220,8 @@ __global__ void k_hb_grid_pairs(const float4* __restrict__ P, const float4* __re
                     if (WRITE) {
                         const int o = base + k;
                         if (o < capacity) {
-                            pairs[2 * (size_t)o] = a;
-                            pairs[2 * (size_t)o + 1] = b;
+                            pairs[2 * (size_t)o] = min(a, b);
+                            pairs[2 * (size_t)o + 1] = max(a, b);
                         }
                     }
                     ++k;
@@ -227,6 +253,7 @@ struct HbBodies {
     const float4* Q;      // (w, x, y, z)
     const float4* omega;  // world angular velocity
     const float* radius;
+    const float* tol;  // support-feature tolerance
     const int* vstart;
     const int* nv;
     const float4* verts;
@@ -257,8 +284,9 @@ __global__ void k_hb_pairs(const int* __restrict__ pairs, int capacity, const in
     const V3 Xa = v3(Pa.x, Pa.y, Pa.z), Xb = v3(Pb.x, Pb.y, Pb.z);
     const Rot Ra = quat_rot(B.Q[a]), Rb = quat_rot(B.Q[b]);
     float wa, wb;
-    const V3 ca = feature_centroid(B.verts + B.vstart[a], B.nv[a], B.radius[a], Ra, Xa, n, wa);
-    const V3 cb = feature_centroid(B.verts + B.vstart[b], B.nv[b], B.radius[b], Rb, Xb, scl(n, -1.0f), wb);
+    const V3 ca = feature_centroid(B.verts + B.vstart[a], B.nv[a], B.radius[a], B.tol[a], Ra, Xa, n, wa);
+    const V3 cb = feature_centroid(B.verts + B.vstart[b], B.nv[b], B.radius[b], B.tol[b], Rb, Xb, scl(n, -1.0f),
+                                   wb);
     const V3 pt = contact_point(ca, wa, cb, wb);
     const float4 va4 = B.V[a], vb4 = B.V[b], wa4 = B.omega[a], wb4 = B.omega[b];
     const V3 va = add(v3(va4.x, va4.y, va4.z), cross(v3(wa4.x, wa4.y, wa4.z), sub(pt, Xa)));
@@ -354,7 +382,7 @@ struct sd_hull_ctx {
     int n = 0, n_verts = 0, nscenes = 0, capacity = 0;
     float4 *P = nullptr, *V = nullptr, *Q = nullptr, *L = nullptr, *I0 = nullptr, *I1 = nullptr;
     float4 *spos = nullptr, *omega = nullptr, *lo = nullptr, *hi = nullptr, *verts = nullptr;
-    float* radius = nullptr;
+    float *radius = nullptr, *ftol = nullptr;
     int *vstart = nullptr, *nv = nullptr, *scene_of = nullptr, *scene_first = nullptr;
     int* pairs = nullptr;
     // broadphase: per-scene grids, the bodies kept out of them (flag per body,
@@ -427,7 +455,7 @@ void hb_free(sd_hull_ctx* c) {
     hb_free_warm(c);
     void** ptrs[] = {(void**)&c->P,          (void**)&c->V,        (void**)&c->Q,          (void**)&c->L,
                      (void**)&c->I0,         (void**)&c->I1,       (void**)&c->spos,       (void**)&c->omega,
-                     (void**)&c->lo,         (void**)&c->hi,       (void**)&c->verts,      (void**)&c->radius,
+                     (void**)&c->lo,         (void**)&c->hi,       (void**)&c->verts,      (void**)&c->radius,       (void**)&c->ftol,
                      (void**)&c->vstart,     (void**)&c->nv,       (void**)&c->scene_of,   (void**)&c->scene_first,
                      (void**)&c->pairs,      (void**)&c->grid,     (void**)&c->gkeys,      (void**)&c->gkeys2,
                      (void**)&c->large,      (void**)&c->llist,
@@ -509,7 +537,7 @@ sd_status hull_enqueue_step(sd_hull_ctx* c, hipStream_t s) {
     st = launch_fill_u32((uint32_t*)c->cnt, 0u, n, s);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->fillc, 0u, n, s);
     if (st != SD_OK) return st;
-    const HbBodies B{c->P, c->V, c->Q, c->omega, c->radius, c->vstart, c->nv, c->verts};
+    const HbBodies B{c->P, c->V, c->Q, c->omega, c->radius, c->ftol, c->vstart, c->nv, c->verts};
     hipLaunchKernelGGL(k_hb_pairs, dim3(div_up(cap, 256)), dim3(256), 0, s, (const int*)c->pairs, cap,
                        (const int*)c->total, B, (const float*)c->sep, (const float*)c->normal, (const int*)c->flags,
                        c->c, c->pf, c->pp, c->cnt, c->err);
@@ -740,6 +768,7 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     ALLOC(c->hi, n);
     ALLOC(c->verts, n_verts);
     ALLOC(c->radius, n);
+    ALLOC(c->ftol, n);
     ALLOC(c->vstart, n);
     ALLOC(c->nv, n);
     ALLOC(c->scene_of, n);
@@ -796,6 +825,9 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     if (!llist.empty())
         SD_HIP(hipMemcpyAsync(c->llist, llist.data(), llist.size() * sizeof(int), hipMemcpyHostToDevice, s));
     c->total = c->goff + n;
+    hipLaunchKernelGGL(k_hb_feature_tol, dim3(div_up(n, 256)), dim3(256), 0, s, (const float4*)c->verts,
+                       (const int*)c->vstart, (const int*)c->nv, n, c->ftol);
+    SD_LAUNCH_CHECK();
     st = launch_fill_u32((uint32_t*)c->err, 0u, 1, s);
     if (st == SD_OK) st = launch_fill_u32((uint32_t*)c->sap_error, 0u, 1, s);
     if (st != SD_OK) return st;
