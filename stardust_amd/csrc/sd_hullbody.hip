@@ -165,38 +165,38 @@ __global__ void k_hb_cell_keys(const float4* __restrict__ P, const int* __restri
     keys[b] = ((uint64_t)cell << 32) | (uint64_t)(uint32_t)b;
 }
 
-// Candidate pairs of body a (one thread per body) whose boxes overlap a's on
-// all three axes (closed intervals), except pairs of two static bodies, each
-// written as (min, max): the bodies after a in its own cell, then those of the
-// 13 cells that follow its own in (dz, dy, dx) order, within a cell in index
-// order (keys sorted by cell, then index), then the scene's large bodies in
-// index order. Overlapping gridded bodies lie in the same or neighbouring
-// cells, so each such pair is found once, from the body whose cell comes
-// first. A large body (static, larger than every moving body) has none of its
-// own. WRITE = false counts them into cnt[a]; WRITE = true writes them from
-// offs[a] (pairs past capacity are dropped: the total exceeds the capacity and
-// the step reports it).
-template <bool WRITE>
-__global__ void k_hb_grid_pairs(const float4* __restrict__ P, const float4* __restrict__ lo,
-                                const float4* __restrict__ hi, const int* __restrict__ scene_of,
-                                const HbGrid* __restrict__ grid, const int* __restrict__ large,
-                                const int* __restrict__ llist, const uint64_t* __restrict__ sorted,
-                                const int* __restrict__ cell_start, int n, int* __restrict__ cnt,
-                                const int* __restrict__ offs, int* __restrict__ pairs, int capacity) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
-    if (large[a]) {
-        if (!WRITE) cnt[a] = 0;
-        return;
-    }
-    const HbGrid g = grid[scene_of[a]];
-    const float4 pa = P[a], la = lo[a], ha = hi[a];
+// Pairs a walk of the grid keeps in a body's own slots (int2 each); a body
+// with more walks the grid again and writes them in place.
+constexpr int HB_GRID_SLOTS = 16;
+
+struct HbGridArgs {
+    const float4* P;
+    const float4* lo;
+    const float4* hi;
+    const int* scene_of;
+    const HbGrid* grid;
+    const int* large;
+    const int* llist;
+    const uint64_t* sorted;  // keys sorted by cell, then index
+    const int* cell_start;
+};
+
+// Candidate pairs of a gridded body a whose boxes overlap a's on all three
+// axes (closed intervals), except pairs of two static bodies, each as (min,
+// max): the bodies after a in its own cell, then those of the 13 cells that
+// follow its own in (dz, dy, dx) order, within a cell in index order, then the
+// scene's large bodies in index order. Overlapping gridded bodies lie in the
+// same or neighbouring cells, so each such pair is found once, from the body
+// whose cell comes first. Pair k goes to emit(k, lo, hi); returns the count.
+template <typename Emit>
+__device__ __forceinline__ int hb_grid_walk(const HbGridArgs& G, int a, Emit emit) {
+    const HbGrid g = G.grid[G.scene_of[a]];
+    const float4 pa = G.P[a], la = G.lo[a], ha = G.hi[a];
     const bool sa = pa.w == 0.0f;  // static (1/m = 0)
     const int cx = hb_cell_axis(pa.x, g.lox, g.inv, g.nx);
     const int cy = hb_cell_axis(pa.y, g.loy, g.inv, g.ny);
     const int cz = hb_cell_axis(pa.z, g.loz, g.inv, g.nz);
     int k = 0;
-    const int base = WRITE ? offs[a] : 0;
     for (int dz = 0; dz <= 1; ++dz) {
         const int z = cz + dz;
         if (z >= g.nz) continue;
@@ -208,43 +208,71 @@ __global__ void k_hb_grid_pairs(const float4* __restrict__ P, const float4* __re
                 if (x < 0 || x >= g.nx || (dz == 0 && dy == 0 && dx < 0)) continue;
                 const bool own = dz == 0 && dy == 0 && dx == 0;
                 const int c = g.cbase + (z * g.ny + y) * g.nx + x;
-                const int t1 = cell_start[c + 1];
-                for (int t = cell_start[c]; t < t1; ++t) {
-                    const int b = (int)(uint32_t)sorted[t];
+                const int t1 = G.cell_start[c + 1];
+                for (int t = G.cell_start[c]; t < t1; ++t) {
+                    const int b = (int)(uint32_t)G.sorted[t];
                     if (own && b <= a) continue;
-                    if (sa && P[b].w == 0.0f) continue;
-                    const float4 lb = lo[b], hb = hi[b];
+                    if (sa && G.P[b].w == 0.0f) continue;
+                    const float4 lb = G.lo[b], hb = G.hi[b];
                     if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z &&
                           la.z <= hb.z))
                         continue;
-                    if (WRITE) {
-                        const int o = base + k;
-                        if (o < capacity) {
-                            pairs[2 * (size_t)o] = min(a, b);
-                            pairs[2 * (size_t)o + 1] = max(a, b);
-                        }
-                    }
-                    ++k;
+                    emit(k++, min(a, b), max(a, b));
                 }
             }
         }
     }
     const int lend = sa ? g.lfirst : g.lfirst + g.lcount;  // large bodies are static
     for (int t = g.lfirst; t < lend; ++t) {
-        const int b = llist[t];
-        const float4 lb = lo[b], hb = hi[b];
+        const int b = G.llist[t];
+        const float4 lb = G.lo[b], hb = G.hi[b];
         if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z && la.z <= hb.z))
             continue;
-        if (WRITE) {
-            const int o = base + k;
-            if (o < capacity) {
-                pairs[2 * (size_t)o] = min(a, b);
-                pairs[2 * (size_t)o + 1] = max(a, b);
-            }
-        }
-        ++k;
+        emit(k++, min(a, b), max(a, b));
     }
-    if (!WRITE) cnt[a] = k;
+    return k;
+}
+
+// One thread per body: its pair count (0 for a large body, which pairs only
+// through the gridded bodies) and its first HB_GRID_SLOTS pairs in its slots.
+__global__ void k_hb_grid_pairs(HbGridArgs G, int n, int* __restrict__ cnt, int2* __restrict__ slots) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    if (G.large[a]) {
+        cnt[a] = 0;
+        return;
+    }
+    int2* mine = slots + (size_t)a * HB_GRID_SLOTS;
+    cnt[a] = hb_grid_walk(G, a, [&](int k, int u, int v) {
+        if (k < HB_GRID_SLOTS) mine[k] = make_int2(u, v);
+    });
+}
+
+// One thread per body: its pairs into the list from offs[a] (pairs past
+// capacity are dropped: the total exceeds the capacity and the step reports
+// it), copied from its slots, or walked again when they did not fit.
+__global__ void k_hb_grid_place(HbGridArgs G, int n, const int* __restrict__ cnt, const int* __restrict__ offs,
+                                const int2* __restrict__ slots, int* __restrict__ pairs, int capacity) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const int m = cnt[a], base = offs[a];
+    if (m == 0) return;
+    auto put = [&](int k, int u, int v) {
+        const int o = base + k;
+        if (o < capacity) {
+            pairs[2 * (size_t)o] = u;
+            pairs[2 * (size_t)o + 1] = v;
+        }
+    };
+    if (m <= HB_GRID_SLOTS) {
+        const int2* mine = slots + (size_t)a * HB_GRID_SLOTS;
+        for (int k = 0; k < m; ++k) {
+            const int2 q = mine[k];
+            put(k, q.x, q.y);
+        }
+    } else {
+        (void)hb_grid_walk(G, a, put);
+    }
 }
 
 struct HbBodies {
@@ -393,6 +421,7 @@ struct sd_hull_ctx {
     long long ncells = 0;
     uint64_t *gkeys = nullptr, *gkeys2 = nullptr;
     int *cell_start = nullptr, *gcnt = nullptr, *goff = nullptr;
+    int2* gslots = nullptr;  // HB_GRID_SLOTS pairs per body
     void* sort_tmp = nullptr;
     int *total = nullptr, *sap_error = nullptr;  // total = goff + n; sap_error: a zero word
     float *sep = nullptr, *normal = nullptr;
@@ -459,7 +488,7 @@ void hb_free(sd_hull_ctx* c) {
                      (void**)&c->vstart,     (void**)&c->nv,       (void**)&c->scene_of,   (void**)&c->scene_first,
                      (void**)&c->pairs,      (void**)&c->grid,     (void**)&c->gkeys,      (void**)&c->gkeys2,
                      (void**)&c->large,      (void**)&c->llist,
-                     (void**)&c->cell_start, (void**)&c->gcnt,     (void**)&c->goff,       (void**)&c->sort_tmp,
+                     (void**)&c->cell_start, (void**)&c->gcnt,     (void**)&c->gslots,     (void**)&c->goff,       (void**)&c->sort_tmp,
                      (void**)&c->sap_error,  (void**)&c->sep,      (void**)&c->normal,     (void**)&c->flags,
                      (void**)&c->iters,      (void**)&c->gjk_scratch, (void**)&c->pf,      (void**)&c->pp,
                      (void**)&c->F,          (void**)&c->T,        (void**)&c->cnt,        (void**)&c->fillc,
@@ -511,17 +540,15 @@ sd_status hull_enqueue_step(sd_hull_ctx* c, hipStream_t s) {
         st = launch_step_starts_u64(K, n, (int)c->ncells, c->cell_start, nullptr, false, nullptr, nullptr, nullptr,
                                     s);
     if (st != SD_OK) return st;
-    hipLaunchKernelGGL((k_hb_grid_pairs<false>), dim3(div_up(n, 256)), dim3(256), 0, s, (const float4*)c->P,
-                       (const float4*)c->lo, (const float4*)c->hi, (const int*)c->scene_of, (const HbGrid*)c->grid,
-                       (const int*)c->large, (const int*)c->llist, (const uint64_t*)K, (const int*)c->cell_start, n,
-                       c->gcnt, (const int*)nullptr, c->pairs, cap);
+    const HbGridArgs G{(const float4*)c->P,      (const float4*)c->lo,      (const float4*)c->hi,
+                       (const int*)c->scene_of,  (const HbGrid*)c->grid,    (const int*)c->large,
+                       (const int*)c->llist,     (const uint64_t*)K,        (const int*)c->cell_start};
+    hipLaunchKernelGGL(k_hb_grid_pairs, dim3(div_up(n, 256)), dim3(256), 0, s, G, n, c->gcnt, c->gslots);
     SD_LAUNCH_CHECK();
     st = exclusive_scan_i32(c->gcnt, c->goff, n, c->goff + n, c->scan_tmp, s);
     if (st != SD_OK) return st;
-    hipLaunchKernelGGL((k_hb_grid_pairs<true>), dim3(div_up(n, 256)), dim3(256), 0, s, (const float4*)c->P,
-                       (const float4*)c->lo, (const float4*)c->hi, (const int*)c->scene_of, (const HbGrid*)c->grid,
-                       (const int*)c->large, (const int*)c->llist, (const uint64_t*)K, (const int*)c->cell_start, n,
-                       c->gcnt, (const int*)c->goff, c->pairs, cap);
+    hipLaunchKernelGGL(k_hb_grid_place, dim3(div_up(n, 256)), dim3(256), 0, s, G, n, (const int*)c->gcnt,
+                       (const int*)c->goff, (const int2*)c->gslots, c->pairs, cap);
     SD_LAUNCH_CHECK();
     if (c->model) {
         st = enqueue_warm_dirs(c->pairs, cap, c->total, c->spos, c->Q, c->desc, c->X, c->model, c->wdir, s);
@@ -781,6 +808,7 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     ALLOC(c->gkeys2, n);
     ALLOC(c->cell_start, ncells + 1);
     ALLOC(c->gcnt, n);
+    ALLOC(c->gslots, (size_t)n * HB_GRID_SLOTS);
     ALLOC(c->goff, n + 1);
     ALLOC(*(char**)&c->sort_tmp, radix_temp_bytes(n));
     ALLOC(c->sap_error, 1);

@@ -94,6 +94,34 @@ def test_hull_contacts_equal_oracle(oracle):
     sc.close()
 
 
+def test_hull_bodies_crowded_cell(oracle):
+    """Forty sphere bodies piled into one grid cell beside tumbling hulls: the
+    first of them has 39 candidate pairs in its own cell, more than a body's
+    slots hold, so the broadphase walks the grid again for it; the others'
+    pairs come from their slots. Candidate order, contacts and the state after
+    two steps equal the oracle's."""
+    say("hull bodies crowded cell")
+    base = scene(0x4B05, 120, 40, 0.12)
+    n_h = 120
+    rng = np.random.default_rng(5)
+    base["pos4"][n_h:n_h + 40, :3] = (np.float32(0.06) + rng.uniform(0.0, 0.002, (40, 3))).astype(np.float32)
+    c = law()
+    st = oracle.HullBodyState(base)
+    op, of, oq = st.contacts(c, nthreads=8)
+    sc = HB.HullBodyScene(c)
+    sc.upload(base)
+    sc.step(1)
+    gp, gf, gq = sc.contacts()
+    assert len(op) >= 40 * 39 // 2  # every pair of the pile touches
+    assert np.array_equal(gp, op) and np.array_equal(gf, of) and np.array_equal(gq, oq)
+    st.step(c, nsteps=1, nthreads=8)
+    compare(sc, st, "step 1")
+    sc.step(1)
+    st.step(c, nsteps=1, nthreads=8)
+    compare(sc, st, "step 2")
+    sc.close()
+
+
 def test_hull_capacity_overflow_reported():
     say("hull capacity")
     bodies = scene(0x4B04, 200, 0, 0.05)  # crowded: many candidate pairs per body
