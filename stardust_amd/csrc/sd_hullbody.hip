@@ -165,8 +165,9 @@ __global__ void k_hb_cell_keys(const float4* __restrict__ P, const int* __restri
     keys[b] = ((uint64_t)cell << 32) | (uint64_t)(uint32_t)b;
 }
 
-// Pairs a walk of the grid keeps in a body's own slots (int2 each); a body
-// with more walks the grid again and writes them in place.
+// Pairs a walk of the grid keeps in a body's own slots (int2 each, slot k of
+// body a at k n + a, so that neighbouring threads touch neighbouring slots); a
+// body with more walks the grid again and writes them in place.
 constexpr int HB_GRID_SLOTS = 16;
 
 struct HbGridArgs {
@@ -242,9 +243,8 @@ __global__ void k_hb_grid_pairs(HbGridArgs G, int n, int* __restrict__ cnt, int2
         cnt[a] = 0;
         return;
     }
-    int2* mine = slots + (size_t)a * HB_GRID_SLOTS;
     cnt[a] = hb_grid_walk(G, a, [&](int k, int u, int v) {
-        if (k < HB_GRID_SLOTS) mine[k] = make_int2(u, v);
+        if (k < HB_GRID_SLOTS) slots[(size_t)k * n + a] = make_int2(u, v);
     });
 }
 
@@ -265,9 +265,8 @@ __global__ void k_hb_grid_place(HbGridArgs G, int n, const int* __restrict__ cnt
         }
     };
     if (m <= HB_GRID_SLOTS) {
-        const int2* mine = slots + (size_t)a * HB_GRID_SLOTS;
         for (int k = 0; k < m; ++k) {
-            const int2 q = mine[k];
+            const int2 q = slots[(size_t)k * n + a];
             put(k, q.x, q.y);
         }
     } else {
