@@ -426,8 +426,8 @@ sd_status sd_lbvh_query(const uint64_t* sorted_keys, int n, const int* left, con
  * src/engine/entities/entity.hpp:64-93) stepped on the GPU in independent
  * scenes. Each step: a uniform grid per scene over the body boxes (fixed at
  * upload: the cell twice the largest moving body's bounding radius; larger
- * static bodies such as wall triangles kept out of it and tested against every
- * body of their scene), GJK/EPA of every candidate pair (the sd_gjk results: TRIANGLE
+ * static bodies such as wall or mesh triangles in a second grid of their own,
+ * bucketed at upload), GJK/EPA of every candidate pair (the sd_gjk results: TRIANGLE
  * and SPHERE shapes of sandbox/primitives.hpp:11-14 included; cold, or
  * warm-started by a learned network, sd_hull_set_warm), and
  * for each touching pair (sep < 0) a contact between the two contact

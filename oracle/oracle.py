@@ -57,7 +57,7 @@ _SIGS = {
     "or_compat_angular_velocity": ([_I, _P, _P, _P, _P], None),
     "or_body_reduce": ([_I, _P, _P, _P, _P, _P, _P, _P, _P], None),
     "or_integrate_bodies": ([_I, _P, _P, _P, _P, _P, _P, _P, C.POINTER(OrContact)], None),
-    "or_hull_grid": ([_I, _P, _P, _P, _P, _P, _P, _I, _P, _P], C.c_long),
+    "or_hull_grid": ([_I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P], C.c_long),
     "or_hull_grid_pairs": ([_I, _P, _P, _P, _P, _I, _P, _P, _P, C.c_long], C.c_long),
     "or_hull_step": ([_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, C.POINTER(OrContact), _I, _I,
                       _P, _P, _P, C.c_long], C.c_long),
@@ -154,20 +154,20 @@ class HullBodyState:
         sf = [0, n] if scene_first is None else scene_first
         self.scene_first = np.ascontiguousarray(sf, np.int32)
         # the broadphase grids, fixed from the initial bodies as at upload
-        self.grid8 = np.zeros((len(self.scene_first) - 1, 8), np.float32)
+        self.grid16 = np.zeros((len(self.scene_first) - 1, 16), np.float32)
         self.large = np.zeros(n, np.int32)
         load().or_hull_grid(n, p(a[0]), p(self.radius), p(self.vstart), p(self.nv), p(self.verts4),
-                            p(self.scene_first), len(self.scene_first) - 1, p(self.grid8), p(self.large))
+                            p(self.scene_first), len(self.scene_first) - 1, p(self.grid16), p(self.large), None)
 
     def grid_pairs(self, lo4, hi4):
         """The candidate pairs of the broadphase grids for the current positions
         and the given boxes."""
         lo4, hi4 = np.ascontiguousarray(lo4, np.float32), np.ascontiguousarray(hi4, np.float32)
         m = load().or_hull_grid_pairs(self.n, p(self.P), p(lo4), p(hi4), p(self.scene_first),
-                                      len(self.scene_first) - 1, p(self.grid8), p(self.large), None, 0)
+                                      len(self.scene_first) - 1, p(self.grid16), p(self.large), None, 0)
         out = np.zeros((max(m, 1), 2), np.int32)
         load().or_hull_grid_pairs(self.n, p(self.P), p(lo4), p(hi4), p(self.scene_first), len(self.scene_first) - 1,
-                                  p(self.grid8), p(self.large), p(out), m)
+                                  p(self.grid16), p(self.large), p(out), m)
         return out[:m]
 
     def step(self, c, nsteps=1, max_iter=64, nthreads=1):
@@ -178,7 +178,7 @@ class HullBodyState:
         for _ in range(nsteps):
             m = load().or_hull_step(self.n, p(self.P), p(self.V), p(self.Q), p(self.L), p(self.I0), p(self.I1),
                                     p(self.radius), p(self.vstart), p(self.nv), p(self.verts4), len(self.verts4),
-                                    p(self.scene_first), len(self.scene_first) - 1, p(self.grid8), p(self.large), C.byref(oc),
+                                    p(self.scene_first), len(self.scene_first) - 1, p(self.grid16), p(self.large), C.byref(oc),
                                     max_iter, nthreads, None, None, None, 0)
             out = m
         return out
@@ -190,7 +190,7 @@ class HullBodyState:
         P, V, Q, L = (x.copy() for x in (self.P, self.V, self.Q, self.L))
         m = load().or_hull_step(self.n, p(P), p(V), p(Q), p(L), p(self.I0), p(self.I1), p(self.radius),
                                 p(self.vstart), p(self.nv), p(self.verts4), len(self.verts4), p(self.scene_first),
-                                len(self.scene_first) - 1, p(self.grid8), p(self.large), C.byref(oc), max_iter, nthreads, None, None,
+                                len(self.scene_first) - 1, p(self.grid16), p(self.large), C.byref(oc), max_iter, nthreads, None, None,
                                 None, 0)
         pairs = np.zeros((max(m, 1), 2), np.int32)
         f = np.zeros((max(m, 1), 4), np.float32)
@@ -198,7 +198,7 @@ class HullBodyState:
         P, V, Q, L = (x.copy() for x in (self.P, self.V, self.Q, self.L))
         load().or_hull_step(self.n, p(P), p(V), p(Q), p(L), p(self.I0), p(self.I1), p(self.radius), p(self.vstart),
                             p(self.nv), p(self.verts4), len(self.verts4), p(self.scene_first),
-                            len(self.scene_first) - 1, p(self.grid8), p(self.large), C.byref(oc), max_iter, nthreads, p(pairs), p(f),
+                            len(self.scene_first) - 1, p(self.grid16), p(self.large), C.byref(oc), max_iter, nthreads, p(pairs), p(f),
                             p(q), m)
         return pairs[:m], f[:m], q[:m]
 

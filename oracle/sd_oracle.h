@@ -126,20 +126,20 @@ void or_body_reduce(int nb, const float* bpos4, const int* first, const int* cou
 void or_integrate_bodies(int nb, float* P, float* V, float* Q, float* L, const float* inv6, const float* f4,
                          const float* t4, const or_contact* c);
 /* Per-scene broadphase grids of the rigid-body context from its initial
- * bodies (8 values per scene) and the bodies kept out of them (large[b]);
- * returns the total cell count. */
+ * bodies (16 values per scene: the grid of the bodies, then that of the
+ * large static bodies, large[b] = 1); returns the total cell count of the
+ * first grids, *large_cells (nullable) that of the second. */
 long or_hull_grid(int n, const float* pos4, const float* radius, const int* vstart, const int* nv,
-                  const float* verts4, const int* scene_first, int nscenes, float* grid8, int* large);
-/* The candidate pairs the body step takes from the grids (positions P with
- * 1/m in w, boxes lo4/hi4); returns the count, the first capacity written. */
+                  const float* verts4, const int* scene_first, int nscenes, float* grid16, int* large,
+                  long* large_cells);
 long or_hull_grid_pairs(int n, const float* P, const float* lo4, const float* hi4, const int* scene_first,
-                        int nscenes, const float* grid8, const int* large, int* pairs, long capacity);
+                        int nscenes, const float* grid16, const int* large, int* pairs, long capacity);
 /* One step of the rigid convex-body context (sd_hull_step) over the grids of
  * or_hull_grid: returns the number of touching pairs; the first out_capacity
  * are written (pairs, force on b, contact point; all nullable). */
 long or_hull_step(int n, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const float* radius,
                   const int* vstart, const int* nv, const float* verts4, int n_verts, const int* scene_first,
-                  int nscenes, const float* grid8, const int* large, const or_contact* c, int max_iter, int nthreads,
+                  int nscenes, const float* grid16, const int* large, const or_contact* c, int max_iter, int nthreads,
                   int* pairs_out, float* force_out, float* point_out, long out_capacity);
 long or_clump_step(int nb, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const int* first,
                    const int* count, const float* local4, int ns, const or_contact* c);

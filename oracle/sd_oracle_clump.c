@@ -513,21 +513,49 @@ static float hb_bound(const float* verts4, int nv, float radius) {
 #define HB_GRID_AXIS_MAX 512
 #define HB_GRID_CELLS_MAX 4194304L
 
-/* Per-scene uniform grid of the rigid-body broadphase, fixed from the initial
- * bodies (sd_hull_upload). rd = the largest bounding radius of the scene's
- * moving bodies (mass > 0; of all its bodies when none moves); a body whose
- * bounding radius exceeds rd (a large static body such as a wall triangle)
- * stays out of the grid and is tested against the scene's bodies directly
- * (large[b] = 1). Cell = 2 rd (inflated by 2^-10, so that two gridded bodies
- * whose boxes overlap have centres at most one cell apart per axis), over the
- * box of the gridded bodies' bounding spheres; the cell grows by 1.25 until no
- * axis has more than 512 cells and the scene no more than 2^22. grid8[8 s ..]
- * = (lo x, y, z, 1 / cell, nx, ny, nz, first cell id) with the last four as
- * int bits. Returns the cell count of all scenes. */
+/* Fits a grid of cell `cell` (inflated by 2^-10) over [lo, hi]: the cell
+ * grows by 1.25 until no axis has more than 512 cells and the grid no more
+ * than 2^22; writes 1 / cell and the cells per axis, returns the cell count. */
+static long hb_grid_fit(float cell, const float* lo, const float* hi, float* inv_out, int* d) {
+    cell = 2.0f * cell * 1.0009765625f;
+    if (!(cell > 0.0f)) cell = 1e-3f;
+    for (;;) {
+        const float inv = 1.0f / cell;
+        long cells = 1;
+        int ok = 1;
+        for (int a = 0; a < 3; ++a) {
+            const float e = ceilf((hi[a] - lo[a]) * inv);
+            d[a] = e >= 1.0f ? (e <= (float)HB_GRID_AXIS_MAX ? (int)e : HB_GRID_AXIS_MAX + 1) : 1;
+            ok = ok && d[a] <= HB_GRID_AXIS_MAX;
+            cells *= d[a];
+        }
+        if (ok && cells <= HB_GRID_CELLS_MAX) {
+            *inv_out = inv;
+            return cells;
+        }
+        cell = cell * 1.25f;
+    }
+}
+
+/* Per-scene grids of the rigid-body broadphase, fixed from the initial bodies
+ * (sd_hull_upload). rd = the largest bounding radius of the scene's moving
+ * bodies (mass > 0; of all its bodies when none moves). A body whose bounding
+ * radius exceeds rd (large[b] = 1: a static body such as a wall or mesh
+ * triangle) goes to the scene's second grid, of the large bodies, and every
+ * other body to the first. Each grid: cell = twice the largest bounding radius
+ * of its bodies (inflated by 2^-10, so that a gridded body and a body of
+ * either grid whose boxes overlap have centres at most one cell of that grid
+ * apart per axis), over the box of its bodies' bounding spheres, the cell
+ * grown by 1.25 until no axis has more than 512 cells and the grid no more
+ * than 2^22. grid16[16 s ..] = (lo x, y, z, 1 / cell, nx, ny, nz, first cell
+ * id) of the first grid, then the same of the second, the cell counts and
+ * first ids as int bits. Returns the cell count of all scenes' first grids;
+ * *large_cells (if given) receives that of the second grids. */
 long or_hull_grid(int n, const float* pos4, const float* radius, const int* vstart, const int* nv,
-                  const float* verts4, const int* scene_first, int nscenes, float* grid8, int* large) {
+                  const float* verts4, const int* scene_first, int nscenes, float* grid16, int* large,
+                  long* large_cells) {
     (void)n;
-    long cbase = 0;
+    long cbase = 0, lbase = 0;
     for (int s = 0; s < nscenes; ++s) {
         float rd = 0.0f, ra = 0.0f;
         int moving = 0;
@@ -540,48 +568,39 @@ long or_hull_grid(int n, const float* pos4, const float* radius, const int* vsta
             }
         }
         if (!moving) rd = ra;
-        float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
-        int first = 1;
+        float lo[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}, hi[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        float rl = 0.0f;
+        int first[2] = {1, 1};
         for (int b = scene_first[s]; b < scene_first[s + 1]; ++b) {
             const float r = hb_bound(verts4 + 4 * (long)vstart[b], nv[b], radius[b]);
-            large[b] = r > rd;
-            if (large[b]) continue;
+            const int k = large[b] = r > rd;
+            if (k) rl = fmaxf(rl, r);
             for (int a = 0; a < 3; ++a) {
                 const float x = pos4[4 * (long)b + a];
-                lo[a] = first ? x - r : fminf(lo[a], x - r);
-                hi[a] = first ? x + r : fmaxf(hi[a], x + r);
+                lo[k][a] = first[k] ? x - r : fminf(lo[k][a], x - r);
+                hi[k][a] = first[k] ? x + r : fmaxf(hi[k][a], x + r);
             }
-            first = 0;
+            first[k] = 0;
         }
-        float cell = 2.0f * rd * 1.0009765625f;
-        if (!(cell > 0.0f)) cell = 1e-3f;
-        float inv;
-        int d[3];
-        for (;;) {
-            inv = 1.0f / cell;
-            long cells = 1;
-            int ok = 1;
-            for (int a = 0; a < 3; ++a) {
-                const float e = ceilf((hi[a] - lo[a]) * inv);
-                d[a] = e >= 1.0f ? (e <= (float)HB_GRID_AXIS_MAX ? (int)e : HB_GRID_AXIS_MAX + 1) : 1;
-                ok = ok && d[a] <= HB_GRID_AXIS_MAX;
-                cells *= d[a];
-            }
-            if (ok && cells <= HB_GRID_CELLS_MAX) break;
-            cell = cell * 1.25f;
+        for (int k = 0; k < 2; ++k) {
+            float inv;
+            int d[3];
+            const long cells = hb_grid_fit(k ? rl : rd, lo[k], hi[k], &inv, d);
+            float* g = grid16 + 16 * (long)s + 8 * k;
+            g[0] = lo[k][0];
+            g[1] = lo[k][1];
+            g[2] = lo[k][2];
+            g[3] = inv;
+            int* gi = (int*)(g + 4);
+            gi[0] = d[0];
+            gi[1] = d[1];
+            gi[2] = d[2];
+            gi[3] = (int)(k ? lbase : cbase);
+            if (k) lbase += cells;
+            else cbase += cells;
         }
-        float* g = grid8 + 8 * (long)s;
-        g[0] = lo[0];
-        g[1] = lo[1];
-        g[2] = lo[2];
-        g[3] = inv;
-        int* gi = (int*)(g + 4);
-        gi[0] = d[0];
-        gi[1] = d[1];
-        gi[2] = d[2];
-        gi[3] = (int)cbase;
-        cbase += (long)d[0] * d[1] * d[2];
     }
+    if (large_cells) *large_cells = lbase;
     return cbase;
 }
 
@@ -610,21 +629,50 @@ static int* hb_push(int* pairs, long* cap, long np, int a, int b) {
     return pairs;
 }
 
+/* The bodies of scene [f, f + k) with large[b] == kind bucketed by their
+ * centre's cell in grid g (8 values): start[0 .. ncell] and order (index
+ * order within a cell); cellof[b] = the body's local cell. */
+static void hb_bucket(const float* P, const int* large, int kind, int f, int k, const float* g, int* cellof,
+                      int** start_out, int** order_out) {
+    const int* gi = (const int*)(g + 4);
+    const int nx = gi[0], ny = gi[1], nz = gi[2];
+    const long ncell = (long)nx * ny * nz;
+    int* start = (int*)calloc((size_t)ncell + 1, sizeof(int));
+    int* order = (int*)malloc((size_t)(k > 0 ? k : 1) * sizeof(int));
+    for (int b = f; b < f + k; ++b) {
+        if (large[b] != kind) continue;
+        const int cx = hb_cell_axis(P[4 * (long)b], g[0], g[3], nx);
+        const int cy = hb_cell_axis(P[4 * (long)b + 1], g[1], g[3], ny);
+        const int cz = hb_cell_axis(P[4 * (long)b + 2], g[2], g[3], nz);
+        cellof[b] = (cz * ny + cy) * nx + cx;
+        ++start[cellof[b] + 1];
+    }
+    for (long c = 0; c < ncell; ++c) start[c + 1] += start[c];
+    int* fill = (int*)calloc((size_t)ncell + 1, sizeof(int));
+    for (int b = f; b < f + k; ++b)
+        if (large[b] == kind) order[start[cellof[b]] + fill[cellof[b]]++] = b; /* stable: index order */
+    free(fill);
+    *start_out = start;
+    *order_out = order;
+}
+
 /* Candidate pairs of the rigid-body step, body a by body a (index order); for
  * each gridded body a, the bodies of its scene whose boxes overlap a's
  * (closed intervals), except pairs of two static bodies (1/m = 0, P.w), each
  * pair written as (min, max): first the gridded bodies of a's own cell after
  * it (b > a) and of the 13 cells that follow its own in (dz, dy, dx) order
  * (dz = 1; dz = 0, dy = 1; dz = dy = 0, dx = 1), in that order and within a
- * cell in index order, then the scene's large bodies in index order. Two
- * gridded bodies whose boxes overlap lie in the same cell or in neighbouring
- * ones (clamping to the border cells keeps that), so the pair is found once,
- * from the body whose cell comes first (the lower index in a shared cell). A large body is static (its bounding
- * radius exceeds every moving body's), so two large bodies never pair and a
- * large body adds no pairs of its own. Returns the pair count (pairs grown by
- * realloc). */
+ * cell in index order, then the large bodies in the 27 cells of the scene's
+ * second grid around a's centre (dz, dy, dx ascending; within a cell in index
+ * order). Two gridded bodies whose boxes overlap lie in the same cell or in
+ * neighbouring ones (clamping to the border cells keeps that), so the pair is
+ * found once, from the body whose cell comes first (the lower index in a
+ * shared cell); a large body overlapping a gridded one lies within one cell
+ * of the second grid. A large body is static (its bounding radius exceeds
+ * every moving body's), so two large bodies never pair and a large body adds
+ * no pairs of its own. Returns the pair count (pairs grown by realloc). */
 static long hb_grid_pairs(int n, const float* P, const float* lo, const float* hi, const int* scene_first,
-                          int nscenes, const float* grid8, const int* large, int** pairs_io, long* cap_io) {
+                          int nscenes, const float* grid16, const int* large, int** pairs_io, long* cap_io) {
     long np = 0;
     int* pairs = *pairs_io;
     long cap = *cap_io;
@@ -632,25 +680,15 @@ static long hb_grid_pairs(int n, const float* P, const float* lo, const float* h
     for (int s = 0; s < nscenes; ++s) {
         const int f = scene_first[s], k = scene_first[s + 1] - f;
         if (k < 2) continue;
-        const float* g = grid8 + 8 * (long)s;
+        const float* g = grid16 + 16 * (long)s;
+        const float* gl = g + 8;
         const int* gi = (const int*)(g + 4);
+        const int* gli = (const int*)(gl + 4);
         const int nx = gi[0], ny = gi[1], nz = gi[2];
-        const long ncell = (long)nx * ny * nz;
-        int* start = (int*)calloc((size_t)ncell + 1, sizeof(int));
-        int* order = (int*)malloc((size_t)k * sizeof(int));
-        for (int b = f; b < f + k; ++b) {
-            if (large[b]) continue;
-            const int cx = hb_cell_axis(P[4 * (long)b], g[0], g[3], nx);
-            const int cy = hb_cell_axis(P[4 * (long)b + 1], g[1], g[3], ny);
-            const int cz = hb_cell_axis(P[4 * (long)b + 2], g[2], g[3], nz);
-            cellof[b] = (cz * ny + cy) * nx + cx;
-            ++start[cellof[b] + 1];
-        }
-        for (long c = 0; c < ncell; ++c) start[c + 1] += start[c];
-        int* fill = (int*)calloc((size_t)ncell, sizeof(int));
-        for (int b = f; b < f + k; ++b)
-            if (!large[b]) order[start[cellof[b]] + fill[cellof[b]]++] = b; /* stable: index order */
-        free(fill);
+        const int lx = gli[0], ly = gli[1], lz = gli[2];
+        int *start, *order, *lstart, *lorder;
+        hb_bucket(P, large, 0, f, k, g, cellof, &start, &order);
+        hb_bucket(P, large, 1, f, k, gl, cellof, &lstart, &lorder);
         for (int a = f; a < f + k; ++a) {
             if (large[a]) continue;
             const int ca = cellof[a];
@@ -669,12 +707,27 @@ static long hb_grid_pairs(int n, const float* P, const float* lo, const float* h
                                 pairs = hb_push(pairs, &cap, np++, a < b ? a : b, a < b ? b : a);
                         }
                     }
-            for (int b = f; b < f + k; ++b)
-                if (large[b] && hb_pair_ok(P, lo, hi, a, b))
-                    pairs = hb_push(pairs, &cap, np++, a < b ? a : b, a < b ? b : a);
+            if (P[4 * (long)a + 3] == 0.0f) continue; /* large bodies are static */
+            const int qx = hb_cell_axis(P[4 * (long)a], gl[0], gl[3], lx);
+            const int qy = hb_cell_axis(P[4 * (long)a + 1], gl[1], gl[3], ly);
+            const int qz = hb_cell_axis(P[4 * (long)a + 2], gl[2], gl[3], lz);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int x = qx + dx, y = qy + dy, z = qz + dz;
+                        if (x < 0 || x >= lx || y < 0 || y >= ly || z < 0 || z >= lz) continue;
+                        const int c = (z * ly + y) * lx + x;
+                        for (int t = lstart[c]; t < lstart[c + 1]; ++t) {
+                            const int b = lorder[t];
+                            if (hb_pair_ok(P, lo, hi, a, b))
+                                pairs = hb_push(pairs, &cap, np++, a < b ? a : b, a < b ? b : a);
+                        }
+                    }
         }
         free(start);
         free(order);
+        free(lstart);
+        free(lorder);
     }
     free(cellof);
     *pairs_io = pairs;
@@ -683,10 +736,10 @@ static long hb_grid_pairs(int n, const float* P, const float* lo, const float* h
 }
 
 long or_hull_grid_pairs(int n, const float* P, const float* lo4, const float* hi4, const int* scene_first,
-                        int nscenes, const float* grid8, const int* large, int* pairs, long capacity) {
+                        int nscenes, const float* grid16, const int* large, int* pairs, long capacity) {
     long cap = 1024;
     int* all = (int*)malloc(2 * (size_t)cap * sizeof(int));
-    const long np = hb_grid_pairs(n, P, lo4, hi4, scene_first, nscenes, grid8, large, &all, &cap);
+    const long np = hb_grid_pairs(n, P, lo4, hi4, scene_first, nscenes, grid16, large, &all, &cap);
     for (long t = 0; t < np && t < capacity; ++t) {
         pairs[2 * t] = all[2 * t];
         pairs[2 * t + 1] = all[2 * t + 1];
@@ -697,7 +750,7 @@ long or_hull_grid_pairs(int n, const float* P, const float* lo4, const float* hi
 
 long or_hull_step(int n, float* P, float* V, float* Q, float* L, const float* I0, const float* I1, const float* radius,
                   const int* vstart, const int* nv, const float* verts4, int n_verts, const int* scene_first,
-                  int nscenes, const float* grid8, const int* large, const or_contact* c, int max_iter, int nthreads,
+                  int nscenes, const float* grid16, const int* large, const or_contact* c, int max_iter, int nthreads,
                   int* pairs_out, float* force_out, float* point_out, long out_capacity) {
     (void)n_verts;
     float* spos = (float*)calloc(4 * (size_t)(n > 0 ? n : 1), sizeof(float));
@@ -720,7 +773,7 @@ long or_hull_step(int n, float* P, float* V, float* Q, float* L, const float* I0
     /* candidate pairs: the per-scene grids */
     long cap = 1024;
     int* pairs = (int*)malloc(2 * (size_t)cap * sizeof(int));
-    const long np = hb_grid_pairs(n, P, lo, hi, scene_first, nscenes, grid8, large, &pairs, &cap);
+    const long np = hb_grid_pairs(n, P, lo, hi, scene_first, nscenes, grid16, large, &pairs, &cap);
     float* sep = (float*)malloc((size_t)(np + 1) * sizeof(float));
     float* nrm = (float*)malloc(3 * (size_t)(np + 1) * sizeof(float));
     int* flags = (int*)malloc((size_t)(np + 1) * sizeof(int));

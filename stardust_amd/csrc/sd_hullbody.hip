@@ -5,8 +5,8 @@
 //   boxes     k_hull_aabb
 //   pairs     a uniform grid per scene (fixed at upload): bodies sorted by
 //             cell, each body's overlapping partners in the later part of its
-//             own cell and the 13 cells after it; large static bodies (walls)
-//             tested against every body of their scene
+//             own cell and the 13 cells after it, then the large static
+//             bodies (walls, mesh triangles) of a second grid around it
 //   narrow    GJK/EPA of every candidate pair: cold, or warm-started from
 //             the directions of an attached learned model (sd_hull_set_warm),
 //             the rejected warm queries redone cold
@@ -128,18 +128,21 @@ __device__ __forceinline__ V3 contact_point(V3 ca, float wa, V3 cb, float wb) {
     return add(ca, scl(sub(cb, ca), t));
 }
 
-// Broadphase grid of one scene, fixed at upload from the initial bodies:
-// origin, 1 / cell, cells per axis, the scene's first global cell id and its
-// bodies kept out of the grid (llist[lfirst .. lfirst + lcount), index
-// order). The cell is twice the largest bounding radius rd of the scene's
-// moving bodies (inflated by 2^-10), so two gridded bodies whose boxes
-// overlap have centres at most one cell apart on every axis; bodies outside
-// the grid are clamped to its border cells, which only merges cells. A body
-// whose bounding radius exceeds rd (a wall triangle) is static and is tested
-// against every gridded body of its scene instead.
+// Broadphase grids of one scene, fixed at upload from the initial bodies:
+// origin, 1 / cell, cells per axis and first global cell id of the grid of
+// its bodies, then the same of the grid of its large bodies. The first cell
+// is twice the largest bounding radius rd of the scene's moving bodies
+// (inflated by 2^-10), so two gridded bodies whose boxes overlap have centres
+// at most one cell apart on every axis; bodies outside a grid are clamped to
+// its border cells, which only merges cells. A body whose bounding radius
+// exceeds rd (a wall or mesh triangle) is static and goes to the second grid,
+// whose cell is twice the largest such radius: a gridded body and a large
+// body whose boxes overlap lie within one cell of it.
 struct HbGrid {
     float lox, loy, loz, inv;
-    int nx, ny, nz, cbase, lfirst, lcount;
+    int nx, ny, nz, cbase;
+    float llox, lloy, lloz, linv;
+    int lnx, lny, lnz, lcbase;
 };
 
 __device__ __forceinline__ int hb_cell_axis(float x, float lo, float inv, int nd) {
@@ -177,7 +180,8 @@ struct HbGridArgs {
     const int* scene_of;
     const HbGrid* grid;
     const int* large;
-    const int* llist;
+    const int* llist;        // large bodies sorted by cell of the second grid, then index
+    const int* lstart;       // their cell starts
     const uint64_t* sorted;  // keys sorted by cell, then index
     const int* cell_start;
 };
@@ -186,9 +190,11 @@ struct HbGridArgs {
 // axes (closed intervals), except pairs of two static bodies, each as (min,
 // max): the bodies after a in its own cell, then those of the 13 cells that
 // follow its own in (dz, dy, dx) order, within a cell in index order, then the
-// scene's large bodies in index order. Overlapping gridded bodies lie in the
-// same or neighbouring cells, so each such pair is found once, from the body
-// whose cell comes first. Pair k goes to emit(k, lo, hi); returns the count.
+// large bodies of the 27 cells of the second grid around a's centre (dz, dy,
+// dx ascending; index order within a cell). Overlapping gridded bodies lie in
+// the same or neighbouring cells, so each such pair is found once, from the
+// body whose cell comes first. Pair k goes to emit(k, lo, hi); returns the
+// count.
 template <typename Emit>
 __device__ __forceinline__ int hb_grid_walk(const HbGridArgs& G, int a, Emit emit) {
     const HbGrid g = G.grid[G.scene_of[a]];
@@ -223,13 +229,31 @@ __device__ __forceinline__ int hb_grid_walk(const HbGridArgs& G, int a, Emit emi
             }
         }
     }
-    const int lend = sa ? g.lfirst : g.lfirst + g.lcount;  // large bodies are static
-    for (int t = g.lfirst; t < lend; ++t) {
-        const int b = G.llist[t];
-        const float4 lb = G.lo[b], hb = G.hi[b];
-        if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z && la.z <= hb.z))
-            continue;
-        emit(k++, min(a, b), max(a, b));
+    if (sa) return k;  // large bodies are static
+    const int qx = hb_cell_axis(pa.x, g.llox, g.linv, g.lnx);
+    const int qy = hb_cell_axis(pa.y, g.lloy, g.linv, g.lny);
+    const int qz = hb_cell_axis(pa.z, g.lloz, g.linv, g.lnz);
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = qz + dz;
+        if (z < 0 || z >= g.lnz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = qy + dy;
+            if (y < 0 || y >= g.lny) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = qx + dx;
+                if (x < 0 || x >= g.lnx) continue;
+                const int c = g.lcbase + (z * g.lny + y) * g.lnx + x;
+                const int t1 = G.lstart[c + 1];
+                for (int t = G.lstart[c]; t < t1; ++t) {
+                    const int b = G.llist[t];
+                    const float4 lb = G.lo[b], hb = G.hi[b];
+                    if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z &&
+                          la.z <= hb.z))
+                        continue;
+                    emit(k++, min(a, b), max(a, b));
+                }
+            }
+        }
     }
     return k;
 }
@@ -416,7 +440,7 @@ struct sd_hull_ctx {
     // per-scene lists), cell keys (two buffers), cell starts, per-body pair
     // counts and offsets (the total at offs[n]), sort scratch
     HbGrid* grid = nullptr;
-    int *large = nullptr, *llist = nullptr;
+    int *large = nullptr, *llist = nullptr, *lstart = nullptr;
     long long ncells = 0;
     uint64_t *gkeys = nullptr, *gkeys2 = nullptr;
     int *cell_start = nullptr, *gcnt = nullptr, *goff = nullptr;
@@ -486,7 +510,7 @@ void hb_free(sd_hull_ctx* c) {
                      (void**)&c->lo,         (void**)&c->hi,       (void**)&c->verts,      (void**)&c->radius,       (void**)&c->ftol,
                      (void**)&c->vstart,     (void**)&c->nv,       (void**)&c->scene_of,   (void**)&c->scene_first,
                      (void**)&c->pairs,      (void**)&c->grid,     (void**)&c->gkeys,      (void**)&c->gkeys2,
-                     (void**)&c->large,      (void**)&c->llist,
+                     (void**)&c->large,      (void**)&c->llist,    (void**)&c->lstart,
                      (void**)&c->cell_start, (void**)&c->gcnt,     (void**)&c->gslots,     (void**)&c->goff,       (void**)&c->sort_tmp,
                      (void**)&c->sap_error,  (void**)&c->sep,      (void**)&c->normal,     (void**)&c->flags,
                      (void**)&c->iters,      (void**)&c->gjk_scratch, (void**)&c->pf,      (void**)&c->pp,
@@ -539,9 +563,10 @@ sd_status hull_enqueue_step(sd_hull_ctx* c, hipStream_t s) {
         st = launch_step_starts_u64(K, n, (int)c->ncells, c->cell_start, nullptr, false, nullptr, nullptr, nullptr,
                                     s);
     if (st != SD_OK) return st;
-    const HbGridArgs G{(const float4*)c->P,      (const float4*)c->lo,      (const float4*)c->hi,
-                       (const int*)c->scene_of,  (const HbGrid*)c->grid,    (const int*)c->large,
-                       (const int*)c->llist,     (const uint64_t*)K,        (const int*)c->cell_start};
+    const HbGridArgs G{(const float4*)c->P,     (const float4*)c->lo,     (const float4*)c->hi,
+                       (const int*)c->scene_of, (const HbGrid*)c->grid,   (const int*)c->large,
+                       (const int*)c->llist,    (const int*)c->lstart,    (const uint64_t*)K,
+                       (const int*)c->cell_start};
     hipLaunchKernelGGL(k_hb_grid_pairs, dim3(div_up(n, 256)), dim3(256), 0, s, G, n, c->gcnt, c->gslots);
     SD_LAUNCH_CHECK();
     st = exclusive_scan_i32(c->gcnt, c->goff, n, c->goff + n, c->scan_tmp, s);
@@ -621,70 +646,103 @@ float hb_bound(const float* verts4, int nv, float radius) {
 constexpr int HB_GRID_AXIS_MAX = 512;
 constexpr long long HB_GRID_CELLS_MAX = 1ll << 22;
 
-// The grid of every scene from the initial bodies: rd = the largest bounding
+// Fits a grid of cell `cell` (twice it, inflated by 2^-10) over [lo, hi]: the
+// cell grows by 1.25 until no axis has more than 512 cells and the grid no
+// more than 2^22; writes 1 / cell and the cells per axis, returns the cells.
+long long hb_grid_fit(float cell, const float* lo, const float* hi, float& inv_out, int* d) {
+    cell = 2.0f * cell * 1.0009765625f;
+    if (!(cell > 0.0f)) cell = 1e-3f;
+    for (;;) {
+        const float inv = 1.0f / cell;
+        long long cells = 1;
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            const float e = ceilf((hi[a] - lo[a]) * inv);
+            d[a] = e >= 1.0f ? (e <= (float)HB_GRID_AXIS_MAX ? (int)e : HB_GRID_AXIS_MAX + 1) : 1;
+            ok = ok && d[a] <= HB_GRID_AXIS_MAX;
+            cells *= d[a];
+        }
+        if (ok && cells <= HB_GRID_CELLS_MAX) {
+            inv_out = inv;
+            return cells;
+        }
+        cell = cell * 1.25f;
+    }
+}
+
+int hb_cell_axis_host(float x, float lo, float inv, int nd) {
+    return (int)fminf(fmaxf(floorf((x - lo) * inv), 0.0f), (float)(nd - 1));
+}
+
+// The grids of every scene from the initial bodies: rd = the largest bounding
 // radius of its moving bodies (of all its bodies when none moves); a body
-// whose bounding radius exceeds rd stays out of the grid (large[b] = 1,
-// listed in llist in index order); cell = 2 rd (times 1 + 2^-10) over the box
-// of the gridded bodies' bounding spheres, grown by 1.25 until no axis has
-// more than 512 cells and the scene no more than 2^22. Returns the cell count
-// of all scenes.
+// whose bounding radius exceeds rd is large (static) and goes to the second
+// grid. Each grid: cell = twice the largest bounding radius of its bodies
+// (times 1 + 2^-10) over the box of their bounding spheres, grown by 1.25
+// until no axis has more than 512 cells and the grid no more than 2^22. The
+// large bodies never move, so they are bucketed here: llist sorted by (second
+// grid cell, index), lstart its cell starts. Returns the cells of the first
+// grids; nlarge_cells receives those of the second.
 long long hb_grids(const float* pos4, const float* radius, const int* vstart, const int* nv, const float* verts4,
                    const std::vector<int>& sfirst, std::vector<HbGrid>& grids, std::vector<int>& large,
-                   std::vector<int>& llist) {
+                   std::vector<int>& llist, std::vector<int>& lstart, long long& nlarge_cells) {
     const int nscenes = (int)sfirst.size() - 1;
     grids.assign(nscenes, HbGrid{});
     large.assign(sfirst[nscenes], 0);
-    llist.clear();
-    long long cbase = 0;
+    std::vector<float> bound(sfirst[nscenes]);
+    for (int b = 0; b < sfirst[nscenes]; ++b) bound[b] = hb_bound(verts4 + 4 * (size_t)vstart[b], nv[b], radius[b]);
+    long long cbase = 0, lbase = 0;
+    std::vector<std::pair<long long, int>> lkeys;  // (second grid cell, body)
     for (int s = 0; s < nscenes; ++s) {
-        float rd = 0.0f, ra = 0.0f, lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+        float rd = 0.0f, ra = 0.0f, rl = 0.0f;
         bool moving = false;
         for (int b = sfirst[s]; b < sfirst[s + 1]; ++b) {
-            const float r = hb_bound(verts4 + 4 * (size_t)vstart[b], nv[b], radius[b]);
-            ra = fmaxf(ra, r);
+            ra = fmaxf(ra, bound[b]);
             if (pos4[4 * (size_t)b + 3] > 0.0f) {
-                rd = fmaxf(rd, r);
+                rd = fmaxf(rd, bound[b]);
                 moving = true;
             }
         }
         if (!moving) rd = ra;
-        const int lfirst = (int)llist.size();
-        bool first = true;
+        float lo[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}, hi[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        bool first[2] = {true, true};
         for (int b = sfirst[s]; b < sfirst[s + 1]; ++b) {
-            const float r = hb_bound(verts4 + 4 * (size_t)vstart[b], nv[b], radius[b]);
-            if (r > rd) {
-                large[b] = 1;
-                llist.push_back(b);
-                continue;
-            }
+            const float r = bound[b];
+            const int k = large[b] = r > rd ? 1 : 0;
+            if (k) rl = fmaxf(rl, r);
             for (int a = 0; a < 3; ++a) {
                 const float x = pos4[4 * (size_t)b + a];
-                lo[a] = first ? x - r : fminf(lo[a], x - r);
-                hi[a] = first ? x + r : fmaxf(hi[a], x + r);
+                lo[k][a] = first[k] ? x - r : fminf(lo[k][a], x - r);
+                hi[k][a] = first[k] ? x + r : fmaxf(hi[k][a], x + r);
             }
-            first = false;
+            first[k] = false;
         }
-        float cell = 2.0f * rd * 1.0009765625f;
-        if (!(cell > 0.0f)) cell = 1e-3f;
-        float inv;
-        int d[3];
-        for (;;) {
-            inv = 1.0f / cell;
-            long long cells = 1;
-            bool ok = true;
-            for (int a = 0; a < 3; ++a) {
-                const float e = ceilf((hi[a] - lo[a]) * inv);
-                d[a] = e >= 1.0f ? (e <= (float)HB_GRID_AXIS_MAX ? (int)e : HB_GRID_AXIS_MAX + 1) : 1;
-                ok = ok && d[a] <= HB_GRID_AXIS_MAX;
-                cells *= d[a];
-            }
-            if (ok && cells <= HB_GRID_CELLS_MAX) break;
-            cell = cell * 1.25f;
+        HbGrid& g = grids[s];
+        int d[3], e[3];
+        float inv, linv;
+        const long long cells = hb_grid_fit(rd, lo[0], hi[0], inv, d);
+        const long long lcells = hb_grid_fit(rl, lo[1], hi[1], linv, e);
+        g = HbGrid{lo[0][0], lo[0][1], lo[0][2], inv,  d[0], d[1], d[2], (int)cbase,
+                   lo[1][0], lo[1][1], lo[1][2], linv, e[0], e[1], e[2], (int)lbase};
+        for (int b = sfirst[s]; b < sfirst[s + 1]; ++b) {
+            if (!large[b]) continue;
+            const int cx = hb_cell_axis_host(pos4[4 * (size_t)b], g.llox, g.linv, g.lnx);
+            const int cy = hb_cell_axis_host(pos4[4 * (size_t)b + 1], g.lloy, g.linv, g.lny);
+            const int cz = hb_cell_axis_host(pos4[4 * (size_t)b + 2], g.lloz, g.linv, g.lnz);
+            lkeys.emplace_back(lbase + ((long long)cz * g.lny + cy) * g.lnx + cx, b);
         }
-        grids[s] = HbGrid{lo[0], lo[1], lo[2], inv, d[0], d[1], d[2], (int)cbase, lfirst,
-                          (int)llist.size() - lfirst};
-        cbase += (long long)d[0] * d[1] * d[2];
+        cbase += cells;
+        lbase += lcells;
     }
+    std::sort(lkeys.begin(), lkeys.end());
+    llist.resize(lkeys.size());
+    lstart.assign(lbase + 1, 0);
+    for (size_t i = 0; i < lkeys.size(); ++i) {
+        llist[i] = lkeys[i].second;
+        ++lstart[lkeys[i].first + 1];
+    }
+    for (long long c = 0; c < lbase; ++c) lstart[c + 1] += lstart[c];
+    nlarge_cells = lbase;
     return cbase;
 }
 
@@ -772,9 +830,12 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     SD_REQUIRE(capl < (1ll << 30), "sd_hull_upload: pair capacity %lld too large", capl);
     const int cap = (int)capl;
     std::vector<HbGrid> grids;
-    std::vector<int> large, llist;
-    const long long ncells = hb_grids(pos4, radius, vstart, nv, verts4, sfirst, grids, large, llist);
-    SD_REQUIRE(ncells < (1ll << 31) - 1, "sd_hull_upload: %lld broadphase cells", ncells);
+    std::vector<int> large, llist, lstart;
+    long long nlarge_cells = 0;
+    const long long ncells = hb_grids(pos4, radius, vstart, nv, verts4, sfirst, grids, large, llist, lstart,
+                                      nlarge_cells);
+    SD_REQUIRE(ncells < (1ll << 31) - 1 && nlarge_cells < (1ll << 31) - 1,
+               "sd_hull_upload: %lld + %lld broadphase cells", ncells, nlarge_cells);
     c->n = n;
     c->n_verts = n_verts;
     c->nscenes = nscenes;
@@ -803,6 +864,7 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     ALLOC(c->grid, nscenes);
     ALLOC(c->large, n);
     ALLOC(c->llist, llist.size());
+    ALLOC(c->lstart, lstart.size());
     ALLOC(c->gkeys, n);
     ALLOC(c->gkeys2, n);
     ALLOC(c->cell_start, ncells + 1);
@@ -851,6 +913,7 @@ extern "C" sd_status sd_hull_upload(sd_hull_ctx* c, int n, const float* pos4, co
     SD_HIP(hipMemcpyAsync(c->large, large.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
     if (!llist.empty())
         SD_HIP(hipMemcpyAsync(c->llist, llist.data(), llist.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    SD_HIP(hipMemcpyAsync(c->lstart, lstart.data(), lstart.size() * sizeof(int), hipMemcpyHostToDevice, s));
     c->total = c->goff + n;
     hipLaunchKernelGGL(k_hb_feature_tol, dim3(div_up(n, 256)), dim3(256), 0, s, (const float4*)c->verts,
                        (const int*)c->vstart, (const int*)c->nv, n, c->ftol);

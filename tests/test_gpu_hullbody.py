@@ -122,6 +122,41 @@ def test_hull_bodies_crowded_cell(oracle):
     sc.close()
 
 
+def test_hull_bodies_on_tessellated_floor(oracle):
+    """Hulls settling on a floor of 18 triangles (3 x 3 squares of 40 mm),
+    each larger than every hull, so the floor goes to the scene's second grid
+    and spans several of its cells: contacts and state after several steps
+    equal the oracle's."""
+    say("hull bodies on a tessellated floor")
+    h = H.generate_hulls(0x4B06, 200, lo=(0.0, 0.0, 0.0), hi=(0.12, 0.12, 0.05), v_sigma=0.3)
+    hb = HB.make_bodies(h, 6)
+    hb["pos4"][:, 2] -= np.float32(0.005)  # the lowest hulls reach into the floor
+    q = np.linspace(0.0, 0.12, 4, dtype=np.float32)
+    tris = []
+    for i in range(3):
+        for j in range(3):
+            a, b, c, d = ((q[i], q[j], 0.0), (q[i + 1], q[j], 0.0), (q[i + 1], q[j + 1], 0.0), (q[i], q[j + 1], 0.0))
+            tris += [(a, b, c), (a, c, d)]
+    bodies, _ = HB.concat([hb, HB.triangle_bodies(np.array(tris, np.float32))])
+    c = law()
+    st = oracle.HullBodyState(bodies)
+    assert st.large[200:].all() and not st.large[:200].any()
+    assert st.grid16[0].view(np.int32)[12:14].min() >= 2
+    op, of, oq = st.contacts(c, nthreads=8)
+    sc = HB.HullBodyScene(c)
+    sc.upload(bodies)
+    sc.step(1)
+    gp, gf, gq = sc.contacts()
+    assert (op[:, 1] >= 200).sum() > 10  # hulls touching floor triangles
+    assert np.array_equal(gp, op) and np.array_equal(gf, of) and np.array_equal(gq, oq)
+    st.step(c, nsteps=1, nthreads=8)
+    compare(sc, st, "step 1")
+    sc.step(6)
+    st.step(c, nsteps=6, nthreads=8)
+    compare(sc, st, "step 7")
+    sc.close()
+
+
 def test_hull_capacity_overflow_reported():
     say("hull capacity")
     bodies = scene(0x4B04, 200, 0, 0.05)  # crowded: many candidate pairs per body

@@ -151,18 +151,29 @@ def test_grid_broadphase_equals_brute_force(oracle):
     the pairs are exactly the pairs of overlapping boxes (closed intervals)
     within a scene, without pairs of two static bodies, each once as (a, b)
     with a < b (brute force). The wall triangles, larger than any moving body,
-    stay out of the grids; one scene lists them after its hulls, one before."""
+    go to each scene's second grid; one scene lists them after its hulls, one
+    before, and a fourth scene has a floor of 18 triangles spanning several
+    cells of that grid."""
     h1 = H.generate_hulls(0x4E01, 300, lo=(0.0, 0.0, 0.0), hi=(0.12, 0.12, 0.12))
     s1, _ = HB.concat([HB.make_bodies(h1, 1), HB.box_walls((0.0, 0.0, 0.0), (0.12, 0.12, 0.12))])
     h2 = H.generate_hulls(0x4E02, 200, lo=(0.0, 0.0, 0.0), hi=(0.1, 0.1, 0.1))
     sp = sphere_bodies(np.random.default_rng(3).uniform(0.0, 0.1, (40, 3)), np.float32(0.006), np.float32(0.01))
     h3 = H.generate_hulls(0x4E03, 150, lo=(0.0, 0.0, 0.0), hi=(0.08, 0.08, 0.08))
     s3, _ = HB.concat([HB.box_walls((0.0, 0.0, 0.0), (0.08, 0.08, 0.08)), HB.make_bodies(h3, 3)])
-    bodies, first = HB.concat([s1, HB.concat([HB.make_bodies(h2, 2), sp])[0], s3])
+    h4 = H.generate_hulls(0x4E04, 150, lo=(0.0, 0.0, 0.0), hi=(0.12, 0.12, 0.03))
+    q = np.linspace(0.0, 0.12, 4, dtype=np.float32)
+    tris = []
+    for i in range(3):
+        for j in range(3):
+            a, b, c, d = ((q[i], q[j], 0.0), (q[i + 1], q[j], 0.0), (q[i + 1], q[j + 1], 0.0), (q[i], q[j + 1], 0.0))
+            tris += [(a, b, c), (a, c, d)]
+    s4, _ = HB.concat([HB.make_bodies(h4, 4), HB.triangle_bodies(np.array(tris, np.float32))])
+    bodies, first = HB.concat([s1, HB.concat([HB.make_bodies(h2, 2), sp])[0], s3, s4])
     n = len(bodies["pos4"])
     st = oracle.HullBodyState(bodies, first)
-    walls = np.r_[np.arange(300, 312), np.arange(first[2], first[2] + 12)]
+    walls = np.r_[np.arange(300, 312), np.arange(first[2], first[2] + 12), np.arange(first[4] - 18, first[4])]
     assert np.array_equal(np.nonzero(st.large)[0], walls)
+    assert st.grid16[3].view(np.int32)[12:14].min() >= 2  # the floor spans several cells of the second grid
     rng = np.random.default_rng(7)
     st.P[:, :3] += rng.normal(0.0, 0.01, (n, 3)).astype(np.float32)
     far = rng.choice(n, 40, replace=False)
@@ -186,4 +197,5 @@ def test_grid_broadphase_equals_brute_force(oracle):
     assert len(got) == len(set(map(tuple, got.tolist())))
     assert set(map(tuple, got.tolist())) == want
     assert len(want) > 300 and any(a in far or b in far for a, b in want)
-    assert any(a in walls[12:] for a, b in want) and any(b in walls[:12] for a, b in want)
+    assert any(a in walls[12:24] for a, b in want) and any(b in walls[:12] for a, b in want)
+    assert sum(b in walls[24:] for a, b in want) > 20  # hulls resting on the floor
