@@ -149,8 +149,9 @@ __device__ __forceinline__ int hb_cell_axis(float x, float lo, float inv, int nd
     return (int)fminf(fmaxf(floorf((x - lo) * inv), 0.0f), (float)(nd - 1));
 }
 
-// key = (global cell of the body's centre) << 32 | body index; a body kept
-// out of the grids takes cell ncells, past every scene's cells
+// key = (global cell of the body's centre) << 32 | body index; a large body
+// (bucketed in its scene's second grid at upload) takes cell ncells, past
+// every scene's cells of the first grids
 __global__ void k_hb_cell_keys(const float4* __restrict__ P, const int* __restrict__ scene_of,
                                const HbGrid* __restrict__ grid, const int* __restrict__ large, int n, int ncells,
                                uint64_t* __restrict__ keys) {
