@@ -157,6 +157,45 @@ def test_hull_bodies_on_tessellated_floor(oracle):
     sc.close()
 
 
+def test_hull_bodies_ragged_scenes(oracle):
+    """Scenes of every awkward size in one context: empty scenes (first,
+    between and last), tumbling hulls, a single sphere, a scene of static
+    walls only (no moving body: their grid is sized by the walls) and five
+    coincident spheres (the pair normal of a zero distance). Contacts and the
+    state after three steps equal the oracle's."""
+    say("hull bodies ragged scenes")
+
+    def spheres(pts, r=0.006, m=0.002):
+        b = HB.triangle_bodies(np.zeros((len(pts), 3, 3), np.float32))
+        b["pos4"][:, :3] = np.asarray(pts, np.float32)
+        b["pos4"][:, 3] = m
+        b["nv"][:] = 0
+        b["vstart"][:] = 0
+        b["verts4"] = np.zeros((0, 4), np.float32)
+        b["radius"][:] = r
+        b["inertia6"][:, :3] = np.float32(0.4 * m * r * r)
+        return b
+
+    h = H.generate_hulls(0x4B07, 60, lo=(0.0, 0.0, 0.0), hi=(0.06, 0.06, 0.06), v_sigma=0.3)
+    bodies, f = HB.concat([HB.make_bodies(h, 7), spheres([[0.5, 0.5, 0.5]]),
+                           HB.box_walls((0.0, 0.0, 0.0), (0.1, 0.1, 0.1)), spheres([[0.2, 0.2, 0.2]] * 5)])
+    first = np.array([0, 0, f[1], f[1], f[2], f[3], f[4], f[4]], np.int32)
+    c = law()
+    st = oracle.HullBodyState(bodies, first)
+    op, of, oq = st.contacts(c, nthreads=8)
+    sc = HB.HullBodyScene(c)
+    sc.upload(bodies, first)
+    sc.step(1)
+    gp, gf, gq = sc.contacts()
+    assert ((op[:, 0] >= f[3]) & (op[:, 1] < f[4])).sum() == 10  # the coincident spheres touch pairwise
+    assert np.array_equal(gp, op) and np.array_equal(gf, of) and np.array_equal(gq, oq)
+    st.step(c, nsteps=1, nthreads=8)
+    sc.step(2)
+    st.step(c, nsteps=2, nthreads=8)
+    compare(sc, st, "step 3")
+    sc.close()
+
+
 def test_hull_capacity_overflow_reported():
     say("hull capacity")
     bodies = scene(0x4B04, 200, 0, 0.05)  # crowded: many candidate pairs per body
