@@ -1,7 +1,9 @@
 // EPA penetration depth on a 16-lane group: four queries per wavefront.
 //
 // A group is one DPP row, so its reductions (the support argmax over hull
-// vertices, the closest-face argmin) are four DPP steps with no LDS traffic.
+// vertices, the closest-face argmin) are four DPP steps over (value, index)
+// pairs; the winning support vertex is then read from the lane that holds it
+// (three ds_bpermute) instead of riding along in every step.
 // The two hulls' vertices (local frame) are loaded into registers once per
 // query, EPA_VSLOT per lane; larger hulls read the rest from memory. The
 // polytope (vertices, faces with unit normal and plane distance, horizon
@@ -88,17 +90,6 @@ __device__ __forceinline__ float dpp_f(float v) {
     return __int_as_float(dpp_i<C>(__float_as_int(v)));
 }
 
-// (value, index, point) with the larger value, the smaller index on ties.
-template <int C>
-__device__ __forceinline__ void amax_step(float& v, int& i, V3& p) {
-    const float ov = dpp_f<C>(v);
-    const int oi = dpp_i<C>(i);
-    const V3 op = v3(dpp_f<C>(p.x), dpp_f<C>(p.y), dpp_f<C>(p.z));
-    const bool t = ov > v || (ov == v && oi < i);
-    v = t ? ov : v;
-    i = t ? oi : i;
-    p = v3(t ? op.x : p.x, t ? op.y : p.y, t ? op.z : p.z);
-}
 template <int C>
 __device__ __forceinline__ void amin_step(float& v, int& i) {
     const float ov = dpp_f<C>(v);
@@ -128,8 +119,20 @@ __device__ __forceinline__ HullRegs load_hull_regs(const Shape& s, int gl) {
     return r;
 }
 
+// (value, index) with the larger value, the smaller index on ties.
+template <int C>
+__device__ __forceinline__ void amax_vi_step(float& v, int& i) {
+    const float ov = dpp_f<C>(v);
+    const int oi = dpp_i<C>(i);
+    const bool t = ov > v || (ov == v && oi < i);
+    v = t ? ov : v;
+    i = t ? oi : i;
+}
+
 // Support point of a shape along u across the group: the vertex with the
 // largest local-frame projection, first index on ties (a serial scan's pick).
+// The arg-max carries (value, index); the winning vertex then comes from the
+// lane that holds it in registers (ds_bpermute), or from memory past them.
 __device__ __forceinline__ V3 grp_support_point(const Shape& s, const HullRegs& hr, V3 u, int gl, int& idx) {
     if (s.nv == 0) {
         idx = -1;
@@ -141,7 +144,6 @@ __device__ __forceinline__ V3 grp_support_point(const Shape& s, const HullRegs& 
     const V3 l = rot_tmul(s.R, u);
     float bv = -3.402823466e38f;
     int bi = 0x7fffffff;
-    V3 bp = v3(0.0f, 0.0f, 0.0f);
 #pragma unroll
     for (int j = 0; j < EPA_VSLOT; ++j) {  // indices ascend within a lane: the first maximum is kept
         const int k = gl + EPA_G * j;
@@ -151,7 +153,6 @@ __device__ __forceinline__ V3 grp_support_point(const Shape& s, const HullRegs& 
             if (d > bv) {
                 bv = d;
                 bi = k;
-                bp = p;
             }
         }
     }
@@ -161,16 +162,26 @@ __device__ __forceinline__ V3 grp_support_point(const Shape& s, const HullRegs& 
         if (d > bv) {
             bv = d;
             bi = k;
-            bp = v3(q.x, q.y, q.z);
         }
     }
-    amax_step<DPP_XOR1>(bv, bi, bp);
-    amax_step<DPP_XOR2>(bv, bi, bp);
-    amax_step<DPP_HALF_MIRROR>(bv, bi, bp);
-    if constexpr (EPA_G == 16) amax_step<DPP_MIRROR>(bv, bi, bp);
-    if (bi == 0x7fffffff) {  // no ordered projection: vertex 0, as a serial scan
-        bi = 0;
-        const float4 q = s.verts[0];
+    amax_vi_step<DPP_XOR1>(bv, bi);
+    amax_vi_step<DPP_XOR2>(bv, bi);
+    amax_vi_step<DPP_HALF_MIRROR>(bv, bi);
+    if constexpr (EPA_G == 16) amax_vi_step<DPP_MIRROR>(bv, bi);
+    if (bi == 0x7fffffff) bi = 0;  // no ordered projection: vertex 0, as a serial scan
+    V3 bp;
+    if (bi < EPA_G * EPA_VSLOT) {
+        const int j = bi / EPA_G;  // equal on the group's lanes
+        V3 mine = hr.v[0];
+#pragma unroll
+        for (int jj = 1; jj < EPA_VSLOT; ++jj)
+            if (j == jj) mine = hr.v[jj];
+        const int src = 4 * ((int)(__lane_id() & (64 - EPA_G)) + (bi & (EPA_G - 1)));
+        bp = v3(__int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mine.x))),
+                __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mine.y))),
+                __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mine.z))));
+    } else {
+        const float4 q = s.verts[bi];
         bp = v3(q.x, q.y, q.z);
     }
     idx = bi;
