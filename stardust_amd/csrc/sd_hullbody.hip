@@ -172,7 +172,7 @@ __global__ void k_hb_cell_keys(const float4* __restrict__ P, const int* __restri
 // Pairs a walk of the grid keeps in a body's own slots (int2 each, slot k of
 // body a at k n + a, so that neighbouring threads touch neighbouring slots); a
 // body with more walks the grid again and writes them in place.
-constexpr int HB_GRID_SLOTS = 16;
+constexpr int HB_GRID_SLOTS = 32;
 
 struct HbGridArgs {
     const float4* P;

@@ -97,7 +97,7 @@ def test_hull_contacts_equal_oracle(oracle):
 def test_hull_bodies_crowded_cell(oracle):
     """Forty sphere bodies piled into one grid cell beside tumbling hulls: the
     first of them has 39 candidate pairs in its own cell, more than a body's
-    slots hold, so the broadphase walks the grid again for it; the others'
+    32 slots hold, so the broadphase walks the grid again for it; the others'
     pairs come from their slots. Candidate order, contacts and the state after
     two steps equal the oracle's."""
     say("hull bodies crowded cell")
