@@ -94,7 +94,7 @@ template <int C>
 __device__ __forceinline__ void amin_step(float& v, int& i) {
     const float ov = dpp_f<C>(v);
     const int oi = dpp_i<C>(i);
-    const bool t = ov < v || (ov == v && oi < i);
+    const bool t = (ov < v) | ((ov == v) & (oi < i));  // bitwise: no branches
     v = t ? ov : v;
     i = t ? oi : i;
 }
@@ -124,7 +124,7 @@ template <int C>
 __device__ __forceinline__ void amax_vi_step(float& v, int& i) {
     const float ov = dpp_f<C>(v);
     const int oi = dpp_i<C>(i);
-    const bool t = ov > v || (ov == v && oi < i);
+    const bool t = (ov > v) | ((ov == v) & (oi < i));  // bitwise: no branches
     v = t ? ov : v;
     i = t ? oi : i;
 }
