@@ -210,7 +210,7 @@ __device__ __forceinline__ void seed_select(const Shape& s, V3 u, SeedSet& out) 
                 mslot = 0;
 #pragma unroll
                 for (int q = 1; q < WARM_K; ++q) {
-                    const bool lower = tv[q] < mval || (tv[q] == mval && ti[q] > mi);
+                    const bool lower = (tv[q] < mval) | ((tv[q] == mval) & (ti[q] > mi));
                     mval = lower ? tv[q] : mval;
                     mi = lower ? ti[q] : mi;
                     mslot = lower ? q : mslot;
@@ -234,12 +234,12 @@ __device__ __forceinline__ MPoint seed_support(const SeedSet& a, const SeedSet& 
 #pragma unroll
     for (int j = 1; j < WARM_K; ++j) {
         const float d = dot(a.p[j], u);
-        const bool t = d > ba || (d == ba && a.idx[j] < ia);
+        const bool t = (d > ba) | ((d == ba) & (a.idx[j] < ia));  // bitwise: no branches
         ba = t ? d : ba;
         ia = t ? a.idx[j] : ia;
         pa = sel3(t, a.p[j], pa);
         const float e = -dot(b.p[j], u);
-        const bool r = e > bb || (e == bb && b.idx[j] < ib);
+        const bool r = (e > bb) | ((e == bb) & (b.idx[j] < ib));
         bb = r ? e : bb;
         ib = r ? b.idx[j] : ib;
         pb = sel3(r, b.p[j], pb);
