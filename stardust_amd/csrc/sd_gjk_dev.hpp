@@ -424,7 +424,7 @@ __device__ __forceinline__ void simplex_push(Simplex& s, const MPoint& w) {
 __device__ __forceinline__ bool simplex_has(const Simplex& s, int key) {
     bool dup = false;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) dup = dup || (i < s.n && key >= 0 && s.w[i].key == key);
+    for (int i = 0; i < 4; ++i) dup = dup | ((i < s.n) & (key >= 0) & (s.w[i].key == key));  // no branches
     return dup;
 }
 
@@ -537,13 +537,13 @@ __device__ __forceinline__ bool feature_sets(const MPoint* w, int n, Features& f
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         if (i < n) {
-            keyed = keyed && w[i].key >= 0;
+            keyed = keyed & (w[i].key >= 0);
             const int a = w[i].key >> 10, b = w[i].key & (GJK_MAX_VERTS_PER_HULL - 1);
             bool da = false, db = false;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                da = da || (j < f.na && f.ia[j] == a);
-                db = db || (j < f.nb && f.ib[j] == b);
+                da = da | ((j < f.na) & (f.ia[j] == a));
+                db = db | ((j < f.nb) & (f.ib[j] == b));
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -668,7 +668,7 @@ __device__ __forceinline__ float cert_delta(const Shape& A, const Shape& B, cons
 }
 
 __device__ __forceinline__ bool in_set(int k, const int* set) {
-    return k == set[0] || k == set[1] || k == set[2] || k == set[3];
+    return (k == set[0]) | (k == set[1]) | (k == set[2]) | (k == set[3]);  // no branches
 }
 
 // Lowest projection of the set's vertices and highest of the others along l
