@@ -37,6 +37,8 @@ namespace sd {
 // (its largest vertex distance from the centre) of its extreme one along the
 // contact normal form the contact feature.
 constexpr float HB_FEATURE_REL = 1e-3f;
+// Vertices in flight together in the contact feature's passes.
+constexpr int HB_FEATURE_BATCH = 8;
 // A warm-started query is kept only beyond this distance from touching (m);
 // nearer ones are redone cold.
 constexpr float HB_WARM_NEAR_MARGIN = 1e-6f;
@@ -78,24 +80,36 @@ __device__ __forceinline__ V3 feature_centroid(const float4* __restrict__ verts,
     spread2 = 0.0f;
     if (nv == 0) return add(X, scl(u, radius));
     const V3 l = rot_tmul(R, u);
+    // vertices read HB_FEATURE_BATCH at a time, their loads in flight
+    // together; indices past the last vertex read it again (harmless for the
+    // maximum, skipped by the sums)
+    const int last = nv - 1;
     float dmax = -3.402823466e38f;
-    for (int k = 0; k < nv; ++k) {
-        const float4 v = verts[k];
-        dmax = fmaxf(dmax, v.x * l.x + v.y * l.y + v.z * l.z);
+    for (int k = 0; k < nv; k += HB_FEATURE_BATCH) {
+        float4 p[HB_FEATURE_BATCH];
+#pragma unroll
+        for (int j = 0; j < HB_FEATURE_BATCH; ++j) p[j] = verts[min(k + j, last)];
+#pragma unroll
+        for (int j = 0; j < HB_FEATURE_BATCH; ++j) dmax = fmaxf(dmax, p[j].x * l.x + p[j].y * l.y + p[j].z * l.z);
     }
     const float cut = dmax - tol;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     int cnt = 0;
     uint32_t sel = 0u;
-    for (int k = 0; k < nv; ++k) {
-        const float4 v = verts[k];
-        const float d = v.x * l.x + v.y * l.y + v.z * l.z;
-        if (d >= cut) {
-            sx = sx + v.x;
-            sy = sy + v.y;
-            sz = sz + v.z;
-            ++cnt;
-            if (k < 32) sel |= 1u << k;
+    for (int k = 0; k < nv; k += HB_FEATURE_BATCH) {
+        float4 p[HB_FEATURE_BATCH];
+#pragma unroll
+        for (int j = 0; j < HB_FEATURE_BATCH; ++j) p[j] = verts[min(k + j, last)];
+#pragma unroll
+        for (int j = 0; j < HB_FEATURE_BATCH; ++j) {
+            const float d = p[j].x * l.x + p[j].y * l.y + p[j].z * l.z;
+            if (k + j < nv && d >= cut) {
+                sx = sx + p[j].x;
+                sy = sy + p[j].y;
+                sz = sz + p[j].z;
+                ++cnt;
+                if (k + j < 32) sel |= 1u << (k + j);
+            }
         }
     }
     const float inv = 1.0f / (float)cnt;
