@@ -219,28 +219,29 @@ __device__ __forceinline__ int hb_grid_walk(const HbGridArgs& G, int a, Emit emi
     const int cy = hb_cell_axis(pa.y, g.loy, g.inv, g.ny);
     const int cz = hb_cell_axis(pa.z, g.loz, g.inv, g.nz);
     int k = 0;
+    // the stencil's rows (dz, dy) in order; a row's cells are consecutive in
+    // the sorted keys, so each row is one window of positions: from a's own
+    // cell (the bodies after a) for the first row, x - 1 .. x + 1 (clamped)
+    // for the others
     for (int dz = 0; dz <= 1; ++dz) {
         const int z = cz + dz;
         if (z >= g.nz) continue;
         for (int dy = -1; dy <= 1; ++dy) {
             const int y = cy + dy;
             if (y < 0 || y >= g.ny || (dz == 0 && dy < 0)) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int x = cx + dx;
-                if (x < 0 || x >= g.nx || (dz == 0 && dy == 0 && dx < 0)) continue;
-                const bool own = dz == 0 && dy == 0 && dx == 0;
-                const int c = g.cbase + (z * g.ny + y) * g.nx + x;
-                const int t1 = G.cell_start[c + 1];
-                for (int t = G.cell_start[c]; t < t1; ++t) {
-                    const int b = (int)(uint32_t)G.sorted[t];
-                    if (own && b <= a) continue;
-                    if (sa && G.P[b].w == 0.0f) continue;
-                    const float4 lb = G.lo[b], hb = G.hi[b];
-                    if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z &&
-                          la.z <= hb.z))
-                        continue;
-                    emit(k++, min(a, b), max(a, b));
-                }
+            const bool own_row = dz == 0 && dy == 0;
+            const int x0 = own_row ? cx : max(cx - 1, 0), x1 = min(cx + 1, g.nx - 1);
+            const int row = g.cbase + (z * g.ny + y) * g.nx;
+            const int t0 = G.cell_start[row + x0], t1 = G.cell_start[row + x1 + 1];
+            const int own_end = own_row ? G.cell_start[row + cx + 1] : t0;  // a's own cell: [t0, own_end)
+            for (int t = t0; t < t1; ++t) {
+                const int b = (int)(uint32_t)G.sorted[t];
+                if (t < own_end && b <= a) continue;
+                if (sa && G.P[b].w == 0.0f) continue;
+                const float4 lb = G.lo[b], hb = G.hi[b];
+                if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z && la.z <= hb.z))
+                    continue;
+                emit(k++, min(a, b), max(a, b));
             }
         }
     }
@@ -248,25 +249,21 @@ __device__ __forceinline__ int hb_grid_walk(const HbGridArgs& G, int a, Emit emi
     const int qx = hb_cell_axis(pa.x, g.llox, g.linv, g.lnx);
     const int qy = hb_cell_axis(pa.y, g.lloy, g.linv, g.lny);
     const int qz = hb_cell_axis(pa.z, g.lloz, g.linv, g.lnz);
+    const int qx0 = max(qx - 1, 0), qx1 = min(qx + 1, g.lnx - 1);
     for (int dz = -1; dz <= 1; ++dz) {
         const int z = qz + dz;
         if (z < 0 || z >= g.lnz) continue;
         for (int dy = -1; dy <= 1; ++dy) {
             const int y = qy + dy;
             if (y < 0 || y >= g.lny) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int x = qx + dx;
-                if (x < 0 || x >= g.lnx) continue;
-                const int c = g.lcbase + (z * g.lny + y) * g.lnx + x;
-                const int t1 = G.lstart[c + 1];
-                for (int t = G.lstart[c]; t < t1; ++t) {
-                    const int b = G.llist[t];
-                    const float4 lb = G.lo[b], hb = G.hi[b];
-                    if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z &&
-                          la.z <= hb.z))
-                        continue;
-                    emit(k++, min(a, b), max(a, b));
-                }
+            const int row = g.lcbase + (z * g.lny + y) * g.lnx;
+            const int t1 = G.lstart[row + qx1 + 1];
+            for (int t = G.lstart[row + qx0]; t < t1; ++t) {
+                const int b = G.llist[t];
+                const float4 lb = G.lo[b], hb = G.hi[b];
+                if (!(lb.x <= ha.x && la.x <= hb.x && lb.y <= ha.y && la.y <= hb.y && lb.z <= ha.z && la.z <= hb.z))
+                    continue;
+                emit(k++, min(a, b), max(a, b));
             }
         }
     }
